@@ -1,0 +1,358 @@
+"""hit_leiden_amd -- MI355X (gfx950) engine for HIT-Leiden's incremental
+hierarchical community update.
+
+Python mirror of the reference crate's API for this path (nameearly/hit-leiden,
+Rust): `RunMode`/`RunConfig` (src/core/config.rs:19-51), `GraphInput`
+(src/core/types.rs:133-148), the `HitLeidenError` variants (src/core/error.rs:19-27),
+`PartitionState` + `hit_leiden()` (src/core/partition/state.rs:19-64,
+src/core/algorithm/hit_leiden.rs:936-1018) and `compute_modularity`
+(hit_leiden.rs:121-161).  Every call goes through the C ABI in
+include/hitleiden.h (libhitleiden.so); there is no CPU fallback -- importing
+this package on a machine without the built library or a gfx950 device
+raises AccelerationError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libhitleiden.so")
+
+HL_OK, HL_INVALID_INPUT, HL_BACKEND, HL_ACCELERATION, HL_OOM = range(5)
+MAP_F, MAP_G, MAP_S_CUR, MAP_S_PRE = range(4)
+SET_ACTIVE, SET_CHANGED, SET_AFFECTED, SET_REFINED = range(4)
+PART_COMMUNITY, PART_REFINED, PART_F0 = range(3)
+MAX_LEVELS = 8
+
+
+class RunMode(enum.IntEnum):
+    """RunMode (src/core/config.rs:19-23)."""
+    DETERMINISTIC = 0
+    THROUGHPUT = 1
+
+
+class HitLeidenError(Exception):
+    """HitLeidenError (src/core/error.rs:19-27)."""
+
+
+class InvalidInput(HitLeidenError):
+    pass
+
+
+class BackendError(HitLeidenError):
+    pass
+
+
+class AccelerationError(HitLeidenError):
+    pass
+
+
+class OutOfMemory(BackendError):
+    pass
+
+
+_ERRORS = {HL_INVALID_INPUT: InvalidInput, HL_BACKEND: BackendError, HL_ACCELERATION: AccelerationError,
+           HL_OOM: OutOfMemory}
+
+
+class _Config(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("mode", C.c_int32), ("levels", C.c_int32), ("max_rounds", C.c_int32),
+                ("device", C.c_int32), ("small_degree", C.c_uint32), ("wave_degree", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
+
+
+class BatchStats(C.Structure):
+    """hl_batch_stats: per-batch counters of one load/apply call."""
+    _fields_ = [("iterations", C.c_uint64), ("levels", C.c_uint32), ("level_n", C.c_uint32 * MAX_LEVELS),
+                ("delta_len", C.c_uint64 * MAX_LEVELS), ("active", C.c_uint64 * MAX_LEVELS),
+                ("evaluated", C.c_uint64 * MAX_LEVELS), ("moved", C.c_uint64 * MAX_LEVELS),
+                ("affected", C.c_uint64 * MAX_LEVELS), ("refined", C.c_uint64 * MAX_LEVELS),
+                ("rounds", C.c_uint64 * MAX_LEVELS), ("bytes_alg", C.c_uint64), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        P = self.levels
+        return {
+            "iterations": self.iterations, "levels": P, "level_n": list(self.level_n[:P]),
+            "delta_len": list(self.delta_len[:P]), "active": list(self.active[:P]),
+            "evaluated": list(self.evaluated[:P]), "moved": list(self.moved[:P]),
+            "affected": list(self.affected[:P]), "refined": list(self.refined[:P]),
+            "rounds": list(self.rounds[:P]), "bytes_alg": self.bytes_alg, "seconds": self.seconds,
+        }
+
+
+_lib = None
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def lib():
+    """Load libhitleiden.so; fails loudly when it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise AccelerationError(f"{_LIB_PATH} is not built (run hit_leiden_amd.build.build_all())")
+        L = C.CDLL(_LIB_PATH)
+        vp, u32p, u64p, f64p, u8p = C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), \
+            C.POINTER(C.c_double), C.POINTER(C.c_uint8)
+        sig = {
+            "hl_create": ([C.POINTER(_Config), C.POINTER(vp)], C.c_int),
+            "hl_destroy": ([vp], None),
+            "hl_load": ([vp, C.c_uint32, C.c_uint64, vp, vp, vp, C.POINTER(BatchStats)], C.c_int),
+            "hl_load_graph": ([vp, C.c_uint32, C.c_uint64, vp, vp, vp], C.c_int),
+            "hl_apply_batch": ([vp, C.c_uint64, vp, vp, vp, C.POINTER(BatchStats)], C.c_int),
+            "hl_apply_batch_device": ([vp, C.c_uint64, vp, vp, vp, C.POINTER(BatchStats)], C.c_int),
+            "hl_level_count": ([vp, C.POINTER(C.c_int32)], C.c_int),
+            "hl_level_size": ([vp, C.c_int32, u32p], C.c_int),
+            "hl_read_partition": ([vp, C.c_int32, C.c_int32, vp, C.c_uint64], C.c_int),
+            "hl_write_partition": ([vp, C.c_int32, C.c_int32, vp, C.c_uint64], C.c_int),
+            "hl_read_frontier": ([vp, C.c_int32, C.c_int32, vp, C.c_uint64], C.c_int),
+            "hl_read_level_graph": ([vp, C.c_int32, vp, vp, vp, C.c_uint64, u64p], C.c_int),
+            "hl_read_level_delta": ([vp, C.c_int32, vp, vp, vp, C.c_uint64, u64p], C.c_int),
+            "hl_twice_m": ([vp, C.c_int32, f64p], C.c_int),
+            "hl_read_hierarchy": ([vp, vp, C.c_uint64, C.c_int32], C.c_int),
+            "hl_modularity": ([vp, C.c_int32, f64p], C.c_int),
+            "hl_modularity_labels": ([vp, vp, C.c_uint64, f64p], C.c_int),
+            "hl_last_error": ([vp], C.c_char_p),
+            "hl_create_error": ([], C.c_char_p),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+@dataclass
+class RunConfig:
+    """RunConfig (src/core/config.rs:25-51) restricted to this path."""
+    resolution: float = 1.0
+    mode: RunMode = RunMode.THROUGHPUT
+    levels: int = 1
+    max_rounds: int = 20
+    device: int = 0
+    small_degree: int = 0
+    wave_degree: int = 0
+
+
+class GraphInput:
+    """GraphInput (src/core/types.rs:133-148): node_count + (u, v, w) edges;
+    w None means unit weights (Option<f64>::unwrap_or(1.0))."""
+
+    def __init__(self, node_count, src, dst, weights=None):
+        self.node_count = int(node_count)
+        self.src = np.ascontiguousarray(src, dtype=np.uint32)
+        self.dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        self.weights = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        if self.src.shape != self.dst.shape or (self.weights is not None and self.weights.shape != self.src.shape):
+            raise InvalidInput("src, dst and weights must have the same length")
+
+    @classmethod
+    def from_edges(cls, node_count, edges):
+        """edges: iterable of (u, v) or (u, v, w)."""
+        e = list(edges)
+        if not e:
+            return cls(node_count, [], [], [])
+        u = [x[0] for x in e]
+        v = [x[1] for x in e]
+        w = [x[2] if len(x) > 2 and x[2] is not None else 1.0 for x in e]
+        return cls(node_count, u, v, w)
+
+    def __len__(self):
+        return len(self.src)
+
+
+class HitLeiden:
+    """Device-resident PartitionState + graph hierarchy on one gfx950 GPU.
+
+    load() runs the initial pass (Algorithm 6 with dG = all edges on the
+    identity state); hit_leiden(delta) applies one edge-delta batch
+    (hit_leiden.rs:936-1018) and returns the iteration count.
+    """
+
+    def __init__(self, config: RunConfig | None = None):
+        self.config = config or RunConfig()
+        c = _Config()
+        c.gamma = float(self.config.resolution)
+        c.mode = int(self.config.mode)
+        c.levels = int(self.config.levels)
+        c.max_rounds = int(self.config.max_rounds)
+        c.device = int(self.config.device)
+        c.small_degree = int(self.config.small_degree)
+        c.wave_degree = int(self.config.wave_degree)
+        h = C.c_void_p()
+        st = lib().hl_create(C.byref(c), C.byref(h))
+        if st != HL_OK:
+            raise _ERRORS.get(st, HitLeidenError)(lib().hl_create_error().decode())
+        self._h = h
+        self.last_stats = BatchStats()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st):
+        if st != HL_OK:
+            msg = lib().hl_last_error(self._h)
+            raise _ERRORS.get(st, HitLeidenError)(msg.decode() if msg else "error")
+
+    # ---- load / update ----
+    def load(self, graph: GraphInput) -> BatchStats:
+        s = BatchStats()
+        self._check(lib().hl_load(self._h, graph.node_count, len(graph), _ptr(graph.src), _ptr(graph.dst),
+                                  _ptr(graph.weights), C.byref(s)))
+        self.last_stats = s
+        return s
+
+    def load_graph(self, graph: GraphInput):
+        self._check(lib().hl_load_graph(self._h, graph.node_count, len(graph), _ptr(graph.src), _ptr(graph.dst),
+                                        _ptr(graph.weights)))
+
+    def hit_leiden(self, delta: GraphInput) -> int:
+        """Apply one edge-delta batch; alpha > 0 inserts/strengthens, alpha < 0 deletes/weakens."""
+        s = BatchStats()
+        self._check(lib().hl_apply_batch(self._h, len(delta), _ptr(delta.src), _ptr(delta.dst),
+                                         _ptr(delta.weights), C.byref(s)))
+        self.last_stats = s
+        return int(s.iterations)
+
+    apply_batch = hit_leiden
+
+    def apply_batch_device(self, k, d_src, d_dst, d_alpha=None) -> BatchStats:
+        """Batch already in device memory: raw device pointers (ints)."""
+        s = BatchStats()
+        self._check(lib().hl_apply_batch_device(self._h, int(k), C.c_void_p(d_src), C.c_void_p(d_dst),
+                                                None if d_alpha is None else C.c_void_p(d_alpha), C.byref(s)))
+        self.last_stats = s
+        return s
+
+    # ---- state readout (PartitionState fields) ----
+    @property
+    def levels(self):
+        x = C.c_int32()
+        self._check(lib().hl_level_count(self._h, C.byref(x)))
+        return x.value
+
+    def level_size(self, level):
+        x = C.c_uint32()
+        self._check(lib().hl_level_size(self._h, level, C.byref(x)))
+        return x.value
+
+    def partition(self, level, which):
+        n = self.level_size(level)
+        out = np.zeros(max(n, 1), dtype=np.uint32)
+        self._check(lib().hl_read_partition(self._h, level, which, _ptr(out), len(out)))
+        return out[:n]
+
+    def write_partition(self, level, which, values):
+        a = np.ascontiguousarray(values, dtype=np.uint32)
+        self._check(lib().hl_write_partition(self._h, level, which, _ptr(a), len(a)))
+
+    @property
+    def community_mapping_per_level(self):
+        return [self.partition(p, MAP_F) for p in range(self.levels)]
+
+    @property
+    def refined_community_mapping_per_level(self):
+        return [self.partition(p, MAP_G) for p in range(self.levels)]
+
+    @property
+    def current_subcommunity_mapping_per_level(self):
+        return [self.partition(p, MAP_S_CUR) for p in range(self.levels)]
+
+    @property
+    def previous_subcommunity_mapping_per_level(self):
+        return [self.partition(p, MAP_S_PRE) for p in range(self.levels)]
+
+    @property
+    def node_to_comm(self):
+        """g^1, the reference's output (hit_leiden.rs:1016)."""
+        return self.partition(0, MAP_G)
+
+    def frontier(self, level, which):
+        n = self.level_size(level)
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        self._check(lib().hl_read_frontier(self._h, level, which, _ptr(out), len(out)))
+        return out[:n].astype(bool)
+
+    def supergraph(self, level):
+        """G^p as (offsets, cols, weights), rows sorted, self-loop = one entry."""
+        nnz = C.c_uint64()
+        self._check(lib().hl_read_level_graph(self._h, level, None, None, None, 0, C.byref(nnz)))
+        n = self.level_size(level)
+        off = np.zeros(n + 1, dtype=np.uint64)
+        col = np.zeros(max(nnz.value, 1), dtype=np.uint32)
+        w = np.zeros(max(nnz.value, 1), dtype=np.float64)
+        self._check(lib().hl_read_level_graph(self._h, level, _ptr(off), _ptr(col), _ptr(w), len(col),
+                                              C.byref(nnz)))
+        return off, col[:nnz.value], w[:nnz.value]
+
+    def level_delta(self, level):
+        k = C.c_uint64()
+        self._check(lib().hl_read_level_delta(self._h, level, None, None, None, 0, C.byref(k)))
+        u = np.zeros(max(k.value, 1), dtype=np.uint32)
+        v = np.zeros(max(k.value, 1), dtype=np.uint32)
+        w = np.zeros(max(k.value, 1), dtype=np.float64)
+        self._check(lib().hl_read_level_delta(self._h, level, _ptr(u), _ptr(v), _ptr(w), len(u), C.byref(k)))
+        return u[:k.value], v[:k.value], w[:k.value]
+
+    def twice_m(self, level=0):
+        x = C.c_double()
+        self._check(lib().hl_twice_m(self._h, level, C.byref(x)))
+        return x.value
+
+    def hierarchy(self, canonical=False):
+        """PartitionResult.hierarchy_levels (types.rs:107-110): P rows of n labels."""
+        n = self.level_size(0)
+        P = self.levels
+        out = np.zeros(max(n * P, 1), dtype=np.uint32)
+        self._check(lib().hl_read_hierarchy(self._h, _ptr(out), len(out), 1 if canonical else 0))
+        return out[:n * P].reshape(P, n)
+
+    def modularity(self, which=PART_COMMUNITY):
+        q = C.c_double()
+        self._check(lib().hl_modularity(self._h, which, C.byref(q)))
+        return q.value
+
+    def modularity_labels(self, labels):
+        a = np.ascontiguousarray(labels, dtype=np.uint32)
+        q = C.c_double()
+        self._check(lib().hl_modularity_labels(self._h, _ptr(a), len(a), C.byref(q)))
+        return q.value
+
+
+def compute_modularity(graph: GraphInput, labels, device=0) -> float:
+    """compute_modularity (hit_leiden.rs:121-161) on the GPU."""
+    h = HitLeiden(RunConfig(device=device))
+    try:
+        h.load_graph(graph)
+        return h.modularity_labels(labels)
+    finally:
+        h.close()
+
+
+__all__ = ["RunMode", "RunConfig", "GraphInput", "HitLeiden", "BatchStats", "compute_modularity",
+           "HitLeidenError", "InvalidInput", "BackendError", "AccelerationError", "OutOfMemory", "lib",
+           "library_path", "MAP_F", "MAP_G", "MAP_S_CUR", "MAP_S_PRE", "SET_ACTIVE", "SET_CHANGED",
+           "SET_AFFECTED", "SET_REFINED", "PART_COMMUNITY", "PART_REFINED", "PART_F0"]

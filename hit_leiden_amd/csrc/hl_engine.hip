@@ -1,0 +1,744 @@
+// hl_engine.hip -- host orchestration of Algorithm 6 (hit_leiden.rs:936-1018)
+// over device-resident levels, and the extern "C" boundary of include/hitleiden.h.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hitleiden.h"
+#include "hl_kernels.h"
+
+namespace hl {
+namespace {
+
+struct Level {
+  u32 n = 0;
+  DevBuf<u64> row_off;
+  DevBuf<u32> row_len, row_cap;
+  DevBuf<double> k;
+  DevBuf<u32> col;
+  DevBuf<double> w;
+  u64 pool_used = 0;
+  DevBuf<double> twice_m;
+  DevBuf<u32> f, g, s_pre, s_cur;
+  DevBuf<u8> A, B, K, R;  // last batch, before def_update propagation
+  DevBuf<Rec> dG;
+  u64 dG_len = 0;
+  u32 iters = 0;
+
+  Csr csr() const { return Csr{n, row_off.p, row_len.p, col.p, w.p, k.p, twice_m.p}; }
+  LevelCsrMut mut() { return LevelCsrMut{n, row_off.p, row_len.p, row_cap.p, col.p, w.p, k.p, twice_m.p}; }
+};
+
+struct KernelTimer {
+  // optional per-stage device timing (HIP events on the engine stream)
+  bool on = false;
+};
+
+}  // namespace
+
+struct Engine {
+  hl_config cfg{};
+  int P = 1;
+  int mode = HL_MODE_THROUGHPUT;
+  hipStream_t s = nullptr;
+  std::vector<std::unique_ptr<Level>> L;
+  u32 n0 = 0;
+  bool loaded = false;
+  std::string err;
+  Thresholds th;
+
+  Temp tmp;
+  ApplyScratch as;
+  ProposeScratch ps;
+  DecoupleScratch dsc;
+  DetMoveScratch dms;
+  RefineScratch rs;
+  AggScratch ag;
+  DevBuf<double> tot, partials, sig, acc;
+  DevBuf<u64> tot_len, cnt, offtmp;
+  DevBuf<u32> active, cto, mx, lab, lab2, first;
+  DevBuf<double> cgain;
+  DevBuf<u8> next, flag8;
+  DevBuf<Rec> dcur, dnext;
+  DevBuf<u32> ks, vs, vidx, starts;
+  DevBuf<u8> heads;
+  std::vector<std::unique_ptr<DevBuf<u8>>> Bp, Rp;
+
+  ~Engine() {
+    L.clear();
+    if (s) (void)hipStreamDestroy(s);
+  }
+
+  u32 dmax(const u32* a, u64 n) {
+    mx.reserve(1, s);
+    fill_u32(mx.p, 0, 1, s);
+    max_u32(a, n, mx.p, s);
+    u32 h = 0;
+    HL_CHECK(hipMemcpyAsync(&h, mx.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+    HL_CHECK(hipStreamSynchronize(s));
+    return h;
+  }
+
+  void reserve_level(Level& lv, u64 n) {
+    lv.row_off.reserve(n, s, true);
+    lv.row_len.reserve(n, s, true);
+    lv.row_cap.reserve(n, s, true);
+    lv.k.reserve(n, s, true);
+    lv.f.reserve(n, s, true);
+    lv.g.reserve(n, s, true);
+    lv.s_pre.reserve(n, s, true);
+    lv.s_cur.reserve(n, s, true);
+    lv.A.reserve(n, s);
+    lv.B.reserve(n, s);
+    lv.K.reserve(n, s);
+    lv.R.reserve(n, s);
+    lv.twice_m.reserve(1, s);
+  }
+
+  // vertex-space growth of level p with fresh ids for new vertices
+  void grow_level(int p, u32 n_new) {
+    Level& lv = *L[p];
+    const u32 n_old = lv.n;
+    if (n_new <= n_old) return;
+    u64 bf = n_old, bg = n_old, bs = n_old;
+    if (n_old) {
+      bf = std::max<u64>(bf, u64(dmax(lv.f.p, n_old)) + 1);
+      bg = std::max<u64>(bg, u64(dmax(lv.g.p, n_old)) + 1);
+      bs = std::max<u64>(bs, u64(std::max(dmax(lv.s_cur.p, n_old), dmax(lv.s_pre.p, n_old))) + 1);
+    }
+    if (p + 1 < P) bs = std::max<u64>(bs, L[p + 1]->n);
+    if (bf + (n_new - n_old) > 0xFFFFFFF0ull || bs + (n_new - n_old) > 0xFFFFFFF0ull)
+      throw InputError("id space exhausted");
+    reserve_level(lv, n_new);
+    grow_init(n_old, n_new, u32(bf), u32(bg), u32(bs), lv.f.p, lv.g.p, lv.s_pre.p, lv.s_cur.p, lv.row_off.p,
+              lv.row_len.p, lv.row_cap.p, lv.k.p, s);
+    lv.n = n_new;
+  }
+
+  // G^p <- G^p (+) dG^p, then 2m
+  void apply_delta_level(Level& lv, const Rec* d, u64 k, hl_batch_stats* st, int p) {
+    ApplyPlan plan;
+    LevelCsrMut g = lv.mut();
+    apply_delta_plan(d, k, g, as, tmp, plan, s);
+    if (plan.nseg) {
+      if (lv.pool_used + plan.need > lv.col.cap) {
+        // compact live rows into a fresh pool with room for this batch
+        u64 live = 0;
+        {
+          DevBuf<u32> col2;
+          DevBuf<double> w2;
+          // size: live rows + need, with headroom
+          u64 live_est = lv.pool_used;  // upper bound of live entries
+          u64 want = std::max<u64>(live_est + plan.need, 1024);
+          col2.reserve(want, s, false, 1.5);
+          w2.reserve(want, s, false, 1.5);
+          live = compact_rows(g, col2.p, w2.p, offtmp, tmp, s);
+          lv.col.swap(col2);
+          lv.w.swap(w2);
+        }
+        lv.pool_used = live;
+        g = lv.mut();
+      }
+      apply_delta_commit(g, lv.pool_used, as, plan, mode == HL_MODE_DETERMINISTIC, s);
+      lv.pool_used += plan.need;
+    }
+    u64 nc = (u64(lv.n) + kSumChunk - 1) / kSumChunk + 1;
+    partials.reserve(nc, s);
+    twice_m_chunked(lv.k.p, lv.n, partials.p, lv.twice_m.p, s);
+    if (st && p < HL_MAX_LEVELS)
+      st->bytes_alg += 16 * k + 24 * plan.old_entries + 12 * plan.need + 20 * plan.nseg;
+  }
+
+  u64 comm_space(Level& lv) {
+    u64 mf = lv.n ? u64(dmax(lv.f.p, lv.n)) + 1 : 0;
+    return std::max<u64>(std::max<u64>(2ull * lv.n, mf), 1);
+  }
+
+  u32 max_degree(Level& lv) { return lv.n ? dmax(lv.row_len.p, lv.n) : 0; }
+
+  // Algorithm 2 (hit_leiden.rs:1024-1192)
+  u32 movement(Level& lv, const Rec* d, u64 k, hl_batch_stats* st, int p) {
+    const u32 n = lv.n;
+    fill_u8(lv.A.p, 0, n, s);
+    fill_u8(lv.B.p, 0, n, s);
+    fill_u8(lv.K.p, 0, n, s);
+    seed(d, k, lv.f.p, lv.s_cur.p, lv.A.p, lv.K.p, s);
+    if (k == 0) fill_u8(lv.A.p, 1, n, s);
+    Csr g = lv.csr();
+    if (mode == HL_MODE_DETERMINISTIC) {
+      u64 len = std::max<u64>(n, n ? u64(dmax(lv.f.p, n)) + 1 : 0);
+      u64 cap = len + n + 1024;
+      tot.reserve(cap, s);
+      label_totals_canonical(lv.f.p, lv.k.p, n, tot.p, cap, ks, vs, vidx, heads, starts, cnt, tmp, s);
+      tot_len.reserve(1, s);
+      HL_CHECK(hipMemcpyAsync(tot_len.p, &len, sizeof(u64), hipMemcpyHostToDevice, s));
+      const u32 maxd = max_degree(lv);
+      bool fresh = true;
+      while (true) {
+        int r = det_move(g, lv.f.p, lv.s_cur.p, tot.p, cap, tot_len.p, lv.A.p, fresh, lv.B.p, lv.K.p, cfg.gamma,
+                         dms, maxd, s);
+        if (r == 0) break;
+        u64 ncap = cap * 2;
+        tot.reserve(ncap, s, true, 1.0);
+        fill_f64(tot.p + cap, 0.0, tot.cap - cap, s);
+        cap = tot.cap;
+        fresh = false;
+      }
+      if (st && p < HL_MAX_LEVELS) {
+        u64 pops = 0;
+        HL_CHECK(hipMemcpyAsync(&pops, dms.pops.p, sizeof(u64), hipMemcpyDeviceToHost, s));
+        HL_CHECK(hipStreamSynchronize(s));
+        st->evaluated[p] += pops;
+        st->rounds[p] = 1;
+      }
+      return 1;
+    }
+    // throughput: synchronous rounds, at most max_rounds (hit_leiden.rs:1077-1099)
+    const u64 ccap = comm_space(lv);
+    tot.reserve(ccap, s);
+    fill_f64(tot.p, 0.0, ccap, s);
+    label_totals_atomic(lv.f.p, lv.k.p, n, tot.p, s);
+    active.reserve(n + 1, s);
+    cto.reserve(n + 1, s);
+    cgain.reserve(n + 1, s);
+    next.reserve(n + 1, s);
+    u64 cnt_a = select_flagged(lv.A.p, n, active.p, tmp, cnt, s);
+    const int max_rounds = cfg.max_rounds > 0 ? cfg.max_rounds : 20;
+    u32 rounds = 0;
+    while (cnt_a > 0 && int(rounds) < max_rounds) {
+      u64 degsum = propose_moves(g, lv.f.p, tot.p, cfg.gamma, active.p, cnt_a, cto.p, cgain.p, ps, th, tmp, s);
+      fill_u8(next.p, 0, n, s);
+      u64 na = decouple_and_apply(g, lv.f.p, lv.s_cur.p, tot.p, ccap, active.p, cnt_a, cto.p, cgain.p, lv.B.p,
+                                  lv.K.p, next.p, dsc, tmp, s);
+      ++rounds;
+      if (st && p < HL_MAX_LEVELS) {
+        st->evaluated[p] += cnt_a;
+        st->bytes_alg += 20 * cnt_a + 16 * degsum + 20 * na;
+      }
+      if (na == 0) break;
+      cnt_a = select_flagged(next.p, n, active.p, tmp, cnt, s);
+    }
+    if (st && p < HL_MAX_LEVELS) st->rounds[p] = rounds;
+    return rounds;
+  }
+
+  // Algorithm 3 (hit_leiden.rs:1265-1416)
+  void refinement(Level& lv, hl_batch_stats* st, int p) {
+    const u32 n = lv.n;
+    fill_u8(lv.R.p, 0, n, s);
+    if (n == 0) return;
+    Csr g = lv.csr();
+    const u64 sspace = u64(dmax(lv.s_cur.p, n)) + 1;
+    cc_split(g, lv.s_cur.p, lv.K.p, lv.R.p, sspace, rs, tmp, s);
+    if (is_identity(lv.s_cur.p, n, rs, s)) fill_u8(lv.R.p, 1, n, s);
+    const u64 cs = comm_space(lv);
+    if (mode == HL_MODE_DETERMINISTIC)
+      refine_det(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, rs, ks, vs, vidx, heads, starts, cnt,
+                 max_degree(lv), tmp, s);
+    else
+      refine_parallel(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, rs, ps, th, tmp, s);
+    (void)st;
+    (void)p;
+  }
+
+  void def_update_maps(bool use_g) {
+    // copies of the per-level sets (the reference mutates its bitvecs)
+    while (int(Bp.size()) < P) Bp.emplace_back(new DevBuf<u8>());
+    for (int p = 0; p < P; ++p) {
+      Level& lv = *L[p];
+      Bp[p]->reserve(lv.n + 1, s);
+      copy_u8(Bp[p]->p, use_g ? lv.R.p : lv.B.p, lv.n, s);
+    }
+    for (int p = P - 1; p >= 0; --p) {
+      Level& lv = *L[p];
+      if (p < P - 1) {
+        Level& up = *L[p + 1];
+        def_pull(Bp[p]->p, use_g ? lv.g.p : lv.f.p, use_g ? up.g.p : up.f.p, lv.s_cur.p, lv.n, s);
+      }
+      if (p > 0) {
+        Level& lo = *L[p - 1];
+        def_push(Bp[p]->p, Bp[p - 1]->p, lo.s_cur.p, lo.n, lv.n, s);
+      }
+    }
+  }
+
+  // Algorithm 6 (hit_leiden.rs:936-1018) for a batch already in dcur[0..k)
+  u64 run_batch(u64 k, hl_batch_stats* st) {
+    u64 cur_len = k;
+    u32 cur_n = n0;
+    u64 total_iters = 0;
+    for (int p = 0; p < P; ++p) {
+      Level& lv = *L[p];
+      grow_level(p, std::max(lv.n, cur_n));
+      lv.dG.reserve(cur_len + 1, s);
+      if (cur_len)
+        HL_CHECK(hipMemcpyAsync(lv.dG.p, dcur.p, cur_len * sizeof(Rec), hipMemcpyDeviceToDevice, s));
+      lv.dG_len = cur_len;
+      apply_delta_level(lv, dcur.p, cur_len, st, p);
+      u32 it = movement(lv, dcur.p, cur_len, st, p);
+      lv.iters = it;
+      total_iters += it;
+      refinement(lv, st, p);
+      if (st && p < HL_MAX_LEVELS) {
+        st->level_n[p] = lv.n;
+        st->delta_len[p] = cur_len;
+        st->active[p] = count_u8(lv.A.p, lv.n, cnt, s);
+        st->moved[p] = count_u8(lv.B.p, lv.n, cnt, s);
+        st->affected[p] = count_u8(lv.K.p, lv.n, cnt, s);
+        st->refined[p] = count_u8(lv.R.p, lv.n, cnt, s);
+        st->bytes_alg += 24 * cur_len + 12 * st->moved[p];
+      }
+      if (p < P - 1) {
+        u64 nR = count_u8(lv.R.p, lv.n, cnt, s);
+        if (cur_len == 0 && nR == 0) {
+          // should_skip_aggregation (hit_leiden.rs:1020-1022)
+          u32 a = lv.n ? dmax(lv.s_cur.p, lv.n) : 0;
+          u32 b = lv.n ? dmax(lv.s_pre.p, lv.n) : 0;
+          cur_n = lv.n ? std::max(a, b) + 1 : 0;
+          cur_len = 0;
+          continue;
+        }
+        u32 nn = 0;
+        u64 kk = aggregate(lv.csr(), dcur.p, cur_len, lv.s_pre.p, lv.s_cur.p, lv.R.p, dnext, &nn, ag, tmp, s);
+        dcur.swap(dnext);
+        cur_len = kk;
+        cur_n = nn;
+        if (st) st->bytes_alg += 80 * kk;
+      }
+    }
+    def_update_maps(false);
+    def_update_maps(true);
+    HL_CHECK(hipStreamSynchronize(s));
+    if (st) {
+      st->iterations = total_iters;
+      st->levels = u32(P);
+    }
+    return total_iters;
+  }
+
+  void reset(u32 n, u64 m_hint) {
+    L.clear();
+    for (int p = 0; p < P; ++p) L.emplace_back(new Level());
+    n0 = n;
+    Level& l0 = *L[0];
+    grow_level(0, n);
+    u64 want = 2 * m_hint + 1024;
+    l0.col.reserve(want, s, false, 1.25);
+    l0.w.reserve(want, s, false, 1.25);
+    l0.pool_used = 0;
+    for (int p = 0; p < P; ++p) {
+      L[p]->twice_m.reserve(1, s);
+      fill_f64(L[p]->twice_m.p, 0.0, 1, s);
+    }
+    loaded = true;
+  }
+
+  void upload_batch(u64 k, const u32* src, const u32* dst, const double* w, u32 n_lim) {
+    std::vector<Rec> h(k);
+    for (u64 i = 0; i < k; ++i) {
+      if (src[i] >= n_lim || dst[i] >= n_lim) throw InputError("endpoint id out of range");
+      double x = w ? w[i] : 1.0;
+      if (!std::isfinite(x)) throw InputError("non-finite weight");
+      h[i] = Rec{src[i], dst[i], x};
+    }
+    dcur.reserve(k + 1, s);
+    if (k) HL_CHECK(hipMemcpyAsync(dcur.p, h.data(), k * sizeof(Rec), hipMemcpyHostToDevice, s));
+  }
+
+  const Level& level(int p) const {
+    if (p < 0 || p >= P) throw InputError("level out of range");
+    return *L[p];
+  }
+  DevBuf<u32>& map_of(Level& lv, int which) {
+    switch (which) {
+      case HL_MAP_F: return lv.f;
+      case HL_MAP_G: return lv.g;
+      case HL_MAP_S_CUR: return lv.s_cur;
+      case HL_MAP_S_PRE: return lv.s_pre;
+      default: throw InputError("unknown map");
+    }
+  }
+  DevBuf<u8>& set_of(Level& lv, int which) {
+    switch (which) {
+      case HL_SET_ACTIVE: return lv.A;
+      case HL_SET_CHANGED: return lv.B;
+      case HL_SET_AFFECTED: return lv.K;
+      case HL_SET_REFINED: return lv.R;
+      default: throw InputError("unknown set");
+    }
+  }
+
+  // community labels f^{P-1} o s^{P-2} o .. o s^0 (or rows of the hierarchy) into lab
+  void compose(int upto, bool with_f, u32* out) {
+    iota_u32(out, 0, n0, s);
+    for (int p = 0; p < upto; ++p) gather_u32(out, L[p]->s_cur.p, n0, s);
+    if (with_f) gather_u32(out, L[upto]->f.p, n0, s);
+  }
+};
+
+}  // namespace hl
+
+using hl::Engine;
+
+struct hl_graph {
+  Engine e;
+};
+
+namespace {
+thread_local std::string g_create_err;
+
+template <class F>
+hl_status guard(const hl_graph* gc, F&& fn) {
+  hl_graph* g = const_cast<hl_graph*>(gc);
+  try {
+    fn();
+    if (g) g->e.err.clear();
+    return HL_OK;
+  } catch (const hl::InputError& x) {
+    if (g) g->e.err = std::string("invalid input: ") + x.what();
+    return HL_INVALID_INPUT;
+  } catch (const hl::HipError& x) {
+    if (g) g->e.err = std::string("backend: ") + x.what();
+    return x.code == hipErrorOutOfMemory ? HL_OOM : HL_BACKEND;
+  } catch (const std::bad_alloc&) {
+    if (g) g->e.err = "host allocation failed";
+    return HL_OOM;
+  } catch (const std::exception& x) {
+    if (g) g->e.err = std::string("backend: ") + x.what();
+    return HL_BACKEND;
+  }
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+extern "C" {
+
+hl_status hl_create(const hl_config* cfg, hl_graph** out) {
+  if (!cfg || !out) {
+    g_create_err = "null argument";
+    return HL_INVALID_INPUT;
+  }
+  *out = nullptr;
+  if (cfg->levels < 1 || cfg->levels > HL_MAX_LEVELS || !(cfg->gamma > 0.0) || !std::isfinite(cfg->gamma) ||
+      (cfg->mode != HL_MODE_DETERMINISTIC && cfg->mode != HL_MODE_THROUGHPUT)) {
+    g_create_err = "invalid config";
+    return HL_INVALID_INPUT;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) {
+    g_create_err = "no HIP device";
+    return HL_ACCELERATION;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess ||
+      std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    g_create_err = std::string("device is not gfx950: ") + prop.gcnArchName;
+    return HL_ACCELERATION;
+  }
+  hl_graph* g = new (std::nothrow) hl_graph();
+  if (!g) return HL_OOM;
+  hl_status st = guard(g, [&] {
+    HL_CHECK(hipSetDevice(cfg->device));
+    HL_CHECK(hipStreamCreateWithFlags(&g->e.s, hipStreamNonBlocking));
+    g->e.cfg = *cfg;
+    g->e.P = cfg->levels;
+    g->e.mode = cfg->mode;
+    if (cfg->small_degree) g->e.th.small_deg = cfg->small_degree;
+    if (cfg->wave_degree) g->e.th.wave_deg = cfg->wave_degree;
+  });
+  if (st != HL_OK) {
+    g_create_err = g->e.err;
+    delete g;
+    return st;
+  }
+  *out = g;
+  return HL_OK;
+}
+
+void hl_destroy(hl_graph* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->e.cfg.device);
+  delete g;
+}
+
+const char* hl_create_error(void) { return g_create_err.c_str(); }
+
+const char* hl_last_error(const hl_graph* g) { return g ? g->e.err.c_str() : "null handle"; }
+
+hl_status hl_load_graph(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                        const double* w) {
+  if (!g || (m && (!src || !dst))) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = g->e;
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    e.upload_batch(m, src, dst, w, n);
+    e.reset(n, m);
+    e.apply_delta_level(*e.L[0], e.dcur.p, m, nullptr, 0);
+    HL_CHECK(hipStreamSynchronize(e.s));
+  });
+}
+
+hl_status hl_load(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, const double* w,
+                  hl_batch_stats* stats) {
+  if (!g || (m && (!src || !dst))) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = g->e;
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    double t0 = now_s();
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    e.upload_batch(m, src, dst, w, n);
+    e.reset(n, m);
+    e.run_batch(m, stats);
+    if (stats) stats->seconds = now_s() - t0;
+  });
+}
+
+namespace {
+__global__ void k_pack_batch(const uint32_t* a, const uint32_t* b, const double* w, uint64_t k, uint32_t n,
+                             hl::Rec* out, uint32_t* bad) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < k; i += uint64_t(gridDim.x) * blockDim.x) {
+    uint32_t u = a[i], v = b[i];
+    double x = w ? w[i] : 1.0;
+    if (u >= n || v >= n || !isfinite(x)) *bad = 1;
+    out[i] = hl::Rec{u, v, x};
+  }
+}
+}  // namespace
+
+hl_status hl_apply_batch(hl_graph* g, uint64_t k, const uint32_t* src, const uint32_t* dst, const double* alpha,
+                         hl_batch_stats* stats) {
+  if (!g || (k && (!src || !dst))) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = g->e;
+    if (!e.loaded) throw hl::InputError("no graph loaded");
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    double t0 = now_s();
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    e.upload_batch(k, src, dst, alpha, e.n0);
+    e.run_batch(k, stats);
+    if (stats) stats->seconds = now_s() - t0;
+  });
+}
+
+hl_status hl_apply_batch_device(hl_graph* g, uint64_t k, const uint32_t* d_src, const uint32_t* d_dst,
+                                const double* d_alpha, hl_batch_stats* stats) {
+  if (!g || (k && (!d_src || !d_dst))) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = g->e;
+    if (!e.loaded) throw hl::InputError("no graph loaded");
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    double t0 = now_s();
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    e.dcur.reserve(k + 1, e.s);
+    e.mx.reserve(1, e.s);
+    hl::fill_u32(e.mx.p, 0, 1, e.s);
+    if (k)
+      hipLaunchKernelGGL(k_pack_batch, hl::blocks_for(k), 256, 0, e.s, d_src, d_dst, d_alpha, k, e.n0, e.dcur.p,
+                         e.mx.p);
+    HL_LAUNCH_CHECK();
+    uint32_t bad = 0;
+    HL_CHECK(hipMemcpyAsync(&bad, e.mx.p, sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
+    HL_CHECK(hipStreamSynchronize(e.s));
+    if (bad) throw hl::InputError("endpoint id out of range or non-finite weight");
+    e.run_batch(k, stats);
+    if (stats) stats->seconds = now_s() - t0;
+  });
+}
+
+hl_status hl_level_count(const hl_graph* g, int32_t* levels) {
+  if (!g || !levels) return HL_INVALID_INPUT;
+  *levels = g->e.P;
+  return HL_OK;
+}
+
+hl_status hl_level_size(const hl_graph* g, int32_t level, uint32_t* n) {
+  if (!g || !n) return HL_INVALID_INPUT;
+  return guard(g, [&] { *n = g->e.level(level).n; });
+}
+
+hl_status hl_read_partition(const hl_graph* g, int32_t level, int32_t which, uint32_t* out, uint64_t cap) {
+  if (!g || !out) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = const_cast<Engine&>(g->e);
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    const hl::Level& lc = e.level(level);
+    hl::Level& lv = const_cast<hl::Level&>(lc);
+    if (cap < lv.n) throw hl::InputError("output buffer too small");
+    if (lv.n)
+      HL_CHECK(hipMemcpyAsync(out, e.map_of(lv, which).p, lv.n * sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
+    HL_CHECK(hipStreamSynchronize(e.s));
+  });
+}
+
+hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const uint32_t* in, uint64_t n) {
+  if (!g || (n && !in)) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = g->e;
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    hl::Level& lv = const_cast<hl::Level&>(e.level(level));
+    if (n != lv.n) throw hl::InputError("length differs from the level size");
+    if (n) HL_CHECK(hipMemcpyAsync(e.map_of(lv, which).p, in, n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));
+    HL_CHECK(hipStreamSynchronize(e.s));
+  });
+}
+
+hl_status hl_read_frontier(const hl_graph* g, int32_t level, int32_t which, uint8_t* out, uint64_t cap) {
+  if (!g || !out) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = const_cast<Engine&>(g->e);
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    hl::Level& lv = const_cast<hl::Level&>(e.level(level));
+    if (cap < lv.n) throw hl::InputError("output buffer too small");
+    if (lv.n) HL_CHECK(hipMemcpyAsync(out, e.set_of(lv, which).p, lv.n, hipMemcpyDeviceToHost, e.s));
+    HL_CHECK(hipStreamSynchronize(e.s));
+  });
+}
+
+hl_status hl_read_level_graph(const hl_graph* g, int32_t level, uint64_t* offsets, uint32_t* cols, double* w,
+                              uint64_t cap, uint64_t* nnz) {
+  if (!g || !nnz) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = const_cast<Engine&>(g->e);
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    hl::Level& lv = const_cast<hl::Level&>(e.level(level));
+    const uint32_t n = lv.n;
+    std::vector<uint32_t> len(n);
+    if (n) HL_CHECK(hipMemcpyAsync(len.data(), lv.row_len.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
+    HL_CHECK(hipStreamSynchronize(e.s));
+    uint64_t total = 0;
+    for (uint32_t v = 0; v < n; ++v) total += len[v];
+    *nnz = total;
+    if (!offsets && !cols && !w) return;
+    if (!offsets || !cols || !w) throw hl::InputError("need offsets, cols and w");
+    if (cap < total) throw hl::InputError("output buffer too small");
+    hl::DevBuf<uint32_t> c2;
+    hl::DevBuf<double> w2;
+    c2.reserve(total + 1, e.s);
+    w2.reserve(total + 1, e.s);
+    // copy into a compact temporary without touching the live rows
+    hl::LevelCsrMut m = lv.mut();
+    hl::DevBuf<uint64_t> offs;
+    hl::DevBuf<uint64_t> lens;
+    lens.reserve(n + 1, e.s);
+    offs.reserve(n + 1, e.s);
+    std::vector<uint64_t> hl64(n);
+    for (uint32_t v = 0; v < n; ++v) hl64[v] = len[v];
+    if (n) HL_CHECK(hipMemcpyAsync(lens.p, hl64.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, e.s));
+    hl::exclusive_scan_u64(lens.p, offs.p, n, e.tmp, e.s);
+    std::vector<uint64_t> hoff(n + 1);
+    HL_CHECK(hipMemcpyAsync(hoff.data(), offs.p, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, e.s));
+    std::vector<uint64_t> roff(n);
+    if (n) HL_CHECK(hipMemcpyAsync(roff.data(), m.row_off, n * sizeof(uint64_t), hipMemcpyDeviceToHost, e.s));
+    HL_CHECK(hipStreamSynchronize(e.s));
+    for (uint32_t v = 0; v < n; ++v) {
+      offsets[v] = hoff[v];
+      if (len[v]) {
+        HL_CHECK(hipMemcpyAsync(cols + hoff[v], lv.col.p + roff[v], len[v] * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, e.s));
+        HL_CHECK(hipMemcpyAsync(w + hoff[v], lv.w.p + roff[v], len[v] * sizeof(double), hipMemcpyDeviceToHost, e.s));
+      }
+    }
+    offsets[n] = total;
+    HL_CHECK(hipStreamSynchronize(e.s));
+  });
+}
+
+hl_status hl_read_level_delta(const hl_graph* g, int32_t level, uint32_t* u, uint32_t* v, double* w, uint64_t cap,
+                              uint64_t* k) {
+  if (!g || !k) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = const_cast<Engine&>(g->e);
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    hl::Level& lv = const_cast<hl::Level&>(e.level(level));
+    *k = lv.dG_len;
+    if (!u && !v && !w) return;
+    if (!u || !v || !w) throw hl::InputError("need u, v and w");
+    if (cap < lv.dG_len) throw hl::InputError("output buffer too small");
+    std::vector<hl::Rec> h(lv.dG_len);
+    if (lv.dG_len)
+      HL_CHECK(hipMemcpyAsync(h.data(), lv.dG.p, lv.dG_len * sizeof(hl::Rec), hipMemcpyDeviceToHost, e.s));
+    HL_CHECK(hipStreamSynchronize(e.s));
+    for (uint64_t i = 0; i < lv.dG_len; ++i) {
+      u[i] = h[i].u;
+      v[i] = h[i].v;
+      w[i] = h[i].w;
+    }
+  });
+}
+
+hl_status hl_twice_m(const hl_graph* g, int32_t level, double* twice_m) {
+  if (!g || !twice_m) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = const_cast<Engine&>(g->e);
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    hl::Level& lv = const_cast<hl::Level&>(e.level(level));
+    HL_CHECK(hipMemcpyAsync(twice_m, lv.twice_m.p, sizeof(double), hipMemcpyDeviceToHost, e.s));
+    HL_CHECK(hipStreamSynchronize(e.s));
+  });
+}
+
+hl_status hl_read_hierarchy(const hl_graph* g, uint32_t* out, uint64_t cap, int32_t canonical) {
+  if (!g || !out) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = const_cast<Engine&>(g->e);
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    if (!e.loaded) throw hl::InputError("no graph loaded");
+    const uint64_t n = e.n0;
+    if (cap < n * uint64_t(e.P)) throw hl::InputError("output buffer too small");
+    e.lab.reserve(n + 1, e.s);
+    for (int p = 0; p < e.P; ++p) {
+      if (p < e.P - 1)
+        e.compose(p + 1, false, e.lab.p);
+      else
+        e.compose(e.P - 1, true, e.lab.p);
+      if (canonical) {
+        hl::DevBuf<uint8_t> fl;
+        hl::DevBuf<uint64_t> pos;
+        hl::canonicalize(e.lab.p, uint32_t(n), e.first, fl, pos, e.tmp, e.s);
+      }
+      if (n) HL_CHECK(hipMemcpyAsync(out + uint64_t(p) * n, e.lab.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
+    }
+    HL_CHECK(hipStreamSynchronize(e.s));
+  });
+}
+
+hl_status hl_modularity(const hl_graph* g, int32_t which, double* q) {
+  if (!g || !q) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = const_cast<Engine&>(g->e);
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    if (!e.loaded) throw hl::InputError("no graph loaded");
+    e.lab.reserve(e.n0 + 1, e.s);
+    if (which == HL_PART_COMMUNITY)
+      e.compose(e.P - 1, true, e.lab.p);
+    else if (which == HL_PART_REFINED)
+      HL_CHECK(hipMemcpyAsync(e.lab.p, e.L[0]->g.p, e.n0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, e.s));
+    else if (which == HL_PART_F0)
+      HL_CHECK(hipMemcpyAsync(e.lab.p, e.L[0]->f.p, e.n0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, e.s));
+    else
+      throw hl::InputError("unknown partition");
+    *q = hl::modularity(e.L[0]->csr(), e.lab.p, e.sig, e.acc, e.mx, e.s);
+  });
+}
+
+hl_status hl_modularity_labels(const hl_graph* g, const uint32_t* labels, uint64_t n, double* q) {
+  if (!g || !q || (n && !labels)) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = const_cast<Engine&>(g->e);
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    if (!e.loaded) throw hl::InputError("no graph loaded");
+    if (n != e.n0) throw hl::InputError("labels length differs from n");
+    e.lab.reserve(n + 1, e.s);
+    if (n) HL_CHECK(hipMemcpyAsync(e.lab.p, labels, n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));
+    *q = hl::modularity(e.L[0]->csr(), e.lab.p, e.sig, e.acc, e.mx, e.s);
+  });
+}
+
+}  // extern "C"

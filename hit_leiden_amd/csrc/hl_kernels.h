@@ -1,0 +1,174 @@
+// hl_kernels.h -- launchers of the gfx950 kernels (defined in hl_kernels.hip).
+#pragma once
+#include "hl_common.h"
+
+namespace hl {
+
+// Level CSR view: rows sorted by neighbour, self-loop stored once with its
+// record weight (it counts twice in every row sum).
+struct Csr {
+  u32 n;
+  const u64* row_off;
+  const u32* row_len;
+  const u32* col;
+  const double* w;
+  const double* k;        // weighted degrees
+  const double* twice_m;  // device scalar
+};
+
+// scratch allocator used by the launchers for library temporaries (rocprim)
+struct Temp {
+  DevBuf<u8> buf;
+  void* get(size_t bytes, hipStream_t s) {
+    buf.reserve(bytes, s);
+    return buf.p;
+  }
+};
+
+// ---- generic ----
+void fill_u8(u8* p, u8 v, u64 n, hipStream_t s);
+void fill_u32(u32* p, u32 v, u64 n, hipStream_t s);
+void fill_f64(double* p, double v, u64 n, hipStream_t s);
+void iota_u32(u32* p, u32 base, u64 n, hipStream_t s);
+void copy_u8(u8* dst, const u8* src, u64 n, hipStream_t s);
+u64 count_u8(const u8* a, u64 n, DevBuf<u64>& cnt, hipStream_t s);
+// *out = max(*out, max(a[0..n)))  (out must be initialised)
+void max_u32(const u32* a, u64 n, u32* out, hipStream_t s);
+// indices i with flags[i] != 0, ascending; returns the count (synchronises)
+u64 select_flagged(const u8* flags, u64 n, u32* out, Temp& tmp, DevBuf<u64>& cnt, hipStream_t s);
+// exclusive scan of u64 values, writes n+1 entries (last = total); returns total
+u64 exclusive_scan_u64(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s);
+// stable sort of (key, value) pairs by key (keys_out/vals_out separate)
+void sort_pairs_u64(const u64* kin, u64* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
+                    hipStream_t s);
+void sort_pairs_u32(const u32* kin, u32* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
+                    hipStream_t s);
+
+// ---- graph maintenance ----
+struct ApplyScratch {
+  DevBuf<u64> keys, keys_s, ukey, ukey_s, newcap, newoff, chunks, chunkbase, cnt;
+  DevBuf<u32> idx, idx_s, runs, uidx, perm, segs, seg_row, seg_len, new_len, ins_pre, del_pre, seg_ins, seg_del;
+  DevBuf<u8> heads, uop;
+  DevBuf<double> uw;
+  DevBuf<u32> ucount;
+};
+
+struct LevelCsrMut {
+  u32 n;
+  u64* row_off;
+  u32* row_len;
+  u32* row_cap;
+  u32* col;
+  double* w;
+  double* k;
+  double* twice_m;
+};
+
+// Plans and applies G <- G (+) dG (in_memory.rs:103-138).  Returns the number of
+// affected rows and writes into *need the pool entries the relocation needs;
+// the engine must call apply_delta_commit after making room.
+struct ApplyPlan {
+  u64 nruns = 0, nupd = 0, nseg = 0, need = 0, total_chunks = 0, old_entries = 0;
+};
+void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a, Temp& tmp, ApplyPlan& plan,
+                      hipStream_t s);
+void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, const ApplyPlan& plan, bool det,
+                        hipStream_t s);
+// compact all rows into a fresh pool [col2, w2) with row_cap = row_len; returns live nnz
+u64 compact_rows(const LevelCsrMut& g, u32* col2, double* w2, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s);
+void twice_m_chunked(const double* k, u32 n, double* partials, double* out, hipStream_t s);
+void grow_init(u32 n_old, u32 n_new, u32 base_f, u32 base_g, u32 base_s, u32* f, u32* g, u32* s_pre, u32* s_cur,
+               u64* row_off, u32* row_len, u32* row_cap, double* k, hipStream_t s);
+
+// ---- movement ----
+void seed(const Rec* d, u64 k, const u32* f, const u32* s, u8* A, u8* K, hipStream_t st);
+void label_totals_atomic(const u32* lab, const double* k, u32 n, double* tot, hipStream_t s);
+// canonical (vertex-order sequential) per-label sums; tot zero-filled to tot_n first
+void label_totals_canonical(const u32* lab, const double* k, u32 n, double* tot, u64 tot_n, DevBuf<u32>& ks,
+                            DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,
+                            DevBuf<u64>& cnt, Temp& tmp, hipStream_t s);
+
+struct ProposeScratch {
+  DevBuf<u32> lists, counts, bigoff_u32;
+  DevBuf<u64> bigsize, bigoff, degsum;
+  DevBuf<u32> hkeys;
+  DevBuf<double> hvals;
+};
+struct Thresholds {
+  u32 small_deg = 16;
+  u32 wave_deg = 512;
+};
+// Throughput-mode proposals (parallel_frontier.rs:155-236) for active[0..cnt):
+// cto[i] = target community or kNone, cgain[i] = gain.  Returns the summed degree.
+u64 propose_moves(const Csr& g, const u32* f, const double* tot, double gamma, const u32* active, u64 cnt,
+                  u32* cto, double* cgain, ProposeScratch& ps, const Thresholds& th, Temp& tmp, hipStream_t s);
+
+struct DecoupleScratch {
+  DevBuf<u32> cidx, order, from, to, node, state, minUto, minUfrom, minAto, minAfrom, undec;
+  DevBuf<u64> keys, keys_s, cnt;
+  DevBuf<u8> flags;
+};
+// select_decoupled_moves (throughput.rs:204-236) as exact priority rounds;
+// applies the accepted moves, marks next-active and K.  Returns accepted count.
+u64 decouple_and_apply(const Csr& g, u32* f, const u32* s, double* tot, u64 comm_cap, const u32* active, u64 cnt,
+                       const u32* cto, const double* cgain, u8* B, u8* K, u8* next, DecoupleScratch& ds, Temp& tmp,
+                       hipStream_t st);
+
+// deterministic sweep (hit_leiden.rs:1108-1188) in one workgroup
+struct DetMoveScratch {
+  DevBuf<u32> bm0, bm1, bm2, gkeys, status;
+  DevBuf<double> gvals;
+  DevBuf<u64> tot_len, pops;
+};
+// returns 0 when the sweep finished, 1 when the community-total array needs to
+// grow (the kernel stopped early, state is resumable)
+int det_move(const Csr& g, u32* f, const u32* s, double* tot, u64 tot_cap, u64* tot_len_dev, const u8* A,
+             bool fresh, u8* B, u8* K, double gamma, DetMoveScratch& ds, u32 max_deg, hipStream_t st);
+
+// ---- refinement ----
+struct RefineScratch {
+  DevBuf<u8> aff, incc, keepflag, single, is_cand, blocked, target;
+  DevBuf<u32> vcc, parent, cnt32, roots, roots_s, rank_keys_u32, newid, parentIdx, sizes, rlist, prop, smax;
+  DevBuf<u64> best64, rkeys, rkeys_s, cnt;
+  DevBuf<double> dS, wS, dpar;
+  DevBuf<u32> gk;
+  DevBuf<double> gv;
+  DevBuf<u32> status;
+  DevBuf<u32> mhead, mnext;
+  DevBuf<u64> kkeys, kkeys_s;
+  DevBuf<u32> rsorted;
+};
+// connected-component split of subcommunities touched by K (hit_leiden.rs:1276-1331)
+void cc_split(const Csr& g, u32* s, const u8* K, u8* R, u64 s_space, RefineScratch& rs, Temp& tmp, hipStream_t st);
+bool is_identity(const u32* s, u32 n, RefineScratch& rs, hipStream_t st);
+// throughput refinement (+connected-targets rule) over R
+void refine_parallel(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space,
+                     RefineScratch& rs, ProposeScratch& ps, const Thresholds& th, Temp& tmp, hipStream_t st);
+// deterministic refinement (hit_leiden.rs:1359-1413)
+void refine_det(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space, RefineScratch& rs,
+                DevBuf<u32>& ks, DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,
+                DevBuf<u64>& cnt, u32 max_deg, Temp& tmp, hipStream_t st);
+
+// ---- aggregation (hit_leiden.rs:1639-1731) ----
+struct AggScratch {
+  DevBuf<Rec> H;
+  DevBuf<u32> rlist, base32, runs, idx, idx_s, keep_idx;
+  DevBuf<u64> cnt64, off64, keys, keys_s, cnt;
+  DevBuf<u8> heads, keep;
+};
+// returns |dG^{p+1}| written to out (grown as needed); *next_n = max(s_cur u s_pre)+1
+u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, const u8* R, DevBuf<Rec>& out,
+              u32* next_n, AggScratch& as, Temp& tmp, hipStream_t st);
+
+// ---- def_update (hit_leiden.rs:1733-1793) ----
+void def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, hipStream_t s);
+void def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np, hipStream_t s);
+
+// ---- modularity (hit_leiden.rs:121-161) ----
+double modularity(const Csr& g, const u32* lab, DevBuf<double>& sig, DevBuf<double>& acc, DevBuf<u32>& mx,
+                  hipStream_t s);
+void gather_u32(u32* cur, const u32* map, u32 n, hipStream_t s);
+void canonicalize(u32* lab, u32 n, DevBuf<u32>& first, DevBuf<u8>& flag, DevBuf<u64>& pos, Temp& tmp,
+                  hipStream_t s);
+
+}  // namespace hl

@@ -1,0 +1,2221 @@
+// hl_kernels.hip -- gfx950 kernels of the HIT-Leiden incremental update.
+//
+// Layout: one CSR per level with rows kept sorted by neighbour id.  Rows that a
+// batch touches are rewritten into freshly bump-allocated pool space (the pool
+// is compacted when full), so insert/delete never shifts a row in place and
+// every row stays canonical for the deterministic mode.
+//
+// All floating-point gain formulas keep the reference's operation order and are
+// compiled with -ffp-contract=off, so a gain here equals the CPU's bit for bit
+// for the same inputs.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include <algorithm>
+
+#include "hl_kernels.h"
+
+namespace hl {
+namespace {
+
+constexpr u8 OP_UPD = 1, OP_DEL = 2, OP_INS = 3;
+constexpr u32 MERGE_CHUNK = 1024;
+constexpr u32 SMALL_MAX = 16;   // thread-per-vertex bucket capacity
+constexpr u32 WAVE_HW = 1024;   // LDS hash slots per wavefront
+constexpr u32 DET_LCAP = 4096;  // LDS hash slots of the deterministic sweeps
+constexpr u32 kDecUndec = 0, kDecAcc = 1, kDecRej = 2;
+
+__device__ __forceinline__ u64 umin64(u64 a, u64 b) { return a < b ? a : b; }
+__device__ __forceinline__ u64 gtid() { return u64(blockIdx.x) * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ u64 gstride() { return u64(gridDim.x) * blockDim.x; }
+
+// hit_leiden.rs:1135-1140 / throughput.rs:335-340 (same operation order)
+__device__ __forceinline__ double move_gain(double wc, double wcur, double t, double gamma, double k, double sc,
+                                            double scand) {
+  return (wc - wcur) / t + gamma * k * (sc - k - scand) / (t * t);
+}
+// hit_leiden.rs:1150-1154, parallel_frontier.rs:207-210
+__device__ __forceinline__ double empty_gain(double wcur, double t, double gamma, double k, double sc) {
+  return -wcur / t + gamma * k * (sc - k) / (t * t);
+}
+// hit_leiden.rs:1546, :1630
+__device__ __forceinline__ double remove_gain(double ws, double t, double gamma, double ds, double dp) {
+  return -ws / t + gamma * ds * (dp - ds) / (t * t);
+}
+
+__device__ __forceinline__ u32 hash32(u32 x) { return x * 2654435761u; }
+
+__device__ __forceinline__ u32 lower_bound_u32(const u32* a, u32 n, u32 x) {
+  u32 lo = 0, hi = n;
+  while (lo < hi) {
+    u32 mid = (lo + hi) >> 1;
+    if (a[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// (gain, key) argmax with ties to the smaller key
+__device__ __forceinline__ void better(double& g, u32& k, double og, u32 ok) {
+  if (og > g || (og == g && ok < k)) {
+    g = og;
+    k = ok;
+  }
+}
+
+__device__ __forceinline__ void wave_argmax(double& g, u32& k) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double og = __shfl_xor(g, o, 64);
+    u32 ok = __shfl_xor(k, o, 64);
+    better(g, k, og, ok);
+  }
+}
+
+// block-wide argmax, result valid in every thread (blockDim multiple of 64, <= 1024)
+__device__ void block_argmax(double& g, u32& k) {
+  __shared__ double sg[16];
+  __shared__ u32 sk[16];
+  wave_argmax(g, k);
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    sg[wv] = g;
+    sk[wv] = k;
+  }
+  __syncthreads();
+  g = sg[0];
+  k = sk[0];
+  for (u32 i = 1; i < nw; ++i) better(g, k, sg[i], sk[i]);
+  __syncthreads();
+}
+
+__device__ double block_sum(double x) {
+  __shared__ double s[16];
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) s[wv] = x;
+  __syncthreads();
+  double t = 0.0;
+  for (u32 i = 0; i < nw; ++i) t += s[i];
+  __syncthreads();
+  return t;
+}
+
+// ------------------------------------------------------------------ generic
+__global__ void k_fill_u8(u8* p, u8 v, u64 n) {
+  for (u64 i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+__global__ void k_fill_u32(u32* p, u32 v, u64 n) {
+  for (u64 i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+__global__ void k_fill_f64(double* p, double v, u64 n) {
+  for (u64 i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+__global__ void k_iota_u32(u32* p, u32 base, u64 n) {
+  for (u64 i = gtid(); i < n; i += gstride()) p[i] = base + u32(i);
+}
+__global__ void k_copy_u8(u8* d, const u8* s, u64 n) {
+  for (u64 i = gtid(); i < n; i += gstride()) d[i] = s[i];
+}
+__global__ void k_max_u32(const u32* a, u64 n, u32* out) {
+  u32 m = 0;
+  for (u64 i = gtid(); i < n; i += gstride()) m = max(m, a[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (u32)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+__global__ void k_count_u8(const u8* a, u64 n, unsigned long long* out) {
+  unsigned long long c = 0;
+  for (u64 i = gtid(); i < n; i += gstride()) c += (a[i] != 0);
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+__global__ void k_scan_tail(const u64* in, u64* out, u64 n) {
+  out[n] = n ? out[n - 1] + in[n - 1] : 0;
+}
+
+// ------------------------------------------------------------------ apply delta
+__global__ void k_canon_keys(const Rec* d, u64 k, u64* keys, u32* idx) {
+  for (u64 i = gtid(); i < k; i += gstride()) {
+    Rec r = d[i];
+    u32 a = min(r.u, r.v), b = max(r.u, r.v);
+    keys[i] = (u64(a) << 32) | b;
+    idx[i] = u32(i);
+  }
+}
+__global__ void k_heads_u64(const u64* keys, u64 k, int shift, u8* heads) {
+  for (u64 i = gtid(); i < k; i += gstride()) heads[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift));
+}
+
+// One thread per canonical key: existing weight (or 0) + alpha_1 + ... in delta
+// order (apply_delta_edges, in_memory.rs:133-138); the key survives iff > 1e-12
+// (in_memory.rs:110).  Emits one directed update per row end.
+__global__ void k_resolve_runs(u64 nruns, const u32* runs, u64 k, const u64* keys, const u32* idx, const Rec* d,
+                               LevelCsrMut g, u64* ukey, u32* uidx, u8* uop, double* uw, u32* ucount) {
+  for (u64 r = gtid(); r < nruns; r += gstride()) {
+    u64 s = runs[r], e = (r + 1 < nruns) ? runs[r + 1] : k;
+    u64 key = keys[s];
+    u32 a = u32(key >> 32), b = u32(key & 0xffffffffu);
+    u64 off = g.row_off[a];
+    u32 len = g.row_len[a];
+    u32 pos = lower_bound_u32(g.col + off, len, b);
+    bool found = pos < len && g.col[off + pos] == b;
+    double acc = found ? g.w[off + pos] : 0.0;
+    for (u64 t = s; t < e; ++t) acc += d[idx[t]].w;
+    u8 op;
+    if (acc > kEpsApply)
+      op = found ? OP_UPD : OP_INS;
+    else if (found)
+      op = OP_DEL;
+    else
+      continue;
+    u32 n_out = (a == b) ? 1u : 2u;
+    u32 base = atomicAdd(ucount, n_out);
+    ukey[base] = (u64(a) << 32) | b;
+    uop[base] = op;
+    uw[base] = acc;
+    uidx[base] = base;
+    if (a != b) {
+      ukey[base + 1] = (u64(b) << 32) | a;
+      uop[base + 1] = op;
+      uw[base + 1] = acc;
+      uidx[base + 1] = base + 1;
+    }
+  }
+}
+
+__global__ void k_row_plan(u64 nseg, const u32* segs, u64 nupd, const u64* ukey_s, const u32* perm, const u8* uop,
+                           const u32* row_len, u32* seg_row, u32* seg_len, u32* new_len, u32* seg_ins, u32* seg_del,
+                           u32* ins_pre, u32* del_pre, u64* newcap, u64* chunks, u64* old_sum) {
+  for (u64 j = gtid(); j < nseg; j += gstride()) {
+    u64 s = segs[j], e = (j + 1 < nseg) ? segs[j + 1] : nupd;
+    u32 r = u32(ukey_s[s] >> 32);
+    u32 ins = 0, del = 0;
+    for (u64 t = s; t < e; ++t) {
+      u8 op = uop[perm[t]];
+      ins_pre[t] = ins;
+      del_pre[t] = del;
+      ins += (op == OP_INS);
+      del += (op == OP_DEL);
+    }
+    u32 old = row_len[r];
+    u32 nl = old + ins - del;
+    seg_row[j] = r;
+    seg_len[j] = u32(e - s);
+    new_len[j] = nl;
+    seg_ins[j] = ins;
+    seg_del[j] = del;
+    newcap[j] = nl;
+    u64 items = u64(old) + (e - s);
+    chunks[j] = (items + MERGE_CHUNK - 1) / MERGE_CHUNK;
+    atomicAdd((unsigned long long*)old_sum, (unsigned long long)old);
+  }
+}
+
+// Merge an old sorted row with its sorted updates into the new location.
+// Existing entry t goes to t - #deleted-before + #inserted-before; an insert
+// goes to lower_bound(old row) - #deleted-before + #inserted-before.
+__global__ __launch_bounds__(256) void k_row_merge(u64 nseg, const u64* chunkbase, const u32* segs,
+                                                    const u32* seg_row, const u32* seg_len, const u32* seg_ins,
+                                                    const u32* seg_del, const u64* ukey_s, const u32* perm,
+                                                    const u8* uop, const double* uw, const u32* ins_pre,
+                                                    const u32* del_pre, const u64* newoff, u64 pool_base,
+                                                    LevelCsrMut g) {
+  u64 b = blockIdx.x;
+  // segment j with chunkbase[j] <= b < chunkbase[j+1]
+  u64 lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    u64 mid = (lo + hi) >> 1;
+    if (chunkbase[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const u64 j = lo;
+  const u64 c = b - chunkbase[j];
+  const u32 r = seg_row[j];
+  const u64 oo = g.row_off[r];
+  const u32 olen = g.row_len[r];
+  const u64 s = segs[j];
+  const u32 ul = seg_len[j];
+  const u64 dst = pool_base + newoff[j];
+  const u64 items = u64(olen) + ul;
+  const u64 end = umin64(items, (c + 1) * MERGE_CHUNK);
+  for (u64 t = c * MERGE_CHUNK + threadIdx.x; t < end; t += blockDim.x) {
+    if (t < olen) {
+      u32 col = g.col[oo + t];
+      // first update with col >= this col
+      u32 lo2 = 0, hi2 = ul;
+      while (lo2 < hi2) {
+        u32 mid = (lo2 + hi2) >> 1;
+        if (u32(ukey_s[s + mid] & 0xffffffffu) < col)
+          lo2 = mid + 1;
+        else
+          hi2 = mid;
+      }
+      u32 q = lo2;
+      u32 dp = q < ul ? del_pre[s + q] : seg_del[j];
+      u32 ip = q < ul ? ins_pre[s + q] : seg_ins[j];
+      bool match = q < ul && u32(ukey_s[s + q] & 0xffffffffu) == col;
+      double w = g.w[oo + t];
+      if (match) {
+        u32 pi = perm[s + q];
+        u8 op = uop[pi];
+        if (op == OP_DEL) continue;
+        w = uw[pi];
+      }
+      u64 pos = t - dp + ip;
+      g.col[dst + pos] = col;
+      g.w[dst + pos] = w;
+    } else {
+      u32 q = u32(t - olen);
+      u32 pi = perm[s + q];
+      if (uop[pi] != OP_INS) continue;
+      u32 col = u32(ukey_s[s + q] & 0xffffffffu);
+      u32 lb = lower_bound_u32(g.col + oo, olen, col);
+      u64 pos = u64(lb) - del_pre[s + q] + ins_pre[s + q];
+      g.col[dst + pos] = col;
+      g.w[dst + pos] = uw[pi];
+    }
+  }
+}
+
+__global__ void k_row_commit(u64 nseg, const u32* seg_row, const u32* new_len, const u64* newoff, u64 pool_base,
+                             LevelCsrMut g) {
+  for (u64 j = gtid(); j < nseg; j += gstride()) {
+    u32 r = seg_row[j];
+    g.row_off[r] = pool_base + newoff[j];
+    g.row_len[r] = new_len[j];
+    g.row_cap[r] = new_len[j];
+  }
+}
+
+// k_v = sequential sum over the sorted row, self-loop counted twice (the order
+// of the canonical CPU oracle; equal to any order for integer weights)
+__global__ void k_row_degree(u64 nseg, const u32* seg_row, LevelCsrMut g) {
+  for (u64 j = gtid(); j < nseg; j += gstride()) {
+    u32 r = seg_row[j];
+    u64 off = g.row_off[r];
+    u32 len = g.row_len[r];
+    double acc = 0.0;
+    for (u32 i = 0; i < len; ++i) {
+      double w = g.w[off + i];
+      acc += w;
+      if (g.col[off + i] == r) acc += w;
+    }
+    g.k[r] = acc;
+  }
+}
+
+__global__ void k_copy_rows(LevelCsrMut g, const u64* newoff, u32* col2, double* w2) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (u64 r = wave; r < g.n; r += nwaves) {
+    u64 o = g.row_off[r], d = newoff[r];
+    u32 len = g.row_len[r];
+    for (u32 i = lane; i < len; i += 64) {
+      col2[d + i] = g.col[o + i];
+      w2[d + i] = g.w[o + i];
+    }
+  }
+}
+__global__ void k_set_offsets(LevelCsrMut g, const u64* newoff) {
+  for (u64 r = gtid(); r < g.n; r += gstride()) {
+    g.row_off[r] = newoff[r];
+    g.row_cap[r] = g.row_len[r];
+  }
+}
+__global__ void k_len_u64(const u32* len, u32 n, u64* out) {
+  for (u64 i = gtid(); i < n; i += gstride()) out[i] = len[i];
+}
+
+// canonical 2m: chunks of 4096 vertices summed sequentially, then the chunk
+// partials summed sequentially in chunk order
+__global__ void k_chunk_partials(const double* k, u32 n, double* part) {
+  u64 c = gtid();
+  u64 nc = (u64(n) + kSumChunk - 1) / kSumChunk;
+  if (c >= nc) return;
+  u64 b = c * kSumChunk, e = umin64(n, b + kSumChunk);
+  double p = 0.0;
+  for (u64 i = b; i < e; ++i) p += k[i];
+  part[c] = p;
+}
+__global__ void k_sum_partials(const double* part, u64 nc, double* out) {
+  double t = 0.0;
+  for (u64 i = 0; i < nc; ++i) t += part[i];
+  *out = t;
+}
+
+__global__ void k_grow_init(u32 n_old, u32 n_new, u32 bf, u32 bg, u32 bs, u32* f, u32* g, u32* s_pre, u32* s_cur,
+                            u64* row_off, u32* row_len, u32* row_cap, double* k) {
+  u64 cnt = n_new - n_old;
+  for (u64 i = gtid(); i < cnt; i += gstride()) {
+    u32 v = n_old + u32(i);
+    f[v] = bf + u32(i);
+    g[v] = bg + u32(i);
+    s_pre[v] = bs + u32(i);
+    s_cur[v] = bs + u32(i);
+    row_off[v] = 0;
+    row_len[v] = 0;
+    row_cap[v] = 0;
+    k[v] = 0.0;
+  }
+}
+
+// ------------------------------------------------------------------ seeding
+// hit_leiden.rs:1037-1052
+__global__ void k_seed(const Rec* d, u64 k, const u32* f, const u32* s, u8* A, u8* K) {
+  for (u64 i = gtid(); i < k; i += gstride()) {
+    Rec r = d[i];
+    bool same_f = f[r.u] == f[r.v];
+    if ((r.w > 0.0 && !same_f) || (r.w < 0.0 && same_f)) {
+      A[r.u] = 1;
+      A[r.v] = 1;
+    }
+    if (s[r.u] == s[r.v]) {
+      K[r.u] = 1;
+      K[r.v] = 1;
+    }
+  }
+}
+
+__global__ void k_label_totals_atomic(const u32* lab, const double* k, u32 n, double* tot) {
+  for (u64 v = gtid(); v < n; v += gstride()) atomicAdd(&tot[lab[v]], k[v]);
+}
+// per label segment (vertices ascending): sequential sum
+__global__ void k_segment_sums(u64 nseg, const u32* starts, u64 n, const u32* keys_s, const u32* vals_s,
+                               const double* k, double* tot) {
+  for (u64 j = gtid(); j < nseg; j += gstride()) {
+    u64 s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : n;
+    double acc = 0.0;
+    for (u64 t = s; t < e; ++t) acc += k[vals_s[t]];
+    tot[keys_s[s]] = acc;
+  }
+}
+__global__ void k_heads_u32(const u32* keys, u64 n, u8* heads) {
+  for (u64 i = gtid(); i < n; i += gstride()) heads[i] = (i == 0 || keys[i] != keys[i - 1]);
+}
+
+// ------------------------------------------------------------------ proposals
+// Throughput movement (execute_shard, parallel_frontier.rs:155-236): self-loops
+// skipped, empty community id = n + v.
+struct MovePolicy {
+  const u32* f;
+  const double* tot;
+  double gamma;
+  u32 n;
+  static constexpr bool kEmpty = true;
+  __device__ bool accept(u32 v, u32 u, u32& key, int& mult) const {
+    if (u == v) return false;
+    key = f[u];
+    mult = 1;
+    return true;
+  }
+  __device__ u32 cur_key(u32 v) const { return f[v]; }
+  __device__ double cur_total(u32 c) const { return tot[c]; }
+  __device__ bool ok(u32) const { return true; }
+  __device__ double cand_total(u32 c) const { return tot[c]; }
+  __device__ u32 empty_id(u32 v) const { return n + v; }
+};
+// Throughput refinement (try_refinement_move, throughput.rs:291-359): neighbours
+// in the same community keyed by subcommunity, self-loop entries counted twice,
+// blocked subcommunities skipped, no empty candidate.
+struct RefinePolicy {
+  const u32* f;
+  const u32* s;
+  const double* dS;
+  const u8* blocked;
+  double gamma;
+  static constexpr bool kEmpty = false;
+  __device__ bool accept(u32 v, u32 u, u32& key, int& mult) const {
+    if (f[u] != f[v]) return false;
+    key = s[u];
+    mult = (u == v) ? 2 : 1;
+    return true;
+  }
+  __device__ u32 cur_key(u32 v) const { return s[v]; }
+  __device__ double cur_total(u32 c) const { return dS[c]; }
+  __device__ bool ok(u32 c) const { return !blocked[c]; }
+  __device__ double cand_total(u32 c) const { return dS[c]; }
+  __device__ u32 empty_id(u32) const { return kNone; }
+};
+
+template <class P>
+__device__ __forceinline__ void finish_choice(const P& pol, u32 v, u32 cur, double wcur, double kv, double t,
+                                              double& bg, u32& best) {
+  if constexpr (P::kEmpty) {
+    double eg = empty_gain(wcur, t, pol.gamma, kv, pol.cur_total(cur));
+    if (eg > bg) {
+      bg = eg;
+      best = pol.empty_id(v);
+    }
+  }
+}
+
+// one thread per vertex, deg <= SMALL_MAX; per-thread LDS slice, row order sums
+template <class P>
+__global__ __launch_bounds__(256) void k_prop_small(P pol, Csr g, const u32* list, u32 cnt, const u32* verts,
+                                                     u32* cto, double* cgain) {
+  __shared__ u32 sk[SMALL_MAX * 256];
+  __shared__ double sv[SMALL_MAX * 256];
+  const u32 t = threadIdx.x;
+  const u64 id = gtid();
+  if (id >= cnt) return;
+  const u32 i = list[id];
+  const u32 v = verts[i];
+  const u64 off = g.row_off[v];
+  const u32 deg = g.row_len[v];
+  const u32 cur = pol.cur_key(v);
+  int nk = 0;
+  for (u32 j = 0; j < deg; ++j) {
+    u32 u = g.col[off + j];
+    double w = g.w[off + j];
+    u32 key;
+    int mult;
+    if (!pol.accept(v, u, key, mult)) continue;
+    int q = 0;
+    while (q < nk && sk[q * 256 + t] != key) ++q;
+    if (q == nk) {
+      sk[q * 256 + t] = key;
+      sv[q * 256 + t] = 0.0;
+      ++nk;
+    }
+    sv[q * 256 + t] += w;
+    if (mult == 2) sv[q * 256 + t] += w;
+  }
+  double wcur = 0.0;
+  for (int q = 0; q < nk; ++q)
+    if (sk[q * 256 + t] == cur) wcur = sv[q * 256 + t];
+  const double kv = g.k[v], tt = *g.twice_m, sc = pol.cur_total(cur);
+  double bg = 0.0;
+  u32 best = kNone;
+  for (int q = 0; q < nk; ++q) {
+    u32 key = sk[q * 256 + t];
+    if (key == cur || !pol.ok(key)) continue;
+    double gn = move_gain(sv[q * 256 + t], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+    if (gn > 0.0) better(bg, best, gn, key);
+  }
+  finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
+  cto[i] = (bg > 0.0) ? best : kNone;
+  cgain[i] = bg;
+}
+
+__device__ __forceinline__ u32 lds_insert(u32* K, u32 cap, u32 key) {
+  u32 h = hash32(key) & (cap - 1);
+  while (true) {
+    u32 old = atomicCAS(&K[h], kNone, key);
+    if (old == kNone || old == key) return h;
+    h = (h + 1) & (cap - 1);
+  }
+}
+__device__ __forceinline__ double table_lookup(const u32* K, const double* V, u32 cap, u32 key) {
+  u32 h = hash32(key) & (cap - 1);
+  while (true) {
+    u32 x = K[h];
+    if (x == key) return V[h];
+    if (x == kNone) return 0.0;
+    h = (h + 1) & (cap - 1);
+  }
+}
+
+// one wavefront per vertex, deg <= WAVE_HW/2; LDS open-addressing table
+template <class P>
+__global__ __launch_bounds__(256) void k_prop_wave(P pol, Csr g, const u32* list, u32 cnt, const u32* verts,
+                                                    u32* cto, double* cgain) {
+  __shared__ u32 hk[4 * WAVE_HW];
+  __shared__ double hv[4 * WAVE_HW];
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  u32* K = hk + wv * WAVE_HW;
+  double* V = hv + wv * WAVE_HW;
+  for (u32 q = lane; q < WAVE_HW; q += 64) {
+    K[q] = kNone;
+    V[q] = 0.0;
+  }
+  __syncthreads();
+  const double tt = *g.twice_m;
+  for (u64 base = u64(blockIdx.x) * 4; base < cnt; base += u64(gridDim.x) * 4) {
+    const u64 id = base + wv;
+    const bool has = id < cnt;
+    u32 i = 0, v = 0, cur = 0, deg = 0;
+    u64 off = 0;
+    if (has) {
+      i = list[id];
+      v = verts[i];
+      off = g.row_off[v];
+      deg = g.row_len[v];
+      cur = pol.cur_key(v);
+      for (u32 j = lane; j < deg; j += 64) {
+        u32 u = g.col[off + j];
+        u32 key;
+        int mult;
+        if (!pol.accept(v, u, key, mult)) continue;
+        double w = g.w[off + j];
+        u32 h = lds_insert(K, WAVE_HW, key);
+        atomicAdd(&V[h], w);
+        if (mult == 2) atomicAdd(&V[h], w);
+      }
+    }
+    __syncthreads();
+    if (has) {
+      const double wcur = table_lookup(K, V, WAVE_HW, cur);
+      const double kv = g.k[v], sc = pol.cur_total(cur);
+      double bg = 0.0;
+      u32 best = kNone;
+      for (u32 q = lane; q < WAVE_HW; q += 64) {
+        u32 key = K[q];
+        if (key == kNone || key == cur || !pol.ok(key)) continue;
+        double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+        if (gn > 0.0) better(bg, best, gn, key);
+      }
+      wave_argmax(bg, best);
+      if (lane == 0) {
+        finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
+        cto[i] = (bg > 0.0) ? best : kNone;
+        cgain[i] = bg;
+      }
+    }
+    __syncthreads();
+    if (has)
+      for (u32 q = lane; q < WAVE_HW; q += 64) {
+        K[q] = kNone;
+        V[q] = 0.0;
+      }
+    __syncthreads();
+  }
+}
+
+// one workgroup per vertex with a private global-memory table (hubs)
+template <class P>
+__global__ __launch_bounds__(256) void k_prop_big(P pol, Csr g, const u32* list, u32 cnt, const u32* verts,
+                                                   const u64* toff, u32* TK, double* TV, u32* cto, double* cgain) {
+  const u64 id = blockIdx.x;
+  if (id >= cnt) return;
+  const u32 i = list[id];
+  const u32 v = verts[i];
+  const u64 off = g.row_off[v];
+  const u32 deg = g.row_len[v];
+  const u32 cur = pol.cur_key(v);
+  const u32 cap = u32(toff[id + 1] - toff[id]);
+  u32* K = TK + toff[id];
+  double* V = TV + toff[id];
+  for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
+    K[q] = kNone;
+    V[q] = 0.0;
+  }
+  __syncthreads();
+  for (u32 j = threadIdx.x; j < deg; j += blockDim.x) {
+    u32 u = g.col[off + j];
+    u32 key;
+    int mult;
+    if (!pol.accept(v, u, key, mult)) continue;
+    double w = g.w[off + j];
+    u32 h = lds_insert(K, cap, key);
+    atomicAdd(&V[h], w);
+    if (mult == 2) atomicAdd(&V[h], w);
+  }
+  __threadfence();
+  __syncthreads();
+  const double tt = *g.twice_m;
+  const double wcur = table_lookup(K, V, cap, cur);
+  const double kv = g.k[v], sc = pol.cur_total(cur);
+  double bg = 0.0;
+  u32 best = kNone;
+  for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
+    u32 key = K[q];
+    if (key == kNone || key == cur || !pol.ok(key)) continue;
+    double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+    if (gn > 0.0) better(bg, best, gn, key);
+  }
+  block_argmax(bg, best);
+  if (threadIdx.x == 0) {
+    finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
+    cto[i] = (bg > 0.0) ? best : kNone;
+    cgain[i] = bg;
+  }
+}
+
+// degree buckets: lists[0..cnt) small, lists[cnt..2cnt) wave, lists[2cnt..3cnt) big
+__global__ void k_bucket(const u32* verts, u64 cnt, const u32* row_len, u32 small_deg, u32 wave_deg, u32* lists,
+                         u32* counts, u64* degsum) {
+  u64 local = 0;
+  for (u64 i = gtid(); i < cnt; i += gstride()) {
+    u32 d = row_len[verts[i]];
+    local += d;
+    u32 b = d <= small_deg ? 0 : (d <= wave_deg ? 1 : 2);
+    u32 p = atomicAdd(&counts[b], 1u);
+    lists[b * cnt + p] = u32(i);
+  }
+  for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
+  if ((threadIdx.x & 63) == 0 && local) atomicAdd((unsigned long long*)degsum, (unsigned long long)local);
+}
+__global__ void k_big_sizes(const u32* list, u32 cnt, const u32* verts, const u32* row_len, u64* sizes) {
+  for (u64 j = gtid(); j < cnt; j += gstride()) {
+    u32 d = row_len[verts[list[j]]];
+    u64 c = 64;
+    while (c < 2ull * d) c <<= 1;
+    sizes[j] = c;
+  }
+}
+
+// ------------------------------------------------------------------ decoupling
+__global__ void k_flags_not_none(const u32* a, u64 n, u8* flags) {
+  for (u64 i = gtid(); i < n; i += gstride()) flags[i] = a[i] != kNone;
+}
+__global__ void k_gain_keys(const u32* cidx, u64 c, const double* cgain, u64* keys, u32* vals) {
+  for (u64 r = gtid(); r < c; r += gstride()) {
+    keys[r] = ~__double_as_longlong(cgain[cidx[r]]);  // gains > 0: descending order
+    vals[r] = u32(r);
+  }
+}
+__global__ void k_dec_init(const u32* order, const u32* cidx, u64 c, const u32* active, const u32* f, const u32* cto,
+                           u32* node, u32* from, u32* to, u32* state) {
+  for (u64 p = gtid(); p < c; p += gstride()) {
+    u32 i = cidx[order[p]];
+    u32 v = active[i];
+    node[p] = v;
+    from[p] = f[v];
+    to[p] = cto[i];
+    state[p] = kDecUndec;
+  }
+}
+__global__ void k_dec_reset(u64 c, const u32* from, const u32* to, u32* a, u32* b, u32* x, u32* y) {
+  for (u64 p = gtid(); p < c; p += gstride()) {
+    u32 f = from[p], t = to[p];
+    a[f] = kNone; a[t] = kNone;
+    b[f] = kNone; b[t] = kNone;
+    x[f] = kNone; x[t] = kNone;
+    y[f] = kNone; y[t] = kNone;
+  }
+}
+__global__ void k_dec_mark(u64 c, const u32* from, const u32* to, const u32* state, u32* minUto, u32* minUfrom,
+                           u32* minAto, u32* minAfrom) {
+  for (u64 p = gtid(); p < c; p += gstride()) {
+    u32 st = state[p];
+    if (st == kDecUndec) {
+      atomicMin(&minUto[to[p]], u32(p));
+      atomicMin(&minUfrom[from[p]], u32(p));
+    } else if (st == kDecAcc) {
+      atomicMin(&minAto[to[p]], u32(p));
+      atomicMin(&minAfrom[from[p]], u32(p));
+    }
+  }
+}
+// greedy (throughput.rs:217-233): reject when an earlier accepted move entered
+// our source or left our target; wait while an earlier undecided move could.
+__global__ void k_dec_decide(u64 c, const u32* from, const u32* to, u32* state, const u32* minUto,
+                             const u32* minUfrom, const u32* minAto, const u32* minAfrom, u32* undec) {
+  u32 local = 0;
+  for (u64 p = gtid(); p < c; p += gstride()) {
+    if (state[p] != kDecUndec) continue;
+    u32 f = from[p], t = to[p], pp = u32(p);
+    if (minAto[f] < pp || minAfrom[t] < pp)
+      state[p] = kDecRej;
+    else if (minUto[f] < pp || minUfrom[t] < pp)
+      ++local;
+    else
+      state[p] = kDecAcc;
+  }
+  for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
+  if ((threadIdx.x & 63) == 0 && local) atomicAdd(undec, local);
+}
+// sequential completion of the greedy for pathological dependency chains
+__global__ void k_dec_sequential(u64 c, const u32* from, const u32* to, u32* state, u8* emit_acc) {
+  if (gtid() != 0) return;
+  for (u64 p = 0; p < c; ++p) {
+    u32 f = from[p], t = to[p];
+    if (state[p] == kDecUndec) state[p] = ((emit_acc[f] & 2) || (emit_acc[t] & 1)) ? kDecRej : kDecAcc;
+    if (state[p] == kDecAcc) {
+      emit_acc[f] |= 1;
+      emit_acc[t] |= 2;
+    }
+  }
+  for (u64 p = 0; p < c; ++p) {
+    emit_acc[from[p]] = 0;
+    emit_acc[to[p]] = 0;
+  }
+}
+__global__ void k_apply_moves(u64 c, const u32* node, const u32* from, const u32* to, const u32* state,
+                              const double* k, u32* f, double* tot, u8* B, u8* accflag) {
+  for (u64 p = gtid(); p < c; p += gstride()) {
+    bool acc = state[p] == kDecAcc;
+    accflag[p] = acc;
+    if (!acc) continue;
+    u32 v = node[p];
+    double kv = k[v];
+    f[v] = to[p];
+    atomicAdd(&tot[from[p]], -kv);
+    atomicAdd(&tot[to[p]], kv);
+    B[v] = 1;
+  }
+}
+// throughput.rs:189-199: after all moves, neighbours outside the new community
+// become active; same-subcommunity pairs go to K
+__global__ __launch_bounds__(256) void k_mark_moved(Csr g, const u32* acc_list, u64 na, const u32* node,
+                                                     const u32* to, const u32* f, const u32* s, u8* next, u8* K) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (u64 q = wave; q < na; q += nwaves) {
+    u32 p = acc_list[q];
+    u32 v = node[p], c = to[p];
+    u64 off = g.row_off[v];
+    u32 deg = g.row_len[v];
+    u32 sv = s[v];
+    bool kv = false;
+    for (u32 j = lane; j < deg; j += 64) {
+      u32 u = g.col[off + j];
+      if (f[u] != c) next[u] = 1;
+      if (s[u] == sv) {
+        K[u] = 1;
+        kv = true;
+      }
+    }
+    if (__any(kv) && lane == 0) K[v] = 1;
+  }
+}
+
+// ------------------------------------------------------------------ deterministic sweep
+struct DetMoveArgs {
+  Csr g;
+  u32* f;
+  const u32* s;
+  double* tot;
+  u64 tot_cap;
+  u64* tot_len;
+  u32 *bm0, *bm1, *bm2;
+  u32 nw2;
+  u8* B;
+  u8* K;
+  double gamma;
+  u32* gk;
+  double* gv;
+  u32* gused;
+  u32 gcap;
+  u32* status;
+  u64* pops;
+};
+
+__device__ __forceinline__ void bm_set(u32* bm0, u32* bm1, u32* bm2, u32 v) {
+  u32 i0 = v >> 5;
+  u32 old = atomicOr(&bm0[i0], 1u << (v & 31));
+  if (old == 0) {
+    u32 i1 = i0 >> 5;
+    u32 o1 = atomicOr(&bm1[i1], 1u << (i0 & 31));
+    if (o1 == 0) atomicOr(&bm2[i1 >> 5], 1u << (i1 & 31));
+  }
+}
+
+__global__ void k_bm_level0(const u8* A, u32 n, u32* bm0, u64 nw0) {
+  for (u64 w = gtid(); w < nw0; w += gstride()) {
+    u32 x = 0;
+    for (u32 b = 0; b < 32; ++b) {
+      u64 v = w * 32 + b;
+      if (v < n && A[v]) x |= 1u << b;
+    }
+    bm0[w] = x;
+  }
+}
+__global__ void k_bm_summary(const u32* lo, u64 nlo, u32* hi, u64 nhi) {
+  for (u64 w = gtid(); w < nhi; w += gstride()) {
+    u32 x = 0;
+    for (u32 b = 0; b < 32; ++b) {
+      u64 i = w * 32 + b;
+      if (i < nlo && lo[i]) x |= 1u << b;
+    }
+    hi[w] = x;
+  }
+}
+
+// Sequential insert used by the deterministic sweeps (one thread): keeps the
+// per-key sums in row order.
+__device__ __forceinline__ u32 seq_insert(u32* K, u32 cap, u32 key, u32* used, u32& nused) {
+  u32 h = hash32(key) & (cap - 1);
+  while (true) {
+    u32 x = K[h];
+    if (x == key) return h;
+    if (x == kNone) {
+      K[h] = key;
+      used[nused++] = h;
+      return h;
+    }
+    h = (h + 1) & (cap - 1);
+  }
+}
+
+// hit_leiden.rs:1108-1188 in one workgroup: pop the lowest active vertex,
+// accumulate w(v,C) in row order (self-loop entries included), argmax over
+// communities in ascending id, empty community id = current array length.
+__global__ __launch_bounds__(256) void k_det_move(DetMoveArgs a) {
+  __shared__ u32 lk[DET_LCAP];
+  __shared__ double lv[DET_LCAP];
+  __shared__ u32 lused[DET_LCAP];
+  __shared__ u32 cl[256];
+  __shared__ double cw[256];
+  __shared__ u32 s_v, s_done, s_nused;
+  __shared__ double s_wcur;
+  __shared__ u32 s_best;
+  __shared__ double s_bg;
+  const u32 tid = threadIdx.x;
+  for (u32 q = tid; q < DET_LCAP; q += 256) {
+    lk[q] = kNone;
+    lv[q] = 0.0;
+  }
+  if (tid == 0) {
+    s_best = kNone;
+    s_bg = 0.0;
+  }
+  __syncthreads();
+  const Csr& g = a.g;
+  const double tt = *g.twice_m;
+  u64 pops = 0;
+  while (true) {
+    if (tid < 64) {
+      u32 found = kNone;
+      for (u32 base = 0; base < a.nw2; base += 64) {
+        u32 w = (base + tid < a.nw2) ? a.bm2[base + tid] : 0u;
+        unsigned long long m = __ballot(w != 0);
+        if (m) {
+          u32 l = u32(__ffsll((long long)m) - 1);
+          u32 word = __shfl(w, l, 64);
+          found = (base + l) * 32 + u32(__ffs(word) - 1);
+          break;
+        }
+      }
+      if (tid == 0) {
+        if (found == kNone) {
+          s_done = 1;
+        } else {
+          u32 i1 = found;
+          u32 w1 = a.bm1[i1];
+          u32 i0 = i1 * 32 + u32(__ffs(w1) - 1);
+          u32 w0 = a.bm0[i0];
+          u32 v = i0 * 32 + u32(__ffs(w0) - 1);
+          w0 &= ~(1u << (v & 31));
+          a.bm0[i0] = w0;
+          if (w0 == 0) {
+            w1 &= ~(1u << (i0 & 31));
+            a.bm1[i1] = w1;
+            if (w1 == 0) a.bm2[i1 >> 5] &= ~(1u << (i1 & 31));
+          }
+          s_v = v;
+          s_done = 0;
+          s_nused = 0;
+        }
+      }
+    }
+    __syncthreads();
+    if (s_done) break;
+    ++pops;
+    const u32 v = s_v;
+    const u32 cur = a.f[v];
+    const double kv = g.k[v];
+    const u64 off = g.row_off[v];
+    const u32 deg = g.row_len[v];
+    const bool in_lds = deg <= DET_LCAP / 2;
+    u32* HK = in_lds ? lk : a.gk;
+    double* HV = in_lds ? lv : a.gv;
+    u32* used = in_lds ? lused : a.gused;
+    const u32 cap = in_lds ? DET_LCAP : a.gcap;
+    for (u32 base = 0; base < deg; base += 256) {
+      u32 j = base + tid;
+      if (j < deg) {
+        u32 u = g.col[off + j];
+        // self-loops skipped, as parallel_frontier.rs:164-166 does
+        cl[tid] = (u == v) ? kNone : a.f[u];
+        cw[tid] = g.w[off + j];
+      }
+      __syncthreads();
+      if (tid == 0) {
+        u32 m = min(256u, deg - base);
+        u32 nu = s_nused;
+        for (u32 q = 0; q < m; ++q) {
+          if (cl[q] == kNone) continue;
+          u32 h = seq_insert(HK, cap, cl[q], used, nu);
+          HV[h] += cw[q];
+        }
+        s_nused = nu;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) s_wcur = table_lookup(HK, HV, cap, cur);
+    __syncthreads();
+    const double wcur = s_wcur;
+    const double sc = a.tot[cur];
+    const u32 nused = s_nused;
+    double bg = 0.0;
+    u32 best = kNone;
+    for (u32 q = tid; q < nused; q += 256) {
+      u32 h = used[q];
+      u32 key = HK[h];
+      if (key == cur) continue;
+      double gn = move_gain(HV[h], wcur, tt, a.gamma, kv, sc, a.tot[key]);
+      if (gn > 0.0) better(bg, best, gn, key);
+    }
+    block_argmax(bg, best);
+    if (tid == 0) {
+      u64 next_empty = *a.tot_len;
+      double eg = empty_gain(wcur, tt, a.gamma, kv, sc);
+      u64 b64 = best;
+      if (eg > bg) {
+        bg = eg;
+        b64 = next_empty;
+      }
+      s_bg = bg;
+      if (bg > 0.0) {
+        if (b64 >= a.tot_cap) {
+          *a.status = 1;  // grow and resume
+          bm_set(a.bm0, a.bm1, a.bm2, v);
+          s_best = kNone;
+        } else {
+          if (b64 >= next_empty) *a.tot_len = b64 + 1;
+          s_best = u32(b64);
+          a.f[v] = u32(b64);
+          a.tot[cur] -= kv;
+          a.tot[b64] += kv;
+          a.B[v] = 1;
+        }
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+    const u32 bst = s_best;
+    const bool moved = s_bg > 0.0 && bst != kNone;
+    const bool stop = s_bg > 0.0 && bst == kNone;
+    if (moved) {
+      const u32 sv = a.s[v];
+      for (u32 j = tid; j < deg; j += 256) {
+        u32 u = g.col[off + j];
+        if (a.f[u] != bst) bm_set(a.bm0, a.bm1, a.bm2, u);
+        if (a.s[u] == sv) {
+          a.K[u] = 1;
+          a.K[v] = 1;
+        }
+      }
+    }
+    for (u32 q = tid; q < nused; q += 256) {
+      u32 h = used[q];
+      HK[h] = kNone;
+      HV[h] = 0.0;
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (tid == 0) s_best = kNone, s_bg = 0.0;
+    __syncthreads();
+    if (stop) break;
+  }
+  if (tid == 0) atomicAdd((unsigned long long*)a.pops, (unsigned long long)pops);
+}
+
+// ------------------------------------------------------------------ CC split
+__global__ void k_aff_mark(const u8* K, const u32* s, u32 n, u8* aff, u32* any) {
+  for (u64 v = gtid(); v < n; v += gstride())
+    if (K[v]) {
+      aff[s[v]] = 1;
+      *any = 1;
+    }
+}
+__global__ void k_incc(const u32* s, u32 n, const u8* aff, u8* incc) {
+  for (u64 v = gtid(); v < n; v += gstride()) incc[v] = aff[s[v]];
+}
+__global__ void k_uf_init(const u32* vcc, u64 m, u32* parent, u32* cnt) {
+  for (u64 i = gtid(); i < m; i += gstride()) {
+    u32 v = vcc[i];
+    parent[v] = v;
+    cnt[v] = 0;
+  }
+}
+__device__ __forceinline__ u32 uf_find(u32* parent, u32 x) {
+  while (true) {
+    u32 p = parent[x];
+    if (p == x) return x;
+    u32 gp = parent[p];
+    if (gp != p) atomicCAS(&parent[x], p, gp);  // path halving
+    x = p;
+  }
+}
+__device__ __forceinline__ void uf_union(u32* parent, u32 a, u32 b) {
+  while (true) {
+    a = uf_find(parent, a);
+    b = uf_find(parent, b);
+    if (a == b) return;
+    u32 hi = max(a, b), lo = min(a, b);
+    if (atomicCAS(&parent[hi], hi, lo) == hi) return;
+    a = hi;
+    b = lo;
+  }
+}
+// edges (v,u) with s(u) = s(v), u > v: union; roots end as the component minimum
+__global__ __launch_bounds__(256) void k_uf_edges(Csr g, const u32* vcc, u64 m, const u32* s, u32* parent) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (u64 q = wave; q < m; q += nwaves) {
+    u32 v = vcc[q];
+    u64 off = g.row_off[v];
+    u32 deg = g.row_len[v];
+    u32 sv = s[v];
+    for (u32 j = lane; j < deg; j += 64) {
+      u32 u = g.col[off + j];
+      if (u > v && s[u] == sv) uf_union(parent, v, u);
+    }
+  }
+}
+__global__ void k_uf_compress(const u32* vcc, u64 m, u32* parent, u32* cnt) {
+  for (u64 i = gtid(); i < m; i += gstride()) {
+    u32 v = vcc[i];
+    u32 r = uf_find(parent, v);
+    parent[v] = r;
+    atomicAdd(&cnt[r], 1u);
+  }
+}
+__global__ void k_zero_best(const u32* vcc, u64 m, const u32* s, u64* best) {
+  for (u64 i = gtid(); i < m; i += gstride()) best[s[vcc[i]]] = 0;
+}
+// keep the LAST largest component (max_by_key, hit_leiden.rs:1317-1322):
+// discovery order is ascending minimum vertex, so the key is (size, root)
+__global__ void k_best_comp(const u32* vcc, u64 m, const u32* parent, const u32* cnt, const u32* s, u64* best) {
+  for (u64 i = gtid(); i < m; i += gstride()) {
+    u32 v = vcc[i];
+    if (parent[v] != v) continue;
+    unsigned long long key = (u64(cnt[v]) << 32) | v;
+    atomicMax((unsigned long long*)&best[s[v]], key);
+  }
+}
+__global__ void k_split_roots(const u32* vcc, u64 m, const u32* parent, const u32* s, const u64* best, u8* flag) {
+  for (u64 i = gtid(); i < m; i += gstride()) {
+    u32 v = vcc[i];
+    flag[i] = (parent[v] == v) && (u32(best[s[v]] & 0xffffffffu) != v);
+  }
+}
+__global__ void k_root_keys(const u32* sel, u64 c, const u32* vcc, const u32* s, u64* keys, u32* vals) {
+  for (u64 r = gtid(); r < c; r += gstride()) {
+    u32 v = vcc[sel[r]];
+    keys[r] = (u64(s[v]) << 32) | v;
+    vals[r] = v;
+  }
+}
+__global__ void k_newids(const u32* roots_sorted, u64 c, u32 base, u32* newid) {
+  for (u64 r = gtid(); r < c; r += gstride()) newid[roots_sorted[r]] = base + u32(r);
+}
+__global__ void k_split_assign(const u32* vcc, u64 m, const u32* parent, const u64* best, const u32* newid, u32* s,
+                               u8* R) {
+  for (u64 i = gtid(); i < m; i += gstride()) {
+    u32 v = vcc[i];
+    u32 r = parent[v];
+    u32 sv = s[v];
+    if (u32(best[sv] & 0xffffffffu) != r) {
+      s[v] = newid[r];
+      R[v] = 1;
+    }
+  }
+}
+__global__ void k_not_identity(const u32* s, u32 n, u32* flag) {
+  for (u64 v = gtid(); v < n; v += gstride())
+    if (s[v] != v) *flag = 1;
+}
+
+// ------------------------------------------------------------------ refinement (throughput)
+__global__ void k_sizes(const u32* s, u32 n, u32* sizes) {
+  for (u64 v = gtid(); v < n; v += gstride()) atomicAdd(&sizes[s[v]], 1u);
+}
+__global__ void k_single(const u32* rl, u64 nr, const u32* s, const u32* sizes, u8* single) {
+  for (u64 i = gtid(); i < nr; i += gstride()) single[i] = sizes[s[rl[i]]] == 1;
+}
+// candidate subcommunities of singleton R vertices (hit_leiden.rs:1569-1586)
+__global__ __launch_bounds__(256) void k_cand_mark(Csr g, const u32* sl, u64 ns, const u32* f, const u32* s,
+                                                    u8* is_cand) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (u64 q = wave; q < ns; q += nwaves) {
+    u32 v = sl[q];
+    u64 off = g.row_off[v];
+    u32 deg = g.row_len[v];
+    u32 fv = f[v], sv = s[v];
+    for (u32 j = lane; j < deg; j += 64) {
+      u32 u = g.col[off + j];
+      if (f[u] == fv && s[u] != sv) is_cand[s[u]] = 1;
+    }
+  }
+}
+__global__ void k_parent_idx(const u32* s, u32 n, const u8* is_cand, u32* pidx) {
+  for (u64 v = gtid(); v < n; v += gstride())
+    if (is_cand[s[v]]) atomicMin(&pidx[s[v]], u32(v));
+}
+// w(S, parent(S)) over all members (hit_leiden.rs:1617-1629), self-loops twice
+__global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u32* s, const u8* is_cand,
+                                             const u32* pidx, double* wS) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (u64 v = wave; v < n; v += nwaves) {
+    u32 sv = s[v];
+    if (!is_cand[sv]) continue;
+    u32 C = f[pidx[sv]];
+    u64 off = g.row_off[v];
+    u32 deg = g.row_len[v];
+    double acc = 0.0;
+    for (u32 j = lane; j < deg; j += 64) {
+      u32 u = g.col[off + j];
+      if (f[u] == C) {
+        double w = g.w[off + j];
+        acc += w;
+        if (u == v) acc += w;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0 && acc != 0.0) atomicAdd(&wS[sv], acc);
+  }
+}
+__global__ void k_blocked(u64 sn, const u8* is_cand, const double* dS, const double* wS, const u32* pidx,
+                          const u32* f, const double* dpar, const double* twice_m, double gamma, u8* blocked) {
+  const double t = *twice_m;
+  for (u64 c = gtid(); c < sn; c += gstride()) {
+    u8 b = 0;
+    if (is_cand[c]) {
+      double ds = dS[c];
+      if (ds > 0.0) {
+        u32 P = f[pidx[c]];
+        b = remove_gain(wS[c], t, gamma, ds, dpar[P]) > kEpsTFilter;
+      }
+    }
+    blocked[c] = b;
+  }
+}
+__global__ void k_targets(const u32* rl, u64 nr, const u32* prop, u8* target) {
+  for (u64 i = gtid(); i < nr; i += gstride())
+    if (prop[i] != kNone) target[prop[i]] = 1;
+}
+__global__ void k_apply_refine(const u32* rl, u64 nr, const u32* prop, const u8* target, u32* s) {
+  for (u64 i = gtid(); i < nr; i += gstride()) {
+    u32 p = prop[i];
+    if (p == kNone) continue;
+    u32 v = rl[i];
+    if (target[s[v]]) continue;  // connected-targets rule
+    s[v] = p;
+  }
+}
+__global__ void k_gather_list(const u32* sel, u64 c, const u32* from, u32* out) {
+  for (u64 i = gtid(); i < c; i += gstride()) out[i] = from[sel[i]];
+}
+
+// ------------------------------------------------------------------ refinement (deterministic)
+struct DetRefArgs {
+  Csr g;
+  const u32* f;
+  u32* s;
+  const u32* order;  // R sorted by (k asc, id asc)
+  u64 nr;
+  u32* sizes;
+  double* dS;
+  const double* dpar;
+  u32* mhead;
+  u32* mnext;
+  double gamma;
+  u32* gk;
+  double* gv;
+  u32* gused;
+  u32 gcap;
+};
+// hit_leiden.rs:1394-1511: singleton merges in order, T-filter per candidate
+// over the current members of S (ascending), sequential per-key sums
+__global__ __launch_bounds__(256) void k_det_refine(DetRefArgs a) {
+  __shared__ u32 lk[DET_LCAP];
+  __shared__ double lv[DET_LCAP];
+  __shared__ u32 lused[DET_LCAP];
+  __shared__ u32 cl[256];
+  __shared__ double cw[256];
+  __shared__ u8 cs[256];
+  __shared__ u32 s_nused, s_best;
+  __shared__ double s_wcur, s_bg;
+  const u32 tid = threadIdx.x;
+  for (u32 q = tid; q < DET_LCAP; q += 256) {
+    lk[q] = kNone;
+    lv[q] = 0.0;
+  }
+  __syncthreads();
+  const Csr& g = a.g;
+  const double tt = *g.twice_m;
+  for (u64 it = 0; it < a.nr; ++it) {
+    const u32 v = a.order[it];
+    const u32 sc = a.s[v];
+    if (a.sizes[sc] != 1) continue;  // uniform across the block
+    const u32 C = a.f[v];
+    const double kv = g.k[v];
+    const u64 off = g.row_off[v];
+    const u32 deg = g.row_len[v];
+    const bool in_lds = deg <= DET_LCAP / 2;
+    u32* HK = in_lds ? lk : a.gk;
+    double* HV = in_lds ? lv : a.gv;
+    u32* used = in_lds ? lused : a.gused;
+    const u32 cap = in_lds ? DET_LCAP : a.gcap;
+    if (tid == 0) s_nused = 0;
+    __syncthreads();
+    for (u32 base = 0; base < deg; base += 256) {
+      u32 j = base + tid;
+      if (j < deg) {
+        u32 u = g.col[off + j];
+        bool in = a.f[u] == C;
+        cl[tid] = in ? a.s[u] : kNone;
+        cw[tid] = g.w[off + j];
+        cs[tid] = (u == v);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        u32 m = min(256u, deg - base);
+        u32 nu = s_nused;
+        for (u32 q = 0; q < m; ++q) {
+          if (cl[q] == kNone) continue;
+          u32 h = seq_insert(HK, cap, cl[q], used, nu);
+          HV[h] += cw[q];
+          if (cs[q]) HV[h] += cw[q];
+        }
+        s_nused = nu;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) s_wcur = table_lookup(HK, HV, cap, sc);
+    __syncthreads();
+    const double wcur = s_wcur;
+    const u32 nused = s_nused;
+    const double dp = a.dpar[C];
+    const double dcur = a.dS[sc];
+    double bg = 0.0;
+    u32 best = kNone;
+    for (u32 q = tid; q < nused; q += 256) {
+      u32 h = used[q];
+      u32 S = HK[h];
+      if (S == sc) continue;
+      double ds = a.dS[S];
+      if (ds > 0.0) {
+        double ws = 0.0;
+        for (u32 i = a.mhead[S]; i != kNone; i = a.mnext[i]) {
+          u64 o2 = g.row_off[i];
+          u32 d2 = g.row_len[i];
+          for (u32 j = 0; j < d2; ++j) {
+            u32 x = g.col[o2 + j];
+            if (a.f[x] == C) {
+              double w = g.w[o2 + j];
+              ws += w;
+              if (x == i) ws += w;
+            }
+          }
+        }
+        if (remove_gain(ws, tt, a.gamma, ds, dp) > kEpsTFilter) continue;
+      }
+      double gn = move_gain(HV[h], wcur, tt, a.gamma, kv, dcur, ds);
+      if (gn > 0.0) better(bg, best, gn, S);
+    }
+    block_argmax(bg, best);
+    if (tid == 0 && bg > 0.0) {
+      a.s[v] = best;
+      a.sizes[sc] -= 1;
+      a.sizes[best] += 1;
+      a.dS[sc] -= kv;
+      a.dS[best] += kv;
+      a.mhead[sc] = kNone;
+      u32 prev = kNone, x = a.mhead[best];
+      while (x != kNone && x < v) {
+        prev = x;
+        x = a.mnext[x];
+      }
+      a.mnext[v] = x;
+      if (prev == kNone)
+        a.mhead[best] = v;
+      else
+        a.mnext[prev] = v;
+    }
+    for (u32 q = tid; q < nused; q += 256) {
+      u32 h = used[q];
+      HK[h] = kNone;
+      HV[h] = 0.0;
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+__global__ void k_member_lists(u64 nseg, const u32* starts, u64 n, const u32* keys_s, const u32* vals_s, u32* mhead,
+                               u32* mnext) {
+  for (u64 j = gtid(); j < nseg; j += gstride()) {
+    u64 s = starts[j], e = (j + 1 < nseg) ? starts[j + 1] : n;
+    mhead[keys_s[s]] = vals_s[s];
+    for (u64 t = s; t < e; ++t) mnext[vals_s[t]] = (t + 1 < e) ? vals_s[t + 1] : kNone;
+  }
+}
+__global__ void k_degree_keys(const u32* rl, u64 nr, const double* k, u64* keys) {
+  for (u64 i = gtid(); i < nr; i += gstride()) keys[i] = (u64)__double_as_longlong(k[rl[i]]);
+}
+
+// ------------------------------------------------------------------ aggregation
+__global__ void k_lift(const Rec* d, u64 k, const u32* s_pre, Rec* H) {
+  for (u64 i = gtid(); i < k; i += gstride()) {
+    Rec r = d[i];
+    H[i] = Rec{s_pre[r.u], s_pre[r.v], r.w};
+  }
+}
+// hit_leiden.rs:1659-1692 with an R-R edge emitted once (v_j not in R or i < j)
+// and the self-loop record weight transferred
+__device__ __forceinline__ bool agg_q(u32 v, u32 u, const u8* R) { return u != v && (!R[u] || v < u); }
+
+__global__ __launch_bounds__(256) void k_agg_count(Csr g, const u32* rl, u64 nr, const u8* R, u64* cnt) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (u64 q = wave; q < nr; q += nwaves) {
+    u32 v = rl[q];
+    u64 off = g.row_off[v];
+    u32 deg = g.row_len[v];
+    u32 c = 0;
+    u32 self = 0;
+    for (u32 j = lane; j < deg; j += 64) {
+      u32 u = g.col[off + j];
+      c += agg_q(v, u, R);
+      if (u == v && fabs(g.w[off + j]) > kEpsSelf) self = 1;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      c += __shfl_xor(c, o, 64);
+      self |= __shfl_xor(self, o, 64);
+    }
+    if (lane == 0) cnt[q] = 2ull * c + (self ? 2 : 0);
+  }
+}
+__global__ __launch_bounds__(256) void k_agg_emit(Csr g, const u32* rl, u64 nr, const u8* R, const u32* s_pre,
+                                                   const u32* s_cur, const u64* offs, u64 base, Rec* H) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (u64 q = wave; q < nr; q += nwaves) {
+    u32 v = rl[q];
+    u64 off = g.row_off[v];
+    u32 deg = g.row_len[v];
+    u64 pos = base + offs[q];
+    const u32 pv = s_pre[v], cv = s_cur[v];
+    double self = 0.0;
+    for (u32 j0 = 0; j0 < deg; j0 += 64) {
+      u32 j = j0 + lane;
+      bool qual = false;
+      u32 u = 0;
+      double w = 0.0;
+      if (j < deg) {
+        u = g.col[off + j];
+        w = g.w[off + j];
+        qual = agg_q(v, u, R);
+        if (u == v) self = w;
+      }
+      unsigned long long m = __ballot(qual);
+      if (qual) {
+        u64 p = pos + 2ull * __popcll(m & lt);
+        H[p] = Rec{pv, s_pre[u], -w};
+        H[p + 1] = Rec{cv, s_cur[u], w};
+      }
+      pos += 2ull * __popcll(m);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      double os = __shfl_xor(self, o, 64);
+      if (os != 0.0) self = os;
+    }
+    if (lane == 0 && fabs(self) > kEpsSelf) {
+      H[pos] = Rec{pv, pv, -self};
+      H[pos + 1] = Rec{cv, cv, self};
+    }
+  }
+}
+__global__ void k_spre_update(const u32* rl, u64 nr, const u32* s_cur, u32* s_pre) {
+  for (u64 i = gtid(); i < nr; i += gstride()) s_pre[rl[i]] = s_cur[rl[i]];
+}
+__global__ void k_hkeys(const Rec* H, u64 n, u64* keys, u32* idx) {
+  for (u64 i = gtid(); i < n; i += gstride()) {
+    Rec r = H[i];
+    u32 a = min(r.u, r.v), b = max(r.u, r.v);
+    keys[i] = (u64(a) << 32) | b;
+    idx[i] = u32(i);
+  }
+}
+// Compress (hit_leiden.rs:1701-1714): per key in emission order, |w| > 1e-9 kept
+__global__ void k_compress(u64 nruns, const u32* runs, u64 n, const u64* keys, const u32* idx, const Rec* H, Rec* out,
+                           u8* keep) {
+  for (u64 r = gtid(); r < nruns; r += gstride()) {
+    u64 s = runs[r], e = (r + 1 < nruns) ? runs[r + 1] : n;
+    double acc = 0.0;
+    for (u64 t = s; t < e; ++t) acc += H[idx[t]].w;
+    u64 key = keys[s];
+    out[r] = Rec{u32(key >> 32), u32(key & 0xffffffffu), acc};
+    keep[r] = fabs(acc) > kEpsCompress;
+  }
+}
+__global__ void k_gather_rec(const u32* sel, u64 c, const Rec* in, Rec* out) {
+  for (u64 i = gtid(); i < c; i += gstride()) out[i] = in[sel[i]];
+}
+
+// ------------------------------------------------------------------ def_update / modularity
+__global__ void k_def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n) {
+  for (u64 v = gtid(); v < n; v += gstride())
+    if (B[v]) fp[v] = fq[sp[v]];
+}
+__global__ void k_def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np) {
+  for (u64 u = gtid(); u < nq; u += gstride()) {
+    u32 x = sq[u];
+    if (x < np && Bp[x]) Bq[u] = 1;
+  }
+}
+__global__ void k_gather_u32(u32* cur, const u32* map, u32 n) {
+  for (u64 v = gtid(); v < n; v += gstride()) cur[v] = map[cur[v]];
+}
+__global__ void k_intra(Csr g, const u32* lab, double* acc) {
+  double local = 0.0;
+  for (u64 v = gtid(); v < g.n; v += gstride()) {
+    u64 off = g.row_off[v];
+    u32 deg = g.row_len[v];
+    u32 lv = lab[v];
+    for (u32 j = 0; j < deg; ++j) {
+      u32 u = g.col[off + j];
+      if (lab[u] == lv) {
+        double w = g.w[off + j];
+        local += (u == v) ? 2.0 * w : w;
+      }
+    }
+  }
+  double t = block_sum(local);
+  if (threadIdx.x == 0 && t != 0.0) atomicAdd(acc, t);
+}
+__global__ void k_sq_sum(const double* sig, u64 n, const double* twice_m, double* acc) {
+  const double t = *twice_m;
+  double local = 0.0;
+  for (u64 i = gtid(); i < n; i += gstride()) {
+    double x = sig[i];
+    local += x * x / t;
+  }
+  double s = block_sum(local);
+  if (threadIdx.x == 0 && s != 0.0) atomicAdd(acc, s);
+}
+__global__ void k_first_seen(const u32* lab, u32 n, u32* first) {
+  for (u64 v = gtid(); v < n; v += gstride()) atomicMin(&first[lab[v]], u32(v));
+}
+__global__ void k_first_flags(const u32* lab, u32 n, const u32* first, u64* flag) {
+  for (u64 v = gtid(); v < n; v += gstride()) flag[v] = first[lab[v]] == v;
+}
+__global__ void k_relabel(u32* lab, u32 n, const u32* first, const u64* pos) {
+  for (u64 v = gtid(); v < n; v += gstride()) lab[v] = u32(pos[first[lab[v]]]);
+}
+
+}  // namespace
+
+// ====================================================================== host side
+
+#define GRID(n) blocks_for((n), 256), 256, 0, s
+
+void fill_u8(u8* p, u8 v, u64 n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_fill_u8, GRID(n), p, v, n);
+  HL_LAUNCH_CHECK();
+}
+void fill_u32(u32* p, u32 v, u64 n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_fill_u32, GRID(n), p, v, n);
+  HL_LAUNCH_CHECK();
+}
+void fill_f64(double* p, double v, u64 n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_fill_f64, GRID(n), p, v, n);
+  HL_LAUNCH_CHECK();
+}
+void iota_u32(u32* p, u32 base, u64 n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_iota_u32, GRID(n), p, base, n);
+  HL_LAUNCH_CHECK();
+}
+void copy_u8(u8* d, const u8* src, u64 n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_copy_u8, GRID(n), d, src, n);
+  HL_LAUNCH_CHECK();
+}
+u64 count_u8(const u8* a, u64 n, DevBuf<u64>& cnt, hipStream_t s) {
+  cnt.reserve(1, s);
+  HL_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(u64), s));
+  if (n) hipLaunchKernelGGL(k_count_u8, blocks_for(n, 256, 4096), 256, 0, s, a, n, (unsigned long long*)cnt.p);
+  HL_LAUNCH_CHECK();
+  u64 h = 0;
+  HL_CHECK(hipMemcpyAsync(&h, cnt.p, sizeof(u64), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  return h;
+}
+void max_u32(const u32* a, u64 n, u32* out, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_max_u32, GRID(n), a, n, out);
+  HL_LAUNCH_CHECK();
+}
+
+u64 select_flagged(const u8* flags, u64 n, u32* out, Temp& tmp, DevBuf<u64>& cnt, hipStream_t s) {
+  if (n == 0) return 0;
+  cnt.reserve(1, s);
+  size_t bytes = 0;
+  rocprim::counting_iterator<u32> it(0);
+  HL_CHECK(rocprim::select(nullptr, bytes, it, flags, out, cnt.p, n, s));
+  void* t = tmp.get(bytes, s);
+  HL_CHECK(rocprim::select(t, bytes, it, flags, out, cnt.p, n, s));
+  u64 c = 0;
+  HL_CHECK(hipMemcpyAsync(&c, cnt.p, sizeof(u64), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  return c;
+}
+
+u64 exclusive_scan_u64(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s) {
+  if (n == 0) {
+    u64 z = 0;
+    HL_CHECK(hipMemcpyAsync(out, &z, sizeof(u64), hipMemcpyHostToDevice, s));
+    HL_CHECK(hipStreamSynchronize(s));
+    return 0;
+  }
+  size_t bytes = 0;
+  HL_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, u64(0), n, rocprim::plus<u64>(), s));
+  void* t = tmp.get(bytes, s);
+  HL_CHECK(rocprim::exclusive_scan(t, bytes, in, out, u64(0), n, rocprim::plus<u64>(), s));
+  hipLaunchKernelGGL(k_scan_tail, 1, 1, 0, s, in, out, n);
+  HL_LAUNCH_CHECK();
+  u64 total = 0;
+  HL_CHECK(hipMemcpyAsync(&total, out + n, sizeof(u64), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  return total;
+}
+
+void sort_pairs_u64(const u64* kin, u64* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
+                    hipStream_t s) {
+  if (n == 0) return;
+  size_t bytes = 0;
+  HL_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
+  void* t = tmp.get(bytes, s);
+  HL_CHECK(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
+}
+void sort_pairs_u32(const u32* kin, u32* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
+                    hipStream_t s) {
+  if (n == 0) return;
+  size_t bytes = 0;
+  HL_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
+  void* t = tmp.get(bytes, s);
+  HL_CHECK(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
+}
+
+static int bits_for(u64 maxval) {
+  int b = 1;
+  while (b < 64 && (maxval >> b)) ++b;
+  return b;
+}
+
+void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a, Temp& tmp, ApplyPlan& plan,
+                      hipStream_t s) {
+  plan = ApplyPlan();
+  if (k == 0) return;
+  a.keys.reserve(k, s);
+  a.keys_s.reserve(k, s);
+  a.idx.reserve(k, s);
+  a.idx_s.reserve(k, s);
+  a.heads.reserve(k, s);
+  a.runs.reserve(k, s);
+  a.cnt.reserve(4, s);
+  hipLaunchKernelGGL(k_canon_keys, GRID(k), d, k, a.keys.p, a.idx.p);
+  HL_LAUNCH_CHECK();
+  sort_pairs_u64(a.keys.p, a.keys_s.p, a.idx.p, a.idx_s.p, k, 64, tmp, s);
+  hipLaunchKernelGGL(k_heads_u64, GRID(k), a.keys_s.p, k, 0, a.heads.p);
+  HL_LAUNCH_CHECK();
+  DevBuf<u64> c1;
+  plan.nruns = select_flagged(a.heads.p, k, a.runs.p, tmp, c1, s);
+  const u64 maxupd = 2 * plan.nruns;
+  a.ukey.reserve(maxupd, s);
+  a.ukey_s.reserve(maxupd, s);
+  a.uidx.reserve(maxupd, s);
+  a.perm.reserve(maxupd, s);
+  a.uop.reserve(maxupd, s);
+  a.uw.reserve(maxupd, s);
+  a.ucount.reserve(1, s);
+  fill_u32(a.ucount.p, 0, 1, s);
+  hipLaunchKernelGGL(k_resolve_runs, GRID(plan.nruns), plan.nruns, a.runs.p, k, a.keys_s.p, a.idx_s.p, d, g,
+                     a.ukey.p, a.uidx.p, a.uop.p, a.uw.p, a.ucount.p);
+  HL_LAUNCH_CHECK();
+  u32 nupd = 0;
+  HL_CHECK(hipMemcpyAsync(&nupd, a.ucount.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  plan.nupd = nupd;
+  if (nupd == 0) return;
+  sort_pairs_u64(a.ukey.p, a.ukey_s.p, a.uidx.p, a.perm.p, nupd, 64, tmp, s);
+  a.heads.reserve(nupd, s);
+  a.segs.reserve(nupd, s);
+  hipLaunchKernelGGL(k_heads_u64, GRID(nupd), a.ukey_s.p, u64(nupd), 32, a.heads.p);
+  HL_LAUNCH_CHECK();
+  plan.nseg = select_flagged(a.heads.p, nupd, a.segs.p, tmp, c1, s);
+  const u64 ns = plan.nseg;
+  a.seg_row.reserve(ns, s);
+  a.seg_len.reserve(ns, s);
+  a.new_len.reserve(ns, s);
+  a.seg_ins.reserve(ns, s);
+  a.seg_del.reserve(ns, s);
+  a.ins_pre.reserve(nupd, s);
+  a.del_pre.reserve(nupd, s);
+  a.newcap.reserve(ns, s);
+  a.newoff.reserve(ns + 1, s);
+  a.chunks.reserve(ns, s);
+  a.chunkbase.reserve(ns + 1, s);
+  HL_CHECK(hipMemsetAsync(a.cnt.p, 0, sizeof(u64), s));
+  hipLaunchKernelGGL(k_row_plan, GRID(ns), ns, a.segs.p, u64(nupd), a.ukey_s.p, a.perm.p, a.uop.p, g.row_len,
+                     a.seg_row.p, a.seg_len.p, a.new_len.p, a.seg_ins.p, a.seg_del.p, a.ins_pre.p, a.del_pre.p,
+                     a.newcap.p, a.chunks.p, a.cnt.p);
+  HL_LAUNCH_CHECK();
+  plan.need = exclusive_scan_u64(a.newcap.p, a.newoff.p, ns, tmp, s);
+  plan.total_chunks = exclusive_scan_u64(a.chunks.p, a.chunkbase.p, ns, tmp, s);
+  u64 olds = 0;
+  HL_CHECK(hipMemcpyAsync(&olds, a.cnt.p, sizeof(u64), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  plan.old_entries = olds;
+}
+
+void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, const ApplyPlan& plan, bool det,
+                        hipStream_t s) {
+  (void)det;
+  if (plan.nseg == 0) return;
+  const u64 ns = plan.nseg;
+  hipLaunchKernelGGL(k_row_merge, dim3(unsigned(plan.total_chunks)), dim3(256), 0, s, ns, a.chunkbase.p, a.segs.p,
+                     a.seg_row.p, a.seg_len.p, a.seg_ins.p, a.seg_del.p, a.ukey_s.p, a.perm.p, a.uop.p, a.uw.p,
+                     a.ins_pre.p, a.del_pre.p, a.newoff.p, pool_base, g);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_row_commit, GRID(ns), ns, a.seg_row.p, a.new_len.p, a.newoff.p, pool_base, g);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_row_degree, GRID(ns), ns, a.seg_row.p, g);
+  HL_LAUNCH_CHECK();
+}
+
+u64 compact_rows(const LevelCsrMut& g, u32* col2, double* w2, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s) {
+  off_tmp.reserve(u64(g.n) + 1, s);
+  DevBuf<u64> lens;
+  lens.reserve(g.n + 1, s);
+  hipLaunchKernelGGL(k_len_u64, GRID(g.n), g.row_len, g.n, lens.p);
+  HL_LAUNCH_CHECK();
+  u64 total = exclusive_scan_u64(lens.p, off_tmp.p, g.n, tmp, s);
+  hipLaunchKernelGGL(k_copy_rows, GRID(u64(g.n) * 64), g, off_tmp.p, col2, w2);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_set_offsets, GRID(g.n), g, off_tmp.p);
+  HL_LAUNCH_CHECK();
+  HL_CHECK(hipStreamSynchronize(s));
+  return total;
+}
+
+void twice_m_chunked(const double* k, u32 n, double* partials, double* out, hipStream_t s) {
+  u64 nc = (u64(n) + kSumChunk - 1) / kSumChunk;
+  if (nc) hipLaunchKernelGGL(k_chunk_partials, GRID(nc), k, n, partials);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_partials, 1, 1, 0, s, partials, nc, out);
+  HL_LAUNCH_CHECK();
+}
+
+void grow_init(u32 n_old, u32 n_new, u32 bf, u32 bg, u32 bs, u32* f, u32* g, u32* s_pre, u32* s_cur, u64* row_off,
+               u32* row_len, u32* row_cap, double* k, hipStream_t s) {
+  u64 c = n_new - n_old;
+  if (c) hipLaunchKernelGGL(k_grow_init, GRID(c), n_old, n_new, bf, bg, bs, f, g, s_pre, s_cur, row_off, row_len,
+                            row_cap, k);
+  HL_LAUNCH_CHECK();
+}
+
+void seed(const Rec* d, u64 k, const u32* f, const u32* sm, u8* A, u8* K, hipStream_t s) {
+  if (k) hipLaunchKernelGGL(k_seed, GRID(k), d, k, f, sm, A, K);
+  HL_LAUNCH_CHECK();
+}
+
+void label_totals_atomic(const u32* lab, const double* k, u32 n, double* tot, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_label_totals_atomic, GRID(n), lab, k, n, tot);
+  HL_LAUNCH_CHECK();
+}
+
+void label_totals_canonical(const u32* lab, const double* k, u32 n, double* tot, u64 tot_n, DevBuf<u32>& ks,
+                            DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,
+                            DevBuf<u64>& cnt, Temp& tmp, hipStream_t s) {
+  fill_f64(tot, 0.0, tot_n, s);
+  if (n == 0) return;
+  ks.reserve(n, s);
+  vs.reserve(n, s);
+  vidx.reserve(n, s);
+  heads.reserve(n, s);
+  starts.reserve(n, s);
+  iota_u32(vidx.p, 0, n, s);
+  sort_pairs_u32(lab, ks.p, vidx.p, vs.p, n, 32, tmp, s);
+  hipLaunchKernelGGL(k_heads_u32, GRID(n), ks.p, u64(n), heads.p);
+  HL_LAUNCH_CHECK();
+  u64 nseg = select_flagged(heads.p, n, starts.p, tmp, cnt, s);
+  hipLaunchKernelGGL(k_segment_sums, GRID(nseg), nseg, starts.p, u64(n), ks.p, vs.p, k, tot);
+  HL_LAUNCH_CHECK();
+}
+
+template <class P>
+static u64 propose_impl(const P& pol, const Csr& g, const u32* verts, u64 cnt, u32* cto, double* cgain,
+                        ProposeScratch& ps, const Thresholds& th, Temp& tmp, hipStream_t s) {
+  if (cnt == 0) return 0;
+  ps.lists.reserve(3 * cnt, s);
+  ps.counts.reserve(4, s);
+  ps.degsum.reserve(1, s);
+  fill_u32(ps.counts.p, 0, 4, s);
+  HL_CHECK(hipMemsetAsync(ps.degsum.p, 0, sizeof(u64), s));
+  u32 small = th.small_deg > SMALL_MAX ? SMALL_MAX : th.small_deg;
+  u32 wave = th.wave_deg > WAVE_HW / 2 ? WAVE_HW / 2 : th.wave_deg;
+  hipLaunchKernelGGL(k_bucket, GRID(cnt), verts, cnt, g.row_len, small, wave, ps.lists.p, ps.counts.p,
+                     ps.degsum.p);
+  HL_LAUNCH_CHECK();
+  u32 counts[4];
+  u64 degsum = 0;
+  HL_CHECK(hipMemcpyAsync(counts, ps.counts.p, 4 * sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipMemcpyAsync(&degsum, ps.degsum.p, sizeof(u64), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  if (counts[0])
+    hipLaunchKernelGGL(k_prop_small<P>, GRID(counts[0]), pol, g, ps.lists.p, counts[0], verts, cto, cgain);
+  HL_LAUNCH_CHECK();
+  if (counts[1]) {
+    unsigned nb = unsigned(std::min<u64>((counts[1] + 3) / 4, 8192));
+    hipLaunchKernelGGL(k_prop_wave<P>, dim3(nb), dim3(256), 0, s, pol, g, ps.lists.p + cnt, counts[1], verts, cto,
+                       cgain);
+    HL_LAUNCH_CHECK();
+  }
+  if (counts[2]) {
+    const u32* bl = ps.lists.p + 2 * cnt;
+    ps.bigsize.reserve(counts[2], s);
+    ps.bigoff.reserve(counts[2] + 1, s);
+    hipLaunchKernelGGL(k_big_sizes, GRID(counts[2]), bl, counts[2], verts, g.row_len, ps.bigsize.p);
+    HL_LAUNCH_CHECK();
+    u64 total = exclusive_scan_u64(ps.bigsize.p, ps.bigoff.p, counts[2], tmp, s);
+    ps.hkeys.reserve(total, s);
+    ps.hvals.reserve(total, s);
+    hipLaunchKernelGGL(k_prop_big<P>, dim3(counts[2]), dim3(256), 0, s, pol, g, bl, counts[2], verts, ps.bigoff.p,
+                       ps.hkeys.p, ps.hvals.p, cto, cgain);
+    HL_LAUNCH_CHECK();
+  }
+  return degsum;
+}
+
+u64 propose_moves(const Csr& g, const u32* f, const double* tot, double gamma, const u32* active, u64 cnt, u32* cto,
+                  double* cgain, ProposeScratch& ps, const Thresholds& th, Temp& tmp, hipStream_t s) {
+  MovePolicy pol{f, tot, gamma, g.n};
+  return propose_impl(pol, g, active, cnt, cto, cgain, ps, th, tmp, s);
+}
+
+u64 decouple_and_apply(const Csr& g, u32* f, const u32* sm, double* tot, u64 comm_cap, const u32* active, u64 cnt,
+                       const u32* cto, const double* cgain, u8* B, u8* K, u8* next, DecoupleScratch& ds, Temp& tmp,
+                       hipStream_t s) {
+  ds.flags.reserve(cnt, s);
+  ds.cidx.reserve(cnt, s);
+  hipLaunchKernelGGL(k_flags_not_none, GRID(cnt), cto, cnt, ds.flags.p);
+  HL_LAUNCH_CHECK();
+  const u64 c = select_flagged(ds.flags.p, cnt, ds.cidx.p, tmp, ds.cnt, s);
+  if (c == 0) return 0;
+  ds.keys.reserve(c, s);
+  ds.keys_s.reserve(c, s);
+  ds.order.reserve(c, s);
+  ds.from.reserve(c, s);
+  ds.to.reserve(c, s);
+  ds.node.reserve(c, s);
+  ds.state.reserve(c, s);
+  ds.undec.reserve(2, s);
+  DevBuf<u32> vals;
+  vals.reserve(c, s);
+  hipLaunchKernelGGL(k_gain_keys, GRID(c), ds.cidx.p, c, cgain, ds.keys.p, vals.p);
+  HL_LAUNCH_CHECK();
+  sort_pairs_u64(ds.keys.p, ds.keys_s.p, vals.p, ds.order.p, c, 64, tmp, s);
+  hipLaunchKernelGGL(k_dec_init, GRID(c), ds.order.p, ds.cidx.p, c, active, f, cto, ds.node.p, ds.from.p, ds.to.p,
+                     ds.state.p);
+  HL_LAUNCH_CHECK();
+  if (ds.minUto.cap < comm_cap) {
+    ds.minUto.reserve(comm_cap, s);
+    ds.minUfrom.reserve(comm_cap, s);
+    ds.minAto.reserve(comm_cap, s);
+    ds.minAfrom.reserve(comm_cap, s);
+  }
+  int iters = 0;
+  while (true) {
+    hipLaunchKernelGGL(k_dec_reset, GRID(c), c, ds.from.p, ds.to.p, ds.minUto.p, ds.minUfrom.p, ds.minAto.p,
+                       ds.minAfrom.p);
+    HL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dec_mark, GRID(c), c, ds.from.p, ds.to.p, ds.state.p, ds.minUto.p, ds.minUfrom.p,
+                       ds.minAto.p, ds.minAfrom.p);
+    HL_LAUNCH_CHECK();
+    fill_u32(ds.undec.p, 0, 1, s);
+    hipLaunchKernelGGL(k_dec_decide, GRID(c), c, ds.from.p, ds.to.p, ds.state.p, ds.minUto.p, ds.minUfrom.p,
+                       ds.minAto.p, ds.minAfrom.p, ds.undec.p);
+    HL_LAUNCH_CHECK();
+    u32 und = 0;
+    HL_CHECK(hipMemcpyAsync(&und, ds.undec.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+    HL_CHECK(hipStreamSynchronize(s));
+    if (und == 0) break;
+    if (++iters >= 48) {
+      DevBuf<u8> ea;
+      ea.reserve(comm_cap, s);
+      fill_u8(ea.p, 0, comm_cap, s);
+      hipLaunchKernelGGL(k_dec_sequential, 1, 64, 0, s, c, ds.from.p, ds.to.p, ds.state.p, ea.p);
+      HL_LAUNCH_CHECK();
+      HL_CHECK(hipStreamSynchronize(s));
+      break;
+    }
+  }
+  ds.flags.reserve(c, s);
+  hipLaunchKernelGGL(k_apply_moves, GRID(c), c, ds.node.p, ds.from.p, ds.to.p, ds.state.p, g.k, f, tot, B,
+                     ds.flags.p);
+  HL_LAUNCH_CHECK();
+  DevBuf<u32> acc;
+  acc.reserve(c, s);
+  const u64 na = select_flagged(ds.flags.p, c, acc.p, tmp, ds.cnt, s);
+  if (na) hipLaunchKernelGGL(k_mark_moved, GRID(na * 64), g, acc.p, na, ds.node.p, ds.to.p, f, sm, next, K);
+  HL_LAUNCH_CHECK();
+  return na;
+}
+
+int det_move(const Csr& g, u32* f, const u32* sm, double* tot, u64 tot_cap, u64* tot_len_dev, const u8* A,
+             bool fresh, u8* B, u8* K, double gamma, DetMoveScratch& ds, u32 max_deg, hipStream_t s) {
+  const u64 n = g.n;
+  const u64 nw0 = (n + 31) / 32, nw1 = (nw0 + 31) / 32, nw2 = (nw1 + 31) / 32;
+  ds.bm0.reserve(nw0 + 1, s);
+  ds.bm1.reserve(nw1 + 1, s);
+  ds.bm2.reserve(nw2 + 1, s);
+  ds.status.reserve(1, s);
+  ds.pops.reserve(1, s);
+  if (fresh) {
+    hipLaunchKernelGGL(k_bm_level0, GRID(nw0), A, u32(n), ds.bm0.p, nw0);
+    HL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bm_summary, GRID(nw1), ds.bm0.p, nw0, ds.bm1.p, nw1);
+    HL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bm_summary, GRID(nw2), ds.bm1.p, nw1, ds.bm2.p, nw2);
+    HL_LAUNCH_CHECK();
+    HL_CHECK(hipMemsetAsync(ds.pops.p, 0, sizeof(u64), s));
+  }
+  u32 gcap = 64;
+  while (gcap < 2ull * max_deg + 2) gcap <<= 1;
+  ds.gkeys.reserve(2 * u64(gcap), s);
+  ds.gvals.reserve(gcap, s);
+  if (max_deg > DET_LCAP / 2) {
+    fill_u32(ds.gkeys.p, kNone, gcap, s);
+    fill_f64(ds.gvals.p, 0.0, gcap, s);
+  }
+  fill_u32(ds.status.p, 0, 1, s);
+  DetMoveArgs a{g, f, sm, tot, tot_cap, tot_len_dev, ds.bm0.p, ds.bm1.p, ds.bm2.p, u32(nw2), B, K, gamma,
+                ds.gkeys.p, ds.gvals.p, ds.gkeys.p + gcap, gcap, ds.status.p, ds.pops.p};
+  hipLaunchKernelGGL(k_det_move, 1, 256, 0, s, a);
+  HL_LAUNCH_CHECK();
+  u32 st = 0;
+  HL_CHECK(hipMemcpyAsync(&st, ds.status.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  return int(st);
+}
+
+void cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_space, RefineScratch& rs, Temp& tmp, hipStream_t s) {
+  const u32 n = g.n;
+  if (n == 0) return;
+  rs.aff.reserve(s_space, s);
+  rs.status.reserve(2, s);
+  fill_u8(rs.aff.p, 0, s_space, s);
+  fill_u32(rs.status.p, 0, 2, s);
+  hipLaunchKernelGGL(k_aff_mark, GRID(n), K, sm, n, rs.aff.p, rs.status.p);
+  HL_LAUNCH_CHECK();
+  rs.smax.reserve(1, s);
+  fill_u32(rs.smax.p, 0, 1, s);
+  max_u32(sm, n, rs.smax.p, s);
+  u32 hs[2] = {0, 0};
+  HL_CHECK(hipMemcpyAsync(&hs[0], rs.status.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipMemcpyAsync(&hs[1], rs.smax.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  if (!hs[0]) return;
+  const u32 next_id = hs[1] + 1;
+  rs.incc.reserve(n, s);
+  rs.vcc.reserve(n, s);
+  hipLaunchKernelGGL(k_incc, GRID(n), sm, n, rs.aff.p, rs.incc.p);
+  HL_LAUNCH_CHECK();
+  const u64 m = select_flagged(rs.incc.p, n, rs.vcc.p, tmp, rs.cnt, s);
+  if (m == 0) return;
+  rs.parent.reserve(n, s);
+  rs.cnt32.reserve(n, s);
+  rs.best64.reserve(s_space, s);
+  rs.keepflag.reserve(m, s);
+  rs.roots.reserve(m, s);
+  rs.roots_s.reserve(m, s);
+  rs.rkeys.reserve(m, s);
+  rs.rkeys_s.reserve(m, s);
+  rs.newid.reserve(n, s);
+  hipLaunchKernelGGL(k_uf_init, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_uf_edges, GRID(m * 64), g, rs.vcc.p, m, sm, rs.parent.p);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_uf_compress, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_zero_best, GRID(m), rs.vcc.p, m, sm, rs.best64.p);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_best_comp, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p, sm, rs.best64.p);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_split_roots, GRID(m), rs.vcc.p, m, rs.parent.p, sm, rs.best64.p, rs.keepflag.p);
+  HL_LAUNCH_CHECK();
+  DevBuf<u32> sel;
+  sel.reserve(m, s);
+  const u64 c = select_flagged(rs.keepflag.p, m, sel.p, tmp, rs.cnt, s);
+  if (c) {
+    DevBuf<u32> vals;
+    vals.reserve(c, s);
+    hipLaunchKernelGGL(k_root_keys, GRID(c), sel.p, c, rs.vcc.p, sm, rs.rkeys.p, vals.p);
+    HL_LAUNCH_CHECK();
+    sort_pairs_u64(rs.rkeys.p, rs.rkeys_s.p, vals.p, rs.roots_s.p, c, 64, tmp, s);
+    hipLaunchKernelGGL(k_newids, GRID(c), rs.roots_s.p, c, next_id, rs.newid.p);
+    HL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_split_assign, GRID(m), rs.vcc.p, m, rs.parent.p, rs.best64.p, rs.newid.p, sm, R);
+    HL_LAUNCH_CHECK();
+  }
+  HL_CHECK(hipStreamSynchronize(s));
+}
+
+bool is_identity(const u32* sm, u32 n, RefineScratch& rs, hipStream_t s) {
+  rs.status.reserve(2, s);
+  fill_u32(rs.status.p, 0, 1, s);
+  if (n) hipLaunchKernelGGL(k_not_identity, GRID(n), sm, n, rs.status.p);
+  HL_LAUNCH_CHECK();
+  u32 h = 0;
+  HL_CHECK(hipMemcpyAsync(&h, rs.status.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  return h == 0;
+}
+
+static u64 smax_plus1(const u32* sm, u32 n, RefineScratch& rs, hipStream_t s) {
+  rs.smax.reserve(1, s);
+  fill_u32(rs.smax.p, 0, 1, s);
+  max_u32(sm, n, rs.smax.p, s);
+  u32 h = 0;
+  HL_CHECK(hipMemcpyAsync(&h, rs.smax.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  return u64(h) + 1;
+}
+
+void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space,
+                     RefineScratch& rs, ProposeScratch& ps, const Thresholds& th, Temp& tmp, hipStream_t s) {
+  const u32 n = g.n;
+  if (n == 0) return;
+  const u64 sn = smax_plus1(sm, n, rs, s);
+  rs.rlist.reserve(n, s);
+  const u64 nr = select_flagged(R, n, rs.rlist.p, tmp, rs.cnt, s);
+  if (nr == 0) return;
+  rs.sizes.reserve(sn, s);
+  rs.dS.reserve(sn, s);
+  fill_u32(rs.sizes.p, 0, sn, s);
+  fill_f64(rs.dS.p, 0.0, sn, s);
+  hipLaunchKernelGGL(k_sizes, GRID(n), sm, n, rs.sizes.p);
+  HL_LAUNCH_CHECK();
+  label_totals_atomic(sm, g.k, n, rs.dS.p, s);
+  rs.single.reserve(nr, s);
+  hipLaunchKernelGGL(k_single, GRID(nr), rs.rlist.p, nr, sm, rs.sizes.p, rs.single.p);
+  HL_LAUNCH_CHECK();
+  DevBuf<u32> sel, sl;
+  sel.reserve(nr, s);
+  const u64 ns = select_flagged(rs.single.p, nr, sel.p, tmp, rs.cnt, s);
+  if (ns == 0) return;
+  sl.reserve(ns, s);
+  hipLaunchKernelGGL(k_gather_list, GRID(ns), sel.p, ns, rs.rlist.p, sl.p);
+  HL_LAUNCH_CHECK();
+  // blocked set (compute_blocked_subcommunities)
+  rs.is_cand.reserve(sn, s);
+  rs.blocked.reserve(sn, s);
+  rs.parentIdx.reserve(sn, s);
+  rs.wS.reserve(sn, s);
+  rs.dpar.reserve(comm_space, s);
+  fill_u8(rs.is_cand.p, 0, sn, s);
+  hipLaunchKernelGGL(k_cand_mark, GRID(ns * 64), g, sl.p, ns, f, sm, rs.is_cand.p);
+  HL_LAUNCH_CHECK();
+  fill_u32(rs.parentIdx.p, kNone, sn, s);
+  hipLaunchKernelGGL(k_parent_idx, GRID(n), sm, n, rs.is_cand.p, rs.parentIdx.p);
+  HL_LAUNCH_CHECK();
+  fill_f64(rs.wS.p, 0.0, sn, s);
+  hipLaunchKernelGGL(k_ws, GRID(u64(n) * 64), g, n, f, sm, rs.is_cand.p, rs.parentIdx.p, rs.wS.p);
+  HL_LAUNCH_CHECK();
+  fill_f64(rs.dpar.p, 0.0, comm_space, s);
+  label_totals_atomic(f, g.k, n, rs.dpar.p, s);
+  hipLaunchKernelGGL(k_blocked, GRID(sn), sn, rs.is_cand.p, rs.dS.p, rs.wS.p, rs.parentIdx.p, f, rs.dpar.p,
+                     g.twice_m, gamma, rs.blocked.p);
+  HL_LAUNCH_CHECK();
+  // proposals against the snapshot
+  rs.prop.reserve(ns, s);
+  DevBuf<double> pg;
+  pg.reserve(ns, s);
+  DevBuf<u32> idx;
+  idx.reserve(ns, s);
+  iota_u32(idx.p, 0, ns, s);
+  RefinePolicy pol{f, sm, rs.dS.p, rs.blocked.p, gamma};
+  // propose over sl (verts = sl, list = iota)
+  propose_impl(pol, g, sl.p, ns, rs.prop.p, pg.p, ps, th, tmp, s);
+  rs.target.reserve(sn, s);
+  fill_u8(rs.target.p, 0, sn, s);
+  hipLaunchKernelGGL(k_targets, GRID(ns), sl.p, ns, rs.prop.p, rs.target.p);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_apply_refine, GRID(ns), sl.p, ns, rs.prop.p, rs.target.p, sm);
+  HL_LAUNCH_CHECK();
+  HL_CHECK(hipStreamSynchronize(s));
+}
+
+void refine_det(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space, RefineScratch& rs,
+                DevBuf<u32>& ks, DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,
+                DevBuf<u64>& cnt, u32 max_deg, Temp& tmp, hipStream_t s) {
+  const u32 n = g.n;
+  if (n == 0) return;
+  const u64 sn = smax_plus1(sm, n, rs, s);
+  rs.rlist.reserve(n, s);
+  const u64 nr = select_flagged(R, n, rs.rlist.p, tmp, rs.cnt, s);
+  if (nr == 0) return;
+  rs.sizes.reserve(sn, s);
+  rs.dS.reserve(sn, s);
+  rs.dpar.reserve(comm_space, s);
+  rs.mhead.reserve(sn, s);
+  rs.mnext.reserve(n, s);
+  fill_u32(rs.sizes.p, 0, sn, s);
+  hipLaunchKernelGGL(k_sizes, GRID(n), sm, n, rs.sizes.p);
+  HL_LAUNCH_CHECK();
+  // canonical subcommunity degrees + sorted member lists from one stable sort
+  label_totals_canonical(sm, g.k, n, rs.dS.p, sn, ks, vs, vidx, heads, starts, cnt, tmp, s);
+  {
+    // starts/ks/vs now hold the (s, v) segments
+    u64 nseg = 0;
+    HL_CHECK(hipMemcpyAsync(&nseg, cnt.p, sizeof(u64), hipMemcpyDeviceToHost, s));
+    HL_CHECK(hipStreamSynchronize(s));
+    fill_u32(rs.mhead.p, kNone, sn, s);
+    hipLaunchKernelGGL(k_member_lists, GRID(nseg), nseg, starts.p, u64(n), ks.p, vs.p, rs.mhead.p, rs.mnext.p);
+    HL_LAUNCH_CHECK();
+  }
+  label_totals_canonical(f, g.k, n, rs.dpar.p, comm_space, ks, vs, vidx, heads, starts, cnt, tmp, s);
+  // R by (k asc, id asc)
+  rs.kkeys.reserve(nr, s);
+  rs.kkeys_s.reserve(nr, s);
+  rs.rsorted.reserve(nr, s);
+  hipLaunchKernelGGL(k_degree_keys, GRID(nr), rs.rlist.p, nr, g.k, rs.kkeys.p);
+  HL_LAUNCH_CHECK();
+  sort_pairs_u64(rs.kkeys.p, rs.kkeys_s.p, rs.rlist.p, rs.rsorted.p, nr, 64, tmp, s);
+  u32 gcap = 64;
+  while (gcap < 2ull * max_deg + 2) gcap <<= 1;
+  rs.gk.reserve(2 * u64(gcap), s);
+  rs.gv.reserve(gcap, s);
+  if (max_deg > DET_LCAP / 2) {
+    fill_u32(rs.gk.p, kNone, gcap, s);
+    fill_f64(rs.gv.p, 0.0, gcap, s);
+  }
+  DetRefArgs a{g, f, sm, rs.rsorted.p, nr, rs.sizes.p, rs.dS.p, rs.dpar.p, rs.mhead.p, rs.mnext.p, gamma,
+               rs.gk.p, rs.gv.p, rs.gk.p + gcap, gcap};
+  hipLaunchKernelGGL(k_det_refine, 1, 256, 0, s, a);
+  HL_LAUNCH_CHECK();
+  HL_CHECK(hipStreamSynchronize(s));
+}
+
+u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, const u8* R, DevBuf<Rec>& out,
+              u32* next_n, AggScratch& as, Temp& tmp, hipStream_t s) {
+  const u32 n = g.n;
+  as.rlist.reserve(n + 1, s);
+  DevBuf<u64> c1;
+  const u64 nr = n ? select_flagged(R, n, as.rlist.p, tmp, c1, s) : 0;
+  as.cnt64.reserve(nr + 1, s);
+  as.off64.reserve(nr + 1, s);
+  u64 nemit = 0;
+  if (nr) {
+    hipLaunchKernelGGL(k_agg_count, GRID(nr * 64), g, as.rlist.p, nr, R, as.cnt64.p);
+    HL_LAUNCH_CHECK();
+    nemit = exclusive_scan_u64(as.cnt64.p, as.off64.p, nr, tmp, s);
+  }
+  const u64 total = k + nemit;
+  as.H.reserve(total + 1, s);
+  if (k) hipLaunchKernelGGL(k_lift, GRID(k), dG, k, s_pre, as.H.p);
+  HL_LAUNCH_CHECK();
+  if (nr) hipLaunchKernelGGL(k_agg_emit, GRID(nr * 64), g, as.rlist.p, nr, R, s_pre, s_cur, as.off64.p, k, as.H.p);
+  HL_LAUNCH_CHECK();
+  // next node count from s_cur and the old s_pre
+  DevBuf<u32> mx;
+  mx.reserve(1, s);
+  fill_u32(mx.p, 0, 1, s);
+  max_u32(s_cur, n, mx.p, s);
+  max_u32(s_pre, n, mx.p, s);
+  u32 hm = 0;
+  HL_CHECK(hipMemcpyAsync(&hm, mx.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  *next_n = hm + 1;
+  if (nr) hipLaunchKernelGGL(k_spre_update, GRID(nr), as.rlist.p, nr, s_cur, s_pre);
+  HL_LAUNCH_CHECK();
+  if (total == 0) return 0;
+  as.keys.reserve(total, s);
+  as.keys_s.reserve(total, s);
+  as.idx.reserve(total, s);
+  as.idx_s.reserve(total, s);
+  as.heads.reserve(total, s);
+  as.runs.reserve(total, s);
+  hipLaunchKernelGGL(k_hkeys, GRID(total), as.H.p, total, as.keys.p, as.idx.p);
+  HL_LAUNCH_CHECK();
+  sort_pairs_u64(as.keys.p, as.keys_s.p, as.idx.p, as.idx_s.p, total, 64, tmp, s);
+  hipLaunchKernelGGL(k_heads_u64, GRID(total), as.keys_s.p, total, 0, as.heads.p);
+  HL_LAUNCH_CHECK();
+  const u64 nruns = select_flagged(as.heads.p, total, as.runs.p, tmp, c1, s);
+  DevBuf<Rec> comp;
+  comp.reserve(nruns, s);
+  as.keep.reserve(nruns, s);
+  hipLaunchKernelGGL(k_compress, GRID(nruns), nruns, as.runs.p, total, as.keys_s.p, as.idx_s.p, as.H.p, comp.p,
+                     as.keep.p);
+  HL_LAUNCH_CHECK();
+  as.keep_idx.reserve(nruns, s);
+  const u64 kept = select_flagged(as.keep.p, nruns, as.keep_idx.p, tmp, c1, s);
+  out.reserve(kept + 1, s);
+  if (kept) hipLaunchKernelGGL(k_gather_rec, GRID(kept), as.keep_idx.p, kept, comp.p, out.p);
+  HL_LAUNCH_CHECK();
+  HL_CHECK(hipStreamSynchronize(s));
+  return kept;
+}
+
+void def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_def_pull, GRID(n), B, fp, fq, sp, n);
+  HL_LAUNCH_CHECK();
+}
+void def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np, hipStream_t s) {
+  if (nq) hipLaunchKernelGGL(k_def_push, GRID(nq), Bp, Bq, sq, nq, np);
+  HL_LAUNCH_CHECK();
+}
+
+double modularity(const Csr& g, const u32* lab, DevBuf<double>& sig, DevBuf<double>& acc, DevBuf<u32>& mx,
+                  hipStream_t s) {
+  const u32 n = g.n;
+  double t = 0.0;
+  HL_CHECK(hipMemcpyAsync(&t, g.twice_m, sizeof(double), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  if (t == 0.0 || n == 0) return 0.0;
+  mx.reserve(1, s);
+  fill_u32(mx.p, 0, 1, s);
+  max_u32(lab, n, mx.p, s);
+  u32 hm = 0;
+  HL_CHECK(hipMemcpyAsync(&hm, mx.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  const u64 L = u64(hm) + 1;
+  sig.reserve(L, s);
+  acc.reserve(2, s);
+  fill_f64(sig.p, 0.0, L, s);
+  fill_f64(acc.p, 0.0, 2, s);
+  label_totals_atomic(lab, g.k, n, sig.p, s);
+  hipLaunchKernelGGL(k_intra, blocks_for(n, 256, 4096), 256, 0, s, g, lab, acc.p);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sq_sum, blocks_for(L, 256, 4096), 256, 0, s, sig.p, L, g.twice_m, acc.p + 1);
+  HL_LAUNCH_CHECK();
+  double h[2];
+  HL_CHECK(hipMemcpyAsync(h, acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  return (h[0] - h[1]) / t;
+}
+
+void gather_u32(u32* cur, const u32* map, u32 n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_gather_u32, GRID(n), cur, map, n);
+  HL_LAUNCH_CHECK();
+}
+
+void canonicalize(u32* lab, u32 n, DevBuf<u32>& first, DevBuf<u8>& flag, DevBuf<u64>& pos, Temp& tmp,
+                  hipStream_t s) {
+  if (n == 0) return;
+  DevBuf<u32> mx;
+  mx.reserve(1, s);
+  fill_u32(mx.p, 0, 1, s);
+  max_u32(lab, n, mx.p, s);
+  u32 hm = 0;
+  HL_CHECK(hipMemcpyAsync(&hm, mx.p, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  const u64 L = u64(hm) + 1;
+  first.reserve(L, s);
+  fill_u32(first.p, kNone, L, s);
+  hipLaunchKernelGGL(k_first_seen, GRID(n), lab, n, first.p);
+  HL_LAUNCH_CHECK();
+  DevBuf<u64> fl;
+  fl.reserve(n + 1, s);
+  pos.reserve(n + 1, s);
+  hipLaunchKernelGGL(k_first_flags, GRID(n), lab, n, first.p, fl.p);
+  HL_LAUNCH_CHECK();
+  exclusive_scan_u64(fl.p, pos.p, n, tmp, s);
+  hipLaunchKernelGGL(k_relabel, GRID(n), lab, n, first.p, pos.p);
+  HL_LAUNCH_CHECK();
+  (void)flag;
+}
+
+}  // namespace hl

@@ -1,0 +1,140 @@
+"""Seeded synthetic workloads (libhl_gen.so): the graphs and edge-delta
+streams of the five configurations, plus small variants for tests.
+
+    small    LFR-style, n=9,877, tau1=2.5, tau2=1.5 in [5,200], mu=0.2, m=21,778,
+             5 batches of 1,000 degree-proportional new edges
+    mid      planted partition, n=1e6, 1,000 log-normal blocks (sigma=1), mean degree 32,
+             mu=0.3, w ~ U[0.5,1.5], 20 batches of 9,000 inserts + 1,000 deletes
+    large    R-MAT scale 24, edge factor 16, (0.57,0.19,0.19,0.05), 10 batches of
+             95,000 R-MAT inserts + 5,000 deletes
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_GEN_SO = os.path.join(_HERE, "libhl_gen.so")
+_g = None
+
+
+def glib():
+    global _g
+    if _g is None:
+        if not os.path.exists(_GEN_SO):
+            from . import build
+            build.build_gen()
+        L = C.CDLL(_GEN_SO)
+        vp = C.c_void_p
+        L.hg_lfr.restype = vp
+        L.hg_lfr.argtypes = [C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_double,
+                             C.c_uint64, C.c_uint64]
+        L.hg_planted.restype = vp
+        L.hg_planted.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_uint64]
+        L.hg_rmat.restype = vp
+        L.hg_rmat.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int]
+        L.hg_sbm.restype = vp
+        L.hg_sbm.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_uint64]
+        L.hg_n.restype = C.c_uint32
+        L.hg_n.argtypes = [vp]
+        L.hg_m.restype = C.c_uint64
+        L.hg_m.argtypes = [vp]
+        L.hg_edges.argtypes = [vp, vp, vp, vp]
+        L.hg_labels.argtypes = [vp, vp]
+        L.hg_make_batches.argtypes = [vp, C.c_uint32, C.c_uint64, C.c_uint64, C.c_int, C.c_double, C.c_double,
+                                      C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64]
+        L.hg_num_batches.restype = C.c_uint32
+        L.hg_num_batches.argtypes = [vp]
+        L.hg_batch_len.restype = C.c_uint64
+        L.hg_batch_len.argtypes = [vp, C.c_uint32]
+        L.hg_batch.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        L.hg_free.argtypes = [vp]
+        _g = L
+    return _g
+
+
+class Workload:
+    """A generated graph (n, u, v, w[, labels]) and its batches [(u, v, alpha)]."""
+
+    def __init__(self, h, weighted):
+        L = glib()
+        self.n = int(L.hg_n(h))
+        m = int(L.hg_m(h))
+        self.u = np.zeros(m, dtype=np.uint32)
+        self.v = np.zeros(m, dtype=np.uint32)
+        self.w = np.zeros(m, dtype=np.float64)
+        L.hg_edges(h, self.u.ctypes.data, self.v.ctypes.data, self.w.ctypes.data)
+        if not weighted:
+            self.w = None
+        lab = np.zeros(max(self.n, 1), dtype=np.uint32)
+        self.labels = lab[:self.n] if L.hg_labels(h, lab.ctypes.data) else None
+        self.batches = []
+        self._h = h
+
+    @property
+    def m(self):
+        return len(self.u)
+
+    def make_batches(self, nb, ins, dels, mode, wlo=1.0, whi=1.0, scale=0, abc=(0.57, 0.19, 0.19), seed=1):
+        L = glib()
+        L.hg_make_batches(self._h, nb, ins, dels, mode, wlo, whi, scale, abc[0], abc[1], abc[2], seed)
+        self.batches = []
+        for i in range(L.hg_num_batches(self._h)):
+            k = int(L.hg_batch_len(self._h, i))
+            u = np.zeros(k, dtype=np.uint32)
+            v = np.zeros(k, dtype=np.uint32)
+            a = np.zeros(k, dtype=np.float64)
+            L.hg_batch(self._h, i, u.ctypes.data, v.ctypes.data, a.ctypes.data)
+            self.batches.append((u, v, a))
+        return self.batches
+
+    def close(self):
+        if self._h:
+            glib().hg_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def lfr(n, mean_deg, mu, target_m=0, tau1=2.5, tau2=1.5, cmin=5, cmax=200, seed=1):
+    return Workload(glib().hg_lfr(n, tau1, mean_deg, tau2, cmin, cmax, mu, target_m, seed), weighted=False)
+
+
+def planted(n, blocks, mean_deg, mu, sigma=1.0, wlo=0.5, whi=1.5, seed=1):
+    return Workload(glib().hg_planted(n, blocks, sigma, mean_deg, mu, wlo, whi, seed), weighted=True)
+
+
+def rmat(scale, ef=16, abc=(0.57, 0.19, 0.19), seed=1, threads=0):
+    return Workload(glib().hg_rmat(scale, ef, abc[0], abc[1], abc[2], seed, threads), weighted=False)
+
+
+def sbm(n, blocks, mean_deg, mu, seed=1):
+    return Workload(glib().hg_sbm(n, blocks, mean_deg, mu, seed), weighted=False)
+
+
+# ---- the named configurations ----
+
+def small_config(seed=7):
+    """LFR-style n=9,877, m=21,778, 5 x 1,000 degree-proportional new edges."""
+    w = lfr(9877, 4.6, 0.2, target_m=21778, seed=seed)
+    w.make_batches(5, 1000, 0, mode=0, seed=seed + 1)
+    return w
+
+
+def mid_config(seed=11, n=1_000_000, blocks=1000, batches=20):
+    """planted partition n=1e6, mean degree 32, mu=0.3, w~U[0.5,1.5];
+    20 x (9,000 new inserts + 1,000 deletes)."""
+    w = planted(n, blocks, 32.0, 0.3, seed=seed)
+    w.make_batches(batches, 9000, 1000, mode=1, wlo=0.5, whi=1.5, seed=seed + 1)
+    return w
+
+
+def large_config(seed=13, scale=24, batches=10, threads=0):
+    """R-MAT scale 24, ef 16; 10 x (95,000 R-MAT inserts + 5,000 deletes)."""
+    w = rmat(scale, 16, seed=seed, threads=threads)
+    w.make_batches(batches, 95000, 5000, mode=2, scale=scale, seed=seed + 1)
+    return w

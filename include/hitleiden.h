@@ -1,0 +1,163 @@
+/*
+ * hitleiden.h -- C ABI of the MI355X (gfx950) HIT-Leiden incremental update engine.
+ *
+ * The reference crate (nameearly/hit-leiden, Rust) exposes this path as Rust
+ * functions only; each entry point below names the reference interface it
+ * replaces.  Plain pointers and sizes, no C++ or torch types: a Rust
+ * build.rs-linked `extern "C"` block binds it as is.
+ *
+ * Ownership: an hl_graph owns every device allocation of one resident graph and
+ * its hierarchy.  Host arrays passed in are borrowed for the duration of the
+ * call.  Calls on one handle are synchronous and must not overlap (the
+ * reference takes `&mut PartitionState`, hit_leiden.rs:936-941).  Bad ids or
+ * non-finite weights return HL_INVALID_INPUT instead of panicking (the
+ * reference panics, hit_leiden.rs:1060; `run` returns InvalidInput,
+ * hit_leiden.rs:31-43).
+ */
+#ifndef HITLEIDEN_H
+#define HITLEIDEN_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* HitLeidenError (src/core/error.rs:19-27) + allocation failure */
+typedef enum {
+  HL_OK = 0,
+  HL_INVALID_INPUT = 1, /* HitLeidenError::InvalidInput */
+  HL_BACKEND = 2,       /* HitLeidenError::Backend: a HIP runtime error */
+  HL_ACCELERATION = 3,  /* HitLeidenError::Acceleration: no gfx950 device */
+  HL_OOM = 4
+} hl_status;
+
+/* RunMode (src/core/config.rs:19-23) */
+typedef enum {
+  HL_MODE_DETERMINISTIC = 0, /* sequential lowest-index sweep (hit_leiden.rs:1108-1188) */
+  HL_MODE_THROUGHPUT = 1     /* synchronous rounds + decoupled moves (throughput.rs:80-236),
+                                refinement keeps merge targets in place so refined
+                                communities stay connected */
+} hl_mode;
+
+/* partition maps of one level (PartitionState, src/core/partition/state.rs:19-32) */
+typedef enum {
+  HL_MAP_F = 0,     /* community_mapping_per_level[p]            f^p  */
+  HL_MAP_G = 1,     /* refined_community_mapping_per_level[p]    g^p  */
+  HL_MAP_S_CUR = 2, /* current_subcommunity_mapping_per_level[p] s_cur^p */
+  HL_MAP_S_PRE = 3  /* previous_subcommunity_mapping_per_level[p] s_pre^p */
+} hl_map;
+
+/* vertex sets of the last batch at one level (hit_leiden.rs:951-952,1031-1033) */
+typedef enum {
+  HL_SET_ACTIVE = 0,   /* A after seeding (hit_leiden.rs:1037-1057) */
+  HL_SET_CHANGED = 1,  /* B: moved by inc_movement */
+  HL_SET_AFFECTED = 2, /* K: affected for refinement */
+  HL_SET_REFINED = 3   /* R: returned by inc_refinement */
+} hl_set;
+
+/* which partition hl_modularity scores */
+typedef enum {
+  HL_PART_COMMUNITY = 0, /* f^{P-1} o s^{P-2} o .. o s^0: top-level communities pulled down */
+  HL_PART_REFINED = 1,   /* g^0 = state.node_to_comm (hit_leiden.rs:1016) */
+  HL_PART_F0 = 2         /* f^0 as stored after def_update */
+} hl_partition;
+
+/* RunConfig (src/core/config.rs:25-51) restricted to this path */
+typedef struct {
+  double gamma;        /* resolution_parameter (config.rs:36) */
+  int32_t mode;        /* hl_mode */
+  int32_t levels;      /* P >= 1 (PartitionState.levels, state.rs:42) */
+  int32_t max_rounds;  /* throughput movement round cap, 0 = 20 (hit_leiden.rs:1077) */
+  int32_t device;      /* HIP device ordinal */
+  uint32_t small_degree; /* autotuned bucket thresholds; 0 = built-in defaults */
+  uint32_t wave_degree;
+  uint32_t reserved[6];
+} hl_config;
+
+#define HL_MAX_LEVELS 8
+
+/* per-batch counters (returned by load / apply) */
+typedef struct {
+  uint64_t iterations;               /* hit_leiden() return value (hit_leiden.rs:1017) */
+  uint32_t levels;
+  uint32_t level_n[HL_MAX_LEVELS];   /* vertices per level after the batch */
+  uint64_t delta_len[HL_MAX_LEVELS]; /* |dG^p| */
+  uint64_t active[HL_MAX_LEVELS];    /* |A| after seeding */
+  uint64_t evaluated[HL_MAX_LEVELS]; /* vertex evaluations summed over rounds */
+  uint64_t moved[HL_MAX_LEVELS];     /* |B| */
+  uint64_t affected[HL_MAX_LEVELS];  /* |K| */
+  uint64_t refined[HL_MAX_LEVELS];   /* |R| */
+  uint64_t rounds[HL_MAX_LEVELS];
+  uint64_t bytes_alg;                /* algorithmic HBM bytes of the batch */
+  double seconds;                    /* host wall time of the call */
+} hl_batch_stats;
+
+typedef struct hl_graph hl_graph;
+
+/* PartitionState::identity + RunConfig (state.rs:35-49, config.rs:25-51) */
+hl_status hl_create(const hl_config* cfg, hl_graph** out);
+void hl_destroy(hl_graph* g);
+
+/* Initial load: G^0 <- edges, then Algorithm 6 with dG = all edges on the
+   identity state (the reference tests' load pattern, hit_leiden.rs:3119-3126,
+   with the empty level-0 CSR pre-pushed).  w == NULL means unit weights
+   (Option<f64>::unwrap_or(1.0), in_memory.rs:49).  stats may be NULL. */
+hl_status hl_load(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, const double* w,
+                  hl_batch_stats* stats);
+
+/* InMemoryGraph::from (in_memory.rs:30-73) only: builds G^0, identity maps,
+   no clustering pass (for restoring a checkpoint with hl_write_partition). */
+hl_status hl_load_graph(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                        const double* w);
+
+/* hit_leiden(state, dG, gamma, mode) (hit_leiden.rs:936-1018): alpha > 0
+   inserts/strengthens, alpha < 0 deletes/weakens; ids must be < n. */
+hl_status hl_apply_batch(hl_graph* g, uint64_t k, const uint32_t* src, const uint32_t* dst, const double* alpha,
+                         hl_batch_stats* stats);
+
+/* Same as hl_apply_batch with the batch already resident in device memory
+   (d_* are device pointers on the handle's device). */
+hl_status hl_apply_batch_device(hl_graph* g, uint64_t k, const uint32_t* d_src, const uint32_t* d_dst,
+                                const double* d_alpha, hl_batch_stats* stats);
+
+hl_status hl_level_count(const hl_graph* g, int32_t* levels);
+hl_status hl_level_size(const hl_graph* g, int32_t level, uint32_t* n);
+
+/* copy-out of state.*_mapping_per_level[level] (state.rs:26-31) */
+hl_status hl_read_partition(const hl_graph* g, int32_t level, int32_t which, uint32_t* out, uint64_t cap);
+/* copy-in (checkpoint restore / seeded starts); values are not validated beyond length */
+hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const uint32_t* in, uint64_t n);
+
+/* last batch's A/B/K/R bitsets at one level, one byte per vertex */
+hl_status hl_read_frontier(const hl_graph* g, int32_t level, int32_t which, uint8_t* out, uint64_t cap);
+
+/* G^p as canonical CSR (rows sorted, self-loop = one entry with its record
+   weight; InMemoryGraph fields, in_memory.rs:20-28).  Call with NULL arrays to
+   get nnz. */
+hl_status hl_read_level_graph(const hl_graph* g, int32_t level, uint64_t* offsets, uint32_t* cols, double* w,
+                              uint64_t cap, uint64_t* nnz);
+/* dG^p of the last batch (the inc_aggregation output consumed by level p) */
+hl_status hl_read_level_delta(const hl_graph* g, int32_t level, uint32_t* u, uint32_t* v, double* w, uint64_t cap,
+                              uint64_t* k);
+/* 2m of level p (in_memory.rs:94-96 x 2) */
+hl_status hl_twice_m(const hl_graph* g, int32_t level, double* twice_m);
+
+/* PartitionResult.hierarchy_levels (types.rs:107-110): P rows of n0 labels;
+   row p < P-1 is s^p o .. o s^0, row P-1 the communities f^{P-1} o s^{P-2} o .. o s^0.
+   canonical != 0 relabels each row first-seen contiguous
+   (canonicalize_community_ids_in_place, hit_leiden.rs:358-370). */
+hl_status hl_read_hierarchy(const hl_graph* g, uint32_t* out, uint64_t cap, int32_t canonical);
+
+/* compute_modularity (hit_leiden.rs:121-161) over G^0 */
+hl_status hl_modularity(const hl_graph* g, int32_t which, double* q);
+hl_status hl_modularity_labels(const hl_graph* g, const uint32_t* labels, uint64_t n, double* q);
+
+/* HitLeidenError Display (error.rs:21-26) of the last failing call */
+const char* hl_last_error(const hl_graph* g);
+/* error text of a failed hl_create (no handle yet) */
+const char* hl_create_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

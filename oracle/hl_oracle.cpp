@@ -16,6 +16,9 @@
 //   D8 (flag OR_CONNECTED, parallel mode only) a singleton that is the target of another
 //      singleton's merge proposal keeps its place, so every refined community stays
 //      connected; the plain throughput mode applies all snapshot proposals (throughput.rs:278-287)
+//   D9 the deterministic sweep skips self-loop entries in w(v, C) as the throughput shard
+//      does (parallel_frontier.rs:164-166); the reference counts them (hit_leiden.rs:1119-1125),
+//      which only matters on levels >= 1 (level-0 graphs carry no self-loops)
 #include "hl_oracle.h"
 
 #include <algorithm>
@@ -372,8 +375,8 @@ MoveResult inc_movement(const Graph& g, const std::vector<Rec>& dG, std::vector<
     const auto& c = g.col[v];
     const auto& w = g.wt[v];
     for (size_t i = 0; i < c.size(); ++i) {
+      if (c[i] == v) continue;  // D9: self-loops skipped as in parallel_frontier.rs:164-166
       buf.push_back({f[c[i]], w[i]});
-      if (c[i] == v) buf.push_back({f[c[i]], w[i]});  // second self-loop entry
     }
     std::stable_sort(buf.begin(), buf.end(),
                      [](const std::pair<u32, double>& a, const std::pair<u32, double>& b) { return a.first < b.first; });

@@ -1,0 +1,305 @@
+"""GPU parity: the gfx950 engine against the CPU oracle, through the C ABI.
+
+Deterministic mode: every per-level map, frontier set, CSR and delta is
+bit-exact, for integer and non-integer weights.  Throughput mode: bit-exact
+for unit weights (every sum is an exact integer in f64); for non-integer
+weights the community totals are accumulated with atomics, so the check is
+modularity within 1e-3 of the oracle plus the structural invariants.
+"""
+import numpy as np
+import pytest
+
+import hit_leiden_amd as hl
+from hit_leiden_amd import gen
+from oracle import hl_oracle as O
+from parity_util import (assert_classes_connected, assert_same_state, assert_supergraph_consistent,
+                         oracle_mode, pair)
+
+pytestmark = pytest.mark.gpu
+
+MODES = [hl.RunMode.DETERMINISTIC, hl.RunMode.THROUGHPUT]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def random_graph(n, m, seed, weighted=False):
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    keep = u != v
+    u, v = np.minimum(u[keep], v[keep]), np.maximum(u[keep], v[keep])
+    key = np.unique(u.astype(np.int64) * n + v)
+    u, v = (key // n).astype(np.uint32), (key % n).astype(np.uint32)
+    w = rng.uniform(0.5, 1.5, len(u)) if weighted else None
+    return u, v, w
+
+
+def random_batch(n, u, v, w, k, seed, weighted=False, dels=0.2):
+    rng = np.random.default_rng(seed)
+    bu, bv, ba = [], [], []
+    nd = int(k * dels)
+    idx = rng.choice(len(u), size=min(nd, len(u)), replace=False)
+    for i in idx:
+        bu.append(u[i])
+        bv.append(v[i])
+        ba.append(-(w[i] if w is not None else 1.0))
+    while len(bu) < k:
+        a, b = rng.integers(0, n, 2)
+        if a == b:
+            continue
+        bu.append(a)
+        bv.append(b)
+        ba.append(rng.uniform(0.5, 1.5) if weighted else 1.0)
+    return np.array(bu, np.uint32), np.array(bv, np.uint32), np.array(ba, np.float64)
+
+
+# ---------------------------------------------------------------- known answers
+
+@pytest.mark.parametrize("mode", MODES)
+def test_movement_empty_community_golden(mode):
+    """hit_leiden.rs:2016-2052 hand-traced: det [3,4,0], throughput [3,4,5]."""
+    g, o = pair(mode, 1, gamma=3.0)
+    gi = hl.GraphInput.from_edges(3, [(0, 1, 10.0), (0, 2, 0.5)])
+    g.load_graph(gi)
+    o.load_graph(3, gi.src, gi.dst, gi.weights)
+    for h in (g,):
+        h.write_partition(0, hl.MAP_F, [0, 0, 0])
+        h.write_partition(0, hl.MAP_S_CUR, [0, 0, 0])
+        h.write_partition(0, hl.MAP_S_PRE, [0, 0, 0])
+    for which in (O.MAP_F, O.MAP_S_CUR, O.MAP_S_PRE):
+        o.write_map(0, which, [0, 0, 0])
+    g.hit_leiden(hl.GraphInput(3, [], [], []))
+    o.apply([], [], [])
+    f = g.partition(0, hl.MAP_F).tolist()
+    assert f == ([3, 4, 0] if mode == hl.RunMode.DETERMINISTIC else [3, 4, 5])
+    assert_same_state(g, o, 1, "golden")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_intra_deletion_activates(mode):
+    """hit_leiden.rs:2520-2573 through the batch API: delete 0-2, node 2 joins 3."""
+    g, o = pair(mode, 1)
+    gi = hl.GraphInput.from_edges(4, [(0, 1, 10.0), (0, 2, 1.0), (2, 3, 10.0)])
+    g.load_graph(gi)
+    o.load_graph(4, gi.src, gi.dst, gi.weights)
+    for which, val in ((hl.MAP_F, [0, 0, 0, 1]), (hl.MAP_S_CUR, [0, 1, 2, 3]), (hl.MAP_S_PRE, [0, 1, 2, 3])):
+        g.write_partition(0, which, val)
+        o.write_map(0, which, val)
+    g.hit_leiden(hl.GraphInput(4, [0], [2], [-1.0]))
+    o.apply([0], [2], [-1.0])
+    f = g.partition(0, hl.MAP_F)
+    assert f[2] == f[3]
+    assert g.frontier(0, hl.SET_CHANGED)[2]
+    assert_same_state(g, o, 1, "deletion")
+
+
+def test_modularity_closed_forms_gpu():
+    tri = hl.GraphInput.from_edges(3, [(0, 1), (1, 2), (0, 2)])
+    assert abs(hl.compute_modularity(tri, [0, 0, 0])) < 1e-12
+    assert abs(hl.compute_modularity(tri, [0, 1, 2]) + 1.0 / 3.0) < 1e-12
+    two = hl.GraphInput.from_edges(6, [(0, 1), (1, 2), (0, 2), (3, 4), (4, 5), (3, 5)])
+    assert abs(hl.compute_modularity(two, [0, 0, 0, 1, 1, 1]) - 0.5) < 1e-12
+    assert abs(hl.compute_modularity(hl.GraphInput.from_edges(3, []), [0, 1, 2])) < 1e-12
+    w = hl.GraphInput.from_edges(2, [(0, 1, 5.0)])
+    assert abs(hl.compute_modularity(w, [0, 1]) + 0.5) < 1e-12
+
+
+# ---------------------------------------------------------------- random graphs
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("levels", [1, 2, 4])
+def test_random_unit_weight_parity(mode, levels):
+    for seed in range(3):
+        n = 300 + 100 * seed
+        u, v, _ = random_graph(n, 4 * n, seed)
+        say(f"random unit mode={int(mode)} P={levels} seed={seed} n={n} m={len(u)}")
+        g, o = pair(mode, levels)
+        g.load(hl.GraphInput(n, u, v))
+        o.load(n, u, v)
+        assert_same_state(g, o, levels, f"load seed {seed}")
+        for b in range(3):
+            bu, bv, ba = random_batch(n, u, v, None, 60, 100 * seed + b)
+            g.hit_leiden(hl.GraphInput(n, bu, bv, ba))
+            o.apply(bu, bv, ba)
+            assert_same_state(g, o, levels, f"seed {seed} batch {b}")
+        assert_supergraph_consistent(g, levels)
+        g.close()
+
+
+@pytest.mark.parametrize("levels", [1, 3])
+def test_random_weighted_det_bit_exact(levels):
+    mode = hl.RunMode.DETERMINISTIC
+    for seed in range(3):
+        n = 250 + 50 * seed
+        u, v, w = random_graph(n, 5 * n, 40 + seed, weighted=True)
+        say(f"random weighted det P={levels} seed={seed}")
+        g, o = pair(mode, levels)
+        g.load(hl.GraphInput(n, u, v, w))
+        o.load(n, u, v, w)
+        assert_same_state(g, o, levels, f"wload {seed}")
+        for b in range(3):
+            bu, bv, ba = random_batch(n, u, v, w, 50, 7 * seed + b, weighted=True)
+            g.hit_leiden(hl.GraphInput(n, bu, bv, ba))
+            o.apply(bu, bv, ba)
+            assert_same_state(g, o, levels, f"w seed {seed} batch {b}")
+        g.close()
+
+
+@pytest.mark.parametrize("levels", [1, 3])
+def test_random_weighted_throughput_quality(levels):
+    mode = hl.RunMode.THROUGHPUT
+    for seed in range(3):
+        n = 400
+        u, v, w = random_graph(n, 6 * n, 70 + seed, weighted=True)
+        say(f"random weighted thr P={levels} seed={seed}")
+        g, o = pair(mode, levels)
+        g.load(hl.GraphInput(n, u, v, w))
+        o.load(n, u, v, w)
+        for b in range(2):
+            bu, bv, ba = random_batch(n, u, v, w, 50, 9 * seed + b, weighted=True)
+            g.hit_leiden(hl.GraphInput(n, bu, bv, ba))
+            o.apply(bu, bv, ba)
+        qg = g.modularity(hl.PART_COMMUNITY)
+        qo = o.modularity(o.hierarchy()[-1])
+        assert abs(qg - qo) <= 1e-3, (qg, qo)
+        assert_supergraph_consistent(g, levels, tol=1e-9)
+        g.close()
+
+
+# ---------------------------------------------------------------- the small configuration
+
+@pytest.mark.parametrize("mode", MODES)
+def test_small_config_parity(mode):
+    """LFR-style n=9,877, m=21,778 + 5 x 1,000 new edges, 4 levels: bit-exact."""
+    w = gen.small_config()
+    P = 4
+    say(f"small config mode={int(mode)}")
+    g, o = pair(mode, P)
+    g.load(hl.GraphInput(w.n, w.u, w.v))
+    o.load(w.n, w.u, w.v)
+    assert_same_state(g, o, P, "small load")
+    for i, (u, v, a) in enumerate(w.batches):
+        g.hit_leiden(hl.GraphInput(w.n, u, v, a))
+        o.apply(u, v, a)
+        assert_same_state(g, o, P, f"small batch {i}")
+        qg = g.modularity(hl.PART_COMMUNITY)
+        qo = o.modularity(o.hierarchy()[-1])
+        assert abs(qg - qo) < 1e-9, (i, qg, qo)
+        assert abs(g.modularity(hl.PART_REFINED) - o.modularity(o.read_map(0, O.MAP_G))) < 1e-9
+    H = g.hierarchy()
+    assert np.array_equal(H, o.hierarchy())
+    assert_supergraph_consistent(g, P)
+    if mode == hl.RunMode.THROUGHPUT or True:
+        # refined partition (s^{P-2} o .. o s^0) classes are connected in G^0
+        off, col, ww = g.supergraph(0)
+        rows = np.repeat(np.arange(w.n), np.diff(off.astype(np.int64)))
+        assert_classes_connected(w.n, rows, col.astype(np.int64), H[P - 2])
+    g.close()
+
+
+# ---------------------------------------------------------------- mid-like (weighted)
+
+def test_midlike_det_bit_exact():
+    w = gen.planted(20000, 40, 16.0, 0.3, seed=21)
+    w.make_batches(3, 300, 60, mode=1, wlo=0.5, whi=1.5, seed=22)
+    say("midlike det")
+    g, o = pair(hl.RunMode.DETERMINISTIC, 2)
+    g.load(hl.GraphInput(w.n, w.u, w.v, w.w))
+    o.load(w.n, w.u, w.v, w.w)
+    assert_same_state(g, o, 2, "midlike load")
+    for i, (u, v, a) in enumerate(w.batches):
+        g.hit_leiden(hl.GraphInput(w.n, u, v, a))
+        o.apply(u, v, a)
+        assert_same_state(g, o, 2, f"midlike batch {i}")
+    assert abs(g.modularity(hl.PART_COMMUNITY) - o.modularity(o.hierarchy()[-1])) < 1e-9
+    g.close()
+
+
+def test_midlike_throughput_quality_and_structure():
+    w = gen.planted(60000, 120, 16.0, 0.3, seed=31)
+    w.make_batches(3, 1000, 200, mode=1, wlo=0.5, whi=1.5, seed=32)
+    P = 4
+    say("midlike throughput")
+    g, o = pair(hl.RunMode.THROUGHPUT, P)
+    g.load(hl.GraphInput(w.n, w.u, w.v, w.w))
+    o.load(w.n, w.u, w.v, w.w)
+    for i, (u, v, a) in enumerate(w.batches):
+        g.hit_leiden(hl.GraphInput(w.n, u, v, a))
+        o.apply(u, v, a)
+        qg = g.modularity(hl.PART_COMMUNITY)
+        qo = o.modularity(o.hierarchy()[-1])
+        assert abs(qg - qo) <= 1e-3, (i, qg, qo)
+    assert_supergraph_consistent(g, P, tol=1e-9)
+    H = g.hierarchy()
+    off, col, ww = g.supergraph(0)
+    rows = np.repeat(np.arange(w.n), np.diff(off.astype(np.int64)))
+    assert_classes_connected(w.n, rows, col.astype(np.int64), H[P - 2])
+    g.close()
+
+
+# ---------------------------------------------------------------- edge cases
+
+@pytest.mark.parametrize("mode", MODES)
+def test_edge_cases(mode):
+    say(f"edge cases mode={int(mode)}")
+    n = 50
+    u, v, _ = random_graph(n, 120, 5)
+    g, o = pair(mode, 2)
+    g.load(hl.GraphInput(n, u, v))
+    o.load(n, u, v)
+    cases = [
+        ("empty batch", [], [], []),
+        ("delete absent edge", [0], [49], [-1.0]),
+        ("duplicate records", [1, 1, 2], [3, 3, 4], [1.0, 1.0, 1.0]),
+        ("insert then delete same key", [5, 5], [6, 6], [2.0, -2.0]),
+        ("self-loop record", [7], [7], [1.5]),
+        ("weaken to zero", [int(u[0])], [int(v[0])], [-1.0]),
+    ]
+    for name, bu, bv, ba in cases:
+        bu, bv, ba = np.array(bu, np.uint32), np.array(bv, np.uint32), np.array(ba, np.float64)
+        g.hit_leiden(hl.GraphInput(n, bu, bv, ba))
+        o.apply(bu, bv, ba)
+        assert_same_state(g, o, 2, name)
+    with pytest.raises(hl.InvalidInput):
+        g.hit_leiden(hl.GraphInput(n, [0], [n], [1.0]))
+    with pytest.raises(hl.InvalidInput):
+        g.hit_leiden(hl.GraphInput(n, [0], [1], [float("nan")]))
+    # the failed calls left the state untouched
+    assert_same_state(g, o, 2, "after errors")
+    g.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_degenerate_graphs(mode):
+    for n, edges in ((1, []), (5, []), (2, [(0, 1)])):
+        g, o = pair(mode, 2)
+        gi = hl.GraphInput.from_edges(n, edges)
+        g.load(gi)
+        o.load(n, gi.src, gi.dst)
+        assert_same_state(g, o, 2, f"degenerate n={n}")
+        g.close()
+
+
+def test_hub_rows_use_big_bucket():
+    """A star with 3,000 leaves forces the workgroup-per-vertex table path."""
+    n = 3200
+    hub_u = np.zeros(3000, np.uint32)
+    hub_v = np.arange(1, 3001, dtype=np.uint32)
+    ru, rv, _ = random_graph(n, 6000, 77)
+    u = np.concatenate([hub_u, ru])
+    v = np.concatenate([hub_v, rv])
+    key = np.unique(np.minimum(u, v).astype(np.int64) * n + np.maximum(u, v))
+    u, v = (key // n).astype(np.uint32), (key % n).astype(np.uint32)
+    for mode in MODES:
+        say(f"hub mode={int(mode)}")
+        g, o = pair(mode, 2)
+        g.load(hl.GraphInput(n, u, v))
+        o.load(n, u, v)
+        assert_same_state(g, o, 2, "hub load")
+        bu, bv, ba = random_batch(n, u, v, None, 200, 78)
+        g.hit_leiden(hl.GraphInput(n, bu, bv, ba))
+        o.apply(bu, bv, ba)
+        assert_same_state(g, o, 2, "hub batch")
+        g.close()
