@@ -1,0 +1,200 @@
+"""Benchmark of the incremental HIT-Leiden update on one MI355X.
+
+A step is one edge-delta batch applied by hl_apply_batch_device to the
+device-resident graph and hierarchy (apply dG, local moving, refinement,
+aggregation per level, def-update), with the batch already in HBM when the
+timed region starts.  Default workload: the planted-partition configuration
+(n = 1,000,000, 1,000 log-normal blocks, mean degree 32, mu = 0.3, weights
+U[0.5,1.5]; batches of 9,000 new-edge inserts + 1,000 deletes), 4 levels,
+throughput mode.
+
+Prints one JSON line: edges/s over the timed steps, p50/p99 batch latency,
+the roofline of the dominant kernel (HIP events on the engine stream) and the
+CPU oracle's rate on a bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "incremental edges/s + p99 update latency vs plain GPU path; HBM GB/s"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name, nbatches, seed):
+    from hit_leiden_amd import gen
+    if name == "mid":
+        w = gen.mid_config(batches=nbatches)
+        desc = {"workload": "planted-partition n=1e6, 1000 log-normal blocks (sigma=1), mean degree 32, mu=0.3, "
+                            "w~U[0.5,1.5]; batches of 9000 new-edge inserts + 1000 deletes",
+                "n": w.n, "m": w.m, "batch_edges": 10000}
+    elif name == "small":
+        w = gen.small_config()
+        while len(w.batches) < nbatches:
+            w.make_batches(nbatches, 1000, 0, mode=0, seed=seed)
+        desc = {"workload": "LFR-style n=9877, m=21778, tau1=2.5, tau2=1.5 in [5,200], mu=0.2; batches of 1000 "
+                            "degree-proportional new edges", "n": w.n, "m": w.m, "batch_edges": 1000}
+    elif name == "large":
+        w = gen.large_config(batches=nbatches)
+        desc = {"workload": "R-MAT scale 24, edge factor 16, (0.57,0.19,0.19,0.05), symmetrised, deduplicated; "
+                            "batches of 95000 R-MAT inserts + 5000 deletes", "n": w.n, "m": w.m,
+                "batch_edges": 100000}
+    else:
+        raise SystemExit(f"unknown config {name}")
+    return w, desc
+
+
+def cpu_baseline(w, levels, gamma, budget_s, threads):
+    """The CPU oracle (a C++ port of the reference path) on this host: load the
+    same graph (untimed), then apply the workload's batches until budget_s of
+    batch time is spent."""
+    from oracle import hl_oracle as O
+    st = O.OracleState(levels=levels, gamma=gamma, mode=O.PAR, threads=threads)
+    t0 = time.time()
+    st.load(w.n, w.u, w.v, w.w)
+    load_s = time.time() - t0
+    spent, edges, done = 0.0, 0, 0
+    for (u, v, a) in w.batches:
+        t = time.perf_counter()
+        st.apply(u, v, a)
+        spent += time.perf_counter() - t
+        edges += len(u)
+        done += 1
+        if spent >= budget_s:
+            break
+    q = st.modularity(st.hierarchy()[-1])
+    return {"value": edges / spent if spent > 0 else 0.0, "unit": "edges/s", "cores": threads, "kind": "port",
+            "sample": f"oracle throughput mode ({levels} levels) on the same graph: load {load_s:.1f}s untimed, "
+                      f"then the first {done} batches ({edges} edges) timed",
+            "load_seconds": load_s, "modularity": q}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="mid", choices=["small", "mid", "large"])
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--mode", default="throughput", choices=["throughput", "deterministic"])
+    ap.add_argument("--gamma", type=float, default=1.0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle batch time")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch of the dominant kernel")
+    ap.add_argument("--dump-profile", default=None)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              f"(the path runs on a single MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import hit_leiden_amd as hl
+    from hit_leiden_amd import hip
+
+    nb = args.warmup + args.steps
+    t0 = time.time()
+    w, desc = make_workload(args.config, max(nb, 1), 1)
+    log(f"[bench] workload {args.config}: n={w.n} m={w.m} batches={len(w.batches)} gen {time.time() - t0:.1f}s")
+    if len(w.batches) < nb:
+        raise SystemExit("not enough batches generated")
+
+    mode = hl.RunMode.THROUGHPUT if args.mode == "throughput" else hl.RunMode.DETERMINISTIC
+    g = hl.HitLeiden(hl.RunConfig(resolution=args.gamma, mode=mode, levels=args.levels))
+    t0 = time.time()
+    ls = g.load(hl.GraphInput(w.n, w.u, w.v, w.w))
+    load_s = time.time() - t0
+    q0 = g.modularity(hl.PART_COMMUNITY)
+    log(f"[bench] load {load_s:.2f}s Q={q0:.4f} levels={ls.as_dict()['level_n']}")
+
+    dev = []
+    for (u, v, a) in w.batches[:nb]:
+        dev.append((len(u), hip.DeviceArray(u), hip.DeviceArray(v), hip.DeviceArray(a)))
+    hip.synchronize()
+
+    # warmup with every kernel family timed: picks the dominant kernel
+    g.profile(True)
+    for i in range(args.warmup):
+        k, du, dv, da = dev[i]
+        g.apply_batch_device(k, du.address, dv.address, da.address)
+    prof_warm = g.profile_read()
+    dominant = max(prof_warm.items(), key=lambda kv: kv[1][1])[0] if prof_warm else None
+    log(f"[bench] warmup profile (ms): " + ", ".join(f"{k}={v[1]:.2f}" for k, v in
+                                                   sorted(prof_warm.items(), key=lambda kv: -kv[1][1])[:8]))
+
+    # timed region: only the dominant kernel is bracketed by events
+    g.profile(True, only=dominant) if dominant else g.profile(False)
+    hip.synchronize()
+    lat, edges = [], 0
+    stats = []
+    t_start = time.perf_counter()
+    for i in range(args.warmup, nb):
+        k, du, dv, da = dev[i]
+        t = time.perf_counter()
+        s = g.apply_batch_device(k, du.address, dv.address, da.address)
+        lat.append(time.perf_counter() - t)
+        edges += k
+        stats.append(s.as_dict())
+    hip.synchronize()
+    total = time.perf_counter() - t_start
+    prof = g.profile_read()
+    g.profile(False)
+    q1 = g.modularity(hl.PART_COMMUNITY)
+    lat_ms = np.array(lat) * 1000.0
+    value = edges / total
+
+    roof = None
+    if dominant and dominant in prof:
+        launches, ms, byts = prof[dominant]
+        per_launch_bytes = byts / launches
+        avg_ms = ms / launches
+        achieved = (per_launch_bytes / (avg_ms * 1e-3)) / 1e9 if avg_ms > 0 else 0.0
+        traffic = None
+        if args.traffic and os.path.exists(args.traffic):
+            tj = json.load(open(args.traffic))
+            traffic = tj.get(dominant)
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dominant,
+                "launches": launches, "avg_launch_us": round(avg_ms * 1000.0, 3),
+                "alg_bytes_per_launch": round(per_launch_bytes, 1)}
+    bytes_alg = sum(s["bytes_alg"] for s in stats)
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(float(lat_ms.mean()), 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(desc, levels=args.levels, mode=args.mode, gamma=args.gamma),
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
+        "path_gbs": round(bytes_alg / total / 1e9, 3), "path_hbm_frac": round(bytes_alg / total / 1e9 / HBM_PEAK_GBS, 6),
+        "modularity_after_load": round(q0, 6), "modularity_after_steps": round(q1, 6), "load_seconds": round(load_s, 3),
+        "roofline": roof,
+    }
+    if args.dump_profile:
+        json.dump({"warmup": prof_warm, "timed": prof, "stats": stats}, open(args.dump_profile, "w"), indent=1)
+    g.close()
+    for _, a, b, c in dev:
+        a.free(), b.free(), c.free()
+
+    if not args.no_cpu_baseline:
+        threads = len(os.sched_getaffinity(0))
+        threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+        log(f"[bench] cpu baseline: oracle with {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(w, args.levels, args.gamma, args.cpu_budget, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

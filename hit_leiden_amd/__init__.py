@@ -121,6 +121,9 @@ def lib():
             "hl_modularity_labels": ([vp, vp, C.c_uint64, f64p], C.c_int),
             "hl_last_error": ([vp], C.c_char_p),
             "hl_create_error": ([], C.c_char_p),
+            "hl_profile_slots": ([], C.c_int32),
+            "hl_profile_enable": ([vp, C.c_int32], C.c_int),
+            "hl_profile_read": ([vp, C.c_int32, C.c_char_p, C.c_uint64, u64p, f64p, u64p], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -334,6 +337,34 @@ class HitLeiden:
         q = C.c_double()
         self._check(lib().hl_modularity(self._h, which, C.byref(q)))
         return q.value
+
+    # ---- device timing ----
+    def profile(self, on=True, only=None):
+        """Start (reset) or stop per-kernel HIP-event timing on the engine stream;
+        only=<kernel name> records that kernel family alone."""
+        code = 1 if on else 0
+        if on and only is not None:
+            code = 2 + self.profile_slot(only)
+        self._check(lib().hl_profile_enable(self._h, code))
+
+    def profile_slot(self, name):
+        buf = C.create_string_buffer(64)
+        for s in range(lib().hl_profile_slots()):
+            self._check(lib().hl_profile_read(self._h, s, buf, 64, None, None, None))
+            if buf.value.decode() == name:
+                return s
+        raise KeyError(name)
+
+    def profile_read(self):
+        """{kernel: (launches, device_ms, algorithmic_bytes)} since profile(True)."""
+        out = {}
+        buf = C.create_string_buffer(64)
+        for s in range(lib().hl_profile_slots()):
+            n, ms, b = C.c_uint64(), C.c_double(), C.c_uint64()
+            self._check(lib().hl_profile_read(self._h, s, buf, 64, C.byref(n), C.byref(ms), C.byref(b)))
+            if n.value:
+                out[buf.value.decode()] = (n.value, ms.value, b.value)
+        return out
 
     def modularity_labels(self, labels):
         a = np.ascontiguousarray(labels, dtype=np.uint32)

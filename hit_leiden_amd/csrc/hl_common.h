@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace hl {
 
@@ -86,6 +87,89 @@ struct DevBuf {
     o.cap = tc;
   }
 };
+
+// ---- optional per-kernel timing with HIP events on the engine stream ----
+enum ProfSlot {
+  P_ROW_MERGE, P_RESOLVE, P_ROW_DEGREE, P_PROP_SMALL, P_PROP_WAVE, P_PROP_BIG, P_DECOUPLE, P_APPLY_MOVES,
+  P_MARK_MOVED, P_LABEL_TOTALS, P_SELECT, P_SORT, P_SCAN, P_CC_SPLIT, P_REFINE, P_AGG_EMIT, P_COMPRESS,
+  P_DEF_UPDATE, P_FILL, P_DET_MOVE, P_DET_REFINE, P_TWICE_M, P_SEED, P_MODULARITY, P_REDUCE, P_COUNT
+};
+inline const char* prof_name(int s) {
+  static const char* names[P_COUNT] = {
+      "row_merge", "resolve_runs", "row_degree", "propose_small", "propose_wave", "propose_big", "decouple",
+      "apply_moves", "mark_moved", "label_totals", "select", "sort", "scan", "cc_split", "refine", "agg_emit",
+      "compress", "def_update", "fill", "det_move", "det_refine", "twice_m", "seed", "modularity", "reduce"};
+  return (s >= 0 && s < P_COUNT) ? names[s] : "?";
+}
+
+struct Profiler {
+  bool on = false;
+  int only = -1;  // >= 0: record this slot only
+  double ms[P_COUNT] = {};
+  u64 launches[P_COUNT] = {};
+  u64 bytes[P_COUNT] = {};
+  struct Pending {
+    hipEvent_t a, b;
+    int slot;
+    u64 bytes;
+  };
+  std::vector<hipEvent_t> pool;
+  std::vector<Pending> pending;
+  hipEvent_t get() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      HL_CHECK(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  void reset() {
+    for (int i = 0; i < P_COUNT; ++i) ms[i] = 0.0, launches[i] = 0, bytes[i] = 0;
+  }
+  // call after the stream has been synchronised
+  void flush() {
+    for (auto& p : pending) {
+      float t = 0.f;
+      HL_CHECK(hipEventElapsedTime(&t, p.a, p.b));
+      ms[p.slot] += t;
+      launches[p.slot] += 1;
+      bytes[p.slot] += p.bytes;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  ~Profiler() {
+    for (auto& p : pending) pool.push_back(p.a), pool.push_back(p.b);
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+extern thread_local Profiler* g_prof;
+
+struct ProfScope {
+  hipStream_t s;
+  int slot;
+  u64 bytes;
+  hipEvent_t a = nullptr;
+  ProfScope(int sl, u64 b, hipStream_t st) : s(st), slot(sl), bytes(b) {
+    if (g_prof && g_prof->on && (g_prof->only < 0 || g_prof->only == sl)) {
+      a = g_prof->get();
+      HL_CHECK(hipEventRecord(a, s));
+    }
+  }
+  void set_bytes(u64 b) { bytes = b; }
+  ~ProfScope() {
+    if (a) {
+      hipEvent_t b = g_prof->get();
+      if (hipEventRecord(b, s) == hipSuccess) g_prof->pending.push_back({a, b, slot, bytes});
+    }
+  }
+};
+#define HL_PROF_CAT2(a, b) a##b
+#define HL_PROF_CAT(a, b) HL_PROF_CAT2(a, b)
+#define HL_PROF(slot, bytes, stream) ::hl::ProfScope HL_PROF_CAT(prof_scope_, __LINE__)(slot, bytes, stream)
 
 inline unsigned blocks_for(u64 n, unsigned bs = 256, u64 max_blocks = 1u << 20) {
   u64 b = (n + bs - 1) / bs;

@@ -51,6 +51,7 @@ struct Engine {
   std::string err;
   Thresholds th;
 
+  Profiler prof;
   Temp tmp;
   ApplyScratch as;
   ProposeScratch ps;
@@ -391,10 +392,25 @@ struct hl_graph {
 namespace {
 thread_local std::string g_create_err;
 
+struct ProfBind {
+  explicit ProfBind(hl::Engine* e) { hl::g_prof = e ? &e->prof : nullptr; }
+  ~ProfBind() {
+    if (hl::g_prof && hl::g_prof->on && !hl::g_prof->pending.empty()) {
+      // stream work of this call has completed when the call returns normally
+      try {
+        hl::g_prof->flush();
+      } catch (...) {
+      }
+    }
+    hl::g_prof = nullptr;
+  }
+};
+
 template <class F>
 hl_status guard(const hl_graph* gc, F&& fn) {
   hl_graph* g = const_cast<hl_graph*>(gc);
   try {
+    ProfBind pb(g ? &g->e : nullptr);
     fn();
     if (g) g->e.err.clear();
     return HL_OK;
@@ -726,6 +742,33 @@ hl_status hl_modularity(const hl_graph* g, int32_t which, double* q) {
       throw hl::InputError("unknown partition");
     *q = hl::modularity(e.L[0]->csr(), e.lab.p, e.sig, e.acc, e.mx, e.s);
   });
+}
+
+int32_t hl_profile_slots(void) { return hl::P_COUNT; }
+
+hl_status hl_profile_enable(hl_graph* g, int32_t on) {
+  if (!g) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    HL_CHECK(hipStreamSynchronize(g->e.s));
+    g->e.prof.flush();
+    g->e.prof.reset();
+    g->e.prof.on = on != 0;
+    g->e.prof.only = on >= 2 ? on - 2 : -1;
+  });
+}
+
+hl_status hl_profile_read(const hl_graph* g, int32_t slot, char* name, uint64_t name_cap, uint64_t* launches,
+                          double* ms, uint64_t* bytes) {
+  if (!g || slot < 0 || slot >= hl::P_COUNT) return HL_INVALID_INPUT;
+  const hl::Profiler& p = g->e.prof;
+  if (name && name_cap) {
+    std::strncpy(name, hl::prof_name(slot), name_cap - 1);
+    name[name_cap - 1] = 0;
+  }
+  if (launches) *launches = p.launches[slot];
+  if (ms) *ms = p.ms[slot];
+  if (bytes) *bytes = p.bytes[slot];
+  return HL_OK;
 }
 
 hl_status hl_modularity_labels(const hl_graph* g, const uint32_t* labels, uint64_t n, double* q) {

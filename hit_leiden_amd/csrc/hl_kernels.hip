@@ -18,6 +18,7 @@
 #include "hl_kernels.h"
 
 namespace hl {
+thread_local Profiler* g_prof = nullptr;
 namespace {
 
 constexpr u8 OP_UPD = 1, OP_DEL = 2, OP_INS = 3;
@@ -641,16 +642,19 @@ __global__ __launch_bounds__(256) void k_prop_big(P pol, Csr g, const u32* list,
 // degree buckets: lists[0..cnt) small, lists[cnt..2cnt) wave, lists[2cnt..3cnt) big
 __global__ void k_bucket(const u32* verts, u64 cnt, const u32* row_len, u32 small_deg, u32 wave_deg, u32* lists,
                          u32* counts, u64* degsum) {
-  u64 local = 0;
+  u64 local[3] = {0, 0, 0};
   for (u64 i = gtid(); i < cnt; i += gstride()) {
     u32 d = row_len[verts[i]];
-    local += d;
     u32 b = d <= small_deg ? 0 : (d <= wave_deg ? 1 : 2);
+    local[b] += d;
     u32 p = atomicAdd(&counts[b], 1u);
     lists[b * cnt + p] = u32(i);
   }
-  for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
-  if ((threadIdx.x & 63) == 0 && local) atomicAdd((unsigned long long*)degsum, (unsigned long long)local);
+  for (int b = 0; b < 3; ++b) {
+    u64 x = local[b];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd((unsigned long long*)&degsum[b], (unsigned long long)x);
+  }
 }
 __global__ void k_big_sizes(const u32* list, u32 cnt, const u32* verts, const u32* row_len, u64* sizes) {
   for (u64 j = gtid(); j < cnt; j += gstride()) {
@@ -1504,28 +1508,34 @@ __global__ void k_relabel(u32* lab, u32 n, const u32* first, const u64* pos) {
 #define GRID(n) blocks_for((n), 256), 256, 0, s
 
 void fill_u8(u8* p, u8 v, u64 n, hipStream_t s) {
+  HL_PROF(P_FILL, n, s);
   if (n) hipLaunchKernelGGL(k_fill_u8, GRID(n), p, v, n);
   HL_LAUNCH_CHECK();
 }
 void fill_u32(u32* p, u32 v, u64 n, hipStream_t s) {
+  HL_PROF(P_FILL, n, s);
   if (n) hipLaunchKernelGGL(k_fill_u32, GRID(n), p, v, n);
   HL_LAUNCH_CHECK();
 }
 void fill_f64(double* p, double v, u64 n, hipStream_t s) {
+  HL_PROF(P_FILL, n, s);
   if (n) hipLaunchKernelGGL(k_fill_f64, GRID(n), p, v, n);
   HL_LAUNCH_CHECK();
 }
 void iota_u32(u32* p, u32 base, u64 n, hipStream_t s) {
+  HL_PROF(P_FILL, n, s);
   if (n) hipLaunchKernelGGL(k_iota_u32, GRID(n), p, base, n);
   HL_LAUNCH_CHECK();
 }
 void copy_u8(u8* d, const u8* src, u64 n, hipStream_t s) {
+  HL_PROF(P_FILL, n, s);
   if (n) hipLaunchKernelGGL(k_copy_u8, GRID(n), d, src, n);
   HL_LAUNCH_CHECK();
 }
 u64 count_u8(const u8* a, u64 n, DevBuf<u64>& cnt, hipStream_t s) {
   cnt.reserve(1, s);
   HL_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(u64), s));
+  HL_PROF(P_REDUCE, n, s);
   if (n) hipLaunchKernelGGL(k_count_u8, blocks_for(n, 256, 4096), 256, 0, s, a, n, (unsigned long long*)cnt.p);
   HL_LAUNCH_CHECK();
   u64 h = 0;
@@ -1534,6 +1544,7 @@ u64 count_u8(const u8* a, u64 n, DevBuf<u64>& cnt, hipStream_t s) {
   return h;
 }
 void max_u32(const u32* a, u64 n, u32* out, hipStream_t s) {
+  HL_PROF(P_REDUCE, 4 * n, s);
   if (n) hipLaunchKernelGGL(k_max_u32, GRID(n), a, n, out);
   HL_LAUNCH_CHECK();
 }
@@ -1545,7 +1556,10 @@ u64 select_flagged(const u8* flags, u64 n, u32* out, Temp& tmp, DevBuf<u64>& cnt
   rocprim::counting_iterator<u32> it(0);
   HL_CHECK(rocprim::select(nullptr, bytes, it, flags, out, cnt.p, n, s));
   void* t = tmp.get(bytes, s);
-  HL_CHECK(rocprim::select(t, bytes, it, flags, out, cnt.p, n, s));
+  {
+    HL_PROF(P_SELECT, n, s);
+    HL_CHECK(rocprim::select(t, bytes, it, flags, out, cnt.p, n, s));
+  }
   u64 c = 0;
   HL_CHECK(hipMemcpyAsync(&c, cnt.p, sizeof(u64), hipMemcpyDeviceToHost, s));
   HL_CHECK(hipStreamSynchronize(s));
@@ -1562,6 +1576,7 @@ u64 exclusive_scan_u64(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s)
   size_t bytes = 0;
   HL_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, u64(0), n, rocprim::plus<u64>(), s));
   void* t = tmp.get(bytes, s);
+  HL_PROF(P_SCAN, 16 * n, s);
   HL_CHECK(rocprim::exclusive_scan(t, bytes, in, out, u64(0), n, rocprim::plus<u64>(), s));
   hipLaunchKernelGGL(k_scan_tail, 1, 1, 0, s, in, out, n);
   HL_LAUNCH_CHECK();
@@ -1577,6 +1592,7 @@ void sort_pairs_u64(const u64* kin, u64* kout, const u32* vin, u32* vout, u64 n,
   size_t bytes = 0;
   HL_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
   void* t = tmp.get(bytes, s);
+  HL_PROF(P_SORT, 24 * n, s);
   HL_CHECK(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
 }
 void sort_pairs_u32(const u32* kin, u32* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
@@ -1585,6 +1601,7 @@ void sort_pairs_u32(const u32* kin, u32* kout, const u32* vin, u32* vout, u64 n,
   size_t bytes = 0;
   HL_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
   void* t = tmp.get(bytes, s);
+  HL_PROF(P_SORT, 16 * n, s);
   HL_CHECK(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
 }
 
@@ -1621,9 +1638,12 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   a.uw.reserve(maxupd, s);
   a.ucount.reserve(1, s);
   fill_u32(a.ucount.p, 0, 1, s);
+  {
+  HL_PROF(P_RESOLVE, 16 * k + 32 * plan.nruns, s);
   hipLaunchKernelGGL(k_resolve_runs, GRID(plan.nruns), plan.nruns, a.runs.p, k, a.keys_s.p, a.idx_s.p, d, g,
                      a.ukey.p, a.uidx.p, a.uop.p, a.uw.p, a.ucount.p);
   HL_LAUNCH_CHECK();
+  }
   u32 nupd = 0;
   HL_CHECK(hipMemcpyAsync(&nupd, a.ucount.p, sizeof(u32), hipMemcpyDeviceToHost, s));
   HL_CHECK(hipStreamSynchronize(s));
@@ -1665,12 +1685,16 @@ void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, co
   (void)det;
   if (plan.nseg == 0) return;
   const u64 ns = plan.nseg;
+  {
+  HL_PROF(P_ROW_MERGE, 12 * (plan.old_entries + plan.need) + 21 * plan.nupd, s);
   hipLaunchKernelGGL(k_row_merge, dim3(unsigned(plan.total_chunks)), dim3(256), 0, s, ns, a.chunkbase.p, a.segs.p,
                      a.seg_row.p, a.seg_len.p, a.seg_ins.p, a.seg_del.p, a.ukey_s.p, a.perm.p, a.uop.p, a.uw.p,
                      a.ins_pre.p, a.del_pre.p, a.newoff.p, pool_base, g);
   HL_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_row_commit, GRID(ns), ns, a.seg_row.p, a.new_len.p, a.newoff.p, pool_base, g);
   HL_LAUNCH_CHECK();
+  HL_PROF(P_ROW_DEGREE, 12 * plan.need + 12 * ns, s);
   hipLaunchKernelGGL(k_row_degree, GRID(ns), ns, a.seg_row.p, g);
   HL_LAUNCH_CHECK();
 }
@@ -1692,6 +1716,7 @@ u64 compact_rows(const LevelCsrMut& g, u32* col2, double* w2, DevBuf<u64>& off_t
 
 void twice_m_chunked(const double* k, u32 n, double* partials, double* out, hipStream_t s) {
   u64 nc = (u64(n) + kSumChunk - 1) / kSumChunk;
+  HL_PROF(P_TWICE_M, 8ull * n, s);
   if (nc) hipLaunchKernelGGL(k_chunk_partials, GRID(nc), k, n, partials);
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum_partials, 1, 1, 0, s, partials, nc, out);
@@ -1707,11 +1732,13 @@ void grow_init(u32 n_old, u32 n_new, u32 bf, u32 bg, u32 bs, u32* f, u32* g, u32
 }
 
 void seed(const Rec* d, u64 k, const u32* f, const u32* sm, u8* A, u8* K, hipStream_t s) {
+  HL_PROF(P_SEED, 24 * k, s);
   if (k) hipLaunchKernelGGL(k_seed, GRID(k), d, k, f, sm, A, K);
   HL_LAUNCH_CHECK();
 }
 
 void label_totals_atomic(const u32* lab, const double* k, u32 n, double* tot, hipStream_t s) {
+  HL_PROF(P_LABEL_TOTALS, 20ull * n, s);
   if (n) hipLaunchKernelGGL(k_label_totals_atomic, GRID(n), lab, k, n, tot);
   HL_LAUNCH_CHECK();
 }
@@ -1741,23 +1768,27 @@ static u64 propose_impl(const P& pol, const Csr& g, const u32* verts, u64 cnt, u
   if (cnt == 0) return 0;
   ps.lists.reserve(3 * cnt, s);
   ps.counts.reserve(4, s);
-  ps.degsum.reserve(1, s);
+  ps.degsum.reserve(3, s);
   fill_u32(ps.counts.p, 0, 4, s);
-  HL_CHECK(hipMemsetAsync(ps.degsum.p, 0, sizeof(u64), s));
+  HL_CHECK(hipMemsetAsync(ps.degsum.p, 0, 3 * sizeof(u64), s));
   u32 small = th.small_deg > SMALL_MAX ? SMALL_MAX : th.small_deg;
   u32 wave = th.wave_deg > WAVE_HW / 2 ? WAVE_HW / 2 : th.wave_deg;
   hipLaunchKernelGGL(k_bucket, GRID(cnt), verts, cnt, g.row_len, small, wave, ps.lists.p, ps.counts.p,
                      ps.degsum.p);
   HL_LAUNCH_CHECK();
   u32 counts[4];
-  u64 degsum = 0;
+  u64 ds3[3] = {0, 0, 0};
   HL_CHECK(hipMemcpyAsync(counts, ps.counts.p, 4 * sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipMemcpyAsync(&degsum, ps.degsum.p, sizeof(u64), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipMemcpyAsync(ds3, ps.degsum.p, 3 * sizeof(u64), hipMemcpyDeviceToHost, s));
   HL_CHECK(hipStreamSynchronize(s));
-  if (counts[0])
+  const u64 degsum = ds3[0] + ds3[1] + ds3[2];
+  if (counts[0]) {
+    HL_PROF(P_PROP_SMALL, 20ull * counts[0] + 16 * ds3[0], s);
     hipLaunchKernelGGL(k_prop_small<P>, GRID(counts[0]), pol, g, ps.lists.p, counts[0], verts, cto, cgain);
+  }
   HL_LAUNCH_CHECK();
   if (counts[1]) {
+    HL_PROF(P_PROP_WAVE, 20ull * counts[1] + 16 * ds3[1], s);
     unsigned nb = unsigned(std::min<u64>((counts[1] + 3) / 4, 8192));
     hipLaunchKernelGGL(k_prop_wave<P>, dim3(nb), dim3(256), 0, s, pol, g, ps.lists.p + cnt, counts[1], verts, cto,
                        cgain);
@@ -1772,6 +1803,7 @@ static u64 propose_impl(const P& pol, const Csr& g, const u32* verts, u64 cnt, u
     u64 total = exclusive_scan_u64(ps.bigsize.p, ps.bigoff.p, counts[2], tmp, s);
     ps.hkeys.reserve(total, s);
     ps.hvals.reserve(total, s);
+    HL_PROF(P_PROP_BIG, 20ull * counts[2] + 16 * ds3[2], s);
     hipLaunchKernelGGL(k_prop_big<P>, dim3(counts[2]), dim3(256), 0, s, pol, g, bl, counts[2], verts, ps.bigoff.p,
                        ps.hkeys.p, ps.hvals.p, cto, cgain);
     HL_LAUNCH_CHECK();
@@ -1818,6 +1850,7 @@ u64 decouple_and_apply(const Csr& g, u32* f, const u32* sm, double* tot, u64 com
   }
   int iters = 0;
   while (true) {
+    HL_PROF(P_DECOUPLE, 64 * c, s);
     hipLaunchKernelGGL(k_dec_reset, GRID(c), c, ds.from.p, ds.to.p, ds.minUto.p, ds.minUfrom.p, ds.minAto.p,
                        ds.minAfrom.p);
     HL_LAUNCH_CHECK();
@@ -1843,12 +1876,16 @@ u64 decouple_and_apply(const Csr& g, u32* f, const u32* sm, double* tot, u64 com
     }
   }
   ds.flags.reserve(c, s);
+  {
+  HL_PROF(P_APPLY_MOVES, 21 * c, s);
   hipLaunchKernelGGL(k_apply_moves, GRID(c), c, ds.node.p, ds.from.p, ds.to.p, ds.state.p, g.k, f, tot, B,
                      ds.flags.p);
   HL_LAUNCH_CHECK();
+  }
   DevBuf<u32> acc;
   acc.reserve(c, s);
   const u64 na = select_flagged(ds.flags.p, c, acc.p, tmp, ds.cnt, s);
+  HL_PROF(P_MARK_MOVED, 24 * na, s);
   if (na) hipLaunchKernelGGL(k_mark_moved, GRID(na * 64), g, acc.p, na, ds.node.p, ds.to.p, f, sm, next, K);
   HL_LAUNCH_CHECK();
   return na;
@@ -1883,6 +1920,7 @@ int det_move(const Csr& g, u32* f, const u32* sm, double* tot, u64 tot_cap, u64*
   fill_u32(ds.status.p, 0, 1, s);
   DetMoveArgs a{g, f, sm, tot, tot_cap, tot_len_dev, ds.bm0.p, ds.bm1.p, ds.bm2.p, u32(nw2), B, K, gamma,
                 ds.gkeys.p, ds.gvals.p, ds.gkeys.p + gcap, gcap, ds.status.p, ds.pops.p};
+  HL_PROF(P_DET_MOVE, 0, s);
   hipLaunchKernelGGL(k_det_move, 1, 256, 0, s, a);
   HL_LAUNCH_CHECK();
   u32 st = 0;
@@ -1924,6 +1962,7 @@ void cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_space, RefineScra
   rs.rkeys.reserve(m, s);
   rs.rkeys_s.reserve(m, s);
   rs.newid.reserve(n, s);
+  HL_PROF(P_CC_SPLIT, 16 * m, s);
   hipLaunchKernelGGL(k_uf_init, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p);
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_uf_edges, GRID(m * 64), g, rs.vcc.p, m, sm, rs.parent.p);
@@ -2006,6 +2045,7 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
   rs.wS.reserve(sn, s);
   rs.dpar.reserve(comm_space, s);
   fill_u8(rs.is_cand.p, 0, sn, s);
+  HL_PROF(P_REFINE, 16ull * n, s);
   hipLaunchKernelGGL(k_cand_mark, GRID(ns * 64), g, sl.p, ns, f, sm, rs.is_cand.p);
   HL_LAUNCH_CHECK();
   fill_u32(rs.parentIdx.p, kNone, sn, s);
@@ -2084,6 +2124,7 @@ void refine_det(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, 
   }
   DetRefArgs a{g, f, sm, rs.rsorted.p, nr, rs.sizes.p, rs.dS.p, rs.dpar.p, rs.mhead.p, rs.mnext.p, gamma,
                rs.gk.p, rs.gv.p, rs.gk.p + gcap, gcap};
+  HL_PROF(P_DET_REFINE, 0, s);
   hipLaunchKernelGGL(k_det_refine, 1, 256, 0, s, a);
   HL_LAUNCH_CHECK();
   HL_CHECK(hipStreamSynchronize(s));
@@ -2099,6 +2140,7 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
   as.off64.reserve(nr + 1, s);
   u64 nemit = 0;
   if (nr) {
+    HL_PROF(P_AGG_EMIT, 8 * nr, s);
     hipLaunchKernelGGL(k_agg_count, GRID(nr * 64), g, as.rlist.p, nr, R, as.cnt64.p);
     HL_LAUNCH_CHECK();
     nemit = exclusive_scan_u64(as.cnt64.p, as.off64.p, nr, tmp, s);
@@ -2107,7 +2149,10 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
   as.H.reserve(total + 1, s);
   if (k) hipLaunchKernelGGL(k_lift, GRID(k), dG, k, s_pre, as.H.p);
   HL_LAUNCH_CHECK();
-  if (nr) hipLaunchKernelGGL(k_agg_emit, GRID(nr * 64), g, as.rlist.p, nr, R, s_pre, s_cur, as.off64.p, k, as.H.p);
+  if (nr) {
+    HL_PROF(P_AGG_EMIT, 16 * nemit + 8 * nr, s);
+    hipLaunchKernelGGL(k_agg_emit, GRID(nr * 64), g, as.rlist.p, nr, R, s_pre, s_cur, as.off64.p, k, as.H.p);
+  }
   HL_LAUNCH_CHECK();
   // next node count from s_cur and the old s_pre
   DevBuf<u32> mx;
@@ -2137,6 +2182,7 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
   DevBuf<Rec> comp;
   comp.reserve(nruns, s);
   as.keep.reserve(nruns, s);
+  HL_PROF(P_COMPRESS, 16 * total + 17 * nruns, s);
   hipLaunchKernelGGL(k_compress, GRID(nruns), nruns, as.runs.p, total, as.keys_s.p, as.idx_s.p, as.H.p, comp.p,
                      as.keep.p);
   HL_LAUNCH_CHECK();
@@ -2150,10 +2196,12 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
 }
 
 void def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, hipStream_t s) {
+  HL_PROF(P_DEF_UPDATE, 12ull * n, s);
   if (n) hipLaunchKernelGGL(k_def_pull, GRID(n), B, fp, fq, sp, n);
   HL_LAUNCH_CHECK();
 }
 void def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np, hipStream_t s) {
+  HL_PROF(P_DEF_UPDATE, 5ull * nq, s);
   if (nq) hipLaunchKernelGGL(k_def_push, GRID(nq), Bp, Bq, sq, nq, np);
   HL_LAUNCH_CHECK();
 }
@@ -2177,6 +2225,7 @@ double modularity(const Csr& g, const u32* lab, DevBuf<double>& sig, DevBuf<doub
   fill_f64(sig.p, 0.0, L, s);
   fill_f64(acc.p, 0.0, 2, s);
   label_totals_atomic(lab, g.k, n, sig.p, s);
+  HL_PROF(P_MODULARITY, 0, s);
   hipLaunchKernelGGL(k_intra, blocks_for(n, 256, 4096), 256, 0, s, g, lab, acc.p);
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sq_sum, blocks_for(L, 256, 4096), 256, 0, s, sig.p, L, g.twice_m, acc.p + 1);

@@ -152,6 +152,18 @@ hl_status hl_read_hierarchy(const hl_graph* g, uint32_t* out, uint64_t cap, int3
 hl_status hl_modularity(const hl_graph* g, int32_t which, double* q);
 hl_status hl_modularity_labels(const hl_graph* g, const uint32_t* labels, uint64_t n, double* q);
 
+/* Per-kernel device timing for roofline accounting.  No counterpart in the
+   reference (it profiles out of process with perf/samply, profiling.rs:74-152).
+   hl_profile_enable(g, 1) resets and starts recording HIP events around each
+   kernel family on the handle's stream (on = 2 + slot records that slot only,
+   on = 0 stops); hl_profile_read returns, per slot,
+   the kernel name, launch count, summed device milliseconds and the summed
+   algorithmic HBM bytes of those launches. */
+int32_t hl_profile_slots(void);
+hl_status hl_profile_enable(hl_graph* g, int32_t on);
+hl_status hl_profile_read(const hl_graph* g, int32_t slot, char* name, uint64_t name_cap, uint64_t* launches,
+                          double* ms, uint64_t* bytes);
+
 /* HitLeidenError Display (error.rs:21-26) of the last failing call */
 const char* hl_last_error(const hl_graph* g);
 /* error text of a failed hl_create (no handle yet) */

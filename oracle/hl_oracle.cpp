@@ -30,6 +30,10 @@
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
+#include <parallel/algorithm>
+#define HLO_SORT __gnu_parallel::sort
+#else
+#define HLO_SORT std::sort
 #endif
 
 namespace {
@@ -130,8 +134,8 @@ void apply_delta(Graph& g, const std::vector<Rec>& d, u32 n_new) {
     u32 a = std::min(d[i].u, d[i].v), b = std::max(d[i].u, d[i].v);
     keyed[i] = {(u64(a) << 32) | b, u32(i)};
   }
-  std::stable_sort(keyed.begin(), keyed.end(),
-                   [](const std::pair<u64, u32>& x, const std::pair<u64, u32>& y) { return x.first < y.first; });
+  // (key, index) pairs are unique, so ordering them equals a stable sort by key
+  HLO_SORT(keyed.begin(), keyed.end());
   std::vector<u32> touched;
   size_t i = 0;
   while (i < keyed.size()) {
@@ -727,8 +731,8 @@ std::vector<Rec> inc_aggregation(const Graph& g, const std::vector<Rec>& dG, std
     u32 a = std::min(H[i].u, H[i].v), b = std::max(H[i].u, H[i].v);
     keyed[i] = {(u64(a) << 32) | b, u32(i)};
   }
-  std::stable_sort(keyed.begin(), keyed.end(),
-                   [](const std::pair<u64, u32>& x, const std::pair<u64, u32>& y) { return x.first < y.first; });
+  // (key, index) pairs are unique, so ordering them equals a stable sort by key
+  HLO_SORT(keyed.begin(), keyed.end());
   std::vector<Rec> out;
   size_t i = 0;
   while (i < keyed.size()) {
