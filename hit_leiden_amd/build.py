@@ -18,7 +18,7 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
 ]
-ENGINE_SOURCES = ["hl_kernels.hip", "hl_engine.hip"]
+ENGINE_SOURCES = ["hl_kernels.hip", "hl_move.hip", "hl_engine.hip"]
 
 
 def _run_parallel(cmds):

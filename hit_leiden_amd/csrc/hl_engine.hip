@@ -10,6 +10,7 @@
 
 #include "../../include/hitleiden.h"
 #include "hl_kernels.h"
+#include "hl_move.h"
 
 namespace hl {
 namespace {
@@ -59,6 +60,7 @@ struct Engine {
   DetMoveScratch dms;
   RefineScratch rs;
   AggScratch ag;
+  MoveScratch mvs;
   DevBuf<double> tot, partials, sig, acc;
   DevBuf<u64> tot_len, cnt, offtmp;
   DevBuf<u32> active, cto, mx, lab, lab2, first;
@@ -147,9 +149,8 @@ struct Engine {
       apply_delta_commit(g, lv.pool_used, as, plan, mode == HL_MODE_DETERMINISTIC, s);
       lv.pool_used += plan.need;
     }
-    u64 nc = (u64(lv.n) + kSumChunk - 1) / kSumChunk + 1;
-    partials.reserve(nc, s);
-    twice_m_chunked(lv.k.p, lv.n, partials.p, lv.twice_m.p, s);
+    partials.reserve(twice_m_scratch(lv.n), s);
+    twice_m_tree(lv.k.p, lv.n, partials.p, lv.twice_m.p, s);
     if (st && p < HL_MAX_LEVELS)
       st->bytes_alg += 16 * k + 24 * plan.old_entries + 12 * plan.need + 20 * plan.nseg;
   }
@@ -164,13 +165,11 @@ struct Engine {
   // Algorithm 2 (hit_leiden.rs:1024-1192)
   u32 movement(Level& lv, const Rec* d, u64 k, hl_batch_stats* st, int p) {
     const u32 n = lv.n;
-    fill_u8(lv.A.p, 0, n, s);
-    fill_u8(lv.B.p, 0, n, s);
-    fill_u8(lv.K.p, 0, n, s);
-    seed(d, k, lv.f.p, lv.s_cur.p, lv.A.p, lv.K.p, s);
-    if (k == 0) fill_u8(lv.A.p, 1, n, s);
+    fill4_u8(lv.A.p, lv.B.p, lv.K.p, lv.R.p, 0, n, s);
+    if (k == 0) fill_u8(lv.A.p, 1, n, s);  // empty delta activates every vertex (hit_leiden.rs:1054-1057)
     Csr g = lv.csr();
     if (mode == HL_MODE_DETERMINISTIC) {
+      seed(d, k, lv.f.p, lv.s_cur.p, lv.A.p, lv.K.p, s);
       u64 len = std::max<u64>(n, n ? u64(dmax(lv.f.p, n)) + 1 : 0);
       u64 cap = len + n + 1024;
       tot.reserve(cap, s);
@@ -203,34 +202,22 @@ struct Engine {
     tot.reserve(ccap, s);
     fill_f64(tot.p, 0.0, ccap, s);
     label_totals_atomic(lv.f.p, lv.k.p, n, tot.p, s);
-    active.reserve(n + 1, s);
-    cto.reserve(n + 1, s);
-    cgain.reserve(n + 1, s);
-    next.reserve(n + 1, s);
-    u64 cnt_a = select_flagged(lv.A.p, n, active.p, tmp, cnt, s);
     const int max_rounds = cfg.max_rounds > 0 ? cfg.max_rounds : 20;
-    u32 rounds = 0;
-    while (cnt_a > 0 && int(rounds) < max_rounds) {
-      u64 degsum = propose_moves(g, lv.f.p, tot.p, cfg.gamma, active.p, cnt_a, cto.p, cgain.p, ps, th, tmp, s);
-      fill_u8(next.p, 0, n, s);
-      u64 na = decouple_and_apply(g, lv.f.p, lv.s_cur.p, tot.p, ccap, active.p, cnt_a, cto.p, cgain.p, lv.B.p,
-                                  lv.K.p, next.p, dsc, tmp, s);
-      ++rounds;
-      if (st && p < HL_MAX_LEVELS) {
-        st->evaluated[p] += cnt_a;
-        st->bytes_alg += 20 * cnt_a + 16 * degsum + 20 * na;
-      }
-      if (na == 0) break;
-      cnt_a = select_flagged(next.p, n, active.p, tmp, cnt, s);
+    MoveCtx m{g, lv.f.p, lv.s_cur.p, tot.p, lv.k.p, lv.A.p, lv.B.p, lv.K.p, cfg.gamma, ccap};
+    MoveStats ms_st;
+    const u32 maxd = max_degree(lv);
+    u32 rounds = movement_throughput(m, d, k, maxd, max_rounds, mvs, th, tmp, &ms_st, s);
+    if (st && p < HL_MAX_LEVELS) {
+      st->rounds[p] = rounds;
+      st->evaluated[p] += ms_st.evaluated;
+      st->bytes_alg += 20 * ms_st.evaluated + 20 * ms_st.moved_rounds;
     }
-    if (st && p < HL_MAX_LEVELS) st->rounds[p] = rounds;
     return rounds;
   }
 
   // Algorithm 3 (hit_leiden.rs:1265-1416)
   void refinement(Level& lv, hl_batch_stats* st, int p) {
     const u32 n = lv.n;
-    fill_u8(lv.R.p, 0, n, s);
     if (n == 0) return;
     Csr g = lv.csr();
     const u64 sspace = u64(dmax(lv.s_cur.p, n)) + 1;
@@ -241,7 +228,7 @@ struct Engine {
       refine_det(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, rs, ks, vs, vidx, heads, starts, cnt,
                  max_degree(lv), tmp, s);
     else
-      refine_parallel(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, rs, ps, th, tmp, s);
+      refine_parallel(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, rs, ps, mvs, max_degree(lv), th, tmp, s);
     (void)st;
     (void)p;
   }
@@ -284,17 +271,19 @@ struct Engine {
       lv.iters = it;
       total_iters += it;
       refinement(lv, st, p);
+      u64 c4[4] = {0, 0, 0, 0};
+      count4_u8(st ? lv.A.p : nullptr, st ? lv.B.p : nullptr, st ? lv.K.p : nullptr, lv.R.p, lv.n, cnt, c4, s);
       if (st && p < HL_MAX_LEVELS) {
         st->level_n[p] = lv.n;
         st->delta_len[p] = cur_len;
-        st->active[p] = count_u8(lv.A.p, lv.n, cnt, s);
-        st->moved[p] = count_u8(lv.B.p, lv.n, cnt, s);
-        st->affected[p] = count_u8(lv.K.p, lv.n, cnt, s);
-        st->refined[p] = count_u8(lv.R.p, lv.n, cnt, s);
+        st->active[p] = c4[0];
+        st->moved[p] = c4[1];
+        st->affected[p] = c4[2];
+        st->refined[p] = c4[3];
         st->bytes_alg += 24 * cur_len + 12 * st->moved[p];
       }
       if (p < P - 1) {
-        u64 nR = count_u8(lv.R.p, lv.n, cnt, s);
+        u64 nR = c4[3];
         if (cur_len == 0 && nR == 0) {
           // should_skip_aggregation (hit_leiden.rs:1020-1022)
           u32 a = lv.n ? dmax(lv.s_cur.p, lv.n) : 0;

@@ -32,6 +32,10 @@ void fill_f64(double* p, double v, u64 n, hipStream_t s);
 void iota_u32(u32* p, u32 base, u64 n, hipStream_t s);
 void copy_u8(u8* dst, const u8* src, u64 n, hipStream_t s);
 u64 count_u8(const u8* a, u64 n, DevBuf<u64>& cnt, hipStream_t s);
+// nonzero counts of up to four byte arrays (null entries count 0), one sync
+void count4_u8(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, DevBuf<u64>& cnt, u64 out[4],
+               hipStream_t s);
+void fill4_u8(u8* a0, u8* a1, u8* a2, u8* a3, u8 v, u64 n, hipStream_t s);
 // *out = max(*out, max(a[0..n)))  (out must be initialised)
 void max_u32(const u32* a, u64 n, u32* out, hipStream_t s);
 // indices i with flags[i] != 0, ascending; returns the count (synchronises)
@@ -76,7 +80,9 @@ void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, co
                         hipStream_t s);
 // compact all rows into a fresh pool [col2, w2) with row_cap = row_len; returns live nnz
 u64 compact_rows(const LevelCsrMut& g, u32* col2, double* w2, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s);
-void twice_m_chunked(const double* k, u32 n, double* partials, double* out, hipStream_t s);
+// canonical 2m (pairwise tree); partials needs twice_m_scratch(n) doubles
+u64 twice_m_scratch(u32 n);
+void twice_m_tree(const double* k, u32 n, double* partials, double* out, hipStream_t s);
 void grow_init(u32 n_old, u32 n_new, u32 base_f, u32 base_g, u32 base_s, u32* f, u32* g, u32* s_pre, u32* s_cur,
                u64* row_off, u32* row_len, u32* row_cap, double* k, hipStream_t s);
 
@@ -142,8 +148,10 @@ struct RefineScratch {
 void cc_split(const Csr& g, u32* s, const u8* K, u8* R, u64 s_space, RefineScratch& rs, Temp& tmp, hipStream_t st);
 bool is_identity(const u32* s, u32 n, RefineScratch& rs, hipStream_t st);
 // throughput refinement (+connected-targets rule) over R
+struct MoveScratch;
 void refine_parallel(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space,
-                     RefineScratch& rs, ProposeScratch& ps, const Thresholds& th, Temp& tmp, hipStream_t st);
+                     RefineScratch& rs, ProposeScratch& ps, MoveScratch& mvs, u32 maxdeg, const Thresholds& th,
+                     Temp& tmp, hipStream_t st);
 // deterministic refinement (hit_leiden.rs:1359-1413)
 void refine_det(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space, RefineScratch& rs,
                 DevBuf<u32>& ks, DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,

@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "hl_kernels.h"
+#include "hl_move.h"
 
 namespace hl {
 thread_local Profiler* g_prof = nullptr;
@@ -123,17 +124,45 @@ __global__ void k_iota_u32(u32* p, u32 base, u64 n) {
 __global__ void k_copy_u8(u8* d, const u8* s, u64 n) {
   for (u64 i = gtid(); i < n; i += gstride()) d[i] = s[i];
 }
-__global__ void k_max_u32(const u32* a, u64 n, u32* out) {
+// grid-stride max with one atomic per workgroup
+__global__ __launch_bounds__(256) void k_max_u32(const u32* a, u64 n, u32* out) {
+  __shared__ u32 sm[4];
   u32 m = 0;
   for (u64 i = gtid(); i < n; i += gstride()) m = max(m, a[i]);
   for (int o = 32; o > 0; o >>= 1) m = max(m, (u32)__shfl_xor((int)m, o, 64));
-  if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = max(max(sm[0], sm[1]), max(sm[2], sm[3]));
+    if (m) atomicMax(out, m);
+  }
 }
-__global__ void k_count_u8(const u8* a, u64 n, unsigned long long* out) {
-  unsigned long long c = 0;
-  for (u64 i = gtid(); i < n; i += gstride()) c += (a[i] != 0);
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+// counts of nonzero bytes in up to four arrays, one atomic per workgroup and array
+__global__ __launch_bounds__(256) void k_count4(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n,
+                                                 unsigned long long* out) {
+  __shared__ unsigned long long sm[4][4];
+  const u8* arr[4] = {a0, a1, a2, a3};
+  unsigned long long c[4] = {0, 0, 0, 0};
+  for (u64 i = gtid(); i < n; i += gstride())
+    for (int j = 0; j < 4; ++j)
+      if (arr[j]) c[j] += (arr[j][i] != 0);
+  for (int j = 0; j < 4; ++j) {
+    for (int o = 32; o > 0; o >>= 1) c[j] += __shfl_xor(c[j], o, 64);
+    if ((threadIdx.x & 63) == 0) sm[j][threadIdx.x >> 6] = c[j];
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long t = sm[threadIdx.x][0] + sm[threadIdx.x][1] + sm[threadIdx.x][2] + sm[threadIdx.x][3];
+    if (t) atomicAdd(&out[threadIdx.x], t);
+  }
+}
+__global__ void k_fill4_u8(u8* a0, u8* a1, u8* a2, u8* a3, u8 v, u64 n) {
+  for (u64 i = gtid(); i < n; i += gstride()) {
+    if (a0) a0[i] = v;
+    if (a1) a1[i] = v;
+    if (a2) a2[i] = v;
+    if (a3) a3[i] = v;
+  }
 }
 __global__ void k_scan_tail(const u64* in, u64* out, u64 n) {
   out[n] = n ? out[n - 1] + in[n - 1] : 0;
@@ -334,21 +363,20 @@ __global__ void k_len_u64(const u32* len, u32 n, u64* out) {
   for (u64 i = gtid(); i < n; i += gstride()) out[i] = len[i];
 }
 
-// canonical 2m: chunks of 4096 vertices summed sequentially, then the chunk
-// partials summed sequentially in chunk order
-__global__ void k_chunk_partials(const double* k, u32 n, double* part) {
-  u64 c = gtid();
-  u64 nc = (u64(n) + kSumChunk - 1) / kSumChunk;
-  if (c >= nc) return;
-  u64 b = c * kSumChunk, e = umin64(n, b + kSumChunk);
-  double p = 0.0;
-  for (u64 i = b; i < e; ++i) p += k[i];
-  part[c] = p;
-}
-__global__ void k_sum_partials(const double* part, u64 nc, double* out) {
-  double t = 0.0;
-  for (u64 i = 0; i < nc; ++i) t += part[i];
-  *out = t;
+// canonical 2m: pairwise (halving) tree over k zero-padded to a power of two.
+// Each workgroup reduces one aligned 2048-element block with the same tree
+// (element i absorbs i+s for s = 1, 2, 4, ...); block results feed the next pass.
+constexpr u32 kTreeBlock = 2048;
+__global__ __launch_bounds__(256) void k_tree_pass(const double* in, u64 n, double* out) {
+  __shared__ double t[kTreeBlock];
+  const u64 base = u64(blockIdx.x) * kTreeBlock;
+  for (u32 i = threadIdx.x; i < kTreeBlock; i += 256) t[i] = (base + i < n) ? in[base + i] : 0.0;
+  __syncthreads();
+  for (u32 s = 1; s < kTreeBlock; s <<= 1) {
+    for (u32 i = threadIdx.x * 2 * s; i < kTreeBlock; i += 256 * 2 * s) t[i] += t[i + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = t[0];
 }
 
 __global__ void k_grow_init(u32 n_old, u32 n_new, u32 bf, u32 bg, u32 bs, u32* f, u32* g, u32* s_pre, u32* s_cur,
@@ -726,6 +754,55 @@ __global__ void k_dec_decide(u64 c, const u32* from, const u32* to, u32* state, 
   for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
   if ((threadIdx.x & 63) == 0 && local) atomicAdd(undec, local);
 }
+// All priority rounds inside one workgroup (small candidate sets): the same
+// reset / mark / decide steps separated by workgroup barriers, no host round trip.
+__global__ __launch_bounds__(1024) void k_dec_block(u64 c, const u32* from, const u32* to, u32* state, u32* minUto,
+                                                     u32* minUfrom, u32* minAto, u32* minAfrom, u32* undec_out) {
+  __shared__ u32 und;
+  for (int iter = 0; iter < 1 << 20; ++iter) {
+    for (u64 p = threadIdx.x; p < c; p += blockDim.x) {
+      u32 f = from[p], t = to[p];
+      minUto[f] = kNone; minUto[t] = kNone;
+      minUfrom[f] = kNone; minUfrom[t] = kNone;
+      minAto[f] = kNone; minAto[t] = kNone;
+      minAfrom[f] = kNone; minAfrom[t] = kNone;
+    }
+    if (threadIdx.x == 0) und = 0;
+    __threadfence();
+    __syncthreads();
+    for (u64 p = threadIdx.x; p < c; p += blockDim.x) {
+      u32 st = state[p];
+      if (st == kDecUndec) {
+        atomicMin(&minUto[to[p]], u32(p));
+        atomicMin(&minUfrom[from[p]], u32(p));
+      } else if (st == kDecAcc) {
+        atomicMin(&minAto[to[p]], u32(p));
+        atomicMin(&minAfrom[from[p]], u32(p));
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    u32 local = 0;
+    for (u64 p = threadIdx.x; p < c; p += blockDim.x) {
+      if (state[p] != kDecUndec) continue;
+      u32 f = from[p], t = to[p], pp = u32(p);
+      if (atomicAdd(&minAto[f], 0u) < pp || atomicAdd(&minAfrom[t], 0u) < pp)
+        state[p] = kDecRej;
+      else if (atomicAdd(&minUto[f], 0u) < pp || atomicAdd(&minUfrom[t], 0u) < pp)
+        ++local;
+      else
+        state[p] = kDecAcc;
+    }
+    if (local) atomicAdd(&und, local);
+    __threadfence();
+    __syncthreads();
+    const u32 u = und;
+    __syncthreads();
+    if (u == 0) break;
+  }
+  if (threadIdx.x == 0) *undec_out = und;
+}
+
 // sequential completion of the greedy for pathological dependency chains
 __global__ void k_dec_sequential(u64 c, const u32* from, const u32* to, u32* state, u8* emit_acc) {
   if (gtid() != 0) return;
@@ -1150,16 +1227,14 @@ __global__ void k_parent_idx(const u32* s, u32 n, const u8* is_cand, u32* pidx) 
 // w(S, parent(S)) over all members (hit_leiden.rs:1617-1629), self-loops twice
 __global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u32* s, const u8* is_cand,
                                              const u32* pidx, double* wS) {
-  const u32 lane = threadIdx.x & 63;
-  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
-  for (u64 v = wave; v < n; v += nwaves) {
+  for (u64 v = gtid(); v < n; v += gstride()) {
     u32 sv = s[v];
     if (!is_cand[sv]) continue;
     u32 C = f[pidx[sv]];
     u64 off = g.row_off[v];
     u32 deg = g.row_len[v];
     double acc = 0.0;
-    for (u32 j = lane; j < deg; j += 64) {
+    for (u32 j = 0; j < deg; ++j) {
       u32 u = g.col[off + j];
       if (f[u] == C) {
         double w = g.w[off + j];
@@ -1167,8 +1242,7 @@ __global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u3
         if (u == v) acc += w;
       }
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (lane == 0 && acc != 0.0) atomicAdd(&wS[sv], acc);
+    if (acc != 0.0) atomicAdd(&wS[sv], acc);
   }
 }
 __global__ void k_blocked(u64 sn, const u8* is_cand, const double* dS, const double* wS, const u32* pidx,
@@ -1532,20 +1606,32 @@ void copy_u8(u8* d, const u8* src, u64 n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_copy_u8, GRID(n), d, src, n);
   HL_LAUNCH_CHECK();
 }
-u64 count_u8(const u8* a, u64 n, DevBuf<u64>& cnt, hipStream_t s) {
-  cnt.reserve(1, s);
-  HL_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(u64), s));
-  HL_PROF(P_REDUCE, n, s);
-  if (n) hipLaunchKernelGGL(k_count_u8, blocks_for(n, 256, 4096), 256, 0, s, a, n, (unsigned long long*)cnt.p);
+void count4_u8(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, DevBuf<u64>& cnt, u64 out[4],
+               hipStream_t s) {
+  cnt.reserve(4, s);
+  HL_CHECK(hipMemsetAsync(cnt.p, 0, 4 * sizeof(u64), s));
+  if (n) {
+    HL_PROF(P_REDUCE, 4 * n, s);
+    hipLaunchKernelGGL(k_count4, blocks_for(n, 256, 1024), 256, 0, s, a0, a1, a2, a3, n,
+                       (unsigned long long*)cnt.p);
+  }
   HL_LAUNCH_CHECK();
-  u64 h = 0;
-  HL_CHECK(hipMemcpyAsync(&h, cnt.p, sizeof(u64), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipMemcpyAsync(out, cnt.p, 4 * sizeof(u64), hipMemcpyDeviceToHost, s));
   HL_CHECK(hipStreamSynchronize(s));
-  return h;
+}
+u64 count_u8(const u8* a, u64 n, DevBuf<u64>& cnt, hipStream_t s) {
+  u64 out[4];
+  count4_u8(a, nullptr, nullptr, nullptr, n, cnt, out, s);
+  return out[0];
+}
+void fill4_u8(u8* a0, u8* a1, u8* a2, u8* a3, u8 v, u64 n, hipStream_t s) {
+  HL_PROF(P_FILL, 4 * n, s);
+  if (n) hipLaunchKernelGGL(k_fill4_u8, GRID(n), a0, a1, a2, a3, v, n);
+  HL_LAUNCH_CHECK();
 }
 void max_u32(const u32* a, u64 n, u32* out, hipStream_t s) {
   HL_PROF(P_REDUCE, 4 * n, s);
-  if (n) hipLaunchKernelGGL(k_max_u32, GRID(n), a, n, out);
+  if (n) hipLaunchKernelGGL(k_max_u32, blocks_for(n, 256, 1024), 256, 0, s, a, n, out);
   HL_LAUNCH_CHECK();
 }
 
@@ -1714,13 +1800,33 @@ u64 compact_rows(const LevelCsrMut& g, u32* col2, double* w2, DevBuf<u64>& off_t
   return total;
 }
 
-void twice_m_chunked(const double* k, u32 n, double* partials, double* out, hipStream_t s) {
-  u64 nc = (u64(n) + kSumChunk - 1) / kSumChunk;
+u64 twice_m_scratch(u32 n) {
+  u64 nb = (u64(n) + kTreeBlock - 1) / kTreeBlock;
+  return 2 * nb + 2;
+}
+
+void twice_m_tree(const double* k, u32 n, double* partials, double* out, hipStream_t s) {
   HL_PROF(P_TWICE_M, 8ull * n, s);
-  if (nc) hipLaunchKernelGGL(k_chunk_partials, GRID(nc), k, n, partials);
-  HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_partials, 1, 1, 0, s, partials, nc, out);
-  HL_LAUNCH_CHECK();
+  if (n == 0) {
+    hipLaunchKernelGGL(k_fill_f64, 1, 64, 0, s, out, 0.0, u64(1));
+    HL_LAUNCH_CHECK();
+    return;
+  }
+  const u64 nb1 = (u64(n) + kTreeBlock - 1) / kTreeBlock;
+  double* bufs[2] = {partials, partials + nb1 + 1};
+  const double* in = k;
+  u64 cnt = n;
+  int which = 0;
+  while (true) {
+    u64 nb = (cnt + kTreeBlock - 1) / kTreeBlock;
+    double* o = (nb == 1) ? out : bufs[which];
+    hipLaunchKernelGGL(k_tree_pass, dim3(unsigned(nb)), dim3(256), 0, s, in, cnt, o);
+    HL_LAUNCH_CHECK();
+    if (nb == 1) break;
+    in = o;
+    cnt = nb;
+    which ^= 1;
+  }
 }
 
 void grow_init(u32 n_old, u32 n_new, u32 bf, u32 bg, u32 bs, u32* f, u32* g, u32* s_pre, u32* s_cur, u64* row_off,
@@ -1849,7 +1955,13 @@ u64 decouple_and_apply(const Csr& g, u32* f, const u32* sm, double* tot, u64 com
     ds.minAfrom.reserve(comm_cap, s);
   }
   int iters = 0;
-  while (true) {
+  if (c <= 16384) {
+    HL_PROF(P_DECOUPLE, 64 * c, s);
+    hipLaunchKernelGGL(k_dec_block, 1, 1024, 0, s, c, ds.from.p, ds.to.p, ds.state.p, ds.minUto.p, ds.minUfrom.p,
+                       ds.minAto.p, ds.minAfrom.p, ds.undec.p);
+    HL_LAUNCH_CHECK();
+  }
+  while (c > 16384) {
     HL_PROF(P_DECOUPLE, 64 * c, s);
     hipLaunchKernelGGL(k_dec_reset, GRID(c), c, ds.from.p, ds.to.p, ds.minUto.p, ds.minUfrom.p, ds.minAto.p,
                        ds.minAfrom.p);
@@ -2014,7 +2126,8 @@ static u64 smax_plus1(const u32* sm, u32 n, RefineScratch& rs, hipStream_t s) {
 }
 
 void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space,
-                     RefineScratch& rs, ProposeScratch& ps, const Thresholds& th, Temp& tmp, hipStream_t s) {
+                     RefineScratch& rs, ProposeScratch& ps, MoveScratch& mvs, u32 maxdeg, const Thresholds& th,
+                     Temp& tmp, hipStream_t s) {
   const u32 n = g.n;
   if (n == 0) return;
   const u64 sn = smax_plus1(sm, n, rs, s);
@@ -2052,29 +2165,17 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
   hipLaunchKernelGGL(k_parent_idx, GRID(n), sm, n, rs.is_cand.p, rs.parentIdx.p);
   HL_LAUNCH_CHECK();
   fill_f64(rs.wS.p, 0.0, sn, s);
-  hipLaunchKernelGGL(k_ws, GRID(u64(n) * 64), g, n, f, sm, rs.is_cand.p, rs.parentIdx.p, rs.wS.p);
+  hipLaunchKernelGGL(k_ws, GRID(n), g, n, f, sm, rs.is_cand.p, rs.parentIdx.p, rs.wS.p);
   HL_LAUNCH_CHECK();
   fill_f64(rs.dpar.p, 0.0, comm_space, s);
   label_totals_atomic(f, g.k, n, rs.dpar.p, s);
   hipLaunchKernelGGL(k_blocked, GRID(sn), sn, rs.is_cand.p, rs.dS.p, rs.wS.p, rs.parentIdx.p, f, rs.dpar.p,
                      g.twice_m, gamma, rs.blocked.p);
   HL_LAUNCH_CHECK();
-  // proposals against the snapshot
-  rs.prop.reserve(ns, s);
-  DevBuf<double> pg;
-  pg.reserve(ns, s);
-  DevBuf<u32> idx;
-  idx.reserve(ns, s);
-  iota_u32(idx.p, 0, ns, s);
-  RefinePolicy pol{f, sm, rs.dS.p, rs.blocked.p, gamma};
-  // propose over sl (verts = sl, list = iota)
-  propose_impl(pol, g, sl.p, ns, rs.prop.p, pg.p, ps, th, tmp, s);
-  rs.target.reserve(sn, s);
-  fill_u8(rs.target.p, 0, sn, s);
-  hipLaunchKernelGGL(k_targets, GRID(ns), sl.p, ns, rs.prop.p, rs.target.p);
-  HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_apply_refine, GRID(ns), sl.p, ns, rs.prop.p, rs.target.p, sm);
-  HL_LAUNCH_CHECK();
+  // proposals against the snapshot, then the connected-targets rule
+  (void)ps;
+  (void)tmp;
+  refine_proposals(g, f, sm, rs.dS.p, rs.blocked.p, gamma, sl.p, ns, sn, mvs, th, maxdeg, s);
   HL_CHECK(hipStreamSynchronize(s));
 }
 

@@ -1,0 +1,106 @@
+// hl_move.h -- throughput-mode movement and refinement proposals with
+// device-side frontier lists (hl_move.hip).
+#pragma once
+#include "hl_kernels.h"
+
+namespace hl {
+
+constexpr u32 kDecSmall = 4096;  // candidates decoupled inside one workgroup
+
+// candidate list: (node, target, gain), count on the device
+struct CandOut {
+  u32* node;
+  u32* to;
+  double* gain;
+  u32* cnt;
+};
+
+// what a movement round reads and writes at one level
+struct MoveCtx {
+  Csr g;
+  u32* f;
+  const u32* s;
+  double* tot;  // community totals (sized comm_cap)
+  const double* k;
+  u8* A;
+  u8* B;
+  u8* K;
+  double gamma;
+  u64 comm_cap;
+};
+
+// throughput movement (parallel_frontier.rs:155-236): self-loops skipped,
+// candidates in ascending id, empty community id = n + v
+struct MovePolicyT {
+  const u32* f;
+  const double* tot;
+  double gamma;
+  u32 n;
+  static constexpr bool kEmpty = true;
+  __device__ bool accept(u32 v, u32 u, u32& key, int& mult) const {
+    if (u == v) return false;
+    key = f[u];
+    mult = 1;
+    return true;
+  }
+  __device__ u32 cur_key(u32 v) const { return f[v]; }
+  __device__ double cur_total(u32 c) const { return tot[c]; }
+  __device__ bool ok(u32) const { return true; }
+  __device__ double cand_total(u32 c) const { return tot[c]; }
+  __device__ u32 empty_id(u32 v) const { return n + v; }
+};
+
+// throughput refinement (try_refinement_move, throughput.rs:291-359):
+// same-community neighbours keyed by subcommunity, self-loop entries twice,
+// blocked subcommunities skipped, no empty candidate
+struct RefinePolicyT {
+  const u32* f;
+  const u32* s;
+  const double* dS;
+  const u8* blocked;
+  double gamma;
+  static constexpr bool kEmpty = false;
+  __device__ bool accept(u32 v, u32 u, u32& key, int& mult) const {
+    if (f[u] != f[v]) return false;
+    key = s[u];
+    mult = (u == v) ? 2 : 1;
+    return true;
+  }
+  __device__ u32 cur_key(u32 v) const { return s[v]; }
+  __device__ double cur_total(u32 c) const { return dS[c]; }
+  __device__ bool ok(u32 c) const { return !blocked[c]; }
+  __device__ double cand_total(u32 c) const { return dS[c]; }
+  __device__ u32 empty_id(u32) const { return kNone; }
+};
+
+struct MoveScratch {
+  DevBuf<u32> mark, list_a, list_b, blists, cnode, cto, counters;
+  DevBuf<double> cgain;
+  DevBuf<u32> minUto, minUfrom, minAto, minAfrom;
+  DevBuf<u32> tk;
+  DevBuf<double> tv;
+  DevBuf<u32> k32, k32b, v32, o1, o2, node, from, to, state, acc;
+  DevBuf<u64> k64, k64b;
+  DevBuf<u8> target;
+  size_t target_cap = 0;
+  u32 epoch = 0;
+  void ensure(u64 n, u64 comm_cap, hipStream_t s);
+  u32 next_epoch(hipStream_t s);
+};
+
+struct MoveStats {
+  u64 active = 0, evaluated = 0, moved_rounds = 0;
+};
+
+// Synchronous rounds (<= max_rounds) of the throughput movement at one level;
+// seeds A/K from the delta (hit_leiden.rs:1037-1057).  Returns the round count.
+u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int max_rounds, MoveScratch& ms,
+                        const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s);
+
+// Snapshot proposals of the singleton vertices `singles` plus the
+// connected-targets rule; updates s in place.
+void refine_proposals(const Csr& g, const u32* f, u32* s, const double* dS, const u8* blocked, double gamma,
+                      const u32* singles, u64 ns, u64 sspace, MoveScratch& ms, const Thresholds& th, u32 maxdeg,
+                      hipStream_t st);
+
+}  // namespace hl

@@ -1,0 +1,875 @@
+// hl_move.hip -- throughput-mode local moving with device-side frontiers.
+//
+// One round (inc_movement_parallel, throughput.rs:80-202):
+//   k_bucket2      split the active list by degree (wave-aggregated appends)
+//   k_prop2_*      per-vertex argmax over neighbour communities (parallel_frontier.rs:155-236);
+//                  a positive proposal is appended to the candidate list
+//   k_dec_small    one workgroup: sort candidates by (gain desc, node asc),
+//                  decoupled priority rounds (throughput.rs:204-236), apply the
+//                  accepted moves, mark next-active neighbours and K
+//                  (throughput.rs:182-199) -- no host round trip inside a round
+// Large candidate sets (the initial pass) take the multi-kernel path.
+// Lists are deduplicated with an epoch-stamped mark array instead of O(n) flags.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "hl_move.h"
+
+namespace hl {
+namespace {
+
+constexpr u32 SMALL_MAX2 = 16;
+constexpr u32 WAVE_HW2 = 1024;
+constexpr u32 kUndec = 0, kAcc = 1, kRej = 2;
+
+__device__ __forceinline__ u64 gtid() { return u64(blockIdx.x) * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ u64 gstride() { return u64(gridDim.x) * blockDim.x; }
+__device__ __forceinline__ u32 hash32(u32 x) { return x * 2654435761u; }
+
+__device__ __forceinline__ double move_gain(double wc, double wcur, double t, double gamma, double k, double sc,
+                                            double scand) {
+  return (wc - wcur) / t + gamma * k * (sc - k - scand) / (t * t);
+}
+__device__ __forceinline__ double empty_gain(double wcur, double t, double gamma, double k, double sc) {
+  return -wcur / t + gamma * k * (sc - k) / (t * t);
+}
+__device__ __forceinline__ void better(double& g, u32& k, double og, u32 ok) {
+  if (og > g || (og == g && ok < k)) {
+    g = og;
+    k = ok;
+  }
+}
+__device__ __forceinline__ void wave_argmax(double& g, u32& k) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double og = __shfl_xor(g, o, 64);
+    u32 ok = __shfl_xor(k, o, 64);
+    better(g, k, og, ok);
+  }
+}
+__device__ void block_argmax(double& g, u32& k) {
+  __shared__ double sg[16];
+  __shared__ u32 sk[16];
+  wave_argmax(g, k);
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    sg[wv] = g;
+    sk[wv] = k;
+  }
+  __syncthreads();
+  g = sg[0];
+  k = sk[0];
+  for (u32 i = 1; i < nw; ++i) better(g, k, sg[i], sk[i]);
+  __syncthreads();
+}
+
+// Append from the lanes that want to; one atomic per wavefront.
+__device__ __forceinline__ u32 wave_append(bool want, u32* cnt) {
+  unsigned long long m = __ballot(want);
+  if (m == 0) return kNone;
+  const u32 lane = threadIdx.x & 63;
+  const u32 leader = u32(__ffsll((long long)m) - 1);
+  u32 base = 0;
+  if (lane == leader) base = atomicAdd(cnt, u32(__popcll(m)));
+  base = __shfl(base, leader, 64);
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  return want ? base + u32(__popcll(m & below)) : kNone;
+}
+// load that observes stores made earlier in the same kernel by other workgroup threads
+__device__ __forceinline__ u32 ld_agent(const u32* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Add v to a list once per epoch.
+__device__ __forceinline__ bool claim(u32* mark, u32 v, u32 epoch) { return atomicExch(&mark[v], epoch) != epoch; }
+
+__device__ __forceinline__ u32 tab_insert(u32* K, u32 cap, u32 key) {
+  u32 h = hash32(key) & (cap - 1);
+  while (true) {
+    u32 old = atomicCAS(&K[h], kNone, key);
+    if (old == kNone || old == key) return h;
+    h = (h + 1) & (cap - 1);
+  }
+}
+__device__ __forceinline__ double tab_lookup(const u32* K, const double* V, u32 cap, u32 key) {
+  u32 h = hash32(key) & (cap - 1);
+  while (true) {
+    u32 x = K[h];
+    if (x == key) return V[h];
+    if (x == kNone) return 0.0;
+    h = (h + 1) & (cap - 1);
+  }
+}
+
+// ------------------------------------------------------------------ seeding
+__global__ void k_seed2(const Rec* d, u64 k, const u32* f, const u32* s, u8* A, u8* K, u32* mark, u32 epoch,
+                        u32* list, u32* cnt) {
+  for (u64 i = gtid(); i < k; i += gstride()) {
+    const Rec r = d[i];
+    const bool same_f = f[r.u] == f[r.v];
+    const bool act = (r.w > 0.0 && !same_f) || (r.w < 0.0 && same_f);
+    const bool kk = s[r.u] == s[r.v];
+    bool cu = act && claim(mark, r.u, epoch);
+    bool cv = act && r.v != r.u && claim(mark, r.v, epoch);
+    if (act) {
+      A[r.u] = 1;
+      A[r.v] = 1;
+    }
+    if (kk) {
+      K[r.u] = 1;
+      K[r.v] = 1;
+    }
+    u32 pu = wave_append(cu, cnt);
+    if (cu) list[pu] = r.u;
+    u32 pv = wave_append(cv, cnt);
+    if (cv) list[pv] = r.v;
+  }
+}
+__global__ void k_iota_list(u32* list, u32 n, u32* cnt) {
+  for (u64 i = gtid(); i < n; i += gstride()) list[i] = u32(i);
+  if (gtid() == 0) *cnt = n;
+}
+
+// ------------------------------------------------------------------ buckets
+__global__ void k_bucket2(const u32* verts, const u32* cnt_dev, const u32* row_len, u32 small_deg, u32 wave_deg,
+                          u32* lists, u64 cap, u32* counts, unsigned long long* degsum) {
+  const u32 cnt = *cnt_dev;
+  unsigned long long local[3] = {0, 0, 0};
+  for (u64 base = u64(blockIdx.x) * blockDim.x; base < cnt; base += gstride()) {
+    u64 i = base + threadIdx.x;
+    u32 b = 3;
+    u32 v = 0;
+    if (i < cnt) {
+      v = verts[i];
+      u32 d = row_len[v];
+      b = d <= small_deg ? 0 : (d <= wave_deg ? 1 : 2);
+      local[b] += d;
+    }
+    for (u32 q = 0; q < 3; ++q) {
+      u32 p = wave_append(b == q, &counts[q]);
+      if (b == q) lists[q * cap + p] = v;
+    }
+  }
+  for (int q = 0; q < 3; ++q) {
+    unsigned long long x = local[q];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(&degsum[q], x);
+  }
+}
+
+// ------------------------------------------------------------------ proposals
+template <class P>
+__device__ __forceinline__ void finish_choice(const P& pol, u32 v, u32 cur, double wcur, double kv, double t,
+                                              double& bg, u32& best) {
+  if constexpr (P::kEmpty) {
+    double eg = empty_gain(wcur, t, pol.gamma, kv, pol.cur_total(cur));
+    if (eg > bg) {
+      bg = eg;
+      best = pol.empty_id(v);
+    }
+  }
+}
+
+// one thread per vertex (deg <= SMALL_MAX2), row-order sums in a private LDS slice
+template <class P>
+__global__ __launch_bounds__(256) void k_prop2_small(P pol, Csr g, const u32* list, const u32* cnt_dev,
+                                                      CandOut out) {
+  __shared__ u32 sk[SMALL_MAX2 * 256];
+  __shared__ double sv[SMALL_MAX2 * 256];
+  const u32 cnt = *cnt_dev;
+  const u32 t = threadIdx.x;
+  const double tt = *g.twice_m;
+  for (u64 base = u64(blockIdx.x) * 256; base < cnt; base += gstride()) {
+    const u64 id = base + t;
+    double bg = 0.0;
+    u32 best = kNone, v = 0;
+    if (id < cnt) {
+      v = list[id];
+      const u64 off = g.row_off[v];
+      const u32 deg = g.row_len[v];
+      const u32 cur = pol.cur_key(v);
+      int nk = 0;
+      for (u32 j = 0; j < deg; ++j) {
+        u32 u = g.col[off + j];
+        double w = g.w[off + j];
+        u32 key;
+        int mult;
+        if (!pol.accept(v, u, key, mult)) continue;
+        int q = 0;
+        while (q < nk && sk[q * 256 + t] != key) ++q;
+        if (q == nk) {
+          sk[q * 256 + t] = key;
+          sv[q * 256 + t] = 0.0;
+          ++nk;
+        }
+        sv[q * 256 + t] += w;
+        if (mult == 2) sv[q * 256 + t] += w;
+      }
+      double wcur = 0.0;
+      for (int q = 0; q < nk; ++q)
+        if (sk[q * 256 + t] == cur) wcur = sv[q * 256 + t];
+      const double kv = g.k[v], sc = pol.cur_total(cur);
+      for (int q = 0; q < nk; ++q) {
+        u32 key = sk[q * 256 + t];
+        if (key == cur || !pol.ok(key)) continue;
+        double gn = move_gain(sv[q * 256 + t], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+        if (gn > 0.0) better(bg, best, gn, key);
+      }
+      finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
+    }
+    const bool want = bg > 0.0;
+    u32 p = wave_append(want, out.cnt);
+    if (want) {
+      out.node[p] = v;
+      out.to[p] = best;
+      out.gain[p] = bg;
+    }
+  }
+}
+
+// one wavefront per vertex, LDS open-addressing table of WAVE_HW2 slots
+template <class P>
+__global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* list, const u32* cnt_dev, CandOut out) {
+  __shared__ u32 hk[4 * WAVE_HW2];
+  __shared__ double hv[4 * WAVE_HW2];
+  const u32 cnt = *cnt_dev;
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  u32* K = hk + wv * WAVE_HW2;
+  double* V = hv + wv * WAVE_HW2;
+  for (u32 q = lane; q < WAVE_HW2; q += 64) {
+    K[q] = kNone;
+    V[q] = 0.0;
+  }
+  __syncthreads();
+  const double tt = *g.twice_m;
+  for (u64 base = u64(blockIdx.x) * 4; base < cnt; base += u64(gridDim.x) * 4) {
+    const u64 id = base + wv;
+    const bool has = id < cnt;
+    u32 v = 0, cur = 0;
+    if (has) {
+      v = list[id];
+      const u64 off = g.row_off[v];
+      const u32 deg = g.row_len[v];
+      cur = pol.cur_key(v);
+      for (u32 j = lane; j < deg; j += 64) {
+        u32 u = g.col[off + j];
+        u32 key;
+        int mult;
+        if (!pol.accept(v, u, key, mult)) continue;
+        double w = g.w[off + j];
+        u32 h = tab_insert(K, WAVE_HW2, key);
+        atomicAdd(&V[h], w);
+        if (mult == 2) atomicAdd(&V[h], w);
+      }
+    }
+    __syncthreads();
+    if (has) {
+      const double wcur = tab_lookup(K, V, WAVE_HW2, cur);
+      const double kv = g.k[v], sc = pol.cur_total(cur);
+      double bg = 0.0;
+      u32 best = kNone;
+      for (u32 q = lane; q < WAVE_HW2; q += 64) {
+        u32 key = K[q];
+        if (key == kNone || key == cur || !pol.ok(key)) continue;
+        double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+        if (gn > 0.0) better(bg, best, gn, key);
+      }
+      wave_argmax(bg, best);
+      if (lane == 0) {
+        finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
+        if (bg > 0.0) {
+          u32 p = atomicAdd(out.cnt, 1u);
+          out.node[p] = v;
+          out.to[p] = best;
+          out.gain[p] = bg;
+        }
+      }
+    }
+    __syncthreads();
+    if (has)
+      for (u32 q = lane; q < WAVE_HW2; q += 64) {
+        K[q] = kNone;
+        V[q] = 0.0;
+      }
+    __syncthreads();
+  }
+}
+
+// one workgroup per vertex (hubs); each resident workgroup owns a table of tcap slots
+template <class P>
+__global__ __launch_bounds__(256) void k_prop2_big(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* TK,
+                                                    double* TV, u32 tcap, CandOut out) {
+  const u32 cnt = *cnt_dev;
+  u32* K = TK + u64(blockIdx.x) * tcap;
+  double* V = TV + u64(blockIdx.x) * tcap;
+  const double tt = *g.twice_m;
+  for (u32 id = blockIdx.x; id < cnt; id += gridDim.x) {
+    const u32 v = list[id];
+    const u64 off = g.row_off[v];
+    const u32 deg = g.row_len[v];
+    const u32 cur = pol.cur_key(v);
+    u32 cap = 64;
+    while (cap < 2 * deg) cap <<= 1;
+    for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
+      K[q] = kNone;
+      V[q] = 0.0;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (u32 j = threadIdx.x; j < deg; j += blockDim.x) {
+      u32 u = g.col[off + j];
+      u32 key;
+      int mult;
+      if (!pol.accept(v, u, key, mult)) continue;
+      double w = g.w[off + j];
+      u32 h = tab_insert(K, cap, key);
+      atomicAdd(&V[h], w);
+      if (mult == 2) atomicAdd(&V[h], w);
+    }
+    __threadfence();
+    __syncthreads();
+    const double wcur = tab_lookup(K, V, cap, cur);
+    const double kv = g.k[v], sc = pol.cur_total(cur);
+    double bg = 0.0;
+    u32 best = kNone;
+    for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
+      u32 key = K[q];
+      if (key == kNone || key == cur || !pol.ok(key)) continue;
+      double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+      if (gn > 0.0) better(bg, best, gn, key);
+    }
+    block_argmax(bg, best);
+    if (threadIdx.x == 0) {
+      finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
+      if (bg > 0.0) {
+        u32 p = atomicAdd(out.cnt, 1u);
+        out.node[p] = v;
+        out.to[p] = best;
+        out.gain[p] = bg;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ decoupling (one workgroup)
+// (gain desc, node asc) as a sortable pair: key = ~bits(gain) (gains are > 0)
+__device__ __forceinline__ bool pri_less(u64 ka, u32 na, u64 kb, u32 nb) { return ka < kb || (ka == kb && na < nb); }
+
+__global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32* minUto, u32* minUfrom,
+                                                     u32* minAto, u32* minAfrom, u32* mark, u32 epoch, u32* next,
+                                                     u32* next_cnt, u32* status) {
+  __shared__ u64 skey[kDecSmall];
+  __shared__ u32 snode[kDecSmall];
+  __shared__ u32 sidx[kDecSmall];
+  __shared__ u32 sfrom[kDecSmall];
+  __shared__ u32 sto[kDecSmall];
+  __shared__ u8 sst[kDecSmall];
+  __shared__ u32 und, nacc;
+  const u32 c = *cand.cnt;
+  const u32 tid = threadIdx.x;
+  if (c > kDecSmall) {
+    if (tid == 0) status[0] = 1;  // multi-kernel path
+    return;
+  }
+  u32 N = 1;
+  while (N < c) N <<= 1;
+  for (u32 i = tid; i < N; i += blockDim.x) {
+    if (i < c) {
+      skey[i] = ~(u64)__double_as_longlong(cand.gain[i]);
+      snode[i] = cand.node[i];
+      sidx[i] = i;
+    } else {
+      skey[i] = ~0ull;
+      snode[i] = kNone;
+      sidx[i] = kNone;
+    }
+  }
+  __syncthreads();
+  // bitonic sort of (key, node) ascending
+  for (u32 kk = 2; kk <= N; kk <<= 1) {
+    for (u32 j = kk >> 1; j > 0; j >>= 1) {
+      for (u32 i = tid; i < N; i += blockDim.x) {
+        u32 ixj = i ^ j;
+        if (ixj > i) {
+          bool up = (i & kk) == 0;
+          bool gt = pri_less(skey[ixj], snode[ixj], skey[i], snode[i]);
+          if (gt == up) {
+            u64 tk = skey[i];
+            skey[i] = skey[ixj];
+            skey[ixj] = tk;
+            u32 tn = snode[i];
+            snode[i] = snode[ixj];
+            snode[ixj] = tn;
+            u32 ti = sidx[i];
+            sidx[i] = sidx[ixj];
+            sidx[ixj] = ti;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (u32 p = tid; p < c; p += blockDim.x) {
+    u32 v = snode[p];
+    sfrom[p] = m.f[v];
+    sto[p] = cand.to[sidx[p]];
+    sst[p] = kUndec;
+  }
+  __syncthreads();
+  // priority rounds: reject if an earlier accepted move entered our source or
+  // left our target; wait while an earlier undecided one could
+  for (int iter = 0; iter < (1 << 20); ++iter) {
+    for (u32 p = tid; p < c; p += blockDim.x) {
+      u32 f = sfrom[p], t = sto[p];
+      atomicExch(&minUto[f], kNone); atomicExch(&minUto[t], kNone);
+      atomicExch(&minUfrom[f], kNone); atomicExch(&minUfrom[t], kNone);
+      atomicExch(&minAto[f], kNone); atomicExch(&minAto[t], kNone);
+      atomicExch(&minAfrom[f], kNone); atomicExch(&minAfrom[t], kNone);
+    }
+    if (tid == 0) und = 0;
+    __threadfence();
+    __syncthreads();
+    for (u32 p = tid; p < c; p += blockDim.x) {
+      u8 st = sst[p];
+      if (st == kUndec) {
+        atomicMin(&minUto[sto[p]], p);
+        atomicMin(&minUfrom[sfrom[p]], p);
+      } else if (st == kAcc) {
+        atomicMin(&minAto[sto[p]], p);
+        atomicMin(&minAfrom[sfrom[p]], p);
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    u32 local = 0;
+    for (u32 p = tid; p < c; p += blockDim.x) {
+      if (sst[p] != kUndec) continue;
+      u32 f = sfrom[p], t = sto[p];
+      if (atomicAdd(&minAto[f], 0u) < p || atomicAdd(&minAfrom[t], 0u) < p)
+        sst[p] = kRej;
+      else if (atomicAdd(&minUto[f], 0u) < p || atomicAdd(&minUfrom[t], 0u) < p)
+        ++local;
+      else
+        sst[p] = kAcc;
+    }
+    if (local) atomicAdd(&und, local);
+    __syncthreads();
+    const u32 u = und;
+    __syncthreads();
+    if (u == 0) break;
+  }
+  // apply (throughput.rs:182-187)
+  if (tid == 0) nacc = 0;
+  __syncthreads();
+  u32 la = 0;
+  for (u32 p = tid; p < c; p += blockDim.x) {
+    if (sst[p] != kAcc) continue;
+    ++la;
+    u32 v = snode[p];
+    double kv = m.k[v];
+    m.f[v] = sto[p];
+    atomicAdd(&m.tot[sfrom[p]], -kv);
+    atomicAdd(&m.tot[sto[p]], kv);
+    m.B[v] = 1;
+  }
+  if (la) atomicAdd(&nacc, la);
+  __threadfence();
+  __syncthreads();
+  // next-active and K marks (throughput.rs:189-199)
+  const u32 lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  for (u32 p = wv; p < c; p += nw) {
+    if (sst[p] != kAcc) continue;
+    const u32 v = snode[p], cto = sto[p];
+    const u64 off = m.g.row_off[v];
+    const u32 deg = m.g.row_len[v];
+    const u32 sv = m.s[v];
+    bool kv = false;
+    for (u32 j0 = 0; j0 < deg; j0 += 64) {
+      u32 j = j0 + lane;
+      bool want = false;
+      u32 u = 0;
+      if (j < deg) {
+        u = m.g.col[off + j];
+        if (ld_agent(&m.f[u]) != cto) want = claim(mark, u, epoch);
+        if (m.s[u] == sv) {
+          m.K[u] = 1;
+          kv = true;
+        }
+      }
+      u32 q = wave_append(want, next_cnt);
+      if (want) next[q] = u;
+    }
+    if (__any(kv) && lane == 0) m.K[v] = 1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    status[0] = 0;
+    status[1] = nacc;
+    *cand.cnt = 0;
+  }
+}
+
+// ------------------------------------------------------------------ large candidate sets
+__global__ void k_cand_node_keys(CandOut cand, u32 c, u32* keys, u32* vals) {
+  for (u64 i = gtid(); i < c; i += gstride()) {
+    keys[i] = cand.node[i];
+    vals[i] = u32(i);
+  }
+}
+__global__ void k_cand_gain_keys(const u32* order1, u32 c, CandOut cand, u64* keys) {
+  for (u64 i = gtid(); i < c; i += gstride()) keys[i] = ~(u64)__double_as_longlong(cand.gain[order1[i]]);
+}
+__global__ void k_cand_arrange(const u32* order2, const u32* order1, u32 c, CandOut cand, const u32* f, u32* node,
+                               u32* from, u32* to, u32* state) {
+  for (u64 p = gtid(); p < c; p += gstride()) {
+    u32 i = order1[order2[p]];
+    u32 v = cand.node[i];
+    node[p] = v;
+    from[p] = f[v];
+    to[p] = cand.to[i];
+    state[p] = kUndec;
+  }
+}
+__global__ void k_reset4(u64 c, const u32* from, const u32* to, u32* a, u32* b, u32* x, u32* y) {
+  for (u64 p = gtid(); p < c; p += gstride()) {
+    u32 f = from[p], t = to[p];
+    a[f] = kNone; a[t] = kNone;
+    b[f] = kNone; b[t] = kNone;
+    x[f] = kNone; x[t] = kNone;
+    y[f] = kNone; y[t] = kNone;
+  }
+}
+__global__ void k_mark4(u64 c, const u32* from, const u32* to, const u32* state, u32* minUto, u32* minUfrom,
+                        u32* minAto, u32* minAfrom) {
+  for (u64 p = gtid(); p < c; p += gstride()) {
+    u32 st = state[p];
+    if (st == kUndec) {
+      atomicMin(&minUto[to[p]], u32(p));
+      atomicMin(&minUfrom[from[p]], u32(p));
+    } else if (st == kAcc) {
+      atomicMin(&minAto[to[p]], u32(p));
+      atomicMin(&minAfrom[from[p]], u32(p));
+    }
+  }
+}
+__global__ void k_decide4(u64 c, const u32* from, const u32* to, u32* state, const u32* minUto, const u32* minUfrom,
+                          const u32* minAto, const u32* minAfrom, u32* undec) {
+  u32 local = 0;
+  for (u64 p = gtid(); p < c; p += gstride()) {
+    if (state[p] != kUndec) continue;
+    u32 f = from[p], t = to[p], pp = u32(p);
+    if (minAto[f] < pp || minAfrom[t] < pp)
+      state[p] = kRej;
+    else if (minUto[f] < pp || minUfrom[t] < pp)
+      ++local;
+    else
+      state[p] = kAcc;
+  }
+  for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
+  if ((threadIdx.x & 63) == 0 && local) atomicAdd(undec, local);
+}
+__global__ void k_seq_finish(u64 c, const u32* from, const u32* to, u32* state, u8* ea) {
+  if (gtid() != 0) return;
+  for (u64 p = 0; p < c; ++p) {
+    u32 f = from[p], t = to[p];
+    if (state[p] == kUndec) state[p] = ((ea[f] & 2) || (ea[t] & 1)) ? kRej : kAcc;
+    if (state[p] == kAcc) {
+      ea[f] |= 1;
+      ea[t] |= 2;
+    }
+  }
+  for (u64 p = 0; p < c; ++p) {
+    ea[from[p]] = 0;
+    ea[to[p]] = 0;
+  }
+}
+__global__ void k_apply4(u64 c, const u32* node, const u32* from, const u32* to, const u32* state, MoveCtx m,
+                         u32* acc_list, u32* acc_cnt) {
+  for (u64 base = u64(blockIdx.x) * blockDim.x; base < c; base += gstride()) {
+    u64 p = base + threadIdx.x;
+    bool acc = p < c && state[p] == kAcc;
+    if (acc) {
+      u32 v = node[p];
+      double kv = m.k[v];
+      m.f[v] = to[p];
+      atomicAdd(&m.tot[from[p]], -kv);
+      atomicAdd(&m.tot[to[p]], kv);
+      m.B[v] = 1;
+    }
+    u32 q = wave_append(acc, acc_cnt);
+    if (acc) acc_list[q] = u32(p);
+  }
+}
+__global__ __launch_bounds__(256) void k_marks4(MoveCtx m, const u32* acc_list, const u32* acc_cnt, const u32* node,
+                                                 const u32* to, u32* mark, u32 epoch, u32* next, u32* next_cnt) {
+  const u32 na = *acc_cnt;
+  const u32 lane = threadIdx.x & 63;
+  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (u64 q = wave; q < na; q += nwaves) {
+    u32 p = acc_list[q];
+    u32 v = node[p], c = to[p];
+    u64 off = m.g.row_off[v];
+    u32 deg = m.g.row_len[v];
+    u32 sv = m.s[v];
+    bool kv = false;
+    for (u32 j0 = 0; j0 < deg; j0 += 64) {
+      u32 j = j0 + lane;
+      bool want = false;
+      u32 u = 0;
+      if (j < deg) {
+        u = m.g.col[off + j];
+        if (m.f[u] != c) want = claim(mark, u, epoch);
+        if (m.s[u] == sv) {
+          m.K[u] = 1;
+          kv = true;
+        }
+      }
+      u32 pos = wave_append(want, next_cnt);
+      if (want) next[pos] = u;
+    }
+    if (__any(kv) && lane == 0) m.K[v] = 1;
+  }
+}
+
+// ------------------------------------------------------------------ refinement helpers
+__global__ void k_ref_targets(CandOut cand, u8* target) {
+  const u32 c = *cand.cnt;
+  for (u64 i = gtid(); i < c; i += gstride()) target[cand.to[i]] = 1;
+}
+__global__ void k_ref_apply(CandOut cand, const u8* target, u32* s) {
+  const u32 c = *cand.cnt;
+  for (u64 i = gtid(); i < c; i += gstride()) {
+    u32 v = cand.node[i];
+    if (!target[s[v]]) s[v] = cand.to[i];  // a target singleton keeps its place
+  }
+}
+__global__ void k_ref_clear_targets(CandOut cand, u8* target) {
+  const u32 c = *cand.cnt;
+  for (u64 i = gtid(); i < c; i += gstride()) target[cand.to[i]] = 0;
+}
+
+}  // namespace
+
+// ====================================================================== host side
+
+static unsigned fixed_grid(u64 work, unsigned bs = 256, unsigned cap = 2048) {
+  u64 b = (work + bs - 1) / bs;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return unsigned(b);
+}
+
+void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
+  if (mark.cap < n + 1) {
+    mark.reserve(n + 1, s, true);
+    // new entries must not equal a live epoch: zero the whole array, restart epochs
+    HL_CHECK(hipMemsetAsync(mark.p, 0, mark.cap * sizeof(u32), s));
+    epoch = 0;
+  }
+  list_a.reserve(n + 1, s);
+  list_b.reserve(n + 1, s);
+  blists.reserve(3 * (n + 1), s);
+  cnode.reserve(n + 1, s);
+  cto.reserve(n + 1, s);
+  cgain.reserve(n + 1, s);
+  counters.reserve(16, s);
+  if (minUto.cap < comm_cap) {
+    minUto.reserve(comm_cap, s);
+    minUfrom.reserve(comm_cap, s);
+    minAto.reserve(comm_cap, s);
+    minAfrom.reserve(comm_cap, s);
+  }
+}
+
+u32 MoveScratch::next_epoch(hipStream_t s) {
+  if (++epoch == 0xFFFFFFF0u) {
+    HL_CHECK(hipMemsetAsync(mark.p, 0, mark.cap * sizeof(u32), s));
+    epoch = 1;
+  }
+  return epoch;
+}
+
+template <class P>
+static void propose_round(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u64 list_cap, u32 maxdeg,
+                          MoveScratch& ms, const Thresholds& th, u64* degsum_out, hipStream_t s) {
+  u32* counts = ms.counters.p + 4;  // [4..7) bucket counts
+  unsigned long long* degsum = (unsigned long long*)(ms.counters.p + 8);  // [8..14) 3 x u64
+  HL_CHECK(hipMemsetAsync(counts, 0, 4 * sizeof(u32) + 3 * sizeof(u64), s));
+  const u32 small = th.small_deg > SMALL_MAX2 ? SMALL_MAX2 : th.small_deg;
+  const u32 wave = th.wave_deg > WAVE_HW2 / 2 ? WAVE_HW2 / 2 : th.wave_deg;
+  hipLaunchKernelGGL(k_bucket2, fixed_grid(list_cap), 256, 0, s, list, cnt_dev, g.row_len, small, wave,
+                     ms.blists.p, list_cap, counts, degsum);
+  HL_LAUNCH_CHECK();
+  CandOut out{ms.cnode.p, ms.cto.p, ms.cgain.p, ms.counters.p + 1};
+  {
+    HL_PROF(P_PROP_SMALL, 0, s);
+    hipLaunchKernelGGL(k_prop2_small<P>, fixed_grid(list_cap), 256, 0, s, pol, g, ms.blists.p, counts, out);
+    HL_LAUNCH_CHECK();
+  }
+  {
+    HL_PROF(P_PROP_WAVE, 0, s);
+    hipLaunchKernelGGL(k_prop2_wave<P>, dim3(fixed_grid(list_cap, 4, 4096)), dim3(256), 0, s, pol, g,
+                       ms.blists.p + list_cap, counts + 1, out);
+    HL_LAUNCH_CHECK();
+  }
+  if (maxdeg > wave) {
+    HL_PROF(P_PROP_BIG, 0, s);
+    u32 tcap = 64;
+    while (tcap < 2 * maxdeg) tcap <<= 1;
+    const unsigned nb = 256;
+    ms.tk.reserve(u64(nb) * tcap, s);
+    ms.tv.reserve(u64(nb) * tcap, s);
+    hipLaunchKernelGGL(k_prop2_big<P>, dim3(nb), dim3(256), 0, s, pol, g, ms.blists.p + 2 * list_cap, counts + 2,
+                       ms.tk.p, ms.tv.p, tcap, out);
+    HL_LAUNCH_CHECK();
+  }
+  (void)degsum_out;
+}
+
+u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int max_rounds, MoveScratch& ms,
+                        const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s) {
+  const u32 n = m.g.n;
+  ms.ensure(n, m.comm_cap, s);
+  u32* cnt_a = ms.counters.p + 0;
+  u32* ccnt = ms.counters.p + 1;
+  u32* next_cnt = ms.counters.p + 2;
+  u32* status = ms.counters.p + 14;  // [14..16)
+  HL_CHECK(hipMemsetAsync(ms.counters.p, 0, 16 * sizeof(u32), s));
+  u32* cur = ms.list_a.p;
+  u32* nxt = ms.list_b.p;
+  if (k == 0) {
+    hipLaunchKernelGGL(k_iota_list, fixed_grid(n, 256, 4096), 256, 0, s, cur, n, cnt_a);
+    HL_LAUNCH_CHECK();
+  } else {
+    const u32 ep = ms.next_epoch(s);
+    HL_PROF(P_SEED, 24 * k, s);
+    hipLaunchKernelGGL(k_seed2, fixed_grid(k), 256, 0, s, d, k, m.f, m.s, m.A, m.K, ms.mark.p, ep, cur, cnt_a);
+    HL_LAUNCH_CHECK();
+  }
+  MovePolicyT pol{m.f, m.tot, m.gamma, n};
+  u32 host_cnt = 0;
+  HL_CHECK(hipMemcpyAsync(&host_cnt, cnt_a, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipStreamSynchronize(s));
+  if (st) st->active = host_cnt;
+  u32 rounds = 0;
+  while (host_cnt > 0 && int(rounds) < max_rounds) {
+    if (st) st->evaluated += host_cnt;
+    propose_round(pol, m.g, cur, cnt_a, host_cnt, maxdeg, ms, th, nullptr, s);
+    const u32 ep = ms.next_epoch(s);
+    HL_CHECK(hipMemsetAsync(next_cnt, 0, sizeof(u32), s));
+    CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
+    {
+      HL_PROF(P_DECOUPLE, 0, s);
+      hipLaunchKernelGGL(k_dec_small, 1, 1024, 0, s, m, cand, ms.minUto.p, ms.minUfrom.p, ms.minAto.p,
+                         ms.minAfrom.p, ms.mark.p, ep, nxt, next_cnt, status);
+      HL_LAUNCH_CHECK();
+    }
+    u32 h[4];
+    HL_CHECK(hipMemcpyAsync(h, ms.counters.p, 4 * sizeof(u32), hipMemcpyDeviceToHost, s));
+    u32 hs[2];
+    HL_CHECK(hipMemcpyAsync(hs, status, 2 * sizeof(u32), hipMemcpyDeviceToHost, s));
+    HL_CHECK(hipStreamSynchronize(s));
+    u32 accepted = hs[1];
+    if (hs[0] == 1) {
+      // many candidates: sort by node, then stably by gain; priority rounds over the grid
+      const u32 c = h[1];
+      ms.k32.reserve(c, s);
+      ms.v32.reserve(c, s);
+      ms.o1.reserve(c, s);
+      ms.k32b.reserve(c, s);
+      ms.k64.reserve(c, s);
+      ms.k64b.reserve(c, s);
+      ms.o2.reserve(c, s);
+      ms.node.reserve(c, s);
+      ms.from.reserve(c, s);
+      ms.to.reserve(c, s);
+      ms.state.reserve(c, s);
+      ms.acc.reserve(c, s);
+      hipLaunchKernelGGL(k_cand_node_keys, fixed_grid(c), 256, 0, s, cand, c, ms.k32.p, ms.v32.p);
+      HL_LAUNCH_CHECK();
+      sort_pairs_u32(ms.k32.p, ms.k32b.p, ms.v32.p, ms.o1.p, c, 32, tmp, s);
+      hipLaunchKernelGGL(k_cand_gain_keys, fixed_grid(c), 256, 0, s, ms.o1.p, c, cand, ms.k64.p);
+      HL_LAUNCH_CHECK();
+      iota_u32(ms.v32.p, 0, c, s);
+      sort_pairs_u64(ms.k64.p, ms.k64b.p, ms.v32.p, ms.o2.p, c, 64, tmp, s);
+      hipLaunchKernelGGL(k_cand_arrange, fixed_grid(c), 256, 0, s, ms.o2.p, ms.o1.p, c, cand, m.f, ms.node.p,
+                         ms.from.p, ms.to.p, ms.state.p);
+      HL_LAUNCH_CHECK();
+      int iters = 0;
+      while (true) {
+        HL_PROF(P_DECOUPLE, 0, s);
+        hipLaunchKernelGGL(k_reset4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.minUto.p,
+                           ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p);
+        hipLaunchKernelGGL(k_mark4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p, ms.minUto.p,
+                           ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p);
+        HL_CHECK(hipMemsetAsync(status + 1, 0, sizeof(u32), s));
+        hipLaunchKernelGGL(k_decide4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p,
+                           ms.minUto.p, ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p, status + 1);
+        HL_LAUNCH_CHECK();
+        u32 und = 0;
+        HL_CHECK(hipMemcpyAsync(&und, status + 1, sizeof(u32), hipMemcpyDeviceToHost, s));
+        HL_CHECK(hipStreamSynchronize(s));
+        if (und == 0) break;
+        if (++iters >= 48) {
+          DevBuf<u8> ea;
+          ea.reserve(m.comm_cap, s);
+          HL_CHECK(hipMemsetAsync(ea.p, 0, m.comm_cap, s));
+          hipLaunchKernelGGL(k_seq_finish, 1, 64, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p, ea.p);
+          HL_LAUNCH_CHECK();
+          HL_CHECK(hipStreamSynchronize(s));
+          break;
+        }
+      }
+      u32* acc_cnt = status + 1;
+      HL_CHECK(hipMemsetAsync(acc_cnt, 0, sizeof(u32), s));
+      {
+        HL_PROF(P_APPLY_MOVES, 0, s);
+        hipLaunchKernelGGL(k_apply4, fixed_grid(c), 256, 0, s, u64(c), ms.node.p, ms.from.p, ms.to.p, ms.state.p, m,
+                           ms.acc.p, acc_cnt);
+        HL_LAUNCH_CHECK();
+      }
+      {
+        HL_PROF(P_MARK_MOVED, 0, s);
+        hipLaunchKernelGGL(k_marks4, fixed_grid(u64(c) * 64, 256, 4096), 256, 0, s, m, ms.acc.p, acc_cnt,
+                           ms.node.p, ms.to.p, ms.mark.p, ep, nxt, next_cnt);
+        HL_LAUNCH_CHECK();
+      }
+      HL_CHECK(hipMemsetAsync(ccnt, 0, sizeof(u32), s));
+      HL_CHECK(hipMemcpyAsync(&accepted, acc_cnt, sizeof(u32), hipMemcpyDeviceToHost, s));
+      HL_CHECK(hipMemcpyAsync(&h[2], next_cnt, sizeof(u32), hipMemcpyDeviceToHost, s));
+      HL_CHECK(hipStreamSynchronize(s));
+    }
+    ++rounds;
+    if (st) st->moved_rounds += accepted;
+    if (accepted == 0) break;  // throughput.rs:177-180 (every accepted gain is > 0)
+    std::swap(cur, nxt);
+    HL_CHECK(hipMemcpyAsync(cnt_a, next_cnt, sizeof(u32), hipMemcpyDeviceToDevice, s));
+    host_cnt = h[2];
+  }
+  return rounds;
+}
+
+void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, const u8* blocked, double gamma,
+                      const u32* singles, u64 ns, u64 sspace, MoveScratch& ms, const Thresholds& th, u32 maxdeg,
+                      hipStream_t s) {
+  ms.ensure(g.n, 1, s);
+  ms.target.reserve(sspace, s);
+  if (ms.target_cap != ms.target.cap) {
+    HL_CHECK(hipMemsetAsync(ms.target.p, 0, ms.target.cap, s));
+    ms.target_cap = ms.target.cap;
+  }
+  u32* cnt = ms.counters.p + 3;
+  u32* ccnt = ms.counters.p + 1;
+  HL_CHECK(hipMemcpyAsync(cnt, &ns, sizeof(u32), hipMemcpyHostToDevice, s));
+  HL_CHECK(hipMemsetAsync(ccnt, 0, sizeof(u32), s));
+  RefinePolicyT pol{f, sm, dS, blocked, gamma};
+  propose_round(pol, g, singles, cnt, ns, maxdeg, ms, th, nullptr, s);
+  CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
+  HL_PROF(P_REFINE, 0, s);
+  hipLaunchKernelGGL(k_ref_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);
+  hipLaunchKernelGGL(k_ref_apply, fixed_grid(ns), 256, 0, s, cand, ms.target.p, sm);
+  hipLaunchKernelGGL(k_ref_clear_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);
+  HL_LAUNCH_CHECK();
+  HL_CHECK(hipMemsetAsync(ccnt, 0, sizeof(u32), s));
+}
+
+}  // namespace hl

@@ -46,7 +46,7 @@ constexpr double kEpsSelf = 1e-12;      // hit_leiden.rs:1687
 constexpr double kEpsCompress = 1e-9;   // hit_leiden.rs:1711
 constexpr double kEpsTFilter = 1e-9;    // hit_leiden.rs:1547,1631
 constexpr int kMaxMoveRounds = 20;      // hit_leiden.rs:1077
-constexpr u64 kChunk = 4096;            // canonical chunk for vertex sums
+
 
 struct Rec {
   u32 u, v;
@@ -59,7 +59,7 @@ struct Graph {
   std::vector<std::vector<u32>> col;
   std::vector<std::vector<double>> wt;
   std::vector<double> k;  // weighted degree: the row sum, self-loop counted twice
-  double twice_m = 0.0;   // in_memory.rs:62 (x2), canonical chunked order
+  double twice_m = 0.0;   // in_memory.rs:62 (x2), canonical pairwise-tree order
 };
 
 double row_degree(const Graph& g, u32 v) {
@@ -73,15 +73,17 @@ double row_degree(const Graph& g, u32 v) {
   return acc;
 }
 
-double chunked_sum(const std::vector<double>& x, u64 n) {
-  double total = 0.0;
-  for (u64 c = 0; c < n; c += kChunk) {
-    double p = 0.0;
-    u64 e = std::min<u64>(n, c + kChunk);
-    for (u64 i = c; i < e; ++i) p += x[i];
-    total += p;
-  }
-  return total;
+// canonical 2m: pairwise (halving) tree over x[0..n) zero-padded to a power of
+// two -- at each level element i (a multiple of 2s) absorbs element i+s
+double tree_sum(const std::vector<double>& x, u64 n) {
+  if (n == 0) return 0.0;
+  u64 N = 1;
+  while (N < n) N <<= 1;
+  std::vector<double> b(N, 0.0);
+  std::copy(x.begin(), x.begin() + n, b.begin());
+  for (u64 s = 1; s < N; s <<= 1)
+    for (u64 i = 0; i + s < N; i += 2 * s) b[i] += b[i + s];
+  return b[0];
 }
 
 void grow_graph(Graph& g, u32 n_new) {
@@ -121,7 +123,7 @@ void erase_entry(Graph& g, u32 r, u32 x) {
 
 void finish_degrees(Graph& g, const std::vector<u32>& touched) {
   for (u32 r : touched) g.k[r] = row_degree(g, r);
-  g.twice_m = chunked_sum(g.k, g.n);
+  g.twice_m = tree_sum(g.k, g.n);
 }
 
 // apply_delta / collect_canonical_edges / apply_delta_edges (in_memory.rs:103-138):

@@ -14,8 +14,8 @@
  *     record weight w and counts as the two entries (w, w) the reference stores.
  *   - argmax over candidate communities / subcommunities in ascending id with
  *     strict '>', the empty community evaluated last with strict '>'.
- *   - k_v = sequential sum over the sorted row; 2m = sum of k_v in chunks of
- *     4096 vertices (chunk partials summed in chunk order).
+ *   - k_v = sequential sum over the sorted row; 2m = pairwise (halving) tree
+ *     over k_v zero-padded to a power of two.
  *   - community / subcommunity degree totals = per-id sequential sums in vertex order.
  *   - split components numbered by (subcommunity asc, min vertex asc).
  *   - aggregation compress sums per key in emission order; output sorted by key.
