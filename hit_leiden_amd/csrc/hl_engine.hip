@@ -29,6 +29,11 @@ struct Level {
   DevBuf<Rec> dG;
   u64 dG_len = 0;
   u32 iters = 0;
+  // throughput mode: community totals maintained across batches
+  DevBuf<double> tot;
+  bool tot_valid = false;
+  u64 lab_bound = 0;  // every f label < lab_bound
+  u32 deg_bound = 0;  // every row length <= deg_bound
 
   Csr csr() const { return Csr{n, row_off.p, row_len.p, col.p, w.p, k.p, twice_m.p}; }
   LevelCsrMut mut() { return LevelCsrMut{n, row_off.p, row_len.p, row_cap.p, col.p, w.p, k.p, twice_m.p}; }
@@ -120,6 +125,25 @@ struct Engine {
     grow_init(n_old, n_new, u32(bf), u32(bg), u32(bs), lv.f.p, lv.g.p, lv.s_pre.p, lv.s_cur.p, lv.row_off.p,
               lv.row_len.p, lv.row_cap.p, lv.k.p, s);
     lv.n = n_new;
+    lv.lab_bound = std::max<u64>(lv.lab_bound, bf + (n_new - n_old));
+  }
+
+  bool par() const { return mode == HL_MODE_THROUGHPUT; }
+
+  // make level p's maintained community totals cover every label
+  void ensure_totals(Level& lv) {
+    const u64 need = std::max<u64>(std::max<u64>(2ull * lv.n, lv.lab_bound), 1) + 1;
+    const u64 old = lv.tot.cap;
+    if (!lv.tot_valid || old < need) {
+      lv.tot.reserve(need, s, lv.tot_valid, 1.25);
+      if (lv.tot_valid) {
+        fill_f64(lv.tot.p + old, 0.0, lv.tot.cap - old, s);
+      } else {
+        fill_f64(lv.tot.p, 0.0, lv.tot.cap, s);
+        label_totals_atomic(lv.f.p, lv.k.p, lv.n, lv.tot.p, s);
+        lv.tot_valid = true;
+      }
+    }
   }
 
   // G^p <- G^p (+) dG^p, then 2m
@@ -146,8 +170,15 @@ struct Engine {
         lv.pool_used = live;
         g = lv.mut();
       }
-      apply_delta_commit(g, lv.pool_used, as, plan, mode == HL_MODE_DETERMINISTIC, s);
+      TotalsRef tr;
+      if (par() && lv.tot_valid) {
+        ensure_totals(lv);  // fresh labels of grown vertices
+        tr.tot = lv.tot.p;
+        tr.f = lv.f.p;
+      }
+      apply_delta_commit(g, lv.pool_used, as, plan, mode == HL_MODE_DETERMINISTIC, tr, s);
       lv.pool_used += plan.need;
+      lv.deg_bound = std::max<u32>(lv.deg_bound, u32(plan.max_row));
     }
     partials.reserve(twice_m_scratch(lv.n), s);
     twice_m_tree(lv.k.p, lv.n, partials.p, lv.twice_m.p, s);
@@ -156,11 +187,13 @@ struct Engine {
   }
 
   u64 comm_space(Level& lv) {
+    if (par() && lv.tot_valid) return std::max<u64>(std::max<u64>(2ull * lv.n, lv.lab_bound), 1);
     u64 mf = lv.n ? u64(dmax(lv.f.p, lv.n)) + 1 : 0;
     return std::max<u64>(std::max<u64>(2ull * lv.n, mf), 1);
   }
 
-  u32 max_degree(Level& lv) { return lv.n ? dmax(lv.row_len.p, lv.n) : 0; }
+  // upper bound of the row lengths (rows only shrink below it)
+  u32 max_degree(Level& lv) { return lv.deg_bound; }
 
   // Algorithm 2 (hit_leiden.rs:1024-1192)
   u32 movement(Level& lv, const Rec* d, u64 k, hl_batch_stats* st, int p) {
@@ -198,12 +231,10 @@ struct Engine {
       return 1;
     }
     // throughput: synchronous rounds, at most max_rounds (hit_leiden.rs:1077-1099)
-    const u64 ccap = comm_space(lv);
-    tot.reserve(ccap, s);
-    fill_f64(tot.p, 0.0, ccap, s);
-    label_totals_atomic(lv.f.p, lv.k.p, n, tot.p, s);
+    ensure_totals(lv);
     const int max_rounds = cfg.max_rounds > 0 ? cfg.max_rounds : 20;
-    MoveCtx m{g, lv.f.p, lv.s_cur.p, tot.p, lv.k.p, lv.A.p, lv.B.p, lv.K.p, cfg.gamma, ccap};
+    MoveCtx m{g, lv.f.p, lv.s_cur.p, lv.tot.p, lv.k.p, lv.A.p, lv.B.p, lv.K.p, cfg.gamma, lv.tot.cap};
+    lv.lab_bound = std::max<u64>(lv.lab_bound, 2ull * n);  // empty-community ids n + v
     MoveStats ms_st;
     const u32 maxd = max_degree(lv);
     u32 rounds = movement_throughput(m, d, k, maxd, max_rounds, mvs, th, tmp, &ms_st, s);
@@ -228,7 +259,8 @@ struct Engine {
       refine_det(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, rs, ks, vs, vidx, heads, starts, cnt,
                  max_degree(lv), tmp, s);
     else
-      refine_parallel(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, rs, ps, mvs, max_degree(lv), th, tmp, s);
+      refine_parallel(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, rs, ps, mvs, max_degree(lv), th, tmp,
+                      lv.tot_valid ? lv.tot.p : nullptr, s);
     (void)st;
     (void)p;
   }
@@ -245,7 +277,15 @@ struct Engine {
       Level& lv = *L[p];
       if (p < P - 1) {
         Level& up = *L[p + 1];
-        def_pull(Bp[p]->p, use_g ? lv.g.p : lv.f.p, use_g ? up.g.p : up.f.p, lv.s_cur.p, lv.n, s);
+        double* tot = nullptr;
+        if (!use_g) {
+          lv.lab_bound = std::max(lv.lab_bound, up.lab_bound);  // f^p takes f^{p+1} labels
+          if (par() && lv.tot_valid) {
+            ensure_totals(lv);
+            tot = lv.tot.p;
+          }
+        }
+        def_pull(Bp[p]->p, use_g ? lv.g.p : lv.f.p, use_g ? up.g.p : up.f.p, lv.s_cur.p, lv.n, tot, lv.k.p, s);
       }
       if (p > 0) {
         Level& lo = *L[p - 1];
@@ -590,6 +630,11 @@ hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const ui
     hl::Level& lv = const_cast<hl::Level&>(e.level(level));
     if (n != lv.n) throw hl::InputError("length differs from the level size");
     if (n) HL_CHECK(hipMemcpyAsync(e.map_of(lv, which).p, in, n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));
+    if (which == HL_MAP_F) {
+      // caller-written labels: rebuild the maintained totals on next use
+      lv.tot_valid = false;
+      lv.lab_bound = n ? std::max<uint64_t>(lv.n, uint64_t(e.dmax(lv.f.p, n)) + 1) : 0;
+    }
     HL_CHECK(hipStreamSynchronize(e.s));
   });
 }

@@ -68,16 +68,27 @@ struct LevelCsrMut {
   double* twice_m;
 };
 
+// Maintained totals a kernel keeps current when it changes k or a label
+// (null pointers: not maintained).  tot[c] = sum of k over f = c,
+// dS[S] = sum of k over s = S, sizes[S] = |{v : s(v) = S}|.
+struct TotalsRef {
+  double* tot = nullptr;
+  const u32* f = nullptr;
+  double* dS = nullptr;
+  u32* sizes = nullptr;
+  const u32* s = nullptr;
+};
+
 // Plans and applies G <- G (+) dG (in_memory.rs:103-138).  Returns the number of
 // affected rows and writes into *need the pool entries the relocation needs;
 // the engine must call apply_delta_commit after making room.
 struct ApplyPlan {
-  u64 nruns = 0, nupd = 0, nseg = 0, need = 0, total_chunks = 0, old_entries = 0;
+  u64 nruns = 0, nupd = 0, nseg = 0, need = 0, total_chunks = 0, old_entries = 0, max_row = 0;
 };
 void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a, Temp& tmp, ApplyPlan& plan,
                       hipStream_t s);
 void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, const ApplyPlan& plan, bool det,
-                        hipStream_t s);
+                        const TotalsRef& tr, hipStream_t s);
 // compact all rows into a fresh pool [col2, w2) with row_cap = row_len; returns live nnz
 u64 compact_rows(const LevelCsrMut& g, u32* col2, double* w2, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s);
 // canonical 2m (pairwise tree); partials needs twice_m_scratch(n) doubles
@@ -151,7 +162,7 @@ bool is_identity(const u32* s, u32 n, RefineScratch& rs, hipStream_t st);
 struct MoveScratch;
 void refine_parallel(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space,
                      RefineScratch& rs, ProposeScratch& ps, MoveScratch& mvs, u32 maxdeg, const Thresholds& th,
-                     Temp& tmp, hipStream_t st);
+                     Temp& tmp, const double* dpar_in, hipStream_t st);
 // deterministic refinement (hit_leiden.rs:1359-1413)
 void refine_det(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space, RefineScratch& rs,
                 DevBuf<u32>& ks, DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,
@@ -169,7 +180,9 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
               u32* next_n, AggScratch& as, Temp& tmp, hipStream_t st);
 
 // ---- def_update (hit_leiden.rs:1733-1793) ----
-void def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, hipStream_t s);
+// f^p(v) = f^{p+1}(s^p(v)) for v in B; tot (may be null) is level p's maintained totals
+void def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, double* tot, const double* k,
+              hipStream_t s);
 void def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np, hipStream_t s);
 
 // ---- modularity (hit_leiden.rs:121-161) ----

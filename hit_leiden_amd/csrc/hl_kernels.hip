@@ -243,6 +243,7 @@ __global__ void k_row_plan(u64 nseg, const u32* segs, u64 nupd, const u64* ukey_
     u64 items = u64(old) + (e - s);
     chunks[j] = (items + MERGE_CHUNK - 1) / MERGE_CHUNK;
     atomicAdd((unsigned long long*)old_sum, (unsigned long long)old);
+    atomicMax((unsigned long long*)(old_sum + 1), (unsigned long long)nl);  // largest new row
   }
 }
 
@@ -326,7 +327,9 @@ __global__ void k_row_commit(u64 nseg, const u32* seg_row, const u32* new_len, c
 
 // k_v = sequential sum over the sorted row, self-loop counted twice (the order
 // of the canonical CPU oracle; equal to any order for integer weights)
-__global__ void k_row_degree(u64 nseg, const u32* seg_row, LevelCsrMut g) {
+// When the maintained community / subcommunity totals are live, the change of
+// k_v is added to the totals of v's community and subcommunity.
+__global__ void k_row_degree(u64 nseg, const u32* seg_row, LevelCsrMut g, TotalsRef tr) {
   for (u64 j = gtid(); j < nseg; j += gstride()) {
     u32 r = seg_row[j];
     u64 off = g.row_off[r];
@@ -337,7 +340,12 @@ __global__ void k_row_degree(u64 nseg, const u32* seg_row, LevelCsrMut g) {
       acc += w;
       if (g.col[off + i] == r) acc += w;
     }
+    const double old = g.k[r];
     g.k[r] = acc;
+    if (acc != old) {
+      if (tr.tot) atomicAdd(&tr.tot[tr.f[r]], acc - old);
+      if (tr.dS) atomicAdd(&tr.dS[tr.s[r]], acc - old);
+    }
   }
 }
 
@@ -1525,9 +1533,16 @@ __global__ void k_gather_rec(const u32* sel, u64 c, const Rec* in, Rec* out) {
 }
 
 // ------------------------------------------------------------------ def_update / modularity
-__global__ void k_def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n) {
-  for (u64 v = gtid(); v < n; v += gstride())
-    if (B[v]) fp[v] = fq[sp[v]];
+__global__ void k_def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, double* tot, const double* k) {
+  for (u64 v = gtid(); v < n; v += gstride()) {
+    if (!B[v]) continue;
+    const u32 old = fp[v], nw = fq[sp[v]];
+    fp[v] = nw;
+    if (tot && old != nw && k[v] != 0.0) {  // keep the maintained community totals current
+      atomicAdd(&tot[old], -k[v]);
+      atomicAdd(&tot[nw], k[v]);
+    }
+  }
 }
 __global__ void k_def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np) {
   for (u64 u = gtid(); u < nq; u += gstride()) {
@@ -1753,21 +1768,22 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   a.newoff.reserve(ns + 1, s);
   a.chunks.reserve(ns, s);
   a.chunkbase.reserve(ns + 1, s);
-  HL_CHECK(hipMemsetAsync(a.cnt.p, 0, sizeof(u64), s));
+  HL_CHECK(hipMemsetAsync(a.cnt.p, 0, 2 * sizeof(u64), s));
   hipLaunchKernelGGL(k_row_plan, GRID(ns), ns, a.segs.p, u64(nupd), a.ukey_s.p, a.perm.p, a.uop.p, g.row_len,
                      a.seg_row.p, a.seg_len.p, a.new_len.p, a.seg_ins.p, a.seg_del.p, a.ins_pre.p, a.del_pre.p,
                      a.newcap.p, a.chunks.p, a.cnt.p);
   HL_LAUNCH_CHECK();
   plan.need = exclusive_scan_u64(a.newcap.p, a.newoff.p, ns, tmp, s);
   plan.total_chunks = exclusive_scan_u64(a.chunks.p, a.chunkbase.p, ns, tmp, s);
-  u64 olds = 0;
-  HL_CHECK(hipMemcpyAsync(&olds, a.cnt.p, sizeof(u64), hipMemcpyDeviceToHost, s));
+  u64 olds[2] = {0, 0};
+  HL_CHECK(hipMemcpyAsync(olds, a.cnt.p, 2 * sizeof(u64), hipMemcpyDeviceToHost, s));
   HL_CHECK(hipStreamSynchronize(s));
-  plan.old_entries = olds;
+  plan.old_entries = olds[0];
+  plan.max_row = olds[1];
 }
 
 void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, const ApplyPlan& plan, bool det,
-                        hipStream_t s) {
+                        const TotalsRef& tr, hipStream_t s) {
   (void)det;
   if (plan.nseg == 0) return;
   const u64 ns = plan.nseg;
@@ -1781,7 +1797,7 @@ void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, co
   hipLaunchKernelGGL(k_row_commit, GRID(ns), ns, a.seg_row.p, a.new_len.p, a.newoff.p, pool_base, g);
   HL_LAUNCH_CHECK();
   HL_PROF(P_ROW_DEGREE, 12 * plan.need + 12 * ns, s);
-  hipLaunchKernelGGL(k_row_degree, GRID(ns), ns, a.seg_row.p, g);
+  hipLaunchKernelGGL(k_row_degree, GRID(ns), ns, a.seg_row.p, g, tr);
   HL_LAUNCH_CHECK();
 }
 
@@ -2127,7 +2143,7 @@ static u64 smax_plus1(const u32* sm, u32 n, RefineScratch& rs, hipStream_t s) {
 
 void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space,
                      RefineScratch& rs, ProposeScratch& ps, MoveScratch& mvs, u32 maxdeg, const Thresholds& th,
-                     Temp& tmp, hipStream_t s) {
+                     Temp& tmp, const double* dpar_in, hipStream_t s) {
   const u32 n = g.n;
   if (n == 0) return;
   const u64 sn = smax_plus1(sm, n, rs, s);
@@ -2156,7 +2172,6 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
   rs.blocked.reserve(sn, s);
   rs.parentIdx.reserve(sn, s);
   rs.wS.reserve(sn, s);
-  rs.dpar.reserve(comm_space, s);
   fill_u8(rs.is_cand.p, 0, sn, s);
   HL_PROF(P_REFINE, 16ull * n, s);
   hipLaunchKernelGGL(k_cand_mark, GRID(ns * 64), g, sl.p, ns, f, sm, rs.is_cand.p);
@@ -2167,9 +2182,14 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
   fill_f64(rs.wS.p, 0.0, sn, s);
   hipLaunchKernelGGL(k_ws, GRID(n), g, n, f, sm, rs.is_cand.p, rs.parentIdx.p, rs.wS.p);
   HL_LAUNCH_CHECK();
-  fill_f64(rs.dpar.p, 0.0, comm_space, s);
-  label_totals_atomic(f, g.k, n, rs.dpar.p, s);
-  hipLaunchKernelGGL(k_blocked, GRID(sn), sn, rs.is_cand.p, rs.dS.p, rs.wS.p, rs.parentIdx.p, f, rs.dpar.p,
+  const double* dpar = dpar_in;  // the maintained community totals, when the caller has them
+  if (!dpar) {
+    rs.dpar.reserve(comm_space, s);
+    fill_f64(rs.dpar.p, 0.0, comm_space, s);
+    label_totals_atomic(f, g.k, n, rs.dpar.p, s);
+    dpar = rs.dpar.p;
+  }
+  hipLaunchKernelGGL(k_blocked, GRID(sn), sn, rs.is_cand.p, rs.dS.p, rs.wS.p, rs.parentIdx.p, f, dpar,
                      g.twice_m, gamma, rs.blocked.p);
   HL_LAUNCH_CHECK();
   // proposals against the snapshot, then the connected-targets rule
@@ -2296,9 +2316,10 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
   return kept;
 }
 
-void def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, hipStream_t s) {
+void def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, double* tot, const double* k,
+              hipStream_t s) {
   HL_PROF(P_DEF_UPDATE, 12ull * n, s);
-  if (n) hipLaunchKernelGGL(k_def_pull, GRID(n), B, fp, fq, sp, n);
+  if (n) hipLaunchKernelGGL(k_def_pull, GRID(n), B, fp, fq, sp, n, tot, k);
   HL_LAUNCH_CHECK();
 }
 void def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np, hipStream_t s) {
