@@ -28,6 +28,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "incremental edges/s + p99 update latency vs plain GPU path; HBM GB/s"
 
+# profiler slots that bracket exactly one kernel (both template instances for
+# the proposal kernels), with the kernel name rocprofv3 reports; the dominant
+# kernel of the roofline is the one of these with the most time
+SLOT_KERNEL = {
+    "propose_small": "k_prop2_small", "propose_wave": "k_prop2_wave", "propose_big": "k_prop2_big",
+    "decouple": "k_dec_small", "apply_moves": "k_apply4", "mark_moved": "k_marks4",
+    "row_merge": "k_row_merge", "resolve_runs": "k_resolve_runs", "row_degree": "k_row_degree",
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -94,7 +103,23 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch of the dominant kernel")
     ap.add_argument("--dump-profile", default=None)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run device-to-device copies of this many bytes (counter calibration)")
+    ap.add_argument("--rocprof-summary", default=None, metavar="DIR",
+                    help="only print a text summary of a rocprofv3 csv output directory")
+    ap.add_argument("--kernel", default=None, help="kernel regex for --rocprof-summary")
+    ap.add_argument("--command", default="", help="command line recorded in --rocprof-summary")
     args = ap.parse_args()
+
+    if args.rocprof_summary:
+        from hit_leiden_amd import profiling
+        print(profiling.summary_text(args.rocprof_summary, args.command, pattern=args.kernel), end="")
+        return
+    if args.copy_probe:
+        from hit_leiden_amd import profiling
+        profiling.copy_probe(args.copy_probe)
+        log(f"[bench] copied {args.copy_probe} bytes device-to-device 4 times")
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -130,7 +155,8 @@ def main():
         k, du, dv, da = dev[i]
         g.apply_batch_device(k, du.address, dv.address, da.address)
     prof_warm = g.profile_read()
-    dominant = max(prof_warm.items(), key=lambda kv: kv[1][1])[0] if prof_warm else None
+    single = {k: v for k, v in prof_warm.items() if k in SLOT_KERNEL}
+    dominant = max(single.items(), key=lambda kv: kv[1][1])[0] if single else None
     log(f"[bench] warmup profile (ms): " + ", ".join(f"{k}={v[1]:.2f}" for k, v in
                                                    sorted(prof_warm.items(), key=lambda kv: -kv[1][1])[:8]))
 
@@ -166,7 +192,8 @@ def main():
             tj = json.load(open(args.traffic))
             traffic = tj.get(dominant)
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dominant,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": SLOT_KERNEL[dominant],
+                "slot": dominant,
                 "launches": launches, "avg_launch_us": round(avg_ms * 1000.0, 3),
                 "alg_bytes_per_launch": round(per_launch_bytes, 1)}
     bytes_alg = sum(s["bytes_alg"] for s in stats)

@@ -92,13 +92,14 @@ struct DevBuf {
 enum ProfSlot {
   P_ROW_MERGE, P_RESOLVE, P_ROW_DEGREE, P_PROP_SMALL, P_PROP_WAVE, P_PROP_BIG, P_DECOUPLE, P_APPLY_MOVES,
   P_MARK_MOVED, P_LABEL_TOTALS, P_SELECT, P_SORT, P_SCAN, P_CC_SPLIT, P_REFINE, P_AGG_EMIT, P_COMPRESS,
-  P_DEF_UPDATE, P_FILL, P_DET_MOVE, P_DET_REFINE, P_TWICE_M, P_SEED, P_MODULARITY, P_REDUCE, P_COUNT
+  P_DEF_UPDATE, P_FILL, P_DET_MOVE, P_DET_REFINE, P_TWICE_M, P_SEED, P_MODULARITY, P_REDUCE, P_DEC_MULTI, P_COUNT
 };
 inline const char* prof_name(int s) {
   static const char* names[P_COUNT] = {
       "row_merge", "resolve_runs", "row_degree", "propose_small", "propose_wave", "propose_big", "decouple",
       "apply_moves", "mark_moved", "label_totals", "select", "sort", "scan", "cc_split", "refine", "agg_emit",
-      "compress", "def_update", "fill", "det_move", "det_refine", "twice_m", "seed", "modularity", "reduce"};
+      "compress", "def_update", "fill", "det_move", "det_refine", "twice_m", "seed", "modularity", "reduce",
+      "decouple_multi"};
   return (s >= 0 && s < P_COUNT) ? names[s] : "?";
 }
 
@@ -125,6 +126,7 @@ struct Profiler {
     pool.pop_back();
     return e;
   }
+  bool records(int sl) const { return on && (only < 0 || only == sl); }
   void reset() {
     for (int i = 0; i < P_COUNT; ++i) ms[i] = 0.0, launches[i] = 0, bytes[i] = 0;
   }

@@ -61,7 +61,6 @@ struct Engine {
   Temp tmp;
   ApplyScratch as;
   ProposeScratch ps;
-  DecoupleScratch dsc;
   DetMoveScratch dms;
   RefineScratch rs;
   AggScratch ag;
@@ -241,7 +240,6 @@ struct Engine {
     if (st && p < HL_MAX_LEVELS) {
       st->rounds[p] = rounds;
       st->evaluated[p] += ms_st.evaluated;
-      st->bytes_alg += 20 * ms_st.evaluated + 20 * ms_st.moved_rounds;
     }
     return rounds;
   }
@@ -294,6 +292,35 @@ struct Engine {
     }
   }
 
+  // Algorithmic bytes of the throughput proposal and decoupling kernels from
+  // the device work counters (deltas since the previous batch):
+  //   movement proposal, per (v, round): 20 + 16 deg(v) + 8 c(v)
+  //   refinement proposal, per singleton: 24 + 20 deg(v) + 16 c(v)
+  //   decoupling: 20 per candidate, 28 per accepted move, 12 per row entry
+  //   scanned for next-active marks
+  unsigned long long dlast[DS_COUNT] = {};
+  void account_work(const unsigned long long* dnow, hl_batch_stats* st) {
+    u64 d[DS_COUNT];
+    for (u32 i = 0; i < DS_COUNT; ++i) d[i] = dnow[i] - dlast[i], dlast[i] = dnow[i];
+    u64 prop[3], total = 0;
+    for (int b = 0; b < 3; ++b) {
+      prop[b] = 20 * d[DS_MOVE + b] + 16 * d[DS_MOVE + 3 + b] + 8 * d[DS_MOVE + 6 + b] + 24 * d[DS_REF + b] +
+                20 * d[DS_REF + 3 + b] + 16 * d[DS_REF + 6 + b];
+      total += prop[b];
+    }
+    const u64 dec = 20 * d[DS_DEC_CAND] + 28 * d[DS_DEC_ACC] + 12 * d[DS_DEC_DEG];
+    const u64 multi = 20 * d[DS_MULTI_CAND], apply = 28 * d[DS_MULTI_ACC], marks = 12 * d[DS_MULTI_DEG];
+    total += dec + multi + apply + marks;
+    if (st) st->bytes_alg += total;
+    if (prof.on) {
+      const int slots[7] = {P_PROP_SMALL, P_PROP_WAVE, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI, P_APPLY_MOVES,
+                            P_MARK_MOVED};
+      const u64 by[7] = {prop[0], prop[1], prop[2], dec, multi, apply, marks};
+      for (int i = 0; i < 7; ++i)
+        if (prof.records(slots[i])) prof.bytes[slots[i]] += by[i];
+    }
+  }
+
   // Algorithm 6 (hit_leiden.rs:936-1018) for a batch already in dcur[0..k)
   u64 run_batch(u64 k, hl_batch_stats* st) {
     u64 cur_len = k;
@@ -342,7 +369,12 @@ struct Engine {
     }
     def_update_maps(false);
     def_update_maps(true);
+    unsigned long long dnow[DS_COUNT];
+    const bool have_dstat = mvs.dstat.p != nullptr;
+    if (have_dstat)
+      HL_CHECK(hipMemcpyAsync(dnow, mvs.dstat.p, sizeof(dnow), hipMemcpyDeviceToHost, s));
     HL_CHECK(hipStreamSynchronize(s));
+    if (have_dstat) account_work(dnow, st);
     if (st) {
       st->iterations = total_iters;
       st->levels = u32(P);

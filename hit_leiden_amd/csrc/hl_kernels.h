@@ -115,22 +115,6 @@ struct Thresholds {
   u32 small_deg = 16;
   u32 wave_deg = 512;
 };
-// Throughput-mode proposals (parallel_frontier.rs:155-236) for active[0..cnt):
-// cto[i] = target community or kNone, cgain[i] = gain.  Returns the summed degree.
-u64 propose_moves(const Csr& g, const u32* f, const double* tot, double gamma, const u32* active, u64 cnt,
-                  u32* cto, double* cgain, ProposeScratch& ps, const Thresholds& th, Temp& tmp, hipStream_t s);
-
-struct DecoupleScratch {
-  DevBuf<u32> cidx, order, from, to, node, state, minUto, minUfrom, minAto, minAfrom, undec;
-  DevBuf<u64> keys, keys_s, cnt;
-  DevBuf<u8> flags;
-};
-// select_decoupled_moves (throughput.rs:204-236) as exact priority rounds;
-// applies the accepted moves, marks next-active and K.  Returns accepted count.
-u64 decouple_and_apply(const Csr& g, u32* f, const u32* s, double* tot, u64 comm_cap, const u32* active, u64 cnt,
-                       const u32* cto, const double* cgain, u8* B, u8* K, u8* next, DecoupleScratch& ds, Temp& tmp,
-                       hipStream_t st);
-
 // deterministic sweep (hit_leiden.rs:1108-1188) in one workgroup
 struct DetMoveScratch {
   DevBuf<u32> bm0, bm1, bm2, gkeys, status;

@@ -437,435 +437,6 @@ __global__ void k_heads_u32(const u32* keys, u64 n, u8* heads) {
   for (u64 i = gtid(); i < n; i += gstride()) heads[i] = (i == 0 || keys[i] != keys[i - 1]);
 }
 
-// ------------------------------------------------------------------ proposals
-// Throughput movement (execute_shard, parallel_frontier.rs:155-236): self-loops
-// skipped, empty community id = n + v.
-struct MovePolicy {
-  const u32* f;
-  const double* tot;
-  double gamma;
-  u32 n;
-  static constexpr bool kEmpty = true;
-  __device__ bool accept(u32 v, u32 u, u32& key, int& mult) const {
-    if (u == v) return false;
-    key = f[u];
-    mult = 1;
-    return true;
-  }
-  __device__ u32 cur_key(u32 v) const { return f[v]; }
-  __device__ double cur_total(u32 c) const { return tot[c]; }
-  __device__ bool ok(u32) const { return true; }
-  __device__ double cand_total(u32 c) const { return tot[c]; }
-  __device__ u32 empty_id(u32 v) const { return n + v; }
-};
-// Throughput refinement (try_refinement_move, throughput.rs:291-359): neighbours
-// in the same community keyed by subcommunity, self-loop entries counted twice,
-// blocked subcommunities skipped, no empty candidate.
-struct RefinePolicy {
-  const u32* f;
-  const u32* s;
-  const double* dS;
-  const u8* blocked;
-  double gamma;
-  static constexpr bool kEmpty = false;
-  __device__ bool accept(u32 v, u32 u, u32& key, int& mult) const {
-    if (f[u] != f[v]) return false;
-    key = s[u];
-    mult = (u == v) ? 2 : 1;
-    return true;
-  }
-  __device__ u32 cur_key(u32 v) const { return s[v]; }
-  __device__ double cur_total(u32 c) const { return dS[c]; }
-  __device__ bool ok(u32 c) const { return !blocked[c]; }
-  __device__ double cand_total(u32 c) const { return dS[c]; }
-  __device__ u32 empty_id(u32) const { return kNone; }
-};
-
-template <class P>
-__device__ __forceinline__ void finish_choice(const P& pol, u32 v, u32 cur, double wcur, double kv, double t,
-                                              double& bg, u32& best) {
-  if constexpr (P::kEmpty) {
-    double eg = empty_gain(wcur, t, pol.gamma, kv, pol.cur_total(cur));
-    if (eg > bg) {
-      bg = eg;
-      best = pol.empty_id(v);
-    }
-  }
-}
-
-// one thread per vertex, deg <= SMALL_MAX; per-thread LDS slice, row order sums
-template <class P>
-__global__ __launch_bounds__(256) void k_prop_small(P pol, Csr g, const u32* list, u32 cnt, const u32* verts,
-                                                     u32* cto, double* cgain) {
-  __shared__ u32 sk[SMALL_MAX * 256];
-  __shared__ double sv[SMALL_MAX * 256];
-  const u32 t = threadIdx.x;
-  const u64 id = gtid();
-  if (id >= cnt) return;
-  const u32 i = list[id];
-  const u32 v = verts[i];
-  const u64 off = g.row_off[v];
-  const u32 deg = g.row_len[v];
-  const u32 cur = pol.cur_key(v);
-  int nk = 0;
-  for (u32 j = 0; j < deg; ++j) {
-    u32 u = g.col[off + j];
-    double w = g.w[off + j];
-    u32 key;
-    int mult;
-    if (!pol.accept(v, u, key, mult)) continue;
-    int q = 0;
-    while (q < nk && sk[q * 256 + t] != key) ++q;
-    if (q == nk) {
-      sk[q * 256 + t] = key;
-      sv[q * 256 + t] = 0.0;
-      ++nk;
-    }
-    sv[q * 256 + t] += w;
-    if (mult == 2) sv[q * 256 + t] += w;
-  }
-  double wcur = 0.0;
-  for (int q = 0; q < nk; ++q)
-    if (sk[q * 256 + t] == cur) wcur = sv[q * 256 + t];
-  const double kv = g.k[v], tt = *g.twice_m, sc = pol.cur_total(cur);
-  double bg = 0.0;
-  u32 best = kNone;
-  for (int q = 0; q < nk; ++q) {
-    u32 key = sk[q * 256 + t];
-    if (key == cur || !pol.ok(key)) continue;
-    double gn = move_gain(sv[q * 256 + t], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
-    if (gn > 0.0) better(bg, best, gn, key);
-  }
-  finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
-  cto[i] = (bg > 0.0) ? best : kNone;
-  cgain[i] = bg;
-}
-
-__device__ __forceinline__ u32 lds_insert(u32* K, u32 cap, u32 key) {
-  u32 h = hash32(key) & (cap - 1);
-  while (true) {
-    u32 old = atomicCAS(&K[h], kNone, key);
-    if (old == kNone || old == key) return h;
-    h = (h + 1) & (cap - 1);
-  }
-}
-__device__ __forceinline__ double table_lookup(const u32* K, const double* V, u32 cap, u32 key) {
-  u32 h = hash32(key) & (cap - 1);
-  while (true) {
-    u32 x = K[h];
-    if (x == key) return V[h];
-    if (x == kNone) return 0.0;
-    h = (h + 1) & (cap - 1);
-  }
-}
-
-// one wavefront per vertex, deg <= WAVE_HW/2; LDS open-addressing table
-template <class P>
-__global__ __launch_bounds__(256) void k_prop_wave(P pol, Csr g, const u32* list, u32 cnt, const u32* verts,
-                                                    u32* cto, double* cgain) {
-  __shared__ u32 hk[4 * WAVE_HW];
-  __shared__ double hv[4 * WAVE_HW];
-  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  u32* K = hk + wv * WAVE_HW;
-  double* V = hv + wv * WAVE_HW;
-  for (u32 q = lane; q < WAVE_HW; q += 64) {
-    K[q] = kNone;
-    V[q] = 0.0;
-  }
-  __syncthreads();
-  const double tt = *g.twice_m;
-  for (u64 base = u64(blockIdx.x) * 4; base < cnt; base += u64(gridDim.x) * 4) {
-    const u64 id = base + wv;
-    const bool has = id < cnt;
-    u32 i = 0, v = 0, cur = 0, deg = 0;
-    u64 off = 0;
-    if (has) {
-      i = list[id];
-      v = verts[i];
-      off = g.row_off[v];
-      deg = g.row_len[v];
-      cur = pol.cur_key(v);
-      for (u32 j = lane; j < deg; j += 64) {
-        u32 u = g.col[off + j];
-        u32 key;
-        int mult;
-        if (!pol.accept(v, u, key, mult)) continue;
-        double w = g.w[off + j];
-        u32 h = lds_insert(K, WAVE_HW, key);
-        atomicAdd(&V[h], w);
-        if (mult == 2) atomicAdd(&V[h], w);
-      }
-    }
-    __syncthreads();
-    if (has) {
-      const double wcur = table_lookup(K, V, WAVE_HW, cur);
-      const double kv = g.k[v], sc = pol.cur_total(cur);
-      double bg = 0.0;
-      u32 best = kNone;
-      for (u32 q = lane; q < WAVE_HW; q += 64) {
-        u32 key = K[q];
-        if (key == kNone || key == cur || !pol.ok(key)) continue;
-        double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
-        if (gn > 0.0) better(bg, best, gn, key);
-      }
-      wave_argmax(bg, best);
-      if (lane == 0) {
-        finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
-        cto[i] = (bg > 0.0) ? best : kNone;
-        cgain[i] = bg;
-      }
-    }
-    __syncthreads();
-    if (has)
-      for (u32 q = lane; q < WAVE_HW; q += 64) {
-        K[q] = kNone;
-        V[q] = 0.0;
-      }
-    __syncthreads();
-  }
-}
-
-// one workgroup per vertex with a private global-memory table (hubs)
-template <class P>
-__global__ __launch_bounds__(256) void k_prop_big(P pol, Csr g, const u32* list, u32 cnt, const u32* verts,
-                                                   const u64* toff, u32* TK, double* TV, u32* cto, double* cgain) {
-  const u64 id = blockIdx.x;
-  if (id >= cnt) return;
-  const u32 i = list[id];
-  const u32 v = verts[i];
-  const u64 off = g.row_off[v];
-  const u32 deg = g.row_len[v];
-  const u32 cur = pol.cur_key(v);
-  const u32 cap = u32(toff[id + 1] - toff[id]);
-  u32* K = TK + toff[id];
-  double* V = TV + toff[id];
-  for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
-    K[q] = kNone;
-    V[q] = 0.0;
-  }
-  __syncthreads();
-  for (u32 j = threadIdx.x; j < deg; j += blockDim.x) {
-    u32 u = g.col[off + j];
-    u32 key;
-    int mult;
-    if (!pol.accept(v, u, key, mult)) continue;
-    double w = g.w[off + j];
-    u32 h = lds_insert(K, cap, key);
-    atomicAdd(&V[h], w);
-    if (mult == 2) atomicAdd(&V[h], w);
-  }
-  __threadfence();
-  __syncthreads();
-  const double tt = *g.twice_m;
-  const double wcur = table_lookup(K, V, cap, cur);
-  const double kv = g.k[v], sc = pol.cur_total(cur);
-  double bg = 0.0;
-  u32 best = kNone;
-  for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
-    u32 key = K[q];
-    if (key == kNone || key == cur || !pol.ok(key)) continue;
-    double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
-    if (gn > 0.0) better(bg, best, gn, key);
-  }
-  block_argmax(bg, best);
-  if (threadIdx.x == 0) {
-    finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
-    cto[i] = (bg > 0.0) ? best : kNone;
-    cgain[i] = bg;
-  }
-}
-
-// degree buckets: lists[0..cnt) small, lists[cnt..2cnt) wave, lists[2cnt..3cnt) big
-__global__ void k_bucket(const u32* verts, u64 cnt, const u32* row_len, u32 small_deg, u32 wave_deg, u32* lists,
-                         u32* counts, u64* degsum) {
-  u64 local[3] = {0, 0, 0};
-  for (u64 i = gtid(); i < cnt; i += gstride()) {
-    u32 d = row_len[verts[i]];
-    u32 b = d <= small_deg ? 0 : (d <= wave_deg ? 1 : 2);
-    local[b] += d;
-    u32 p = atomicAdd(&counts[b], 1u);
-    lists[b * cnt + p] = u32(i);
-  }
-  for (int b = 0; b < 3; ++b) {
-    u64 x = local[b];
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if ((threadIdx.x & 63) == 0 && x) atomicAdd((unsigned long long*)&degsum[b], (unsigned long long)x);
-  }
-}
-__global__ void k_big_sizes(const u32* list, u32 cnt, const u32* verts, const u32* row_len, u64* sizes) {
-  for (u64 j = gtid(); j < cnt; j += gstride()) {
-    u32 d = row_len[verts[list[j]]];
-    u64 c = 64;
-    while (c < 2ull * d) c <<= 1;
-    sizes[j] = c;
-  }
-}
-
-// ------------------------------------------------------------------ decoupling
-__global__ void k_flags_not_none(const u32* a, u64 n, u8* flags) {
-  for (u64 i = gtid(); i < n; i += gstride()) flags[i] = a[i] != kNone;
-}
-__global__ void k_gain_keys(const u32* cidx, u64 c, const double* cgain, u64* keys, u32* vals) {
-  for (u64 r = gtid(); r < c; r += gstride()) {
-    keys[r] = ~__double_as_longlong(cgain[cidx[r]]);  // gains > 0: descending order
-    vals[r] = u32(r);
-  }
-}
-__global__ void k_dec_init(const u32* order, const u32* cidx, u64 c, const u32* active, const u32* f, const u32* cto,
-                           u32* node, u32* from, u32* to, u32* state) {
-  for (u64 p = gtid(); p < c; p += gstride()) {
-    u32 i = cidx[order[p]];
-    u32 v = active[i];
-    node[p] = v;
-    from[p] = f[v];
-    to[p] = cto[i];
-    state[p] = kDecUndec;
-  }
-}
-__global__ void k_dec_reset(u64 c, const u32* from, const u32* to, u32* a, u32* b, u32* x, u32* y) {
-  for (u64 p = gtid(); p < c; p += gstride()) {
-    u32 f = from[p], t = to[p];
-    a[f] = kNone; a[t] = kNone;
-    b[f] = kNone; b[t] = kNone;
-    x[f] = kNone; x[t] = kNone;
-    y[f] = kNone; y[t] = kNone;
-  }
-}
-__global__ void k_dec_mark(u64 c, const u32* from, const u32* to, const u32* state, u32* minUto, u32* minUfrom,
-                           u32* minAto, u32* minAfrom) {
-  for (u64 p = gtid(); p < c; p += gstride()) {
-    u32 st = state[p];
-    if (st == kDecUndec) {
-      atomicMin(&minUto[to[p]], u32(p));
-      atomicMin(&minUfrom[from[p]], u32(p));
-    } else if (st == kDecAcc) {
-      atomicMin(&minAto[to[p]], u32(p));
-      atomicMin(&minAfrom[from[p]], u32(p));
-    }
-  }
-}
-// greedy (throughput.rs:217-233): reject when an earlier accepted move entered
-// our source or left our target; wait while an earlier undecided move could.
-__global__ void k_dec_decide(u64 c, const u32* from, const u32* to, u32* state, const u32* minUto,
-                             const u32* minUfrom, const u32* minAto, const u32* minAfrom, u32* undec) {
-  u32 local = 0;
-  for (u64 p = gtid(); p < c; p += gstride()) {
-    if (state[p] != kDecUndec) continue;
-    u32 f = from[p], t = to[p], pp = u32(p);
-    if (minAto[f] < pp || minAfrom[t] < pp)
-      state[p] = kDecRej;
-    else if (minUto[f] < pp || minUfrom[t] < pp)
-      ++local;
-    else
-      state[p] = kDecAcc;
-  }
-  for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
-  if ((threadIdx.x & 63) == 0 && local) atomicAdd(undec, local);
-}
-// All priority rounds inside one workgroup (small candidate sets): the same
-// reset / mark / decide steps separated by workgroup barriers, no host round trip.
-__global__ __launch_bounds__(1024) void k_dec_block(u64 c, const u32* from, const u32* to, u32* state, u32* minUto,
-                                                     u32* minUfrom, u32* minAto, u32* minAfrom, u32* undec_out) {
-  __shared__ u32 und;
-  for (int iter = 0; iter < 1 << 20; ++iter) {
-    for (u64 p = threadIdx.x; p < c; p += blockDim.x) {
-      u32 f = from[p], t = to[p];
-      minUto[f] = kNone; minUto[t] = kNone;
-      minUfrom[f] = kNone; minUfrom[t] = kNone;
-      minAto[f] = kNone; minAto[t] = kNone;
-      minAfrom[f] = kNone; minAfrom[t] = kNone;
-    }
-    if (threadIdx.x == 0) und = 0;
-    __threadfence();
-    __syncthreads();
-    for (u64 p = threadIdx.x; p < c; p += blockDim.x) {
-      u32 st = state[p];
-      if (st == kDecUndec) {
-        atomicMin(&minUto[to[p]], u32(p));
-        atomicMin(&minUfrom[from[p]], u32(p));
-      } else if (st == kDecAcc) {
-        atomicMin(&minAto[to[p]], u32(p));
-        atomicMin(&minAfrom[from[p]], u32(p));
-      }
-    }
-    __threadfence();
-    __syncthreads();
-    u32 local = 0;
-    for (u64 p = threadIdx.x; p < c; p += blockDim.x) {
-      if (state[p] != kDecUndec) continue;
-      u32 f = from[p], t = to[p], pp = u32(p);
-      if (atomicAdd(&minAto[f], 0u) < pp || atomicAdd(&minAfrom[t], 0u) < pp)
-        state[p] = kDecRej;
-      else if (atomicAdd(&minUto[f], 0u) < pp || atomicAdd(&minUfrom[t], 0u) < pp)
-        ++local;
-      else
-        state[p] = kDecAcc;
-    }
-    if (local) atomicAdd(&und, local);
-    __threadfence();
-    __syncthreads();
-    const u32 u = und;
-    __syncthreads();
-    if (u == 0) break;
-  }
-  if (threadIdx.x == 0) *undec_out = und;
-}
-
-// sequential completion of the greedy for pathological dependency chains
-__global__ void k_dec_sequential(u64 c, const u32* from, const u32* to, u32* state, u8* emit_acc) {
-  if (gtid() != 0) return;
-  for (u64 p = 0; p < c; ++p) {
-    u32 f = from[p], t = to[p];
-    if (state[p] == kDecUndec) state[p] = ((emit_acc[f] & 2) || (emit_acc[t] & 1)) ? kDecRej : kDecAcc;
-    if (state[p] == kDecAcc) {
-      emit_acc[f] |= 1;
-      emit_acc[t] |= 2;
-    }
-  }
-  for (u64 p = 0; p < c; ++p) {
-    emit_acc[from[p]] = 0;
-    emit_acc[to[p]] = 0;
-  }
-}
-__global__ void k_apply_moves(u64 c, const u32* node, const u32* from, const u32* to, const u32* state,
-                              const double* k, u32* f, double* tot, u8* B, u8* accflag) {
-  for (u64 p = gtid(); p < c; p += gstride()) {
-    bool acc = state[p] == kDecAcc;
-    accflag[p] = acc;
-    if (!acc) continue;
-    u32 v = node[p];
-    double kv = k[v];
-    f[v] = to[p];
-    atomicAdd(&tot[from[p]], -kv);
-    atomicAdd(&tot[to[p]], kv);
-    B[v] = 1;
-  }
-}
-// throughput.rs:189-199: after all moves, neighbours outside the new community
-// become active; same-subcommunity pairs go to K
-__global__ __launch_bounds__(256) void k_mark_moved(Csr g, const u32* acc_list, u64 na, const u32* node,
-                                                     const u32* to, const u32* f, const u32* s, u8* next, u8* K) {
-  const u32 lane = threadIdx.x & 63;
-  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
-  for (u64 q = wave; q < na; q += nwaves) {
-    u32 p = acc_list[q];
-    u32 v = node[p], c = to[p];
-    u64 off = g.row_off[v];
-    u32 deg = g.row_len[v];
-    u32 sv = s[v];
-    bool kv = false;
-    for (u32 j = lane; j < deg; j += 64) {
-      u32 u = g.col[off + j];
-      if (f[u] != c) next[u] = 1;
-      if (s[u] == sv) {
-        K[u] = 1;
-        kv = true;
-      }
-    }
-    if (__any(kv) && lane == 0) K[v] = 1;
-  }
-}
-
 // ------------------------------------------------------------------ deterministic sweep
 struct DetMoveArgs {
   Csr g;
@@ -918,6 +489,16 @@ __global__ void k_bm_summary(const u32* lo, u64 nlo, u32* hi, u64 nhi) {
   }
 }
 
+// Open-addressing lookup (linear probing); 0.0 for an absent key.
+__device__ __forceinline__ double table_lookup(const u32* K, const double* V, u32 cap, u32 key) {
+  u32 h = hash32(key) & (cap - 1);
+  while (true) {
+    u32 x = K[h];
+    if (x == key) return V[h];
+    if (x == kNone) return 0.0;
+    h = (h + 1) & (cap - 1);
+  }
+}
 // Sequential insert used by the deterministic sweeps (one thread): keeps the
 // per-key sums in row order.
 __device__ __forceinline__ u32 seq_insert(u32* K, u32 cap, u32 key, u32* used, u32& nused) {
@@ -1266,19 +847,6 @@ __global__ void k_blocked(u64 sn, const u8* is_cand, const double* dS, const dou
       }
     }
     blocked[c] = b;
-  }
-}
-__global__ void k_targets(const u32* rl, u64 nr, const u32* prop, u8* target) {
-  for (u64 i = gtid(); i < nr; i += gstride())
-    if (prop[i] != kNone) target[prop[i]] = 1;
-}
-__global__ void k_apply_refine(const u32* rl, u64 nr, const u32* prop, const u8* target, u32* s) {
-  for (u64 i = gtid(); i < nr; i += gstride()) {
-    u32 p = prop[i];
-    if (p == kNone) continue;
-    u32 v = rl[i];
-    if (target[s[v]]) continue;  // connected-targets rule
-    s[v] = p;
   }
 }
 __global__ void k_gather_list(const u32* sel, u64 c, const u32* from, u32* out) {
@@ -1882,141 +1450,6 @@ void label_totals_canonical(const u32* lab, const double* k, u32 n, double* tot,
   u64 nseg = select_flagged(heads.p, n, starts.p, tmp, cnt, s);
   hipLaunchKernelGGL(k_segment_sums, GRID(nseg), nseg, starts.p, u64(n), ks.p, vs.p, k, tot);
   HL_LAUNCH_CHECK();
-}
-
-template <class P>
-static u64 propose_impl(const P& pol, const Csr& g, const u32* verts, u64 cnt, u32* cto, double* cgain,
-                        ProposeScratch& ps, const Thresholds& th, Temp& tmp, hipStream_t s) {
-  if (cnt == 0) return 0;
-  ps.lists.reserve(3 * cnt, s);
-  ps.counts.reserve(4, s);
-  ps.degsum.reserve(3, s);
-  fill_u32(ps.counts.p, 0, 4, s);
-  HL_CHECK(hipMemsetAsync(ps.degsum.p, 0, 3 * sizeof(u64), s));
-  u32 small = th.small_deg > SMALL_MAX ? SMALL_MAX : th.small_deg;
-  u32 wave = th.wave_deg > WAVE_HW / 2 ? WAVE_HW / 2 : th.wave_deg;
-  hipLaunchKernelGGL(k_bucket, GRID(cnt), verts, cnt, g.row_len, small, wave, ps.lists.p, ps.counts.p,
-                     ps.degsum.p);
-  HL_LAUNCH_CHECK();
-  u32 counts[4];
-  u64 ds3[3] = {0, 0, 0};
-  HL_CHECK(hipMemcpyAsync(counts, ps.counts.p, 4 * sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipMemcpyAsync(ds3, ps.degsum.p, 3 * sizeof(u64), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
-  const u64 degsum = ds3[0] + ds3[1] + ds3[2];
-  if (counts[0]) {
-    HL_PROF(P_PROP_SMALL, 20ull * counts[0] + 16 * ds3[0], s);
-    hipLaunchKernelGGL(k_prop_small<P>, GRID(counts[0]), pol, g, ps.lists.p, counts[0], verts, cto, cgain);
-  }
-  HL_LAUNCH_CHECK();
-  if (counts[1]) {
-    HL_PROF(P_PROP_WAVE, 20ull * counts[1] + 16 * ds3[1], s);
-    unsigned nb = unsigned(std::min<u64>((counts[1] + 3) / 4, 8192));
-    hipLaunchKernelGGL(k_prop_wave<P>, dim3(nb), dim3(256), 0, s, pol, g, ps.lists.p + cnt, counts[1], verts, cto,
-                       cgain);
-    HL_LAUNCH_CHECK();
-  }
-  if (counts[2]) {
-    const u32* bl = ps.lists.p + 2 * cnt;
-    ps.bigsize.reserve(counts[2], s);
-    ps.bigoff.reserve(counts[2] + 1, s);
-    hipLaunchKernelGGL(k_big_sizes, GRID(counts[2]), bl, counts[2], verts, g.row_len, ps.bigsize.p);
-    HL_LAUNCH_CHECK();
-    u64 total = exclusive_scan_u64(ps.bigsize.p, ps.bigoff.p, counts[2], tmp, s);
-    ps.hkeys.reserve(total, s);
-    ps.hvals.reserve(total, s);
-    HL_PROF(P_PROP_BIG, 20ull * counts[2] + 16 * ds3[2], s);
-    hipLaunchKernelGGL(k_prop_big<P>, dim3(counts[2]), dim3(256), 0, s, pol, g, bl, counts[2], verts, ps.bigoff.p,
-                       ps.hkeys.p, ps.hvals.p, cto, cgain);
-    HL_LAUNCH_CHECK();
-  }
-  return degsum;
-}
-
-u64 propose_moves(const Csr& g, const u32* f, const double* tot, double gamma, const u32* active, u64 cnt, u32* cto,
-                  double* cgain, ProposeScratch& ps, const Thresholds& th, Temp& tmp, hipStream_t s) {
-  MovePolicy pol{f, tot, gamma, g.n};
-  return propose_impl(pol, g, active, cnt, cto, cgain, ps, th, tmp, s);
-}
-
-u64 decouple_and_apply(const Csr& g, u32* f, const u32* sm, double* tot, u64 comm_cap, const u32* active, u64 cnt,
-                       const u32* cto, const double* cgain, u8* B, u8* K, u8* next, DecoupleScratch& ds, Temp& tmp,
-                       hipStream_t s) {
-  ds.flags.reserve(cnt, s);
-  ds.cidx.reserve(cnt, s);
-  hipLaunchKernelGGL(k_flags_not_none, GRID(cnt), cto, cnt, ds.flags.p);
-  HL_LAUNCH_CHECK();
-  const u64 c = select_flagged(ds.flags.p, cnt, ds.cidx.p, tmp, ds.cnt, s);
-  if (c == 0) return 0;
-  ds.keys.reserve(c, s);
-  ds.keys_s.reserve(c, s);
-  ds.order.reserve(c, s);
-  ds.from.reserve(c, s);
-  ds.to.reserve(c, s);
-  ds.node.reserve(c, s);
-  ds.state.reserve(c, s);
-  ds.undec.reserve(2, s);
-  DevBuf<u32> vals;
-  vals.reserve(c, s);
-  hipLaunchKernelGGL(k_gain_keys, GRID(c), ds.cidx.p, c, cgain, ds.keys.p, vals.p);
-  HL_LAUNCH_CHECK();
-  sort_pairs_u64(ds.keys.p, ds.keys_s.p, vals.p, ds.order.p, c, 64, tmp, s);
-  hipLaunchKernelGGL(k_dec_init, GRID(c), ds.order.p, ds.cidx.p, c, active, f, cto, ds.node.p, ds.from.p, ds.to.p,
-                     ds.state.p);
-  HL_LAUNCH_CHECK();
-  if (ds.minUto.cap < comm_cap) {
-    ds.minUto.reserve(comm_cap, s);
-    ds.minUfrom.reserve(comm_cap, s);
-    ds.minAto.reserve(comm_cap, s);
-    ds.minAfrom.reserve(comm_cap, s);
-  }
-  int iters = 0;
-  if (c <= 16384) {
-    HL_PROF(P_DECOUPLE, 64 * c, s);
-    hipLaunchKernelGGL(k_dec_block, 1, 1024, 0, s, c, ds.from.p, ds.to.p, ds.state.p, ds.minUto.p, ds.minUfrom.p,
-                       ds.minAto.p, ds.minAfrom.p, ds.undec.p);
-    HL_LAUNCH_CHECK();
-  }
-  while (c > 16384) {
-    HL_PROF(P_DECOUPLE, 64 * c, s);
-    hipLaunchKernelGGL(k_dec_reset, GRID(c), c, ds.from.p, ds.to.p, ds.minUto.p, ds.minUfrom.p, ds.minAto.p,
-                       ds.minAfrom.p);
-    HL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_dec_mark, GRID(c), c, ds.from.p, ds.to.p, ds.state.p, ds.minUto.p, ds.minUfrom.p,
-                       ds.minAto.p, ds.minAfrom.p);
-    HL_LAUNCH_CHECK();
-    fill_u32(ds.undec.p, 0, 1, s);
-    hipLaunchKernelGGL(k_dec_decide, GRID(c), c, ds.from.p, ds.to.p, ds.state.p, ds.minUto.p, ds.minUfrom.p,
-                       ds.minAto.p, ds.minAfrom.p, ds.undec.p);
-    HL_LAUNCH_CHECK();
-    u32 und = 0;
-    HL_CHECK(hipMemcpyAsync(&und, ds.undec.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-    HL_CHECK(hipStreamSynchronize(s));
-    if (und == 0) break;
-    if (++iters >= 48) {
-      DevBuf<u8> ea;
-      ea.reserve(comm_cap, s);
-      fill_u8(ea.p, 0, comm_cap, s);
-      hipLaunchKernelGGL(k_dec_sequential, 1, 64, 0, s, c, ds.from.p, ds.to.p, ds.state.p, ea.p);
-      HL_LAUNCH_CHECK();
-      HL_CHECK(hipStreamSynchronize(s));
-      break;
-    }
-  }
-  ds.flags.reserve(c, s);
-  {
-  HL_PROF(P_APPLY_MOVES, 21 * c, s);
-  hipLaunchKernelGGL(k_apply_moves, GRID(c), c, ds.node.p, ds.from.p, ds.to.p, ds.state.p, g.k, f, tot, B,
-                     ds.flags.p);
-  HL_LAUNCH_CHECK();
-  }
-  DevBuf<u32> acc;
-  acc.reserve(c, s);
-  const u64 na = select_flagged(ds.flags.p, c, acc.p, tmp, ds.cnt, s);
-  HL_PROF(P_MARK_MOVED, 24 * na, s);
-  if (na) hipLaunchKernelGGL(k_mark_moved, GRID(na * 64), g, acc.p, na, ds.node.p, ds.to.p, f, sm, next, K);
-  HL_LAUNCH_CHECK();
-  return na;
 }
 
 int det_move(const Csr& g, u32* f, const u32* sm, double* tot, u64 tot_cap, u64* tot_len_dev, const u8* A,

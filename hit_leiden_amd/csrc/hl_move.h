@@ -73,7 +73,24 @@ struct RefinePolicyT {
   __device__ u32 empty_id(u32) const { return kNone; }
 };
 
+// Device work counters (monotone u64), read by the host once per batch.
+// A proposal block (DS_MOVE for movement, DS_REF for refinement) holds per
+// degree bucket b (0 lane, 1 wave, 2 workgroup): [b] vertices evaluated,
+// [3 + b] row entries read, [6 + b] distinct candidate communities.
+enum DevStat : u32 {
+  DS_MOVE = 0,
+  DS_REF = 9,
+  DS_DEC_CAND = 18,   // candidates decoupled in one workgroup
+  DS_DEC_ACC = 19,    // of which accepted
+  DS_DEC_DEG = 20,    // row entries scanned for next-active marks
+  DS_MULTI_CAND = 21, // the same for the multi-kernel path
+  DS_MULTI_ACC = 22,
+  DS_MULTI_DEG = 23,
+  DS_COUNT = 24
+};
+
 struct MoveScratch {
+  DevBuf<unsigned long long> dstat;
   DevBuf<u32> mark, list_a, list_b, blists, cnode, cto, counters;
   DevBuf<double> cgain;
   DevBuf<u32> minUto, minUfrom, minAto, minAfrom;

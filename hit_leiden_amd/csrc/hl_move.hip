@@ -130,9 +130,9 @@ __global__ void k_iota_list(u32* list, u32 n, u32* cnt) {
 
 // ------------------------------------------------------------------ buckets
 __global__ void k_bucket2(const u32* verts, const u32* cnt_dev, const u32* row_len, u32 small_deg, u32 wave_deg,
-                          u32* lists, u64 cap, u32* counts, unsigned long long* degsum) {
+                          u32* lists, u64 cap, u32* counts, unsigned long long* st) {
   const u32 cnt = *cnt_dev;
-  unsigned long long local[3] = {0, 0, 0};
+  unsigned long long local[3] = {0, 0, 0}, lcnt[3] = {0, 0, 0};
   for (u64 base = u64(blockIdx.x) * blockDim.x; base < cnt; base += gstride()) {
     u64 i = base + threadIdx.x;
     u32 b = 3;
@@ -142,6 +142,7 @@ __global__ void k_bucket2(const u32* verts, const u32* cnt_dev, const u32* row_l
       u32 d = row_len[v];
       b = d <= small_deg ? 0 : (d <= wave_deg ? 1 : 2);
       local[b] += d;
+      lcnt[b] += 1;
     }
     for (u32 q = 0; q < 3; ++q) {
       u32 p = wave_append(b == q, &counts[q]);
@@ -149,10 +150,21 @@ __global__ void k_bucket2(const u32* verts, const u32* cnt_dev, const u32* row_l
     }
   }
   for (int q = 0; q < 3; ++q) {
-    unsigned long long x = local[q];
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if ((threadIdx.x & 63) == 0 && x) atomicAdd(&degsum[q], x);
+    unsigned long long x = local[q], y = lcnt[q];
+    for (int o = 32; o > 0; o >>= 1) {
+      x += __shfl_xor(x, o, 64);
+      y += __shfl_xor(y, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && y) {
+      atomicAdd(&st[q], y);
+      atomicAdd(&st[3 + q], x);
+    }
   }
+}
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned long long x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst, x);
 }
 
 // ------------------------------------------------------------------ proposals
@@ -171,12 +183,13 @@ __device__ __forceinline__ void finish_choice(const P& pol, u32 v, u32 cur, doub
 // one thread per vertex (deg <= SMALL_MAX2), row-order sums in a private LDS slice
 template <class P>
 __global__ __launch_bounds__(256) void k_prop2_small(P pol, Csr g, const u32* list, const u32* cnt_dev,
-                                                      CandOut out) {
+                                                      CandOut out, unsigned long long* cst) {
   __shared__ u32 sk[SMALL_MAX2 * 256];
   __shared__ double sv[SMALL_MAX2 * 256];
   const u32 cnt = *cnt_dev;
   const u32 t = threadIdx.x;
   const double tt = *g.twice_m;
+  unsigned long long ncand = 0;
   for (u64 base = u64(blockIdx.x) * 256; base < cnt; base += gstride()) {
     const u64 id = base + t;
     double bg = 0.0;
@@ -203,6 +216,7 @@ __global__ __launch_bounds__(256) void k_prop2_small(P pol, Csr g, const u32* li
         sv[q * 256 + t] += w;
         if (mult == 2) sv[q * 256 + t] += w;
       }
+      ncand += nk;
       double wcur = 0.0;
       for (int q = 0; q < nk; ++q)
         if (sk[q * 256 + t] == cur) wcur = sv[q * 256 + t];
@@ -223,52 +237,62 @@ __global__ __launch_bounds__(256) void k_prop2_small(P pol, Csr g, const u32* li
       out.gain[p] = bg;
     }
   }
+  wave_add_u64(cst, ncand);
 }
 
-// one wavefront per vertex, LDS open-addressing table of WAVE_HW2 slots
+// one wavefront per vertex, LDS open-addressing table of up to WAVE_HW2 slots
+// (a power of two >= 2 deg, so only the slots a vertex can use are cleared)
 template <class P>
-__global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* list, const u32* cnt_dev, CandOut out) {
+__global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* list, const u32* cnt_dev, CandOut out,
+                                                     unsigned long long* cst) {
   __shared__ u32 hk[4 * WAVE_HW2];
   __shared__ double hv[4 * WAVE_HW2];
   const u32 cnt = *cnt_dev;
   const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   u32* K = hk + wv * WAVE_HW2;
   double* V = hv + wv * WAVE_HW2;
-  for (u32 q = lane; q < WAVE_HW2; q += 64) {
-    K[q] = kNone;
-    V[q] = 0.0;
-  }
-  __syncthreads();
   const double tt = *g.twice_m;
+  unsigned long long ncand = 0;
   for (u64 base = u64(blockIdx.x) * 4; base < cnt; base += u64(gridDim.x) * 4) {
     const u64 id = base + wv;
     const bool has = id < cnt;
-    u32 v = 0, cur = 0;
+    u32 v = 0, cur = 0, deg = 0, cap = 64;
+    u64 off = 0;
     if (has) {
       v = list[id];
-      const u64 off = g.row_off[v];
-      const u32 deg = g.row_len[v];
+      off = g.row_off[v];
+      deg = g.row_len[v];
       cur = pol.cur_key(v);
+      while (cap < 2 * deg) cap <<= 1;
+      for (u32 q = lane; q < cap; q += 64) {
+        K[q] = kNone;
+        V[q] = 0.0;
+      }
+    }
+    __syncthreads();
+    if (has) {
       for (u32 j = lane; j < deg; j += 64) {
         u32 u = g.col[off + j];
         u32 key;
         int mult;
         if (!pol.accept(v, u, key, mult)) continue;
         double w = g.w[off + j];
-        u32 h = tab_insert(K, WAVE_HW2, key);
+        u32 h = tab_insert(K, cap, key);
         atomicAdd(&V[h], w);
         if (mult == 2) atomicAdd(&V[h], w);
       }
     }
     __syncthreads();
     if (has) {
-      const double wcur = tab_lookup(K, V, WAVE_HW2, cur);
+      const double wcur = tab_lookup(K, V, cap, cur);
       const double kv = g.k[v], sc = pol.cur_total(cur);
       double bg = 0.0;
       u32 best = kNone;
-      for (u32 q = lane; q < WAVE_HW2; q += 64) {
+      for (u32 q = lane; q < cap; q += 64) {
         u32 key = K[q];
-        if (key == kNone || key == cur || !pol.ok(key)) continue;
+        if (key == kNone) continue;
+        ++ncand;
+        if (key == cur || !pol.ok(key)) continue;
         double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
         if (gn > 0.0) better(bg, best, gn, key);
       }
@@ -284,20 +308,16 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
       }
     }
     __syncthreads();
-    if (has)
-      for (u32 q = lane; q < WAVE_HW2; q += 64) {
-        K[q] = kNone;
-        V[q] = 0.0;
-      }
-    __syncthreads();
   }
+  wave_add_u64(cst, ncand);
 }
 
 // one workgroup per vertex (hubs); each resident workgroup owns a table of tcap slots
 template <class P>
 __global__ __launch_bounds__(256) void k_prop2_big(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* TK,
-                                                    double* TV, u32 tcap, CandOut out) {
+                                                    double* TV, u32 tcap, CandOut out, unsigned long long* cst) {
   const u32 cnt = *cnt_dev;
+  unsigned long long ncand = 0;
   u32* K = TK + u64(blockIdx.x) * tcap;
   double* V = TV + u64(blockIdx.x) * tcap;
   const double tt = *g.twice_m;
@@ -332,7 +352,9 @@ __global__ __launch_bounds__(256) void k_prop2_big(P pol, Csr g, const u32* list
     u32 best = kNone;
     for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
       u32 key = K[q];
-      if (key == kNone || key == cur || !pol.ok(key)) continue;
+      if (key == kNone) continue;
+      ++ncand;
+      if (key == cur || !pol.ok(key)) continue;
       double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
       if (gn > 0.0) better(bg, best, gn, key);
     }
@@ -348,6 +370,7 @@ __global__ __launch_bounds__(256) void k_prop2_big(P pol, Csr g, const u32* list
     }
     __syncthreads();
   }
+  wave_add_u64(cst, ncand);
 }
 
 // ------------------------------------------------------------------ decoupling (one workgroup)
@@ -356,7 +379,7 @@ __device__ __forceinline__ bool pri_less(u64 ka, u32 na, u64 kb, u32 nb) { retur
 
 __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32* minUto, u32* minUfrom,
                                                      u32* minAto, u32* minAfrom, u32* mark, u32 epoch, u32* next,
-                                                     u32* next_cnt, u32* status) {
+                                                     u32* next_cnt, u32* status, unsigned long long* dst) {
   __shared__ u64 skey[kDecSmall];
   __shared__ u32 snode[kDecSmall];
   __shared__ u32 sidx[kDecSmall];
@@ -476,11 +499,13 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
   __syncthreads();
   // next-active and K marks (throughput.rs:189-199)
   const u32 lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  unsigned long long scanned = 0;
   for (u32 p = wv; p < c; p += nw) {
     if (sst[p] != kAcc) continue;
     const u32 v = snode[p], cto = sto[p];
     const u64 off = m.g.row_off[v];
     const u32 deg = m.g.row_len[v];
+    if (lane == 0) scanned += deg;
     const u32 sv = m.s[v];
     bool kv = false;
     for (u32 j0 = 0; j0 < deg; j0 += 64) {
@@ -500,11 +525,14 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
     }
     if (__any(kv) && lane == 0) m.K[v] = 1;
   }
+  if (lane == 0 && scanned) atomicAdd(&dst[DS_DEC_DEG], scanned);
   __syncthreads();
   if (tid == 0) {
     status[0] = 0;
     status[1] = nacc;
     *cand.cnt = 0;
+    atomicAdd(&dst[DS_DEC_CAND], (unsigned long long)c);
+    atomicAdd(&dst[DS_DEC_ACC], (unsigned long long)nacc);
   }
 }
 
@@ -519,7 +547,8 @@ __global__ void k_cand_gain_keys(const u32* order1, u32 c, CandOut cand, u64* ke
   for (u64 i = gtid(); i < c; i += gstride()) keys[i] = ~(u64)__double_as_longlong(cand.gain[order1[i]]);
 }
 __global__ void k_cand_arrange(const u32* order2, const u32* order1, u32 c, CandOut cand, const u32* f, u32* node,
-                               u32* from, u32* to, u32* state) {
+                               u32* from, u32* to, u32* state, unsigned long long* dst) {
+  if (gtid() == 0) atomicAdd(&dst[DS_MULTI_CAND], (unsigned long long)c);
   for (u64 p = gtid(); p < c; p += gstride()) {
     u32 i = order1[order2[p]];
     u32 v = cand.node[i];
@@ -583,12 +612,15 @@ __global__ void k_seq_finish(u64 c, const u32* from, const u32* to, u32* state, 
   }
 }
 __global__ void k_apply4(u64 c, const u32* node, const u32* from, const u32* to, const u32* state, MoveCtx m,
-                         u32* acc_list, u32* acc_cnt) {
+                         u32* acc_list, u32* acc_cnt, unsigned long long* dst) {
+  unsigned long long na = 0, nd = 0;
   for (u64 base = u64(blockIdx.x) * blockDim.x; base < c; base += gstride()) {
     u64 p = base + threadIdx.x;
     bool acc = p < c && state[p] == kAcc;
     if (acc) {
       u32 v = node[p];
+      ++na;
+      nd += m.g.row_len[v];
       double kv = m.k[v];
       m.f[v] = to[p];
       atomicAdd(&m.tot[from[p]], -kv);
@@ -598,6 +630,8 @@ __global__ void k_apply4(u64 c, const u32* node, const u32* from, const u32* to,
     u32 q = wave_append(acc, acc_cnt);
     if (acc) acc_list[q] = u32(p);
   }
+  wave_add_u64(&dst[DS_MULTI_ACC], na);
+  wave_add_u64(&dst[DS_MULTI_DEG], nd);
 }
 __global__ __launch_bounds__(256) void k_marks4(MoveCtx m, const u32* acc_list, const u32* acc_cnt, const u32* node,
                                                  const u32* to, u32* mark, u32 epoch, u32* next, u32* next_cnt) {
@@ -659,6 +693,10 @@ static unsigned fixed_grid(u64 work, unsigned bs = 256, unsigned cap = 2048) {
 }
 
 void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
+  if (!dstat.p) {
+    dstat.reserve(DS_COUNT, s);
+    HL_CHECK(hipMemsetAsync(dstat.p, 0, DS_COUNT * sizeof(unsigned long long), s));
+  }
   if (mark.cap < n + 1) {
     mark.reserve(n + 1, s, true);
     // new entries must not equal a live epoch: zero the whole array, restart epochs
@@ -690,25 +728,24 @@ u32 MoveScratch::next_epoch(hipStream_t s) {
 
 template <class P>
 static void propose_round(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u64 list_cap, u32 maxdeg,
-                          MoveScratch& ms, const Thresholds& th, u64* degsum_out, hipStream_t s) {
+                          MoveScratch& ms, const Thresholds& th, unsigned long long* st, hipStream_t s) {
   u32* counts = ms.counters.p + 4;  // [4..7) bucket counts
-  unsigned long long* degsum = (unsigned long long*)(ms.counters.p + 8);  // [8..14) 3 x u64
-  HL_CHECK(hipMemsetAsync(counts, 0, 4 * sizeof(u32) + 3 * sizeof(u64), s));
+  HL_CHECK(hipMemsetAsync(counts, 0, 4 * sizeof(u32), s));
   const u32 small = th.small_deg > SMALL_MAX2 ? SMALL_MAX2 : th.small_deg;
   const u32 wave = th.wave_deg > WAVE_HW2 / 2 ? WAVE_HW2 / 2 : th.wave_deg;
   hipLaunchKernelGGL(k_bucket2, fixed_grid(list_cap), 256, 0, s, list, cnt_dev, g.row_len, small, wave,
-                     ms.blists.p, list_cap, counts, degsum);
+                     ms.blists.p, list_cap, counts, st);
   HL_LAUNCH_CHECK();
   CandOut out{ms.cnode.p, ms.cto.p, ms.cgain.p, ms.counters.p + 1};
   {
     HL_PROF(P_PROP_SMALL, 0, s);
-    hipLaunchKernelGGL(k_prop2_small<P>, fixed_grid(list_cap), 256, 0, s, pol, g, ms.blists.p, counts, out);
+    hipLaunchKernelGGL(k_prop2_small<P>, fixed_grid(list_cap), 256, 0, s, pol, g, ms.blists.p, counts, out, st + 6);
     HL_LAUNCH_CHECK();
   }
   {
     HL_PROF(P_PROP_WAVE, 0, s);
     hipLaunchKernelGGL(k_prop2_wave<P>, dim3(fixed_grid(list_cap, 4, 4096)), dim3(256), 0, s, pol, g,
-                       ms.blists.p + list_cap, counts + 1, out);
+                       ms.blists.p + list_cap, counts + 1, out, st + 7);
     HL_LAUNCH_CHECK();
   }
   if (maxdeg > wave) {
@@ -719,10 +756,9 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
     ms.tk.reserve(u64(nb) * tcap, s);
     ms.tv.reserve(u64(nb) * tcap, s);
     hipLaunchKernelGGL(k_prop2_big<P>, dim3(nb), dim3(256), 0, s, pol, g, ms.blists.p + 2 * list_cap, counts + 2,
-                       ms.tk.p, ms.tv.p, tcap, out);
+                       ms.tk.p, ms.tv.p, tcap, out, st + 8);
     HL_LAUNCH_CHECK();
   }
-  (void)degsum_out;
 }
 
 u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int max_rounds, MoveScratch& ms,
@@ -753,14 +789,14 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
   u32 rounds = 0;
   while (host_cnt > 0 && int(rounds) < max_rounds) {
     if (st) st->evaluated += host_cnt;
-    propose_round(pol, m.g, cur, cnt_a, host_cnt, maxdeg, ms, th, nullptr, s);
+    propose_round(pol, m.g, cur, cnt_a, host_cnt, maxdeg, ms, th, ms.dstat.p + DS_MOVE, s);
     const u32 ep = ms.next_epoch(s);
     HL_CHECK(hipMemsetAsync(next_cnt, 0, sizeof(u32), s));
     CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
     {
       HL_PROF(P_DECOUPLE, 0, s);
       hipLaunchKernelGGL(k_dec_small, 1, 1024, 0, s, m, cand, ms.minUto.p, ms.minUfrom.p, ms.minAto.p,
-                         ms.minAfrom.p, ms.mark.p, ep, nxt, next_cnt, status);
+                         ms.minAfrom.p, ms.mark.p, ep, nxt, next_cnt, status, ms.dstat.p);
       HL_LAUNCH_CHECK();
     }
     u32 h[4];
@@ -792,11 +828,11 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
       iota_u32(ms.v32.p, 0, c, s);
       sort_pairs_u64(ms.k64.p, ms.k64b.p, ms.v32.p, ms.o2.p, c, 64, tmp, s);
       hipLaunchKernelGGL(k_cand_arrange, fixed_grid(c), 256, 0, s, ms.o2.p, ms.o1.p, c, cand, m.f, ms.node.p,
-                         ms.from.p, ms.to.p, ms.state.p);
+                         ms.from.p, ms.to.p, ms.state.p, ms.dstat.p);
       HL_LAUNCH_CHECK();
       int iters = 0;
       while (true) {
-        HL_PROF(P_DECOUPLE, 0, s);
+        HL_PROF(P_DEC_MULTI, 0, s);
         hipLaunchKernelGGL(k_reset4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.minUto.p,
                            ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p);
         hipLaunchKernelGGL(k_mark4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p, ms.minUto.p,
@@ -824,7 +860,7 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
       {
         HL_PROF(P_APPLY_MOVES, 0, s);
         hipLaunchKernelGGL(k_apply4, fixed_grid(c), 256, 0, s, u64(c), ms.node.p, ms.from.p, ms.to.p, ms.state.p, m,
-                           ms.acc.p, acc_cnt);
+                           ms.acc.p, acc_cnt, ms.dstat.p);
         HL_LAUNCH_CHECK();
       }
       {
@@ -862,7 +898,7 @@ void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, con
   HL_CHECK(hipMemcpyAsync(cnt, &ns, sizeof(u32), hipMemcpyHostToDevice, s));
   HL_CHECK(hipMemsetAsync(ccnt, 0, sizeof(u32), s));
   RefinePolicyT pol{f, sm, dS, blocked, gamma};
-  propose_round(pol, g, singles, cnt, ns, maxdeg, ms, th, nullptr, s);
+  propose_round(pol, g, singles, cnt, ns, maxdeg, ms, th, ms.dstat.p + DS_REF, s);
   CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
   HL_PROF(P_REFINE, 0, s);
   hipLaunchKernelGGL(k_ref_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);

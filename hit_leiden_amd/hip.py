@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 
 _hip = None
-H2D, D2H = 1, 2
+H2D, D2H, D2D = 1, 2, 3
 
 
 def hip():
@@ -61,6 +61,12 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def copy_d2d(dst: DeviceArray, src: DeviceArray, nbytes: int):
+    if nbytes > min(dst.nbytes, src.nbytes):
+        raise ValueError("copy larger than the buffers")
+    _ok(hip().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy")
 
 
 def synchronize():
