@@ -13,6 +13,7 @@
 namespace hl {
 
 using u8 = uint8_t;
+using u16 = uint16_t;
 using u32 = uint32_t;
 using u64 = uint64_t;
 
@@ -27,7 +28,9 @@ constexpr double kEpsApply = 1e-12;    // in_memory.rs:110
 constexpr double kEpsSelf = 1e-12;     // hit_leiden.rs:1687
 constexpr double kEpsCompress = 1e-9;  // hit_leiden.rs:1711
 constexpr double kEpsTFilter = 1e-9;   // hit_leiden.rs:1547,1631
-constexpr u32 kSumChunk = 4096;        // canonical chunk of the 2m sum
+// Same-address device atomics serialise (about 11 ns each on MI355X), so a
+// kernel that ends in one atomic per workgroup keeps its grid near this size.
+constexpr u32 kReduceBlocks = 256;
 
 struct HipError : std::runtime_error {
   hipError_t code;
@@ -48,6 +51,14 @@ struct InputError : std::runtime_error {
   } while (0)
 
 #define HL_LAUNCH_CHECK() HL_CHECK(hipGetLastError())
+
+// Small device-to-host readbacks (counts, maxima, flags) go through a pinned
+// staging area: with a pageable destination hipMemcpyAsync waits for its own
+// round trip.  d2h_async enqueues the copy on the stream; stream_sync waits for
+// the stream and then delivers that stream's staged values to their
+// destinations, which must stay alive until then.
+void d2h_async(void* dst, const void* src, size_t bytes, hipStream_t s);
+void stream_sync(hipStream_t s);
 
 // Grow-only device buffer.  reserve(n, keep) reallocates when the capacity is
 // short; keep copies the old contents (stream-ordered).
@@ -71,7 +82,7 @@ struct DevBuf {
     HL_CHECK(hipMalloc(&np, nc * sizeof(T)));
     if (keep && p && cap) HL_CHECK(hipMemcpyAsync(np, p, cap * sizeof(T), hipMemcpyDeviceToDevice, s));
     if (p) {
-      HL_CHECK(hipStreamSynchronize(s));
+      stream_sync(s);
       HL_CHECK(hipFree(p));
     }
     p = np;

@@ -33,6 +33,7 @@ struct Level {
   DevBuf<double> tot;
   bool tot_valid = false;
   u64 lab_bound = 0;  // every f label < lab_bound
+  u64 s_bound = 0;    // every s_cur / s_pre label < s_bound
   u32 deg_bound = 0;  // every row length <= deg_bound
 
   Csr csr() const { return Csr{n, row_off.p, row_len.p, col.p, w.p, k.p, twice_m.p}; }
@@ -66,7 +67,7 @@ struct Engine {
   AggScratch ag;
   MoveScratch mvs;
   DevBuf<double> tot, partials, sig, acc;
-  DevBuf<u64> tot_len, cnt, offtmp;
+  DevBuf<u64> tot_len, cnt, offtmp, lvcnt;
   DevBuf<u32> active, cto, mx, lab, lab2, first;
   DevBuf<double> cgain;
   DevBuf<u8> next, flag8;
@@ -80,13 +81,23 @@ struct Engine {
     if (s) (void)hipStreamDestroy(s);
   }
 
+  // maxima of up to four u32 arrays of length n with one synchronisation
+  void dmax4(const u32* const a[4], u64 n, u32 out[4]) {
+    mx.reserve(4, s);
+    fill_u32(mx.p, 0, 4, s);
+    for (int i = 0; i < 4; ++i)
+      if (a[i]) max_u32(a[i], n, mx.p + i, s);
+    hl::d2h_async(out, mx.p, 4 * sizeof(u32), s);
+    hl::stream_sync(s);
+  }
+
   u32 dmax(const u32* a, u64 n) {
     mx.reserve(1, s);
     fill_u32(mx.p, 0, 1, s);
     max_u32(a, n, mx.p, s);
     u32 h = 0;
-    HL_CHECK(hipMemcpyAsync(&h, mx.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-    HL_CHECK(hipStreamSynchronize(s));
+    hl::d2h_async(&h, mx.p, sizeof(u32), s);
+    hl::stream_sync(s);
     return h;
   }
 
@@ -113,9 +124,12 @@ struct Engine {
     if (n_new <= n_old) return;
     u64 bf = n_old, bg = n_old, bs = n_old;
     if (n_old) {
-      bf = std::max<u64>(bf, u64(dmax(lv.f.p, n_old)) + 1);
-      bg = std::max<u64>(bg, u64(dmax(lv.g.p, n_old)) + 1);
-      bs = std::max<u64>(bs, u64(std::max(dmax(lv.s_cur.p, n_old), dmax(lv.s_pre.p, n_old))) + 1);
+      const u32* arrs[4] = {lv.f.p, lv.g.p, lv.s_cur.p, lv.s_pre.p};
+      u32 h[4];
+      dmax4(arrs, n_old, h);
+      bf = std::max<u64>(bf, u64(h[0]) + 1);
+      bg = std::max<u64>(bg, u64(h[1]) + 1);
+      bs = std::max<u64>(bs, u64(std::max(h[2], h[3])) + 1);
     }
     if (p + 1 < P) bs = std::max<u64>(bs, L[p + 1]->n);
     if (bf + (n_new - n_old) > 0xFFFFFFF0ull || bs + (n_new - n_old) > 0xFFFFFFF0ull)
@@ -125,6 +139,7 @@ struct Engine {
               lv.row_len.p, lv.row_cap.p, lv.k.p, s);
     lv.n = n_new;
     lv.lab_bound = std::max<u64>(lv.lab_bound, bf + (n_new - n_old));
+    lv.s_bound = std::max<u64>(lv.s_bound, bs + (n_new - n_old));
   }
 
   bool par() const { return mode == HL_MODE_THROUGHPUT; }
@@ -222,8 +237,8 @@ struct Engine {
       }
       if (st && p < HL_MAX_LEVELS) {
         u64 pops = 0;
-        HL_CHECK(hipMemcpyAsync(&pops, dms.pops.p, sizeof(u64), hipMemcpyDeviceToHost, s));
-        HL_CHECK(hipStreamSynchronize(s));
+        hl::d2h_async(&pops, dms.pops.p, sizeof(u64), s);
+        hl::stream_sync(s);
         st->evaluated[p] += pops;
         st->rounds[p] = 1;
       }
@@ -249,15 +264,16 @@ struct Engine {
     const u32 n = lv.n;
     if (n == 0) return;
     Csr g = lv.csr();
-    const u64 sspace = u64(dmax(lv.s_cur.p, n)) + 1;
-    cc_split(g, lv.s_cur.p, lv.K.p, lv.R.p, sspace, rs, tmp, s);
-    if (is_identity(lv.s_cur.p, n, rs, s)) fill_u8(lv.R.p, 1, n, s);
+    bool identity = false;
+    const u64 sn = cc_split(g, lv.s_cur.p, lv.K.p, lv.R.p, std::max<u64>(lv.s_bound, 1), &identity, rs, tmp, s);
+    lv.s_bound = std::max(lv.s_bound, sn);
+    if (identity) fill_u8(lv.R.p, 1, n, s);
     const u64 cs = comm_space(lv);
     if (mode == HL_MODE_DETERMINISTIC)
-      refine_det(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, rs, ks, vs, vidx, heads, starts, cnt,
+      refine_det(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, sn, rs, ks, vs, vidx, heads, starts, cnt,
                  max_degree(lv), tmp, s);
     else
-      refine_parallel(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, rs, ps, mvs, max_degree(lv), th, tmp,
+      refine_parallel(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, sn, rs, ps, mvs, max_degree(lv), th, tmp,
                       lv.tot_valid ? lv.tot.p : nullptr, s);
     (void)st;
     (void)p;
@@ -323,6 +339,7 @@ struct Engine {
 
   // Algorithm 6 (hit_leiden.rs:936-1018) for a batch already in dcur[0..k)
   u64 run_batch(u64 k, hl_batch_stats* st) {
+    lvcnt.reserve(4 * HL_MAX_LEVELS, s);
     u64 cur_len = k;
     u32 cur_n = n0;
     u64 total_iters = 0;
@@ -338,20 +355,15 @@ struct Engine {
       lv.iters = it;
       total_iters += it;
       refinement(lv, st, p);
-      u64 c4[4] = {0, 0, 0, 0};
-      count4_u8(st ? lv.A.p : nullptr, st ? lv.B.p : nullptr, st ? lv.K.p : nullptr, lv.R.p, lv.n, cnt, c4, s);
       if (st && p < HL_MAX_LEVELS) {
+        // frontier sizes, read back with the batch's last synchronisation
+        count4_u8_async(lv.A.p, lv.B.p, lv.K.p, lv.R.p, lv.n, lvcnt.p + 4 * p, s);
         st->level_n[p] = lv.n;
         st->delta_len[p] = cur_len;
-        st->active[p] = c4[0];
-        st->moved[p] = c4[1];
-        st->affected[p] = c4[2];
-        st->refined[p] = c4[3];
-        st->bytes_alg += 24 * cur_len + 12 * st->moved[p];
+        st->bytes_alg += 24 * cur_len;
       }
       if (p < P - 1) {
-        u64 nR = c4[3];
-        if (cur_len == 0 && nR == 0) {
+        if (cur_len == 0 && count_u8(lv.R.p, lv.n, cnt, s) == 0) {
           // should_skip_aggregation (hit_leiden.rs:1020-1022)
           u32 a = lv.n ? dmax(lv.s_cur.p, lv.n) : 0;
           u32 b = lv.n ? dmax(lv.s_pre.p, lv.n) : 0;
@@ -372,9 +384,20 @@ struct Engine {
     unsigned long long dnow[DS_COUNT];
     const bool have_dstat = mvs.dstat.p != nullptr;
     if (have_dstat)
-      HL_CHECK(hipMemcpyAsync(dnow, mvs.dstat.p, sizeof(dnow), hipMemcpyDeviceToHost, s));
-    HL_CHECK(hipStreamSynchronize(s));
+      hl::d2h_async(dnow, mvs.dstat.p, sizeof(dnow), s);
+    u64 c4[4 * HL_MAX_LEVELS];
+    const int np = std::min(P, HL_MAX_LEVELS);
+    if (st) hl::d2h_async(c4, lvcnt.p, 4 * np * sizeof(u64), s);
+    hl::stream_sync(s);
     if (have_dstat) account_work(dnow, st);
+    if (st)
+      for (int p = 0; p < np; ++p) {
+        st->active[p] = c4[4 * p];
+        st->moved[p] = c4[4 * p + 1];
+        st->affected[p] = c4[4 * p + 2];
+        st->refined[p] = c4[4 * p + 3];
+        st->bytes_alg += 12 * st->moved[p];
+      }
     if (st) {
       st->iterations = total_iters;
       st->levels = u32(P);
@@ -558,7 +581,7 @@ hl_status hl_load_graph(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src
     e.upload_batch(m, src, dst, w, n);
     e.reset(n, m);
     e.apply_delta_level(*e.L[0], e.dcur.p, m, nullptr, 0);
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::stream_sync(e.s);
   });
 }
 
@@ -621,8 +644,8 @@ hl_status hl_apply_batch_device(hl_graph* g, uint64_t k, const uint32_t* d_src, 
                          e.mx.p);
     HL_LAUNCH_CHECK();
     uint32_t bad = 0;
-    HL_CHECK(hipMemcpyAsync(&bad, e.mx.p, sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::d2h_async(&bad, e.mx.p, sizeof(uint32_t), e.s);
+    hl::stream_sync(e.s);
     if (bad) throw hl::InputError("endpoint id out of range or non-finite weight");
     e.run_batch(k, stats);
     if (stats) stats->seconds = now_s() - t0;
@@ -650,7 +673,7 @@ hl_status hl_read_partition(const hl_graph* g, int32_t level, int32_t which, uin
     if (cap < lv.n) throw hl::InputError("output buffer too small");
     if (lv.n)
       HL_CHECK(hipMemcpyAsync(out, e.map_of(lv, which).p, lv.n * sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::stream_sync(e.s);
   });
 }
 
@@ -666,8 +689,10 @@ hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const ui
       // caller-written labels: rebuild the maintained totals on next use
       lv.tot_valid = false;
       lv.lab_bound = n ? std::max<uint64_t>(lv.n, uint64_t(e.dmax(lv.f.p, n)) + 1) : 0;
+    } else if ((which == HL_MAP_S_CUR || which == HL_MAP_S_PRE) && n) {
+      lv.s_bound = std::max<uint64_t>(lv.s_bound, uint64_t(e.dmax(e.map_of(lv, which).p, n)) + 1);
     }
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::stream_sync(e.s);
   });
 }
 
@@ -679,7 +704,7 @@ hl_status hl_read_frontier(const hl_graph* g, int32_t level, int32_t which, uint
     hl::Level& lv = const_cast<hl::Level&>(e.level(level));
     if (cap < lv.n) throw hl::InputError("output buffer too small");
     if (lv.n) HL_CHECK(hipMemcpyAsync(out, e.set_of(lv, which).p, lv.n, hipMemcpyDeviceToHost, e.s));
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::stream_sync(e.s);
   });
 }
 
@@ -693,7 +718,7 @@ hl_status hl_read_level_graph(const hl_graph* g, int32_t level, uint64_t* offset
     const uint32_t n = lv.n;
     std::vector<uint32_t> len(n);
     if (n) HL_CHECK(hipMemcpyAsync(len.data(), lv.row_len.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::stream_sync(e.s);
     uint64_t total = 0;
     for (uint32_t v = 0; v < n; ++v) total += len[v];
     *nnz = total;
@@ -718,7 +743,7 @@ hl_status hl_read_level_graph(const hl_graph* g, int32_t level, uint64_t* offset
     HL_CHECK(hipMemcpyAsync(hoff.data(), offs.p, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, e.s));
     std::vector<uint64_t> roff(n);
     if (n) HL_CHECK(hipMemcpyAsync(roff.data(), m.row_off, n * sizeof(uint64_t), hipMemcpyDeviceToHost, e.s));
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::stream_sync(e.s);
     for (uint32_t v = 0; v < n; ++v) {
       offsets[v] = hoff[v];
       if (len[v]) {
@@ -728,7 +753,7 @@ hl_status hl_read_level_graph(const hl_graph* g, int32_t level, uint64_t* offset
       }
     }
     offsets[n] = total;
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::stream_sync(e.s);
   });
 }
 
@@ -746,7 +771,7 @@ hl_status hl_read_level_delta(const hl_graph* g, int32_t level, uint32_t* u, uin
     std::vector<hl::Rec> h(lv.dG_len);
     if (lv.dG_len)
       HL_CHECK(hipMemcpyAsync(h.data(), lv.dG.p, lv.dG_len * sizeof(hl::Rec), hipMemcpyDeviceToHost, e.s));
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::stream_sync(e.s);
     for (uint64_t i = 0; i < lv.dG_len; ++i) {
       u[i] = h[i].u;
       v[i] = h[i].v;
@@ -762,7 +787,7 @@ hl_status hl_twice_m(const hl_graph* g, int32_t level, double* twice_m) {
     HL_CHECK(hipSetDevice(e.cfg.device));
     hl::Level& lv = const_cast<hl::Level&>(e.level(level));
     HL_CHECK(hipMemcpyAsync(twice_m, lv.twice_m.p, sizeof(double), hipMemcpyDeviceToHost, e.s));
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::stream_sync(e.s);
   });
 }
 
@@ -787,7 +812,7 @@ hl_status hl_read_hierarchy(const hl_graph* g, uint32_t* out, uint64_t cap, int3
       }
       if (n) HL_CHECK(hipMemcpyAsync(out + uint64_t(p) * n, e.lab.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
     }
-    HL_CHECK(hipStreamSynchronize(e.s));
+    hl::stream_sync(e.s);
   });
 }
 

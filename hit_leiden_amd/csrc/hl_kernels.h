@@ -35,6 +35,8 @@ u64 count_u8(const u8* a, u64 n, DevBuf<u64>& cnt, hipStream_t s);
 // nonzero counts of up to four byte arrays (null entries count 0), one sync
 void count4_u8(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, DevBuf<u64>& cnt, u64 out[4],
                hipStream_t s);
+// the same into out[0..4) on the device, without synchronising
+void count4_u8_async(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, u64* out, hipStream_t s);
 void fill4_u8(u8* a0, u8* a1, u8* a2, u8* a3, u8 v, u64 n, hipStream_t s);
 // *out = max(*out, max(a[0..n)))  (out must be initialised)
 void max_u32(const u32* a, u64 n, u32* out, hipStream_t s);
@@ -42,6 +44,8 @@ void max_u32(const u32* a, u64 n, u32* out, hipStream_t s);
 u64 select_flagged(const u8* flags, u64 n, u32* out, Temp& tmp, DevBuf<u64>& cnt, hipStream_t s);
 // exclusive scan of u64 values, writes n+1 entries (last = total); returns total
 u64 exclusive_scan_u64(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s);
+// the same without synchronising: the total is left in out[n]
+void exclusive_scan_u64_async(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s);
 // stable sort of (key, value) pairs by key (keys_out/vals_out separate)
 void sort_pairs_u64(const u64* kin, u64* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
                     hipStream_t s);
@@ -55,6 +59,7 @@ struct ApplyScratch {
   DevBuf<u8> heads, uop;
   DevBuf<double> uw;
   DevBuf<u32> ucount;
+  DevBuf<u64> c1;
 };
 
 struct LevelCsrMut {
@@ -138,18 +143,21 @@ struct RefineScratch {
   DevBuf<u32> mhead, mnext;
   DevBuf<u64> kkeys, kkeys_s;
   DevBuf<u32> rsorted;
+  DevBuf<u32> sel, sl, vals;
 };
-// connected-component split of subcommunities touched by K (hit_leiden.rs:1276-1331)
-void cc_split(const Csr& g, u32* s, const u8* K, u8* R, u64 s_space, RefineScratch& rs, Temp& tmp, hipStream_t st);
-bool is_identity(const u32* s, u32 n, RefineScratch& rs, hipStream_t st);
+// connected-component split of subcommunities touched by K (hit_leiden.rs:1276-1331).
+// s_bound: every label of s is below it.  Returns max(s) + 1 after the split and
+// sets *identity when s is the identity map after it (hit_leiden.rs:1333-1339).
+u64 cc_split(const Csr& g, u32* s, const u8* K, u8* R, u64 s_bound, bool* identity, RefineScratch& rs, Temp& tmp,
+             hipStream_t st);
 // throughput refinement (+connected-targets rule) over R
 struct MoveScratch;
-void refine_parallel(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space,
+void refine_parallel(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space, u64 sn,
                      RefineScratch& rs, ProposeScratch& ps, MoveScratch& mvs, u32 maxdeg, const Thresholds& th,
                      Temp& tmp, const double* dpar_in, hipStream_t st);
 // deterministic refinement (hit_leiden.rs:1359-1413)
-void refine_det(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space, RefineScratch& rs,
-                DevBuf<u32>& ks, DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,
+void refine_det(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space, u64 sn,
+                RefineScratch& rs, DevBuf<u32>& ks, DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,
                 DevBuf<u64>& cnt, u32 max_deg, Temp& tmp, hipStream_t st);
 
 // ---- aggregation (hit_leiden.rs:1639-1731) ----
@@ -158,6 +166,9 @@ struct AggScratch {
   DevBuf<u32> rlist, base32, runs, idx, idx_s, keep_idx;
   DevBuf<u64> cnt64, off64, keys, keys_s, cnt;
   DevBuf<u8> heads, keep;
+  DevBuf<u64> c1;
+  DevBuf<u32> mx;
+  DevBuf<Rec> comp;
 };
 // returns |dG^{p+1}| written to out (grown as needed); *next_n = max(s_cur u s_pre)+1
 u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, const u8* R, DevBuf<Rec>& out,

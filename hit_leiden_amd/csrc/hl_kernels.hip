@@ -14,12 +14,73 @@
 #include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
+#include <cstring>
 
 #include "hl_kernels.h"
 #include "hl_move.h"
 
 namespace hl {
 thread_local Profiler* g_prof = nullptr;
+
+namespace {
+struct Staging {
+  u8* buf = nullptr;
+  size_t cap = 0, used = 0;
+  struct Item {
+    void* dst;
+    size_t off, bytes;
+    hipStream_t s;
+  };
+  std::vector<Item> items;
+  ~Staging() {
+    if (buf) (void)hipHostFree(buf);
+  }
+};
+Staging& staging() {
+  thread_local Staging st;
+  return st;
+}
+}  // namespace
+
+void stream_sync(hipStream_t s) {
+  HL_CHECK(hipStreamSynchronize(s));
+  Staging& st = staging();
+  size_t keep = 0;
+  for (size_t i = 0; i < st.items.size(); ++i) {
+    const auto& it = st.items[i];
+    if (it.s == s)
+      std::memcpy(it.dst, st.buf + it.off, it.bytes);
+    else
+      st.items[keep++] = it;
+  }
+  st.items.resize(keep);
+  if (keep == 0) st.used = 0;
+}
+
+void d2h_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  Staging& st = staging();
+  size_t off = (st.used + 15) & ~size_t(15);
+  if (off + bytes > st.cap) {
+    if (!st.items.empty()) {
+      // deliver what is staged (every stream with pending values), then reuse the area
+      std::vector<hipStream_t> ss;
+      for (const auto& it : st.items)
+        if (std::find(ss.begin(), ss.end(), it.s) == ss.end()) ss.push_back(it.s);
+      for (hipStream_t x : ss) stream_sync(x);
+    }
+    off = 0;
+    if (bytes > st.cap) {
+      if (st.buf) HL_CHECK(hipHostFree(st.buf));
+      st.buf = nullptr;
+      const size_t cap = std::max<size_t>(bytes, 1 << 16);
+      HL_CHECK(hipHostMalloc((void**)&st.buf, cap, hipHostMallocDefault));
+      st.cap = cap;
+    }
+  }
+  HL_CHECK(hipMemcpyAsync(st.buf + off, src, bytes, hipMemcpyDeviceToHost, s));
+  st.items.push_back({dst, off, bytes, s});
+  st.used = off + bytes;
+}
 namespace {
 
 constexpr u8 OP_UPD = 1, OP_DEL = 2, OP_INS = 3;
@@ -113,6 +174,9 @@ __global__ void k_fill_u8(u8* p, u8 v, u64 n) {
   for (u64 i = gtid(); i < n; i += gstride()) p[i] = v;
 }
 __global__ void k_fill_u32(u32* p, u32 v, u64 n) {
+  for (u64 i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+__global__ void k_fill_u64(u64* p, u64 v, u64 n) {
   for (u64 i = gtid(); i < n; i += gstride()) p[i] = v;
 }
 __global__ void k_fill_f64(double* p, double v, u64 n) {
@@ -680,12 +744,19 @@ __global__ __launch_bounds__(256) void k_det_move(DetMoveArgs a) {
 }
 
 // ------------------------------------------------------------------ CC split
-__global__ void k_aff_mark(const u8* K, const u32* s, u32 n, u8* aff, u32* any) {
-  for (u64 v = gtid(); v < n; v += gstride())
+// aff[s(v)] for v in K; status[0] = some v in K, status[1] = s is not the identity
+__global__ void k_aff_mark(const u8* K, const u32* s, u32 n, u8* aff, u32* status) {
+  bool any = false, nid = false;
+  for (u64 v = gtid(); v < n; v += gstride()) {
+    const u32 sv = s[v];
     if (K[v]) {
-      aff[s[v]] = 1;
-      *any = 1;
+      aff[sv] = 1;
+      any = true;
     }
+    nid |= sv != v;
+  }
+  if (__any(any) && (threadIdx.x & 63) == 0) status[0] = 1;
+  if (__any(nid) && (threadIdx.x & 63) == 0) status[1] = 1;
 }
 __global__ void k_incc(const u32* s, u32 n, const u8* aff, u8* incc) {
   for (u64 v = gtid(); v < n; v += gstride()) incc[v] = aff[s[v]];
@@ -781,17 +852,14 @@ __global__ void k_split_assign(const u32* vcc, u64 m, const u32* parent, const u
     }
   }
 }
-__global__ void k_not_identity(const u32* s, u32 n, u32* flag) {
-  for (u64 v = gtid(); v < n; v += gstride())
-    if (s[v] != v) *flag = 1;
-}
 
 // ------------------------------------------------------------------ refinement (throughput)
 __global__ void k_sizes(const u32* s, u32 n, u32* sizes) {
   for (u64 v = gtid(); v < n; v += gstride()) atomicAdd(&sizes[s[v]], 1u);
 }
-__global__ void k_single(const u32* rl, u64 nr, const u32* s, const u32* sizes, u8* single) {
-  for (u64 i = gtid(); i < nr; i += gstride()) single[i] = sizes[s[rl[i]]] == 1;
+// v in R and alone in its subcommunity
+__global__ void k_single(const u8* R, u32 n, const u32* s, const u32* sizes, u8* single) {
+  for (u64 v = gtid(); v < n; v += gstride()) single[v] = R[v] && sizes[s[v]] == 1;
 }
 // candidate subcommunities of singleton R vertices (hit_leiden.rs:1569-1586)
 __global__ __launch_bounds__(256) void k_cand_mark(Csr g, const u32* sl, u64 ns, const u32* f, const u32* s,
@@ -1195,12 +1263,20 @@ void count4_u8(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, De
   HL_CHECK(hipMemsetAsync(cnt.p, 0, 4 * sizeof(u64), s));
   if (n) {
     HL_PROF(P_REDUCE, 4 * n, s);
-    hipLaunchKernelGGL(k_count4, blocks_for(n, 256, 1024), 256, 0, s, a0, a1, a2, a3, n,
+    hipLaunchKernelGGL(k_count4, blocks_for(n, 256, kReduceBlocks), 256, 0, s, a0, a1, a2, a3, n,
                        (unsigned long long*)cnt.p);
   }
   HL_LAUNCH_CHECK();
-  HL_CHECK(hipMemcpyAsync(out, cnt.p, 4 * sizeof(u64), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
+  d2h_async(out, cnt.p, 4 * sizeof(u64), s);
+  stream_sync(s);
+}
+void count4_u8_async(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, u64* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, out, u64(0), u64(4));
+  if (n) {
+    HL_PROF(P_REDUCE, 4 * n, s);
+    hipLaunchKernelGGL(k_count4, blocks_for(n, 256, kReduceBlocks), 256, 0, s, a0, a1, a2, a3, n, (unsigned long long*)out);
+  }
+  HL_LAUNCH_CHECK();
 }
 u64 count_u8(const u8* a, u64 n, DevBuf<u64>& cnt, hipStream_t s) {
   u64 out[4];
@@ -1214,7 +1290,7 @@ void fill4_u8(u8* a0, u8* a1, u8* a2, u8* a3, u8 v, u64 n, hipStream_t s) {
 }
 void max_u32(const u32* a, u64 n, u32* out, hipStream_t s) {
   HL_PROF(P_REDUCE, 4 * n, s);
-  if (n) hipLaunchKernelGGL(k_max_u32, blocks_for(n, 256, 1024), 256, 0, s, a, n, out);
+  if (n) hipLaunchKernelGGL(k_max_u32, blocks_for(n, 256, kReduceBlocks), 256, 0, s, a, n, out);
   HL_LAUNCH_CHECK();
 }
 
@@ -1230,16 +1306,31 @@ u64 select_flagged(const u8* flags, u64 n, u32* out, Temp& tmp, DevBuf<u64>& cnt
     HL_CHECK(rocprim::select(t, bytes, it, flags, out, cnt.p, n, s));
   }
   u64 c = 0;
-  HL_CHECK(hipMemcpyAsync(&c, cnt.p, sizeof(u64), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
+  d2h_async(&c, cnt.p, sizeof(u64), s);
+  stream_sync(s);
   return c;
+}
+
+void exclusive_scan_u64_async(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s) {
+  if (n == 0) {
+    hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, out, u64(0), u64(1));
+    HL_LAUNCH_CHECK();
+    return;
+  }
+  size_t bytes = 0;
+  HL_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, u64(0), n, rocprim::plus<u64>(), s));
+  void* t = tmp.get(bytes, s);
+  HL_PROF(P_SCAN, 16 * n, s);
+  HL_CHECK(rocprim::exclusive_scan(t, bytes, in, out, u64(0), n, rocprim::plus<u64>(), s));
+  hipLaunchKernelGGL(k_scan_tail, 1, 1, 0, s, in, out, n);
+  HL_LAUNCH_CHECK();
 }
 
 u64 exclusive_scan_u64(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s) {
   if (n == 0) {
     u64 z = 0;
     HL_CHECK(hipMemcpyAsync(out, &z, sizeof(u64), hipMemcpyHostToDevice, s));
-    HL_CHECK(hipStreamSynchronize(s));
+    stream_sync(s);
     return 0;
   }
   size_t bytes = 0;
@@ -1250,8 +1341,8 @@ u64 exclusive_scan_u64(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s)
   hipLaunchKernelGGL(k_scan_tail, 1, 1, 0, s, in, out, n);
   HL_LAUNCH_CHECK();
   u64 total = 0;
-  HL_CHECK(hipMemcpyAsync(&total, out + n, sizeof(u64), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
+  d2h_async(&total, out + n, sizeof(u64), s);
+  stream_sync(s);
   return total;
 }
 
@@ -1296,7 +1387,7 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   sort_pairs_u64(a.keys.p, a.keys_s.p, a.idx.p, a.idx_s.p, k, 64, tmp, s);
   hipLaunchKernelGGL(k_heads_u64, GRID(k), a.keys_s.p, k, 0, a.heads.p);
   HL_LAUNCH_CHECK();
-  DevBuf<u64> c1;
+  DevBuf<u64>& c1 = a.c1;
   plan.nruns = select_flagged(a.heads.p, k, a.runs.p, tmp, c1, s);
   const u64 maxupd = 2 * plan.nruns;
   a.ukey.reserve(maxupd, s);
@@ -1314,8 +1405,8 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   HL_LAUNCH_CHECK();
   }
   u32 nupd = 0;
-  HL_CHECK(hipMemcpyAsync(&nupd, a.ucount.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
+  d2h_async(&nupd, a.ucount.p, sizeof(u32), s);
+  stream_sync(s);
   plan.nupd = nupd;
   if (nupd == 0) return;
   sort_pairs_u64(a.ukey.p, a.ukey_s.p, a.uidx.p, a.perm.p, nupd, 64, tmp, s);
@@ -1341,13 +1432,17 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
                      a.seg_row.p, a.seg_len.p, a.new_len.p, a.seg_ins.p, a.seg_del.p, a.ins_pre.p, a.del_pre.p,
                      a.newcap.p, a.chunks.p, a.cnt.p);
   HL_LAUNCH_CHECK();
-  plan.need = exclusive_scan_u64(a.newcap.p, a.newoff.p, ns, tmp, s);
-  plan.total_chunks = exclusive_scan_u64(a.chunks.p, a.chunkbase.p, ns, tmp, s);
-  u64 olds[2] = {0, 0};
-  HL_CHECK(hipMemcpyAsync(olds, a.cnt.p, 2 * sizeof(u64), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
-  plan.old_entries = olds[0];
-  plan.max_row = olds[1];
+  exclusive_scan_u64_async(a.newcap.p, a.newoff.p, ns, tmp, s);
+  exclusive_scan_u64_async(a.chunks.p, a.chunkbase.p, ns, tmp, s);
+  u64 h[4] = {0, 0, 0, 0};
+  d2h_async(h, a.cnt.p, 2 * sizeof(u64), s);
+  d2h_async(h + 2, a.newoff.p + ns, sizeof(u64), s);
+  d2h_async(h + 3, a.chunkbase.p + ns, sizeof(u64), s);
+  stream_sync(s);
+  plan.old_entries = h[0];
+  plan.max_row = h[1];
+  plan.need = h[2];
+  plan.total_chunks = h[3];
 }
 
 void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, const ApplyPlan& plan, bool det,
@@ -1380,7 +1475,7 @@ u64 compact_rows(const LevelCsrMut& g, u32* col2, double* w2, DevBuf<u64>& off_t
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_set_offsets, GRID(g.n), g, off_tmp.p);
   HL_LAUNCH_CHECK();
-  HL_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
   return total;
 }
 
@@ -1485,38 +1580,39 @@ int det_move(const Csr& g, u32* f, const u32* sm, double* tot, u64 tot_cap, u64*
   hipLaunchKernelGGL(k_det_move, 1, 256, 0, s, a);
   HL_LAUNCH_CHECK();
   u32 st = 0;
-  HL_CHECK(hipMemcpyAsync(&st, ds.status.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
+  d2h_async(&st, ds.status.p, sizeof(u32), s);
+  stream_sync(s);
   return int(st);
 }
 
-void cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_space, RefineScratch& rs, Temp& tmp, hipStream_t s) {
+u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* identity, RefineScratch& rs,
+             Temp& tmp, hipStream_t s) {
   const u32 n = g.n;
-  if (n == 0) return;
-  rs.aff.reserve(s_space, s);
-  rs.status.reserve(2, s);
-  fill_u8(rs.aff.p, 0, s_space, s);
-  fill_u32(rs.status.p, 0, 2, s);
+  *identity = true;
+  if (n == 0) return 1;
+  rs.aff.reserve(s_bound, s);
+  rs.status.reserve(4, s);
+  rs.smax.reserve(1, s);
+  fill_u8(rs.aff.p, 0, s_bound, s);
+  fill_u32(rs.status.p, 0, 3, s);  // [0] any affected, [1] not identity, [2] max label
   hipLaunchKernelGGL(k_aff_mark, GRID(n), K, sm, n, rs.aff.p, rs.status.p);
   HL_LAUNCH_CHECK();
-  rs.smax.reserve(1, s);
-  fill_u32(rs.smax.p, 0, 1, s);
-  max_u32(sm, n, rs.smax.p, s);
-  u32 hs[2] = {0, 0};
-  HL_CHECK(hipMemcpyAsync(&hs[0], rs.status.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipMemcpyAsync(&hs[1], rs.smax.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
-  if (!hs[0]) return;
-  const u32 next_id = hs[1] + 1;
+  max_u32(sm, n, rs.status.p + 2, s);
+  u32 hs[3] = {0, 0, 0};
+  d2h_async(hs, rs.status.p, 3 * sizeof(u32), s);
+  stream_sync(s);
+  *identity = hs[1] == 0;
+  const u32 next_id = hs[2] + 1;
+  if (!hs[0]) return next_id;
   rs.incc.reserve(n, s);
   rs.vcc.reserve(n, s);
   hipLaunchKernelGGL(k_incc, GRID(n), sm, n, rs.aff.p, rs.incc.p);
   HL_LAUNCH_CHECK();
   const u64 m = select_flagged(rs.incc.p, n, rs.vcc.p, tmp, rs.cnt, s);
-  if (m == 0) return;
+  if (m == 0) return next_id;
   rs.parent.reserve(n, s);
   rs.cnt32.reserve(n, s);
-  rs.best64.reserve(s_space, s);
+  rs.best64.reserve(next_id, s);
   rs.keepflag.reserve(m, s);
   rs.roots.reserve(m, s);
   rs.roots_s.reserve(m, s);
@@ -1536,11 +1632,11 @@ void cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_space, RefineScra
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_split_roots, GRID(m), rs.vcc.p, m, rs.parent.p, sm, rs.best64.p, rs.keepflag.p);
   HL_LAUNCH_CHECK();
-  DevBuf<u32> sel;
+  DevBuf<u32>& sel = rs.sel;
   sel.reserve(m, s);
   const u64 c = select_flagged(rs.keepflag.p, m, sel.p, tmp, rs.cnt, s);
   if (c) {
-    DevBuf<u32> vals;
+    DevBuf<u32>& vals = rs.vals;
     vals.reserve(c, s);
     hipLaunchKernelGGL(k_root_keys, GRID(c), sel.p, c, rs.vcc.p, sm, rs.rkeys.p, vals.p);
     HL_LAUNCH_CHECK();
@@ -1549,40 +1645,16 @@ void cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_space, RefineScra
     HL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_split_assign, GRID(m), rs.vcc.p, m, rs.parent.p, rs.best64.p, rs.newid.p, sm, R);
     HL_LAUNCH_CHECK();
+    *identity = false;  // split ids exceed every vertex id of an identity map
   }
-  HL_CHECK(hipStreamSynchronize(s));
+  return next_id + c;  // split ids are next_id .. next_id + c - 1; the kept parts keep theirs
 }
 
-bool is_identity(const u32* sm, u32 n, RefineScratch& rs, hipStream_t s) {
-  rs.status.reserve(2, s);
-  fill_u32(rs.status.p, 0, 1, s);
-  if (n) hipLaunchKernelGGL(k_not_identity, GRID(n), sm, n, rs.status.p);
-  HL_LAUNCH_CHECK();
-  u32 h = 0;
-  HL_CHECK(hipMemcpyAsync(&h, rs.status.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
-  return h == 0;
-}
-
-static u64 smax_plus1(const u32* sm, u32 n, RefineScratch& rs, hipStream_t s) {
-  rs.smax.reserve(1, s);
-  fill_u32(rs.smax.p, 0, 1, s);
-  max_u32(sm, n, rs.smax.p, s);
-  u32 h = 0;
-  HL_CHECK(hipMemcpyAsync(&h, rs.smax.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
-  return u64(h) + 1;
-}
-
-void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space,
+void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space, u64 sn,
                      RefineScratch& rs, ProposeScratch& ps, MoveScratch& mvs, u32 maxdeg, const Thresholds& th,
                      Temp& tmp, const double* dpar_in, hipStream_t s) {
   const u32 n = g.n;
   if (n == 0) return;
-  const u64 sn = smax_plus1(sm, n, rs, s);
-  rs.rlist.reserve(n, s);
-  const u64 nr = select_flagged(R, n, rs.rlist.p, tmp, rs.cnt, s);
-  if (nr == 0) return;
   rs.sizes.reserve(sn, s);
   rs.dS.reserve(sn, s);
   fill_u32(rs.sizes.p, 0, sn, s);
@@ -1590,16 +1662,13 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
   hipLaunchKernelGGL(k_sizes, GRID(n), sm, n, rs.sizes.p);
   HL_LAUNCH_CHECK();
   label_totals_atomic(sm, g.k, n, rs.dS.p, s);
-  rs.single.reserve(nr, s);
-  hipLaunchKernelGGL(k_single, GRID(nr), rs.rlist.p, nr, sm, rs.sizes.p, rs.single.p);
+  rs.single.reserve(n, s);
+  hipLaunchKernelGGL(k_single, GRID(n), R, n, sm, rs.sizes.p, rs.single.p);
   HL_LAUNCH_CHECK();
-  DevBuf<u32> sel, sl;
-  sel.reserve(nr, s);
-  const u64 ns = select_flagged(rs.single.p, nr, sel.p, tmp, rs.cnt, s);
+  DevBuf<u32>& sl = rs.sl;
+  sl.reserve(n, s);
+  const u64 ns = select_flagged(rs.single.p, n, sl.p, tmp, rs.cnt, s);
   if (ns == 0) return;
-  sl.reserve(ns, s);
-  hipLaunchKernelGGL(k_gather_list, GRID(ns), sel.p, ns, rs.rlist.p, sl.p);
-  HL_LAUNCH_CHECK();
   // blocked set (compute_blocked_subcommunities)
   rs.is_cand.reserve(sn, s);
   rs.blocked.reserve(sn, s);
@@ -1629,15 +1698,13 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
   (void)ps;
   (void)tmp;
   refine_proposals(g, f, sm, rs.dS.p, rs.blocked.p, gamma, sl.p, ns, sn, mvs, th, maxdeg, s);
-  HL_CHECK(hipStreamSynchronize(s));
 }
 
-void refine_det(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space, RefineScratch& rs,
-                DevBuf<u32>& ks, DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,
-                DevBuf<u64>& cnt, u32 max_deg, Temp& tmp, hipStream_t s) {
+void refine_det(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space, u64 sn,
+                RefineScratch& rs, DevBuf<u32>& ks, DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads,
+                DevBuf<u32>& starts, DevBuf<u64>& cnt, u32 max_deg, Temp& tmp, hipStream_t s) {
   const u32 n = g.n;
   if (n == 0) return;
-  const u64 sn = smax_plus1(sm, n, rs, s);
   rs.rlist.reserve(n, s);
   const u64 nr = select_flagged(R, n, rs.rlist.p, tmp, rs.cnt, s);
   if (nr == 0) return;
@@ -1654,8 +1721,8 @@ void refine_det(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, 
   {
     // starts/ks/vs now hold the (s, v) segments
     u64 nseg = 0;
-    HL_CHECK(hipMemcpyAsync(&nseg, cnt.p, sizeof(u64), hipMemcpyDeviceToHost, s));
-    HL_CHECK(hipStreamSynchronize(s));
+    d2h_async(&nseg, cnt.p, sizeof(u64), s);
+    stream_sync(s);
     fill_u32(rs.mhead.p, kNone, sn, s);
     hipLaunchKernelGGL(k_member_lists, GRID(nseg), nseg, starts.p, u64(n), ks.p, vs.p, rs.mhead.p, rs.mnext.p);
     HL_LAUNCH_CHECK();
@@ -1681,15 +1748,25 @@ void refine_det(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, 
   HL_PROF(P_DET_REFINE, 0, s);
   hipLaunchKernelGGL(k_det_refine, 1, 256, 0, s, a);
   HL_LAUNCH_CHECK();
-  HL_CHECK(hipStreamSynchronize(s));
+  stream_sync(s);
 }
 
 u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, const u8* R, DevBuf<Rec>& out,
               u32* next_n, AggScratch& as, Temp& tmp, hipStream_t s) {
   const u32 n = g.n;
   as.rlist.reserve(n + 1, s);
-  DevBuf<u64> c1;
+  DevBuf<u64>& c1 = as.c1;
+  // next node count from s_cur and the old s_pre (read back with the next synchronisation)
+  DevBuf<u32>& mx = as.mx;
+  mx.reserve(1, s);
+  fill_u32(mx.p, 0, 1, s);
+  max_u32(s_cur, n, mx.p, s);
+  max_u32(s_pre, n, mx.p, s);
+  u32 hm = 0;
+  d2h_async(&hm, mx.p, sizeof(u32), s);
   const u64 nr = n ? select_flagged(R, n, as.rlist.p, tmp, c1, s) : 0;
+  if (!n) stream_sync(s);
+  *next_n = hm + 1;
   as.cnt64.reserve(nr + 1, s);
   as.off64.reserve(nr + 1, s);
   u64 nemit = 0;
@@ -1708,16 +1785,6 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
     hipLaunchKernelGGL(k_agg_emit, GRID(nr * 64), g, as.rlist.p, nr, R, s_pre, s_cur, as.off64.p, k, as.H.p);
   }
   HL_LAUNCH_CHECK();
-  // next node count from s_cur and the old s_pre
-  DevBuf<u32> mx;
-  mx.reserve(1, s);
-  fill_u32(mx.p, 0, 1, s);
-  max_u32(s_cur, n, mx.p, s);
-  max_u32(s_pre, n, mx.p, s);
-  u32 hm = 0;
-  HL_CHECK(hipMemcpyAsync(&hm, mx.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
-  *next_n = hm + 1;
   if (nr) hipLaunchKernelGGL(k_spre_update, GRID(nr), as.rlist.p, nr, s_cur, s_pre);
   HL_LAUNCH_CHECK();
   if (total == 0) return 0;
@@ -1733,7 +1800,7 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
   hipLaunchKernelGGL(k_heads_u64, GRID(total), as.keys_s.p, total, 0, as.heads.p);
   HL_LAUNCH_CHECK();
   const u64 nruns = select_flagged(as.heads.p, total, as.runs.p, tmp, c1, s);
-  DevBuf<Rec> comp;
+  DevBuf<Rec>& comp = as.comp;
   comp.reserve(nruns, s);
   as.keep.reserve(nruns, s);
   HL_PROF(P_COMPRESS, 16 * total + 17 * nruns, s);
@@ -1745,7 +1812,6 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
   out.reserve(kept + 1, s);
   if (kept) hipLaunchKernelGGL(k_gather_rec, GRID(kept), as.keep_idx.p, kept, comp.p, out.p);
   HL_LAUNCH_CHECK();
-  HL_CHECK(hipStreamSynchronize(s));
   return kept;
 }
 
@@ -1765,15 +1831,15 @@ double modularity(const Csr& g, const u32* lab, DevBuf<double>& sig, DevBuf<doub
                   hipStream_t s) {
   const u32 n = g.n;
   double t = 0.0;
-  HL_CHECK(hipMemcpyAsync(&t, g.twice_m, sizeof(double), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
+  d2h_async(&t, g.twice_m, sizeof(double), s);
+  stream_sync(s);
   if (t == 0.0 || n == 0) return 0.0;
   mx.reserve(1, s);
   fill_u32(mx.p, 0, 1, s);
   max_u32(lab, n, mx.p, s);
   u32 hm = 0;
-  HL_CHECK(hipMemcpyAsync(&hm, mx.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
+  d2h_async(&hm, mx.p, sizeof(u32), s);
+  stream_sync(s);
   const u64 L = u64(hm) + 1;
   sig.reserve(L, s);
   acc.reserve(2, s);
@@ -1786,8 +1852,8 @@ double modularity(const Csr& g, const u32* lab, DevBuf<double>& sig, DevBuf<doub
   hipLaunchKernelGGL(k_sq_sum, blocks_for(L, 256, 4096), 256, 0, s, sig.p, L, g.twice_m, acc.p + 1);
   HL_LAUNCH_CHECK();
   double h[2];
-  HL_CHECK(hipMemcpyAsync(h, acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
+  d2h_async(h, acc.p, 2 * sizeof(double), s);
+  stream_sync(s);
   return (h[0] - h[1]) / t;
 }
 
@@ -1804,8 +1870,8 @@ void canonicalize(u32* lab, u32 n, DevBuf<u32>& first, DevBuf<u8>& flag, DevBuf<
   fill_u32(mx.p, 0, 1, s);
   max_u32(lab, n, mx.p, s);
   u32 hm = 0;
-  HL_CHECK(hipMemcpyAsync(&hm, mx.p, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
+  d2h_async(&hm, mx.p, sizeof(u32), s);
+  stream_sync(s);
   const u64 L = u64(hm) + 1;
   first.reserve(L, s);
   fill_u32(first.p, kNone, L, s);

@@ -20,10 +20,21 @@ namespace {
 constexpr u32 SMALL_MAX2 = 16;
 constexpr u32 WAVE_HW2 = 1024;
 constexpr u32 kUndec = 0, kAcc = 1, kRej = 2;
+constexpr u32 kDecLds = 1024;   // candidates whose priority rounds run on LDS slots
+constexpr u32 kDecHash = 4096;  // LDS hash slots for their 2 x kDecLds communities
+constexpr u32 kWaveBlocks = 768;  // 3 resident workgroups per CU at 52 KB of LDS each
 
 __device__ __forceinline__ u64 gtid() { return u64(blockIdx.x) * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ u64 gstride() { return u64(gridDim.x) * blockDim.x; }
-__device__ __forceinline__ u32 hash32(u32 x) { return x * 2654435761u; }
+// integer mix (lowbias32) so that the low bits used as a table index depend on every key bit
+__device__ __forceinline__ u32 hash32(u32 x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
 
 __device__ __forceinline__ double move_gain(double wc, double wcur, double t, double gamma, double k, double sc,
                                             double scand) {
@@ -78,6 +89,15 @@ __device__ __forceinline__ u32 wave_append(bool want, u32* cnt) {
 __device__ __forceinline__ u32 ld_agent(const u32* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Slot of key in an LDS open-addressing table (linear probing, inserts if absent).
+__device__ __forceinline__ u32 lds_insert(u32* K, u32 cap, u32 key) {
+  u32 h = hash32(key) & (cap - 1);
+  while (true) {
+    const u32 old = atomicCAS(&K[h], kNone, key);
+    if (old == kNone || old == key) return h;
+    h = (h + 1) & (cap - 1);
+  }
+}
 // Add v to a list once per epoch.
 __device__ __forceinline__ bool claim(u32* mark, u32 v, u32 epoch) { return atomicExch(&mark[v], epoch) != epoch; }
 
@@ -128,6 +148,22 @@ __global__ void k_iota_list(u32* list, u32 n, u32* cnt) {
   if (gtid() == 0) *cnt = n;
 }
 
+// One device atomic per workgroup (same-address atomics serialise); every
+// thread of the block must call it.
+__device__ void block_add_u64(unsigned long long* dst, unsigned long long x) {
+  __shared__ unsigned long long part[16];
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) part[wv] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (u32 i = 0; i < nw; ++i) t += part[i];
+    if (t) atomicAdd(dst, t);
+  }
+}
+
 // ------------------------------------------------------------------ buckets
 __global__ void k_bucket2(const u32* verts, const u32* cnt_dev, const u32* row_len, u32 small_deg, u32 wave_deg,
                           u32* lists, u64 cap, u32* counts, unsigned long long* st) {
@@ -150,22 +186,11 @@ __global__ void k_bucket2(const u32* verts, const u32* cnt_dev, const u32* row_l
     }
   }
   for (int q = 0; q < 3; ++q) {
-    unsigned long long x = local[q], y = lcnt[q];
-    for (int o = 32; o > 0; o >>= 1) {
-      x += __shfl_xor(x, o, 64);
-      y += __shfl_xor(y, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0 && y) {
-      atomicAdd(&st[q], y);
-      atomicAdd(&st[3 + q], x);
-    }
+    block_add_u64(&st[q], lcnt[q]);
+    block_add_u64(&st[3 + q], local[q]);
   }
 }
 
-__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned long long x) {
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  if ((threadIdx.x & 63) == 0 && x) atomicAdd(dst, x);
-}
 
 // ------------------------------------------------------------------ proposals
 template <class P>
@@ -237,16 +262,40 @@ __global__ __launch_bounds__(256) void k_prop2_small(P pol, Csr g, const u32* li
       out.gain[p] = bg;
     }
   }
-  wave_add_u64(cst, ncand);
+  block_add_u64(cst, ncand);
 }
 
 // one wavefront per vertex, LDS open-addressing table of up to WAVE_HW2 slots
 // (a power of two >= 2 deg, so only the slots a vertex can use are cleared)
+// Proposals are staged in LDS and appended with one device atomic per flush.
+constexpr u32 kStage = 256;
+__device__ void flush_stage(u32* sn, u32* st, double* sg, u32& scnt, u32& sbase, CandOut out) {
+  __syncthreads();
+  const u32 c = scnt;
+  if (c) {
+    if (threadIdx.x == 0) sbase = atomicAdd(out.cnt, c);
+    __syncthreads();
+    const u32 b = sbase;
+    for (u32 i = threadIdx.x; i < c; i += blockDim.x) {
+      out.node[b + i] = sn[i];
+      out.to[b + i] = st[i];
+      out.gain[b + i] = sg[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) scnt = 0;
+  }
+  __syncthreads();
+}
+
 template <class P>
 __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* list, const u32* cnt_dev, CandOut out,
                                                      unsigned long long* cst) {
   __shared__ u32 hk[4 * WAVE_HW2];
   __shared__ double hv[4 * WAVE_HW2];
+  __shared__ u32 stn[kStage], stt[kStage];
+  __shared__ double stg[kStage];
+  __shared__ u32 scnt, sbase;
+  if (threadIdx.x == 0) scnt = 0;
   const u32 cnt = *cnt_dev;
   const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   u32* K = hk + wv * WAVE_HW2;
@@ -300,16 +349,18 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
       if (lane == 0) {
         finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
         if (bg > 0.0) {
-          u32 p = atomicAdd(out.cnt, 1u);
-          out.node[p] = v;
-          out.to[p] = best;
-          out.gain[p] = bg;
+          const u32 p = atomicAdd(&scnt, 1u);  // LDS
+          stn[p] = v;
+          stt[p] = best;
+          stg[p] = bg;
         }
       }
     }
     __syncthreads();
+    if (scnt > kStage - 4) flush_stage(stn, stt, stg, scnt, sbase, out);
   }
-  wave_add_u64(cst, ncand);
+  flush_stage(stn, stt, stg, scnt, sbase, out);
+  block_add_u64(cst, ncand);
 }
 
 // one workgroup per vertex (hubs); each resident workgroup owns a table of tcap slots
@@ -370,7 +421,7 @@ __global__ __launch_bounds__(256) void k_prop2_big(P pol, Csr g, const u32* list
     }
     __syncthreads();
   }
-  wave_add_u64(cst, ncand);
+  block_add_u64(cst, ncand);
 }
 
 // ------------------------------------------------------------------ decoupling (one workgroup)
@@ -386,6 +437,8 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
   __shared__ u32 sfrom[kDecSmall];
   __shared__ u32 sto[kDecSmall];
   __shared__ u8 sst[kDecSmall];
+  __shared__ u32 mAto[kDecHash], mAfrom[kDecHash];
+  __shared__ u16 fslot[kDecLds], tslot[kDecLds];
   __shared__ u32 und, nacc;
   const u32 c = *cand.cnt;
   const u32 tid = threadIdx.x;
@@ -393,110 +446,219 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
     if (tid == 0) status[0] = 1;  // multi-kernel path
     return;
   }
-  u32 N = 1;
-  while (N < c) N <<= 1;
-  for (u32 i = tid; i < N; i += blockDim.x) {
-    if (i < c) {
-      skey[i] = ~(u64)__double_as_longlong(cand.gain[i]);
-      snode[i] = cand.node[i];
-      sidx[i] = i;
-    } else {
-      skey[i] = ~0ull;
-      snode[i] = kNone;
-      sidx[i] = kNone;
-    }
-  }
-  __syncthreads();
-  // bitonic sort of (key, node) ascending
-  for (u32 kk = 2; kk <= N; kk <<= 1) {
-    for (u32 j = kk >> 1; j > 0; j >>= 1) {
-      for (u32 i = tid; i < N; i += blockDim.x) {
-        u32 ixj = i ^ j;
-        if (ixj > i) {
-          bool up = (i & kk) == 0;
-          bool gt = pri_less(skey[ixj], snode[ixj], skey[i], snode[i]);
-          if (gt == up) {
-            u64 tk = skey[i];
-            skey[i] = skey[ixj];
-            skey[ixj] = tk;
-            u32 tn = snode[i];
-            snode[i] = snode[ixj];
-            snode[ixj] = tn;
-            u32 ti = sidx[i];
-            sidx[i] = sidx[ixj];
-            sidx[ixj] = ti;
+  if (c <= 64) {
+    // one wavefront: register bitonic sort on (gain desc, node asc), then the
+    // greedy acceptance of throughput.rs:217-233 in priority order by ballots
+    if (tid < 64) {
+      const u32 lane = tid;
+      u64 key = ~0ull;
+      u32 node = kNone, to = 0, from = 0;
+      if (lane < c) {
+        key = ~(u64)__double_as_longlong(cand.gain[lane]);
+        node = cand.node[lane];
+        to = cand.to[lane];
+        from = m.f[node];
+      }
+      for (u32 kk = 2; kk <= 64; kk <<= 1) {
+        for (u32 j = kk >> 1; j > 0; j >>= 1) {
+          const u32 klo = __shfl_xor(u32(key), j, 64), khi = __shfl_xor(u32(key >> 32), j, 64);
+          const u64 ok = (u64(khi) << 32) | klo;
+          const u32 on = __shfl_xor(node, j, 64), ot = __shfl_xor(to, j, 64), of = __shfl_xor(from, j, 64);
+          const bool keep_min = ((lane & j) == 0) == ((lane & kk) == 0);
+          const bool take = keep_min ? pri_less(ok, on, key, node) : pri_less(key, node, ok, on);
+          if (take) {
+            key = ok;
+            node = on;
+            to = ot;
+            from = of;
           }
         }
       }
-      __syncthreads();
+      bool acc = false;
+      for (u32 p = 0; p < c; ++p) {
+        const u32 fp = __shfl(from, p, 64), tp = __shfl(to, p, 64);
+        const bool conf = lane < p && acc && (to == fp || from == tp);
+        const unsigned long long bm = __ballot(conf);
+        if (lane == p) acc = bm == 0;
+      }
+      if (lane < c) {
+        snode[lane] = node;
+        sto[lane] = to;
+        sfrom[lane] = from;
+        sst[lane] = acc ? kAcc : kRej;
+      }
+      if (acc) {
+        const double kv = m.k[node];
+        m.f[node] = to;
+        atomicAdd(&m.tot[from], -kv);
+        atomicAdd(&m.tot[to], kv);
+        m.B[node] = 1;
+      }
+      const unsigned long long am = __ballot(acc);
+      if (lane == 0) nacc = u32(__popcll(am));
     }
-  }
-  for (u32 p = tid; p < c; p += blockDim.x) {
-    u32 v = snode[p];
-    sfrom[p] = m.f[v];
-    sto[p] = cand.to[sidx[p]];
-    sst[p] = kUndec;
-  }
-  __syncthreads();
-  // priority rounds: reject if an earlier accepted move entered our source or
-  // left our target; wait while an earlier undecided one could
-  for (int iter = 0; iter < (1 << 20); ++iter) {
-    for (u32 p = tid; p < c; p += blockDim.x) {
-      u32 f = sfrom[p], t = sto[p];
-      atomicExch(&minUto[f], kNone); atomicExch(&minUto[t], kNone);
-      atomicExch(&minUfrom[f], kNone); atomicExch(&minUfrom[t], kNone);
-      atomicExch(&minAto[f], kNone); atomicExch(&minAto[t], kNone);
-      atomicExch(&minAfrom[f], kNone); atomicExch(&minAfrom[t], kNone);
-    }
-    if (tid == 0) und = 0;
     __threadfence();
     __syncthreads();
-    for (u32 p = tid; p < c; p += blockDim.x) {
-      u8 st = sst[p];
-      if (st == kUndec) {
-        atomicMin(&minUto[sto[p]], p);
-        atomicMin(&minUfrom[sfrom[p]], p);
-      } else if (st == kAcc) {
-        atomicMin(&minAto[sto[p]], p);
-        atomicMin(&minAfrom[sfrom[p]], p);
+  } else {
+    u32 N = 1;
+    while (N < c) N <<= 1;
+    for (u32 i = tid; i < N; i += blockDim.x) {
+      if (i < c) {
+        skey[i] = ~(u64)__double_as_longlong(cand.gain[i]);
+        snode[i] = cand.node[i];
+        sidx[i] = i;
+      } else {
+        skey[i] = ~0ull;
+        snode[i] = kNone;
+        sidx[i] = kNone;
       }
     }
+    __syncthreads();
+    // bitonic sort of (key, node) ascending
+    for (u32 kk = 2; kk <= N; kk <<= 1) {
+      for (u32 j = kk >> 1; j > 0; j >>= 1) {
+        for (u32 i = tid; i < N; i += blockDim.x) {
+          u32 ixj = i ^ j;
+          if (ixj > i) {
+            bool up = (i & kk) == 0;
+            bool gt = pri_less(skey[ixj], snode[ixj], skey[i], snode[i]);
+            if (gt == up) {
+              u64 tk = skey[i];
+              skey[i] = skey[ixj];
+              skey[ixj] = tk;
+              u32 tn = snode[i];
+              snode[i] = snode[ixj];
+              snode[ixj] = tn;
+              u32 ti = sidx[i];
+              sidx[i] = sidx[ixj];
+              sidx[ixj] = ti;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (u32 p = tid; p < c; p += blockDim.x) {
+      u32 v = snode[p];
+      sfrom[p] = m.f[v];
+      sto[p] = cand.to[sidx[p]];
+      sst[p] = kUndec;
+    }
+    __syncthreads();
+    // priority rounds: reject if an earlier accepted move entered our source or
+    // left our target; wait while an earlier undecided one could
+    if (c <= kDecLds) {
+      // communities -> slots of an LDS hash table (keys in the dead sort keys,
+      // per-slot minima in LDS), so a round touches no device memory
+      u32* hk = reinterpret_cast<u32*>(skey);  // kDecHash keys
+      u32* mUto = hk + kDecHash;
+      u32* mUfrom = sidx;
+      for (u32 q = tid; q < kDecHash; q += blockDim.x) hk[q] = kNone;
+      __syncthreads();
+      for (u32 p = tid; p < c; p += blockDim.x) {
+        fslot[p] = u16(lds_insert(hk, kDecHash, sfrom[p]));
+        tslot[p] = u16(lds_insert(hk, kDecHash, sto[p]));
+      }
+      __syncthreads();
+      for (int iter = 0; iter < (1 << 20); ++iter) {
+        for (u32 q = tid; q < kDecHash; q += blockDim.x) {
+          mUto[q] = kNone;
+          mUfrom[q] = kNone;
+          mAto[q] = kNone;
+          mAfrom[q] = kNone;
+        }
+        if (tid == 0) und = 0;
+        __syncthreads();
+        for (u32 p = tid; p < c; p += blockDim.x) {
+          const u8 st = sst[p];
+          if (st == kUndec) {
+            atomicMin(&mUto[tslot[p]], p);
+            atomicMin(&mUfrom[fslot[p]], p);
+          } else if (st == kAcc) {
+            atomicMin(&mAto[tslot[p]], p);
+            atomicMin(&mAfrom[fslot[p]], p);
+          }
+        }
+        __syncthreads();
+        u32 local = 0;
+        for (u32 p = tid; p < c; p += blockDim.x) {
+          if (sst[p] != kUndec) continue;
+          const u32 f = fslot[p], t = tslot[p];
+          if (mAto[f] < p || mAfrom[t] < p)
+            sst[p] = kRej;
+          else if (mUto[f] < p || mUfrom[t] < p)
+            ++local;
+          else
+            sst[p] = kAcc;
+        }
+        if (local) atomicAdd(&und, local);
+        __syncthreads();
+        const u32 u = und;
+        __syncthreads();
+        if (u == 0) break;
+      }
+    } else {
+      // priority rounds: reject if an earlier accepted move entered our source or
+      // left our target; wait while an earlier undecided one could
+      for (int iter = 0; iter < (1 << 20); ++iter) {
+        for (u32 p = tid; p < c; p += blockDim.x) {
+          u32 f = sfrom[p], t = sto[p];
+          atomicExch(&minUto[f], kNone); atomicExch(&minUto[t], kNone);
+          atomicExch(&minUfrom[f], kNone); atomicExch(&minUfrom[t], kNone);
+          atomicExch(&minAto[f], kNone); atomicExch(&minAto[t], kNone);
+          atomicExch(&minAfrom[f], kNone); atomicExch(&minAfrom[t], kNone);
+        }
+        if (tid == 0) und = 0;
+        __threadfence();
+        __syncthreads();
+        for (u32 p = tid; p < c; p += blockDim.x) {
+          u8 st = sst[p];
+          if (st == kUndec) {
+            atomicMin(&minUto[sto[p]], p);
+            atomicMin(&minUfrom[sfrom[p]], p);
+          } else if (st == kAcc) {
+            atomicMin(&minAto[sto[p]], p);
+            atomicMin(&minAfrom[sfrom[p]], p);
+          }
+        }
+        __threadfence();
+        __syncthreads();
+        u32 local = 0;
+        for (u32 p = tid; p < c; p += blockDim.x) {
+          if (sst[p] != kUndec) continue;
+          u32 f = sfrom[p], t = sto[p];
+          if (atomicAdd(&minAto[f], 0u) < p || atomicAdd(&minAfrom[t], 0u) < p)
+            sst[p] = kRej;
+          else if (atomicAdd(&minUto[f], 0u) < p || atomicAdd(&minUfrom[t], 0u) < p)
+            ++local;
+          else
+            sst[p] = kAcc;
+        }
+        if (local) atomicAdd(&und, local);
+        __syncthreads();
+        const u32 u = und;
+        __syncthreads();
+        if (u == 0) break;
+      }
+    }
+    // apply (throughput.rs:182-187)
+    if (tid == 0) nacc = 0;
+    __syncthreads();
+    u32 la = 0;
+    for (u32 p = tid; p < c; p += blockDim.x) {
+      if (sst[p] != kAcc) continue;
+      ++la;
+      u32 v = snode[p];
+      double kv = m.k[v];
+      m.f[v] = sto[p];
+      atomicAdd(&m.tot[sfrom[p]], -kv);
+      atomicAdd(&m.tot[sto[p]], kv);
+      m.B[v] = 1;
+    }
+    if (la) atomicAdd(&nacc, la);
     __threadfence();
     __syncthreads();
-    u32 local = 0;
-    for (u32 p = tid; p < c; p += blockDim.x) {
-      if (sst[p] != kUndec) continue;
-      u32 f = sfrom[p], t = sto[p];
-      if (atomicAdd(&minAto[f], 0u) < p || atomicAdd(&minAfrom[t], 0u) < p)
-        sst[p] = kRej;
-      else if (atomicAdd(&minUto[f], 0u) < p || atomicAdd(&minUfrom[t], 0u) < p)
-        ++local;
-      else
-        sst[p] = kAcc;
-    }
-    if (local) atomicAdd(&und, local);
-    __syncthreads();
-    const u32 u = und;
-    __syncthreads();
-    if (u == 0) break;
   }
-  // apply (throughput.rs:182-187)
-  if (tid == 0) nacc = 0;
-  __syncthreads();
-  u32 la = 0;
-  for (u32 p = tid; p < c; p += blockDim.x) {
-    if (sst[p] != kAcc) continue;
-    ++la;
-    u32 v = snode[p];
-    double kv = m.k[v];
-    m.f[v] = sto[p];
-    atomicAdd(&m.tot[sfrom[p]], -kv);
-    atomicAdd(&m.tot[sto[p]], kv);
-    m.B[v] = 1;
-  }
-  if (la) atomicAdd(&nacc, la);
-  __threadfence();
-  __syncthreads();
   // next-active and K marks (throughput.rs:189-199)
   const u32 lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   unsigned long long scanned = 0;
@@ -630,8 +792,8 @@ __global__ void k_apply4(u64 c, const u32* node, const u32* from, const u32* to,
     u32 q = wave_append(acc, acc_cnt);
     if (acc) acc_list[q] = u32(p);
   }
-  wave_add_u64(&dst[DS_MULTI_ACC], na);
-  wave_add_u64(&dst[DS_MULTI_DEG], nd);
+  block_add_u64(&dst[DS_MULTI_ACC], na);
+  block_add_u64(&dst[DS_MULTI_DEG], nd);
 }
 __global__ __launch_bounds__(256) void k_marks4(MoveCtx m, const u32* acc_list, const u32* acc_cnt, const u32* node,
                                                  const u32* to, u32* mark, u32 epoch, u32* next, u32* next_cnt) {
@@ -744,7 +906,7 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
   }
   {
     HL_PROF(P_PROP_WAVE, 0, s);
-    hipLaunchKernelGGL(k_prop2_wave<P>, dim3(fixed_grid(list_cap, 4, 4096)), dim3(256), 0, s, pol, g,
+    hipLaunchKernelGGL(k_prop2_wave<P>, dim3(fixed_grid(list_cap, 4, kWaveBlocks)), dim3(256), 0, s, pol, g,
                        ms.blists.p + list_cap, counts + 1, out, st + 7);
     HL_LAUNCH_CHECK();
   }
@@ -783,8 +945,8 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
   }
   MovePolicyT pol{m.f, m.tot, m.gamma, n};
   u32 host_cnt = 0;
-  HL_CHECK(hipMemcpyAsync(&host_cnt, cnt_a, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HL_CHECK(hipStreamSynchronize(s));
+  d2h_async(&host_cnt, cnt_a, sizeof(u32), s);
+  stream_sync(s);
   if (st) st->active = host_cnt;
   u32 rounds = 0;
   while (host_cnt > 0 && int(rounds) < max_rounds) {
@@ -800,10 +962,10 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
       HL_LAUNCH_CHECK();
     }
     u32 h[4];
-    HL_CHECK(hipMemcpyAsync(h, ms.counters.p, 4 * sizeof(u32), hipMemcpyDeviceToHost, s));
+    d2h_async(h, ms.counters.p, 4 * sizeof(u32), s);
     u32 hs[2];
-    HL_CHECK(hipMemcpyAsync(hs, status, 2 * sizeof(u32), hipMemcpyDeviceToHost, s));
-    HL_CHECK(hipStreamSynchronize(s));
+    d2h_async(hs, status, 2 * sizeof(u32), s);
+    stream_sync(s);
     u32 accepted = hs[1];
     if (hs[0] == 1) {
       // many candidates: sort by node, then stably by gain; priority rounds over the grid
@@ -842,8 +1004,8 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
                            ms.minUto.p, ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p, status + 1);
         HL_LAUNCH_CHECK();
         u32 und = 0;
-        HL_CHECK(hipMemcpyAsync(&und, status + 1, sizeof(u32), hipMemcpyDeviceToHost, s));
-        HL_CHECK(hipStreamSynchronize(s));
+        d2h_async(&und, status + 1, sizeof(u32), s);
+        stream_sync(s);
         if (und == 0) break;
         if (++iters >= 48) {
           DevBuf<u8> ea;
@@ -851,7 +1013,7 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
           HL_CHECK(hipMemsetAsync(ea.p, 0, m.comm_cap, s));
           hipLaunchKernelGGL(k_seq_finish, 1, 64, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p, ea.p);
           HL_LAUNCH_CHECK();
-          HL_CHECK(hipStreamSynchronize(s));
+          stream_sync(s);
           break;
         }
       }
@@ -870,9 +1032,9 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
         HL_LAUNCH_CHECK();
       }
       HL_CHECK(hipMemsetAsync(ccnt, 0, sizeof(u32), s));
-      HL_CHECK(hipMemcpyAsync(&accepted, acc_cnt, sizeof(u32), hipMemcpyDeviceToHost, s));
-      HL_CHECK(hipMemcpyAsync(&h[2], next_cnt, sizeof(u32), hipMemcpyDeviceToHost, s));
-      HL_CHECK(hipStreamSynchronize(s));
+      d2h_async(&accepted, acc_cnt, sizeof(u32), s);
+      d2h_async(&h[2], next_cnt, sizeof(u32), s);
+      stream_sync(s);
     }
     ++rounds;
     if (st) st->moved_rounds += accepted;
@@ -895,7 +1057,7 @@ void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, con
   }
   u32* cnt = ms.counters.p + 3;
   u32* ccnt = ms.counters.p + 1;
-  HL_CHECK(hipMemcpyAsync(cnt, &ns, sizeof(u32), hipMemcpyHostToDevice, s));
+  fill_u32(cnt, u32(ns), 1, s);
   HL_CHECK(hipMemsetAsync(ccnt, 0, sizeof(u32), s));
   RefinePolicyT pol{f, sm, dS, blocked, gamma};
   propose_round(pol, g, singles, cnt, ns, maxdeg, ms, th, ms.dstat.p + DS_REF, s);
