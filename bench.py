@@ -101,7 +101,9 @@ def main():
     ap.add_argument("--gamma", type=float, default=1.0)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle batch time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic", default=None, help="JSON with measured HBM bytes per launch of the dominant kernel")
+    ap.add_argument("--traffic", default=None,
+                    help="JSON {slot: HBM bytes per launch} from the FETCH_SIZE / WRITE_SIZE counter runs "
+                         "(default: profiles/traffic_<config>.json when present)")
     ap.add_argument("--dump-profile", default=None)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run device-to-device copies of this many bytes (counter calibration)")
@@ -188,9 +190,11 @@ def main():
         avg_ms = ms / launches
         achieved = (per_launch_bytes / (avg_ms * 1e-3)) / 1e9 if avg_ms > 0 else 0.0
         traffic = None
-        if args.traffic and os.path.exists(args.traffic):
-            tj = json.load(open(args.traffic))
-            traffic = tj.get(dominant)
+        tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_{args.config}.json")
+        if os.path.exists(tpath):
+            tj = json.load(open(tpath))
+            traffic = tj.get(dominant, {}).get("bytes_per_launch") if isinstance(tj.get(dominant), dict) \
+                else tj.get(dominant)
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": SLOT_KERNEL[dominant],
                 "slot": dominant,

@@ -413,6 +413,33 @@ __global__ void k_row_degree(u64 nseg, const u32* seg_row, LevelCsrMut g, Totals
   }
 }
 
+// Throughput mode: one wavefront per updated row, a fixed-shape tree sum
+// (the deterministic mode keeps the sequential row fold above).
+__global__ __launch_bounds__(256) void k_row_degree_wave(u64 nseg, const u32* seg_row, LevelCsrMut g, TotalsRef tr) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (u64 j = wave; j < nseg; j += nwaves) {
+    const u32 r = seg_row[j];
+    const u64 off = g.row_off[r];
+    const u32 len = g.row_len[r];
+    double acc = 0.0;
+    for (u32 i = lane; i < len; i += 64) {
+      const double w = g.w[off + i];
+      acc += w;
+      if (g.col[off + i] == r) acc += w;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) {
+      const double old = g.k[r];
+      g.k[r] = acc;
+      if (acc != old) {
+        if (tr.tot) atomicAdd(&tr.tot[tr.f[r]], acc - old);
+        if (tr.dS) atomicAdd(&tr.dS[tr.s[r]], acc - old);
+      }
+    }
+  }
+}
+
 __global__ void k_copy_rows(LevelCsrMut g, const u64* newoff, u32* col2, double* w2) {
   const u32 lane = threadIdx.x & 63;
   const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
@@ -882,24 +909,41 @@ __global__ void k_parent_idx(const u32* s, u32 n, const u8* is_cand, u32* pidx) 
     if (is_cand[s[v]]) atomicMin(&pidx[s[v]], u32(v));
 }
 // w(S, parent(S)) over all members (hit_leiden.rs:1617-1629), self-loops twice
+// Each wavefront takes 64 vertices and walks the rows of those in candidate
+// subcommunities together (rows can be long at the upper levels).
 __global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u32* s, const u8* is_cand,
                                              const u32* pidx, double* wS) {
-  for (u64 v = gtid(); v < n; v += gstride()) {
-    u32 sv = s[v];
-    if (!is_cand[sv]) continue;
-    u32 C = f[pidx[sv]];
-    u64 off = g.row_off[v];
-    u32 deg = g.row_len[v];
-    double acc = 0.0;
-    for (u32 j = 0; j < deg; ++j) {
-      u32 u = g.col[off + j];
-      if (f[u] == C) {
-        double w = g.w[off + j];
-        acc += w;
-        if (u == v) acc += w;
-      }
+  const u32 lane = threadIdx.x & 63;
+  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
+  for (u64 base = wave * 64; base < n; base += nwaves * 64) {
+    const u64 v0 = base + lane;
+    u32 sv = 0;
+    bool q = false;
+    if (v0 < n) {
+      sv = s[v0];
+      q = is_cand[sv];
     }
-    if (acc != 0.0) atomicAdd(&wS[sv], acc);
+    unsigned long long mask = __ballot(q);
+    while (mask) {
+      const u32 i = u32(__ffsll((long long)mask) - 1);
+      mask &= mask - 1;
+      const u32 v = u32(base + i);
+      const u32 svv = __shfl(sv, i, 64);
+      const u32 C = f[pidx[svv]];
+      const u64 off = g.row_off[v];
+      const u32 deg = g.row_len[v];
+      double acc = 0.0;
+      for (u32 j = lane; j < deg; j += 64) {
+        const u32 u = g.col[off + j];
+        if (f[u] == C) {
+          const double w = g.w[off + j];
+          acc += w;
+          if (u == v) acc += w;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if (lane == 0 && acc != 0.0) atomicAdd(&wS[svv], acc);
+    }
   }
 }
 __global__ void k_blocked(u64 sn, const u8* is_cand, const double* dS, const double* wS, const u32* pidx,
@@ -1447,7 +1491,6 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
 
 void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, const ApplyPlan& plan, bool det,
                         const TotalsRef& tr, hipStream_t s) {
-  (void)det;
   if (plan.nseg == 0) return;
   const u64 ns = plan.nseg;
   {
@@ -1460,7 +1503,10 @@ void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, co
   hipLaunchKernelGGL(k_row_commit, GRID(ns), ns, a.seg_row.p, a.new_len.p, a.newoff.p, pool_base, g);
   HL_LAUNCH_CHECK();
   HL_PROF(P_ROW_DEGREE, 12 * plan.need + 12 * ns, s);
-  hipLaunchKernelGGL(k_row_degree, GRID(ns), ns, a.seg_row.p, g, tr);
+  if (det)
+    hipLaunchKernelGGL(k_row_degree, GRID(ns), ns, a.seg_row.p, g, tr);
+  else
+    hipLaunchKernelGGL(k_row_degree_wave, blocks_for(ns * 64, 256, 4096), 256, 0, s, ns, a.seg_row.p, g, tr);
   HL_LAUNCH_CHECK();
 }
 
@@ -1682,7 +1728,7 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
   hipLaunchKernelGGL(k_parent_idx, GRID(n), sm, n, rs.is_cand.p, rs.parentIdx.p);
   HL_LAUNCH_CHECK();
   fill_f64(rs.wS.p, 0.0, sn, s);
-  hipLaunchKernelGGL(k_ws, GRID(n), g, n, f, sm, rs.is_cand.p, rs.parentIdx.p, rs.wS.p);
+  hipLaunchKernelGGL(k_ws, blocks_for(n, 256, 4096), 256, 0, s, g, n, f, sm, rs.is_cand.p, rs.parentIdx.p, rs.wS.p);
   HL_LAUNCH_CHECK();
   const double* dpar = dpar_in;  // the maintained community totals, when the caller has them
   if (!dpar) {

@@ -430,7 +430,8 @@ __device__ __forceinline__ bool pri_less(u64 ka, u32 na, u64 kb, u32 nb) { retur
 
 __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32* minUto, u32* minUfrom,
                                                      u32* minAto, u32* minAfrom, u32* mark, u32 epoch, u32* next,
-                                                     u32* next_cnt, u32* status, unsigned long long* dst) {
+                                                     u32* next_cnt, u32* status, unsigned long long* dst,
+                                                     u32* cur_cnt, u32* bucket_counts) {
   __shared__ u64 skey[kDecSmall];
   __shared__ u32 snode[kDecSmall];
   __shared__ u32 sidx[kDecSmall];
@@ -692,9 +693,21 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
   if (tid == 0) {
     status[0] = 0;
     status[1] = nacc;
+    // the next round starts from zero counts without a host-side reset
     *cand.cnt = 0;
+    *cur_cnt = 0;
+    for (int i = 0; i < 4; ++i) bucket_counts[i] = 0;
     atomicAdd(&dst[DS_DEC_CAND], (unsigned long long)c);
     atomicAdd(&dst[DS_DEC_ACC], (unsigned long long)nacc);
+  }
+}
+
+// zero the per-round counters the kernels above leave set
+__global__ void k_round_reset(u32* cur_cnt, u32* ccnt, u32* bucket_counts) {
+  if (threadIdx.x == 0) {
+    if (cur_cnt) *cur_cnt = 0;
+    if (ccnt) *ccnt = 0;
+    for (int i = 0; i < 4; ++i) bucket_counts[i] = 0;
   }
 }
 
@@ -891,8 +904,7 @@ u32 MoveScratch::next_epoch(hipStream_t s) {
 template <class P>
 static void propose_round(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u64 list_cap, u32 maxdeg,
                           MoveScratch& ms, const Thresholds& th, unsigned long long* st, hipStream_t s) {
-  u32* counts = ms.counters.p + 4;  // [4..7) bucket counts
-  HL_CHECK(hipMemsetAsync(counts, 0, 4 * sizeof(u32), s));
+  u32* counts = ms.counters.p + 4;  // [4..7) bucket counts, zero on entry
   const u32 small = th.small_deg > SMALL_MAX2 ? SMALL_MAX2 : th.small_deg;
   const u32 wave = th.wave_deg > WAVE_HW2 / 2 ? WAVE_HW2 / 2 : th.wave_deg;
   hipLaunchKernelGGL(k_bucket2, fixed_grid(list_cap), 256, 0, s, list, cnt_dev, g.row_len, small, wave,
@@ -927,10 +939,15 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
                         const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s) {
   const u32 n = m.g.n;
   ms.ensure(n, m.comm_cap, s);
-  u32* cnt_a = ms.counters.p + 0;
+  // counters: [0], [2] list counts (alternating), [1] candidates, [4..8) bucket
+  // counts, [14] overflow flag, [15] accepted moves; a round leaves all but the
+  // next list's count at zero
+  u32* lcnt[2] = {ms.counters.p + 0, ms.counters.p + 2};
   u32* ccnt = ms.counters.p + 1;
-  u32* next_cnt = ms.counters.p + 2;
-  u32* status = ms.counters.p + 14;  // [14..16)
+  u32* bcounts = ms.counters.p + 4;
+  u32* status = ms.counters.p + 14;
+  int ci = 0;
+  u32* cnt_a = lcnt[0];
   HL_CHECK(hipMemsetAsync(ms.counters.p, 0, 16 * sizeof(u32), s));
   u32* cur = ms.list_a.p;
   u32* nxt = ms.list_b.p;
@@ -951,23 +968,23 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
   u32 rounds = 0;
   while (host_cnt > 0 && int(rounds) < max_rounds) {
     if (st) st->evaluated += host_cnt;
-    propose_round(pol, m.g, cur, cnt_a, host_cnt, maxdeg, ms, th, ms.dstat.p + DS_MOVE, s);
+    u32* cur_cnt = lcnt[ci];
+    u32* next_cnt = lcnt[ci ^ 1];
+    propose_round(pol, m.g, cur, cur_cnt, host_cnt, maxdeg, ms, th, ms.dstat.p + DS_MOVE, s);
     const u32 ep = ms.next_epoch(s);
-    HL_CHECK(hipMemsetAsync(next_cnt, 0, sizeof(u32), s));
     CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
     {
       HL_PROF(P_DECOUPLE, 0, s);
       hipLaunchKernelGGL(k_dec_small, 1, 1024, 0, s, m, cand, ms.minUto.p, ms.minUfrom.p, ms.minAto.p,
-                         ms.minAfrom.p, ms.mark.p, ep, nxt, next_cnt, status, ms.dstat.p);
+                         ms.minAfrom.p, ms.mark.p, ep, nxt, next_cnt, status, ms.dstat.p, cur_cnt, bcounts);
       HL_LAUNCH_CHECK();
     }
-    u32 h[4];
-    d2h_async(h, ms.counters.p, 4 * sizeof(u32), s);
-    u32 hs[2];
-    d2h_async(hs, status, 2 * sizeof(u32), s);
+    u32 h[16];
+    d2h_async(h, ms.counters.p, 16 * sizeof(u32), s);
     stream_sync(s);
-    u32 accepted = hs[1];
-    if (hs[0] == 1) {
+    u32 accepted = h[15];
+    u32 next_host = h[2 * (ci ^ 1)];
+    if (h[14] == 1) {
       // many candidates: sort by node, then stably by gain; priority rounds over the grid
       const u32 c = h[1];
       ms.k32.reserve(c, s);
@@ -1031,17 +1048,19 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
                            ms.node.p, ms.to.p, ms.mark.p, ep, nxt, next_cnt);
         HL_LAUNCH_CHECK();
       }
-      HL_CHECK(hipMemsetAsync(ccnt, 0, sizeof(u32), s));
       d2h_async(&accepted, acc_cnt, sizeof(u32), s);
-      d2h_async(&h[2], next_cnt, sizeof(u32), s);
+      d2h_async(&next_host, next_cnt, sizeof(u32), s);
+      hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, cur_cnt, ccnt, bcounts);
+      HL_LAUNCH_CHECK();
+      fill_u32(status, 0, 2, s);
       stream_sync(s);
     }
     ++rounds;
     if (st) st->moved_rounds += accepted;
     if (accepted == 0) break;  // throughput.rs:177-180 (every accepted gain is > 0)
     std::swap(cur, nxt);
-    HL_CHECK(hipMemcpyAsync(cnt_a, next_cnt, sizeof(u32), hipMemcpyDeviceToDevice, s));
-    host_cnt = h[2];
+    ci ^= 1;
+    host_cnt = next_host;
   }
   return rounds;
 }
@@ -1058,7 +1077,8 @@ void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, con
   u32* cnt = ms.counters.p + 3;
   u32* ccnt = ms.counters.p + 1;
   fill_u32(cnt, u32(ns), 1, s);
-  HL_CHECK(hipMemsetAsync(ccnt, 0, sizeof(u32), s));
+  hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, nullptr, ccnt, ms.counters.p + 4);
+  HL_LAUNCH_CHECK();
   RefinePolicyT pol{f, sm, dS, blocked, gamma};
   propose_round(pol, g, singles, cnt, ns, maxdeg, ms, th, ms.dstat.p + DS_REF, s);
   CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
@@ -1067,7 +1087,8 @@ void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, con
   hipLaunchKernelGGL(k_ref_apply, fixed_grid(ns), 256, 0, s, cand, ms.target.p, sm);
   hipLaunchKernelGGL(k_ref_clear_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);
   HL_LAUNCH_CHECK();
-  HL_CHECK(hipMemsetAsync(ccnt, 0, sizeof(u32), s));
+  hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, nullptr, ccnt, ms.counters.p + 4);
+  HL_LAUNCH_CHECK();
 }
 
 }  // namespace hl
