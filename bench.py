@@ -32,7 +32,8 @@ METRIC = "incremental edges/s + p99 update latency vs plain GPU path; HBM GB/s"
 # the proposal kernels), with the kernel name rocprofv3 reports; the dominant
 # kernel of the roofline is the one of these with the most time
 SLOT_KERNEL = {
-    "propose_small": "k_prop2_small", "propose_wave": "k_prop2_wave", "propose_big": "k_prop2_big",
+    "propose_small": "k_prop2_small", "propose_wave": "k_prop2_wave", "propose_lds": "k_prop2_lds",
+    "propose_big": "k_prop2_big",
     "decouple": "k_dec_small", "apply_moves": "k_apply4", "mark_moved": "k_marks4",
     "row_merge": "k_row_merge", "resolve_runs": "k_resolve_runs", "row_degree": "k_row_degree",
 }
@@ -144,7 +145,9 @@ def main():
     ls = g.load(hl.GraphInput(w.n, w.u, w.v, w.w))
     load_s = time.time() - t0
     q0 = g.modularity(hl.PART_COMMUNITY)
-    log(f"[bench] load {load_s:.2f}s Q={q0:.4f} levels={ls.as_dict()['level_n']}")
+    free, total = hip.mem_info()
+    log(f"[bench] load {load_s:.2f}s Q={q0:.4f} levels={ls.as_dict()['level_n']} "
+        f"device memory in use {(total - free) / 2**30:.1f} GiB")
 
     dev = []
     for (u, v, a) in w.batches[:nb]:
@@ -156,6 +159,8 @@ def main():
     for i in range(args.warmup):
         k, du, dv, da = dev[i]
         g.apply_batch_device(k, du.address, dv.address, da.address)
+    free, total = hip.mem_info()
+    log(f"[bench] after warmup: device memory in use {(total - free) / 2**30:.1f} GiB")
     prof_warm = g.profile_read()
     single = {k: v for k, v in prof_warm.items() if k in SLOT_KERNEL}
     dominant = max(single.items(), key=lambda kv: kv[1][1])[0] if single else None
@@ -221,7 +226,17 @@ def main():
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
         log(f"[bench] cpu baseline: oracle with {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(w, args.levels, args.gamma, args.cpu_budget, threads)
+        if args.config == "large":
+            # the oracle needs minutes for the scale-24 load: time it on a scale-20
+            # R-MAT of the same parameters and batch shape instead
+            from hit_leiden_amd import gen
+            del w
+            ws = gen.large_config(scale=20, batches=nb)
+            cb = cpu_baseline(ws, args.levels, args.gamma, args.cpu_budget, threads)
+            cb["sample"] = "R-MAT scale 20 (same a,b,c, edge factor, batch shape) instead of scale 24: " + cb["sample"]
+            out["cpu_baseline"] = cb
+        else:
+            out["cpu_baseline"] = cpu_baseline(w, args.levels, args.gamma, args.cpu_budget, threads)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -144,6 +144,24 @@ struct Engine {
 
   bool par() const { return mode == HL_MODE_THROUGHPUT; }
 
+  // Release scratch sized by the largest call so far (the initial load is a
+  // whole-graph batch); it grows back to batch size on demand.
+  template <class T>
+  static void renew(T& x) {
+    x.~T();
+    new (&x) T();
+  }
+  void trim() {
+    stream_sync(s);
+    renew(as);
+    renew(ag);
+    renew(rs);
+    renew(ps);
+    renew(tmp);
+    renew(dnext);
+    mvs.trim();
+  }
+
   // make level p's maintained community totals cover every label
   void ensure_totals(Level& lv) {
     const u64 need = std::max<u64>(std::max<u64>(2ull * lv.n, lv.lab_bound), 1) + 1;
@@ -167,20 +185,18 @@ struct Engine {
     apply_delta_plan(d, k, g, as, tmp, plan, s);
     if (plan.nseg) {
       if (lv.pool_used + plan.need > lv.col.cap) {
-        // compact live rows into a fresh pool with room for this batch
-        u64 live = 0;
+        // compact live rows into a fresh pool with room for this batch and headroom
+        const u64 live = compact_plan(g, offtmp, tmp, s);
         {
           DevBuf<u32> col2;
           DevBuf<double> w2;
-          // size: live rows + need, with headroom
-          u64 live_est = lv.pool_used;  // upper bound of live entries
-          u64 want = std::max<u64>(live_est + plan.need, 1024);
-          col2.reserve(want, s, false, 1.5);
-          w2.reserve(want, s, false, 1.5);
-          live = compact_rows(g, col2.p, w2.p, offtmp, tmp, s);
+          const u64 want = std::max<u64>(live + plan.need, 1024);
+          col2.reserve(want, s, false, 1.25);
+          w2.reserve(want, s, false, 1.25);
+          compact_copy(g, offtmp.p, col2.p, w2.p, s);
           lv.col.swap(col2);
           lv.w.swap(w2);
-        }
+        }  // the old pool is released here, after compact_copy synchronised
         lv.pool_used = live;
         g = lv.mut();
       }
@@ -318,10 +334,10 @@ struct Engine {
   void account_work(const unsigned long long* dnow, hl_batch_stats* st) {
     u64 d[DS_COUNT];
     for (u32 i = 0; i < DS_COUNT; ++i) d[i] = dnow[i] - dlast[i], dlast[i] = dnow[i];
-    u64 prop[3], total = 0;
-    for (int b = 0; b < 3; ++b) {
-      prop[b] = 20 * d[DS_MOVE + b] + 16 * d[DS_MOVE + 3 + b] + 8 * d[DS_MOVE + 6 + b] + 24 * d[DS_REF + b] +
-                20 * d[DS_REF + 3 + b] + 16 * d[DS_REF + 6 + b];
+    u64 prop[kBuckets], total = 0;
+    for (u32 b = 0; b < kBuckets; ++b) {
+      prop[b] = 20 * d[DS_MOVE + b] + 16 * d[DS_MOVE + kBuckets + b] + 8 * d[DS_MOVE + 2 * kBuckets + b] +
+                24 * d[DS_REF + b] + 20 * d[DS_REF + kBuckets + b] + 16 * d[DS_REF + 2 * kBuckets + b];
       total += prop[b];
     }
     const u64 dec = 20 * d[DS_DEC_CAND] + 28 * d[DS_DEC_ACC] + 12 * d[DS_DEC_DEG];
@@ -329,10 +345,10 @@ struct Engine {
     total += dec + multi + apply + marks;
     if (st) st->bytes_alg += total;
     if (prof.on) {
-      const int slots[7] = {P_PROP_SMALL, P_PROP_WAVE, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI, P_APPLY_MOVES,
-                            P_MARK_MOVED};
-      const u64 by[7] = {prop[0], prop[1], prop[2], dec, multi, apply, marks};
-      for (int i = 0; i < 7; ++i)
+      const int slots[8] = {P_PROP_SMALL, P_PROP_WAVE, P_PROP_LDS, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI,
+                            P_APPLY_MOVES, P_MARK_MOVED};
+      const u64 by[8] = {prop[0], prop[1], prop[2], prop[3] + prop[4], dec, multi, apply, marks + 0};
+      for (int i = 0; i < 8; ++i)
         if (prof.records(slots[i])) prof.bytes[slots[i]] += by[i];
     }
   }
@@ -596,6 +612,7 @@ hl_status hl_load(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, cons
     e.upload_batch(m, src, dst, w, n);
     e.reset(n, m);
     e.run_batch(m, stats);
+    e.trim();
     if (stats) stats->seconds = now_s() - t0;
   });
 }

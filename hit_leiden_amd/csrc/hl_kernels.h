@@ -94,8 +94,11 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
                       hipStream_t s);
 void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, const ApplyPlan& plan, bool det,
                         const TotalsRef& tr, hipStream_t s);
-// compact all rows into a fresh pool [col2, w2) with row_cap = row_len; returns live nnz
-u64 compact_rows(const LevelCsrMut& g, u32* col2, double* w2, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s);
+// Pool compaction in two steps: compact_plan returns the live entries and
+// leaves the new row offsets in off_tmp; compact_copy moves every row into
+// [col2, w2) (row_cap = row_len) and synchronises.
+u64 compact_plan(const LevelCsrMut& g, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s);
+void compact_copy(const LevelCsrMut& g, const u64* offs, u32* col2, double* w2, hipStream_t s);
 // canonical 2m (pairwise tree); partials needs twice_m_scratch(n) doubles
 u64 twice_m_scratch(u32 n);
 void twice_m_tree(const double* k, u32 n, double* partials, double* out, hipStream_t s);

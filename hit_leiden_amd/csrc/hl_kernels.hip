@@ -1510,19 +1510,22 @@ void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, co
   HL_LAUNCH_CHECK();
 }
 
-u64 compact_rows(const LevelCsrMut& g, u32* col2, double* w2, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s) {
+u64 compact_plan(const LevelCsrMut& g, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s) {
   off_tmp.reserve(u64(g.n) + 1, s);
   DevBuf<u64> lens;
   lens.reserve(g.n + 1, s);
   hipLaunchKernelGGL(k_len_u64, GRID(g.n), g.row_len, g.n, lens.p);
   HL_LAUNCH_CHECK();
-  u64 total = exclusive_scan_u64(lens.p, off_tmp.p, g.n, tmp, s);
-  hipLaunchKernelGGL(k_copy_rows, GRID(u64(g.n) * 64), g, off_tmp.p, col2, w2);
+  const u64 total = exclusive_scan_u64(lens.p, off_tmp.p, g.n, tmp, s);  // synchronises
+  return total;
+}
+
+void compact_copy(const LevelCsrMut& g, const u64* offs, u32* col2, double* w2, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_rows, GRID(u64(g.n) * 64), g, offs, col2, w2);
   HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_set_offsets, GRID(g.n), g, off_tmp.p);
+  hipLaunchKernelGGL(k_set_offsets, GRID(g.n), g, offs);
   HL_LAUNCH_CHECK();
   stream_sync(s);
-  return total;
 }
 
 u64 twice_m_scratch(u32 n) {

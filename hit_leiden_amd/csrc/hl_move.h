@@ -15,6 +15,14 @@ struct CandOut {
   u32* cnt;
 };
 
+// accepted moves whose neighbour lists are marked over the grid
+struct FlatMarks {
+  u32* node;
+  u32* to;
+  u64* pre;  // exclusive prefix of the list lengths, pre[c] = total
+  u32* c;    // device count (0: nothing to do)
+};
+
 // what a movement round reads and writes at one level
 struct MoveCtx {
   Csr g;
@@ -75,18 +83,19 @@ struct RefinePolicyT {
 
 // Device work counters (monotone u64), read by the host once per batch.
 // A proposal block (DS_MOVE for movement, DS_REF for refinement) holds per
-// degree bucket b (0 lane, 1 wave, 2 workgroup): [b] vertices evaluated,
-// [3 + b] row entries read, [6 + b] distinct candidate communities.
+// degree bucket b (kBuckets of them): [b] vertices evaluated, [kBuckets + b]
+// row entries read, [2 kBuckets + b] distinct candidate communities.
+constexpr u32 kBuckets = 5;  // lane, wavefront, workgroup+LDS, workgroup+HBM, giant
 enum DevStat : u32 {
   DS_MOVE = 0,
-  DS_REF = 9,
-  DS_DEC_CAND = 18,   // candidates decoupled in one workgroup
-  DS_DEC_ACC = 19,    // of which accepted
-  DS_DEC_DEG = 20,    // row entries scanned for next-active marks
-  DS_MULTI_CAND = 21, // the same for the multi-kernel path
-  DS_MULTI_ACC = 22,
-  DS_MULTI_DEG = 23,
-  DS_COUNT = 24
+  DS_REF = 3 * kBuckets,
+  DS_DEC_CAND = 6 * kBuckets,  // candidates decoupled in one workgroup
+  DS_DEC_ACC,                  // of which accepted
+  DS_DEC_DEG,                  // row entries scanned for next-active marks
+  DS_MULTI_CAND,               // the same for the multi-kernel path
+  DS_MULTI_ACC,
+  DS_MULTI_DEG,
+  DS_COUNT
 };
 
 struct MoveScratch {
@@ -94,14 +103,17 @@ struct MoveScratch {
   DevBuf<u32> mark, list_a, list_b, blists, cnode, cto, counters;
   DevBuf<double> cgain;
   DevBuf<u32> minUto, minUfrom, minAto, minAfrom;
-  DevBuf<u32> tk;
+  DevBuf<u32> tk;  // workgroup hub tables
   DevBuf<double> tv;
   DevBuf<u32> k32, k32b, v32, o1, o2, node, from, to, state, acc;
   DevBuf<u64> k64, k64b;
+  DevBuf<u32> mnode, mto;
+  DevBuf<u64> mpre, mdeg;
   DevBuf<u8> target;
   size_t target_cap = 0;
   u32 epoch = 0;
   void ensure(u64 n, u64 comm_cap, hipStream_t s);
+  void trim();  // release the candidate-sized buffers of the multi-kernel path
   u32 next_epoch(hipStream_t s);
 };
 

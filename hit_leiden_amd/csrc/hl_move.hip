@@ -22,7 +22,14 @@ constexpr u32 WAVE_HW2 = 1024;
 constexpr u32 kUndec = 0, kAcc = 1, kRej = 2;
 constexpr u32 kDecLds = 1024;   // candidates whose priority rounds run on LDS slots
 constexpr u32 kDecHash = 4096;  // LDS hash slots for their 2 x kDecLds communities
-constexpr u32 kWaveBlocks = 768;  // 3 resident workgroups per CU at 52 KB of LDS each
+constexpr u32 kWaveBlocks = 768;
+constexpr u32 kLdsDeg = 4096;       // workgroup-per-vertex with an LDS table up to this degree
+constexpr u32 kLdsCap = 8192;       // its table slots (big_cap(kLdsDeg))
+constexpr u32 kBigDeg = 1u << 18;   // workgroup-per-vertex with an HBM table up to this degree
+constexpr u64 kMarksInBlock = 8192;  // neighbour entries marked inside the decoupling workgroup
+constexpr unsigned kMarksBlocks = 1024;
+constexpr u32 kMarkStage = 2048;
+constexpr u64 kBigTableBytes = 3ull << 30;  // hub tables of the workgroup-per-vertex proposals  // 3 resident workgroups per CU at 52 KB of LDS each
 
 __device__ __forceinline__ u64 gtid() { return u64(blockIdx.x) * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ u64 gstride() { return u64(gridDim.x) * blockDim.x; }
@@ -98,8 +105,18 @@ __device__ __forceinline__ u32 lds_insert(u32* K, u32 cap, u32 key) {
     h = (h + 1) & (cap - 1);
   }
 }
-// Add v to a list once per epoch.
-__device__ __forceinline__ bool claim(u32* mark, u32 v, u32 epoch) { return atomicExch(&mark[v], epoch) != epoch; }
+// Add v to a list once per epoch (a plain read first spares the atomic for
+// vertices already claimed, which hubs make common).
+__device__ __forceinline__ bool claim(u32* mark, u32 v, u32 epoch) {
+  if (mark[v] == epoch) return false;
+  return atomicExch(&mark[v], epoch) != epoch;
+}
+// Workgroup-table slots for a hub of degree deg (load factor at most 2/3).
+__host__ __device__ inline u32 big_cap(u32 deg) {
+  u32 c = 64;
+  while (c < deg + (deg >> 1)) c <<= 1;
+  return c;
+}
 
 __device__ __forceinline__ u32 tab_insert(u32* K, u32 cap, u32 key) {
   u32 h = hash32(key) & (cap - 1);
@@ -165,29 +182,36 @@ __device__ void block_add_u64(unsigned long long* dst, unsigned long long x) {
 }
 
 // ------------------------------------------------------------------ buckets
+// Split the active list by degree: lane (<= small_deg), wavefront (<= wave_deg),
+// workgroup with an LDS table (<= kLdsDeg), workgroup with an HBM table
+// (<= kBigDeg), giant.
 __global__ void k_bucket2(const u32* verts, const u32* cnt_dev, const u32* row_len, u32 small_deg, u32 wave_deg,
                           u32* lists, u64 cap, u32* counts, unsigned long long* st) {
   const u32 cnt = *cnt_dev;
-  unsigned long long local[3] = {0, 0, 0}, lcnt[3] = {0, 0, 0};
+  unsigned long long local[kBuckets] = {}, lcnt[kBuckets] = {};
   for (u64 base = u64(blockIdx.x) * blockDim.x; base < cnt; base += gstride()) {
     u64 i = base + threadIdx.x;
-    u32 b = 3;
+    u32 b = kBuckets;
     u32 v = 0;
     if (i < cnt) {
       v = verts[i];
-      u32 d = row_len[v];
-      b = d <= small_deg ? 0 : (d <= wave_deg ? 1 : 2);
-      local[b] += d;
-      lcnt[b] += 1;
+      const u32 d = row_len[v];
+      b = d <= small_deg ? 0 : d <= wave_deg ? 1 : d <= kLdsDeg ? 2 : d <= kBigDeg ? 3 : 4;
+#pragma unroll
+      for (u32 q = 0; q < kBuckets; ++q)
+        if (q == b) {
+          local[q] += d;
+          lcnt[q] += 1;
+        }
     }
-    for (u32 q = 0; q < 3; ++q) {
+    for (u32 q = 0; q < kBuckets; ++q) {
       u32 p = wave_append(b == q, &counts[q]);
       if (b == q) lists[q * cap + p] = v;
     }
   }
-  for (int q = 0; q < 3; ++q) {
+  for (u32 q = 0; q < kBuckets; ++q) {
     block_add_u64(&st[q], lcnt[q]);
-    block_add_u64(&st[3 + q], local[q]);
+    block_add_u64(&st[kBuckets + q], local[q]);
   }
 }
 
@@ -363,10 +387,70 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
   block_add_u64(cst, ncand);
 }
 
-// one workgroup per vertex (hubs); each resident workgroup owns a table of tcap slots
+// one workgroup per vertex of degree <= kLdsDeg, the table in LDS
 template <class P>
-__global__ __launch_bounds__(256) void k_prop2_big(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* TK,
-                                                    double* TV, u32 tcap, CandOut out, unsigned long long* cst) {
+__global__ __launch_bounds__(1024) void k_prop2_lds(P pol, Csr g, const u32* list, const u32* cnt_dev, CandOut out,
+                                                     unsigned long long* cst) {
+  __shared__ u32 K[kLdsCap];
+  __shared__ double V[kLdsCap];
+  const u32 cnt = *cnt_dev;
+  unsigned long long ncand = 0;
+  const double tt = *g.twice_m;
+  for (u32 id = blockIdx.x; id < cnt; id += gridDim.x) {
+    const u32 v = list[id];
+    const u64 off = g.row_off[v];
+    const u32 deg = g.row_len[v];
+    const u32 cur = pol.cur_key(v);
+    const u32 cap = big_cap(deg);
+    for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
+      K[q] = kNone;
+      V[q] = 0.0;
+    }
+    __syncthreads();
+    for (u32 j = threadIdx.x; j < deg; j += blockDim.x) {
+      const u32 u = g.col[off + j];
+      u32 key;
+      int mult;
+      if (!pol.accept(v, u, key, mult)) continue;
+      const double w = g.w[off + j];
+      const u32 h = tab_insert(K, cap, key);
+      atomicAdd(&V[h], w);
+      if (mult == 2) atomicAdd(&V[h], w);
+    }
+    __syncthreads();
+    const double wcur = tab_lookup(K, V, cap, cur);
+    const double kv = g.k[v], sc = pol.cur_total(cur);
+    double bg = 0.0;
+    u32 best = kNone;
+    for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
+      const u32 key = K[q];
+      if (key == kNone) continue;
+      ++ncand;
+      if (key == cur || !pol.ok(key)) continue;
+      const double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+      if (gn > 0.0) better(bg, best, gn, key);
+    }
+    block_argmax(bg, best);
+    if (threadIdx.x == 0) {
+      finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
+      if (bg > 0.0) {
+        const u32 p = atomicAdd(out.cnt, 1u);
+        out.node[p] = v;
+        out.to[p] = best;
+        out.gain[p] = bg;
+      }
+    }
+    __syncthreads();
+  }
+  block_add_u64(cst, ncand);
+}
+
+// one workgroup per vertex (hubs); each resident workgroup owns an HBM table
+// of tcap slots (cleared per vertex: coalesced, cheaper than tracking the
+// used slots, whose clears and scans are random accesses)
+template <class P>
+__global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* TK,
+                                                     double* TV, u32 tcap, CandOut out, unsigned long long* cst) {
   const u32 cnt = *cnt_dev;
   unsigned long long ncand = 0;
   u32* K = TK + u64(blockIdx.x) * tcap;
@@ -377,23 +461,35 @@ __global__ __launch_bounds__(256) void k_prop2_big(P pol, Csr g, const u32* list
     const u64 off = g.row_off[v];
     const u32 deg = g.row_len[v];
     const u32 cur = pol.cur_key(v);
-    u32 cap = 64;
-    while (cap < 2 * deg) cap <<= 1;
+    const u32 cap = big_cap(deg);
     for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
       K[q] = kNone;
       V[q] = 0.0;
     }
-    __threadfence_block();
+    __threadfence();
     __syncthreads();
-    for (u32 j = threadIdx.x; j < deg; j += blockDim.x) {
-      u32 u = g.col[off + j];
-      u32 key;
-      int mult;
-      if (!pol.accept(v, u, key, mult)) continue;
-      double w = g.w[off + j];
-      u32 h = tab_insert(K, cap, key);
-      atomicAdd(&V[h], w);
-      if (mult == 2) atomicAdd(&V[h], w);
+    // four row entries per thread in flight: the gathers are independent
+    for (u32 j0 = threadIdx.x; j0 < deg; j0 += 4 * blockDim.x) {
+      u32 uu[4], kk[4];
+      double ww[4];
+      int mm[4];
+      bool ok[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const u32 j = j0 + r * blockDim.x;
+        ok[r] = j < deg;
+        uu[r] = ok[r] ? g.col[off + j] : 0;
+        ww[r] = ok[r] ? g.w[off + j] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ok[r] = ok[r] && pol.accept(v, uu[r], kk[r], mm[r]);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (!ok[r]) continue;
+        const u32 h = tab_insert(K, cap, kk[r]);
+        atomicAdd(&V[h], ww[r]);
+        if (mm[r] == 2) atomicAdd(&V[h], ww[r]);
+      }
     }
     __threadfence();
     __syncthreads();
@@ -402,18 +498,18 @@ __global__ __launch_bounds__(256) void k_prop2_big(P pol, Csr g, const u32* list
     double bg = 0.0;
     u32 best = kNone;
     for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
-      u32 key = K[q];
+      const u32 key = K[q];
       if (key == kNone) continue;
       ++ncand;
       if (key == cur || !pol.ok(key)) continue;
-      double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+      const double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
       if (gn > 0.0) better(bg, best, gn, key);
     }
     block_argmax(bg, best);
     if (threadIdx.x == 0) {
       finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
       if (bg > 0.0) {
-        u32 p = atomicAdd(out.cnt, 1u);
+        const u32 p = atomicAdd(out.cnt, 1u);
         out.node[p] = v;
         out.to[p] = best;
         out.gain[p] = bg;
@@ -431,7 +527,7 @@ __device__ __forceinline__ bool pri_less(u64 ka, u32 na, u64 kb, u32 nb) { retur
 __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32* minUto, u32* minUfrom,
                                                      u32* minAto, u32* minAfrom, u32* mark, u32 epoch, u32* next,
                                                      u32* next_cnt, u32* status, unsigned long long* dst,
-                                                     u32* cur_cnt, u32* bucket_counts) {
+                                                     u32* cur_cnt, u32* bucket_counts, FlatMarks flat) {
   __shared__ u64 skey[kDecSmall];
   __shared__ u32 snode[kDecSmall];
   __shared__ u32 sidx[kDecSmall];
@@ -441,10 +537,14 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
   __shared__ u32 mAto[kDecHash], mAfrom[kDecHash];
   __shared__ u16 fslot[kDecLds], tslot[kDecLds];
   __shared__ u32 und, nacc;
+  __shared__ u64 wsum[16], mtotal;
   const u32 c = *cand.cnt;
   const u32 tid = threadIdx.x;
   if (c > kDecSmall) {
-    if (tid == 0) status[0] = 1;  // multi-kernel path
+    if (tid == 0) {
+      status[0] = 1;  // multi-kernel path
+      *flat.c = 0;
+    }
     return;
   }
   if (c <= 64) {
@@ -660,35 +760,73 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
     __threadfence();
     __syncthreads();
   }
-  // next-active and K marks (throughput.rs:189-199)
-  const u32 lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
-  unsigned long long scanned = 0;
-  for (u32 p = wv; p < c; p += nw) {
-    if (sst[p] != kAcc) continue;
-    const u32 v = snode[p], cto = sto[p];
-    const u64 off = m.g.row_off[v];
-    const u32 deg = m.g.row_len[v];
-    if (lane == 0) scanned += deg;
-    const u32 sv = m.s[v];
-    bool kv = false;
-    for (u32 j0 = 0; j0 < deg; j0 += 64) {
-      u32 j = j0 + lane;
+  // next-active and K marks (throughput.rs:189-199) over the flattened
+  // neighbour lists of the accepted moves, so that a hub spreads over the
+  // whole block; long lists go to k_marks_flat, which spreads them over the grid
+  const u32 lane = tid & 63, wv = tid >> 6;
+  u64* pre = skey;  // exclusive degree prefix over positions (the sort keys are dead)
+  {
+    const u32 base = tid * 4;  // c <= 4 * blockDim.x
+    u64 loc[4], sum = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const u32 p = base + r;
+      loc[r] = sum;
+      if (p < c && sst[p] == kAcc) sum += m.g.row_len[snode[p]];
+    }
+    u64 x = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+      const u32 lo = __shfl_up(u32(x), o, 64), hi = __shfl_up(u32(x >> 32), o, 64);
+      if (lane >= u32(o)) x += (u64(hi) << 32) | lo;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    u64 before = 0;
+    for (u32 i = 0; i < wv; ++i) before += wsum[i];
+    const u64 excl = before + x - sum;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (base + r < c) pre[base + r] = excl + loc[r];
+    if (tid == blockDim.x - 1) mtotal = before + x;
+    __syncthreads();
+  }
+  const u64 total = mtotal;
+  if (total <= kMarksInBlock) {
+    for (u64 t0 = 0; t0 < total; t0 += blockDim.x) {
+      const u64 t = t0 + tid;
       bool want = false;
       u32 u = 0;
-      if (j < deg) {
-        u = m.g.col[off + j];
+      if (t < total) {
+        u32 lo = 0, hi = c;  // pre[lo] <= t < pre[hi], with pre[c] = total
+        while (hi - lo > 1) {
+          const u32 mid = (lo + hi) >> 1;
+          if (pre[mid] <= t) lo = mid;
+          else hi = mid;
+        }
+        const u32 v = snode[lo], cto = sto[lo];
+        u = m.g.col[m.g.row_off[v] + (t - pre[lo])];
         if (ld_agent(&m.f[u]) != cto) want = claim(mark, u, epoch);
-        if (m.s[u] == sv) {
+        if (m.s[u] == m.s[v]) {
           m.K[u] = 1;
-          kv = true;
+          m.K[v] = 1;
         }
       }
-      u32 q = wave_append(want, next_cnt);
+      const u32 q = wave_append(want, next_cnt);
       if (want) next[q] = u;
     }
-    if (__any(kv) && lane == 0) m.K[v] = 1;
+    if (tid == 0) *flat.c = 0;
+  } else {
+    for (u32 p = tid; p < c; p += blockDim.x) {
+      flat.node[p] = snode[p];
+      flat.to[p] = sto[p];
+      flat.pre[p] = pre[p];
+    }
+    if (tid == 0) {
+      flat.pre[c] = total;
+      *flat.c = c;
+    }
   }
-  if (lane == 0 && scanned) atomicAdd(&dst[DS_DEC_DEG], scanned);
+  if (tid == 0 && total) atomicAdd(&dst[total <= kMarksInBlock ? DS_DEC_DEG : DS_MULTI_DEG], (unsigned long long)total);
   __syncthreads();
   if (tid == 0) {
     status[0] = 0;
@@ -696,10 +834,61 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
     // the next round starts from zero counts without a host-side reset
     *cand.cnt = 0;
     *cur_cnt = 0;
-    for (int i = 0; i < 4; ++i) bucket_counts[i] = 0;
+    for (u32 i = 0; i < kBuckets; ++i) bucket_counts[i] = 0;
     atomicAdd(&dst[DS_DEC_CAND], (unsigned long long)c);
     atomicAdd(&dst[DS_DEC_ACC], (unsigned long long)nacc);
   }
+}
+
+// Marks of long neighbour lists over the whole grid: positions p < *flat.c
+// with flat.pre the exclusive prefix of their list lengths (flat.pre[c] =
+// total); next-active vertices are staged per workgroup and appended with one
+// device atomic per flush.
+__global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u32* mark, u32 epoch, u32* next,
+                                                    u32* next_cnt) {
+  __shared__ u32 sv_[kMarkStage];
+  __shared__ u32 scnt, sbase;
+  const u32 c = *flat.c;
+  if (c == 0) return;
+  if (threadIdx.x == 0) scnt = 0;
+  __syncthreads();
+  const u64 total = flat.pre[c];
+  for (u64 base = u64(blockIdx.x) * blockDim.x; base < total; base += u64(gridDim.x) * blockDim.x) {
+    const u64 t = base + threadIdx.x;
+    if (t < total) {
+      u32 lo = 0, hi = c;
+      while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (flat.pre[mid] <= t) lo = mid;
+        else hi = mid;
+      }
+      const u32 v = flat.node[lo], cto = flat.to[lo];
+      const u32 u = m.g.col[m.g.row_off[v] + (t - flat.pre[lo])];
+      if (m.f[u] != cto && claim(mark, u, epoch)) sv_[atomicAdd(&scnt, 1u)] = u;
+      if (m.s[u] == m.s[v]) {
+        m.K[u] = 1;
+        m.K[v] = 1;
+      }
+    }
+    __syncthreads();
+    if (scnt > kMarkStage - blockDim.x) {
+      if (threadIdx.x == 0) sbase = atomicAdd(next_cnt, scnt);
+      __syncthreads();
+      for (u32 i = threadIdx.x; i < scnt; i += blockDim.x) next[sbase + i] = sv_[i];
+      __syncthreads();
+      if (threadIdx.x == 0) scnt = 0;
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  if (scnt) {
+    if (threadIdx.x == 0) sbase = atomicAdd(next_cnt, scnt);
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < scnt; i += blockDim.x) next[sbase + i] = sv_[i];
+  }
+}
+__global__ void k_acc_degrees(u64 c, const u32* node, const u32* state, const u32* row_len, u64* deg) {
+  for (u64 p = gtid(); p < c; p += gstride()) deg[p] = state[p] == kAcc ? row_len[node[p]] : 0;
 }
 
 // zero the per-round counters the kernels above leave set
@@ -707,7 +896,7 @@ __global__ void k_round_reset(u32* cur_cnt, u32* ccnt, u32* bucket_counts) {
   if (threadIdx.x == 0) {
     if (cur_cnt) *cur_cnt = 0;
     if (ccnt) *ccnt = 0;
-    for (int i = 0; i < 4; ++i) bucket_counts[i] = 0;
+    for (u32 i = 0; i < kBuckets; ++i) bucket_counts[i] = 0;
   }
 }
 
@@ -808,36 +997,6 @@ __global__ void k_apply4(u64 c, const u32* node, const u32* from, const u32* to,
   block_add_u64(&dst[DS_MULTI_ACC], na);
   block_add_u64(&dst[DS_MULTI_DEG], nd);
 }
-__global__ __launch_bounds__(256) void k_marks4(MoveCtx m, const u32* acc_list, const u32* acc_cnt, const u32* node,
-                                                 const u32* to, u32* mark, u32 epoch, u32* next, u32* next_cnt) {
-  const u32 na = *acc_cnt;
-  const u32 lane = threadIdx.x & 63;
-  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
-  for (u64 q = wave; q < na; q += nwaves) {
-    u32 p = acc_list[q];
-    u32 v = node[p], c = to[p];
-    u64 off = m.g.row_off[v];
-    u32 deg = m.g.row_len[v];
-    u32 sv = m.s[v];
-    bool kv = false;
-    for (u32 j0 = 0; j0 < deg; j0 += 64) {
-      u32 j = j0 + lane;
-      bool want = false;
-      u32 u = 0;
-      if (j < deg) {
-        u = m.g.col[off + j];
-        if (m.f[u] != c) want = claim(mark, u, epoch);
-        if (m.s[u] == sv) {
-          m.K[u] = 1;
-          kv = true;
-        }
-      }
-      u32 pos = wave_append(want, next_cnt);
-      if (want) next[pos] = u;
-    }
-    if (__any(kv) && lane == 0) m.K[v] = 1;
-  }
-}
 
 // ------------------------------------------------------------------ refinement helpers
 __global__ void k_ref_targets(CandOut cand, u8* target) {
@@ -880,17 +1039,27 @@ void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
   }
   list_a.reserve(n + 1, s);
   list_b.reserve(n + 1, s);
-  blists.reserve(3 * (n + 1), s);
+  blists.reserve(kBuckets * (n + 1), s);
   cnode.reserve(n + 1, s);
   cto.reserve(n + 1, s);
   cgain.reserve(n + 1, s);
   counters.reserve(16, s);
+  mnode.reserve(kDecSmall + 1, s);
+  mto.reserve(kDecSmall + 1, s);
+  mpre.reserve(kDecSmall + 2, s);
   if (minUto.cap < comm_cap) {
     minUto.reserve(comm_cap, s);
     minUfrom.reserve(comm_cap, s);
     minAto.reserve(comm_cap, s);
     minAfrom.reserve(comm_cap, s);
   }
+}
+
+void MoveScratch::trim() {
+  for (DevBuf<u32>* b : {&tk, &k32, &k32b, &v32, &o1, &o2, &node, &from, &to, &state, &acc}) b->release();
+  tv.release();
+  k64.release();
+  k64b.release();
 }
 
 u32 MoveScratch::next_epoch(hipStream_t s) {
@@ -904,33 +1073,43 @@ u32 MoveScratch::next_epoch(hipStream_t s) {
 template <class P>
 static void propose_round(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u64 list_cap, u32 maxdeg,
                           MoveScratch& ms, const Thresholds& th, unsigned long long* st, hipStream_t s) {
-  u32* counts = ms.counters.p + 4;  // [4..7) bucket counts, zero on entry
+  u32* counts = ms.counters.p + 4;  // [4..4 + kBuckets) bucket counts, zero on entry
   const u32 small = th.small_deg > SMALL_MAX2 ? SMALL_MAX2 : th.small_deg;
   const u32 wave = th.wave_deg > WAVE_HW2 / 2 ? WAVE_HW2 / 2 : th.wave_deg;
   hipLaunchKernelGGL(k_bucket2, fixed_grid(list_cap), 256, 0, s, list, cnt_dev, g.row_len, small, wave,
                      ms.blists.p, list_cap, counts, st);
   HL_LAUNCH_CHECK();
   CandOut out{ms.cnode.p, ms.cto.p, ms.cgain.p, ms.counters.p + 1};
+  unsigned long long* cst = st + 2 * kBuckets;
   {
     HL_PROF(P_PROP_SMALL, 0, s);
-    hipLaunchKernelGGL(k_prop2_small<P>, fixed_grid(list_cap), 256, 0, s, pol, g, ms.blists.p, counts, out, st + 6);
+    hipLaunchKernelGGL(k_prop2_small<P>, fixed_grid(list_cap), 256, 0, s, pol, g, ms.blists.p, counts, out, cst);
     HL_LAUNCH_CHECK();
   }
   {
     HL_PROF(P_PROP_WAVE, 0, s);
     hipLaunchKernelGGL(k_prop2_wave<P>, dim3(fixed_grid(list_cap, 4, kWaveBlocks)), dim3(256), 0, s, pol, g,
-                       ms.blists.p + list_cap, counts + 1, out, st + 7);
+                       ms.blists.p + list_cap, counts + 1, out, cst + 1);
     HL_LAUNCH_CHECK();
   }
   if (maxdeg > wave) {
+    HL_PROF(P_PROP_LDS, 0, s);
+    hipLaunchKernelGGL(k_prop2_lds<P>, dim3(fixed_grid(list_cap, 1, 256)), dim3(1024), 0, s, pol, g,
+                       ms.blists.p + 2 * list_cap, counts + 2, out, cst + 2);
+    HL_LAUNCH_CHECK();
+  }
+  // HBM tables: one per resident workgroup, at most kBigTableBytes in all
+  for (u32 b = 3; b < kBuckets; ++b) {
+    const u32 lo = b == 3 ? kLdsDeg : kBigDeg;
+    if (maxdeg <= lo) break;
     HL_PROF(P_PROP_BIG, 0, s);
-    u32 tcap = 64;
-    while (tcap < 2 * maxdeg) tcap <<= 1;
-    const unsigned nb = 256;
+    const u32 tcap = big_cap(b == 3 ? std::min(maxdeg, kBigDeg) : maxdeg);
+    const u64 per = u64(tcap) * (sizeof(u32) + sizeof(double));
+    const unsigned nb = unsigned(std::max<u64>(1, std::min<u64>(512, kBigTableBytes / per)));
     ms.tk.reserve(u64(nb) * tcap, s);
     ms.tv.reserve(u64(nb) * tcap, s);
-    hipLaunchKernelGGL(k_prop2_big<P>, dim3(nb), dim3(256), 0, s, pol, g, ms.blists.p + 2 * list_cap, counts + 2,
-                       ms.tk.p, ms.tv.p, tcap, out, st + 8);
+    hipLaunchKernelGGL(k_prop2_big<P>, dim3(nb), dim3(1024), 0, s, pol, g, ms.blists.p + b * list_cap, counts + b,
+                       ms.tk.p, ms.tv.p, tcap, out, cst + b);
     HL_LAUNCH_CHECK();
   }
 }
@@ -973,10 +1152,16 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
     propose_round(pol, m.g, cur, cur_cnt, host_cnt, maxdeg, ms, th, ms.dstat.p + DS_MOVE, s);
     const u32 ep = ms.next_epoch(s);
     CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
+    const FlatMarks fm{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13};
     {
       HL_PROF(P_DECOUPLE, 0, s);
       hipLaunchKernelGGL(k_dec_small, 1, 1024, 0, s, m, cand, ms.minUto.p, ms.minUfrom.p, ms.minAto.p,
-                         ms.minAfrom.p, ms.mark.p, ep, nxt, next_cnt, status, ms.dstat.p, cur_cnt, bcounts);
+                         ms.minAfrom.p, ms.mark.p, ep, nxt, next_cnt, status, ms.dstat.p, cur_cnt, bcounts, fm);
+      HL_LAUNCH_CHECK();
+    }
+    {
+      HL_PROF(P_MARK_MOVED, 0, s);
+      hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fm, ms.mark.p, ep, nxt, next_cnt);
       HL_LAUNCH_CHECK();
     }
     u32 h[16];
@@ -1044,8 +1229,15 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
       }
       {
         HL_PROF(P_MARK_MOVED, 0, s);
-        hipLaunchKernelGGL(k_marks4, fixed_grid(u64(c) * 64, 256, 4096), 256, 0, s, m, ms.acc.p, acc_cnt,
-                           ms.node.p, ms.to.p, ms.mark.p, ep, nxt, next_cnt);
+        ms.mdeg.reserve(c, s);
+        ms.mpre.reserve(u64(c) + 2, s);
+        hipLaunchKernelGGL(k_acc_degrees, fixed_grid(c), 256, 0, s, u64(c), ms.node.p, ms.state.p, m.g.row_len,
+                           ms.mdeg.p);
+        HL_LAUNCH_CHECK();
+        exclusive_scan_u64_async(ms.mdeg.p, ms.mpre.p, c, tmp, s);
+        fill_u32(ms.counters.p + 13, c, 1, s);
+        const FlatMarks fmm{ms.node.p, ms.to.p, ms.mpre.p, ms.counters.p + 13};
+        hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fmm, ms.mark.p, ep, nxt, next_cnt);
         HL_LAUNCH_CHECK();
       }
       d2h_async(&accepted, acc_cnt, sizeof(u32), s);

@@ -26,6 +26,7 @@ def hip():
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip.hipDeviceSynchronize.argtypes = []
         _hip.hipSetDevice.argtypes = [C.c_int]
+        _hip.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     return _hip
 
 
@@ -67,6 +68,13 @@ def copy_d2d(dst: DeviceArray, src: DeviceArray, nbytes: int):
     if nbytes > min(dst.nbytes, src.nbytes):
         raise ValueError("copy larger than the buffers")
     _ok(hip().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy")
+
+
+def mem_info():
+    """(free, total) bytes of device memory."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _ok(hip().hipMemGetInfo(C.byref(f), C.byref(t)), "hipMemGetInfo")
+    return f.value, t.value
 
 
 def synchronize():
