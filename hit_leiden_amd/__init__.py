@@ -124,6 +124,9 @@ def lib():
             "hl_profile_slots": ([], C.c_int32),
             "hl_profile_enable": ([vp, C.c_int32], C.c_int),
             "hl_profile_read": ([vp, C.c_int32, C.c_char_p, C.c_uint64, u64p, f64p, u64p], C.c_int),
+            "hl_load_graphs": ([vp, C.c_uint32, vp, C.c_uint64, vp, vp, vp, C.POINTER(BatchStats)], C.c_int),
+            "hl_graph_count": ([vp, u32p], C.c_int),
+            "hl_modularity_graphs": ([vp, vp, C.c_uint64], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -135,6 +138,15 @@ def lib():
 
 def _ptr(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _first_seen(labels):
+    """canonicalize_community_ids_in_place (hit_leiden.rs:358-370) on a numpy row."""
+    _, first, inv = np.unique(labels, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    return rank[inv].astype(np.uint32)
 
 
 @dataclass
@@ -249,6 +261,68 @@ class HitLeiden:
                                                 None if d_alpha is None else C.c_void_p(d_alpha), C.byref(s)))
         self.last_stats = s
         return s
+
+    # ---- several independent graphs in one engine (batched form) ----
+    def load_graphs(self, graphs) -> BatchStats:
+        """Load independent graphs (a list of GraphInput) into one engine; graph i
+        owns the union vertex ids [graph_offsets[i], graph_offsets[i+1])."""
+        sizes = np.array([g.node_count for g in graphs], dtype=np.uint64)
+        off = np.zeros(len(graphs) + 1, dtype=np.uint64)
+        np.cumsum(sizes, out=off[1:])
+        if off[-1] >= 2**32:
+            raise InvalidInput("too many vertices for u32 ids")
+        self.graph_offsets = off.astype(np.uint32)
+        src = np.concatenate([g.src.astype(np.uint64) + off[i] for i, g in enumerate(graphs)] or [np.zeros(0)])
+        dst = np.concatenate([g.dst.astype(np.uint64) + off[i] for i, g in enumerate(graphs)] or [np.zeros(0)])
+        weighted = any(g.weights is not None for g in graphs)
+        w = np.concatenate([g.weights if g.weights is not None else np.ones(len(g), np.float64)
+                            for g in graphs]) if weighted else None
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        s = BatchStats()
+        self._check(lib().hl_load_graphs(self._h, len(graphs), _ptr(self.graph_offsets), len(src), _ptr(src),
+                                         _ptr(dst), _ptr(w), C.byref(s)))
+        self.last_stats = s
+        return s
+
+    def graphs_delta(self, deltas):
+        """Union-id arrays of {graph index: GraphInput batch}."""
+        off = self.graph_offsets.astype(np.uint64)
+        items = sorted(deltas.items())
+        src = np.concatenate([d.src.astype(np.uint64) + off[i] for i, d in items] or [np.zeros(0)])
+        dst = np.concatenate([d.dst.astype(np.uint64) + off[i] for i, d in items] or [np.zeros(0)])
+        a = np.concatenate([d.weights if d.weights is not None else np.ones(len(d), np.float64)
+                            for _, d in items] or [np.zeros(0)])
+        return (np.ascontiguousarray(src, dtype=np.uint32), np.ascontiguousarray(dst, dtype=np.uint32),
+                np.ascontiguousarray(a, dtype=np.float64))
+
+    def apply_graphs(self, deltas) -> BatchStats:
+        """One pass over the batches of several graphs ({graph index: GraphInput})."""
+        src, dst, a = self.graphs_delta(deltas)
+        s = BatchStats()
+        self._check(lib().hl_apply_batch(self._h, len(src), _ptr(src), _ptr(dst), _ptr(a), C.byref(s)))
+        self.last_stats = s
+        return s
+
+    @property
+    def graph_count(self):
+        x = C.c_uint32()
+        self._check(lib().hl_graph_count(self._h, C.byref(x)))
+        return x.value
+
+    def modularity_graphs(self):
+        q = np.zeros(max(self.graph_count, 1), dtype=np.float64)
+        self._check(lib().hl_modularity_graphs(self._h, _ptr(q), len(q)))
+        return q[:self.graph_count]
+
+    def graph_hierarchy(self, i, canonical=True):
+        """Hierarchy rows restricted to graph i (first-seen relabelled per row when canonical)."""
+        h = self.hierarchy(canonical=False)
+        lo, hi = int(self.graph_offsets[i]), int(self.graph_offsets[i + 1])
+        rows = [r[lo:hi].copy() for r in h]
+        if canonical:
+            rows = [_first_seen(r) for r in rows]
+        return rows
 
     # ---- state readout (PartitionState fields) ----
     @property

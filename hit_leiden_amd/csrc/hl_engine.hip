@@ -35,8 +35,12 @@ struct Level {
   u64 lab_bound = 0;  // every f label < lab_bound
   u64 s_bound = 0;    // every s_cur / s_pre label < s_bound
   u32 deg_bound = 0;  // every row length <= deg_bound
+  // several graphs per engine: graph of each vertex id (the graph count for
+  // ids no vertex uses); entries below gid_init are initialised
+  DevBuf<u32> gid;
+  u64 gid_init = 0;
 
-  Csr csr() const { return Csr{n, row_off.p, row_len.p, col.p, w.p, k.p, twice_m.p}; }
+  Csr csr() const { return Csr{n, row_off.p, row_len.p, col.p, w.p, k.p, twice_m.p, gid.p}; }
   LevelCsrMut mut() { return LevelCsrMut{n, row_off.p, row_len.p, row_cap.p, col.p, w.p, k.p, twice_m.p}; }
 };
 
@@ -55,6 +59,11 @@ struct Engine {
   std::vector<std::unique_ptr<Level>> L;
   u32 n0 = 0;
   bool loaded = false;
+  // several independent graphs in one engine (0: one graph)
+  u32 G = 0;
+  std::vector<u32> goff;  // G + 1 vertex offsets
+  DevBuf<u32> goff_d, gcnt, mcg;
+  DevBuf<u8> part, gnid;
   std::string err;
   Thresholds th;
 
@@ -140,6 +149,15 @@ struct Engine {
     lv.n = n_new;
     lv.lab_bound = std::max<u64>(lv.lab_bound, bf + (n_new - n_old));
     lv.s_bound = std::max<u64>(lv.s_bound, bs + (n_new - n_old));
+    if (multi()) ensure_gid(lv, n_new);
+  }
+
+  bool multi() const { return G > 0; }
+  void ensure_gid(Level& lv, u64 need) {
+    if (need <= lv.gid_init) return;
+    lv.gid.reserve(need, s, true, 1.25);
+    fill_u32(lv.gid.p + lv.gid_init, G, lv.gid.cap - lv.gid_init, s);
+    lv.gid_init = lv.gid.cap;
   }
 
   bool par() const { return mode == HL_MODE_THROUGHPUT; }
@@ -206,12 +224,18 @@ struct Engine {
         tr.tot = lv.tot.p;
         tr.f = lv.f.p;
       }
+      if (multi()) {
+        tr.tm = lv.twice_m.p;  // each graph's 2m follows its rows' degree changes
+        tr.gid = lv.gid.p;
+      }
       apply_delta_commit(g, lv.pool_used, as, plan, mode == HL_MODE_DETERMINISTIC, tr, s);
       lv.pool_used += plan.need;
       lv.deg_bound = std::max<u32>(lv.deg_bound, u32(plan.max_row));
     }
-    partials.reserve(twice_m_scratch(lv.n), s);
-    twice_m_tree(lv.k.p, lv.n, partials.p, lv.twice_m.p, s);
+    if (!multi()) {
+      partials.reserve(twice_m_scratch(lv.n), s);
+      twice_m_tree(lv.k.p, lv.n, partials.p, lv.twice_m.p, s);
+    }
     if (st && p < HL_MAX_LEVELS)
       st->bytes_alg += 16 * k + 24 * plan.old_entries + 12 * plan.need + 20 * plan.nseg;
   }
@@ -229,7 +253,12 @@ struct Engine {
   u32 movement(Level& lv, const Rec* d, u64 k, hl_batch_stats* st, int p) {
     const u32 n = lv.n;
     fill4_u8(lv.A.p, lv.B.p, lv.K.p, lv.R.p, 0, n, s);
-    if (k == 0) fill_u8(lv.A.p, 1, n, s);  // empty delta activates every vertex (hit_leiden.rs:1054-1057)
+    if (k == 0 && !multi()) fill_u8(lv.A.p, 1, n, s);  // empty delta activates every vertex (hit_leiden.rs:1054-1057)
+    if (multi()) {
+      // records per graph at this level; at level 0 they name the graphs taking part
+      graph_counts(d, k, lv.gid.p, G, gcnt.p, s);
+      if (p == 0) part_from_counts(gcnt.p, G, part.p, s);
+    }
     Csr g = lv.csr();
     if (mode == HL_MODE_DETERMINISTIC) {
       seed(d, k, lv.f.p, lv.s_cur.p, lv.A.p, lv.K.p, s);
@@ -264,6 +293,11 @@ struct Engine {
     ensure_totals(lv);
     const int max_rounds = cfg.max_rounds > 0 ? cfg.max_rounds : 20;
     MoveCtx m{g, lv.f.p, lv.s_cur.p, lv.tot.p, lv.k.p, lv.A.p, lv.B.p, lv.K.p, cfg.gamma, lv.tot.cap};
+    if (multi()) {
+      m.gid = lv.gid.p;
+      m.part = part.p;
+      m.gcnt = gcnt.p;
+    }
     lv.lab_bound = std::max<u64>(lv.lab_bound, 2ull * n);  // empty-community ids n + v
     MoveStats ms_st;
     const u32 maxd = max_degree(lv);
@@ -283,7 +317,10 @@ struct Engine {
     bool identity = false;
     const u64 sn = cc_split(g, lv.s_cur.p, lv.K.p, lv.R.p, std::max<u64>(lv.s_bound, 1), &identity, rs, tmp, s);
     lv.s_bound = std::max(lv.s_bound, sn);
-    if (identity) fill_u8(lv.R.p, 1, n, s);
+    if (multi())
+      identity_graphs_to_R(lv.s_cur.p, lv.gid.p, n, G, part.p, gnid.p, lv.R.p, s);  // per graph
+    else if (identity)
+      fill_u8(lv.R.p, 1, n, s);
     const u64 cs = comm_space(lv);
     if (mode == HL_MODE_DETERMINISTIC)
       refine_det(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, sn, rs, ks, vs, vidx, heads, starts, cnt,
@@ -291,8 +328,13 @@ struct Engine {
     else
       refine_parallel(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, sn, rs, ps, mvs, max_degree(lv), th, tmp,
                       lv.tot_valid ? lv.tot.p : nullptr, s);
+    if (multi() && p + 1 < P) {
+      // graphs of the next level's vertices: the graph of any member
+      Level& up = *L[p + 1];
+      ensure_gid(up, sn);
+      scatter_gid(lv.R.p, lv.s_cur.p, lv.gid.p, n, up.gid.p, s);
+    }
     (void)st;
-    (void)p;
   }
 
   void def_update_maps(bool use_g) {
@@ -432,16 +474,30 @@ struct Engine {
     l0.w.reserve(want, s, false, 1.25);
     l0.pool_used = 0;
     for (int p = 0; p < P; ++p) {
-      L[p]->twice_m.reserve(1, s);
-      fill_f64(L[p]->twice_m.p, 0.0, 1, s);
+      L[p]->twice_m.reserve(u64(G) + 1, s);
+      fill_f64(L[p]->twice_m.p, 0.0, u64(G) + 1, s);
+    }
+    if (multi()) {
+      gcnt.reserve(u64(G) + 1, s);
+      part.reserve(u64(G) + 1, s);
+      gnid.reserve(u64(G) + 1, s);
+      fill_u8(part.p, 0, u64(G) + 1, s);
+      goff_d.reserve(u64(G) + 1, s);
+      HL_CHECK(hipMemcpyAsync(goff_d.p, goff.data(), (u64(G) + 1) * sizeof(u32), hipMemcpyHostToDevice, s));
+      gid_from_offsets(goff_d.p, G, n, l0.gid.p, s);
+      stream_sync(s);  // goff's host memory is read before it can change
     }
     loaded = true;
   }
 
+  u32 graph_of(u32 v) const {
+    return u32(std::upper_bound(goff.begin(), goff.end(), v) - goff.begin()) - 1;
+  }
   void upload_batch(u64 k, const u32* src, const u32* dst, const double* w, u32 n_lim) {
     std::vector<Rec> h(k);
     for (u64 i = 0; i < k; ++i) {
       if (src[i] >= n_lim || dst[i] >= n_lim) throw InputError("endpoint id out of range");
+      if (multi() && graph_of(src[i]) != graph_of(dst[i])) throw InputError("record joins two graphs");
       double x = w ? w[i] : 1.0;
       if (!std::isfinite(x)) throw InputError("non-finite weight");
       h[i] = Rec{src[i], dst[i], x};
@@ -609,6 +665,8 @@ hl_status hl_load(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, cons
     HL_CHECK(hipSetDevice(e.cfg.device));
     double t0 = now_s();
     if (stats) std::memset(stats, 0, sizeof(*stats));
+    e.G = 0;
+    e.goff.clear();
     e.upload_batch(m, src, dst, w, n);
     e.reset(n, m);
     e.run_batch(m, stats);
@@ -617,13 +675,42 @@ hl_status hl_load(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, cons
   });
 }
 
+hl_status hl_load_graphs(hl_graph* g, uint32_t ngraphs, const uint32_t* vertex_offsets, uint64_t m,
+                         const uint32_t* src, const uint32_t* dst, const double* w, hl_batch_stats* stats) {
+  if (!g || !vertex_offsets || ngraphs == 0 || (m && (!src || !dst))) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = g->e;
+    if (e.mode != HL_MODE_THROUGHPUT) throw hl::InputError("several graphs per engine need the throughput mode");
+    if (vertex_offsets[0] != 0) throw hl::InputError("vertex_offsets[0] must be 0");
+    for (uint32_t i = 0; i < ngraphs; ++i)
+      if (vertex_offsets[i + 1] < vertex_offsets[i]) throw hl::InputError("vertex_offsets must not decrease");
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    double t0 = now_s();
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    e.G = ngraphs;
+    e.goff.assign(vertex_offsets, vertex_offsets + ngraphs + 1);
+    const uint32_t n = vertex_offsets[ngraphs];
+    e.upload_batch(m, src, dst, w, n);
+    e.reset(n, m);
+    e.run_batch(m, stats);
+    e.trim();
+    if (stats) stats->seconds = now_s() - t0;
+  });
+}
+
+hl_status hl_graph_count(const hl_graph* g, uint32_t* ngraphs) {
+  if (!g || !ngraphs) return HL_INVALID_INPUT;
+  *ngraphs = g->e.G;
+  return HL_OK;
+}
+
 namespace {
 __global__ void k_pack_batch(const uint32_t* a, const uint32_t* b, const double* w, uint64_t k, uint32_t n,
-                             hl::Rec* out, uint32_t* bad) {
+                             const uint32_t* gid, hl::Rec* out, uint32_t* bad) {
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < k; i += uint64_t(gridDim.x) * blockDim.x) {
     uint32_t u = a[i], v = b[i];
     double x = w ? w[i] : 1.0;
-    if (u >= n || v >= n || !isfinite(x)) *bad = 1;
+    if (u >= n || v >= n || !isfinite(x) || (gid && gid[u] != gid[v])) *bad = 1;
     out[i] = hl::Rec{u, v, x};
   }
 }
@@ -657,13 +744,13 @@ hl_status hl_apply_batch_device(hl_graph* g, uint64_t k, const uint32_t* d_src, 
     e.mx.reserve(1, e.s);
     hl::fill_u32(e.mx.p, 0, 1, e.s);
     if (k)
-      hipLaunchKernelGGL(k_pack_batch, hl::blocks_for(k), 256, 0, e.s, d_src, d_dst, d_alpha, k, e.n0, e.dcur.p,
-                         e.mx.p);
+      hipLaunchKernelGGL(k_pack_batch, hl::blocks_for(k), 256, 0, e.s, d_src, d_dst, d_alpha, k, e.n0,
+                         e.multi() ? e.L[0]->gid.p : nullptr, e.dcur.p, e.mx.p);
     HL_LAUNCH_CHECK();
     uint32_t bad = 0;
     hl::d2h_async(&bad, e.mx.p, sizeof(uint32_t), e.s);
     hl::stream_sync(e.s);
-    if (bad) throw hl::InputError("endpoint id out of range or non-finite weight");
+    if (bad) throw hl::InputError("endpoint id out of range, non-finite weight or a record joining two graphs");
     e.run_batch(k, stats);
     if (stats) stats->seconds = now_s() - t0;
   });
@@ -848,7 +935,22 @@ hl_status hl_modularity(const hl_graph* g, int32_t which, double* q) {
       HL_CHECK(hipMemcpyAsync(e.lab.p, e.L[0]->f.p, e.n0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, e.s));
     else
       throw hl::InputError("unknown partition");
+    if (e.multi()) throw hl::InputError("several graphs per engine: use hl_modularity_graphs");
     *q = hl::modularity(e.L[0]->csr(), e.lab.p, e.sig, e.acc, e.mx, e.s);
+  });
+}
+
+hl_status hl_modularity_graphs(const hl_graph* g, double* q, uint64_t cap) {
+  if (!g || !q) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = const_cast<Engine&>(g->e);
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    if (!e.loaded) throw hl::InputError("no graph loaded");
+    if (!e.multi()) throw hl::InputError("the engine holds one graph: use hl_modularity");
+    if (cap < e.G) throw hl::InputError("output buffer too small");
+    e.lab.reserve(e.n0 + 1, e.s);
+    e.compose(e.P - 1, true, e.lab.p);
+    hl::modularity_graphs(e.L[0]->csr(), e.lab.p, e.G, q, e.sig, e.mcg, e.acc, e.mx, e.s);
   });
 }
 
@@ -885,6 +987,7 @@ hl_status hl_modularity_labels(const hl_graph* g, const uint32_t* labels, uint64
     Engine& e = const_cast<Engine&>(g->e);
     HL_CHECK(hipSetDevice(e.cfg.device));
     if (!e.loaded) throw hl::InputError("no graph loaded");
+    if (e.multi()) throw hl::InputError("several graphs per engine: use hl_modularity_graphs");
     if (n != e.n0) throw hl::InputError("labels length differs from n");
     e.lab.reserve(n + 1, e.s);
     if (n) HL_CHECK(hipMemcpyAsync(e.lab.p, labels, n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));

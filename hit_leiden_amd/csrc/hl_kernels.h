@@ -13,8 +13,11 @@ struct Csr {
   const u32* col;
   const double* w;
   const double* k;        // weighted degrees
-  const double* twice_m;  // device scalar
+  const double* twice_m;  // device scalar, or per graph when gid is set
+  const u32* gid = nullptr;  // graph of each vertex (several independent graphs in one engine)
 };
+// 2m of the graph vertex v belongs to
+__device__ __forceinline__ double tm_of(const Csr& g, u32 v) { return g.gid ? g.twice_m[g.gid[v]] : g.twice_m[0]; }
 
 // scratch allocator used by the launchers for library temporaries (rocprim)
 struct Temp {
@@ -82,6 +85,8 @@ struct TotalsRef {
   double* dS = nullptr;
   u32* sizes = nullptr;
   const u32* s = nullptr;
+  double* tm = nullptr;       // per-graph 2m, kept current from the row sums
+  const u32* gid = nullptr;
 };
 
 // Plans and applies G <- G (+) dG (in_memory.rs:103-138).  Returns the number of
@@ -189,5 +194,20 @@ double modularity(const Csr& g, const u32* lab, DevBuf<double>& sig, DevBuf<doub
 void gather_u32(u32* cur, const u32* map, u32 n, hipStream_t s);
 void canonicalize(u32* lab, u32 n, DevBuf<u32>& first, DevBuf<u8>& flag, DevBuf<u64>& pos, Temp& tmp,
                   hipStream_t s);
+
+// ---- several independent graphs in one engine (the batched form) ----
+// gid[v] = graph of level-0 vertex v from the G+1 vertex offsets
+void gid_from_offsets(const u32* offs, u32 G, u32 n, u32* gid, hipStream_t s);
+// cnt[g] = records of graph g (cnt has G + 1 entries)
+void graph_counts(const Rec* d, u64 k, const u32* gid, u32 G, u32* cnt, hipStream_t s);
+void part_from_counts(const u32* cnt, u32 G, u8* part, hipStream_t s);
+// R = all vertices of every taking-part graph whose s is the identity (hit_leiden.rs:1333-1339, per graph)
+void identity_graphs_to_R(const u32* s, const u32* gid, u32 n, u32 G, const u8* part, u8* nid, u8* R,
+                          hipStream_t st);
+// gid_q[s(v)] = gid_p[v] for v in R
+void scatter_gid(const u8* R, const u32* s, const u32* gid_p, u32 n, u32* gid_q, hipStream_t st);
+// compute_modularity (hit_leiden.rs:121-161) of each graph of a level-0 CSR with gid set
+void modularity_graphs(const Csr& g, const u32* lab, u32 G, double* out, DevBuf<double>& sig, DevBuf<u32>& cg,
+                       DevBuf<double>& acc, DevBuf<u32>& mx, hipStream_t s);
 
 }  // namespace hl

@@ -435,6 +435,7 @@ __global__ __launch_bounds__(256) void k_row_degree_wave(u64 nseg, const u32* se
       if (acc != old) {
         if (tr.tot) atomicAdd(&tr.tot[tr.f[r]], acc - old);
         if (tr.dS) atomicAdd(&tr.dS[tr.s[r]], acc - old);
+        if (tr.tm) atomicAdd(&tr.tm[tr.gid[r]], acc - old);
       }
     }
   }
@@ -947,14 +948,16 @@ __global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u3
   }
 }
 __global__ void k_blocked(u64 sn, const u8* is_cand, const double* dS, const double* wS, const u32* pidx,
-                          const u32* f, const double* dpar, const double* twice_m, double gamma, u8* blocked) {
-  const double t = *twice_m;
+                          const u32* f, const double* dpar, const double* twice_m, const u32* gid, double gamma,
+                          u8* blocked) {
   for (u64 c = gtid(); c < sn; c += gstride()) {
     u8 b = 0;
     if (is_cand[c]) {
       double ds = dS[c];
       if (ds > 0.0) {
-        u32 P = f[pidx[c]];
+        const u32 x = pidx[c];  // a member of subcommunity c
+        const double t = gid ? twice_m[gid[x]] : twice_m[0];
+        u32 P = f[x];
         b = remove_gain(wS[c], t, gamma, ds, dpar[P]) > kEpsTFilter;
       }
     }
@@ -1268,6 +1271,98 @@ __global__ void k_first_flags(const u32* lab, u32 n, const u32* first, u64* flag
 }
 __global__ void k_relabel(u32* lab, u32 n, const u32* first, const u64* pos) {
   for (u64 v = gtid(); v < n; v += gstride()) lab[v] = u32(pos[first[lab[v]]]);
+}
+
+// ------------------------------------------------------------------ several graphs per engine
+// Vertex ranges of the graphs are contiguous at level 0; gid maps every vertex
+// of a level to its graph (G = no graph: an unused id of the level).
+__global__ void k_gid_from_offsets(const u32* offs, u32 G, u32 n, u32* gid) {
+  for (u64 v = gtid(); v < n; v += gstride()) {
+    u32 lo = 0, hi = G;  // offs[lo] <= v < offs[hi]
+    while (hi - lo > 1) {
+      const u32 mid = (lo + hi) >> 1;
+      if (offs[mid] <= v) lo = mid;
+      else hi = mid;
+    }
+    gid[v] = lo;
+  }
+}
+__global__ void k_graph_counts(const Rec* d, u64 k, const u32* gid, u32* cnt) {
+  for (u64 i = gtid(); i < k; i += gstride()) atomicAdd(&cnt[gid[d[i].u]], 1u);
+}
+__global__ void k_part_from_counts(const u32* cnt, u32 G, u8* part) {
+  for (u64 g = gtid(); g <= G; g += gstride()) part[g] = g < G && cnt[g] > 0;
+}
+__global__ void k_graph_nonident(const u32* s, const u32* gid, u32 n, u8* nid) {
+  for (u64 v = gtid(); v < n; v += gstride())
+    if (s[v] != v) {
+      const u32 g = gid[v];
+      if (!nid[g]) nid[g] = 1;
+    }
+}
+__global__ void k_fill_R_graphs(const u32* gid, u32 n, const u8* part, const u8* nid, u8* R) {
+  for (u64 v = gtid(); v < n; v += gstride()) {
+    const u32 g = gid[v];
+    if (part[g] && !nid[g]) R[v] = 1;
+  }
+}
+// the graph of each next-level vertex: the graph of a member
+__global__ void k_scatter_gid(const u8* R, const u32* s, const u32* gid_p, u32 n, u32* gid_q) {
+  for (u64 v = gtid(); v < n; v += gstride())
+    if (R[v]) gid_q[s[v]] = gid_p[v];
+}
+// Per-graph modularity terms: sig[c] and the graph of community c, then the
+// intra-community weight and the sum of sig^2 per graph.  Each thread walks a
+// contiguous chunk (vertices of a graph are mostly adjacent) and adds to the
+// device totals when the graph changes.
+__global__ void k_sig_graph(const u32* lab, const double* k, const u32* gid, u32 n, double* sig, u32* cg) {
+  for (u64 v = gtid(); v < n; v += gstride()) {
+    const u32 c = lab[v];
+    atomicAdd(&sig[c], k[v]);
+    cg[c] = gid[v];
+  }
+}
+__global__ void k_intra_graphs(Csr g, const u32* lab, u64 chunk, double* intra) {
+  const u64 t = gtid();
+  const u64 v0 = t * chunk, v1 = v0 + chunk < g.n ? v0 + chunk : g.n;
+  u32 cur = kNone;
+  double acc = 0.0;
+  for (u64 v = v0; v < v1; ++v) {
+    const u32 gv = g.gid[v];
+    if (gv != cur) {
+      if (acc != 0.0) atomicAdd(&intra[cur], acc);
+      cur = gv;
+      acc = 0.0;
+    }
+    const u64 off = g.row_off[v];
+    const u32 deg = g.row_len[v], lv = lab[v];
+    for (u32 j = 0; j < deg; ++j) {
+      const u32 u = g.col[off + j];
+      if (lab[u] == lv) {
+        const double w = g.w[off + j];
+        acc += (u == v) ? 2.0 * w : w;
+      }
+    }
+  }
+  if (acc != 0.0) atomicAdd(&intra[cur], acc);
+}
+__global__ void k_sq_graphs(const double* sig, const u32* cg, u64 L, u64 chunk, double* sq) {
+  const u64 t = gtid();
+  const u64 c0 = t * chunk, c1 = c0 + chunk < L ? c0 + chunk : L;
+  u32 cur = kNone;
+  double acc = 0.0;
+  for (u64 c = c0; c < c1; ++c) {
+    const double x = sig[c];
+    if (x == 0.0) continue;
+    const u32 gc = cg[c];
+    if (gc != cur) {
+      if (acc != 0.0) atomicAdd(&sq[cur], acc);
+      cur = gc;
+      acc = 0.0;
+    }
+    acc += x * x;
+  }
+  if (acc != 0.0) atomicAdd(&sq[cur], acc);
 }
 
 }  // namespace
@@ -1741,7 +1836,7 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
     dpar = rs.dpar.p;
   }
   hipLaunchKernelGGL(k_blocked, GRID(sn), sn, rs.is_cand.p, rs.dS.p, rs.wS.p, rs.parentIdx.p, f, dpar,
-                     g.twice_m, gamma, rs.blocked.p);
+                     g.twice_m, g.gid, gamma, rs.blocked.p);
   HL_LAUNCH_CHECK();
   // proposals against the snapshot, then the connected-targets rule
   (void)ps;
@@ -1935,6 +2030,69 @@ void canonicalize(u32* lab, u32 n, DevBuf<u32>& first, DevBuf<u8>& flag, DevBuf<
   hipLaunchKernelGGL(k_relabel, GRID(n), lab, n, first.p, pos.p);
   HL_LAUNCH_CHECK();
   (void)flag;
+}
+
+
+// ---- several graphs per engine ----
+void gid_from_offsets(const u32* offs, u32 G, u32 n, u32* gid, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_gid_from_offsets, GRID(n), offs, G, n, gid);
+  HL_LAUNCH_CHECK();
+}
+void graph_counts(const Rec* d, u64 k, const u32* gid, u32 G, u32* cnt, hipStream_t s) {
+  fill_u32(cnt, 0, u64(G) + 1, s);
+  if (k) hipLaunchKernelGGL(k_graph_counts, GRID(k), d, k, gid, cnt);
+  HL_LAUNCH_CHECK();
+}
+void part_from_counts(const u32* cnt, u32 G, u8* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_part_from_counts, GRID(u64(G) + 1), cnt, G, part);
+  HL_LAUNCH_CHECK();
+}
+void identity_graphs_to_R(const u32* sm, const u32* gid, u32 n, u32 G, const u8* part, u8* nid, u8* R,
+                          hipStream_t s) {
+  fill_u8(nid, 0, u64(G) + 1, s);
+  if (n) {
+    hipLaunchKernelGGL(k_graph_nonident, GRID(n), sm, gid, n, nid);
+    hipLaunchKernelGGL(k_fill_R_graphs, GRID(n), gid, n, part, nid, R);
+  }
+  HL_LAUNCH_CHECK();
+}
+void scatter_gid(const u8* R, const u32* sm, const u32* gid_p, u32 n, u32* gid_q, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_scatter_gid, GRID(n), R, sm, gid_p, n, gid_q);
+  HL_LAUNCH_CHECK();
+}
+void modularity_graphs(const Csr& g, const u32* lab, u32 G, double* out, DevBuf<double>& sig, DevBuf<u32>& cg,
+                       DevBuf<double>& acc, DevBuf<u32>& mx, hipStream_t s) {
+  const u32 n = g.n;
+  std::vector<double> h(2 * (u64(G) + 1), 0.0), tm(u64(G) + 1, 0.0);
+  mx.reserve(1, s);
+  fill_u32(mx.p, 0, 1, s);
+  if (n) max_u32(lab, n, mx.p, s);
+  u32 hm = 0;
+  d2h_async(&hm, mx.p, sizeof(u32), s);
+  stream_sync(s);
+  const u64 L = u64(hm) + 1;
+  sig.reserve(L, s);
+  cg.reserve(L, s);
+  acc.reserve(2 * (u64(G) + 1), s);
+  fill_f64(sig.p, 0.0, L, s);
+  fill_u32(cg.p, G, L, s);
+  fill_f64(acc.p, 0.0, 2 * (u64(G) + 1), s);
+  if (n) {
+    HL_PROF(P_MODULARITY, 0, s);
+    hipLaunchKernelGGL(k_sig_graph, GRID(n), lab, g.k, g.gid, n, sig.p, cg.p);
+    const u64 chunk = 64;
+    hipLaunchKernelGGL(k_intra_graphs, GRID((n + chunk - 1) / chunk), g, lab, chunk, acc.p);
+    hipLaunchKernelGGL(k_sq_graphs, GRID((L + chunk - 1) / chunk), sig.p, cg.p, L, chunk, acc.p + G + 1);
+  }
+  HL_LAUNCH_CHECK();
+  HL_CHECK(hipMemcpyAsync(h.data(), acc.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HL_CHECK(hipMemcpyAsync(tm.data(), g.twice_m, tm.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  stream_sync(s);
+  // compute_modularity (hit_leiden.rs:121-161) per graph
+  for (u32 i = 0; i < G; ++i) {
+    const double t = tm[i];
+    out[i] = t == 0.0 ? 0.0 : (h[i] - h[G + 1 + i] / t) / t;
+  }
 }
 
 }  // namespace hl

@@ -35,6 +35,12 @@ struct MoveCtx {
   u8* K;
   double gamma;
   u64 comm_cap;
+  // several graphs per engine: graph of each vertex, graphs taking part in the
+  // batch and their record counts at this level (an empty count activates the
+  // whole graph, hit_leiden.rs:1054-1057 per graph)
+  const u32* gid = nullptr;
+  const u8* part = nullptr;
+  const u32* gcnt = nullptr;
 };
 
 // throughput movement (parallel_frontier.rs:155-236): self-loops skipped,

@@ -160,6 +160,23 @@ __global__ void k_seed2(const Rec* d, u64 k, const u32* f, const u32* s, u8* A, 
     if (cv) list[pv] = r.v;
   }
 }
+// vertices of taking-part graphs without records at this level
+__global__ void k_activate_graphs(const u32* gid, u32 n, const u8* part, const u32* gcnt, u8* A, u32* mark,
+                                  u32 epoch, u32* list, u32* cnt) {
+  for (u64 base = u64(blockIdx.x) * blockDim.x; base < n; base += gstride()) {
+    const u64 v = base + threadIdx.x;
+    bool want = false;
+    if (v < n) {
+      const u32 g = gid[v];
+      if (part[g] && gcnt[g] == 0) {
+        A[v] = 1;
+        want = claim(mark, u32(v), epoch);
+      }
+    }
+    const u32 p = wave_append(want, cnt);
+    if (want) list[p] = u32(v);
+  }
+}
 __global__ void k_iota_list(u32* list, u32 n, u32* cnt) {
   for (u64 i = gtid(); i < n; i += gstride()) list[i] = u32(i);
   if (gtid() == 0) *cnt = n;
@@ -237,7 +254,6 @@ __global__ __launch_bounds__(256) void k_prop2_small(P pol, Csr g, const u32* li
   __shared__ double sv[SMALL_MAX2 * 256];
   const u32 cnt = *cnt_dev;
   const u32 t = threadIdx.x;
-  const double tt = *g.twice_m;
   unsigned long long ncand = 0;
   for (u64 base = u64(blockIdx.x) * 256; base < cnt; base += gstride()) {
     const u64 id = base + t;
@@ -269,7 +285,7 @@ __global__ __launch_bounds__(256) void k_prop2_small(P pol, Csr g, const u32* li
       double wcur = 0.0;
       for (int q = 0; q < nk; ++q)
         if (sk[q * 256 + t] == cur) wcur = sv[q * 256 + t];
-      const double kv = g.k[v], sc = pol.cur_total(cur);
+      const double kv = g.k[v], sc = pol.cur_total(cur), tt = tm_of(g, v);
       for (int q = 0; q < nk; ++q) {
         u32 key = sk[q * 256 + t];
         if (key == cur || !pol.ok(key)) continue;
@@ -324,7 +340,6 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
   const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   u32* K = hk + wv * WAVE_HW2;
   double* V = hv + wv * WAVE_HW2;
-  const double tt = *g.twice_m;
   unsigned long long ncand = 0;
   for (u64 base = u64(blockIdx.x) * 4; base < cnt; base += u64(gridDim.x) * 4) {
     const u64 id = base + wv;
@@ -358,7 +373,7 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
     __syncthreads();
     if (has) {
       const double wcur = tab_lookup(K, V, cap, cur);
-      const double kv = g.k[v], sc = pol.cur_total(cur);
+      const double kv = g.k[v], sc = pol.cur_total(cur), tt = tm_of(g, v);
       double bg = 0.0;
       u32 best = kNone;
       for (u32 q = lane; q < cap; q += 64) {
@@ -395,7 +410,6 @@ __global__ __launch_bounds__(1024) void k_prop2_lds(P pol, Csr g, const u32* lis
   __shared__ double V[kLdsCap];
   const u32 cnt = *cnt_dev;
   unsigned long long ncand = 0;
-  const double tt = *g.twice_m;
   for (u32 id = blockIdx.x; id < cnt; id += gridDim.x) {
     const u32 v = list[id];
     const u64 off = g.row_off[v];
@@ -419,7 +433,7 @@ __global__ __launch_bounds__(1024) void k_prop2_lds(P pol, Csr g, const u32* lis
     }
     __syncthreads();
     const double wcur = tab_lookup(K, V, cap, cur);
-    const double kv = g.k[v], sc = pol.cur_total(cur);
+    const double kv = g.k[v], sc = pol.cur_total(cur), tt = tm_of(g, v);
     double bg = 0.0;
     u32 best = kNone;
     for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
@@ -455,7 +469,6 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
   unsigned long long ncand = 0;
   u32* K = TK + u64(blockIdx.x) * tcap;
   double* V = TV + u64(blockIdx.x) * tcap;
-  const double tt = *g.twice_m;
   for (u32 id = blockIdx.x; id < cnt; id += gridDim.x) {
     const u32 v = list[id];
     const u64 off = g.row_off[v];
@@ -494,7 +507,7 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
     __threadfence();
     __syncthreads();
     const double wcur = tab_lookup(K, V, cap, cur);
-    const double kv = g.k[v], sc = pol.cur_total(cur);
+    const double kv = g.k[v], sc = pol.cur_total(cur), tt = tm_of(g, v);
     double bg = 0.0;
     u32 best = kNone;
     for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
@@ -1130,14 +1143,21 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
   HL_CHECK(hipMemsetAsync(ms.counters.p, 0, 16 * sizeof(u32), s));
   u32* cur = ms.list_a.p;
   u32* nxt = ms.list_b.p;
-  if (k == 0) {
+  if (k == 0 && !m.gid) {
     hipLaunchKernelGGL(k_iota_list, fixed_grid(n, 256, 4096), 256, 0, s, cur, n, cnt_a);
     HL_LAUNCH_CHECK();
   } else {
     const u32 ep = ms.next_epoch(s);
-    HL_PROF(P_SEED, 24 * k, s);
-    hipLaunchKernelGGL(k_seed2, fixed_grid(k), 256, 0, s, d, k, m.f, m.s, m.A, m.K, ms.mark.p, ep, cur, cnt_a);
-    HL_LAUNCH_CHECK();
+    if (k) {
+      HL_PROF(P_SEED, 24 * k, s);
+      hipLaunchKernelGGL(k_seed2, fixed_grid(k), 256, 0, s, d, k, m.f, m.s, m.A, m.K, ms.mark.p, ep, cur, cnt_a);
+      HL_LAUNCH_CHECK();
+    }
+    if (m.gid) {
+      hipLaunchKernelGGL(k_activate_graphs, fixed_grid(n, 256, 4096), 256, 0, s, m.gid, n, m.part, m.gcnt, m.A,
+                         ms.mark.p, ep, cur, cnt_a);
+      HL_LAUNCH_CHECK();
+    }
   }
   MovePolicyT pol{m.f, m.tot, m.gamma, n};
   u32 host_cnt = 0;

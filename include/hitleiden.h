@@ -152,6 +152,24 @@ hl_status hl_read_hierarchy(const hl_graph* g, uint32_t* out, uint64_t cap, int3
 hl_status hl_modularity(const hl_graph* g, int32_t which, double* q);
 hl_status hl_modularity_labels(const hl_graph* g, const uint32_t* labels, uint64_t n, double* q);
 
+/* ---- Batched form: several independent graphs in one engine ----
+   No counterpart in the reference, whose PartitionState holds one graph
+   (state.rs:19-64); a caller with many graphs would hold one state per graph
+   and call hit_leiden once per graph and batch.  Here graph i owns the vertex
+   ids [vertex_offsets[i], vertex_offsets[i+1]) of one engine (throughput mode
+   only), every edge and delta record joins two vertices of one graph, and one
+   hl_apply_batch / hl_apply_batch_device call applies the records of any
+   subset of the graphs in one pass: each graph gets exactly the update
+   hit_leiden would give it alone (same partition up to label values; 2m,
+   empty-delta activation and the identity rule are per graph), graphs without
+   records are left untouched.  hl_read_* return the union; hl_modularity does
+   not apply (use hl_modularity_graphs). */
+hl_status hl_load_graphs(hl_graph* g, uint32_t ngraphs, const uint32_t* vertex_offsets, uint64_t m,
+                         const uint32_t* src, const uint32_t* dst, const double* w, hl_batch_stats* stats);
+hl_status hl_graph_count(const hl_graph* g, uint32_t* ngraphs);
+/* compute_modularity (hit_leiden.rs:121-161) of every graph's communities, q[0..ngraphs) */
+hl_status hl_modularity_graphs(const hl_graph* g, double* q, uint64_t cap);
+
 /* Per-kernel device timing for roofline accounting.  No counterpart in the
    reference (it profiles out of process with perf/samply, profiling.rs:74-152).
    hl_profile_enable(g, 1) resets and starts recording HIP events around each
