@@ -91,12 +91,101 @@ def cpu_baseline(w, levels, gamma, budget_s, threads):
             "load_seconds": load_s, "modularity": q}
 
 
+def bench_batched(args):
+    """The batched form: many independent graphs in one engine, one pass per set
+    of batches, against the plain path (one engine per graph, one call per batch)
+    on a sample of the same graphs."""
+    import hit_leiden_amd as hl
+    from hit_leiden_amd import gen, hip
+    nb = args.warmup + args.steps
+    t0 = time.time()
+    ws = gen.batched_config(count=args.graphs, steps=nb)
+    graphs = [hl.GraphInput(w.n, w.u, w.v) for w in ws]
+    sum_n, sum_m = sum(w.n for w in ws), sum(w.m for w in ws)
+    log(f"[bench] batched: {len(ws)} graphs, {sum_n} vertices, {sum_m} edges, gen {time.time() - t0:.1f}s")
+    cfg = hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=args.levels, resolution=args.gamma)
+    u = hl.HitLeiden(cfg)
+    t0 = time.time()
+    u.load_graphs(graphs)
+    load_s = time.time() - t0
+    dev = []
+    for t in range(nb):
+        src, dst, a = u.graphs_delta({i: hl.GraphInput(w.n, *w.batches[t]) for i, w in enumerate(ws)})
+        dev.append((len(src), hip.DeviceArray(src), hip.DeviceArray(dst), hip.DeviceArray(a)))
+    hip.synchronize()
+    for t in range(args.warmup):
+        k, du, dv, da = dev[t]
+        u.apply_batch_device(k, du.address, dv.address, da.address)
+    lat, edges = [], 0
+    hip.synchronize()
+    for t in range(args.warmup, nb):
+        k, du, dv, da = dev[t]
+        t1 = time.perf_counter()
+        u.apply_batch_device(k, du.address, dv.address, da.address)
+        lat.append(time.perf_counter() - t1)
+        edges += k
+    hip.synchronize()
+    batched = edges / sum(lat)
+    q = u.modularity_graphs()
+    u.close()
+    for _, a, b, c in dev:
+        a.free(), b.free(), c.free()
+    log(f"[bench] batched pass: {np.mean(lat) * 1e3:.2f} ms for {edges // args.steps} records; load {load_s:.1f}s")
+
+    # plain path: one engine and one call per graph and batch, on a sample
+    rng = np.random.default_rng(3)
+    sample = rng.choice(len(ws), size=min(args.plain_sample, len(ws)), replace=False)
+    plain_t, plain_e, per_graph = 0.0, 0, []
+    for i in sample:
+        w = ws[i]
+        e = hl.HitLeiden(cfg)
+        e.load(graphs[i])
+        bd = [(len(b[0]), hip.DeviceArray(b[0]), hip.DeviceArray(b[1]), hip.DeviceArray(b[2])) for b in w.batches]
+        hip.synchronize()
+        for t in range(args.warmup):
+            k, du, dv, da = bd[t]
+            e.apply_batch_device(k, du.address, dv.address, da.address)
+        for t in range(args.warmup, nb):
+            k, du, dv, da = bd[t]
+            t1 = time.perf_counter()
+            e.apply_batch_device(k, du.address, dv.address, da.address)
+            dt = time.perf_counter() - t1
+            plain_t += dt
+            plain_e += k
+            per_graph.append(dt)
+        e.close()
+        for _, a, b, c in bd:
+            a.free(), b.free(), c.free()
+    plain = plain_e / plain_t
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": METRIC, "value": round(batched, 1), "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(float(lat_ms.mean()), 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(ws)} SBM graphs, n log-uniform in [200, 50000], 4-64 blocks, mean degree 8, "
+                               "mu=0.25, unit weights; per graph and step one batch of k new edges, k log-uniform "
+                               "in [16, 2000]; a step is one pass over every graph's batch",
+                   "graphs": len(ws), "vertices": sum_n, "edges": sum_m, "records_per_step": edges // args.steps,
+                   "levels": args.levels, "mode": "throughput", "gamma": args.gamma},
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
+        "plain_edges_per_s": round(plain, 1), "plain_sample_graphs": len(sample),
+        "plain_p50_ms_per_graph": round(float(np.percentile(per_graph, 50)) * 1e3, 4),
+        "plain_p99_ms_per_graph": round(float(np.percentile(per_graph, 99)) * 1e3, 4),
+        "speedup_vs_plain": round(batched / plain, 2), "load_seconds": round(load_s, 3),
+        "modularity_mean": round(float(np.mean(q)), 6), "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="mid", choices=["small", "mid", "large"])
+    ap.add_argument("--config", default="mid", choices=["small", "mid", "large", "batched"])
+    ap.add_argument("--graphs", type=int, default=4096, help="graphs of the batched configuration")
+    ap.add_argument("--plain-sample", type=int, default=64,
+                    help="graphs timed one by one for the plain path of the batched configuration")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--mode", default="throughput", choices=["throughput", "deterministic"])
     ap.add_argument("--gamma", type=float, default=1.0)
@@ -128,6 +217,10 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               f"(the path runs on a single MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "batched":
+        bench_batched(args)
+        return
 
     import hit_leiden_amd as hl
     from hit_leiden_amd import hip

@@ -138,3 +138,23 @@ def large_config(seed=13, scale=24, batches=10, threads=0):
     w = rmat(scale, 16, seed=seed, threads=threads)
     w.make_batches(batches, 95000, 5000, mode=2, scale=scale, seed=seed + 1)
     return w
+
+
+def batched_config(count=4096, steps=1, seed=19, threads=16):
+    """SBM graphs with n log-uniform in [200, 50,000], 4-64 blocks, mean degree
+    8, mu = 0.25, unit weights; each gets `steps` successive batches of k new-edge
+    insertions with uniform endpoints, k log-uniform in [16, 2,000] per graph."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(200), np.log(50000), count)).astype(np.int64)
+    blocks = rng.integers(4, 65, count)
+    ks = np.exp(rng.uniform(np.log(16), np.log(2000), count)).astype(np.int64)
+
+    def one(i):
+        w = sbm(int(ns[i]), int(blocks[i]), 8.0, 0.25, seed=seed * 100003 + i)
+        w.make_batches(steps, int(ks[i]), 0, mode=1, seed=seed * 7919 + i)
+        return w
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(one, range(count)))
+
