@@ -92,6 +92,23 @@ def library_path():
     return _LIB_PATH
 
 
+class _Request(C.Structure):
+    _fields_ = [("graph", C.c_uint32), ("k", C.c_uint64), ("src", C.c_void_p), ("dst", C.c_void_p),
+                ("alpha", C.c_void_p), ("arrival", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    """hl_serve_stats: passes, records, largest pass (requests), wall and busy seconds."""
+    _fields_ = [("passes", C.c_uint64), ("records", C.c_uint64), ("max_pass_requests", C.c_uint64),
+                ("seconds", C.c_double), ("busy_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_EMIT = C.CFUNCTYPE(None, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint32)
+
+
 def lib():
     """Load libhitleiden.so; fails loudly when it was not built."""
     global _lib
@@ -125,6 +142,8 @@ def lib():
             "hl_profile_enable": ([vp, C.c_int32], C.c_int),
             "hl_profile_read": ([vp, C.c_int32, C.c_char_p, C.c_uint64, u64p, f64p, u64p], C.c_int),
             "hl_load_graphs": ([vp, C.c_uint32, vp, C.c_uint64, vp, vp, vp, C.POINTER(BatchStats)], C.c_int),
+            "hl_serve": ([vp, C.POINTER(_Request), C.c_uint64, C.c_uint64, vp, vp, _EMIT, vp,
+                          C.POINTER(ServeStats)], C.c_int),
             "hl_graph_count": ([vp, u32p], C.c_int),
             "hl_modularity_graphs": ([vp, vp, C.c_uint64], C.c_int),
         }
@@ -304,6 +323,30 @@ class HitLeiden:
         self.last_stats = s
         return s
 
+    def serve(self, requests, max_pass_records=1 << 20, emit=None):
+        """Open-loop serving of [(graph, GraphInput batch with graph-local ids, arrival seconds)]
+        (hl_serve).  emit(request, labels) receives each request's community labels
+        right after its pass.  Returns (done seconds, pass index, ServeStats)."""
+        keep, reqs = [], (_Request * max(len(requests), 1))()
+        for i, (gi, b, t) in enumerate(requests):
+            src = np.ascontiguousarray(b.src, dtype=np.uint32)
+            dst = np.ascontiguousarray(b.dst, dtype=np.uint32)
+            a = None if b.weights is None else np.ascontiguousarray(b.weights, dtype=np.float64)
+            keep.append((src, dst, a))
+            reqs[i] = _Request(int(gi), len(src), src.ctypes.data, dst.ctypes.data,
+                               None if a is None else a.ctypes.data, float(t))
+        done = np.zeros(max(len(requests), 1), dtype=np.float64)
+        pass_of = np.zeros(max(len(requests), 1), dtype=np.uint64)
+        st = ServeStats()
+        cb = _EMIT(0)
+        if emit is not None:
+            def _cb(_ctx, r, labels, n):
+                emit(int(r), np.ctypeslib.as_array(labels, shape=(n,)).copy() if n else np.zeros(0, np.uint32))
+            cb = _EMIT(_cb)
+        self._check(lib().hl_serve(self._h, reqs, len(requests), int(max_pass_records), _ptr(done), _ptr(pass_of),
+                                   cb, None, C.byref(st)))
+        return done[:len(requests)], pass_of[:len(requests)], st
+
     @property
     def graph_count(self):
         x = C.c_uint32()
@@ -457,7 +500,7 @@ def compute_modularity(graph: GraphInput, labels, device=0) -> float:
         h.close()
 
 
-__all__ = ["RunMode", "RunConfig", "GraphInput", "HitLeiden", "BatchStats", "compute_modularity",
+__all__ = ["ServeStats", "RunMode", "RunConfig", "GraphInput", "HitLeiden", "BatchStats", "compute_modularity",
            "HitLeidenError", "InvalidInput", "BackendError", "AccelerationError", "OutOfMemory", "lib",
            "library_path", "MAP_F", "MAP_G", "MAP_S_CUR", "MAP_S_PRE", "SET_ACTIVE", "SET_CHANGED",
            "SET_AFFECTED", "SET_REFINED", "PART_COMMUNITY", "PART_REFINED", "PART_F0"]

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hitleiden.h"
@@ -951,6 +952,112 @@ hl_status hl_modularity_graphs(const hl_graph* g, double* q, uint64_t cap) {
     e.lab.reserve(e.n0 + 1, e.s);
     e.compose(e.P - 1, true, e.lab.p);
     hl::modularity_graphs(e.L[0]->csr(), e.lab.p, e.G, q, e.sig, e.mcg, e.acc, e.mx, e.s);
+  });
+}
+
+hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t max_pass_records, double* done,
+                   uint64_t* pass_of, hl_emit_fn emit, void* ctx, hl_serve_stats* stats) {
+  if (!g || (nreq && (!reqs || !done))) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = g->e;
+    if (!e.loaded || !e.multi()) throw hl::InputError("hl_serve needs graphs loaded with hl_load_graphs");
+    for (uint64_t r = 0; r < nreq; ++r) {
+      const hl_request& q = reqs[r];
+      if (q.graph >= e.G || (q.k && (!q.src || !q.dst)) || !std::isfinite(q.arrival))
+        throw hl::InputError("invalid request");
+      const uint32_t ng = e.goff[q.graph + 1] - e.goff[q.graph];
+      for (uint64_t i = 0; i < q.k; ++i) {
+        if (q.src[i] >= ng || q.dst[i] >= ng) throw hl::InputError("request endpoint outside its graph");
+        if (q.alpha && !std::isfinite(q.alpha[i])) throw hl::InputError("non-finite weight");
+      }
+    }
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    std::vector<uint64_t> order(nreq);
+    for (uint64_t r = 0; r < nreq; ++r) order[r] = r;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint64_t a, uint64_t b) { return reqs[a].arrival < reqs[b].arrival; });
+    std::vector<std::vector<uint64_t>> waiting(e.G);  // per graph, in arrival order
+    std::vector<size_t> head(e.G, 0);
+    std::vector<uint32_t> active;                     // graphs with waiting requests
+    std::vector<char> is_active(e.G, 0);
+    std::vector<hl::Rec> h;
+    std::vector<uint32_t> lab;
+    hl_serve_stats st{};
+    uint64_t next = 0, completed = 0, pass = 0;
+    const double t0 = now_s();
+    while (completed < nreq) {
+      double now = now_s() - t0;
+      while (next < nreq && reqs[order[next]].arrival <= now) {
+        const uint64_t r = order[next++];
+        const uint32_t gr = reqs[r].graph;
+        waiting[gr].push_back(r);
+        if (!is_active[gr]) {
+          is_active[gr] = 1;
+          active.push_back(gr);
+        }
+      }
+      if (active.empty()) {
+        const double wait = reqs[order[next]].arrival - now;
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
+        continue;
+      }
+      // admit the oldest waiting request of each graph, oldest first, within the record budget
+      std::vector<std::pair<double, uint32_t>> heads;
+      for (uint32_t gr : active) heads.push_back({reqs[waiting[gr][head[gr]]].arrival, gr});
+      std::sort(heads.begin(), heads.end());
+      std::vector<uint64_t> admitted;
+      uint64_t recs = 0;
+      for (const auto& hg : heads) {
+        const uint64_t r = waiting[hg.second][head[hg.second]];
+        if (!admitted.empty() && (max_pass_records == 0 || recs + reqs[r].k > max_pass_records)) break;
+        admitted.push_back(r);
+        recs += reqs[r].k;
+      }
+      h.resize(recs);
+      uint64_t o = 0;
+      for (uint64_t r : admitted) {
+        const hl_request& q = reqs[r];
+        const uint32_t base = e.goff[q.graph];
+        for (uint64_t i = 0; i < q.k; ++i) h[o++] = hl::Rec{base + q.src[i], base + q.dst[i], q.alpha ? q.alpha[i] : 1.0};
+      }
+      const double tp = now_s();
+      e.dcur.reserve(recs + 1, e.s);
+      if (recs) HL_CHECK(hipMemcpyAsync(e.dcur.p, h.data(), recs * sizeof(hl::Rec), hipMemcpyHostToDevice, e.s));
+      e.run_batch(recs, nullptr);
+      const double tdone = now_s() - t0;
+      st.busy_seconds += now_s() - tp;
+      if (emit) {
+        e.lab.reserve(e.n0 + 1, e.s);
+        e.compose(e.P - 1, true, e.lab.p);
+      }
+      for (uint64_t r : admitted) {
+        const uint32_t gr = reqs[r].graph;
+        done[r] = tdone;
+        if (pass_of) pass_of[r] = pass;
+        if (emit) {
+          const uint32_t lo = e.goff[gr], ng = e.goff[gr + 1] - lo;
+          lab.resize(std::max<uint32_t>(ng, 1));
+          if (ng)
+            HL_CHECK(hipMemcpyAsync(lab.data(), e.lab.p + lo, ng * sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
+          hl::stream_sync(e.s);
+          emit(ctx, r, lab.data(), ng);
+        }
+        if (++head[gr] == waiting[gr].size()) {
+          waiting[gr].clear();
+          head[gr] = 0;
+          is_active[gr] = 0;
+        }
+      }
+      active.erase(std::remove_if(active.begin(), active.end(), [&](uint32_t gr) { return !is_active[gr]; }),
+                   active.end());
+      completed += admitted.size();
+      st.records += recs;
+      st.max_pass_requests = std::max<uint64_t>(st.max_pass_requests, admitted.size());
+      ++pass;
+    }
+    st.passes = pass;
+    st.seconds = now_s() - t0;
+    if (stats) *stats = st;
   });
 }
 

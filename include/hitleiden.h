@@ -170,6 +170,42 @@ hl_status hl_graph_count(const hl_graph* g, uint32_t* ngraphs);
 /* compute_modularity (hit_leiden.rs:121-161) of every graph's communities, q[0..ngraphs) */
 hl_status hl_modularity_graphs(const hl_graph* g, double* q, uint64_t cap);
 
+/* ---- Served form ----
+   No counterpart in the reference (README goal 3, a resident service, is
+   stated there but not built).  hl_serve runs an open-loop request schedule on
+   a several-graph engine: request r becomes visible at reqs[r].arrival seconds
+   after the call starts; each pass admits, in arrival order, the oldest
+   waiting request of every graph (requests of one graph stay in order, one per
+   pass) while the pass holds at most max_pass_records delta records (the
+   device footprint of a pass grows with them; 0 admits one request per pass,
+   the plain one-batch-per-call path), applies them together in one
+   hit_leiden pass and, after it, reports each request's completion time
+   (done[r], seconds from the start) and pass index, and hands the graph's
+   community labels (graph-local ids, the top hierarchy row) to emit when emit
+   is not NULL.  Each request's result is the one it gets alone: the graphs
+   of one pass do not interact. */
+typedef struct {
+  uint32_t graph;       /* graph index */
+  uint64_t k;           /* records */
+  const uint32_t* src;  /* graph-local vertex ids */
+  const uint32_t* dst;
+  const double* alpha;  /* NULL: +1.0 each (insertions) */
+  double arrival;       /* seconds after the start of hl_serve */
+} hl_request;
+
+typedef struct {
+  uint64_t passes;
+  uint64_t records;
+  uint64_t max_pass_requests;
+  double seconds;       /* wall time from the start to the last completion */
+  double busy_seconds;  /* time spent inside passes */
+} hl_serve_stats;
+
+typedef void (*hl_emit_fn)(void* ctx, uint64_t request, const uint32_t* labels, uint32_t n);
+
+hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t max_pass_records, double* done,
+                   uint64_t* pass_of, hl_emit_fn emit, void* ctx, hl_serve_stats* stats);
+
 /* Per-kernel device timing for roofline accounting.  No counterpart in the
    reference (it profiles out of process with perf/samply, profiling.rs:74-152).
    hl_profile_enable(g, 1) resets and starts recording HIP events around each

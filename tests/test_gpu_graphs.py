@@ -133,3 +133,45 @@ def test_graph_set_errors():
     with pytest.raises(hl.InvalidInput):
         d.load_graphs(graphs)
     d.close()
+
+
+def served_requests(graphs, count, rate, seed):
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    zipf = 1.0 / np.arange(1, len(graphs) + 1) ** 1.1
+    pick = rng.choice(len(graphs), size=count, p=zipf / zipf.sum())
+    return [(int(g), insert_batch(graphs[g], int(rng.integers(5, 60)), seed * 7 + i), float(t[i]))
+            for i, g in enumerate(pick)]
+
+
+@pytest.mark.parametrize("budget", [1 << 20, 0])
+def test_served_results_match_alone(budget):
+    """Continuous batching: every request's communities equal the ones it gets
+    alone (its graph's earlier requests applied one by one); budget 0 is the
+    plain one-request-per-pass path."""
+    graphs = make_graphs(10, 31)
+    u = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=P))
+    u.load_graphs(graphs)
+    reqs = served_requests(graphs, 60, 2000.0, 4)
+    got = {}
+    done, pass_of, st = u.serve(reqs, max_pass_records=budget, emit=lambda r, lab: got.__setitem__(r, lab))
+    say(f"budget={budget} passes={st.passes} max_pass_requests={st.max_pass_requests} busy={st.busy_seconds:.3f}s")
+    assert len(got) == len(reqs) and np.all(done > 0)
+    if budget == 0:
+        assert st.passes == len(reqs)
+    else:
+        assert st.passes < len(reqs)  # arrivals faster than passes were coalesced
+    # per graph, requests complete in arrival order and in different passes
+    for g in range(len(graphs)):
+        mine = sorted((t, i) for i, (gg, _, t) in enumerate(reqs) if gg == g)
+        ps = [int(pass_of[i]) for _, i in mine]
+        assert ps == sorted(ps) and len(set(ps)) == len(ps)
+    for g, graph in enumerate(graphs):
+        e = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=P))
+        e.load(graph)
+        for _, i in sorted((t, i) for i, (gg, _, t) in enumerate(reqs) if gg == g):
+            e.hit_leiden(reqs[i][1])
+            want = _first_seen(e.hierarchy()[-1])
+            np.testing.assert_array_equal(_first_seen(got[i]), want, err_msg=f"request {i} of graph {g}")
+        e.close()
+    u.close()
