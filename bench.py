@@ -177,12 +177,77 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_served(args):
+    """The served form: resident graphs under Poisson request streams, one open-loop
+    run per offered rate with continuous batching, then the plain path (one
+    request per pass) at the highest rate."""
+    import hit_leiden_amd as hl
+    from hit_leiden_amd import gen
+    t0 = time.time()
+    ws = gen.served_config(count=args.graphs if args.graphs != 4096 else 256)
+    sizes = [w.n for w in ws]
+    log(f"[bench] served: {len(ws)} graphs, {sum(sizes)} vertices, {sum(w.m for w in ws)} edges, "
+        f"gen {time.time() - t0:.1f}s")
+    cfg = hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=args.levels, resolution=args.gamma)
+    u = hl.HitLeiden(cfg)
+    t0 = time.time()
+    u.load_graphs([hl.GraphInput(w.n, w.u, w.v) for w in ws])
+    load_s = time.time() - t0
+    log(f"[bench] load {load_s:.1f}s")
+    del ws
+    rates = [float(x) for x in args.rates.split(",")]
+    runs = []
+
+    def run(rate, count, budget, seed, coalesce):
+        rq = gen.served_requests(sizes, rate, count, seed=seed)
+        reqs = [(g, hl.GraphInput(sizes[g], a, b), t) for g, a, b, t in rq]
+        done, pass_of, st = u.serve(reqs, max_pass_records=budget, coalesce=coalesce)
+        lat = (done - np.array([t for _, _, _, t in rq])) * 1e3
+        edges = sum(len(a) for _, a, _, _ in rq)
+        r = {"offered_req_per_s": rate, "requests": count, "completed_req_per_s": round(count / st.seconds, 2),
+             "edges_per_s": round(edges / st.seconds, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+             "p99_ms": round(float(np.percentile(lat, 99)), 3),
+             "first_community_p50_ms": round(float(np.percentile(lat, 50)), 3),
+             "passes": int(st.passes), "requests_per_pass": round(count / max(st.passes, 1), 2),
+             "busy_frac": round(st.busy_seconds / st.seconds, 3), "max_pass_records": budget,
+             "coalesce": coalesce}
+        log(f"[bench] rate {rate:g}/s budget {budget}: {r}")
+        return r
+
+    for i, rate in enumerate(rates):
+        runs.append(run(rate, int(min(max(rate * args.served_seconds, 200), 8000)), args.pass_records, 100 + i,
+                        not args.no_coalesce))
+    top = rates[-1]
+    plain = run(top, args.plain_requests, 0, 999, False)
+    best = runs[-1]
+    out = {
+        "metric": METRIC, "value": best["edges_per_s"], "unit": "edges/s", "n_gpus": 1, "steps": len(runs),
+        "warmup": 0, "ms_per_step": best["p50_ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(sizes)} resident LFR-style graphs, n log-uniform in [1e4, 2e6], degree "
+                               "exponent 2.5, mean degree 10, mu=0.2; Poisson requests of 10-5000 insertions "
+                               "(log-uniform) to a Zipf(1.1) graph", "vertices": int(sum(sizes)),
+                   "rates": rates, "levels": args.levels, "mode": "throughput", "gamma": args.gamma},
+        "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "runs": runs, "plain": plain,
+        "speedup_vs_plain": round(best["completed_req_per_s"] / plain["completed_req_per_s"], 2),
+        "load_seconds": round(load_s, 3), "roofline": None, "cpu_baseline": None,
+    }
+    u.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="mid", choices=["small", "mid", "large", "batched"])
+    ap.add_argument("--config", default="mid", choices=["small", "mid", "large", "batched", "served"])
+    ap.add_argument("--rates", default="100,500,2000", help="offered request rates of the served configuration")
+    ap.add_argument("--served-seconds", type=float, default=3.0, help="arrival span per offered rate")
+    ap.add_argument("--pass-records", type=int, default=1 << 20, help="record budget of a served pass")
+    ap.add_argument("--plain-requests", type=int, default=600, help="requests of the one-per-pass run")
+    ap.add_argument("--no-coalesce", action="store_true",
+                    help="served passes take one request per graph instead of all its waiting ones")
     ap.add_argument("--graphs", type=int, default=4096, help="graphs of the batched configuration")
     ap.add_argument("--plain-sample", type=int, default=64,
                     help="graphs timed one by one for the plain path of the batched configuration")
@@ -220,6 +285,9 @@ def main():
 
     if args.config == "batched":
         bench_batched(args)
+        return
+    if args.config == "served":
+        bench_served(args)
         return
 
     import hit_leiden_amd as hl

@@ -63,7 +63,7 @@ _ERRORS = {HL_INVALID_INPUT: InvalidInput, HL_BACKEND: BackendError, HL_ACCELERA
 class _Config(C.Structure):
     _fields_ = [("gamma", C.c_double), ("mode", C.c_int32), ("levels", C.c_int32), ("max_rounds", C.c_int32),
                 ("device", C.c_int32), ("small_degree", C.c_uint32), ("wave_degree", C.c_uint32),
-                ("reserved", C.c_uint32 * 6)]
+                ("load_pass_records", C.c_uint32), ("reserved", C.c_uint32 * 5)]
 
 
 class BatchStats(C.Structure):
@@ -142,7 +142,7 @@ def lib():
             "hl_profile_enable": ([vp, C.c_int32], C.c_int),
             "hl_profile_read": ([vp, C.c_int32, C.c_char_p, C.c_uint64, u64p, f64p, u64p], C.c_int),
             "hl_load_graphs": ([vp, C.c_uint32, vp, C.c_uint64, vp, vp, vp, C.POINTER(BatchStats)], C.c_int),
-            "hl_serve": ([vp, C.POINTER(_Request), C.c_uint64, C.c_uint64, vp, vp, _EMIT, vp,
+            "hl_serve": ([vp, C.POINTER(_Request), C.c_uint64, C.c_uint64, C.c_int32, vp, vp, _EMIT, vp,
                           C.POINTER(ServeStats)], C.c_int),
             "hl_graph_count": ([vp, u32p], C.c_int),
             "hl_modularity_graphs": ([vp, vp, C.c_uint64], C.c_int),
@@ -178,6 +178,7 @@ class RunConfig:
     device: int = 0
     small_degree: int = 0
     wave_degree: int = 0
+    load_pass_records: int = 0
 
 
 class GraphInput:
@@ -225,6 +226,7 @@ class HitLeiden:
         c.device = int(self.config.device)
         c.small_degree = int(self.config.small_degree)
         c.wave_degree = int(self.config.wave_degree)
+        c.load_pass_records = int(self.config.load_pass_records)
         h = C.c_void_p()
         st = lib().hl_create(C.byref(c), C.byref(h))
         if st != HL_OK:
@@ -323,7 +325,7 @@ class HitLeiden:
         self.last_stats = s
         return s
 
-    def serve(self, requests, max_pass_records=1 << 20, emit=None):
+    def serve(self, requests, max_pass_records=1 << 20, emit=None, coalesce=False):
         """Open-loop serving of [(graph, GraphInput batch with graph-local ids, arrival seconds)]
         (hl_serve).  emit(request, labels) receives each request's community labels
         right after its pass.  Returns (done seconds, pass index, ServeStats)."""
@@ -343,8 +345,8 @@ class HitLeiden:
             def _cb(_ctx, r, labels, n):
                 emit(int(r), np.ctypeslib.as_array(labels, shape=(n,)).copy() if n else np.zeros(0, np.uint32))
             cb = _EMIT(_cb)
-        self._check(lib().hl_serve(self._h, reqs, len(requests), int(max_pass_records), _ptr(done), _ptr(pass_of),
-                                   cb, None, C.byref(st)))
+        self._check(lib().hl_serve(self._h, reqs, len(requests), int(max_pass_records), int(bool(coalesce)),
+                                   _ptr(done), _ptr(pass_of), cb, None, C.byref(st)))
         return done[:len(requests)], pass_of[:len(requests)], st
 
     @property

@@ -586,6 +586,8 @@ hl_status guard(const hl_graph* gc, F&& fn) {
   }
 }
 
+constexpr uint64_t kLoadPassRecords = 1ull << 26;  // edges per pass of hl_load_graphs
+
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -691,9 +693,39 @@ hl_status hl_load_graphs(hl_graph* g, uint32_t ngraphs, const uint32_t* vertex_o
     e.G = ngraphs;
     e.goff.assign(vertex_offsets, vertex_offsets + ngraphs + 1);
     const uint32_t n = vertex_offsets[ngraphs];
-    e.upload_batch(m, src, dst, w, n);
+    // group the edges by graph (counting sort), then load the graphs in passes
+    // of at most kLoadPassRecords records: graphs are independent, so each gets
+    // the same load as in one pass over everything, with bounded scratch
+    std::vector<uint64_t> start(size_t(ngraphs) + 1, 0);
+    std::vector<uint32_t> gof(m);
+    for (uint64_t i = 0; i < m; ++i) {
+      if (src[i] >= n || dst[i] >= n) throw hl::InputError("endpoint id out of range");
+      const uint32_t a = e.graph_of(src[i]);
+      if (a != e.graph_of(dst[i])) throw hl::InputError("edge joins two graphs");
+      if (w && !std::isfinite(w[i])) throw hl::InputError("non-finite weight");
+      gof[i] = a;
+      ++start[a + 1];
+    }
+    for (uint32_t i = 0; i < ngraphs; ++i) start[i + 1] += start[i];
+    std::vector<hl::Rec> h(m);
+    {
+      std::vector<uint64_t> pos(start.begin(), start.end() - 1);
+      for (uint64_t i = 0; i < m; ++i) h[pos[gof[i]]++] = hl::Rec{src[i], dst[i], w ? w[i] : 1.0};
+    }
+    std::vector<uint32_t>().swap(gof);
     e.reset(n, m);
-    e.run_batch(m, stats);
+    uint32_t g0 = 0;
+    do {
+      uint32_t g1 = g0 + 1;
+      const uint64_t budget = e.cfg.load_pass_records ? e.cfg.load_pass_records : kLoadPassRecords;
+      while (g1 < ngraphs && start[g1 + 1] - start[g0] <= budget) ++g1;
+      const uint64_t k = start[g1] - start[g0];
+      e.dcur.reserve(k + 1, e.s);
+      if (k) HL_CHECK(hipMemcpyAsync(e.dcur.p, h.data() + start[g0], k * sizeof(hl::Rec), hipMemcpyHostToDevice, e.s));
+      if (stats) std::memset(stats, 0, sizeof(*stats));
+      e.run_batch(k, stats);
+      g0 = g1;
+    } while (g0 < ngraphs);
     e.trim();
     if (stats) stats->seconds = now_s() - t0;
   });
@@ -955,8 +987,8 @@ hl_status hl_modularity_graphs(const hl_graph* g, double* q, uint64_t cap) {
   });
 }
 
-hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t max_pass_records, double* done,
-                   uint64_t* pass_of, hl_emit_fn emit, void* ctx, hl_serve_stats* stats) {
+hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t max_pass_records, int32_t coalesce,
+                   double* done, uint64_t* pass_of, hl_emit_fn emit, void* ctx, hl_serve_stats* stats) {
   if (!g || (nreq && (!reqs || !done))) return HL_INVALID_INPUT;
   return guard(g, [&] {
     Engine& e = g->e;
@@ -1005,13 +1037,22 @@ hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t 
       std::vector<std::pair<double, uint32_t>> heads;
       for (uint32_t gr : active) heads.push_back({reqs[waiting[gr][head[gr]]].arrival, gr});
       std::sort(heads.begin(), heads.end());
-      std::vector<uint64_t> admitted;
+      std::vector<uint64_t> admitted;  // grouped by graph, each graph's requests in arrival order
       uint64_t recs = 0;
+      bool full = false;
       for (const auto& hg : heads) {
-        const uint64_t r = waiting[hg.second][head[hg.second]];
-        if (!admitted.empty() && (max_pass_records == 0 || recs + reqs[r].k > max_pass_records)) break;
-        admitted.push_back(r);
-        recs += reqs[r].k;
+        const uint32_t gr = hg.second;
+        for (size_t j = head[gr]; j < waiting[gr].size() && !full; ++j) {
+          const uint64_t r = waiting[gr][j];
+          if (!admitted.empty() && (max_pass_records == 0 || recs + reqs[r].k > max_pass_records)) {
+            full = true;
+            break;
+          }
+          admitted.push_back(r);
+          recs += reqs[r].k;
+          if (!coalesce) break;  // one request of this graph per pass
+        }
+        if (full) break;
       }
       h.resize(recs);
       uint64_t o = 0;

@@ -158,3 +158,33 @@ def batched_config(count=4096, steps=1, seed=19, threads=16):
     with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
         return list(ex.map(one, range(count)))
 
+
+def served_config(count=256, seed=23, threads=16):
+    """LFR-style graphs with n log-uniform in [10,000, 2,000,000], degree
+    exponent 2.5, mean degree 10, mu = 0.2, unit weights."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(10000), np.log(2000000), count)).astype(np.int64)
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(lambda i: lfr(int(ns[i]), 10.0, 0.2, seed=seed * 100003 + i), range(count)))
+
+
+def served_requests(sizes, rate, count, seed=29, kmin=10, kmax=5000, zipf_s=1.1):
+    """Poisson arrivals at `rate` per second; each request inserts k uniform
+    random vertex pairs (k log-uniform in [kmin, kmax]) into a graph drawn
+    Zipf(zipf_s) by index.  Returns [(graph, src, dst, arrival)] with
+    graph-local ids."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    z = 1.0 / np.arange(1, len(sizes) + 1) ** zipf_s
+    pick = rng.choice(len(sizes), size=count, p=z / z.sum())
+    ks = np.exp(rng.uniform(np.log(kmin), np.log(kmax), count)).astype(np.int64)
+    out = []
+    for i in range(count):
+        n = int(sizes[pick[i]])
+        a = rng.integers(0, n, ks[i])
+        b = rng.integers(0, n - 1, ks[i])
+        b = b + (b >= a)  # no self-loops
+        out.append((int(pick[i]), a.astype(np.uint32), b.astype(np.uint32), float(t[i])))
+    return out
+

@@ -71,7 +71,8 @@ typedef struct {
   int32_t device;      /* HIP device ordinal */
   uint32_t small_degree; /* autotuned bucket thresholds; 0 = built-in defaults */
   uint32_t wave_degree;
-  uint32_t reserved[6];
+  uint32_t load_pass_records; /* hl_load_graphs: edges per loading pass, 0 = 2^26 */
+  uint32_t reserved[5];
 } hl_config;
 
 #define HL_MAX_LEVELS 8
@@ -182,8 +183,11 @@ hl_status hl_modularity_graphs(const hl_graph* g, double* q, uint64_t cap);
    hit_leiden pass and, after it, reports each request's completion time
    (done[r], seconds from the start) and pass index, and hands the graph's
    community labels (graph-local ids, the top hierarchy row) to emit when emit
-   is not NULL.  Each request's result is the one it gets alone: the graphs
-   of one pass do not interact. */
+   is not NULL.  With coalesce != 0 a pass takes every waiting request of a
+   graph (in arrival order, within the budget) and applies their records as
+   one batch of that graph; their results are the graph after that batch.
+   Each graph's result is the one it gets alone: the graphs of one pass do not
+   interact. */
 typedef struct {
   uint32_t graph;       /* graph index */
   uint64_t k;           /* records */
@@ -203,8 +207,8 @@ typedef struct {
 
 typedef void (*hl_emit_fn)(void* ctx, uint64_t request, const uint32_t* labels, uint32_t n);
 
-hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t max_pass_records, double* done,
-                   uint64_t* pass_of, hl_emit_fn emit, void* ctx, hl_serve_stats* stats);
+hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t max_pass_records, int32_t coalesce,
+                   double* done, uint64_t* pass_of, hl_emit_fn emit, void* ctx, hl_serve_stats* stats);
 
 /* Per-kernel device timing for roofline accounting.  No counterpart in the
    reference (it profiles out of process with perf/samply, profiling.rs:74-152).

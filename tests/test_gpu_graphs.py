@@ -101,7 +101,8 @@ def test_each_graph_updates_as_if_alone():
 
 def test_graphs_match_the_oracle():
     graphs = make_graphs(6, 17)
-    u = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=P))
+    # small loading passes: the graphs load over several passes, as large sets do
+    u = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=P, load_pass_records=3000))
     u.load_graphs(graphs)
     deltas = {i: insert_batch(g, 40, 7 + i, dels=4) for i, g in enumerate(graphs)}
     u.apply_graphs(deltas)
@@ -144,34 +145,40 @@ def served_requests(graphs, count, rate, seed):
             for i, g in enumerate(pick)]
 
 
-@pytest.mark.parametrize("budget", [1 << 20, 0])
-def test_served_results_match_alone(budget):
-    """Continuous batching: every request's communities equal the ones it gets
-    alone (its graph's earlier requests applied one by one); budget 0 is the
-    plain one-request-per-pass path."""
+@pytest.mark.parametrize("budget,coalesce", [(1 << 20, False), (0, False), (1 << 20, True)])
+def test_served_results_match_alone(budget, coalesce):
+    """Continuous batching: every request's communities equal the ones its graph
+    gets alone (its earlier requests applied one by one, or, when coalesced, the
+    requests of each pass applied as one batch); budget 0 is the plain
+    one-request-per-pass path."""
     graphs = make_graphs(10, 31)
     u = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=P))
     u.load_graphs(graphs)
     reqs = served_requests(graphs, 60, 2000.0, 4)
     got = {}
-    done, pass_of, st = u.serve(reqs, max_pass_records=budget, emit=lambda r, lab: got.__setitem__(r, lab))
-    say(f"budget={budget} passes={st.passes} max_pass_requests={st.max_pass_requests} busy={st.busy_seconds:.3f}s")
+    done, pass_of, st = u.serve(reqs, max_pass_records=budget, coalesce=coalesce,
+                                emit=lambda r, lab: got.__setitem__(r, lab))
+    say(f"budget={budget} coalesce={coalesce} passes={st.passes} max_pass_requests={st.max_pass_requests}")
     assert len(got) == len(reqs) and np.all(done > 0)
     if budget == 0:
         assert st.passes == len(reqs)
     else:
-        assert st.passes < len(reqs)  # arrivals faster than passes were coalesced
-    # per graph, requests complete in arrival order and in different passes
-    for g in range(len(graphs)):
+        assert st.passes < len(reqs)  # arrivals faster than passes were batched
+    for g, graph in enumerate(graphs):
         mine = sorted((t, i) for i, (gg, _, t) in enumerate(reqs) if gg == g)
         ps = [int(pass_of[i]) for _, i in mine]
-        assert ps == sorted(ps) and len(set(ps)) == len(ps)
-    for g, graph in enumerate(graphs):
+        assert ps == sorted(ps)  # a graph's requests complete in arrival order
+        if not coalesce:
+            assert len(set(ps)) == len(ps)
         e = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=P))
         e.load(graph)
-        for _, i in sorted((t, i) for i, (gg, _, t) in enumerate(reqs) if gg == g):
-            e.hit_leiden(reqs[i][1])
+        for p in sorted(set(ps)):
+            group = [i for _, i in mine if int(pass_of[i]) == p]
+            b = [reqs[i][1] for i in group]
+            e.hit_leiden(hl.GraphInput(graph.node_count, np.concatenate([x.src for x in b]),
+                                       np.concatenate([x.dst for x in b]), np.concatenate([x.weights for x in b])))
             want = _first_seen(e.hierarchy()[-1])
-            np.testing.assert_array_equal(_first_seen(got[i]), want, err_msg=f"request {i} of graph {g}")
+            for i in group:
+                np.testing.assert_array_equal(_first_seen(got[i]), want, err_msg=f"request {i} of graph {g}")
         e.close()
     u.close()
