@@ -288,6 +288,8 @@ def main():
                     help="JSON {slot: HBM bytes per launch} from the FETCH_SIZE / WRITE_SIZE counter runs "
                          "(default: profiles/traffic_<config>.json when present)")
     ap.add_argument("--dump-profile", default=None)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the degree-bucket thresholds on the warmup batches first")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run device-to-device copies of this many bytes (counter calibration)")
     ap.add_argument("--rocprof-summary", default=None, metavar="DIR",
@@ -329,7 +331,15 @@ def main():
         raise SystemExit("not enough batches generated")
 
     mode = hl.RunMode.THROUGHPUT if args.mode == "throughput" else hl.RunMode.DETERMINISTIC
-    g = hl.HitLeiden(hl.RunConfig(resolution=args.gamma, mode=mode, levels=args.levels))
+    small_deg = wave_deg = 0
+    if args.autotune and mode == hl.RunMode.THROUGHPUT:
+        from hit_leiden_amd import autotune
+        t0 = time.time()
+        small_deg, wave_deg = autotune.tune(hl.GraphInput(w.n, w.u, w.v, w.w), w.batches[:args.warmup],
+                                            levels=args.levels, gamma=args.gamma, log=log)
+        log(f"[bench] autotune picked small_degree={small_deg} wave_degree={wave_deg} in {time.time() - t0:.1f}s")
+    g = hl.HitLeiden(hl.RunConfig(resolution=args.gamma, mode=mode, levels=args.levels, small_degree=small_deg,
+                                  wave_degree=wave_deg))
     t0 = time.time()
     ls = g.load(hl.GraphInput(w.n, w.u, w.v, w.w))
     load_s = time.time() - t0
@@ -399,7 +409,8 @@ def main():
         "metric": METRIC, "value": round(value, 1), "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(float(lat_ms.mean()), 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": dict(desc, levels=args.levels, mode=args.mode, gamma=args.gamma),
+        "config": dict(desc, levels=args.levels, mode=args.mode, gamma=args.gamma,
+                       small_degree=small_deg or 16, wave_degree=wave_deg or 512),
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "path_gbs": round(bytes_alg / total / 1e9, 3), "path_hbm_frac": round(bytes_alg / total / 1e9 / HBM_PEAK_GBS, 6),
         "modularity_after_load": round(q0, 6), "modularity_after_steps": round(q1, 6), "load_seconds": round(load_s, 3),

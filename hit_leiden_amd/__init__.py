@@ -145,6 +145,7 @@ def lib():
             "hl_serve": ([vp, C.POINTER(_Request), C.c_uint64, C.c_uint64, C.c_int32, vp, vp, _EMIT, vp,
                           C.POINTER(ServeStats)], C.c_int),
             "hl_graph_count": ([vp, u32p], C.c_int),
+            "hl_restore_level": ([vp, C.c_int32, C.c_uint32, C.c_uint64, vp, vp, vp, vp, vp, vp, vp], C.c_int),
             "hl_modularity_graphs": ([vp, vp, C.c_uint64], C.c_int),
         }
         for name, (args, res) in sig.items():

@@ -833,6 +833,44 @@ hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const ui
   });
 }
 
+hl_status hl_restore_level(hl_graph* g, int32_t level, uint32_t n, uint64_t m, const uint32_t* src,
+                           const uint32_t* dst, const double* w, const uint32_t* f, const uint32_t* gmap,
+                           const uint32_t* s_pre, const uint32_t* s_cur) {
+  if (!g || (m && (!src || !dst))) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = g->e;
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    if (level < 0 || level >= e.P) throw hl::InputError("level out of range");
+    if (level == 0) {
+      e.G = 0;
+      e.goff.clear();
+      e.upload_batch(m, src, dst, w, n);
+      e.reset(n, m);
+    } else {
+      if (!e.loaded) throw hl::InputError("restore level 0 first");
+      if (e.L[level]->n != 0) throw hl::InputError("level already restored");
+      e.upload_batch(m, src, dst, w, n);
+      e.grow_level(level, n);
+    }
+    hl::Level& lv = *e.L[level];
+    if (lv.n != n) throw hl::InputError("level size differs");
+    e.apply_delta_level(lv, e.dcur.p, m, nullptr, level);
+    const uint32_t* maps[4] = {f, gmap, s_pre, s_cur};
+    const int which[4] = {HL_MAP_F, HL_MAP_G, HL_MAP_S_PRE, HL_MAP_S_CUR};
+    for (int i = 0; i < 4; ++i)
+      if (maps[i] && n)
+        HL_CHECK(hipMemcpyAsync(e.map_of(lv, which[i]).p, maps[i], n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));
+    hl::stream_sync(e.s);
+    // bounds the batches rely on, from the restored labels
+    const uint32_t* arrs[4] = {lv.f.p, lv.g.p, lv.s_cur.p, lv.s_pre.p};
+    uint32_t mx[4] = {0, 0, 0, 0};
+    if (n) e.dmax4(arrs, n, mx);
+    lv.tot_valid = false;
+    lv.lab_bound = n ? std::max<uint64_t>(n, uint64_t(mx[0]) + 1) : 0;
+    lv.s_bound = n ? std::max<uint64_t>(n, uint64_t(std::max(mx[2], mx[3])) + 1) : 0;
+  });
+}
+
 hl_status hl_read_frontier(const hl_graph* g, int32_t level, int32_t which, uint8_t* out, uint64_t cap) {
   if (!g || !out) return HL_INVALID_INPUT;
   return guard(g, [&] {

@@ -129,6 +129,16 @@ hl_status hl_read_partition(const hl_graph* g, int32_t level, int32_t which, uin
 /* copy-in (checkpoint restore / seeded starts); values are not validated beyond length */
 hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const uint32_t* in, uint64_t n);
 
+/* Checkpoint restore: rebuild level `level` of a one-graph engine from its
+   graph (m records, each undirected edge once, a self-loop once with its record
+   weight: the hl_read_level_graph content) and its four maps (n entries each;
+   NULL leaves a map at the identity).  Restore levels 0..P-1 in order; level 0
+   resets the handle.  No counterpart in the reference, which keeps state in
+   memory only (README goal 4 lists persistence). */
+hl_status hl_restore_level(hl_graph* g, int32_t level, uint32_t n, uint64_t m, const uint32_t* src,
+                           const uint32_t* dst, const double* w, const uint32_t* f, const uint32_t* gmap,
+                           const uint32_t* s_pre, const uint32_t* s_cur);
+
 /* last batch's A/B/K/R bitsets at one level, one byte per vertex */
 hl_status hl_read_frontier(const hl_graph* g, int32_t level, int32_t which, uint8_t* out, uint64_t cap);
 
