@@ -40,6 +40,11 @@ struct Level {
   // ids no vertex uses); entries below gid_init are initialised
   DevBuf<u32> gid;
   u64 gid_init = 0;
+  // throughput mode: subcommunity sizes and degrees over s_cur labels, kept
+  // current by every kernel that changes s_cur or k
+  DevBuf<u32> ssz;
+  DevBuf<double> sdg;
+  bool sub_valid = false;
 
   Csr csr() const { return Csr{n, row_off.p, row_len.p, col.p, w.p, k.p, twice_m.p, gid.p}; }
   LevelCsrMut mut() { return LevelCsrMut{n, row_off.p, row_len.p, row_cap.p, col.p, w.p, k.p, twice_m.p}; }
@@ -151,7 +156,29 @@ struct Engine {
     lv.lab_bound = std::max<u64>(lv.lab_bound, bf + (n_new - n_old));
     lv.s_bound = std::max<u64>(lv.s_bound, bs + (n_new - n_old));
     if (multi()) ensure_gid(lv, n_new);
+    if (lv.sub_valid) {
+      ensure_sub(lv);
+      sub_add_vertices(lv.s_cur.p, n_old, n_new, lv.ssz.p, s);  // fresh singletons, k = 0
+    }
   }
+
+  // maintained subcommunity sizes/degrees: room for every label below s_bound
+  // plus one new split id per vertex; computed from scratch when invalid
+  void ensure_sub(Level& lv) {
+    const u64 need = std::max<u64>(lv.s_bound, lv.n) + lv.n + 1;
+    if (lv.sub_valid && lv.ssz.cap >= need) return;
+    const u64 old = lv.ssz.cap;
+    lv.ssz.reserve(need, s, lv.sub_valid, 1.25);
+    lv.sdg.reserve(lv.ssz.cap, s, lv.sub_valid, 1.0);
+    if (lv.sub_valid) {
+      fill_u32(lv.ssz.p + old, 0, lv.ssz.cap - old, s);
+      fill_f64(lv.sdg.p + old, 0.0, lv.sdg.cap - old, s);
+    } else {
+      sub_recompute(lv.s_cur.p, lv.k.p, lv.n, lv.ssz.p, lv.sdg.p, lv.ssz.cap, s);
+      lv.sub_valid = true;
+    }
+  }
+  SubRef sub_of(Level& lv) { return lv.sub_valid ? SubRef{lv.ssz.p, lv.sdg.p} : SubRef{}; }
 
   bool multi() const { return G > 0; }
   void ensure_gid(Level& lv, u64 need) {
@@ -228,6 +255,10 @@ struct Engine {
       if (multi()) {
         tr.tm = lv.twice_m.p;  // each graph's 2m follows its rows' degree changes
         tr.gid = lv.gid.p;
+      }
+      if (par() && lv.sub_valid) {
+        tr.dS = lv.sdg.p;
+        tr.s = lv.s_cur.p;
       }
       apply_delta_commit(g, lv.pool_used, as, plan, mode == HL_MODE_DETERMINISTIC, tr, s);
       lv.pool_used += plan.need;
@@ -316,7 +347,9 @@ struct Engine {
     if (n == 0) return;
     Csr g = lv.csr();
     bool identity = false;
-    const u64 sn = cc_split(g, lv.s_cur.p, lv.K.p, lv.R.p, std::max<u64>(lv.s_bound, 1), &identity, rs, tmp, s);
+    if (par()) ensure_sub(lv);
+    const u64 sn = cc_split(g, lv.s_cur.p, lv.K.p, lv.R.p, std::max<u64>(lv.s_bound, 1), &identity, sub_of(lv), rs,
+                            tmp, s);
     lv.s_bound = std::max(lv.s_bound, sn);
     if (multi())
       identity_graphs_to_R(lv.s_cur.p, lv.gid.p, n, G, part.p, gnid.p, lv.R.p, s);  // per graph
@@ -328,7 +361,7 @@ struct Engine {
                  max_degree(lv), tmp, s);
     else
       refine_parallel(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, sn, rs, ps, mvs, max_degree(lv), th, tmp,
-                      lv.tot_valid ? lv.tot.p : nullptr, s);
+                      lv.tot_valid ? lv.tot.p : nullptr, sub_of(lv), s);
     if (multi() && p + 1 < P) {
       // graphs of the next level's vertices: the graph of any member
       Level& up = *L[p + 1];
@@ -828,6 +861,7 @@ hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const ui
       lv.lab_bound = n ? std::max<uint64_t>(lv.n, uint64_t(e.dmax(lv.f.p, n)) + 1) : 0;
     } else if ((which == HL_MAP_S_CUR || which == HL_MAP_S_PRE) && n) {
       lv.s_bound = std::max<uint64_t>(lv.s_bound, uint64_t(e.dmax(e.map_of(lv, which).p, n)) + 1);
+      if (which == HL_MAP_S_CUR) lv.sub_valid = false;
     }
     hl::stream_sync(e.s);
   });
@@ -866,6 +900,7 @@ hl_status hl_restore_level(hl_graph* g, int32_t level, uint32_t n, uint64_t m, c
     uint32_t mx[4] = {0, 0, 0, 0};
     if (n) e.dmax4(arrs, n, mx);
     lv.tot_valid = false;
+    lv.sub_valid = false;
     lv.lab_bound = n ? std::max<uint64_t>(n, uint64_t(mx[0]) + 1) : 0;
     lv.s_bound = n ? std::max<uint64_t>(n, uint64_t(std::max(mx[2], mx[3])) + 1) : 0;
   });

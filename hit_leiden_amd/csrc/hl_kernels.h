@@ -153,16 +153,28 @@ struct RefineScratch {
   DevBuf<u32> rsorted;
   DevBuf<u32> sel, sl, vals;
 };
+// Subcommunity sizes and degrees a caller keeps current across batches
+// (throughput mode): size[S] = |{v : s(v) = S}|, deg[S] = sum of k over s = S.
+// Null pointers: recomputed where needed.
+struct SubRef {
+  u32* size = nullptr;
+  double* deg = nullptr;
+};
+
 // connected-component split of subcommunities touched by K (hit_leiden.rs:1276-1331).
 // s_bound: every label of s is below it.  Returns max(s) + 1 after the split and
 // sets *identity when s is the identity map after it (hit_leiden.rs:1333-1339).
-u64 cc_split(const Csr& g, u32* s, const u8* K, u8* R, u64 s_bound, bool* identity, RefineScratch& rs, Temp& tmp,
-             hipStream_t st);
+u64 cc_split(const Csr& g, u32* s, const u8* K, u8* R, u64 s_bound, bool* identity, const SubRef& sub,
+             RefineScratch& rs, Temp& tmp, hipStream_t st);
+// size[s(v)] += 1 for the vertices [n0, n1) a level grew by (their k is 0)
+void sub_add_vertices(const u32* s, u32 n0, u32 n1, u32* size, hipStream_t st);
+// from scratch: size and deg over every vertex (arrays zero-filled first, cap entries)
+void sub_recompute(const u32* s, const double* k, u32 n, u32* size, double* deg, u64 cap, hipStream_t st);
 // throughput refinement (+connected-targets rule) over R
 struct MoveScratch;
 void refine_parallel(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space, u64 sn,
                      RefineScratch& rs, ProposeScratch& ps, MoveScratch& mvs, u32 maxdeg, const Thresholds& th,
-                     Temp& tmp, const double* dpar_in, hipStream_t st);
+                     Temp& tmp, const double* dpar_in, const SubRef& sub, hipStream_t st);
 // deterministic refinement (hit_leiden.rs:1359-1413)
 void refine_det(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space, u64 sn,
                 RefineScratch& rs, DevBuf<u32>& ks, DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,

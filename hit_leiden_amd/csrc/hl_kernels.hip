@@ -869,16 +869,26 @@ __global__ void k_newids(const u32* roots_sorted, u64 c, u32 base, u32* newid) {
   for (u64 r = gtid(); r < c; r += gstride()) newid[roots_sorted[r]] = base + u32(r);
 }
 __global__ void k_split_assign(const u32* vcc, u64 m, const u32* parent, const u64* best, const u32* newid, u32* s,
-                               u8* R) {
+                               u8* R, const double* k, SubRef sub) {
   for (u64 i = gtid(); i < m; i += gstride()) {
     u32 v = vcc[i];
     u32 r = parent[v];
     u32 sv = s[v];
     if (u32(best[sv] & 0xffffffffu) != r) {
-      s[v] = newid[r];
+      const u32 to = newid[r];
+      s[v] = to;
       R[v] = 1;
+      if (sub.size) {
+        atomicSub(&sub.size[sv], 1u);
+        atomicAdd(&sub.size[to], 1u);
+        atomicAdd(&sub.deg[sv], -k[v]);
+        atomicAdd(&sub.deg[to], k[v]);
+      }
     }
   }
+}
+__global__ void k_sub_add(const u32* s, u32 n0, u32 n1, u32* size) {
+  for (u64 v = n0 + gtid(); v < n1; v += gstride()) atomicAdd(&size[s[v]], 1u);
 }
 
 // ------------------------------------------------------------------ refinement (throughput)
@@ -1729,8 +1739,21 @@ int det_move(const Csr& g, u32* f, const u32* sm, double* tot, u64 tot_cap, u64*
   return int(st);
 }
 
-u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* identity, RefineScratch& rs,
-             Temp& tmp, hipStream_t s) {
+void sub_add_vertices(const u32* sm, u32 n0, u32 n1, u32* size, hipStream_t s) {
+  if (n1 > n0) hipLaunchKernelGGL(k_sub_add, GRID(n1 - n0), sm, n0, n1, size);
+  HL_LAUNCH_CHECK();
+}
+
+void sub_recompute(const u32* sm, const double* k, u32 n, u32* size, double* deg, u64 cap, hipStream_t s) {
+  fill_u32(size, 0, cap, s);
+  fill_f64(deg, 0.0, cap, s);
+  if (n) hipLaunchKernelGGL(k_sizes, GRID(n), sm, n, size);
+  HL_LAUNCH_CHECK();
+  label_totals_atomic(sm, k, n, deg, s);
+}
+
+u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* identity, const SubRef& sub,
+             RefineScratch& rs, Temp& tmp, hipStream_t s) {
   const u32 n = g.n;
   *identity = true;
   if (n == 0) return 1;
@@ -1787,7 +1810,7 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
     sort_pairs_u64(rs.rkeys.p, rs.rkeys_s.p, vals.p, rs.roots_s.p, c, 64, tmp, s);
     hipLaunchKernelGGL(k_newids, GRID(c), rs.roots_s.p, c, next_id, rs.newid.p);
     HL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_split_assign, GRID(m), rs.vcc.p, m, rs.parent.p, rs.best64.p, rs.newid.p, sm, R);
+    hipLaunchKernelGGL(k_split_assign, GRID(m), rs.vcc.p, m, rs.parent.p, rs.best64.p, rs.newid.p, sm, R, g.k, sub);
     HL_LAUNCH_CHECK();
     *identity = false;  // split ids exceed every vertex id of an identity map
   }
@@ -1796,18 +1819,21 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
 
 void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space, u64 sn,
                      RefineScratch& rs, ProposeScratch& ps, MoveScratch& mvs, u32 maxdeg, const Thresholds& th,
-                     Temp& tmp, const double* dpar_in, hipStream_t s) {
+                     Temp& tmp, const double* dpar_in, const SubRef& sub, hipStream_t s) {
   const u32 n = g.n;
   if (n == 0) return;
-  rs.sizes.reserve(sn, s);
-  rs.dS.reserve(sn, s);
-  fill_u32(rs.sizes.p, 0, sn, s);
-  fill_f64(rs.dS.p, 0.0, sn, s);
-  hipLaunchKernelGGL(k_sizes, GRID(n), sm, n, rs.sizes.p);
-  HL_LAUNCH_CHECK();
-  label_totals_atomic(sm, g.k, n, rs.dS.p, s);
+  // subcommunity sizes and degrees (hit_leiden.rs:1341-1357): kept by the caller, or recomputed
+  u32* sizes = sub.size;
+  double* dS = sub.deg;
+  if (!sizes) {
+    rs.sizes.reserve(sn, s);
+    rs.dS.reserve(sn, s);
+    sub_recompute(sm, g.k, n, rs.sizes.p, rs.dS.p, sn, s);
+    sizes = rs.sizes.p;
+    dS = rs.dS.p;
+  }
   rs.single.reserve(n, s);
-  hipLaunchKernelGGL(k_single, GRID(n), R, n, sm, rs.sizes.p, rs.single.p);
+  hipLaunchKernelGGL(k_single, GRID(n), R, n, sm, sizes, rs.single.p);
   HL_LAUNCH_CHECK();
   DevBuf<u32>& sl = rs.sl;
   sl.reserve(n, s);
@@ -1835,13 +1861,13 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
     label_totals_atomic(f, g.k, n, rs.dpar.p, s);
     dpar = rs.dpar.p;
   }
-  hipLaunchKernelGGL(k_blocked, GRID(sn), sn, rs.is_cand.p, rs.dS.p, rs.wS.p, rs.parentIdx.p, f, dpar,
+  hipLaunchKernelGGL(k_blocked, GRID(sn), sn, rs.is_cand.p, dS, rs.wS.p, rs.parentIdx.p, f, dpar,
                      g.twice_m, g.gid, gamma, rs.blocked.p);
   HL_LAUNCH_CHECK();
   // proposals against the snapshot, then the connected-targets rule
   (void)ps;
   (void)tmp;
-  refine_proposals(g, f, sm, rs.dS.p, rs.blocked.p, gamma, sl.p, ns, sn, mvs, th, maxdeg, s);
+  refine_proposals(g, f, sm, dS, rs.blocked.p, gamma, sl.p, ns, sn, mvs, th, maxdeg, sub, s);
 }
 
 void refine_det(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space, u64 sn,

@@ -134,8 +134,9 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
 
 // Snapshot proposals of the singleton vertices `singles` plus the
 // connected-targets rule; updates s in place.
+// sub (may be empty): maintained subcommunity sizes/degrees to update with the moves
 void refine_proposals(const Csr& g, const u32* f, u32* s, const double* dS, const u8* blocked, double gamma,
                       const u32* singles, u64 ns, u64 sspace, MoveScratch& ms, const Thresholds& th, u32 maxdeg,
-                      hipStream_t st);
+                      const SubRef& sub, hipStream_t st);
 
 }  // namespace hl

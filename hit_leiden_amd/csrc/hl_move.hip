@@ -1016,11 +1016,18 @@ __global__ void k_ref_targets(CandOut cand, u8* target) {
   const u32 c = *cand.cnt;
   for (u64 i = gtid(); i < c; i += gstride()) target[cand.to[i]] = 1;
 }
-__global__ void k_ref_apply(CandOut cand, const u8* target, u32* s) {
+__global__ void k_ref_apply(CandOut cand, const u8* target, u32* s, const double* k, SubRef sub) {
   const u32 c = *cand.cnt;
   for (u64 i = gtid(); i < c; i += gstride()) {
-    u32 v = cand.node[i];
-    if (!target[s[v]]) s[v] = cand.to[i];  // a target singleton keeps its place
+    const u32 v = cand.node[i], from = s[v], to = cand.to[i];
+    if (target[from]) continue;  // a target singleton keeps its place
+    s[v] = to;
+    if (sub.size) {
+      atomicSub(&sub.size[from], 1u);
+      atomicAdd(&sub.size[to], 1u);
+      atomicAdd(&sub.deg[from], -k[v]);
+      atomicAdd(&sub.deg[to], k[v]);
+    }
   }
 }
 __global__ void k_ref_clear_targets(CandOut cand, u8* target) {
@@ -1279,7 +1286,7 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
 
 void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, const u8* blocked, double gamma,
                       const u32* singles, u64 ns, u64 sspace, MoveScratch& ms, const Thresholds& th, u32 maxdeg,
-                      hipStream_t s) {
+                      const SubRef& sub, hipStream_t s) {
   ms.ensure(g.n, 1, s);
   ms.target.reserve(sspace, s);
   if (ms.target_cap != ms.target.cap) {
@@ -1296,7 +1303,7 @@ void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, con
   CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
   HL_PROF(P_REFINE, 0, s);
   hipLaunchKernelGGL(k_ref_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);
-  hipLaunchKernelGGL(k_ref_apply, fixed_grid(ns), 256, 0, s, cand, ms.target.p, sm);
+  hipLaunchKernelGGL(k_ref_apply, fixed_grid(ns), 256, 0, s, cand, ms.target.p, sm, g.k, sub);
   hipLaunchKernelGGL(k_ref_clear_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, nullptr, ccnt, ms.counters.p + 4);
