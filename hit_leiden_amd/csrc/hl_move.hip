@@ -462,9 +462,18 @@ __global__ __launch_bounds__(1024) void k_prop2_lds(P pol, Csr g, const u32* lis
 // one workgroup per vertex (hubs); each resident workgroup owns an HBM table
 // of tcap slots (cleared per vertex: coalesced, cheaper than tracking the
 // used slots, whose clears and scans are random accesses)
+// An LDS table in front of the HBM one absorbs the heavy keys: a hub's
+// neighbours often share a few big communities, and device atomics on one
+// slot serialise.  Keys that find no LDS slot within kCacheProbe probes go to
+// the HBM table; the LDS entries are flushed into it after the row.
+constexpr u32 kCacheSlots = 4096;
+constexpr u32 kCacheProbe = 8;
+
 template <class P>
 __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* TK,
                                                      double* TV, u32 tcap, CandOut out, unsigned long long* cst) {
+  __shared__ u32 LK[kCacheSlots];
+  __shared__ double LV[kCacheSlots];
   const u32 cnt = *cnt_dev;
   unsigned long long ncand = 0;
   u32* K = TK + u64(blockIdx.x) * tcap;
@@ -478,6 +487,10 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
     for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
       K[q] = kNone;
       V[q] = 0.0;
+    }
+    for (u32 q = threadIdx.x; q < kCacheSlots; q += blockDim.x) {
+      LK[q] = kNone;
+      LV[q] = 0.0;
     }
     __threadfence();
     __syncthreads();
@@ -499,10 +512,32 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (!ok[r]) continue;
-        const u32 h = tab_insert(K, cap, kk[r]);
-        atomicAdd(&V[h], ww[r]);
-        if (mm[r] == 2) atomicAdd(&V[h], ww[r]);
+        u32 h = hash32(kk[r]) & (kCacheSlots - 1);
+        bool cached = false;
+        for (u32 t = 0; t < kCacheProbe; ++t) {
+          const u32 old = atomicCAS(&LK[h], kNone, kk[r]);
+          if (old == kNone || old == kk[r]) {
+            cached = true;
+            break;
+          }
+          h = (h + 1) & (kCacheSlots - 1);
+        }
+        if (cached) {
+          atomicAdd(&LV[h], ww[r]);
+          if (mm[r] == 2) atomicAdd(&LV[h], ww[r]);
+        } else {
+          const u32 hh = tab_insert(K, cap, kk[r]);
+          atomicAdd(&V[hh], ww[r]);
+          if (mm[r] == 2) atomicAdd(&V[hh], ww[r]);
+        }
       }
+    }
+    __syncthreads();
+    for (u32 q = threadIdx.x; q < kCacheSlots; q += blockDim.x) {
+      const u32 key = LK[q];
+      if (key == kNone) continue;
+      const u32 hh = tab_insert(K, cap, key);
+      atomicAdd(&V[hh], LV[q]);
     }
     __threadfence();
     __syncthreads();
