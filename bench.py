@@ -310,7 +310,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              f"(the path runs on a single MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+              f"(the path runs on a single MI355X; sharding over several GPUs is not built)", file=sys.stderr)
         sys.exit(2)
 
     if args.config == "batched":

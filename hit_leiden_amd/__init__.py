@@ -145,6 +145,7 @@ def lib():
             "hl_serve": ([vp, C.POINTER(_Request), C.c_uint64, C.c_uint64, C.c_int32, vp, vp, _EMIT, vp,
                           C.POINTER(ServeStats)], C.c_int),
             "hl_graph_count": ([vp, u32p], C.c_int),
+            "hl_compare_partitions": ([vp, vp, vp, C.c_uint64, C.c_uint32, C.c_uint32, vp], C.c_int),
             "hl_restore_level": ([vp, C.c_int32, C.c_uint32, C.c_uint64, vp, vp, vp, vp, vp, vp, vp], C.c_int),
             "hl_modularity_graphs": ([vp, vp, C.c_uint64], C.c_int),
         }
@@ -262,6 +263,24 @@ class HitLeiden:
         self.last_stats = s
         return s
 
+    def run_static(self, graph: GraphInput, max_passes=10):
+        """Static clustering with the incremental kernels: load, then empty-delta
+        passes (each re-evaluates every vertex, hit_leiden.rs:1054-1057) until the
+        hierarchy stops changing or max_passes.  This is not the reference's
+        multilevel_leiden (hit_leiden.rs:30-118), whose sequential shuffled order
+        it does not follow; it returns (passes, modularity of the communities)."""
+        self.load(graph)
+        empty = GraphInput(graph.node_count, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+        prev = self.hierarchy()
+        passes = 0
+        for passes in range(1, max_passes + 1):
+            self.hit_leiden(empty)
+            cur = self.hierarchy()
+            if all(np.array_equal(x, y) for x, y in zip(prev, cur)):
+                break
+            prev = cur
+        return passes, self.modularity(PART_COMMUNITY)
+
     def load_graph(self, graph: GraphInput):
         self._check(lib().hl_load_graph(self._h, graph.node_count, len(graph), _ptr(graph.src), _ptr(graph.dst),
                                         _ptr(graph.weights)))
@@ -349,6 +368,22 @@ class HitLeiden:
         self._check(lib().hl_serve(self._h, reqs, len(requests), int(max_pass_records), int(bool(coalesce)),
                                    _ptr(done), _ptr(pass_of), cb, None, C.byref(st)))
         return done[:len(requests)], pass_of[:len(requests)], st
+
+    def compare_partitions(self, a, b, a_largest=None, b_largest=None):
+        """contingency_metrics (hit_leiden_incremental.rs:577-666) on the device:
+        (nmi, purity a->b, purity b->a, Jaccard of the largest communities; the
+        largest are found here unless given)."""
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        if len(a) != len(b):
+            raise InvalidInput("labelings differ in length")
+        if a_largest is None:
+            a_largest = int(np.bincount(a).argmax()) if len(a) else 0
+        if b_largest is None:
+            b_largest = int(np.bincount(b).argmax()) if len(b) else 0
+        out = np.zeros(4, dtype=np.float64)
+        self._check(lib().hl_compare_partitions(self._h, _ptr(a), _ptr(b), len(a), a_largest, b_largest, _ptr(out)))
+        return tuple(float(x) for x in out)
 
     @property
     def graph_count(self):

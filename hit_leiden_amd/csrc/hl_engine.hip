@@ -1175,6 +1175,24 @@ hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t 
   });
 }
 
+hl_status hl_compare_partitions(const hl_graph* g, const uint32_t* a, const uint32_t* b, uint64_t n,
+                                uint32_t a_largest, uint32_t b_largest, double* out) {
+  if (!g || !out || (n && (!a || !b))) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = const_cast<Engine&>(g->e);
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    hl::DevBuf<uint32_t> da, db;
+    da.reserve(n + 1, e.s);
+    db.reserve(n + 1, e.s);
+    if (n) {
+      HL_CHECK(hipMemcpyAsync(da.p, a, n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));
+      HL_CHECK(hipMemcpyAsync(db.p, b, n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));
+    }
+    hl::compare_partitions(da.p, db.p, n, a_largest, b_largest, out, e.tmp, e.s);
+    hl::stream_sync(e.s);
+  });
+}
+
 int32_t hl_profile_slots(void) { return hl::P_COUNT; }
 
 hl_status hl_profile_enable(hl_graph* g, int32_t on) {

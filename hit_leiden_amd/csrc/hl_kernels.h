@@ -207,6 +207,11 @@ void gather_u32(u32* cur, const u32* map, u32 n, hipStream_t s);
 void canonicalize(u32* lab, u32 n, DevBuf<u32>& first, DevBuf<u8>& flag, DevBuf<u64>& pos, Temp& tmp,
                   hipStream_t s);
 
+// ---- partition comparison (contingency_metrics, hit_leiden_incremental.rs:577-666) ----
+// out = {NMI = MI / sqrt(H_a H_b), purity a->b, purity b->a, Jaccard of communities la and lb}
+void compare_partitions(const u32* a, const u32* b, u64 n, u32 la, u32 lb, double out[4], Temp& tmp,
+                        hipStream_t s);
+
 // ---- several independent graphs in one engine (the batched form) ----
 // gid[v] = graph of level-0 vertex v from the G+1 vertex offsets
 void gid_from_offsets(const u32* offs, u32 G, u32 n, u32* gid, hipStream_t s);

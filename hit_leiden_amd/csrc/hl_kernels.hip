@@ -14,6 +14,7 @@
 #include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "hl_kernels.h"
@@ -1283,6 +1284,52 @@ __global__ void k_relabel(u32* lab, u32 n, const u32* first, const u64* pos) {
   for (u64 v = gtid(); v < n; v += gstride()) lab[v] = u32(pos[first[lab[v]]]);
 }
 
+// ------------------------------------------------------------------ partition comparison
+__global__ void k_pair_keys(const u32* a, const u32* b, u64 n, u64* keys, u32* sa, u32* sb) {
+  for (u64 i = gtid(); i < n; i += gstride()) {
+    keys[i] = (u64(a[i]) << 32) | b[i];
+    atomicAdd(&sa[a[i]], 1u);
+    atomicAdd(&sb[b[i]], 1u);
+  }
+}
+// one thread per (a, b) run of the sorted pairs: mutual-information term,
+// best overlap per label, and the overlap of the two named communities
+__global__ void k_run_terms(const u64* keys, const u32* starts, u64 nruns, u64 n, const u32* sa, const u32* sb,
+                            u32* besta, u32* bestb, u32 la, u32 lb, double* mi, u32* inter) {
+  double local = 0.0;
+  const double nf = double(n);
+  for (u64 r = gtid(); r < nruns; r += gstride()) {
+    const u64 st = starts[r], en = r + 1 < nruns ? starts[r + 1] : n;
+    const u32 c = u32(en - st);
+    const u64 key = keys[st];
+    const u32 x = u32(key >> 32), y = u32(key);
+    const double phs = c / nf, ph = sa[x] / nf, ps = sb[y] / nf;
+    local += phs * log(phs / (ph * ps));
+    atomicMax(&besta[x], c);
+    atomicMax(&bestb[y], c);
+    if (x == la && y == lb) *inter = c;
+  }
+  const double t = block_sum(local);
+  if (threadIdx.x == 0 && t != 0.0) atomicAdd(mi, t);
+}
+// entropy of the label sizes and the sum of the best overlaps
+__global__ void k_label_terms(const u32* size, const u32* best, u64 L, u64 n, double* ent, unsigned long long* pur) {
+  double h = 0.0;
+  unsigned long long b = 0;
+  for (u64 c = gtid(); c < L; c += gstride()) {
+    const u32 x = size[c];
+    if (x) {
+      const double p = double(x) / double(n);
+      h -= p * log(p);
+      b += best[c];
+    }
+  }
+  const double t = block_sum(h);
+  for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o, 64);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(pur, b);
+  if (threadIdx.x == 0 && t != 0.0) atomicAdd(ent, t);
+}
+
 // ------------------------------------------------------------------ several graphs per engine
 // Vertex ranges of the graphs are contiguous at level 0; gid maps every vertex
 // of a level to its graph (G = no graph: an unused id of the level).
@@ -2119,6 +2166,74 @@ void modularity_graphs(const Csr& g, const u32* lab, u32 G, double* out, DevBuf<
     const double t = tm[i];
     out[i] = t == 0.0 ? 0.0 : (h[i] - h[G + 1 + i] / t) / t;
   }
+}
+
+
+// contingency_metrics (hit_leiden_incremental.rs:577-666) on the device
+void compare_partitions(const u32* a, const u32* b, u64 n, u32 la, u32 lb, double out[4], Temp& tmp,
+                        hipStream_t s) {
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  if (n == 0) return;
+  DevBuf<u32> mx;
+  mx.reserve(2, s);
+  fill_u32(mx.p, 0, 2, s);
+  max_u32(a, n, mx.p, s);
+  max_u32(b, n, mx.p + 1, s);
+  u32 hm[2];
+  d2h_async(hm, mx.p, sizeof(hm), s);
+  stream_sync(s);
+  const u64 LA = u64(hm[0]) + 1, LB = u64(hm[1]) + 1;
+  DevBuf<u32> sa, sb, besta, bestb, idx, idx_s, starts, misc;
+  DevBuf<u64> keys, keys_s, cnt;
+  DevBuf<u8> heads;
+  DevBuf<double> acc;
+  DevBuf<unsigned long long> pur;
+  sa.reserve(LA, s);
+  sb.reserve(LB, s);
+  besta.reserve(LA, s);
+  bestb.reserve(LB, s);
+  fill_u32(sa.p, 0, LA, s);
+  fill_u32(sb.p, 0, LB, s);
+  fill_u32(besta.p, 0, LA, s);
+  fill_u32(bestb.p, 0, LB, s);
+  keys.reserve(n, s);
+  keys_s.reserve(n, s);
+  idx.reserve(n, s);
+  idx_s.reserve(n, s);
+  heads.reserve(n, s);
+  starts.reserve(n, s);
+  hipLaunchKernelGGL(k_pair_keys, GRID(n), a, b, n, keys.p, sa.p, sb.p);
+  HL_LAUNCH_CHECK();
+  iota_u32(idx.p, 0, n, s);
+  sort_pairs_u64(keys.p, keys_s.p, idx.p, idx_s.p, n, 64, tmp, s);
+  hipLaunchKernelGGL(k_heads_u64, GRID(n), keys_s.p, n, 0, heads.p);
+  HL_LAUNCH_CHECK();
+  const u64 nruns = select_flagged(heads.p, n, starts.p, tmp, cnt, s);
+  acc.reserve(3, s);
+  fill_f64(acc.p, 0.0, 3, s);
+  misc.reserve(1, s);
+  fill_u32(misc.p, 0, 1, s);
+  pur.reserve(2, s);
+  HL_CHECK(hipMemsetAsync(pur.p, 0, 2 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_run_terms, GRID(nruns), keys_s.p, starts.p, nruns, n, sa.p, sb.p, besta.p, bestb.p, la, lb,
+                     acc.p, misc.p);
+  hipLaunchKernelGGL(k_label_terms, GRID(LA), sa.p, besta.p, LA, n, acc.p + 1, pur.p);
+  hipLaunchKernelGGL(k_label_terms, GRID(LB), sb.p, bestb.p, LB, n, acc.p + 2, pur.p + 1);
+  HL_LAUNCH_CHECK();
+  double h[3];
+  unsigned long long p[2];
+  u32 inter = 0, szl[2] = {0, 0};
+  d2h_async(h, acc.p, sizeof(h), s);
+  d2h_async(p, pur.p, sizeof(p), s);
+  d2h_async(&inter, misc.p, sizeof(u32), s);
+  if (la < LA) d2h_async(&szl[0], sa.p + la, sizeof(u32), s);
+  if (lb < LB) d2h_async(&szl[1], sb.p + lb, sizeof(u32), s);
+  stream_sync(s);
+  out[0] = (h[1] > 0.0 && h[2] > 0.0) ? h[0] / std::sqrt(h[1] * h[2]) : 0.0;
+  out[1] = double(p[0]) / double(n);
+  out[2] = double(p[1]) / double(n);
+  const u64 uni = u64(szl[0]) + szl[1] - inter;
+  out[3] = uni ? double(inter) / double(uni) : 0.0;
 }
 
 }  // namespace hl

@@ -1221,16 +1221,19 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
                          ms.minAfrom.p, ms.mark.p, ep, nxt, next_cnt, status, ms.dstat.p, cur_cnt, bcounts, fm);
       HL_LAUNCH_CHECK();
     }
-    {
-      HL_PROF(P_MARK_MOVED, 0, s);
-      hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fm, ms.mark.p, ep, nxt, next_cnt);
-      HL_LAUNCH_CHECK();
-    }
     u32 h[16];
     d2h_async(h, ms.counters.p, 16 * sizeof(u32), s);
     stream_sync(s);
     u32 accepted = h[15];
     u32 next_host = h[2 * (ci ^ 1)];
+    if (h[14] != 1 && h[13] != 0) {
+      // neighbour lists too long for the decoupling workgroup: marks over the grid
+      HL_PROF(P_MARK_MOVED, 0, s);
+      hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fm, ms.mark.p, ep, nxt, next_cnt);
+      HL_LAUNCH_CHECK();
+      d2h_async(&next_host, next_cnt, sizeof(u32), s);
+      stream_sync(s);
+    }
     if (h[14] == 1) {
       // many candidates: sort by node, then stably by gain; priority rounds over the grid
       const u32 c = h[1];

@@ -163,6 +163,13 @@ hl_status hl_read_hierarchy(const hl_graph* g, uint32_t* out, uint64_t cap, int3
 hl_status hl_modularity(const hl_graph* g, int32_t which, double* q);
 hl_status hl_modularity_labels(const hl_graph* g, const uint32_t* labels, uint64_t n, double* q);
 
+/* contingency_metrics (hit_leiden_incremental.rs:577-666, the benchmark's quality
+   check) on the device for two labelings a, b of n vertices: out[0] NMI =
+   MI / sqrt(H(a) H(b)), out[1] purity a->b, out[2] purity b->a, out[3] the
+   Jaccard index of community a_largest of a and b_largest of b. */
+hl_status hl_compare_partitions(const hl_graph* g, const uint32_t* a, const uint32_t* b, uint64_t n,
+                                uint32_t a_largest, uint32_t b_largest, double* out);
+
 /* ---- Batched form: several independent graphs in one engine ----
    No counterpart in the reference, whose PartitionState holds one graph
    (state.rs:19-64); a caller with many graphs would hold one state per graph

@@ -354,3 +354,60 @@ def test_autotune_picks_a_candidate():
     batches = [random_batch(n, u, v, None, 200, 6 + i) for i in range(2)]
     best = autotune.tune(hl.GraphInput(n, u, v), batches, levels=2)
     assert best[0] in autotune.SMALL and best[1] in autotune.WAVE
+
+
+# ---------------------------------------------------------------- quality harness
+
+def contingency_numpy(a, b, la, lb):
+    """contingency_metrics (hit_leiden_incremental.rs:577-666) restated with numpy."""
+    n = len(a)
+    pairs, cnt = np.unique(np.stack([a, b], 1), axis=0, return_counts=True)
+    sa = np.bincount(a)
+    sb = np.bincount(b)
+    pa, pb = sa[sa > 0] / n, sb[sb > 0] / n
+    ha, hb = -(pa * np.log(pa)).sum(), -(pb * np.log(pb)).sum()
+    phs = cnt / n
+    mi = (phs * np.log(phs / (sa[pairs[:, 0]] / n * sb[pairs[:, 1]] / n))).sum()
+    nmi = mi / np.sqrt(ha * hb) if ha > 0 and hb > 0 else 0.0
+    besta = np.zeros(len(sa), np.int64)
+    bestb = np.zeros(len(sb), np.int64)
+    np.maximum.at(besta, pairs[:, 0], cnt)
+    np.maximum.at(bestb, pairs[:, 1], cnt)
+    inter = cnt[(pairs[:, 0] == la) & (pairs[:, 1] == lb)].sum()
+    uni = sa[la] + sb[lb] - inter
+    return nmi, besta.sum() / n, bestb.sum() / n, inter / uni if uni else 0.0
+
+
+def test_compare_partitions_matches_numpy_and_sklearn():
+    from sklearn.metrics import normalized_mutual_info_score
+    g = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT))
+    rng = np.random.default_rng(3)
+    n = 200000
+    a = rng.integers(0, 500, n).astype(np.uint32)
+    b = ((a // 3) + (rng.random(n) < 0.2) * rng.integers(0, 50, n)).astype(np.uint32)
+    got = g.compare_partitions(a, b)
+    la, lb = int(np.bincount(a).argmax()), int(np.bincount(b).argmax())
+    want = contingency_numpy(a, b, la, lb)
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12)
+    assert abs(got[0] - normalized_mutual_info_score(a, b, average_method="geometric")) < 1e-9
+    # identical labelings: NMI 1, purity 1, Jaccard 1; one community against singletons: NMI 0
+    assert g.compare_partitions(a, a) == pytest.approx((1.0, 1.0, 1.0, 1.0))
+    z = np.zeros(1000, np.uint32)
+    assert g.compare_partitions(z, np.arange(1000, dtype=np.uint32))[0] == 0.0
+    g.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_static_run_reaches_a_fixpoint(mode):
+    w = gen.small_config()
+    g = hl.HitLeiden(hl.RunConfig(mode=mode, levels=3))
+    g.load(hl.GraphInput(w.n, w.u, w.v))
+    q_load = g.modularity()
+    passes, q = g.run_static(hl.GraphInput(w.n, w.u, w.v), max_passes=25)
+    say(f"static mode={int(mode)} passes={passes} Q {q_load:.4f} -> {q:.4f}")
+    assert passes < 25 and q >= q_load - 1e-3
+    h = g.hierarchy()
+    g.hit_leiden(hl.GraphInput(w.n, np.zeros(0, np.uint32), np.zeros(0, np.uint32)))
+    for a, b in zip(h, g.hierarchy()):
+        np.testing.assert_array_equal(a, b)
+    g.close()
