@@ -32,7 +32,7 @@ METRIC = "incremental edges/s + p99 update latency vs plain GPU path; HBM GB/s"
 # the proposal kernels), with the kernel name rocprofv3 reports; the dominant
 # kernel of the roofline is the one of these with the most time
 SLOT_KERNEL = {
-    "propose_small": "k_prop2_small", "propose_wave": "k_prop2_wave", "propose_lds": "k_prop2_lds",
+    "propose_small": "k_prop2_first", "propose_wave": "k_prop2_wave", "propose_lds": "k_prop2_lds",
     "propose_big": "k_prop2_big",
     "decouple": "k_dec_small", "apply_moves": "k_apply4", "mark_moved": "k_marks4",
     "row_merge": "k_row_merge", "resolve_runs": "k_resolve_runs", "row_degree": "k_row_degree",

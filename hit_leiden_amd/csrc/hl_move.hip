@@ -1,7 +1,8 @@
 // hl_move.hip -- throughput-mode local moving with device-side frontiers.
 //
 // One round (inc_movement_parallel, throughput.rs:80-202):
-//   k_bucket2      split the active list by degree (wave-aggregated appends)
+//   k_prop2_first  split the active list by degree (wave-aggregated appends) and
+//                  evaluate the low-degree vertices in the same pass
 //   k_prop2_*      per-vertex argmax over neighbour communities (parallel_frontier.rs:155-236);
 //                  a positive proposal is appended to the candidate list
 //   k_dec_small    one workgroup: sort candidates by (gain desc, node asc),
@@ -198,40 +199,29 @@ __device__ void block_add_u64(unsigned long long* dst, unsigned long long x) {
   }
 }
 
-// ------------------------------------------------------------------ buckets
-// Split the active list by degree: lane (<= small_deg), wavefront (<= wave_deg),
-// workgroup with an LDS table (<= kLdsDeg), workgroup with an HBM table
-// (<= kBigDeg), giant.
-__global__ void k_bucket2(const u32* verts, const u32* cnt_dev, const u32* row_len, u32 small_deg, u32 wave_deg,
-                          u32* lists, u64 cap, u32* counts, unsigned long long* st) {
-  const u32 cnt = *cnt_dev;
-  unsigned long long local[kBuckets] = {}, lcnt[kBuckets] = {};
-  for (u64 base = u64(blockIdx.x) * blockDim.x; base < cnt; base += gstride()) {
-    u64 i = base + threadIdx.x;
-    u32 b = kBuckets;
-    u32 v = 0;
-    if (i < cnt) {
-      v = verts[i];
-      const u32 d = row_len[v];
-      b = d <= small_deg ? 0 : d <= wave_deg ? 1 : d <= kLdsDeg ? 2 : d <= kBigDeg ? 3 : 4;
+// K counters at once: dst[idx[i]] += vals[i], one device atomic per counter and
+// workgroup; every thread of the block must call it (blockDim <= 1024).
+template <int K>
+__device__ void block_add_multi(unsigned long long* dst, const u32 (&idx)[K], unsigned long long (&vals)[K]) {
+  __shared__ unsigned long long part[16][K];
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
-      for (u32 q = 0; q < kBuckets; ++q)
-        if (q == b) {
-          local[q] += d;
-          lcnt[q] += 1;
-        }
-    }
-    for (u32 q = 0; q < kBuckets; ++q) {
-      u32 p = wave_append(b == q, &counts[q]);
-      if (b == q) lists[q * cap + p] = v;
-    }
+  for (int i = 0; i < K; ++i) {
+    unsigned long long x = vals[i];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    vals[i] = x;
   }
-  for (u32 q = 0; q < kBuckets; ++q) {
-    block_add_u64(&st[q], lcnt[q]);
-    block_add_u64(&st[kBuckets + q], local[q]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < K; ++i) part[wv][i] = vals[i];
+  __syncthreads();
+  if (threadIdx.x < u32(K)) {
+    unsigned long long t = 0;
+    for (u32 w = 0; w < nw; ++w) t += part[w][threadIdx.x];
+    if (t) atomicAdd(&dst[idx[threadIdx.x]], t);
   }
 }
-
 
 // ------------------------------------------------------------------ proposals
 template <class P>
@@ -246,23 +236,41 @@ __device__ __forceinline__ void finish_choice(const P& pol, u32 v, u32 cur, doub
   }
 }
 
-// one thread per vertex (deg <= SMALL_MAX2), row-order sums in a private LDS slice
+// The round's first kernel: one thread per active vertex; vertices up to
+// small_deg are evaluated here (row-order sums in a private LDS slice), the
+// others are appended to the wavefront / workgroup bucket lists (lane,
+// wavefront <= wave_deg, LDS workgroup <= kLdsDeg, HBM workgroup <= kBigDeg,
+// giant) for the kernels that follow.
 template <class P>
-__global__ __launch_bounds__(256) void k_prop2_small(P pol, Csr g, const u32* list, const u32* cnt_dev,
-                                                      CandOut out, unsigned long long* cst) {
+__global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* list, const u32* cnt_dev,
+                                                      u32 small_deg, u32 wave_deg, u32* lists, u64 cap, u32* counts,
+                                                      CandOut out, unsigned long long* st) {
   __shared__ u32 sk[SMALL_MAX2 * 256];
   __shared__ double sv[SMALL_MAX2 * 256];
   const u32 cnt = *cnt_dev;
   const u32 t = threadIdx.x;
-  unsigned long long ncand = 0;
+  unsigned long long vals[2 * kBuckets + 1] = {};  // per bucket: vertices, row entries; small-bucket candidates
   for (u64 base = u64(blockIdx.x) * 256; base < cnt; base += gstride()) {
     const u64 id = base + t;
     double bg = 0.0;
-    u32 best = kNone, v = 0;
+    u32 best = kNone, v = 0, b = kBuckets, deg = 0;
     if (id < cnt) {
       v = list[id];
+      deg = g.row_len[v];
+      b = deg <= small_deg ? 0 : deg <= wave_deg ? 1 : deg <= kLdsDeg ? 2 : deg <= kBigDeg ? 3 : 4;
+#pragma unroll
+      for (u32 q = 0; q < kBuckets; ++q)
+        if (q == b) {
+          vals[q] += 1;
+          vals[kBuckets + q] += deg;
+        }
+    }
+    for (u32 q = 1; q < kBuckets; ++q) {
+      const u32 p = wave_append(b == q, &counts[q]);
+      if (b == q) lists[q * cap + p] = v;
+    }
+    if (b == 0) {
       const u64 off = g.row_off[v];
-      const u32 deg = g.row_len[v];
       const u32 cur = pol.cur_key(v);
       int nk = 0;
       for (u32 j = 0; j < deg; ++j) {
@@ -281,7 +289,7 @@ __global__ __launch_bounds__(256) void k_prop2_small(P pol, Csr g, const u32* li
         sv[q * 256 + t] += w;
         if (mult == 2) sv[q * 256 + t] += w;
       }
-      ncand += nk;
+      vals[2 * kBuckets] += nk;
       double wcur = 0.0;
       for (int q = 0; q < nk; ++q)
         if (sk[q * 256 + t] == cur) wcur = sv[q * 256 + t];
@@ -302,7 +310,11 @@ __global__ __launch_bounds__(256) void k_prop2_small(P pol, Csr g, const u32* li
       out.gain[p] = bg;
     }
   }
-  block_add_u64(cst, ncand);
+  u32 idx[2 * kBuckets + 1];
+#pragma unroll
+  for (u32 q = 0; q < 2 * kBuckets; ++q) idx[q] = q;
+  idx[2 * kBuckets] = 2 * kBuckets;  // the small bucket's candidates: st[2 kBuckets + 0]
+  block_add_multi(st, idx, vals);
 }
 
 // one wavefront per vertex, LDS open-addressing table of up to WAVE_HW2 slots
@@ -1131,14 +1143,12 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
   u32* counts = ms.counters.p + 4;  // [4..4 + kBuckets) bucket counts, zero on entry
   const u32 small = th.small_deg > SMALL_MAX2 ? SMALL_MAX2 : th.small_deg;
   const u32 wave = th.wave_deg > WAVE_HW2 / 2 ? WAVE_HW2 / 2 : th.wave_deg;
-  hipLaunchKernelGGL(k_bucket2, fixed_grid(list_cap), 256, 0, s, list, cnt_dev, g.row_len, small, wave,
-                     ms.blists.p, list_cap, counts, st);
-  HL_LAUNCH_CHECK();
   CandOut out{ms.cnode.p, ms.cto.p, ms.cgain.p, ms.counters.p + 1};
   unsigned long long* cst = st + 2 * kBuckets;
   {
     HL_PROF(P_PROP_SMALL, 0, s);
-    hipLaunchKernelGGL(k_prop2_small<P>, fixed_grid(list_cap), 256, 0, s, pol, g, ms.blists.p, counts, out, cst);
+    hipLaunchKernelGGL(k_prop2_first<P>, fixed_grid(list_cap), 256, 0, s, pol, g, list, cnt_dev, small, wave,
+                       ms.blists.p, list_cap, counts, out, st);
     HL_LAUNCH_CHECK();
   }
   {
