@@ -110,6 +110,7 @@ struct MoveScratch {
   DevBuf<double> cgain;
   DevBuf<u32> minUto, minUfrom, minAto, minAfrom;
   DevBuf<u32> tk;  // workgroup hub tables
+  DevBuf<u32> hovf;  // hubs whose keys overflow the LDS tables
   DevBuf<double> tv;
   DevBuf<u32> k32, k32b, v32, o1, o2, node, from, to, state, acc;
   DevBuf<u64> k64, k64b;

@@ -30,7 +30,8 @@ constexpr u32 kBigDeg = 1u << 18;   // workgroup-per-vertex with an HBM table up
 constexpr u64 kMarksInBlock = 8192;  // neighbour entries marked inside the decoupling workgroup
 constexpr unsigned kMarksBlocks = 1024;
 constexpr u32 kMarkStage = 2048;
-constexpr u64 kBigTableBytes = 3ull << 30;  // hub tables of the workgroup-per-vertex proposals  // 3 resident workgroups per CU at 52 KB of LDS each
+constexpr u64 kBigTableBytes = 3ull << 30;  // HBM hub tables of the overflow proposals
+constexpr u32 kOvfBlocks = 64;              // workgroups (and HBM tables) for the overflow list
 
 __device__ __forceinline__ u64 gtid() { return u64(blockIdx.x) * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ u64 gstride() { return u64(gridDim.x) * blockDim.x; }
@@ -471,8 +472,115 @@ __global__ __launch_bounds__(1024) void k_prop2_lds(P pol, Csr g, const u32* lis
   block_add_u64(cst, ncand);
 }
 
-// one workgroup per vertex (hubs); each resident workgroup owns an HBM table
-// of tcap slots (cleared per vertex: coalesced, cheaper than tracking the
+// Hubs, first try: one workgroup per vertex with an LDS table of S slots.  A
+// hub's neighbours mostly fall in few communities (R-MAT scale 24: a median of
+// about 120 distinct ones per hub, at most a few thousand), so the table rarely
+// fills; a vertex whose keys do not fit (a key finds no slot within kHubProbe
+// probes) is appended to the overflow list for k_prop2_big instead.
+constexpr u32 kHubProbe = 64;
+
+__device__ __forceinline__ u32 lds_try_insert(u32* K, u32 mask, u32 key) {
+  u32 h = hash32(key) & mask;
+  for (u32 t = 0; t < kHubProbe; ++t) {
+    const u32 old = atomicCAS(&K[h], kNone, key);
+    if (old == kNone || old == key) return h;
+    h = (h + 1) & mask;
+  }
+  return kNone;
+}
+__device__ __forceinline__ double lds_find(const u32* K, const double* V, u32 mask, u32 key) {
+  u32 h = hash32(key) & mask;
+  for (u32 t = 0; t < kHubProbe; ++t) {
+    const u32 x = K[h];
+    if (x == key) return V[h];
+    if (x == kNone) return 0.0;
+    h = (h + 1) & mask;
+  }
+  return 0.0;
+}
+
+template <class P, u32 T, u32 S>
+__global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* ovf,
+                                                 u32* ovf_cnt, CandOut out, unsigned long long* cst) {
+  __shared__ u32 K[S];
+  __shared__ double V[S];
+  __shared__ u32 full;
+  const u32 cnt = *cnt_dev;
+  unsigned long long ncand = 0;
+  for (u32 id = blockIdx.x; id < cnt; id += gridDim.x) {
+    const u32 v = list[id];
+    const u64 off = g.row_off[v];
+    const u32 deg = g.row_len[v];
+    const u32 cur = pol.cur_key(v);
+    for (u32 q = threadIdx.x; q < S; q += T) {
+      K[q] = kNone;
+      V[q] = 0.0;
+    }
+    if (threadIdx.x == 0) full = 0;
+    __syncthreads();
+    // four row entries per thread in flight: the gathers are independent
+    for (u32 j0 = threadIdx.x; j0 < deg; j0 += 4 * T) {
+      if (full) break;
+      u32 uu[4], kk[4];
+      double ww[4];
+      int mm[4];
+      bool ok[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const u32 j = j0 + r * T;
+        ok[r] = j < deg;
+        uu[r] = ok[r] ? g.col[off + j] : 0;
+        ww[r] = ok[r] ? g.w[off + j] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ok[r] = ok[r] && pol.accept(v, uu[r], kk[r], mm[r]);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (!ok[r]) continue;
+        const u32 h = lds_try_insert(K, S - 1, kk[r]);
+        if (h == kNone) {
+          full = 1;
+          break;
+        }
+        atomicAdd(&V[h], ww[r]);
+        if (mm[r] == 2) atomicAdd(&V[h], ww[r]);
+      }
+    }
+    __syncthreads();
+    if (full) {  // uniform: nothing writes it between the barrier above and the one below
+      if (threadIdx.x == 0) ovf[atomicAdd(ovf_cnt, 1u)] = v;
+      __syncthreads();
+      continue;
+    }
+    const double wcur = lds_find(K, V, S - 1, cur);
+    const double kv = g.k[v], sc = pol.cur_total(cur), tt = tm_of(g, v);
+    double bg = 0.0;
+    u32 best = kNone;
+    for (u32 q = threadIdx.x; q < S; q += T) {
+      const u32 key = K[q];
+      if (key == kNone) continue;
+      ++ncand;
+      if (key == cur || !pol.ok(key)) continue;
+      const double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+      if (gn > 0.0) better(bg, best, gn, key);
+    }
+    block_argmax(bg, best);
+    if (threadIdx.x == 0) {
+      finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
+      if (bg > 0.0) {
+        const u32 p = atomicAdd(out.cnt, 1u);
+        out.node[p] = v;
+        out.to[p] = best;
+        out.gain[p] = bg;
+      }
+    }
+    __syncthreads();
+  }
+  block_add_u64(cst, ncand);
+}
+
+// Hubs whose keys overflowed the LDS table: one workgroup per vertex; each
+// resident workgroup owns an HBM table of tcap slots (cleared per vertex: coalesced, cheaper than tracking the
 // used slots, whose clears and scans are random accesses)
 // An LDS table in front of the HBM one absorbs the heavy keys: a hub's
 // neighbours often share a few big communities, and device atomics on one
@@ -894,7 +1002,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
     // the next round starts from zero counts without a host-side reset
     *cand.cnt = 0;
     *cur_cnt = 0;
-    for (u32 i = 0; i < kBuckets; ++i) bucket_counts[i] = 0;
+    for (u32 i = 0; i <= kBuckets; ++i) bucket_counts[i] = 0;  // + the hub overflow count
     atomicAdd(&dst[DS_DEC_CAND], (unsigned long long)c);
     atomicAdd(&dst[DS_DEC_ACC], (unsigned long long)nacc);
   }
@@ -956,7 +1064,7 @@ __global__ void k_round_reset(u32* cur_cnt, u32* ccnt, u32* bucket_counts) {
   if (threadIdx.x == 0) {
     if (cur_cnt) *cur_cnt = 0;
     if (ccnt) *ccnt = 0;
-    for (u32 i = 0; i < kBuckets; ++i) bucket_counts[i] = 0;
+    for (u32 i = 0; i <= kBuckets; ++i) bucket_counts[i] = 0;  // + the hub overflow count
   }
 }
 
@@ -991,16 +1099,22 @@ __global__ void k_reset4(u64 c, const u32* from, const u32* to, u32* a, u32* b, 
     y[f] = kNone; y[t] = kNone;
   }
 }
+__device__ __forceinline__ void min_if_lower(u32* a, u32 x) {
+  if (*a > x) atomicMin(a, x);
+}
 __global__ void k_mark4(u64 c, const u32* from, const u32* to, const u32* state, u32* minUto, u32* minUfrom,
                         u32* minAto, u32* minAfrom) {
   for (u64 p = gtid(); p < c; p += gstride()) {
+    // values only fall during the kernel, so a plain read at or below p means
+    // the atomic would change nothing: big communities take one atomic per
+    // cache refresh instead of one per candidate
     u32 st = state[p];
     if (st == kUndec) {
-      atomicMin(&minUto[to[p]], u32(p));
-      atomicMin(&minUfrom[from[p]], u32(p));
+      min_if_lower(&minUto[to[p]], u32(p));
+      min_if_lower(&minUfrom[from[p]], u32(p));
     } else if (st == kAcc) {
-      atomicMin(&minAto[to[p]], u32(p));
-      atomicMin(&minAfrom[from[p]], u32(p));
+      min_if_lower(&minAto[to[p]], u32(p));
+      min_if_lower(&minAfrom[from[p]], u32(p));
     }
   }
 }
@@ -1123,7 +1237,7 @@ void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
 }
 
 void MoveScratch::trim() {
-  for (DevBuf<u32>* b : {&tk, &k32, &k32b, &v32, &o1, &o2, &node, &from, &to, &state, &acc}) b->release();
+  for (DevBuf<u32>* b : {&tk, &hovf, &k32, &k32b, &v32, &o1, &o2, &node, &from, &to, &state, &acc}) b->release();
   tv.release();
   k64.release();
   k64b.release();
@@ -1140,7 +1254,7 @@ u32 MoveScratch::next_epoch(hipStream_t s) {
 template <class P>
 static void propose_round(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u64 list_cap, u32 maxdeg,
                           MoveScratch& ms, const Thresholds& th, unsigned long long* st, hipStream_t s) {
-  u32* counts = ms.counters.p + 4;  // [4..4 + kBuckets) bucket counts, zero on entry
+  u32* counts = ms.counters.p + 4;  // [4..4 + kBuckets] bucket counts and hub overflow count, zero on entry
   const u32 small = th.small_deg > SMALL_MAX2 ? SMALL_MAX2 : th.small_deg;
   const u32 wave = th.wave_deg > WAVE_HW2 / 2 ? WAVE_HW2 / 2 : th.wave_deg;
   CandOut out{ms.cnode.p, ms.cto.p, ms.cgain.p, ms.counters.p + 1};
@@ -1163,18 +1277,27 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
                        ms.blists.p + 2 * list_cap, counts + 2, out, cst + 2);
     HL_LAUNCH_CHECK();
   }
-  // HBM tables: one per resident workgroup, at most kBigTableBytes in all
-  for (u32 b = 3; b < kBuckets; ++b) {
-    const u32 lo = b == 3 ? kLdsDeg : kBigDeg;
-    if (maxdeg <= lo) break;
+  if (maxdeg > kLdsDeg) {
+    // hubs: LDS tables first; the vertices whose keys overflow them go to HBM
+    // tables (one per resident workgroup, at most kBigTableBytes in all)
     HL_PROF(P_PROP_BIG, 0, s);
-    const u32 tcap = big_cap(b == 3 ? std::min(maxdeg, kBigDeg) : maxdeg);
+    u32* ovf_cnt = counts + kBuckets;
+    ms.hovf.reserve(list_cap, s);
+    hipLaunchKernelGGL((k_prop2_hub<P, 512, 4096>), dim3(768), dim3(512), 0, s, pol, g, ms.blists.p + 3 * list_cap,
+                       counts + 3, ms.hovf.p, ovf_cnt, out, cst + 3);
+    HL_LAUNCH_CHECK();
+    if (maxdeg > kBigDeg) {
+      hipLaunchKernelGGL((k_prop2_hub<P, 1024, 8192>), dim3(256), dim3(1024), 0, s, pol, g,
+                         ms.blists.p + 4 * list_cap, counts + 4, ms.hovf.p, ovf_cnt, out, cst + 4);
+      HL_LAUNCH_CHECK();
+    }
+    const u32 tcap = big_cap(maxdeg);
     const u64 per = u64(tcap) * (sizeof(u32) + sizeof(double));
-    const unsigned nb = unsigned(std::max<u64>(1, std::min<u64>(512, kBigTableBytes / per)));
+    const unsigned nb = unsigned(std::max<u64>(1, std::min<u64>(kOvfBlocks, kBigTableBytes / per)));
     ms.tk.reserve(u64(nb) * tcap, s);
     ms.tv.reserve(u64(nb) * tcap, s);
-    hipLaunchKernelGGL(k_prop2_big<P>, dim3(nb), dim3(1024), 0, s, pol, g, ms.blists.p + b * list_cap, counts + b,
-                       ms.tk.p, ms.tv.p, tcap, out, cst + b);
+    hipLaunchKernelGGL(k_prop2_big<P>, dim3(nb), dim3(1024), 0, s, pol, g, ms.hovf.p, ovf_cnt, ms.tk.p, ms.tv.p,
+                       tcap, out, cst + 3);
     HL_LAUNCH_CHECK();
   }
 }
@@ -1183,8 +1306,8 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
                         const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s) {
   const u32 n = m.g.n;
   ms.ensure(n, m.comm_cap, s);
-  // counters: [0], [2] list counts (alternating), [1] candidates, [4..8) bucket
-  // counts, [14] overflow flag, [15] accepted moves; a round leaves all but the
+  // counters: [0], [2] list counts (alternating), [1] candidates, [4..9) bucket
+  // counts, [9] hub overflow count, [13] flat mark count, [14] overflow flag, [15] accepted moves; a round leaves all but the
   // next list's count at zero
   u32* lcnt[2] = {ms.counters.p + 0, ms.counters.p + 2};
   u32* ccnt = ms.counters.p + 1;

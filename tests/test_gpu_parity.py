@@ -309,6 +309,33 @@ def test_hub_rows_use_big_bucket(leaves):
         g.close()
 
 
+@pytest.mark.parametrize("hub_deg,n", [(12000, 30000), (280000, 300000)])
+def test_hubs_over_few_communities(hub_deg, n):
+    """Hubs whose neighbours fall in a few communities (an SBM with 20 blocks):
+    after the first rounds their keys fit the LDS hub tables; in the first
+    rounds (singleton communities) they overflow to the HBM tables.  The second
+    case has a hub above 2^18 neighbours (the giant bucket)."""
+    w = gen.sbm(n, 20, 8.0, 0.2, seed=5)
+    rng = np.random.default_rng(6)
+    hubs = [np.full(hub_deg, h, np.uint32) for h in (0, 1, 2)]
+    nbrs = [rng.choice(np.arange(3, n, dtype=np.uint32), size=hub_deg, replace=False) for _ in hubs]
+    u = np.concatenate([w.u] + hubs)
+    v = np.concatenate([w.v] + nbrs)
+    key = np.unique(np.minimum(u, v).astype(np.int64) * n + np.maximum(u, v))
+    u, v = (key // n).astype(np.uint32), (key % n).astype(np.uint32)
+    say(f"hubs deg={hub_deg} n={n} m={len(u)}")
+    g, o = pair(hl.RunMode.THROUGHPUT, 2)
+    g.load(hl.GraphInput(n, u, v))
+    o.load(n, u, v)
+    assert_same_state(g, o, 2, "hubs load")
+    for step in range(2):
+        bu, bv, ba = random_batch(n, u, v, None, 500, 90 + step)
+        g.hit_leiden(hl.GraphInput(n, bu, bv, ba))
+        o.apply(bu, bv, ba)
+        assert_same_state(g, o, 2, f"hubs batch {step}")
+    g.close()
+
+
 # ---------------------------------------------------------------- checkpoint / autotune
 
 def assert_engines_equal(a, b, levels, tag):
