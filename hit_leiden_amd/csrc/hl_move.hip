@@ -1002,7 +1002,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
     // the next round starts from zero counts without a host-side reset
     *cand.cnt = 0;
     *cur_cnt = 0;
-    for (u32 i = 0; i <= kBuckets; ++i) bucket_counts[i] = 0;  // + the hub overflow count
+    for (u32 i = 0; i < kBuckets + 2; ++i) bucket_counts[i] = 0;  // + the two overflow counts
     atomicAdd(&dst[DS_DEC_CAND], (unsigned long long)c);
     atomicAdd(&dst[DS_DEC_ACC], (unsigned long long)nacc);
   }
@@ -1064,7 +1064,7 @@ __global__ void k_round_reset(u32* cur_cnt, u32* ccnt, u32* bucket_counts) {
   if (threadIdx.x == 0) {
     if (cur_cnt) *cur_cnt = 0;
     if (ccnt) *ccnt = 0;
-    for (u32 i = 0; i <= kBuckets; ++i) bucket_counts[i] = 0;  // + the hub overflow count
+    for (u32 i = 0; i < kBuckets + 2; ++i) bucket_counts[i] = 0;  // + the two overflow counts
   }
 }
 
@@ -1254,7 +1254,9 @@ u32 MoveScratch::next_epoch(hipStream_t s) {
 template <class P>
 static void propose_round(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u64 list_cap, u32 maxdeg,
                           MoveScratch& ms, const Thresholds& th, unsigned long long* st, hipStream_t s) {
-  u32* counts = ms.counters.p + 4;  // [4..4 + kBuckets] bucket counts and hub overflow count, zero on entry
+  // [4..4 + kBuckets) bucket counts, then the overflow counts of the hub and
+  // LDS-workgroup tables; all zero on entry
+  u32* counts = ms.counters.p + 4;
   const u32 small = th.small_deg > SMALL_MAX2 ? SMALL_MAX2 : th.small_deg;
   const u32 wave = th.wave_deg > WAVE_HW2 / 2 ? WAVE_HW2 / 2 : th.wave_deg;
   CandOut out{ms.cnode.p, ms.cto.p, ms.cgain.p, ms.counters.p + 1};
@@ -1271,10 +1273,18 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
                        ms.blists.p + list_cap, counts + 1, out, cst + 1);
     HL_LAUNCH_CHECK();
   }
+  if (maxdeg > wave) ms.hovf.reserve(2 * list_cap, s);
   if (maxdeg > wave) {
+    // a small LDS table per vertex first (six workgroups per CU); vertices
+    // with more distinct keys than it holds take the table sized for the degree
     HL_PROF(P_PROP_LDS, 0, s);
-    hipLaunchKernelGGL(k_prop2_lds<P>, dim3(fixed_grid(list_cap, 1, 256)), dim3(1024), 0, s, pol, g,
-                       ms.blists.p + 2 * list_cap, counts + 2, out, cst + 2);
+    u32* ovf_cnt = counts + kBuckets + 1;
+    u32* ovf = ms.hovf.p + list_cap;
+    hipLaunchKernelGGL((k_prop2_hub<P, 256, 2048>), dim3(1536), dim3(256), 0, s, pol, g, ms.blists.p + 2 * list_cap,
+                       counts + 2, ovf, ovf_cnt, out, cst + 2);
+    HL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_prop2_lds<P>, dim3(fixed_grid(list_cap, 1, 256)), dim3(1024), 0, s, pol, g, ovf, ovf_cnt,
+                       out, cst + 2);
     HL_LAUNCH_CHECK();
   }
   if (maxdeg > kLdsDeg) {
@@ -1282,7 +1292,6 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
     // tables (one per resident workgroup, at most kBigTableBytes in all)
     HL_PROF(P_PROP_BIG, 0, s);
     u32* ovf_cnt = counts + kBuckets;
-    ms.hovf.reserve(list_cap, s);
     hipLaunchKernelGGL((k_prop2_hub<P, 512, 4096>), dim3(768), dim3(512), 0, s, pol, g, ms.blists.p + 3 * list_cap,
                        counts + 3, ms.hovf.p, ovf_cnt, out, cst + 3);
     HL_LAUNCH_CHECK();
@@ -1307,7 +1316,8 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
   const u32 n = m.g.n;
   ms.ensure(n, m.comm_cap, s);
   // counters: [0], [2] list counts (alternating), [1] candidates, [4..9) bucket
-  // counts, [9] hub overflow count, [13] flat mark count, [14] overflow flag, [15] accepted moves; a round leaves all but the
+  // counts, [9], [10] overflow counts of the hub and LDS-workgroup tables,
+  // [13] flat mark count, [14] overflow flag, [15] accepted moves; a round leaves all but the
   // next list's count at zero
   u32* lcnt[2] = {ms.counters.p + 0, ms.counters.p + 2};
   u32* ccnt = ms.counters.p + 1;
