@@ -423,7 +423,7 @@ struct Engine {
     if (prof.on) {
       const int slots[8] = {P_PROP_SMALL, P_PROP_WAVE, P_PROP_LDS, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI,
                             P_APPLY_MOVES, P_MARK_MOVED};
-      const u64 by[8] = {prop[0], prop[1], prop[2], prop[3] + prop[4], dec, multi, apply, marks + 0};
+      const u64 by[8] = {prop[0], prop[1] + prop[2], prop[3], prop[4] + prop[5], dec, multi, apply, marks};
       for (int i = 0; i < 8; ++i)
         if (prof.records(slots[i])) prof.bytes[slots[i]] += by[i];
     }

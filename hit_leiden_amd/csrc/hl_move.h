@@ -91,7 +91,7 @@ struct RefinePolicyT {
 // A proposal block (DS_MOVE for movement, DS_REF for refinement) holds per
 // degree bucket b (kBuckets of them): [b] vertices evaluated, [kBuckets + b]
 // row entries read, [2 kBuckets + b] distinct candidate communities.
-constexpr u32 kBuckets = 5;  // lane, wavefront, workgroup+LDS, workgroup+HBM, giant
+constexpr u32 kBuckets = 6;  // lane, half wavefront, wavefront, LDS workgroup, hub workgroup, giant
 enum DevStat : u32 {
   DS_MOVE = 0,
   DS_REF = 3 * kBuckets,

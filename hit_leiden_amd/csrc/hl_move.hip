@@ -19,11 +19,14 @@ namespace hl {
 namespace {
 
 constexpr u32 SMALL_MAX2 = 16;
-constexpr u32 WAVE_HW2 = 1024;
+constexpr u32 WAVE_HW2 = 1024;  // table slots of a wavefront-per-vertex proposal
+constexpr u32 kHalfDeg = 128;   // half-wavefront-per-vertex proposals up to this degree
 constexpr u32 kUndec = 0, kAcc = 1, kRej = 2;
 constexpr u32 kDecLds = 1024;   // candidates whose priority rounds run on LDS slots
 constexpr u32 kDecHash = 4096;  // LDS hash slots for their 2 x kDecLds communities
 constexpr u32 kWaveBlocks = 768;
+constexpr u32 kHalfBlocks = 1280;
+static_assert(4 + kBuckets + 2 <= 13, "bucket and overflow counts must end before the flat mark count");
 constexpr u32 kLdsDeg = 4096;       // workgroup-per-vertex with an LDS table up to this degree
 constexpr u32 kLdsCap = 8192;       // its table slots (big_cap(kLdsDeg))
 constexpr u32 kBigDeg = 1u << 18;   // workgroup-per-vertex with an HBM table up to this degree
@@ -239,12 +242,13 @@ __device__ __forceinline__ void finish_choice(const P& pol, u32 v, u32 cur, doub
 
 // The round's first kernel: one thread per active vertex; vertices up to
 // small_deg are evaluated here (row-order sums in a private LDS slice), the
-// others are appended to the wavefront / workgroup bucket lists (lane,
-// wavefront <= wave_deg, LDS workgroup <= kLdsDeg, HBM workgroup <= kBigDeg,
-// giant) for the kernels that follow.
+// others are appended to the bucket lists of the kernels that follow (half
+// wavefront <= half_deg, wavefront <= wave_deg, LDS workgroup <= kLdsDeg, hub
+// workgroup <= kBigDeg, giant).
 template <class P>
 __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* list, const u32* cnt_dev,
-                                                      u32 small_deg, u32 wave_deg, u32* lists, u64 cap, u32* counts,
+                                                      u32 small_deg, u32 half_deg, u32 wave_deg, u32* lists, u64 cap,
+                                                      u32* counts,
                                                       CandOut out, unsigned long long* st) {
   __shared__ u32 sk[SMALL_MAX2 * 256];
   __shared__ double sv[SMALL_MAX2 * 256];
@@ -258,7 +262,12 @@ __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* li
     if (id < cnt) {
       v = list[id];
       deg = g.row_len[v];
-      b = deg <= small_deg ? 0 : deg <= wave_deg ? 1 : deg <= kLdsDeg ? 2 : deg <= kBigDeg ? 3 : 4;
+      b = deg <= small_deg ? 0
+          : deg <= half_deg ? 1
+          : deg <= wave_deg ? 2
+          : deg <= kLdsDeg  ? 3
+          : deg <= kBigDeg  ? 4
+                            : 5;
 #pragma unroll
       for (u32 q = 0; q < kBuckets; ++q)
         if (q == b) {
@@ -318,8 +327,6 @@ __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* li
   block_add_multi(st, idx, vals);
 }
 
-// one wavefront per vertex, LDS open-addressing table of up to WAVE_HW2 slots
-// (a power of two >= 2 deg, so only the slots a vertex can use are cleared)
 // Proposals are staged in LDS and appended with one device atomic per flush.
 constexpr u32 kStage = 256;
 __device__ void flush_stage(u32* sn, u32* st, double* sg, u32& scnt, u32& sbase, CandOut out) {
@@ -340,22 +347,35 @@ __device__ void flush_stage(u32* sn, u32* st, double* sg, u32& scnt, u32& sbase,
   __syncthreads();
 }
 
-template <class P>
+// G lanes per vertex (a wavefront or half of one), an LDS open-addressing
+// table of up to CAP slots per vertex (a power of two >= 2 deg, so only the
+// slots a vertex can use are cleared); 256 / G vertices per workgroup pass
+template <int G>
+__device__ __forceinline__ void group_argmax(double& g, u32& k) {
+  for (int o = G / 2; o > 0; o >>= 1) {
+    double og = __shfl_xor(g, o, 64);
+    u32 ok = __shfl_xor(k, o, 64);
+    better(g, k, og, ok);
+  }
+}
+
+template <class P, u32 G, u32 CAP>
 __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* list, const u32* cnt_dev, CandOut out,
                                                      unsigned long long* cst) {
-  __shared__ u32 hk[4 * WAVE_HW2];
-  __shared__ double hv[4 * WAVE_HW2];
+  constexpr u32 NG = 256 / G;
+  __shared__ u32 hk[NG * CAP];
+  __shared__ double hv[NG * CAP];
   __shared__ u32 stn[kStage], stt[kStage];
   __shared__ double stg[kStage];
   __shared__ u32 scnt, sbase;
   if (threadIdx.x == 0) scnt = 0;
   const u32 cnt = *cnt_dev;
-  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  u32* K = hk + wv * WAVE_HW2;
-  double* V = hv + wv * WAVE_HW2;
+  const u32 lane = threadIdx.x % G, grp = threadIdx.x / G;
+  u32* K = hk + grp * CAP;
+  double* V = hv + grp * CAP;
   unsigned long long ncand = 0;
-  for (u64 base = u64(blockIdx.x) * 4; base < cnt; base += u64(gridDim.x) * 4) {
-    const u64 id = base + wv;
+  for (u64 base = u64(blockIdx.x) * NG; base < cnt; base += u64(gridDim.x) * NG) {
+    const u64 id = base + grp;
     const bool has = id < cnt;
     u32 v = 0, cur = 0, deg = 0, cap = 64;
     u64 off = 0;
@@ -365,14 +385,14 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
       deg = g.row_len[v];
       cur = pol.cur_key(v);
       while (cap < 2 * deg) cap <<= 1;
-      for (u32 q = lane; q < cap; q += 64) {
+      for (u32 q = lane; q < cap; q += G) {
         K[q] = kNone;
         V[q] = 0.0;
       }
     }
     __syncthreads();
     if (has) {
-      for (u32 j = lane; j < deg; j += 64) {
+      for (u32 j = lane; j < deg; j += G) {
         u32 u = g.col[off + j];
         u32 key;
         int mult;
@@ -384,12 +404,12 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
       }
     }
     __syncthreads();
+    double bg = 0.0, wcur = 0.0;
+    u32 best = kNone;
     if (has) {
-      const double wcur = tab_lookup(K, V, cap, cur);
+      wcur = tab_lookup(K, V, cap, cur);
       const double kv = g.k[v], sc = pol.cur_total(cur), tt = tm_of(g, v);
-      double bg = 0.0;
-      u32 best = kNone;
-      for (u32 q = lane; q < cap; q += 64) {
+      for (u32 q = lane; q < cap; q += G) {
         u32 key = K[q];
         if (key == kNone) continue;
         ++ncand;
@@ -397,19 +417,19 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
         double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
         if (gn > 0.0) better(bg, best, gn, key);
       }
-      wave_argmax(bg, best);
-      if (lane == 0) {
-        finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
-        if (bg > 0.0) {
-          const u32 p = atomicAdd(&scnt, 1u);  // LDS
-          stn[p] = v;
-          stt[p] = best;
-          stg[p] = bg;
-        }
+    }
+    group_argmax<G>(bg, best);  // every lane of the wavefront takes part in the shuffles
+    if (has && lane == 0) {
+      finish_choice(pol, v, cur, wcur, g.k[v], tm_of(g, v), bg, best);
+      if (bg > 0.0) {
+        const u32 p = atomicAdd(&scnt, 1u);  // LDS
+        stn[p] = v;
+        stt[p] = best;
+        stg[p] = bg;
       }
     }
     __syncthreads();
-    if (scnt > kStage - 4) flush_stage(stn, stt, stg, scnt, sbase, out);
+    if (scnt > kStage - NG) flush_stage(stn, stt, stg, scnt, sbase, out);
   }
   flush_stage(stn, stt, stg, scnt, sbase, out);
   block_add_u64(cst, ncand);
@@ -1259,19 +1279,25 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
   u32* counts = ms.counters.p + 4;
   const u32 small = th.small_deg > SMALL_MAX2 ? SMALL_MAX2 : th.small_deg;
   const u32 wave = th.wave_deg > WAVE_HW2 / 2 ? WAVE_HW2 / 2 : th.wave_deg;
+  const u32 half = std::max(small, std::min(kHalfDeg, wave));
   CandOut out{ms.cnode.p, ms.cto.p, ms.cgain.p, ms.counters.p + 1};
   unsigned long long* cst = st + 2 * kBuckets;
   {
     HL_PROF(P_PROP_SMALL, 0, s);
-    hipLaunchKernelGGL(k_prop2_first<P>, fixed_grid(list_cap), 256, 0, s, pol, g, list, cnt_dev, small, wave,
+    hipLaunchKernelGGL(k_prop2_first<P>, fixed_grid(list_cap), 256, 0, s, pol, g, list, cnt_dev, small, half, wave,
                        ms.blists.p, list_cap, counts, out, st);
     HL_LAUNCH_CHECK();
   }
-  {
+  if (maxdeg > small) {
     HL_PROF(P_PROP_WAVE, 0, s);
-    hipLaunchKernelGGL(k_prop2_wave<P>, dim3(fixed_grid(list_cap, 4, kWaveBlocks)), dim3(256), 0, s, pol, g,
-                       ms.blists.p + list_cap, counts + 1, out, cst + 1);
+    hipLaunchKernelGGL((k_prop2_wave<P, 32, 2 * kHalfDeg>), dim3(fixed_grid(list_cap, 8, kHalfBlocks)), dim3(256), 0,
+                       s, pol, g, ms.blists.p + list_cap, counts + 1, out, cst + 1);
     HL_LAUNCH_CHECK();
+    if (maxdeg > half) {
+      hipLaunchKernelGGL((k_prop2_wave<P, 64, WAVE_HW2>), dim3(fixed_grid(list_cap, 4, kWaveBlocks)), dim3(256), 0, s,
+                         pol, g, ms.blists.p + 2 * list_cap, counts + 2, out, cst + 2);
+      HL_LAUNCH_CHECK();
+    }
   }
   if (maxdeg > wave) ms.hovf.reserve(2 * list_cap, s);
   if (maxdeg > wave) {
@@ -1280,11 +1306,11 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
     HL_PROF(P_PROP_LDS, 0, s);
     u32* ovf_cnt = counts + kBuckets + 1;
     u32* ovf = ms.hovf.p + list_cap;
-    hipLaunchKernelGGL((k_prop2_hub<P, 256, 2048>), dim3(1536), dim3(256), 0, s, pol, g, ms.blists.p + 2 * list_cap,
-                       counts + 2, ovf, ovf_cnt, out, cst + 2);
+    hipLaunchKernelGGL((k_prop2_hub<P, 256, 2048>), dim3(1536), dim3(256), 0, s, pol, g, ms.blists.p + 3 * list_cap,
+                       counts + 3, ovf, ovf_cnt, out, cst + 3);
     HL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_prop2_lds<P>, dim3(fixed_grid(list_cap, 1, 256)), dim3(1024), 0, s, pol, g, ovf, ovf_cnt,
-                       out, cst + 2);
+                       out, cst + 3);
     HL_LAUNCH_CHECK();
   }
   if (maxdeg > kLdsDeg) {
@@ -1292,12 +1318,12 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
     // tables (one per resident workgroup, at most kBigTableBytes in all)
     HL_PROF(P_PROP_BIG, 0, s);
     u32* ovf_cnt = counts + kBuckets;
-    hipLaunchKernelGGL((k_prop2_hub<P, 512, 4096>), dim3(768), dim3(512), 0, s, pol, g, ms.blists.p + 3 * list_cap,
-                       counts + 3, ms.hovf.p, ovf_cnt, out, cst + 3);
+    hipLaunchKernelGGL((k_prop2_hub<P, 512, 4096>), dim3(768), dim3(512), 0, s, pol, g, ms.blists.p + 4 * list_cap,
+                       counts + 4, ms.hovf.p, ovf_cnt, out, cst + 4);
     HL_LAUNCH_CHECK();
     if (maxdeg > kBigDeg) {
       hipLaunchKernelGGL((k_prop2_hub<P, 1024, 8192>), dim3(256), dim3(1024), 0, s, pol, g,
-                         ms.blists.p + 4 * list_cap, counts + 4, ms.hovf.p, ovf_cnt, out, cst + 4);
+                         ms.blists.p + 5 * list_cap, counts + 5, ms.hovf.p, ovf_cnt, out, cst + 5);
       HL_LAUNCH_CHECK();
     }
     const u32 tcap = big_cap(maxdeg);
@@ -1306,7 +1332,7 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
     ms.tk.reserve(u64(nb) * tcap, s);
     ms.tv.reserve(u64(nb) * tcap, s);
     hipLaunchKernelGGL(k_prop2_big<P>, dim3(nb), dim3(1024), 0, s, pol, g, ms.hovf.p, ovf_cnt, ms.tk.p, ms.tv.p,
-                       tcap, out, cst + 3);
+                       tcap, out, cst + 4);
     HL_LAUNCH_CHECK();
   }
 }
@@ -1315,8 +1341,8 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
                         const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s) {
   const u32 n = m.g.n;
   ms.ensure(n, m.comm_cap, s);
-  // counters: [0], [2] list counts (alternating), [1] candidates, [4..9) bucket
-  // counts, [9], [10] overflow counts of the hub and LDS-workgroup tables,
+  // counters: [0], [2] list counts (alternating), [1] candidates, [4..10) bucket
+  // counts, [10], [11] overflow counts of the hub and LDS-workgroup tables,
   // [13] flat mark count, [14] overflow flag, [15] accepted moves; a round leaves all but the
   // next list's count at zero
   u32* lcnt[2] = {ms.counters.p + 0, ms.counters.p + 2};
