@@ -231,14 +231,19 @@ struct Engine {
     apply_delta_plan(d, k, g, as, tmp, plan, s);
     if (plan.nseg) {
       if (lv.pool_used + plan.need > lv.col.cap) {
-        // compact live rows into a fresh pool with room for this batch and headroom
+        // compact live rows into a fresh pool with room for this batch and
+        // headroom: a quarter of the live entries, or up to three more batches
+        // that rewrite as many rows as this one, within half the live entries
+        // (merged rows move to the pool's end, and hub-heavy batches rewrite a
+        // large share of the entries)
         const u64 live = compact_plan(g, offtmp, tmp, s);
         {
           DevBuf<u32> col2;
           DevBuf<double> w2;
-          const u64 want = std::max<u64>(live + plan.need, 1024);
-          col2.reserve(want, s, false, 1.25);
-          w2.reserve(want, s, false, 1.25);
+          const u64 room = std::max(live / 4, std::min(3 * plan.need, live / 2));
+          const u64 want = std::max<u64>(live + plan.need + room, 1024);
+          col2.reserve(want, s, false, 1.0);
+          w2.reserve(want, s, false, 1.0);
           compact_copy(g, offtmp.p, col2.p, w2.p, s);
           lv.col.swap(col2);
           lv.w.swap(w2);

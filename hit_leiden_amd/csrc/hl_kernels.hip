@@ -1247,17 +1247,42 @@ __global__ void k_def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np) 
 __global__ void k_gather_u32(u32* cur, const u32* map, u32 n) {
   for (u64 v = gtid(); v < n; v += gstride()) cur[v] = map[cur[v]];
 }
+// One lane per vertex for rows up to 64 entries; longer rows (hubs) are taken
+// by the whole wavefront, one after another, so no lane walks a hub row alone.
 __global__ void k_intra(Csr g, const u32* lab, double* acc) {
   double local = 0.0;
-  for (u64 v = gtid(); v < g.n; v += gstride()) {
-    u64 off = g.row_off[v];
-    u32 deg = g.row_len[v];
-    u32 lv = lab[v];
-    for (u32 j = 0; j < deg; ++j) {
-      u32 u = g.col[off + j];
-      if (lab[u] == lv) {
-        double w = g.w[off + j];
-        local += (u == v) ? 2.0 * w : w;
+  const u32 lane = threadIdx.x & 63;
+  for (u64 base = gtid() - lane; base < g.n; base += gstride()) {
+    const u64 v = base + lane;
+    u64 off = 0;
+    u32 deg = 0, lv = 0;
+    if (v < g.n) {
+      off = g.row_off[v];
+      deg = g.row_len[v];
+      lv = lab[v];
+    }
+    const bool wide = deg > 64;
+    if (!wide)
+      for (u32 j = 0; j < deg; ++j) {
+        const u32 u = g.col[off + j];
+        if (lab[u] == lv) {
+          const double w = g.w[off + j];
+          local += (u == v) ? 2.0 * w : w;
+        }
+      }
+    unsigned long long m = __ballot(wide);
+    while (m) {
+      const int src = __ffsll((long long)m) - 1;
+      m &= m - 1;
+      const u64 bv = __shfl((unsigned long long)v, src, 64);
+      const u64 boff = __shfl((unsigned long long)off, src, 64);
+      const u32 bdeg = __shfl(deg, src, 64), blv = __shfl(lv, src, 64);
+      for (u32 j = lane; j < bdeg; j += 64) {
+        const u32 u = g.col[boff + j];
+        if (lab[u] == blv) {
+          const double w = g.w[boff + j];
+          local += (u == bv) ? 2.0 * w : w;
+        }
       }
     }
   }

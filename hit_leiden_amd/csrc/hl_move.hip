@@ -1035,7 +1035,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
 __global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u32* mark, u32 epoch, u32* next,
                                                     u32* next_cnt) {
   __shared__ u32 sv_[kMarkStage];
-  __shared__ u32 scnt, sbase;
+  __shared__ u32 scnt, sbase, span[2];
   const u32 c = *flat.c;
   if (c == 0) return;
   if (threadIdx.x == 0) scnt = 0;
@@ -1043,8 +1043,21 @@ __global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u
   const u64 total = flat.pre[c];
   for (u64 base = u64(blockIdx.x) * blockDim.x; base < total; base += u64(gridDim.x) * blockDim.x) {
     const u64 t = base + threadIdx.x;
-    if (t < total) {
+    // the segments of the pass's first and last entries bound every thread's
+    // search, so the per-thread searches take a few cached steps
+    if (threadIdx.x == 0 || threadIdx.x == blockDim.x - 1) {
+      const u64 e = threadIdx.x == 0 ? base : (t < total ? t : total - 1);
       u32 lo = 0, hi = c;
+      while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (flat.pre[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      span[threadIdx.x == 0 ? 0 : 1] = lo;
+    }
+    __syncthreads();
+    if (t < total) {
+      u32 lo = span[0], hi = span[1] + 1;
       while (hi - lo > 1) {
         const u32 mid = (lo + hi) >> 1;
         if (flat.pre[mid] <= t) lo = mid;
@@ -1119,23 +1132,58 @@ __global__ void k_reset4(u64 c, const u32* from, const u32* to, u32* a, u32* b, 
     y[f] = kNone; y[t] = kNone;
   }
 }
+// The minima only fall during a kernel, so a plain read at or below x means
+// the atomic would change nothing.
 __device__ __forceinline__ void min_if_lower(u32* a, u32 x) {
   if (*a > x) atomicMin(a, x);
 }
-__global__ void k_mark4(u64 c, const u32* from, const u32* to, const u32* state, u32* minUto, u32* minUfrom,
-                        u32* minAto, u32* minAfrom) {
-  for (u64 p = gtid(); p < c; p += gstride()) {
-    // values only fall during the kernel, so a plain read at or below p means
-    // the atomic would change nothing: big communities take one atomic per
-    // cache refresh instead of one per candidate
-    u32 st = state[p];
-    if (st == kUndec) {
-      min_if_lower(&minUto[to[p]], u32(p));
-      min_if_lower(&minUfrom[from[p]], u32(p));
-    } else if (st == kAcc) {
-      min_if_lower(&minAto[to[p]], u32(p));
-      min_if_lower(&minAfrom[from[p]], u32(p));
+// Per workgroup pass of kMarkChunk candidates, the minima are first taken in
+// LDS tables (one per array, keyed by community), so a community that many
+// candidates share (a giant one) takes one device atomic per pass instead of
+// one per candidate.
+constexpr u32 kMarkChunk = 512;
+constexpr u32 kMarkSlots = 1024;  // >= 2 kMarkChunk: at most half full
+__device__ __forceinline__ void lds_min_insert(u32* K, u32* M, u32 key, u32 p) {
+  u32 h = hash32(key) & (kMarkSlots - 1);
+  while (true) {
+    const u32 old = atomicCAS(&K[h], kNone, key);
+    if (old == kNone || old == key) {
+      atomicMin(&M[h], p);
+      return;
     }
+    h = (h + 1) & (kMarkSlots - 1);
+  }
+}
+__global__ __launch_bounds__(256) void k_mark4(u64 c, const u32* from, const u32* to, const u32* state, u32* minUto,
+                                               u32* minUfrom, u32* minAto, u32* minAfrom) {
+  __shared__ u32 K[4 * kMarkSlots], M[4 * kMarkSlots];
+  for (u64 base = u64(blockIdx.x) * kMarkChunk; base < c; base += u64(gridDim.x) * kMarkChunk) {
+    for (u32 q = threadIdx.x; q < 4 * kMarkSlots; q += blockDim.x) {
+      K[q] = kNone;
+      M[q] = kNone;
+    }
+    __syncthreads();
+    for (u32 r = threadIdx.x; r < kMarkChunk; r += blockDim.x) {
+      const u64 p = base + r;
+      if (p >= c) break;
+      const u32 st = state[p];
+      if (st == kUndec) {
+        lds_min_insert(K, M, to[p], u32(p));
+        lds_min_insert(K + kMarkSlots, M + kMarkSlots, from[p], u32(p));
+      } else if (st == kAcc) {
+        lds_min_insert(K + 2 * kMarkSlots, M + 2 * kMarkSlots, to[p], u32(p));
+        lds_min_insert(K + 3 * kMarkSlots, M + 3 * kMarkSlots, from[p], u32(p));
+      }
+    }
+    __syncthreads();
+    for (u32 q = threadIdx.x; q < 4 * kMarkSlots; q += blockDim.x) {
+      const u32 key = K[q];
+      if (key == kNone) continue;
+      const u32 a = q / kMarkSlots;
+      u32* dst = a == 0 ? minUto : a == 1 ? minUfrom : a == 2 ? minAto : minAfrom;
+      min_if_lower(&dst[key], M[q]);
+    }
+    __syncthreads();
   }
 }
 __global__ void k_decide4(u64 c, const u32* from, const u32* to, u32* state, const u32* minUto, const u32* minUfrom,
