@@ -330,6 +330,11 @@ struct Engine {
     ensure_totals(lv);
     const int max_rounds = cfg.max_rounds > 0 ? cfg.max_rounds : 20;
     MoveCtx m{g, lv.f.p, lv.s_cur.p, lv.tot.p, lv.k.p, lv.A.p, lv.B.p, lv.K.p, cfg.gamma, lv.tot.cap};
+    // the neighbour-community cache spares the proposals their gathers of f
+    // but costs a binary search in the neighbour's row per entry of each
+    // moved vertex; on R-MAT scale 24 the searches cost more than the gathers
+    // saved, so it is opt-in
+    if (cfg.key_cache == 1) m.pool = lv.col.cap;
     if (multi()) {
       m.gid = lv.gid.p;
       m.part = part.p;
@@ -640,7 +645,7 @@ hl_status hl_create(const hl_config* cfg, hl_graph** out) {
   }
   *out = nullptr;
   if (cfg->levels < 1 || cfg->levels > HL_MAX_LEVELS || !(cfg->gamma > 0.0) || !std::isfinite(cfg->gamma) ||
-      (cfg->mode != HL_MODE_DETERMINISTIC && cfg->mode != HL_MODE_THROUGHPUT)) {
+      (cfg->mode != HL_MODE_DETERMINISTIC && cfg->mode != HL_MODE_THROUGHPUT) || cfg->key_cache > 1) {
     g_create_err = "invalid config";
     return HL_INVALID_INPUT;
   }

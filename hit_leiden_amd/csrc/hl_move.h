@@ -20,7 +20,8 @@ struct FlatMarks {
   u32* node;
   u32* to;
   u64* pre;  // exclusive prefix of the list lengths, pre[c] = total
-  u32* c;    // device count (0: nothing to do)
+  u32* c;    // device count for k_marks_flat (0: nothing to do)
+  u32* ck_c = nullptr;  // device count for the cache updates of k_ck_flat
 };
 
 // what a movement round reads and writes at one level
@@ -41,6 +42,13 @@ struct MoveCtx {
   const u32* gid = nullptr;
   const u8* part = nullptr;
   const u32* gcnt = nullptr;
+  // neighbour-community cache: ck[e] = f[col[e]] for every row entry e of
+  // vertex v with col[e] != v, kNone for v's self-loop.  Rebuilt when the
+  // rounds of a level start and kept current by the moves, so the proposals
+  // read it with their row instead of gathering f at random.  pool: entries
+  // the row pool holds (the cache's size); 0 leaves the cache off.
+  u32* ck = nullptr;
+  u64 pool = 0;
 };
 
 // throughput movement (parallel_frontier.rs:155-236): self-loops skipped,
@@ -50,11 +58,18 @@ struct MovePolicyT {
   const double* tot;
   double gamma;
   u32 n;
+  const u32* ck = nullptr;  // neighbour-community cache (MoveCtx::ck), or none
   static constexpr bool kEmpty = true;
-  __device__ bool accept(u32 v, u32 u, u32& key, int& mult) const {
+  // row entry e of vertex v (neighbour col[e])
+  __device__ bool accept_e(u32 v, const u32* col, u64 e, u32& key, int& mult) const {
+    mult = 1;
+    if (ck) {
+      key = ck[e];
+      return key != kNone;
+    }
+    const u32 u = col[e];
     if (u == v) return false;
     key = f[u];
-    mult = 1;
     return true;
   }
   __device__ u32 cur_key(u32 v) const { return f[v]; }
@@ -74,7 +89,8 @@ struct RefinePolicyT {
   const u8* blocked;
   double gamma;
   static constexpr bool kEmpty = false;
-  __device__ bool accept(u32 v, u32 u, u32& key, int& mult) const {
+  __device__ bool accept_e(u32 v, const u32* col, u64 e, u32& key, int& mult) const {
+    const u32 u = col[e];
     if (f[u] != f[v]) return false;
     key = s[u];
     mult = (u == v) ? 2 : 1;
@@ -117,6 +133,7 @@ struct MoveScratch {
   DevBuf<u32> mnode, mto;
   DevBuf<u64> mpre, mdeg;
   DevBuf<u8> target;
+  DevBuf<u32> ck;  // neighbour-community cache of the level in movement
   size_t target_cap = 0;
   u32 epoch = 0;
   void ensure(u64 n, u64 comm_cap, hipStream_t s);

@@ -26,7 +26,16 @@ constexpr u32 kDecLds = 1024;   // candidates whose priority rounds run on LDS s
 constexpr u32 kDecHash = 4096;  // LDS hash slots for their 2 x kDecLds communities
 constexpr u32 kWaveBlocks = 768;
 constexpr u32 kHalfBlocks = 1280;
-static_assert(4 + kBuckets + 2 <= 13, "bucket and overflow counts must end before the flat mark count");
+constexpr u32 kTabBlocks = 2048;  // small-table proposals: up to 8 workgroups per CU
+static_assert(4 + kBuckets <= 13, "bucket counts must end before the flat mark count");
+// counters[kOvfSlot + i]: entries of overflow list i (vertices whose distinct
+// neighbour keys did not fit the small tables of their bucket)
+constexpr u32 kOvfSlot = 16, kOvfLists = 4;
+// bucket counts start at counters[4]
+__device__ __forceinline__ void clear_bucket_counts(u32* bucket_counts) {
+  for (u32 i = 0; i < kBuckets; ++i) bucket_counts[i] = 0;
+  for (u32 i = 0; i < kOvfLists; ++i) bucket_counts[kOvfSlot - 4 + i] = 0;
+}
 constexpr u32 kLdsDeg = 4096;       // workgroup-per-vertex with an LDS table up to this degree
 constexpr u32 kLdsCap = 8192;       // its table slots (big_cap(kLdsDeg))
 constexpr u32 kBigDeg = 1u << 18;   // workgroup-per-vertex with an HBM table up to this degree
@@ -116,6 +125,52 @@ __device__ __forceinline__ bool claim(u32* mark, u32 v, u32 epoch) {
   if (mark[v] == epoch) return false;
   return atomicExch(&mark[v], epoch) != epoch;
 }
+// Keep the neighbour-community cache current after v moved to community to:
+// the entry for v in neighbour u's row (rows sorted by neighbour id) gets to.
+__device__ __forceinline__ void ck_update(const MoveCtx& m, u32 u, u32 v, u32 to) {
+  if (!m.ck || u == v) return;
+  const u64 off = m.g.row_off[u];
+  const u32 len = m.g.row_len[u];
+  u32 lo = 0, hi = len;
+  while (lo < hi) {
+    const u32 mid = (lo + hi) >> 1;
+    if (m.g.col[off + mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < len && m.g.col[off + lo] == v) m.ck[off + lo] = to;
+}
+// Rebuild the cache for every row: one lane per row up to 64 entries, the
+// whole wavefront for a longer one.
+__global__ void k_key_cache(Csr g, const u32* f, u32* ck) {
+  const u32 lane = threadIdx.x & 63;
+  for (u64 base = gtid() - lane; base < g.n; base += gstride()) {
+    const u64 v = base + lane;
+    u64 off = 0;
+    u32 deg = 0;
+    if (v < g.n) {
+      off = g.row_off[v];
+      deg = g.row_len[v];
+    }
+    const bool wide = deg > 64;
+    if (!wide)
+      for (u32 j = 0; j < deg; ++j) {
+        const u32 u = g.col[off + j];
+        ck[off + j] = u == v ? kNone : f[u];
+      }
+    unsigned long long wm = __ballot(wide);
+    while (wm) {
+      const int src = __ffsll((long long)wm) - 1;
+      wm &= wm - 1;
+      const u64 bv = __shfl((unsigned long long)v, src, 64);
+      const u64 boff = __shfl((unsigned long long)off, src, 64);
+      const u32 bdeg = __shfl(deg, src, 64);
+      for (u32 j = lane; j < bdeg; j += 64) {
+        const u32 u = g.col[boff + j];
+        ck[boff + j] = u == bv ? kNone : f[u];
+      }
+    }
+  }
+}
 // Workgroup-table slots for a hub of degree deg (load factor at most 2/3).
 __host__ __device__ inline u32 big_cap(u32 deg) {
   u32 c = 64;
@@ -123,10 +178,21 @@ __host__ __device__ inline u32 big_cap(u32 deg) {
   return c;
 }
 
+// Slot of key in an LDS table of cap slots (linear probing, inserts if absent).
 __device__ __forceinline__ u32 tab_insert(u32* K, u32 cap, u32 key) {
   u32 h = hash32(key) & (cap - 1);
   while (true) {
     u32 old = atomicCAS(&K[h], kNone, key);
+    if (old == kNone || old == key) return h;
+    h = (h + 1) & (cap - 1);
+  }
+}
+// The same for the HBM tables, with atomics only: a plain read there may be
+// served by the CU's L1, which device atomics do not update.
+__device__ __forceinline__ u32 hbm_insert(u32* K, u32 cap, u32 key) {
+  u32 h = hash32(key) & (cap - 1);
+  while (true) {
+    const u32 old = atomicCAS(&K[h], kNone, key);
     if (old == kNone || old == key) return h;
     h = (h + 1) & (cap - 1);
   }
@@ -284,11 +350,10 @@ __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* li
       const u32 cur = pol.cur_key(v);
       int nk = 0;
       for (u32 j = 0; j < deg; ++j) {
-        u32 u = g.col[off + j];
         double w = g.w[off + j];
         u32 key;
         int mult;
-        if (!pol.accept(v, u, key, mult)) continue;
+        if (!pol.accept_e(v, g.col, off + j, key, mult)) continue;
         int q = 0;
         while (q < nk && sk[q * 256 + t] != key) ++q;
         if (q == nk) {
@@ -393,10 +458,9 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
     __syncthreads();
     if (has) {
       for (u32 j = lane; j < deg; j += G) {
-        u32 u = g.col[off + j];
         u32 key;
         int mult;
-        if (!pol.accept(v, u, key, mult)) continue;
+        if (!pol.accept_e(v, g.col, off + j, key, mult)) continue;
         double w = g.w[off + j];
         u32 h = tab_insert(K, cap, key);
         atomicAdd(&V[h], w);
@@ -455,10 +519,9 @@ __global__ __launch_bounds__(1024) void k_prop2_lds(P pol, Csr g, const u32* lis
     }
     __syncthreads();
     for (u32 j = threadIdx.x; j < deg; j += blockDim.x) {
-      const u32 u = g.col[off + j];
       u32 key;
       int mult;
-      if (!pol.accept(v, u, key, mult)) continue;
+      if (!pol.accept_e(v, g.col, off + j, key, mult)) continue;
       const double w = g.w[off + j];
       const u32 h = tab_insert(K, cap, key);
       atomicAdd(&V[h], w);
@@ -541,7 +604,7 @@ __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, 
     // four row entries per thread in flight: the gathers are independent
     for (u32 j0 = threadIdx.x; j0 < deg; j0 += 4 * T) {
       if (full) break;
-      u32 uu[4], kk[4];
+      u32 kk[4];
       double ww[4];
       int mm[4];
       bool ok[4];
@@ -549,11 +612,9 @@ __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, 
       for (int r = 0; r < 4; ++r) {
         const u32 j = j0 + r * T;
         ok[r] = j < deg;
-        uu[r] = ok[r] ? g.col[off + j] : 0;
         ww[r] = ok[r] ? g.w[off + j] : 0.0;
+        ok[r] = ok[r] && pol.accept_e(v, g.col, off + j, kk[r], mm[r]);
       }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) ok[r] = ok[r] && pol.accept(v, uu[r], kk[r], mm[r]);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (!ok[r]) continue;
@@ -599,6 +660,95 @@ __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, 
   block_add_u64(cst, ncand);
 }
 
+// G lanes per vertex with a small LDS table of at most CAP slots: the
+// distinct neighbour communities of a vertex are usually far fewer than its
+// degree (R-MAT scale 24 after a batch: at most 36 for degrees 17-128, 56 for
+// 129-512), and small tables give more resident wavefronts and less to clear
+// and scan.  A vertex whose keys do not fit is appended to the overflow list
+// for the degree-sized tables instead.
+template <class P, u32 G, u32 CAP>
+__global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* ovf,
+                                                    u32* ovf_cnt, CandOut out, unsigned long long* cst) {
+  constexpr u32 NG = 256 / G;
+  __shared__ u32 hk[NG * CAP];
+  __shared__ double hv[NG * CAP];
+  __shared__ u32 gfull[NG];
+  __shared__ u32 stn[kStage], stt[kStage];
+  __shared__ double stg[kStage];
+  __shared__ u32 scnt, sbase;
+  if (threadIdx.x == 0) scnt = 0;
+  const u32 cnt = *cnt_dev;
+  const u32 lane = threadIdx.x % G, grp = threadIdx.x / G;
+  u32* K = hk + grp * CAP;
+  double* V = hv + grp * CAP;
+  unsigned long long ncand = 0;
+  for (u64 base = u64(blockIdx.x) * NG; base < cnt; base += u64(gridDim.x) * NG) {
+    const u64 id = base + grp;
+    const bool has = id < cnt;
+    u32 v = 0, cur = 0, deg = 0, cap = 16;
+    u64 off = 0;
+    if (has) {
+      v = list[id];
+      off = g.row_off[v];
+      deg = g.row_len[v];
+      cur = pol.cur_key(v);
+      while (cap < 2 * deg && cap < CAP) cap <<= 1;
+      for (u32 q = lane; q < cap; q += G) {
+        K[q] = kNone;
+        V[q] = 0.0;
+      }
+    }
+    if (lane == 0) gfull[grp] = 0;
+    __syncthreads();
+    if (has) {
+      for (u32 j = lane; j < deg; j += G) {
+        u32 key;
+        int mult;
+        if (!pol.accept_e(v, g.col, off + j, key, mult)) continue;
+        const double w = g.w[off + j];
+        const u32 h = lds_try_insert(K, cap - 1, key);
+        if (h == kNone) {
+          gfull[grp] = 1;
+          break;
+        }
+        atomicAdd(&V[h], w);
+        if (mult == 2) atomicAdd(&V[h], w);
+      }
+    }
+    __syncthreads();
+    const bool spill = has && gfull[grp] != 0;  // written before the barrier above, cleared after the next
+    if (spill && lane == 0) ovf[atomicAdd(ovf_cnt, 1u)] = v;
+    double bg = 0.0, wcur = 0.0;
+    u32 best = kNone;
+    if (has && !spill) {
+      wcur = lds_find(K, V, cap - 1, cur);
+      const double kv = g.k[v], sc = pol.cur_total(cur), tt = tm_of(g, v);
+      for (u32 q = lane; q < cap; q += G) {
+        const u32 key = K[q];
+        if (key == kNone) continue;
+        ++ncand;
+        if (key == cur || !pol.ok(key)) continue;
+        const double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+        if (gn > 0.0) better(bg, best, gn, key);
+      }
+    }
+    group_argmax<G>(bg, best);  // every lane of the wavefront takes part in the shuffles
+    if (has && !spill && lane == 0) {
+      finish_choice(pol, v, cur, wcur, g.k[v], tm_of(g, v), bg, best);
+      if (bg > 0.0) {
+        const u32 p = atomicAdd(&scnt, 1u);  // LDS
+        stn[p] = v;
+        stt[p] = best;
+        stg[p] = bg;
+      }
+    }
+    __syncthreads();
+    if (scnt > kStage - NG) flush_stage(stn, stt, stg, scnt, sbase, out);
+  }
+  flush_stage(stn, stt, stg, scnt, sbase, out);
+  block_add_u64(cst, ncand);
+}
+
 // Hubs whose keys overflowed the LDS table: one workgroup per vertex; each
 // resident workgroup owns an HBM table of tcap slots (cleared per vertex: coalesced, cheaper than tracking the
 // used slots, whose clears and scans are random accesses)
@@ -636,7 +786,7 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
     __syncthreads();
     // four row entries per thread in flight: the gathers are independent
     for (u32 j0 = threadIdx.x; j0 < deg; j0 += 4 * blockDim.x) {
-      u32 uu[4], kk[4];
+      u32 kk[4];
       double ww[4];
       int mm[4];
       bool ok[4];
@@ -644,11 +794,9 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
       for (int r = 0; r < 4; ++r) {
         const u32 j = j0 + r * blockDim.x;
         ok[r] = j < deg;
-        uu[r] = ok[r] ? g.col[off + j] : 0;
         ww[r] = ok[r] ? g.w[off + j] : 0.0;
+        ok[r] = ok[r] && pol.accept_e(v, g.col, off + j, kk[r], mm[r]);
       }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) ok[r] = ok[r] && pol.accept(v, uu[r], kk[r], mm[r]);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (!ok[r]) continue;
@@ -666,7 +814,7 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
           atomicAdd(&LV[h], ww[r]);
           if (mm[r] == 2) atomicAdd(&LV[h], ww[r]);
         } else {
-          const u32 hh = tab_insert(K, cap, kk[r]);
+          const u32 hh = hbm_insert(K, cap, kk[r]);
           atomicAdd(&V[hh], ww[r]);
           if (mm[r] == 2) atomicAdd(&V[hh], ww[r]);
         }
@@ -676,7 +824,7 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
     for (u32 q = threadIdx.x; q < kCacheSlots; q += blockDim.x) {
       const u32 key = LK[q];
       if (key == kNone) continue;
-      const u32 hh = tab_insert(K, cap, key);
+      const u32 hh = hbm_insert(K, cap, key);
       atomicAdd(&V[hh], LV[q]);
     }
     __threadfence();
@@ -732,6 +880,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
     if (tid == 0) {
       status[0] = 1;  // multi-kernel path
       *flat.c = 0;
+      if (m.ck) *flat.ck_c = 0;
     }
     return;
   }
@@ -1003,7 +1152,9 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
       if (want) next[q] = u;
     }
     if (tid == 0) *flat.c = 0;
-  } else {
+  }
+  if (total > kMarksInBlock || m.ck) {
+    // the moves for k_marks_flat (flat.c set) and for k_ck_flat (ck_c set)
     for (u32 p = tid; p < c; p += blockDim.x) {
       flat.node[p] = snode[p];
       flat.to[p] = sto[p];
@@ -1011,7 +1162,8 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
     }
     if (tid == 0) {
       flat.pre[c] = total;
-      *flat.c = c;
+      if (total > kMarksInBlock) *flat.c = c;
+      if (m.ck) *flat.ck_c = c;
     }
   }
   if (tid == 0 && total) atomicAdd(&dst[total <= kMarksInBlock ? DS_DEC_DEG : DS_MULTI_DEG], (unsigned long long)total);
@@ -1022,7 +1174,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
     // the next round starts from zero counts without a host-side reset
     *cand.cnt = 0;
     *cur_cnt = 0;
-    for (u32 i = 0; i < kBuckets + 2; ++i) bucket_counts[i] = 0;  // + the two overflow counts
+    clear_bucket_counts(bucket_counts);
     atomicAdd(&dst[DS_DEC_CAND], (unsigned long long)c);
     atomicAdd(&dst[DS_DEC_ACC], (unsigned long long)nacc);
   }
@@ -1088,6 +1240,40 @@ __global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u
     for (u32 i = threadIdx.x; i < scnt; i += blockDim.x) next[sbase + i] = sv_[i];
   }
 }
+// The moves' cache updates over the grid: entry t of the moved vertices'
+// concatenated rows (one thread each) updates v's entry in neighbour u's row.
+__global__ __launch_bounds__(256) void k_ck_flat(MoveCtx m, FlatMarks flat) {
+  __shared__ u32 span[2];
+  const u32 c = *flat.ck_c;
+  if (c == 0) return;
+  const u64 total = flat.pre[c];
+  for (u64 base = u64(blockIdx.x) * blockDim.x; base < total; base += u64(gridDim.x) * blockDim.x) {
+    const u64 t = base + threadIdx.x;
+    if (threadIdx.x == 0 || threadIdx.x == blockDim.x - 1) {
+      const u64 e = threadIdx.x == 0 ? base : (t < total ? t : total - 1);
+      u32 lo = 0, hi = c;
+      while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (flat.pre[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      span[threadIdx.x == 0 ? 0 : 1] = lo;
+    }
+    __syncthreads();
+    if (t < total) {
+      u32 lo = span[0], hi = span[1] + 1;
+      while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (flat.pre[mid] <= t) lo = mid;
+        else hi = mid;
+      }
+      const u32 v = flat.node[lo];
+      const u32 u = m.g.col[m.g.row_off[v] + (t - flat.pre[lo])];
+      ck_update(m, u, v, flat.to[lo]);
+    }
+    __syncthreads();
+  }
+}
 __global__ void k_acc_degrees(u64 c, const u32* node, const u32* state, const u32* row_len, u64* deg) {
   for (u64 p = gtid(); p < c; p += gstride()) deg[p] = state[p] == kAcc ? row_len[node[p]] : 0;
 }
@@ -1097,7 +1283,7 @@ __global__ void k_round_reset(u32* cur_cnt, u32* ccnt, u32* bucket_counts) {
   if (threadIdx.x == 0) {
     if (cur_cnt) *cur_cnt = 0;
     if (ccnt) *ccnt = 0;
-    for (u32 i = 0; i < kBuckets + 2; ++i) bucket_counts[i] = 0;  // + the two overflow counts
+    clear_bucket_counts(bucket_counts);
   }
 }
 
@@ -1292,7 +1478,7 @@ void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
   cnode.reserve(n + 1, s);
   cto.reserve(n + 1, s);
   cgain.reserve(n + 1, s);
-  counters.reserve(16, s);
+  counters.reserve(32, s);
   mnode.reserve(kDecSmall + 1, s);
   mto.reserve(kDecSmall + 1, s);
   mpre.reserve(kDecSmall + 2, s);
@@ -1305,7 +1491,7 @@ void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
 }
 
 void MoveScratch::trim() {
-  for (DevBuf<u32>* b : {&tk, &hovf, &k32, &k32b, &v32, &o1, &o2, &node, &from, &to, &state, &acc}) b->release();
+  for (DevBuf<u32>* b : {&tk, &hovf, &ck, &k32, &k32b, &v32, &o1, &o2, &node, &from, &to, &state, &acc}) b->release();
   tv.release();
   k64.release();
   k64b.release();
@@ -1322,8 +1508,8 @@ u32 MoveScratch::next_epoch(hipStream_t s) {
 template <class P>
 static void propose_round(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u64 list_cap, u32 maxdeg,
                           MoveScratch& ms, const Thresholds& th, unsigned long long* st, hipStream_t s) {
-  // [4..4 + kBuckets) bucket counts, then the overflow counts of the hub and
-  // LDS-workgroup tables; all zero on entry
+  // counters [4..4 + kBuckets) bucket counts, [kOvfSlot..kOvfSlot + kOvfLists)
+  // overflow counts; all zero on entry
   u32* counts = ms.counters.p + 4;
   const u32 small = th.small_deg > SMALL_MAX2 ? SMALL_MAX2 : th.small_deg;
   const u32 wave = th.wave_deg > WAVE_HW2 / 2 ? WAVE_HW2 / 2 : th.wave_deg;
@@ -1336,42 +1522,48 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
                        ms.blists.p, list_cap, counts, out, st);
     HL_LAUNCH_CHECK();
   }
+  // each bucket past the first: small tables first, then the tables sized for
+  // the degree for the vertices listed as overflowing them
+  u32* ovf_cnt = ms.counters.p + kOvfSlot;
+  if (maxdeg > small) ms.hovf.reserve(kOvfLists * list_cap, s);
+  u32* ovf[kOvfLists];
+  for (u32 i = 0; i < kOvfLists; ++i) ovf[i] = ms.hovf.p + i * list_cap;
   if (maxdeg > small) {
     HL_PROF(P_PROP_WAVE, 0, s);
+    hipLaunchKernelGGL((k_prop2_tab<P, 32, 64>), dim3(fixed_grid(list_cap, 8, kTabBlocks)), dim3(256), 0, s, pol, g,
+                       ms.blists.p + list_cap, counts + 1, ovf[0], ovf_cnt + 0, out, cst + 1);
+    HL_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_prop2_wave<P, 32, 2 * kHalfDeg>), dim3(fixed_grid(list_cap, 8, kHalfBlocks)), dim3(256), 0,
-                       s, pol, g, ms.blists.p + list_cap, counts + 1, out, cst + 1);
+                       s, pol, g, ovf[0], ovf_cnt + 0, out, cst + 1);
     HL_LAUNCH_CHECK();
     if (maxdeg > half) {
+      hipLaunchKernelGGL((k_prop2_tab<P, 64, 128>), dim3(fixed_grid(list_cap, 4, kTabBlocks)), dim3(256), 0, s, pol,
+                         g, ms.blists.p + 2 * list_cap, counts + 2, ovf[1], ovf_cnt + 1, out, cst + 2);
+      HL_LAUNCH_CHECK();
       hipLaunchKernelGGL((k_prop2_wave<P, 64, WAVE_HW2>), dim3(fixed_grid(list_cap, 4, kWaveBlocks)), dim3(256), 0, s,
-                         pol, g, ms.blists.p + 2 * list_cap, counts + 2, out, cst + 2);
+                         pol, g, ovf[1], ovf_cnt + 1, out, cst + 2);
       HL_LAUNCH_CHECK();
     }
   }
-  if (maxdeg > wave) ms.hovf.reserve(2 * list_cap, s);
   if (maxdeg > wave) {
-    // a small LDS table per vertex first (six workgroups per CU); vertices
-    // with more distinct keys than it holds take the table sized for the degree
     HL_PROF(P_PROP_LDS, 0, s);
-    u32* ovf_cnt = counts + kBuckets + 1;
-    u32* ovf = ms.hovf.p + list_cap;
-    hipLaunchKernelGGL((k_prop2_hub<P, 256, 2048>), dim3(1536), dim3(256), 0, s, pol, g, ms.blists.p + 3 * list_cap,
-                       counts + 3, ovf, ovf_cnt, out, cst + 3);
+    hipLaunchKernelGGL((k_prop2_hub<P, 256, 2048>), dim3(1536), dim3(256), 0, s, pol, g,
+                       ms.blists.p + 3 * list_cap, counts + 3, ovf[2], ovf_cnt + 2, out, cst + 3);
     HL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_prop2_lds<P>, dim3(fixed_grid(list_cap, 1, 256)), dim3(1024), 0, s, pol, g, ovf, ovf_cnt,
-                       out, cst + 3);
+    hipLaunchKernelGGL(k_prop2_lds<P>, dim3(fixed_grid(list_cap, 1, 256)), dim3(1024), 0, s, pol, g, ovf[2],
+                       ovf_cnt + 2, out, cst + 3);
     HL_LAUNCH_CHECK();
   }
   if (maxdeg > kLdsDeg) {
-    // hubs: LDS tables first; the vertices whose keys overflow them go to HBM
-    // tables (one per resident workgroup, at most kBigTableBytes in all)
+    // hubs: LDS tables first, then HBM tables (one per resident workgroup, at
+    // most kBigTableBytes in all) for the vertices whose keys overflow them
     HL_PROF(P_PROP_BIG, 0, s);
-    u32* ovf_cnt = counts + kBuckets;
-    hipLaunchKernelGGL((k_prop2_hub<P, 512, 4096>), dim3(768), dim3(512), 0, s, pol, g, ms.blists.p + 4 * list_cap,
-                       counts + 4, ms.hovf.p, ovf_cnt, out, cst + 4);
+    hipLaunchKernelGGL((k_prop2_hub<P, 512, 4096>), dim3(768), dim3(512), 0, s, pol, g,
+                       ms.blists.p + 4 * list_cap, counts + 4, ovf[3], ovf_cnt + 3, out, cst + 4);
     HL_LAUNCH_CHECK();
     if (maxdeg > kBigDeg) {
       hipLaunchKernelGGL((k_prop2_hub<P, 1024, 8192>), dim3(256), dim3(1024), 0, s, pol, g,
-                         ms.blists.p + 5 * list_cap, counts + 5, ms.hovf.p, ovf_cnt, out, cst + 5);
+                         ms.blists.p + 5 * list_cap, counts + 5, ovf[3], ovf_cnt + 3, out, cst + 5);
       HL_LAUNCH_CHECK();
     }
     const u32 tcap = big_cap(maxdeg);
@@ -1379,27 +1571,34 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
     const unsigned nb = unsigned(std::max<u64>(1, std::min<u64>(kOvfBlocks, kBigTableBytes / per)));
     ms.tk.reserve(u64(nb) * tcap, s);
     ms.tv.reserve(u64(nb) * tcap, s);
-    hipLaunchKernelGGL(k_prop2_big<P>, dim3(nb), dim3(1024), 0, s, pol, g, ms.hovf.p, ovf_cnt, ms.tk.p, ms.tv.p,
+    hipLaunchKernelGGL(k_prop2_big<P>, dim3(nb), dim3(1024), 0, s, pol, g, ovf[3], ovf_cnt + 3, ms.tk.p, ms.tv.p,
                        tcap, out, cst + 4);
     HL_LAUNCH_CHECK();
   }
 }
 
-u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int max_rounds, MoveScratch& ms,
+u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, int max_rounds, MoveScratch& ms,
                         const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s) {
+  MoveCtx m = m_in;
   const u32 n = m.g.n;
   ms.ensure(n, m.comm_cap, s);
+  if (m.pool) {
+    ms.ck.reserve(m.pool, s);
+    m.ck = ms.ck.p;
+    hipLaunchKernelGGL(k_key_cache, fixed_grid(n, 256, 4096), 256, 0, s, m.g, m.f, m.ck);
+    HL_LAUNCH_CHECK();
+  }
   // counters: [0], [2] list counts (alternating), [1] candidates, [4..10) bucket
-  // counts, [10], [11] overflow counts of the hub and LDS-workgroup tables,
-  // [13] flat mark count, [14] overflow flag, [15] accepted moves; a round leaves all but the
-  // next list's count at zero
+  // counts, [13] flat mark count, [14] overflow flag, [15] accepted moves,
+  // [16..20) table overflow counts; a round leaves all but the next list's
+  // count at zero
   u32* lcnt[2] = {ms.counters.p + 0, ms.counters.p + 2};
   u32* ccnt = ms.counters.p + 1;
   u32* bcounts = ms.counters.p + 4;
   u32* status = ms.counters.p + 14;
   int ci = 0;
   u32* cnt_a = lcnt[0];
-  HL_CHECK(hipMemsetAsync(ms.counters.p, 0, 16 * sizeof(u32), s));
+  HL_CHECK(hipMemsetAsync(ms.counters.p, 0, 32 * sizeof(u32), s));
   u32* cur = ms.list_a.p;
   u32* nxt = ms.list_b.p;
   if (k == 0 && !m.gid) {
@@ -1418,7 +1617,7 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
       HL_LAUNCH_CHECK();
     }
   }
-  MovePolicyT pol{m.f, m.tot, m.gamma, n};
+  MovePolicyT pol{m.f, m.tot, m.gamma, n, m.ck};
   u32 host_cnt = 0;
   d2h_async(&host_cnt, cnt_a, sizeof(u32), s);
   stream_sync(s);
@@ -1431,11 +1630,16 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
     propose_round(pol, m.g, cur, cur_cnt, host_cnt, maxdeg, ms, th, ms.dstat.p + DS_MOVE, s);
     const u32 ep = ms.next_epoch(s);
     CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
-    const FlatMarks fm{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13};
+    const FlatMarks fm{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 12};
     {
       HL_PROF(P_DECOUPLE, 0, s);
       hipLaunchKernelGGL(k_dec_small, 1, 1024, 0, s, m, cand, ms.minUto.p, ms.minUfrom.p, ms.minAto.p,
                          ms.minAfrom.p, ms.mark.p, ep, nxt, next_cnt, status, ms.dstat.p, cur_cnt, bcounts, fm);
+      HL_LAUNCH_CHECK();
+    }
+    if (m.ck) {
+      HL_PROF(P_MARK_MOVED, 0, s);
+      hipLaunchKernelGGL(k_ck_flat, kMarksBlocks, 256, 0, s, m, fm);
       HL_LAUNCH_CHECK();
     }
     u32 h[16];
@@ -1518,9 +1722,13 @@ u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int m
         HL_LAUNCH_CHECK();
         exclusive_scan_u64_async(ms.mdeg.p, ms.mpre.p, c, tmp, s);
         fill_u32(ms.counters.p + 13, c, 1, s);
-        const FlatMarks fmm{ms.node.p, ms.to.p, ms.mpre.p, ms.counters.p + 13};
+        const FlatMarks fmm{ms.node.p, ms.to.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 13};
         hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fmm, ms.mark.p, ep, nxt, next_cnt);
         HL_LAUNCH_CHECK();
+        if (m.ck) {
+          hipLaunchKernelGGL(k_ck_flat, kMarksBlocks, 256, 0, s, m, fmm);
+          HL_LAUNCH_CHECK();
+        }
       }
       d2h_async(&accepted, acc_cnt, sizeof(u32), s);
       d2h_async(&next_host, next_cnt, sizeof(u32), s);

@@ -72,7 +72,9 @@ typedef struct {
   uint32_t small_degree; /* autotuned bucket thresholds; 0 = built-in defaults */
   uint32_t wave_degree;
   uint32_t load_pass_records; /* hl_load_graphs: edges per loading pass, 0 = 2^26 */
-  uint32_t reserved[5];
+  uint32_t key_cache;  /* throughput movement: cache of each row entry's neighbour community,
+                          kept current by the moves; 0 = off, 1 = on */
+  uint32_t reserved[4];
 } hl_config;
 
 #define HL_MAX_LEVELS 8

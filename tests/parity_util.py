@@ -16,8 +16,8 @@ def oracle_mode(mode):
     return O.DET if mode == hl.RunMode.DETERMINISTIC else O.PAR
 
 
-def pair(mode, levels, gamma=1.0):
-    g = hl.HitLeiden(hl.RunConfig(resolution=gamma, mode=mode, levels=levels))
+def pair(mode, levels, gamma=1.0, **cfg):
+    g = hl.HitLeiden(hl.RunConfig(resolution=gamma, mode=mode, levels=levels, **cfg))
     o = O.OracleState(levels=levels, gamma=gamma, mode=oracle_mode(mode))
     return g, o
 

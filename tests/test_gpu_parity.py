@@ -309,13 +309,14 @@ def test_hub_rows_use_big_bucket(leaves):
         g.close()
 
 
-@pytest.mark.parametrize("hub_deg,n", [(2000, 30000), (12000, 30000), (280000, 300000)])
+@pytest.mark.parametrize("hub_deg,n", [(100, 30000), (300, 30000), (2000, 30000), (12000, 30000), (280000, 300000)])
 def test_hubs_over_few_communities(hub_deg, n):
     """High-degree vertices whose neighbours fall in a few communities (an SBM
-    with 20 blocks): after the first rounds their keys fit the small LDS tables;
-    in the first rounds (singleton communities) they overflow to the tables
-    sized for the degree (LDS for 2,000, HBM for 12,000).  The last case has a
-    hub above 2^18 neighbours (the giant bucket)."""
+    with 20 blocks): after the first rounds their keys fit the small LDS tables
+    of their bucket (half wavefront, wavefront, workgroup, hub); in the first
+    rounds (singleton communities) they overflow to the tables sized for the
+    degree (LDS up to 4,096, HBM above).  The last case has a hub above 2^18
+    neighbours (the giant bucket)."""
     w = gen.sbm(n, 20, 8.0, 0.2, seed=5)
     rng = np.random.default_rng(6)
     hubs = [np.full(hub_deg, h, np.uint32) for h in (0, 1, 2)]
