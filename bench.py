@@ -29,13 +29,17 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "incremental edges/s + p99 update latency vs plain GPU path; HBM GB/s"
 
 # profiler slots that bracket exactly one kernel (both template instances for
-# the proposal kernels), with the kernel name rocprofv3 reports; the dominant
-# kernel of the roofline is the one of these with the most time
+# the proposal kernels: movement and refinement), with the name rocprofv3
+# reports; the dominant kernel of the roofline is the one of these with the
+# most time
 SLOT_KERNEL = {
-    "propose_small": "k_prop2_first", "propose_wave": "k_prop2_wave", "propose_lds": "k_prop2_lds",
-    "propose_big": "k_prop2_big",
-    "decouple": "k_dec_small", "apply_moves": "k_apply4", "mark_moved": "k_marks4",
-    "row_merge": "k_row_merge", "resolve_runs": "k_resolve_runs", "row_degree": "k_row_degree",
+    "propose_small": "k_prop2_first", "try_half": "k_prop2_tab<hl::(Move|Refine)PolicyT, 32u?, 64u?>",
+    "try_wave": "k_prop2_tab<hl::(Move|Refine)PolicyT, 64u?, 128u?>",
+    "try_lds": "k_prop2_hub<hl::(Move|Refine)PolicyT, 256u?, 2048u?>", "propose_lds": "k_prop2_lds",
+    "try_hub": "k_prop2_hub<hl::(Move|Refine)PolicyT, 512u?, 4096u?>",
+    "try_giant": "k_prop2_hub<hl::(Move|Refine)PolicyT, 1024u?, 8192u?>", "propose_big": "k_prop2_big",
+    "decouple": "k_dec_small", "apply_moves": "k_apply4",
+    "row_merge": "k_row_merge", "resolve_runs": "k_resolve_runs",
 }
 
 
@@ -288,6 +292,8 @@ def main():
                     help="JSON {slot: HBM bytes per launch} from the FETCH_SIZE / WRITE_SIZE counter runs "
                          "(default: profiles/traffic_<config>.json when present)")
     ap.add_argument("--dump-profile", default=None)
+    ap.add_argument("--key-cache", action="store_true",
+                    help="cache each row entry's neighbour community for the throughput movement")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the degree-bucket thresholds on the warmup batches first")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -339,7 +345,7 @@ def main():
                                             levels=args.levels, gamma=args.gamma, log=log)
         log(f"[bench] autotune picked small_degree={small_deg} wave_degree={wave_deg} in {time.time() - t0:.1f}s")
     g = hl.HitLeiden(hl.RunConfig(resolution=args.gamma, mode=mode, levels=args.levels, small_degree=small_deg,
-                                  wave_degree=wave_deg))
+                                  wave_degree=wave_deg, key_cache=int(args.key_cache)))
     t0 = time.time()
     ls = g.load(hl.GraphInput(w.n, w.u, w.v, w.w))
     load_s = time.time() - t0
@@ -400,7 +406,9 @@ def main():
             traffic = tj.get(dominant, {}).get("bytes_per_launch") if isinstance(tj.get(dominant), dict) \
                 else tj.get(dominant)
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": SLOT_KERNEL[dominant],
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": SLOT_KERNEL[dominant].replace("hl::(Move|Refine)PolicyT, ", "").replace("u?", ""),
+                "kernel_regex": SLOT_KERNEL[dominant],
                 "slot": dominant,
                 "launches": launches, "avg_launch_us": round(avg_ms * 1000.0, 3),
                 "alg_bytes_per_launch": round(per_launch_bytes, 1)}
@@ -410,7 +418,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(float(lat_ms.mean()), 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": dict(desc, levels=args.levels, mode=args.mode, gamma=args.gamma,
-                       small_degree=small_deg or 16, wave_degree=wave_deg or 512),
+                       small_degree=small_deg or 16, wave_degree=wave_deg or 512,
+                       key_cache=int(args.key_cache)),
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "path_gbs": round(bytes_alg / total / 1e9, 3), "path_hbm_frac": round(bytes_alg / total / 1e9 / HBM_PEAK_GBS, 6),
         "modularity_after_load": round(q0, 6), "modularity_after_steps": round(q1, 6), "load_seconds": round(load_s, 3),

@@ -23,6 +23,8 @@ struct Level {
   DevBuf<double> k;
   DevBuf<u32> col;
   DevBuf<double> w;
+  DevBuf<u32> col_spare;  // the other half of the double-buffered row pool (compaction target)
+  DevBuf<double> w_spare;
   u64 pool_used = 0;
   DevBuf<double> twice_m;
   DevBuf<u32> f, g, s_pre, s_cur;
@@ -199,6 +201,10 @@ struct Engine {
   }
   void trim() {
     stream_sync(s);
+    for (auto& l : L) {
+      l->col_spare.release();
+      l->w_spare.release();
+    }
     renew(as);
     renew(ag);
     renew(rs);
@@ -231,23 +237,24 @@ struct Engine {
     apply_delta_plan(d, k, g, as, tmp, plan, s);
     if (plan.nseg) {
       if (lv.pool_used + plan.need > lv.col.cap) {
-        // compact live rows into a fresh pool with room for this batch and
-        // headroom: a quarter of the live entries, or up to three more batches
-        // that rewrite as many rows as this one, within half the live entries
-        // (merged rows move to the pool's end, and hub-heavy batches rewrite a
-        // large share of the entries)
+        // compact live rows into the spare pool (kept across batches, so no
+        // allocation once it has grown) with room for this batch and headroom:
+        // a quarter of the live entries, or three more batches that rewrite as
+        // many rows as this one, within twice the live entries (merged rows
+        // move to the pool's end, and hub-heavy batches rewrite a large share
+        // of the entries)
         const u64 live = compact_plan(g, offtmp, tmp, s);
-        {
-          DevBuf<u32> col2;
-          DevBuf<double> w2;
-          const u64 room = std::max(live / 4, std::min(3 * plan.need, live / 2));
-          const u64 want = std::max<u64>(live + plan.need + room, 1024);
-          col2.reserve(want, s, false, 1.0);
-          w2.reserve(want, s, false, 1.0);
-          compact_copy(g, offtmp.p, col2.p, w2.p, s);
-          lv.col.swap(col2);
-          lv.w.swap(w2);
-        }  // the old pool is released here, after compact_copy synchronised
+        const u64 room = std::max(live / 4, std::min(3 * plan.need, 2 * live));
+        const u64 want = std::max<u64>(live + plan.need + room, 1024);
+        if (lv.col_spare.cap < live + plan.need + room / 2) {
+          lv.col_spare.release();
+          lv.w_spare.release();
+          lv.col_spare.reserve(want, s, false, 1.0);
+          lv.w_spare.reserve(want, s, false, 1.0);
+        }
+        compact_copy(g, offtmp.p, lv.col_spare.p, lv.w_spare.p, s);
+        lv.col.swap(lv.col_spare);
+        lv.w.swap(lv.w_spare);
         lv.pool_used = live;
         g = lv.mut();
       }
@@ -420,10 +427,14 @@ struct Engine {
   void account_work(const unsigned long long* dnow, hl_batch_stats* st) {
     u64 d[DS_COUNT];
     for (u32 i = 0; i < DS_COUNT; ++i) d[i] = dnow[i] - dlast[i], dlast[i] = dnow[i];
-    u64 prop[kBuckets], total = 0;
+    u64 prop[kBuckets], tried[kBuckets], total = 0;
     for (u32 b = 0; b < kBuckets; ++b) {
       prop[b] = 20 * d[DS_MOVE + b] + 16 * d[DS_MOVE + kBuckets + b] + 8 * d[DS_MOVE + 2 * kBuckets + b] +
                 24 * d[DS_REF + b] + 20 * d[DS_REF + kBuckets + b] + 16 * d[DS_REF + 2 * kBuckets + b];
+      // the part that the bucket's first-try kernel evaluated
+      const u32 tm = DS_TRY_MOVE + 3 * b, tr = DS_TRY_REF + 3 * b;
+      tried[b] = 20 * d[tm] + 16 * d[tm + 1] + 8 * d[tm + 2] + 24 * d[tr] + 20 * d[tr + 1] + 16 * d[tr + 2];
+      if (tried[b] > prop[b]) tried[b] = prop[b];
       total += prop[b];
     }
     const u64 dec = 20 * d[DS_DEC_CAND] + 28 * d[DS_DEC_ACC] + 12 * d[DS_DEC_DEG];
@@ -431,10 +442,12 @@ struct Engine {
     total += dec + multi + apply + marks;
     if (st) st->bytes_alg += total;
     if (prof.on) {
-      const int slots[8] = {P_PROP_SMALL, P_PROP_WAVE, P_PROP_LDS, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI,
-                            P_APPLY_MOVES, P_MARK_MOVED};
-      const u64 by[8] = {prop[0], prop[1] + prop[2], prop[3], prop[4] + prop[5], dec, multi, apply, marks};
-      for (int i = 0; i < 8; ++i)
+      const int slots[13] = {P_PROP_SMALL, P_TRY_HALF, P_TRY_WAVE, P_PROP_WAVE, P_TRY_LDS, P_PROP_LDS, P_TRY_HUB,
+                             P_TRY_GIANT, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI, P_APPLY_MOVES, P_MARK_MOVED};
+      const u64 by[13] = {prop[0], tried[1], tried[2], prop[1] + prop[2] - tried[1] - tried[2], tried[3],
+                          prop[3] - tried[3], tried[4], tried[5], prop[4] + prop[5] - tried[4] - tried[5],
+                          dec, multi, apply, marks};
+      for (int i = 0; i < 13; ++i)
         if (prof.records(slots[i])) prof.bytes[slots[i]] += by[i];
     }
   }

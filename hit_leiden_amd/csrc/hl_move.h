@@ -117,7 +117,11 @@ enum DevStat : u32 {
   DS_MULTI_CAND,               // the same for the multi-kernel path
   DS_MULTI_ACC,
   DS_MULTI_DEG,
-  DS_COUNT
+  // vertices, row entries and candidates that the first-try (small-table)
+  // kernel of each bucket evaluated without spilling to its overflow kernel
+  DS_TRY_MOVE,
+  DS_TRY_REF = DS_TRY_MOVE + 3 * kBuckets,
+  DS_COUNT = DS_TRY_REF + 3 * kBuckets
 };
 
 struct MoveScratch {

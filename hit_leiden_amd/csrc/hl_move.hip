@@ -39,7 +39,7 @@ __device__ __forceinline__ void clear_bucket_counts(u32* bucket_counts) {
 constexpr u32 kLdsDeg = 4096;       // workgroup-per-vertex with an LDS table up to this degree
 constexpr u32 kLdsCap = 8192;       // its table slots (big_cap(kLdsDeg))
 constexpr u32 kBigDeg = 1u << 18;   // workgroup-per-vertex with an HBM table up to this degree
-constexpr u64 kMarksInBlock = 8192;  // neighbour entries marked inside the decoupling workgroup
+constexpr u64 kMarksInBlock = 2048;  // neighbour entries marked inside the decoupling workgroup
 constexpr unsigned kMarksBlocks = 1024;
 constexpr u32 kMarkStage = 2048;
 constexpr u64 kBigTableBytes = 3ull << 30;  // HBM hub tables of the overflow proposals
@@ -584,12 +584,13 @@ __device__ __forceinline__ double lds_find(const u32* K, const double* V, u32 ma
 
 template <class P, u32 T, u32 S>
 __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* ovf,
-                                                 u32* ovf_cnt, CandOut out, unsigned long long* cst) {
+                                                 u32* ovf_cnt, CandOut out, unsigned long long* cst,
+                                                 unsigned long long* tst) {
   __shared__ u32 K[S];
   __shared__ double V[S];
   __shared__ u32 full;
   const u32 cnt = *cnt_dev;
-  unsigned long long ncand = 0;
+  unsigned long long ncand = 0, nv = 0, ne = 0;
   for (u32 id = blockIdx.x; id < cnt; id += gridDim.x) {
     const u32 v = list[id];
     const u64 off = g.row_off[v];
@@ -647,6 +648,8 @@ __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, 
     }
     block_argmax(bg, best);
     if (threadIdx.x == 0) {
+      nv += 1;
+      ne += deg;
       finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
       if (bg > 0.0) {
         const u32 p = atomicAdd(out.cnt, 1u);
@@ -658,6 +661,9 @@ __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, 
     __syncthreads();
   }
   block_add_u64(cst, ncand);
+  const u32 idx[3] = {0, 1, 2};
+  unsigned long long vals[3] = {nv, ne, ncand};
+  block_add_multi(tst, idx, vals);
 }
 
 // G lanes per vertex with a small LDS table of at most CAP slots: the
@@ -668,7 +674,8 @@ __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, 
 // for the degree-sized tables instead.
 template <class P, u32 G, u32 CAP>
 __global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* ovf,
-                                                    u32* ovf_cnt, CandOut out, unsigned long long* cst) {
+                                                    u32* ovf_cnt, CandOut out, unsigned long long* cst,
+                                                    unsigned long long* tst) {
   constexpr u32 NG = 256 / G;
   __shared__ u32 hk[NG * CAP];
   __shared__ double hv[NG * CAP];
@@ -681,7 +688,7 @@ __global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list
   const u32 lane = threadIdx.x % G, grp = threadIdx.x / G;
   u32* K = hk + grp * CAP;
   double* V = hv + grp * CAP;
-  unsigned long long ncand = 0;
+  unsigned long long ncand = 0, nv = 0, ne = 0;
   for (u64 base = u64(blockIdx.x) * NG; base < cnt; base += u64(gridDim.x) * NG) {
     const u64 id = base + grp;
     const bool has = id < cnt;
@@ -734,6 +741,8 @@ __global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list
     }
     group_argmax<G>(bg, best);  // every lane of the wavefront takes part in the shuffles
     if (has && !spill && lane == 0) {
+      nv += 1;
+      ne += deg;
       finish_choice(pol, v, cur, wcur, g.k[v], tm_of(g, v), bg, best);
       if (bg > 0.0) {
         const u32 p = atomicAdd(&scnt, 1u);  // LDS
@@ -747,6 +756,9 @@ __global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list
   }
   flush_stage(stn, stt, stg, scnt, sbase, out);
   block_add_u64(cst, ncand);
+  const u32 idx[3] = {0, 1, 2};
+  unsigned long long vals[3] = {nv, ne, ncand};
+  block_add_multi(tst, idx, vals);
 }
 
 // Hubs whose keys overflowed the LDS table: one workgroup per vertex; each
@@ -1507,9 +1519,11 @@ u32 MoveScratch::next_epoch(hipStream_t s) {
 
 template <class P>
 static void propose_round(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u64 list_cap, u32 maxdeg,
-                          MoveScratch& ms, const Thresholds& th, unsigned long long* st, hipStream_t s) {
+                          MoveScratch& ms, const Thresholds& th, unsigned long long* st, unsigned long long* tst,
+                          hipStream_t s) {
   // counters [4..4 + kBuckets) bucket counts, [kOvfSlot..kOvfSlot + kOvfLists)
-  // overflow counts; all zero on entry
+  // overflow counts; all zero on entry.  st: the proposal block of the device
+  // work counters; tst: the first-try block (3 counters per bucket)
   u32* counts = ms.counters.p + 4;
   const u32 small = th.small_deg > SMALL_MAX2 ? SMALL_MAX2 : th.small_deg;
   const u32 wave = th.wave_deg > WAVE_HW2 / 2 ? WAVE_HW2 / 2 : th.wave_deg;
@@ -1529,27 +1543,40 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
   u32* ovf[kOvfLists];
   for (u32 i = 0; i < kOvfLists; ++i) ovf[i] = ms.hovf.p + i * list_cap;
   if (maxdeg > small) {
-    HL_PROF(P_PROP_WAVE, 0, s);
-    hipLaunchKernelGGL((k_prop2_tab<P, 32, 64>), dim3(fixed_grid(list_cap, 8, kTabBlocks)), dim3(256), 0, s, pol, g,
-                       ms.blists.p + list_cap, counts + 1, ovf[0], ovf_cnt + 0, out, cst + 1);
-    HL_LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_prop2_wave<P, 32, 2 * kHalfDeg>), dim3(fixed_grid(list_cap, 8, kHalfBlocks)), dim3(256), 0,
-                       s, pol, g, ovf[0], ovf_cnt + 0, out, cst + 1);
-    HL_LAUNCH_CHECK();
-    if (maxdeg > half) {
-      hipLaunchKernelGGL((k_prop2_tab<P, 64, 128>), dim3(fixed_grid(list_cap, 4, kTabBlocks)), dim3(256), 0, s, pol,
-                         g, ms.blists.p + 2 * list_cap, counts + 2, ovf[1], ovf_cnt + 1, out, cst + 2);
+    {
+      HL_PROF(P_TRY_HALF, 0, s);
+      hipLaunchKernelGGL((k_prop2_tab<P, 32, 64>), dim3(fixed_grid(list_cap, 8, kTabBlocks)), dim3(256), 0, s, pol,
+                         g, ms.blists.p + list_cap, counts + 1, ovf[0], ovf_cnt + 0, out, cst + 1, tst + 3 * 1);
       HL_LAUNCH_CHECK();
+    }
+    {
+      HL_PROF(P_PROP_WAVE, 0, s);
+      hipLaunchKernelGGL((k_prop2_wave<P, 32, 2 * kHalfDeg>), dim3(fixed_grid(list_cap, 8, kHalfBlocks)), dim3(256),
+                         0, s, pol, g, ovf[0], ovf_cnt + 0, out, cst + 1);
+      HL_LAUNCH_CHECK();
+    }
+    if (maxdeg > half) {
+      {
+        HL_PROF(P_TRY_WAVE, 0, s);
+        hipLaunchKernelGGL((k_prop2_tab<P, 64, 128>), dim3(fixed_grid(list_cap, 4, kTabBlocks)), dim3(256), 0, s,
+                           pol, g, ms.blists.p + 2 * list_cap, counts + 2, ovf[1], ovf_cnt + 1, out, cst + 2,
+                           tst + 3 * 2);
+        HL_LAUNCH_CHECK();
+      }
+      HL_PROF(P_PROP_WAVE, 0, s);
       hipLaunchKernelGGL((k_prop2_wave<P, 64, WAVE_HW2>), dim3(fixed_grid(list_cap, 4, kWaveBlocks)), dim3(256), 0, s,
                          pol, g, ovf[1], ovf_cnt + 1, out, cst + 2);
       HL_LAUNCH_CHECK();
     }
   }
   if (maxdeg > wave) {
+    {
+      HL_PROF(P_TRY_LDS, 0, s);
+      hipLaunchKernelGGL((k_prop2_hub<P, 256, 2048>), dim3(1536), dim3(256), 0, s, pol, g,
+                         ms.blists.p + 3 * list_cap, counts + 3, ovf[2], ovf_cnt + 2, out, cst + 3, tst + 3 * 3);
+      HL_LAUNCH_CHECK();
+    }
     HL_PROF(P_PROP_LDS, 0, s);
-    hipLaunchKernelGGL((k_prop2_hub<P, 256, 2048>), dim3(1536), dim3(256), 0, s, pol, g,
-                       ms.blists.p + 3 * list_cap, counts + 3, ovf[2], ovf_cnt + 2, out, cst + 3);
-    HL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_prop2_lds<P>, dim3(fixed_grid(list_cap, 1, 256)), dim3(1024), 0, s, pol, g, ovf[2],
                        ovf_cnt + 2, out, cst + 3);
     HL_LAUNCH_CHECK();
@@ -1557,13 +1584,16 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
   if (maxdeg > kLdsDeg) {
     // hubs: LDS tables first, then HBM tables (one per resident workgroup, at
     // most kBigTableBytes in all) for the vertices whose keys overflow them
-    HL_PROF(P_PROP_BIG, 0, s);
-    hipLaunchKernelGGL((k_prop2_hub<P, 512, 4096>), dim3(768), dim3(512), 0, s, pol, g,
-                       ms.blists.p + 4 * list_cap, counts + 4, ovf[3], ovf_cnt + 3, out, cst + 4);
-    HL_LAUNCH_CHECK();
+    {
+      HL_PROF(P_TRY_HUB, 0, s);
+      hipLaunchKernelGGL((k_prop2_hub<P, 512, 4096>), dim3(768), dim3(512), 0, s, pol, g,
+                         ms.blists.p + 4 * list_cap, counts + 4, ovf[3], ovf_cnt + 3, out, cst + 4, tst + 3 * 4);
+      HL_LAUNCH_CHECK();
+    }
     if (maxdeg > kBigDeg) {
+      HL_PROF(P_TRY_GIANT, 0, s);
       hipLaunchKernelGGL((k_prop2_hub<P, 1024, 8192>), dim3(256), dim3(1024), 0, s, pol, g,
-                         ms.blists.p + 5 * list_cap, counts + 5, ovf[3], ovf_cnt + 3, out, cst + 5);
+                         ms.blists.p + 5 * list_cap, counts + 5, ovf[3], ovf_cnt + 3, out, cst + 5, tst + 3 * 5);
       HL_LAUNCH_CHECK();
     }
     const u32 tcap = big_cap(maxdeg);
@@ -1571,6 +1601,7 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
     const unsigned nb = unsigned(std::max<u64>(1, std::min<u64>(kOvfBlocks, kBigTableBytes / per)));
     ms.tk.reserve(u64(nb) * tcap, s);
     ms.tv.reserve(u64(nb) * tcap, s);
+    HL_PROF(P_PROP_BIG, 0, s);
     hipLaunchKernelGGL(k_prop2_big<P>, dim3(nb), dim3(1024), 0, s, pol, g, ovf[3], ovf_cnt + 3, ms.tk.p, ms.tv.p,
                        tcap, out, cst + 4);
     HL_LAUNCH_CHECK();
@@ -1627,7 +1658,8 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     if (st) st->evaluated += host_cnt;
     u32* cur_cnt = lcnt[ci];
     u32* next_cnt = lcnt[ci ^ 1];
-    propose_round(pol, m.g, cur, cur_cnt, host_cnt, maxdeg, ms, th, ms.dstat.p + DS_MOVE, s);
+    propose_round(pol, m.g, cur, cur_cnt, host_cnt, maxdeg, ms, th, ms.dstat.p + DS_MOVE,
+                  ms.dstat.p + DS_TRY_MOVE, s);
     const u32 ep = ms.next_epoch(s);
     CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
     const FlatMarks fm{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 12};
@@ -1642,19 +1674,19 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
       hipLaunchKernelGGL(k_ck_flat, kMarksBlocks, 256, 0, s, m, fm);
       HL_LAUNCH_CHECK();
     }
+    {
+      // neighbour lists too long for the decoupling workgroup: marks over the
+      // grid (the kernel returns at once when the workgroup marked them all or
+      // the candidates took the multi-kernel path: the count it reads is 0)
+      HL_PROF(P_MARK_MOVED, 0, s);
+      hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fm, ms.mark.p, ep, nxt, next_cnt);
+      HL_LAUNCH_CHECK();
+    }
     u32 h[16];
     d2h_async(h, ms.counters.p, 16 * sizeof(u32), s);
     stream_sync(s);
     u32 accepted = h[15];
     u32 next_host = h[2 * (ci ^ 1)];
-    if (h[14] != 1 && h[13] != 0) {
-      // neighbour lists too long for the decoupling workgroup: marks over the grid
-      HL_PROF(P_MARK_MOVED, 0, s);
-      hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fm, ms.mark.p, ep, nxt, next_cnt);
-      HL_LAUNCH_CHECK();
-      d2h_async(&next_host, next_cnt, sizeof(u32), s);
-      stream_sync(s);
-    }
     if (h[14] == 1) {
       // many candidates: sort by node, then stably by gain; priority rounds over the grid
       const u32 c = h[1];
@@ -1762,7 +1794,7 @@ void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, con
   hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, nullptr, ccnt, ms.counters.p + 4);
   HL_LAUNCH_CHECK();
   RefinePolicyT pol{f, sm, dS, blocked, gamma};
-  propose_round(pol, g, singles, cnt, ns, maxdeg, ms, th, ms.dstat.p + DS_REF, s);
+  propose_round(pol, g, singles, cnt, ns, maxdeg, ms, th, ms.dstat.p + DS_REF, ms.dstat.p + DS_TRY_REF, s);
   CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
   HL_PROF(P_REFINE, 0, s);
   hipLaunchKernelGGL(k_ref_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);

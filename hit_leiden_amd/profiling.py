@@ -167,3 +167,68 @@ def summary_text(d, command, top=20, pattern=None):
         for k, (nd, mean) in sorted(cw.items()):
             lines.append(f"counter {k} (KiB) over {nd} dispatches of /{pattern}/ after the load: mean {mean:.1f}")
     return "\n".join(lines) + "\n"
+
+
+def traffic_entry(fetch_dir, write_dir, pattern, fetch_factor, write_factor, kernel):
+    """HBM bytes per launch of the kernels matching pattern, from two --pmc
+    runs (FETCH_SIZE, WRITE_SIZE), dispatches after the load only."""
+    fm = window_counter_means(fetch_dir, pattern).get("FETCH_SIZE")
+    wm = window_counter_means(write_dir, pattern).get("WRITE_SIZE")
+    if not fm or not wm:
+        return None
+    return {"kernel": kernel, "dispatches": fm[0], "fetch_kib": round(fm[1], 1), "write_kib": round(wm[1], 1),
+            "bytes_per_launch": int(round(fm[1] * 1024 * fetch_factor + wm[1] * 1024 * write_factor))}
+
+
+def calibration(copy_fetch_dir, copy_write_dir, copy_bytes):
+    """True/counted byte ratios of a device-to-device copy of copy_bytes."""
+    f = counter_means(copy_fetch_dir, "copyBuffer").get("FETCH_SIZE")
+    w = counter_means(copy_write_dir, "copyBuffer").get("WRITE_SIZE")
+    ff = copy_bytes / (f[1] * 1024) if f and f[1] > 0 else 1.0
+    wf = copy_bytes / (w[1] * 1024) if w and w[1] > 0 else 1.0
+    return {"copy_bytes": copy_bytes, "fetch_kib": round(f[1], 1) if f else None,
+            "write_kib": round(w[1], 1) if w else None, "fetch_factor": round(ff, 4), "write_factor": round(wf, 4)}
+
+
+def main(argv=None):
+    """python -m hit_leiden_amd.profiling summary DIR --command C [--kernel RX]
+    python -m hit_leiden_amd.profiling traffic --out F --fetch D --write D
+        --copy-fetch D --copy-write D --copy-bytes N --command C SLOT=REGEX ..."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="hit_leiden_amd.profiling")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("summary")
+    a.add_argument("dir")
+    a.add_argument("--command", default="")
+    a.add_argument("--kernel", default=None)
+    b = sub.add_parser("traffic")
+    b.add_argument("--out", required=True)
+    b.add_argument("--fetch", required=True)
+    b.add_argument("--write", required=True)
+    b.add_argument("--copy-fetch", required=True)
+    b.add_argument("--copy-write", required=True)
+    b.add_argument("--copy-bytes", type=int, required=True)
+    b.add_argument("--command", default="")
+    b.add_argument("slots", nargs="+", help="slot=kernel regex")
+    args = ap.parse_args(argv)
+    if args.cmd == "summary":
+        print(summary_text(args.dir, args.command, pattern=args.kernel), end="")
+        return
+    cal = calibration(args.copy_fetch, args.copy_write, args.copy_bytes)
+    out = {"_method": "per-dispatch FETCH_SIZE and WRITE_SIZE (KiB) from two rocprofv3 --pmc runs of '"
+                      + args.command + "', averaged over the dispatches after the initial load, each scaled by "
+                      "the ratio of true bytes to the counter for a device-to-device copy of "
+                      f"{args.copy_bytes} bytes", "_calibration": cal}
+    for s in args.slots:
+        slot, rx = s.split("=", 1)
+        e = traffic_entry(args.fetch, args.write, rx, cal["fetch_factor"], cal["write_factor"], rx)
+        if e:
+            out[slot] = e
+    with open(args.out, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+
+
+if __name__ == "__main__":
+    main()
