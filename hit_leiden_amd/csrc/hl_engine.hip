@@ -342,6 +342,10 @@ struct Engine {
     // moved vertex; on R-MAT scale 24 the searches cost more than the gathers
     // saved, so it is opt-in
     if (cfg.key_cache == 1) m.pool = lv.col.cap;
+    // hub tables (on unless turned off): slots for the tables of the rows
+    // above the wavefront bucket, at most 2^26 (1 GiB)
+    if (cfg.hub_tables == 0 && cfg.key_cache == 0)
+      m.ht.pool = std::min<u64>(u64(1) << 26, std::max<u64>(u64(1) << 16, lv.pool_used / 4));
     if (multi()) {
       m.gid = lv.gid.p;
       m.part = part.p;
@@ -437,17 +441,21 @@ struct Engine {
       if (tried[b] > prop[b]) tried[b] = prop[b];
       total += prop[b];
     }
+    // hub-table evaluations: 20 + 16 per table slot (key, count, weight) + 8 per live candidate
+    const u64 htab = 20 * d[DS_HT] + 16 * d[DS_HT + 1] + 8 * d[DS_HT + 2];
+    total += htab;
     const u64 dec = 20 * d[DS_DEC_CAND] + 28 * d[DS_DEC_ACC] + 12 * d[DS_DEC_DEG];
     const u64 multi = 20 * d[DS_MULTI_CAND], apply = 28 * d[DS_MULTI_ACC], marks = 12 * d[DS_MULTI_DEG];
     total += dec + multi + apply + marks;
     if (st) st->bytes_alg += total;
     if (prof.on) {
-      const int slots[13] = {P_PROP_SMALL, P_TRY_HALF, P_TRY_WAVE, P_PROP_WAVE, P_TRY_LDS, P_PROP_LDS, P_TRY_HUB,
-                             P_TRY_GIANT, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI, P_APPLY_MOVES, P_MARK_MOVED};
-      const u64 by[13] = {prop[0], tried[1], tried[2], prop[1] + prop[2] - tried[1] - tried[2], tried[3],
+      const int slots[14] = {P_PROP_SMALL, P_TRY_HALF, P_TRY_WAVE, P_PROP_WAVE, P_TRY_LDS, P_PROP_LDS, P_TRY_HUB,
+                             P_TRY_GIANT, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI, P_APPLY_MOVES, P_MARK_MOVED,
+                             P_PROP_TABLE};
+      const u64 by[14] = {prop[0], tried[1], tried[2], prop[1] + prop[2] - tried[1] - tried[2], tried[3],
                           prop[3] - tried[3], tried[4], tried[5], prop[4] + prop[5] - tried[4] - tried[5],
-                          dec, multi, apply, marks};
-      for (int i = 0; i < 13; ++i)
+                          dec, multi, apply, marks, htab};
+      for (int i = 0; i < 14; ++i)
         if (prof.records(slots[i])) prof.bytes[slots[i]] += by[i];
     }
   }
@@ -658,7 +666,8 @@ hl_status hl_create(const hl_config* cfg, hl_graph** out) {
   }
   *out = nullptr;
   if (cfg->levels < 1 || cfg->levels > HL_MAX_LEVELS || !(cfg->gamma > 0.0) || !std::isfinite(cfg->gamma) ||
-      (cfg->mode != HL_MODE_DETERMINISTIC && cfg->mode != HL_MODE_THROUGHPUT) || cfg->key_cache > 1) {
+      (cfg->mode != HL_MODE_DETERMINISTIC && cfg->mode != HL_MODE_THROUGHPUT) || cfg->key_cache > 1 ||
+      cfg->hub_tables > 1) {
     g_create_err = "invalid config";
     return HL_INVALID_INPUT;
   }

@@ -22,6 +22,30 @@ struct FlatMarks {
   u64* pre;  // exclusive prefix of the list lengths, pre[c] = total
   u32* c;    // device count for k_marks_flat (0: nothing to do)
   u32* ck_c = nullptr;  // device count for the cache updates of k_ck_flat
+  u32* from = nullptr;  // source community of each move (hub-table updates)
+};
+
+// Hub tables: for a vertex of degree above the LDS workgroup bucket, the
+// weight of its row per neighbour community, built by its first full
+// evaluation in the rounds of a level and kept current by the neighbours'
+// moves, so that later rounds evaluate it from the table (its distinct
+// neighbour communities) instead of its row.  A table is valid while bit v of
+// `bits` is set; a new level's rounds clear the bitmap and empty the pool.
+// Keys are never removed: a community whose last entry left keeps weight 0
+// and is skipped (row weights are > 1e-12, so a live key's sum is never 0 in
+// exact arithmetic; with inexact sums a key that lost all its entries keeps a
+// residue of a few ulps and acts as a non-neighbour community, whose gain is
+// never above the empty community's).  A table that fills up is dropped and
+// rebuilt by the next full evaluation.
+struct HubTables {
+  u32* bits = nullptr;   // validity bitmap (n bits); nullptr: tables off
+  u64* off = nullptr;    // first slot of v's table in the pool
+  u32* cap = nullptr;    // its slots (a power of two)
+  u32* key = nullptr;    // pool: community id or kNone
+  double* w = nullptr;   //       weight of v's row entries in that community
+  unsigned long long* used = nullptr;  // pool slots handed out (in chunks)
+  u64 pool = 0;
+  __device__ bool valid(u32 v) const { return (bits[v >> 5] >> (v & 31)) & 1u; }
 };
 
 // what a movement round reads and writes at one level
@@ -49,6 +73,7 @@ struct MoveCtx {
   // the row pool holds (the cache's size); 0 leaves the cache off.
   u32* ck = nullptr;
   u64 pool = 0;
+  HubTables ht;
 };
 
 // throughput movement (parallel_frontier.rs:155-236): self-loops skipped,
@@ -59,7 +84,10 @@ struct MovePolicyT {
   double gamma;
   u32 n;
   const u32* ck = nullptr;  // neighbour-community cache (MoveCtx::ck), or none
+  HubTables ht;             // hub tables (MoveCtx::ht), or none
   static constexpr bool kEmpty = true;
+  static constexpr bool kTables = true;
+  __device__ bool table_ok(u32 v) const { return ht.bits && ht.valid(v); }
   // row entry e of vertex v (neighbour col[e])
   __device__ bool accept_e(u32 v, const u32* col, u64 e, u32& key, int& mult) const {
     mult = 1;
@@ -89,6 +117,8 @@ struct RefinePolicyT {
   const u8* blocked;
   double gamma;
   static constexpr bool kEmpty = false;
+  static constexpr bool kTables = false;
+  __device__ bool table_ok(u32) const { return false; }
   __device__ bool accept_e(u32 v, const u32* col, u64 e, u32& key, int& mult) const {
     const u32 u = col[e];
     if (f[u] != f[v]) return false;
@@ -121,7 +151,9 @@ enum DevStat : u32 {
   // kernel of each bucket evaluated without spilling to its overflow kernel
   DS_TRY_MOVE,
   DS_TRY_REF = DS_TRY_MOVE + 3 * kBuckets,
-  DS_COUNT = DS_TRY_REF + 3 * kBuckets
+  // hub-table evaluations: vertices, table slots scanned, live candidates
+  DS_HT = DS_TRY_REF + 3 * kBuckets,
+  DS_COUNT = DS_HT + 3
 };
 
 struct MoveScratch {
@@ -138,6 +170,12 @@ struct MoveScratch {
   DevBuf<u64> mpre, mdeg;
   DevBuf<u8> target;
   DevBuf<u32> ck;  // neighbour-community cache of the level in movement
+  DevBuf<u32> mfrom;  // source communities of the moves marked over the grid
+  // hub tables (HubTables)
+  DevBuf<u32> ht_bits, ht_cap, ht_key;
+  DevBuf<u64> ht_off;
+  DevBuf<double> ht_w;
+  DevBuf<unsigned long long> ht_used;
   size_t target_cap = 0;
   u32 epoch = 0;
   void ensure(u64 n, u64 comm_cap, hipStream_t s);

@@ -32,8 +32,12 @@ static_assert(4 + kBuckets <= 13, "bucket counts must end before the flat mark c
 // neighbour keys did not fit the small tables of their bucket)
 constexpr u32 kOvfSlot = 16, kOvfLists = 4;
 // bucket counts start at counters[4]
+// list index (and count slot counters[4 + kHtList]) of the vertices evaluated
+// from their hub tables
+constexpr u32 kHtList = kBuckets;
+static_assert(4 + kHtList < 12, "the table-list count must end before the cache count");
 __device__ __forceinline__ void clear_bucket_counts(u32* bucket_counts) {
-  for (u32 i = 0; i < kBuckets; ++i) bucket_counts[i] = 0;
+  for (u32 i = 0; i <= kHtList; ++i) bucket_counts[i] = 0;
   for (u32 i = 0; i < kOvfLists; ++i) bucket_counts[kOvfSlot - 4 + i] = 0;
 }
 constexpr u32 kLdsDeg = 4096;       // workgroup-per-vertex with an LDS table up to this degree
@@ -248,6 +252,9 @@ __global__ void k_activate_graphs(const u32* gid, u32 n, const u8* part, const u
     if (want) list[p] = u32(v);
   }
 }
+__global__ void k_fill_u64_one(unsigned long long* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
 __global__ void k_iota_list(u32* list, u32 n, u32* cnt) {
   for (u64 i = gtid(); i < n; i += gstride()) list[i] = u32(i);
   if (gtid() == 0) *cnt = n;
@@ -324,7 +331,7 @@ __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* li
   for (u64 base = u64(blockIdx.x) * 256; base < cnt; base += gstride()) {
     const u64 id = base + t;
     double bg = 0.0;
-    u32 best = kNone, v = 0, b = kBuckets, deg = 0;
+    u32 best = kNone, v = 0, b = 0xFFu, deg = 0;  // b: bucket, kHtList, or none
     if (id < cnt) {
       v = list[id];
       deg = g.row_len[v];
@@ -334,6 +341,8 @@ __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* li
           : deg <= kLdsDeg  ? 3
           : deg <= kBigDeg  ? 4
                             : 5;
+      if constexpr (P::kTables)
+        if (b >= 4 && pol.table_ok(v)) b = kHtList;  // counted by k_prop2_ht
 #pragma unroll
       for (u32 q = 0; q < kBuckets; ++q)
         if (q == b) {
@@ -344,6 +353,10 @@ __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* li
     for (u32 q = 1; q < kBuckets; ++q) {
       const u32 p = wave_append(b == q, &counts[q]);
       if (b == q) lists[q * cap + p] = v;
+    }
+    if constexpr (P::kTables) {
+      const u32 p = wave_append(b == kHtList, &counts[kHtList]);
+      if (b == kHtList) lists[kHtList * cap + p] = v;
     }
     if (b == 0) {
       const u64 off = g.row_off[v];
@@ -582,13 +595,74 @@ __device__ __forceinline__ double lds_find(const u32* K, const double* V, u32 ma
   return 0.0;
 }
 
+// Hub table of vertex v written from its workgroup's LDS table (K, V of S
+// slots).  cap = the power of two >= 2 c + 16 for c distinct keys, so that new
+// keys from the neighbours' moves fit; when cap <= S / 2 the keys are rehashed
+// into the LDS scratch (K2, V2) first, else the LDS table is the table
+// (cap = S, fewer slots than the row has entries).  The HBM table takes plain
+// stores only (the kernels that read it come later); pool space comes in
+// chunks of up to four tables per workgroup (one device atomic per chunk); no
+// table when the pool is spent.
+constexpr u64 kHtChunk = 8192;
+template <u32 S>
+__device__ void build_hub_table(const HubTables& ht, u32 v, const u32* K, const double* V, u32* K2, double* V2,
+                                u32 c, u64& chunk_at, u64& chunk_end) {
+  __shared__ u64 s_off;
+  u32 cap = 32;
+  while (cap < 2 * c + 16) cap <<= 1;
+  if (cap > S / 2) cap = S;
+  if (threadIdx.x == 0) {
+    if (chunk_at + cap > chunk_end) {
+      const u64 want = cap >= kHtChunk / 4 ? cap : 4 * cap;
+      chunk_at = atomicAdd(ht.used, (unsigned long long)want);
+      chunk_end = chunk_at + want;
+      if (chunk_end > ht.pool) chunk_at = chunk_end = ht.pool;  // spent
+    }
+    s_off = chunk_at + cap <= chunk_end ? chunk_at : ~0ull;
+    if (s_off != ~0ull) chunk_at += cap;
+  }
+  if (cap < S) {
+    for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
+      K2[q] = kNone;
+      V2[q] = 0.0;
+    }
+    __syncthreads();
+    for (u32 q = threadIdx.x; q < S; q += blockDim.x) {
+      const u32 key = K[q];
+      if (key == kNone) continue;
+      V2[tab_insert(K2, cap, key)] = V[q];
+    }
+    K = K2;
+    V = V2;
+  }
+  __syncthreads();
+  const u64 off = s_off;
+  if (off == ~0ull) return;
+  for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
+    ht.key[off + q] = K[q];
+    ht.w[off + q] = V[q];
+  }
+  if (threadIdx.x == 0) {
+    ht.off[v] = off;
+    ht.cap[v] = cap;
+    atomicOr(&ht.bits[v >> 5], 1u << (v & 31));
+  }
+}
+
 template <class P, u32 T, u32 S>
 __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* ovf,
                                                  u32* ovf_cnt, CandOut out, unsigned long long* cst,
                                                  unsigned long long* tst) {
+  // hub tables for the rows above the LDS workgroup bucket (S >= 4096 slots)
+  constexpr bool kTab = P::kTables && S >= 4096;
+  constexpr u32 S2 = kTab ? S / 2 : 1;
   __shared__ u32 K[S];
   __shared__ double V[S];
+  __shared__ u32 K2[S2];  // the hub table rehashed to its own size
+  __shared__ double V2[S2];
   __shared__ u32 full;
+  __shared__ u32 csum[16];
+  u64 chunk_at = 0, chunk_end = 0;  // this workgroup's pool chunk (thread 0)
   const u32 cnt = *cnt_dev;
   unsigned long long ncand = 0, nv = 0, ne = 0;
   for (u32 id = blockIdx.x; id < cnt; id += gridDim.x) {
@@ -637,15 +711,16 @@ __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, 
     const double wcur = lds_find(K, V, S - 1, cur);
     const double kv = g.k[v], sc = pol.cur_total(cur), tt = tm_of(g, v);
     double bg = 0.0;
-    u32 best = kNone;
+    u32 best = kNone, lc = 0;
     for (u32 q = threadIdx.x; q < S; q += T) {
       const u32 key = K[q];
       if (key == kNone) continue;
-      ++ncand;
+      ++lc;
       if (key == cur || !pol.ok(key)) continue;
       const double gn = move_gain(V[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
       if (gn > 0.0) better(bg, best, gn, key);
     }
+    ncand += lc;
     block_argmax(bg, best);
     if (threadIdx.x == 0) {
       nv += 1;
@@ -658,12 +733,125 @@ __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, 
         out.gain[p] = bg;
       }
     }
+    if constexpr (kTab) {
+      if (pol.ht.bits) {
+        // distinct keys of the row, then the table
+        u32 x = lc;
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if ((threadIdx.x & 63) == 0) csum[threadIdx.x >> 6] = x;
+        __syncthreads();
+        u32 c = 0;
+        for (u32 i = 0; i < T / 64; ++i) c += csum[i];
+        build_hub_table<S>(pol.ht, v, K, V, K2, V2, c, chunk_at, chunk_end);
+      }
+    }
     __syncthreads();
   }
   block_add_u64(cst, ncand);
   const u32 idx[3] = {0, 1, 2};
   unsigned long long vals[3] = {nv, ne, ncand};
   block_add_multi(tst, idx, vals);
+}
+
+// Vertices with a valid hub table: one wavefront per vertex scans the table's
+// slots (the live keys are its neighbour communities) instead of the row.
+template <class P>
+__global__ __launch_bounds__(256) void k_prop2_ht(P pol, Csr g, const u32* list, const u32* cnt_dev, CandOut out,
+                                                   unsigned long long* hst) {
+  __shared__ u32 stn[kStage], stt[kStage];
+  __shared__ double stg[kStage];
+  __shared__ u32 scnt, sbase;
+  if (threadIdx.x == 0) scnt = 0;
+  const u32 cnt = *cnt_dev;
+  const u32 lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  unsigned long long nv = 0, nslots = 0, ncand = 0;
+  __syncthreads();
+  for (u64 base = u64(blockIdx.x) * 4; base < cnt; base += u64(gridDim.x) * 4) {
+    const u64 id = base + grp;
+    const bool has = id < cnt;  // uniform over the wavefront
+    double bg = 0.0, wcur = 0.0, kv = 0.0, sc = 0.0, tt = 0.0;
+    u32 best = kNone, v = 0, cur = 0;
+    if (has) {
+      v = list[id];
+      cur = pol.cur_key(v);
+      const u64 off = pol.ht.off[v];
+      const u32 cap = pol.ht.cap[v];
+      const u32* TK = pol.ht.key + off;
+      const double* TW = pol.ht.w + off;
+      for (u32 q = lane; q < cap; q += 64)
+        if (TK[q] == cur) wcur = TW[q];  // 0 when v has no neighbour left there
+      for (int o = 32; o > 0; o >>= 1) wcur += __shfl_xor(wcur, o, 64);  // one lane at most holds it
+      kv = g.k[v];
+      sc = pol.cur_total(cur);
+      tt = tm_of(g, v);
+      for (u32 q = lane; q < cap; q += 64) {
+        const u32 key = TK[q];
+        const double wq = TW[q];
+        if (key == kNone || wq == 0.0) continue;
+        ++ncand;
+        if (key == cur) continue;
+        const double gn = move_gain(wq, wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+        if (gn > 0.0) better(bg, best, gn, key);
+      }
+      if (lane == 0) {
+        nv += 1;
+        nslots += cap;
+      }
+    }
+    wave_argmax(bg, best);
+    if (has && lane == 0) {
+      finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
+      if (bg > 0.0) {
+        const u32 p = atomicAdd(&scnt, 1u);  // LDS
+        stn[p] = v;
+        stt[p] = best;
+        stg[p] = bg;
+      }
+    }
+    __syncthreads();
+    if (scnt > kStage - 4) flush_stage(stn, stt, stg, scnt, sbase, out);
+  }
+  flush_stage(stn, stt, stg, scnt, sbase, out);
+  const u32 idx[3] = {0, 1, 2};
+  unsigned long long vals[3] = {nv, nslots, ncand};
+  block_add_multi(hst, idx, vals);
+}
+
+// A neighbour v of hub u moved from community `from` to `to` (row weight w):
+// keep u's table current.  The lookup of `from` (a key since before this
+// kernel) takes plain loads; a key that finds no slot drops the table (u is
+// then evaluated from its row, which rebuilds it).
+__device__ void ht_move(const HubTables& ht, u32 u, u32 from, u32 to, double w) {
+  if (!ht.valid(u)) return;
+  const u64 off = ht.off[u];
+  const u32 cap = ht.cap[u];
+  u32* TK = ht.key + off;
+  u32 h = hash32(from) & (cap - 1);
+  bool found = false;
+  for (u32 t = 0; t < cap; ++t) {
+    const u32 k = TK[h];
+    if (k == from) {
+      found = true;
+      break;
+    }
+    if (k == kNone) break;
+    h = (h + 1) & (cap - 1);
+  }
+  if (!found) {  // cannot happen for a table built from the current row
+    atomicAnd(&ht.bits[u >> 5], ~(1u << (u & 31)));
+    return;
+  }
+  atomicAdd(&ht.w[off + h], -w);
+  h = hash32(to) & (cap - 1);
+  for (u32 t = 0; t < cap; ++t) {
+    const u32 old = atomicCAS(&TK[h], kNone, to);
+    if (old == kNone || old == to) {
+      atomicAdd(&ht.w[off + h], w);
+      return;
+    }
+    h = (h + 1) & (cap - 1);
+  }
+  atomicAnd(&ht.bits[u >> 5], ~(1u << (u & 31)));
 }
 
 // G lanes per vertex with a small LDS table of at most CAP slots: the
@@ -1153,7 +1341,9 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
           else hi = mid;
         }
         const u32 v = snode[lo], cto = sto[lo];
-        u = m.g.col[m.g.row_off[v] + (t - pre[lo])];
+        const u64 e = m.g.row_off[v] + (t - pre[lo]);
+        u = m.g.col[e];
+        if (m.ht.bits && u != v) ht_move(m.ht, u, sfrom[lo], cto, m.g.w[e]);
         if (ld_agent(&m.f[u]) != cto) want = claim(mark, u, epoch);
         if (m.s[u] == m.s[v]) {
           m.K[u] = 1;
@@ -1171,6 +1361,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
       flat.node[p] = snode[p];
       flat.to[p] = sto[p];
       flat.pre[p] = pre[p];
+      if (flat.from) flat.from[p] = sfrom[p];
     }
     if (tid == 0) {
       flat.pre[c] = total;
@@ -1228,7 +1419,9 @@ __global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u
         else hi = mid;
       }
       const u32 v = flat.node[lo], cto = flat.to[lo];
-      const u32 u = m.g.col[m.g.row_off[v] + (t - flat.pre[lo])];
+      const u64 e = m.g.row_off[v] + (t - flat.pre[lo]);
+      const u32 u = m.g.col[e];
+      if (m.ht.bits && u != v) ht_move(m.ht, u, flat.from[lo], cto, m.g.w[e]);
       if (m.f[u] != cto && claim(mark, u, epoch)) sv_[atomicAdd(&scnt, 1u)] = u;
       if (m.s[u] == m.s[v]) {
         m.K[u] = 1;
@@ -1486,13 +1679,14 @@ void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
   }
   list_a.reserve(n + 1, s);
   list_b.reserve(n + 1, s);
-  blists.reserve(kBuckets * (n + 1), s);
+  blists.reserve((kHtList + 1) * (n + 1), s);
   cnode.reserve(n + 1, s);
   cto.reserve(n + 1, s);
   cgain.reserve(n + 1, s);
   counters.reserve(32, s);
   mnode.reserve(kDecSmall + 1, s);
   mto.reserve(kDecSmall + 1, s);
+  mfrom.reserve(kDecSmall + 1, s);
   mpre.reserve(kDecSmall + 2, s);
   if (minUto.cap < comm_cap) {
     minUto.reserve(comm_cap, s);
@@ -1503,7 +1697,9 @@ void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
 }
 
 void MoveScratch::trim() {
-  for (DevBuf<u32>* b : {&tk, &hovf, &ck, &k32, &k32b, &v32, &o1, &o2, &node, &from, &to, &state, &acc}) b->release();
+  for (DevBuf<u32>* b : {&tk, &hovf, &ck, &k32, &k32b, &v32, &o1, &o2, &node, &from, &to, &state, &acc, &ht_key})
+    b->release();
+  ht_w.release();
   tv.release();
   k64.release();
   k64b.release();
@@ -1535,6 +1731,14 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
     hipLaunchKernelGGL(k_prop2_first<P>, fixed_grid(list_cap), 256, 0, s, pol, g, list, cnt_dev, small, half, wave,
                        ms.blists.p, list_cap, counts, out, st);
     HL_LAUNCH_CHECK();
+  }
+  if constexpr (P::kTables) {
+    if (pol.ht.bits) {
+      HL_PROF(P_PROP_TABLE, 0, s);
+      hipLaunchKernelGGL(k_prop2_ht<P>, dim3(fixed_grid(list_cap, 4, 2048)), dim3(256), 0, s, pol, g,
+                         ms.blists.p + kHtList * list_cap, counts + kHtList, out, ms.dstat.p + DS_HT);
+      HL_LAUNCH_CHECK();
+    }
   }
   // each bucket past the first: small tables first, then the tables sized for
   // the degree for the vertices listed as overflowing them
@@ -1648,7 +1852,23 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
       HL_LAUNCH_CHECK();
     }
   }
-  MovePolicyT pol{m.f, m.tot, m.gamma, n, m.ck};
+  if (m.ht.pool && !m.ck && maxdeg > kLdsDeg) {
+    // hub tables: a cleared bitmap and an empty pool for this level's rounds
+    const u64 words = (u64(n) + 32) / 32;
+    ms.ht_bits.reserve(words, s);
+    fill_u32(ms.ht_bits.p, 0, words, s);
+    ms.ht_off.reserve(n + 1, s);
+    ms.ht_cap.reserve(n + 1, s);
+    ms.ht_key.reserve(m.ht.pool, s);
+    ms.ht_w.reserve(m.ht.pool, s);
+    ms.ht_used.reserve(1, s);
+    hipLaunchKernelGGL(k_fill_u64_one, 1, 64, 0, s, ms.ht_used.p);
+    HL_LAUNCH_CHECK();
+    m.ht = HubTables{ms.ht_bits.p, ms.ht_off.p, ms.ht_cap.p, ms.ht_key.p, ms.ht_w.p, ms.ht_used.p, m.ht.pool};
+  } else {
+    m.ht = HubTables{};
+  }
+  MovePolicyT pol{m.f, m.tot, m.gamma, n, m.ck, m.ht};
   u32 host_cnt = 0;
   d2h_async(&host_cnt, cnt_a, sizeof(u32), s);
   stream_sync(s);
@@ -1662,7 +1882,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
                   ms.dstat.p + DS_TRY_MOVE, s);
     const u32 ep = ms.next_epoch(s);
     CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
-    const FlatMarks fm{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 12};
+    const FlatMarks fm{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 12, ms.mfrom.p};
     {
       HL_PROF(P_DECOUPLE, 0, s);
       hipLaunchKernelGGL(k_dec_small, 1, 1024, 0, s, m, cand, ms.minUto.p, ms.minUfrom.p, ms.minAto.p,
@@ -1754,7 +1974,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
         HL_LAUNCH_CHECK();
         exclusive_scan_u64_async(ms.mdeg.p, ms.mpre.p, c, tmp, s);
         fill_u32(ms.counters.p + 13, c, 1, s);
-        const FlatMarks fmm{ms.node.p, ms.to.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 13};
+        const FlatMarks fmm{ms.node.p, ms.to.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 13, ms.from.p};
         hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fmm, ms.mark.p, ep, nxt, next_cnt);
         HL_LAUNCH_CHECK();
         if (m.ck) {

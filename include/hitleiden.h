@@ -74,7 +74,10 @@ typedef struct {
   uint32_t load_pass_records; /* hl_load_graphs: edges per loading pass, 0 = 2^26 */
   uint32_t key_cache;  /* throughput movement: cache of each row entry's neighbour community,
                           kept current by the moves; 0 = off, 1 = on */
-  uint32_t reserved[4];
+  uint32_t hub_tables;  /* throughput movement: rows above the wavefront bucket evaluated from
+                          per-vertex community tables kept current by the moves; 0 = on, 1 = off
+                          (off when key_cache is on) */
+  uint32_t reserved[3];
 } hl_config;
 
 #define HL_MAX_LEVELS 8
