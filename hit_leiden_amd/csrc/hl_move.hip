@@ -342,7 +342,7 @@ __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* li
           : deg <= kBigDeg  ? 4
                             : 5;
       if constexpr (P::kTables)
-        if (b >= 4 && pol.table_ok(v)) b = kHtList;  // counted by k_prop2_ht
+        if (b >= 3 && pol.table_ok(v)) b = kHtList;  // counted by k_prop2_ht
 #pragma unroll
       for (u32 q = 0; q < kBuckets; ++q)
         if (q == b) {
@@ -653,8 +653,8 @@ template <class P, u32 T, u32 S>
 __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* ovf,
                                                  u32* ovf_cnt, CandOut out, unsigned long long* cst,
                                                  unsigned long long* tst) {
-  // hub tables for the rows above the LDS workgroup bucket (S >= 4096 slots)
-  constexpr bool kTab = P::kTables && S >= 4096;
+  // hub tables for the rows above the wavefront bucket
+  constexpr bool kTab = P::kTables;
   constexpr u32 S2 = kTab ? S / 2 : 1;
   __shared__ u32 K[S];
   __shared__ double V[S];
@@ -842,10 +842,14 @@ __device__ void ht_move(const HubTables& ht, u32 u, u32 from, u32 to, double w) 
     return;
   }
   atomicAdd(&ht.w[off + h], -w);
+  // `to`: an existing key is found by plain loads (no returning atomic on the
+  // common path); an empty-looking slot is claimed with a compare-and-swap,
+  // which also settles a load that missed a key inserted during this kernel
   h = hash32(to) & (cap - 1);
   for (u32 t = 0; t < cap; ++t) {
-    const u32 old = atomicCAS(&TK[h], kNone, to);
-    if (old == kNone || old == to) {
+    u32 k = TK[h];
+    if (k == kNone) k = atomicCAS(&TK[h], kNone, to);
+    if (k == kNone || k == to) {
       atomicAdd(&ht.w[off + h], w);
       return;
     }
@@ -1852,7 +1856,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
       HL_LAUNCH_CHECK();
     }
   }
-  if (m.ht.pool && !m.ck && maxdeg > kLdsDeg) {
+  if (m.ht.pool && !m.ck && maxdeg > std::min(th.wave_deg, WAVE_HW2 / 2)) {
     // hub tables: a cleared bitmap and an empty pool for this level's rounds
     const u64 words = (u64(n) + 32) / 32;
     ms.ht_bits.reserve(words, s);
