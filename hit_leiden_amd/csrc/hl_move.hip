@@ -114,6 +114,11 @@ __device__ __forceinline__ u32 wave_append(bool want, u32* cnt) {
 __device__ __forceinline__ u32 ld_agent(const u32* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// the store such a load observes once a barrier orders the two (an agent-scope
+// atomic store needs no device-wide fence to be seen by ld_agent)
+__device__ __forceinline__ void st_agent(u32* p, u32 x) {
+  __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 // Slot of key in an LDS open-addressing table (linear probing, inserts if absent).
 __device__ __forceinline__ u32 lds_insert(u32* K, u32 cap, u32 key) {
   u32 h = hash32(key) & (cap - 1);
@@ -1131,7 +1136,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
       }
       if (acc) {
         const double kv = m.k[node];
-        m.f[node] = to;
+        st_agent(&m.f[node], to);
         atomicAdd(&m.tot[from], -kv);
         atomicAdd(&m.tot[to], kv);
         m.B[node] = 1;
@@ -1139,7 +1144,6 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
       const unsigned long long am = __ballot(acc);
       if (lane == 0) nacc = u32(__popcll(am));
     }
-    __threadfence();
     __syncthreads();
   } else {
     u32 N = 1;
@@ -1292,13 +1296,12 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
       ++la;
       u32 v = snode[p];
       double kv = m.k[v];
-      m.f[v] = sto[p];
+      st_agent(&m.f[v], sto[p]);
       atomicAdd(&m.tot[sfrom[p]], -kv);
       atomicAdd(&m.tot[sto[p]], kv);
       m.B[v] = 1;
     }
     if (la) atomicAdd(&nacc, la);
-    __threadfence();
     __syncthreads();
   }
   // next-active and K marks (throughput.rs:189-199) over the flattened
