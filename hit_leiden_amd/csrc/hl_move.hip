@@ -319,7 +319,7 @@ __device__ __forceinline__ void finish_choice(const P& pol, u32 v, u32 cur, doub
 }
 
 // The round's first kernel: one thread per active vertex; vertices up to
-// small_deg are evaluated here (row-order sums in a private LDS slice), the
+// small_deg are evaluated here (row-order sums per key in registers), the
 // others are appended to the bucket lists of the kernels that follow (half
 // wavefront <= half_deg, wavefront <= wave_deg, LDS workgroup <= kLdsDeg, hub
 // workgroup <= kBigDeg, giant).
@@ -328,8 +328,6 @@ __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* li
                                                       u32 small_deg, u32 half_deg, u32 wave_deg, u32* lists, u64 cap,
                                                       u32* counts,
                                                       CandOut out, unsigned long long* st) {
-  __shared__ u32 sk[SMALL_MAX2 * 256];
-  __shared__ double sv[SMALL_MAX2 * 256];
   const u32 cnt = *cnt_dev;
   const u32 t = threadIdx.x;
   unsigned long long vals[2 * kBuckets + 1] = {};  // per bucket: vertices, row entries; small-bucket candidates
@@ -355,52 +353,105 @@ __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* li
           vals[kBuckets + q] += deg;
         }
     }
-    for (u32 q = 1; q < kBuckets; ++q) {
-      const u32 p = wave_append(b == q, &counts[q]);
-      if (b == q) lists[q * cap + p] = v;
-    }
-    if constexpr (P::kTables) {
-      const u32 p = wave_append(b == kHtList, &counts[kHtList]);
-      if (b == kHtList) lists[kHtList * cap + p] = v;
+    {
+      // the bucket lists 1 .. kHtList: one device atomic per list and
+      // workgroup pass (same-address atomics serialise, so one per wavefront
+      // costs milliseconds when millions of vertices are active)
+      constexpr u32 NL = kHtList;  // lists 1 .. NL
+      __shared__ u32 wcnt[NL][4], woff[NL][4];
+      const u32 lane = t & 63, wv = t >> 6;
+      const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+      unsigned long long mine = 0;
+#pragma unroll
+      for (u32 q = 1; q <= NL; ++q) {
+        const unsigned long long mq = __ballot(b == q);
+        if (b == q) mine = mq;
+        if (lane == 0) wcnt[q - 1][wv] = u32(__popcll(mq));
+      }
+      __syncthreads();
+      if (t < NL) {
+        const u32 tot = wcnt[t][0] + wcnt[t][1] + wcnt[t][2] + wcnt[t][3];
+        u32 base = tot ? atomicAdd(&counts[t + 1], tot) : 0;
+        for (u32 w = 0; w < 4; ++w) {
+          woff[t][w] = base;
+          base += wcnt[t][w];
+        }
+      }
+      __syncthreads();
+      if (b >= 1 && b <= NL) lists[b * cap + woff[b - 1][wv] + u32(__popcll(mine & below))] = v;
     }
     if (b == 0) {
+      // row-order sums per distinct key in registers (static indices only),
+      // so the kernel needs no LDS and keeps more wavefronts resident
       const u64 off = g.row_off[v];
       const u32 cur = pol.cur_key(v);
-      int nk = 0;
+      u32 rk[SMALL_MAX2];
+      double rw[SMALL_MAX2];
+      u32 nk = 0;
       for (u32 j = 0; j < deg; ++j) {
-        double w = g.w[off + j];
+        const double w = g.w[off + j];
         u32 key;
         int mult;
         if (!pol.accept_e(v, g.col, off + j, key, mult)) continue;
-        int q = 0;
-        while (q < nk && sk[q * 256 + t] != key) ++q;
-        if (q == nk) {
-          sk[q * 256 + t] = key;
-          sv[q * 256 + t] = 0.0;
+        bool found = false;
+#pragma unroll
+        for (u32 q = 0; q < SMALL_MAX2; ++q)
+          if (q < nk && rk[q] == key) {
+            rw[q] += w;
+            if (mult == 2) rw[q] += w;
+            found = true;
+          }
+        if (!found) {
+#pragma unroll
+          for (u32 q = 0; q < SMALL_MAX2; ++q)
+            if (q == nk) {
+              rk[q] = key;
+              rw[q] = w;
+              if (mult == 2) rw[q] += w;
+            }
           ++nk;
         }
-        sv[q * 256 + t] += w;
-        if (mult == 2) sv[q * 256 + t] += w;
       }
       vals[2 * kBuckets] += nk;
       double wcur = 0.0;
-      for (int q = 0; q < nk; ++q)
-        if (sk[q * 256 + t] == cur) wcur = sv[q * 256 + t];
+#pragma unroll
+      for (u32 q = 0; q < SMALL_MAX2; ++q)
+        if (q < nk && rk[q] == cur) wcur = rw[q];
       const double kv = g.k[v], sc = pol.cur_total(cur), tt = tm_of(g, v);
-      for (int q = 0; q < nk; ++q) {
-        u32 key = sk[q * 256 + t];
+#pragma unroll
+      for (u32 q = 0; q < SMALL_MAX2; ++q) {
+        if (q >= nk) continue;
+        const u32 key = rk[q];
         if (key == cur || !pol.ok(key)) continue;
-        double gn = move_gain(sv[q * 256 + t], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
+        const double gn = move_gain(rw[q], wcur, tt, pol.gamma, kv, sc, pol.cand_total(key));
         if (gn > 0.0) better(bg, best, gn, key);
       }
       finish_choice(pol, v, cur, wcur, kv, tt, bg, best);
     }
-    const bool want = bg > 0.0;
-    u32 p = wave_append(want, out.cnt);
-    if (want) {
-      out.node[p] = v;
-      out.to[p] = best;
-      out.gain[p] = bg;
+    {
+      // the candidates: one device atomic per workgroup pass
+      __shared__ u32 ccnt[4], coff[4];
+      const u32 lane = t & 63, wv = t >> 6;
+      const bool want = bg > 0.0;
+      const unsigned long long m = __ballot(want);
+      if (lane == 0) ccnt[wv] = u32(__popcll(m));
+      __syncthreads();
+      if (t == 0) {
+        const u32 tot = ccnt[0] + ccnt[1] + ccnt[2] + ccnt[3];
+        u32 base = tot ? atomicAdd(out.cnt, tot) : 0;
+        for (u32 w = 0; w < 4; ++w) {
+          coff[w] = base;
+          base += ccnt[w];
+        }
+      }
+      __syncthreads();
+      if (want) {
+        const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const u32 p = coff[wv] + u32(__popcll(m & below));
+        out.node[p] = v;
+        out.to[p] = best;
+        out.gain[p] = bg;
+      }
     }
   }
   u32 idx[2 * kBuckets + 1];
@@ -475,14 +526,26 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
     }
     __syncthreads();
     if (has) {
-      for (u32 j = lane; j < deg; j += G) {
-        u32 key;
-        int mult;
-        if (!pol.accept_e(v, g.col, off + j, key, mult)) continue;
-        double w = g.w[off + j];
-        u32 h = tab_insert(K, cap, key);
-        atomicAdd(&V[h], w);
-        if (mult == 2) atomicAdd(&V[h], w);
+      // four row entries per lane in flight: their gathers are independent
+      for (u32 j0 = lane; j0 < deg; j0 += 4 * G) {
+        u32 kk[4];
+        double ww[4];
+        int mm[4];
+        bool ok[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const u32 j = j0 + r * G;
+          ok[r] = j < deg;
+          ww[r] = ok[r] ? g.w[off + j] : 0.0;
+          ok[r] = ok[r] && pol.accept_e(v, g.col, off + j, kk[r], mm[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (!ok[r]) continue;
+          const u32 h = tab_insert(K, cap, kk[r]);
+          atomicAdd(&V[h], ww[r]);
+          if (mm[r] == 2) atomicAdd(&V[h], ww[r]);
+        }
       }
     }
     __syncthreads();
@@ -610,7 +673,8 @@ __device__ __forceinline__ double lds_find(const u32* K, const double* V, u32 ma
 // table when the pool is spent.
 constexpr u64 kHtChunk = 8192;
 template <u32 S>
-__device__ void build_hub_table(const HubTables& ht, u32 v, const u32* K, const double* V, u32* K2, double* V2,
+__device__ __forceinline__ void build_hub_table(const HubTables& ht, u32 v, const u32* K, const double* V, u32* K2,
+                                                double* V2,
                                 u32 c, u64& chunk_at, u64& chunk_end) {
   __shared__ u64 s_off;
   u32 cap = 32;
@@ -695,17 +759,18 @@ __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, 
         ww[r] = ok[r] ? g.w[off + j] : 0.0;
         ok[r] = ok[r] && pol.accept_e(v, g.col, off + j, kk[r], mm[r]);
       }
+      bool spilled = false;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        if (!ok[r]) continue;
+        if (!ok[r] || spilled) continue;
         const u32 h = lds_try_insert(K, S - 1, kk[r]);
-        if (h == kNone) {
-          full = 1;
-          break;
+        spilled = h == kNone;
+        if (!spilled) {
+          atomicAdd(&V[h], ww[r]);
+          if (mm[r] == 2) atomicAdd(&V[h], ww[r]);
         }
-        atomicAdd(&V[h], ww[r]);
-        if (mm[r] == 2) atomicAdd(&V[h], ww[r]);
       }
+      if (spilled) full = 1;
     }
     __syncthreads();
     if (full) {  // uniform: nothing writes it between the barrier above and the one below
@@ -905,19 +970,32 @@ __global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list
     if (lane == 0) gfull[grp] = 0;
     __syncthreads();
     if (has) {
-      for (u32 j = lane; j < deg; j += G) {
-        u32 key;
-        int mult;
-        if (!pol.accept_e(v, g.col, off + j, key, mult)) continue;
-        const double w = g.w[off + j];
-        const u32 h = lds_try_insert(K, cap - 1, key);
-        if (h == kNone) {
-          gfull[grp] = 1;
-          break;
+      // four row entries per lane in flight: their gathers are independent
+      bool spilled = false;
+      for (u32 j0 = lane; j0 < deg && !spilled; j0 += 4 * G) {
+        u32 kk[4];
+        double ww[4];
+        int mm[4];
+        bool ok[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const u32 j = j0 + r * G;
+          ok[r] = j < deg;
+          ww[r] = ok[r] ? g.w[off + j] : 0.0;
+          ok[r] = ok[r] && pol.accept_e(v, g.col, off + j, kk[r], mm[r]);
         }
-        atomicAdd(&V[h], w);
-        if (mult == 2) atomicAdd(&V[h], w);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (!ok[r] || spilled) continue;
+          const u32 h = lds_try_insert(K, cap - 1, kk[r]);
+          spilled = h == kNone;
+          if (!spilled) {
+            atomicAdd(&V[h], ww[r]);
+            if (mm[r] == 2) atomicAdd(&V[h], ww[r]);
+          }
+        }
       }
+      if (spilled) gfull[grp] = 1;
     }
     __syncthreads();
     const bool spill = has && gfull[grp] != 0;  // written before the barrier above, cleared after the next
