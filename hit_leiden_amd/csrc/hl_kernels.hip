@@ -897,12 +897,30 @@ __global__ __launch_bounds__(256) void k_uf_edges(Csr g, const u32* vcc, u64 m, 
     }
   }
 }
+// cnt[key] += 1 for the lanes that want it, one atomic per distinct key of the
+// wavefront (the members of a large component share its root, and atomics on
+// one address serialise)
+__device__ __forceinline__ void wave_count_keyed(u32* cnt, u32 key, bool want) {
+  unsigned long long act = __ballot(want);
+  const u32 lane = threadIdx.x & 63;
+  while (act) {
+    const int leader = __ffsll((long long)act) - 1;
+    const u32 k = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(want && key == k) & act;
+    if (lane == u32(leader)) atomicAdd(&cnt[k], u32(__popcll(same)));
+    act &= ~same;
+  }
+}
 __global__ void k_uf_compress(const u32* vcc, u64 m, u32* parent, u32* cnt) {
-  for (u64 i = gtid(); i < m; i += gstride()) {
-    u32 v = vcc[i];
-    u32 r = uf_find(parent, v);
-    parent[v] = r;
-    atomicAdd(&cnt[r], 1u);
+  for (u64 base = gtid() - (threadIdx.x & 63); base < m; base += gstride()) {
+    const u64 i = base + (threadIdx.x & 63);
+    u32 r = 0;
+    if (i < m) {
+      const u32 v = vcc[i];
+      r = uf_find(parent, v);
+      parent[v] = r;
+    }
+    wave_count_keyed(cnt, r, i < m);
   }
 }
 __global__ void k_zero_best(const u32* vcc, u64 m, const u32* s, u64* best) {
@@ -1435,7 +1453,11 @@ __global__ void k_gid_from_offsets(const u32* offs, u32 G, u32 n, u32* gid) {
   }
 }
 __global__ void k_graph_counts(const Rec* d, u64 k, const u32* gid, u32* cnt) {
-  for (u64 i = gtid(); i < k; i += gstride()) atomicAdd(&cnt[gid[d[i].u]], 1u);
+  // a graph's records are contiguous: one atomic per graph and wavefront
+  for (u64 base = gtid() - (threadIdx.x & 63); base < k; base += gstride()) {
+    const u64 i = base + (threadIdx.x & 63);
+    wave_count_keyed(cnt, i < k ? gid[d[i].u] : 0u, i < k);
+  }
 }
 __global__ void k_part_from_counts(const u32* cnt, u32 G, u8* part) {
   for (u64 g = gtid(); g <= G; g += gstride()) part[g] = g < G && cnt[g] > 0;

@@ -110,6 +110,29 @@ __device__ __forceinline__ u32 wave_append(bool want, u32* cnt) {
   const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
   return want ? base + u32(__popcll(m & below)) : kNone;
 }
+// The same with one device atomic per workgroup pass; every thread of the
+// block must call it, in step (blockDim.x <= 1024).
+__device__ __forceinline__ u32 block_append(bool want, u32* cnt) {
+  __shared__ u32 bw[16], bo[16];
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  const unsigned long long m = __ballot(want);
+  if (lane == 0) bw[wv] = u32(__popcll(m));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 tot = 0;
+    for (u32 w = 0; w < nw; ++w) tot += bw[w];
+    u32 base = tot ? atomicAdd(cnt, tot) : 0;
+    for (u32 w = 0; w < nw; ++w) {
+      bo[w] = base;
+      base += bw[w];
+    }
+  }
+  __syncthreads();
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const u32 p = bo[wv] + u32(__popcll(m & below));
+  __syncthreads();  // bw and bo are reused by the next call
+  return want ? p : kNone;
+}
 // load that observes stores made earlier in the same kernel by other workgroup threads
 __device__ __forceinline__ u32 ld_agent(const u32* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -219,24 +242,29 @@ __device__ __forceinline__ double tab_lookup(const u32* K, const double* V, u32 
 // ------------------------------------------------------------------ seeding
 __global__ void k_seed2(const Rec* d, u64 k, const u32* f, const u32* s, u8* A, u8* K, u32* mark, u32 epoch,
                         u32* list, u32* cnt) {
-  for (u64 i = gtid(); i < k; i += gstride()) {
-    const Rec r = d[i];
-    const bool same_f = f[r.u] == f[r.v];
-    const bool act = (r.w > 0.0 && !same_f) || (r.w < 0.0 && same_f);
-    const bool kk = s[r.u] == s[r.v];
-    bool cu = act && claim(mark, r.u, epoch);
-    bool cv = act && r.v != r.u && claim(mark, r.v, epoch);
-    if (act) {
-      A[r.u] = 1;
-      A[r.v] = 1;
+  for (u64 base = u64(blockIdx.x) * blockDim.x; base < k; base += gstride()) {  // uniform per workgroup
+    const u64 i = base + threadIdx.x;
+    bool cu = false, cv = false;
+    Rec r{0, 0, 0.0};
+    if (i < k) {
+      r = d[i];
+      const bool same_f = f[r.u] == f[r.v];
+      const bool act = (r.w > 0.0 && !same_f) || (r.w < 0.0 && same_f);
+      const bool kk = s[r.u] == s[r.v];
+      cu = act && claim(mark, r.u, epoch);
+      cv = act && r.v != r.u && claim(mark, r.v, epoch);
+      if (act) {
+        A[r.u] = 1;
+        A[r.v] = 1;
+      }
+      if (kk) {
+        K[r.u] = 1;
+        K[r.v] = 1;
+      }
     }
-    if (kk) {
-      K[r.u] = 1;
-      K[r.v] = 1;
-    }
-    u32 pu = wave_append(cu, cnt);
+    const u32 pu = block_append(cu, cnt);
     if (cu) list[pu] = r.u;
-    u32 pv = wave_append(cv, cnt);
+    const u32 pv = block_append(cv, cnt);
     if (cv) list[pv] = r.v;
   }
 }
@@ -253,7 +281,7 @@ __global__ void k_activate_graphs(const u32* gid, u32 n, const u8* part, const u
         want = claim(mark, u32(v), epoch);
       }
     }
-    const u32 p = wave_append(want, cnt);
+    const u32 p = block_append(want, cnt);
     if (want) list[p] = u32(v);
   }
 }
@@ -951,15 +979,33 @@ __global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list
   u32* K = hk + grp * CAP;
   double* V = hv + grp * CAP;
   unsigned long long ncand = 0, nv = 0, ne = 0;
-  for (u64 base = u64(blockIdx.x) * NG; base < cnt; base += u64(gridDim.x) * NG) {
+  // the next pass's vertex and row header are loaded one pass ahead, so
+  // their latency overlaps this pass's gathers
+  const u64 step = u64(gridDim.x) * NG;
+  u32 v_nx = 0, deg_nx = 0;
+  u64 off_nx = 0;
+  {
+    const u64 id0 = u64(blockIdx.x) * NG + grp;
+    if (id0 < cnt) {
+      v_nx = list[id0];
+      off_nx = g.row_off[v_nx];
+      deg_nx = g.row_len[v_nx];
+    }
+  }
+  for (u64 base = u64(blockIdx.x) * NG; base < cnt; base += step) {
     const u64 id = base + grp;
     const bool has = id < cnt;
     u32 v = 0, cur = 0, deg = 0, cap = 16;
     u64 off = 0;
     if (has) {
-      v = list[id];
-      off = g.row_off[v];
-      deg = g.row_len[v];
+      v = v_nx;
+      off = off_nx;
+      deg = deg_nx;
+      if (id + step < cnt) {
+        v_nx = list[id + step];
+        off_nx = g.row_off[v_nx];
+        deg_nx = g.row_len[v_nx];
+      }
       cur = pol.cur_key(v);
       while (cap < 2 * deg && cap < CAP) cap <<= 1;
       for (u32 q = lane; q < cap; q += G) {
