@@ -702,8 +702,7 @@ __device__ __forceinline__ double lds_find(const u32* K, const double* V, u32 ma
 constexpr u64 kHtChunk = 8192;
 template <u32 S>
 __device__ __forceinline__ void build_hub_table(const HubTables& ht, u32 v, const u32* K, const double* V, u32* K2,
-                                                double* V2,
-                                u32 c, u64& chunk_at, u64& chunk_end) {
+                                                double* V2, u32 c, u64& chunk_at, u64& chunk_end) {
   __shared__ u64 s_off;
   u32 cap = 32;
   while (cap < 2 * c + 16) cap <<= 1;
