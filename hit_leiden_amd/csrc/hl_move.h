@@ -39,13 +39,17 @@ struct FlatMarks {
 // rebuilt by the next full evaluation.
 struct HubTables {
   u32* bits = nullptr;   // validity bitmap (n bits); nullptr: tables off
-  u64* off = nullptr;    // first slot of v's table in the pool
-  u32* cap = nullptr;    // its slots (a power of two)
+  u64* loc = nullptr;    // first slot of v's table in the pool (bits 0-47) and log2 of its slots (56-63)
   u32* key = nullptr;    // pool: community id or kNone
   double* w = nullptr;   //       weight of v's row entries in that community
   unsigned long long* used = nullptr;  // pool slots handed out (in chunks)
   u64 pool = 0;
   __device__ bool valid(u32 v) const { return (bits[v >> 5] >> (v & 31)) & 1u; }
+  __device__ void where(u32 v, u64& off, u32& cap) const {
+    const u64 x = loc[v];
+    off = x & ((u64(1) << 48) - 1);
+    cap = 1u << u32(x >> 56);
+  }
 };
 
 // what a movement round reads and writes at one level
@@ -172,8 +176,8 @@ struct MoveScratch {
   DevBuf<u32> ck;  // neighbour-community cache of the level in movement
   DevBuf<u32> mfrom;  // source communities of the moves marked over the grid
   // hub tables (HubTables)
-  DevBuf<u32> ht_bits, ht_cap, ht_key;
-  DevBuf<u64> ht_off;
+  DevBuf<u32> ht_bits, ht_key;
+  DevBuf<u64> ht_loc;
   DevBuf<double> ht_w;
   DevBuf<unsigned long long> ht_used;
   size_t target_cap = 0;

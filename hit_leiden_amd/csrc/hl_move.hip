@@ -739,8 +739,7 @@ __device__ __forceinline__ void build_hub_table(const HubTables& ht, u32 v, cons
     ht.w[off + q] = V[q];
   }
   if (threadIdx.x == 0) {
-    ht.off[v] = off;
-    ht.cap[v] = cap;
+    ht.loc[v] = off | (u64(__ffs(int(cap)) - 1) << 56);
     atomicOr(&ht.bits[v >> 5], 1u << (v & 31));
   }
 }
@@ -871,8 +870,9 @@ __global__ __launch_bounds__(256) void k_prop2_ht(P pol, Csr g, const u32* list,
     if (has) {
       v = list[id];
       cur = pol.cur_key(v);
-      const u64 off = pol.ht.off[v];
-      const u32 cap = pol.ht.cap[v];
+      u64 off;
+      u32 cap;
+      pol.ht.where(v, off, cap);
       const u32* TK = pol.ht.key + off;
       const double* TW = pol.ht.w + off;
       for (u32 q = lane; q < cap; q += 64)
@@ -920,8 +920,9 @@ __global__ __launch_bounds__(256) void k_prop2_ht(P pol, Csr g, const u32* list,
 // then evaluated from its row, which rebuilds it).
 __device__ void ht_move(const HubTables& ht, u32 u, u32 from, u32 to, double w) {
   if (!ht.valid(u)) return;
-  const u64 off = ht.off[u];
-  const u32 cap = ht.cap[u];
+  u64 off;
+  u32 cap;
+  ht.where(u, off, cap);
   u32* TK = ht.key + off;
   u32 h = hash32(from) & (cap - 1);
   bool found = false;
@@ -1987,14 +1988,13 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     const u64 words = (u64(n) + 32) / 32;
     ms.ht_bits.reserve(words, s);
     fill_u32(ms.ht_bits.p, 0, words, s);
-    ms.ht_off.reserve(n + 1, s);
-    ms.ht_cap.reserve(n + 1, s);
+    ms.ht_loc.reserve(n + 1, s);
     ms.ht_key.reserve(m.ht.pool, s);
     ms.ht_w.reserve(m.ht.pool, s);
     ms.ht_used.reserve(1, s);
     hipLaunchKernelGGL(k_fill_u64_one, 1, 64, 0, s, ms.ht_used.p);
     HL_LAUNCH_CHECK();
-    m.ht = HubTables{ms.ht_bits.p, ms.ht_off.p, ms.ht_cap.p, ms.ht_key.p, ms.ht_w.p, ms.ht_used.p, m.ht.pool};
+    m.ht = HubTables{ms.ht_bits.p, ms.ht_loc.p, ms.ht_key.p, ms.ht_w.p, ms.ht_used.p, m.ht.pool};
   } else {
     m.ht = HubTables{};
   }
