@@ -58,6 +58,20 @@ def test_invalid_config_rejected():
         hl.HitLeiden(hl.RunConfig(resolution=-1.0))
 
 
+def test_config_flags_checked_before_the_device():
+    """hl_config's key_cache and hub_tables take 0 or 1; anything else is an
+    invalid config (reported before any device is looked for)."""
+    import hit_leiden_amd as hl
+    L = hl.lib()
+    for field in ("key_cache", "hub_tables"):
+        c = hl._Config()
+        c.gamma, c.mode, c.levels = 1.0, 1, 2
+        setattr(c, field, 2)
+        h = ctypes.c_void_p()
+        assert L.hl_create(ctypes.byref(c), ctypes.byref(h)) == hl.HL_INVALID_INPUT
+        assert not h.value
+
+
 def test_graph_input_shapes():
     import hit_leiden_amd as hl
     g = hl.GraphInput.from_edges(4, [(0, 1), (1, 2, 2.5)])
