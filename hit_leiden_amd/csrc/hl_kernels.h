@@ -45,6 +45,8 @@ void fill4_u8(u8* a0, u8* a1, u8* a2, u8* a3, u8 v, u64 n, hipStream_t s);
 void max_u32(const u32* a, u64 n, u32* out, hipStream_t s);
 // indices i with flags[i] != 0, ascending; returns the count (synchronises)
 u64 select_flagged(const u8* flags, u64 n, u32* out, Temp& tmp, DevBuf<u64>& cnt, hipStream_t s);
+// the same without the read-back: the count lands in *cnt_dev
+void select_flagged_async(const u8* flags, u64 n, u32* out, Temp& tmp, u64* cnt_dev, hipStream_t s);
 // exclusive scan of u64 values, writes n+1 entries (last = total); returns total
 u64 exclusive_scan_u64(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s);
 // the same without synchronising: the total is left in out[n]
@@ -186,7 +188,7 @@ struct AggScratch {
   DevBuf<u32> rlist, base32, runs, idx, idx_s, keep_idx;
   DevBuf<u64> cnt64, off64, keys, keys_s, cnt;
   DevBuf<u8> heads, keep;
-  DevBuf<u64> c1;
+  DevBuf<u64> c1, c2;
   DevBuf<u32> mx;
   DevBuf<Rec> comp;
 };

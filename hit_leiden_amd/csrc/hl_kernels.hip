@@ -314,36 +314,58 @@ __global__ void k_heads_u64(const u64* keys, u64 k, int shift, u8* heads) {
 // One thread per canonical key: existing weight (or 0) + alpha_1 + ... in delta
 // order (apply_delta_edges, in_memory.rs:133-138); the key survives iff > 1e-12
 // (in_memory.rs:110).  Emits one directed update per row end.
-__global__ void k_resolve_runs(u64 nruns, const u32* runs, u64 k, const u64* keys, const u32* idx, const Rec* d,
-                               LevelCsrMut g, u64* ukey, u32* uidx, u8* uop, double* uw, u32* ucount) {
-  for (u64 r = gtid(); r < nruns; r += gstride()) {
-    u64 s = runs[r], e = (r + 1 < nruns) ? runs[r + 1] : k;
-    u64 key = keys[s];
-    u32 a = u32(key >> 32), b = u32(key & 0xffffffffu);
-    u64 off = g.row_off[a];
-    u32 len = g.row_len[a];
-    u32 pos = lower_bound_u32(g.col + off, len, b);
-    bool found = pos < len && g.col[off + pos] == b;
-    double acc = found ? g.w[off + pos] : 0.0;
-    for (u64 t = s; t < e; ++t) acc += d[idx[t]].w;
-    u8 op;
-    if (acc > kEpsApply)
-      op = found ? OP_UPD : OP_INS;
-    else if (found)
-      op = OP_DEL;
-    else
-      continue;
-    u32 n_out = (a == b) ? 1u : 2u;
-    u32 base = atomicAdd(ucount, n_out);
-    ukey[base] = (u64(a) << 32) | b;
-    uop[base] = op;
-    uw[base] = acc;
-    uidx[base] = base;
-    if (a != b) {
-      ukey[base + 1] = (u64(b) << 32) | a;
-      uop[base + 1] = op;
-      uw[base + 1] = acc;
-      uidx[base + 1] = base + 1;
+__global__ void k_resolve_runs(const unsigned long long* nruns_dev, const u32* runs, u64 k, const u64* keys,
+                               const u32* idx, const Rec* d, LevelCsrMut g, u64* ukey, u32* uidx, u8* uop,
+                               double* uw, u32* ucount) {
+  // nruns from the device (the run selection's count), so the host need not
+  // read it back first; outputs appended with one atomic per wavefront
+  const u64 nruns = *nruns_dev;
+  const u32 lane = threadIdx.x & 63;
+  for (u64 base = gtid() - lane; base < nruns; base += gstride()) {  // uniform per wavefront
+    const u64 r = base + lane;
+    u32 n_out = 0, a = 0, b = 0;
+    u8 op = 0;
+    double acc = 0.0;
+    if (r < nruns) {
+      const u64 s = runs[r], e = (r + 1 < nruns) ? runs[r + 1] : k;
+      const u64 key = keys[s];
+      a = u32(key >> 32);
+      b = u32(key & 0xffffffffu);
+      const u64 off = g.row_off[a];
+      const u32 len = g.row_len[a];
+      const u32 pos = lower_bound_u32(g.col + off, len, b);
+      const bool found = pos < len && g.col[off + pos] == b;
+      acc = found ? g.w[off + pos] : 0.0;
+      for (u64 t = s; t < e; ++t) acc += d[idx[t]].w;
+      if (acc > kEpsApply) {
+        op = found ? OP_UPD : OP_INS;
+        n_out = (a == b) ? 1u : 2u;
+      } else if (found) {
+        op = OP_DEL;
+        n_out = (a == b) ? 1u : 2u;
+      }
+    }
+    u32 incl = n_out;
+    for (int o = 1; o < 64; o <<= 1) {
+      const u32 y = __shfl_up(incl, o, 64);
+      if (lane >= u32(o)) incl += y;
+    }
+    const u32 tot = __shfl(incl, 63, 64);
+    u32 wbase = 0;
+    if (lane == 0 && tot) wbase = atomicAdd(ucount, tot);
+    wbase = __shfl(wbase, 0, 64);
+    if (n_out) {
+      const u32 at = wbase + incl - n_out;
+      ukey[at] = (u64(a) << 32) | b;
+      uop[at] = op;
+      uw[at] = acc;
+      uidx[at] = at;
+      if (n_out == 2) {
+        ukey[at + 1] = (u64(b) << 32) | a;
+        uop[at + 1] = op;
+        uw[at + 1] = acc;
+        uidx[at + 1] = at + 1;
+      }
     }
   }
 }
@@ -351,6 +373,9 @@ __global__ void k_resolve_runs(u64 nruns, const u32* runs, u64 k, const u64* key
 __global__ void k_row_plan(u64 nseg, const u32* segs, u64 nupd, const u64* ukey_s, const u32* perm, const u8* uop,
                            const u32* row_len, u32* seg_row, u32* seg_len, u32* new_len, u32* seg_ins, u32* seg_del,
                            u32* ins_pre, u32* del_pre, u64* newcap, u64* chunks, u64* old_sum) {
+  // old_sum[0] += old row lengths, old_sum[1] = max new length: per-thread
+  // partials, one atomic each per workgroup
+  unsigned long long osum = 0, omax = 0;
   for (u64 j = gtid(); j < nseg; j += gstride()) {
     u64 s = segs[j], e = (j + 1 < nseg) ? segs[j + 1] : nupd;
     u32 r = u32(ukey_s[s] >> 32);
@@ -372,8 +397,28 @@ __global__ void k_row_plan(u64 nseg, const u32* segs, u64 nupd, const u64* ukey_
     newcap[j] = nl;
     u64 items = u64(old) + (e - s);
     chunks[j] = (items + MERGE_CHUNK - 1) / MERGE_CHUNK;
-    atomicAdd((unsigned long long*)old_sum, (unsigned long long)old);
-    atomicMax((unsigned long long*)(old_sum + 1), (unsigned long long)nl);  // largest new row
+    osum += old;
+    omax = omax > nl ? omax : nl;
+  }
+  __shared__ unsigned long long ps[4], pm[4];
+  for (int o = 32; o > 0; o >>= 1) {
+    osum += __shfl_xor(osum, o, 64);
+    const unsigned long long y = __shfl_xor(omax, o, 64);
+    omax = omax > y ? omax : y;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ps[threadIdx.x >> 6] = osum;
+    pm[threadIdx.x >> 6] = omax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0, m = 0;
+    for (u32 w = 0; w < blockDim.x / 64; ++w) {
+      t += ps[w];
+      m = m > pm[w] ? m : pm[w];
+    }
+    if (t) atomicAdd((unsigned long long*)old_sum, t);
+    if (m) atomicMax((unsigned long long*)(old_sum + 1), m);
   }
 }
 
@@ -1294,8 +1339,12 @@ __global__ void k_hkeys(const Rec* H, u64 n, u64* keys, u32* idx) {
   }
 }
 // Compress (hit_leiden.rs:1701-1714): per key in emission order, |w| > 1e-9 kept
-__global__ void k_compress(u64 nruns, const u32* runs, u64 n, const u64* keys, const u32* idx, const Rec* H, Rec* out,
-                           u8* keep) {
+// nruns from the device (the run selection's count); keep[nruns..n) is
+// cleared so that the survivors can be selected over all n flags
+__global__ void k_compress(const unsigned long long* nruns_dev, const u32* runs, u64 n, const u64* keys,
+                           const u32* idx, const Rec* H, Rec* out, u8* keep) {
+  const u64 nruns = *nruns_dev;
+  for (u64 r = nruns + gtid(); r < n; r += gstride()) keep[r] = 0;
   for (u64 r = gtid(); r < nruns; r += gstride()) {
     u64 s = runs[r], e = (r + 1 < nruns) ? runs[r + 1] : n;
     double acc = 0.0;
@@ -1602,6 +1651,21 @@ void max_u32(const u32* a, u64 n, u32* out, hipStream_t s) {
   HL_LAUNCH_CHECK();
 }
 
+void select_flagged_async(const u8* flags, u64 n, u32* out, Temp& tmp, u64* cnt_dev, hipStream_t s) {
+  if (n == 0) {
+    hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, cnt_dev, u64(0), u64(1));
+    HL_LAUNCH_CHECK();
+    return;
+  }
+  const u64 nb = (n + kSelChunk - 1) / kSelChunk;
+  u32* bcnt = static_cast<u32*>(tmp.get(nb * sizeof(u32), s));
+  HL_PROF(P_SELECT, n, s);
+  hipLaunchKernelGGL(k_sel_count, dim3(unsigned(nb)), dim3(256), 0, s, flags, n, bcnt);
+  hipLaunchKernelGGL(k_sel_write, dim3(unsigned(nb)), dim3(256), 0, s, flags, n, bcnt, out,
+                     reinterpret_cast<unsigned long long*>(cnt_dev));
+  HL_LAUNCH_CHECK();
+}
+
 u64 select_flagged(const u8* flags, u64 n, u32* out, Temp& tmp, DevBuf<u64>& cnt, hipStream_t s) {
   if (n == 0) return 0;
   cnt.reserve(1, s);
@@ -1697,8 +1761,11 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   hipLaunchKernelGGL(k_heads_u64, GRID(k), a.keys_s.p, k, 0, a.heads.p);
   HL_LAUNCH_CHECK();
   DevBuf<u64>& c1 = a.c1;
-  plan.nruns = select_flagged(a.heads.p, k, a.runs.p, tmp, c1, s);
-  const u64 maxupd = 2 * plan.nruns;
+  c1.reserve(1, s);
+  // the runs' count stays on the device: the resolve kernel reads it (grid
+  // sized for k >= runs), and one read-back returns both counts
+  select_flagged_async(a.heads.p, k, a.runs.p, tmp, c1.p, s);
+  const u64 maxupd = 2 * k;
   a.ukey.reserve(maxupd, s);
   a.ukey_s.reserve(maxupd, s);
   a.uidx.reserve(maxupd, s);
@@ -1708,15 +1775,19 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   a.ucount.reserve(1, s);
   fill_u32(a.ucount.p, 0, 1, s);
   {
-  HL_PROF(P_RESOLVE, 16 * k + 32 * plan.nruns, s);
-  hipLaunchKernelGGL(k_resolve_runs, GRID(plan.nruns), plan.nruns, a.runs.p, k, a.keys_s.p, a.idx_s.p, d, g,
-                     a.ukey.p, a.uidx.p, a.uop.p, a.uw.p, a.ucount.p);
+  HL_PROF(P_RESOLVE, 0, s);
+  hipLaunchKernelGGL(k_resolve_runs, GRID(k), reinterpret_cast<const unsigned long long*>(c1.p), a.runs.p, k,
+                     a.keys_s.p, a.idx_s.p, d, g, a.ukey.p, a.uidx.p, a.uop.p, a.uw.p, a.ucount.p);
   HL_LAUNCH_CHECK();
   }
   u32 nupd = 0;
+  u64 nruns = 0;
   d2h_async(&nupd, a.ucount.p, sizeof(u32), s);
+  d2h_async(&nruns, c1.p, sizeof(u64), s);
   stream_sync(s);
+  plan.nruns = nruns;
   plan.nupd = nupd;
+  if (g_prof && g_prof->records(P_RESOLVE)) g_prof->bytes[P_RESOLVE] += 16 * k + 32 * nruns;
   if (nupd == 0) return;
   sort_pairs_u64(a.ukey.p, a.ukey_s.p, a.uidx.p, a.perm.p, nupd, 64, tmp, s);
   a.heads.reserve(nupd, s);
@@ -1925,17 +1996,22 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
   hipLaunchKernelGGL(k_aff_mark, GRID(n), K, sm, n, rs.aff.p, rs.status.p);
   HL_LAUNCH_CHECK();
   max_u32(sm, n, rs.status.p + 2, s);
+  // the members of affected subcommunities are listed before the read-back,
+  // so one synchronisation returns the flags, the label bound and their count
+  rs.incc.reserve(n, s);
+  rs.vcc.reserve(n, s);
+  rs.cnt.reserve(1, s);
+  hipLaunchKernelGGL(k_incc, GRID(n), sm, n, rs.aff.p, rs.incc.p);
+  HL_LAUNCH_CHECK();
+  select_flagged_async(rs.incc.p, n, rs.vcc.p, tmp, rs.cnt.p, s);
   u32 hs[3] = {0, 0, 0};
+  u64 m = 0;
   d2h_async(hs, rs.status.p, 3 * sizeof(u32), s);
+  d2h_async(&m, rs.cnt.p, sizeof(u64), s);
   stream_sync(s);
   *identity = hs[1] == 0;
   const u32 next_id = hs[2] + 1;
   if (!hs[0]) return next_id;
-  rs.incc.reserve(n, s);
-  rs.vcc.reserve(n, s);
-  hipLaunchKernelGGL(k_incc, GRID(n), sm, n, rs.aff.p, rs.incc.p);
-  HL_LAUNCH_CHECK();
-  const u64 m = select_flagged(rs.incc.p, n, rs.vcc.p, tmp, rs.cnt, s);
   if (m == 0) return next_id;
   rs.parent.reserve(n, s);
   rs.cnt32.reserve(n, s);
@@ -2129,16 +2205,21 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
   sort_pairs_u64(as.keys.p, as.keys_s.p, as.idx.p, as.idx_s.p, total, 64, tmp, s);
   hipLaunchKernelGGL(k_heads_u64, GRID(total), as.keys_s.p, total, 0, as.heads.p);
   HL_LAUNCH_CHECK();
-  const u64 nruns = select_flagged(as.heads.p, total, as.runs.p, tmp, c1, s);
+  // the runs' count stays on the device (sizes use total >= runs), so the
+  // survivors' count is the only read-back
+  c1.reserve(1, s);
+  select_flagged_async(as.heads.p, total, as.runs.p, tmp, c1.p, s);
   DevBuf<Rec>& comp = as.comp;
-  comp.reserve(nruns, s);
-  as.keep.reserve(nruns, s);
-  HL_PROF(P_COMPRESS, 16 * total + 17 * nruns, s);
-  hipLaunchKernelGGL(k_compress, GRID(nruns), nruns, as.runs.p, total, as.keys_s.p, as.idx_s.p, as.H.p, comp.p,
-                     as.keep.p);
-  HL_LAUNCH_CHECK();
-  as.keep_idx.reserve(nruns, s);
-  const u64 kept = select_flagged(as.keep.p, nruns, as.keep_idx.p, tmp, c1, s);
+  comp.reserve(total, s);
+  as.keep.reserve(total, s);
+  {
+    HL_PROF(P_COMPRESS, 33 * total, s);
+    hipLaunchKernelGGL(k_compress, GRID(total), reinterpret_cast<const unsigned long long*>(c1.p), as.runs.p, total,
+                       as.keys_s.p, as.idx_s.p, as.H.p, comp.p, as.keep.p);
+    HL_LAUNCH_CHECK();
+  }
+  as.keep_idx.reserve(total, s);
+  const u64 kept = select_flagged(as.keep.p, total, as.keep_idx.p, tmp, as.c2, s);
   out.reserve(kept + 1, s);
   if (kept) hipLaunchKernelGGL(k_gather_rec, GRID(kept), as.keep_idx.p, kept, comp.p, out.p);
   HL_LAUNCH_CHECK();
