@@ -441,7 +441,7 @@ struct Engine {
       if (tried[b] > prop[b]) tried[b] = prop[b];
       total += prop[b];
     }
-    // hub-table evaluations: 20 + 16 per table slot (key, count, weight) + 8 per live candidate
+    // hub-table evaluations: 20 + 16 per table slot (key and weight) + 8 per live candidate
     const u64 htab = 20 * d[DS_HT] + 16 * d[DS_HT + 1] + 8 * d[DS_HT + 2];
     total += htab;
     const u64 dec = 20 * d[DS_DEC_CAND] + 28 * d[DS_DEC_ACC] + 12 * d[DS_DEC_DEG];

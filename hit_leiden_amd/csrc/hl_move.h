@@ -37,11 +37,17 @@ struct FlatMarks {
 // residue of a few ulps and acts as a non-neighbour community, whose gain is
 // never above the empty community's).  A table that fills up is dropped and
 // rebuilt by the next full evaluation.
+// one table slot: the key and its weight share a 16-byte slot, so an update
+// touches one line per key
+struct alignas(16) HtSlot {
+  u32 key;
+  u32 pad;
+  double w;
+};
 struct HubTables {
   u32* bits = nullptr;   // validity bitmap (n bits); nullptr: tables off
   u64* loc = nullptr;    // first slot of v's table in the pool (bits 0-47) and log2 of its slots (56-63)
-  u32* key = nullptr;    // pool: community id or kNone
-  double* w = nullptr;   //       weight of v's row entries in that community
+  HtSlot* slot = nullptr;  // pool: community id (or kNone) and the weight of v's row entries in it
   unsigned long long* used = nullptr;  // pool slots handed out (in chunks)
   u64 pool = 0;
   __device__ bool valid(u32 v) const { return (bits[v >> 5] >> (v & 31)) & 1u; }
@@ -176,9 +182,9 @@ struct MoveScratch {
   DevBuf<u32> ck;  // neighbour-community cache of the level in movement
   DevBuf<u32> mfrom;  // source communities of the moves marked over the grid
   // hub tables (HubTables)
-  DevBuf<u32> ht_bits, ht_key;
+  DevBuf<u32> ht_bits;
+  DevBuf<HtSlot> ht_slot;
   DevBuf<u64> ht_loc;
-  DevBuf<double> ht_w;
   DevBuf<unsigned long long> ht_used;
   size_t target_cap = 0;
   u32 epoch = 0;

@@ -735,8 +735,7 @@ __device__ __forceinline__ void build_hub_table(const HubTables& ht, u32 v, cons
   const u64 off = s_off;
   if (off == ~0ull) return;
   for (u32 q = threadIdx.x; q < cap; q += blockDim.x) {
-    ht.key[off + q] = K[q];
-    ht.w[off + q] = V[q];
+    ht.slot[off + q] = HtSlot{K[q], 0u, V[q]};
   }
   if (threadIdx.x == 0) {
     ht.loc[v] = off | (u64(__ffs(int(cap)) - 1) << 56);
@@ -873,17 +872,17 @@ __global__ __launch_bounds__(256) void k_prop2_ht(P pol, Csr g, const u32* list,
       u64 off;
       u32 cap;
       pol.ht.where(v, off, cap);
-      const u32* TK = pol.ht.key + off;
-      const double* TW = pol.ht.w + off;
+      const HtSlot* T = pol.ht.slot + off;
       for (u32 q = lane; q < cap; q += 64)
-        if (TK[q] == cur) wcur = TW[q];  // 0 when v has no neighbour left there
+        if (T[q].key == cur) wcur = T[q].w;  // 0 when v has no neighbour left there
       for (int o = 32; o > 0; o >>= 1) wcur += __shfl_xor(wcur, o, 64);  // one lane at most holds it
       kv = g.k[v];
       sc = pol.cur_total(cur);
       tt = tm_of(g, v);
       for (u32 q = lane; q < cap; q += 64) {
-        const u32 key = TK[q];
-        const double wq = TW[q];
+        const HtSlot sl = T[q];
+        const u32 key = sl.key;
+        const double wq = sl.w;
         if (key == kNone || wq == 0.0) continue;
         ++ncand;
         if (key == cur) continue;
@@ -923,11 +922,11 @@ __device__ void ht_move(const HubTables& ht, u32 u, u32 from, u32 to, double w) 
   u64 off;
   u32 cap;
   ht.where(u, off, cap);
-  u32* TK = ht.key + off;
+  HtSlot* T = ht.slot + off;
   u32 h = hash32(from) & (cap - 1);
   bool found = false;
   for (u32 t = 0; t < cap; ++t) {
-    const u32 k = TK[h];
+    const u32 k = T[h].key;
     if (k == from) {
       found = true;
       break;
@@ -939,16 +938,16 @@ __device__ void ht_move(const HubTables& ht, u32 u, u32 from, u32 to, double w) 
     atomicAnd(&ht.bits[u >> 5], ~(1u << (u & 31)));
     return;
   }
-  atomicAdd(&ht.w[off + h], -w);
+  atomicAdd(&T[h].w, -w);
   // `to`: an existing key is found by plain loads (no returning atomic on the
   // common path); an empty-looking slot is claimed with a compare-and-swap,
   // which also settles a load that missed a key inserted during this kernel
   h = hash32(to) & (cap - 1);
   for (u32 t = 0; t < cap; ++t) {
-    u32 k = TK[h];
-    if (k == kNone) k = atomicCAS(&TK[h], kNone, to);
+    u32 k = T[h].key;
+    if (k == kNone) k = atomicCAS(&T[h].key, kNone, to);
     if (k == kNone || k == to) {
-      atomicAdd(&ht.w[off + h], w);
+      atomicAdd(&T[h].w, w);
       return;
     }
     h = (h + 1) & (cap - 1);
@@ -1828,9 +1827,9 @@ void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
 }
 
 void MoveScratch::trim() {
-  for (DevBuf<u32>* b : {&tk, &hovf, &ck, &k32, &k32b, &v32, &o1, &o2, &node, &from, &to, &state, &acc, &ht_key})
+  for (DevBuf<u32>* b : {&tk, &hovf, &ck, &k32, &k32b, &v32, &o1, &o2, &node, &from, &to, &state, &acc})
     b->release();
-  ht_w.release();
+  ht_slot.release();
   tv.release();
   k64.release();
   k64b.release();
@@ -1989,12 +1988,11 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     ms.ht_bits.reserve(words, s);
     fill_u32(ms.ht_bits.p, 0, words, s);
     ms.ht_loc.reserve(n + 1, s);
-    ms.ht_key.reserve(m.ht.pool, s);
-    ms.ht_w.reserve(m.ht.pool, s);
+    ms.ht_slot.reserve(m.ht.pool, s);
     ms.ht_used.reserve(1, s);
     hipLaunchKernelGGL(k_fill_u64_one, 1, 64, 0, s, ms.ht_used.p);
     HL_LAUNCH_CHECK();
-    m.ht = HubTables{ms.ht_bits.p, ms.ht_loc.p, ms.ht_key.p, ms.ht_w.p, ms.ht_used.p, m.ht.pool};
+    m.ht = HubTables{ms.ht_bits.p, ms.ht_loc.p, ms.ht_slot.p, ms.ht_used.p, m.ht.pool};
   } else {
     m.ht = HubTables{};
   }
