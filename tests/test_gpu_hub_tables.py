@@ -105,3 +105,32 @@ def test_tables_weighted():
     assert_classes_connected(n, r[keep], col[keep].astype(np.int64), g.partition(0, hl.MAP_S_CUR))
     assert_supergraph_consistent(g, 3, tol=1e-9)
     g.close()
+
+
+def test_tables_in_the_batched_form():
+    """Several graphs with hubs in one engine: the tables are per vertex, so each
+    graph still gets exactly the update it gets alone."""
+    from hit_leiden_amd import _first_seen
+    graphs = []
+    for i, (n, hub) in enumerate([(6000, 5000), (9000, 700), (4000, 0), (12000, 9000)]):
+        if hub:
+            u, v = with_hubs(n, hub, 2, 40 + i)
+        else:
+            w = gen.sbm(n, 8, 8.0, 0.2, seed=40 + i)
+            u, v = w.u, w.v
+        graphs.append(hl.GraphInput(n, u, v))
+    batches = [random_batch(g.node_count, g.src, g.dst, None, 300, 60 + i) for i, g in enumerate(graphs)]
+    cfg = hl.RunConfig(mode=THR, levels=3)
+    e = hl.HitLeiden(cfg)
+    e.load_graphs(graphs)
+    e.apply_graphs({i: hl.GraphInput(g.node_count, *batches[i]) for i, g in enumerate(graphs)})
+    qs = e.modularity_graphs()
+    for i, g in enumerate(graphs):
+        a = hl.HitLeiden(cfg)
+        a.load(g)
+        a.hit_leiden(hl.GraphInput(g.node_count, *batches[i]))
+        for p, (x, y) in enumerate(zip(a.hierarchy(), e.graph_hierarchy(i))):
+            np.testing.assert_array_equal(_first_seen(x), _first_seen(y), err_msg=f"graph {i} row {p}")
+        assert abs(qs[i] - a.modularity(hl.PART_COMMUNITY)) < 1e-12
+        a.close()
+    e.close()
