@@ -1269,42 +1269,67 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
     }
     __syncthreads();
   } else {
-    u32 N = 1;
-    while (N < c) N <<= 1;
-    for (u32 i = tid; i < N; i += blockDim.x) {
-      if (i < c) {
-        skey[i] = ~(u64)__double_as_longlong(cand.gain[i]);
-        snode[i] = cand.node[i];
-        sidx[i] = i;
-      } else {
-        skey[i] = ~0ull;
-        snode[i] = kNone;
-        sidx[i] = kNone;
+    if (c <= 1024) {
+      // rank sort on (key, node), one candidate per thread: every thread counts
+      // the candidates ordered before its own (nodes are distinct, so ranks are
+      // too); all threads read the same entry at a time, an LDS broadcast
+      u64 mk = ~0ull;
+      u32 mn = kNone;
+      if (tid < c) {
+        mk = ~(u64)__double_as_longlong(cand.gain[tid]);
+        mn = cand.node[tid];
+        skey[tid] = mk;
+        snode[tid] = mn;
       }
-    }
-    __syncthreads();
-    // bitonic sort of (key, node) ascending
-    for (u32 kk = 2; kk <= N; kk <<= 1) {
-      for (u32 j = kk >> 1; j > 0; j >>= 1) {
-        for (u32 i = tid; i < N; i += blockDim.x) {
-          u32 ixj = i ^ j;
-          if (ixj > i) {
-            bool up = (i & kk) == 0;
-            bool gt = pri_less(skey[ixj], snode[ixj], skey[i], snode[i]);
-            if (gt == up) {
-              u64 tk = skey[i];
-              skey[i] = skey[ixj];
-              skey[ixj] = tk;
-              u32 tn = snode[i];
-              snode[i] = snode[ixj];
-              snode[ixj] = tn;
-              u32 ti = sidx[i];
-              sidx[i] = sidx[ixj];
-              sidx[ixj] = ti;
+      __syncthreads();
+      u32 rank = 0;
+      if (tid < c)
+        for (u32 j = 0; j < c; ++j) rank += pri_less(skey[j], snode[j], mk, mn) ? 1u : 0u;
+      __syncthreads();
+      if (tid < c) {
+        skey[rank] = mk;
+        snode[rank] = mn;
+        sidx[rank] = tid;
+      }
+      __syncthreads();
+    } else {
+      u32 N = 1;
+      while (N < c) N <<= 1;
+      for (u32 i = tid; i < N; i += blockDim.x) {
+        if (i < c) {
+          skey[i] = ~(u64)__double_as_longlong(cand.gain[i]);
+          snode[i] = cand.node[i];
+          sidx[i] = i;
+        } else {
+          skey[i] = ~0ull;
+          snode[i] = kNone;
+          sidx[i] = kNone;
+        }
+      }
+      __syncthreads();
+      // bitonic sort of (key, node) ascending
+      for (u32 kk = 2; kk <= N; kk <<= 1) {
+        for (u32 j = kk >> 1; j > 0; j >>= 1) {
+          for (u32 i = tid; i < N; i += blockDim.x) {
+            u32 ixj = i ^ j;
+            if (ixj > i) {
+              bool up = (i & kk) == 0;
+              bool gt = pri_less(skey[ixj], snode[ixj], skey[i], snode[i]);
+              if (gt == up) {
+                u64 tk = skey[i];
+                skey[i] = skey[ixj];
+                skey[ixj] = tk;
+                u32 tn = snode[i];
+                snode[i] = snode[ixj];
+                snode[ixj] = tn;
+                u32 ti = sidx[i];
+                sidx[i] = sidx[ixj];
+                sidx[ixj] = ti;
+              }
             }
           }
+          __syncthreads();
         }
-        __syncthreads();
       }
     }
     for (u32 p = tid; p < c; p += blockDim.x) {
