@@ -189,11 +189,19 @@ __global__ void k_iota_u32(u32* p, u32 base, u64 n) {
 __global__ void k_copy_u8(u8* d, const u8* s, u64 n) {
   for (u64 i = gtid(); i < n; i += gstride()) d[i] = s[i];
 }
-// grid-stride max with one atomic per workgroup
+// grid-stride max with one atomic per workgroup; 16-byte loads where the
+// array is aligned (device allocations are)
 __global__ __launch_bounds__(256) void k_max_u32(const u32* a, u64 n, u32* out) {
   __shared__ u32 sm[4];
   u32 m = 0;
-  for (u64 i = gtid(); i < n; i += gstride()) m = max(m, a[i]);
+  const bool vec = (reinterpret_cast<uintptr_t>(a) & 15) == 0;
+  const u64 nv = vec ? n / 4 : 0;
+  const uint4* a4 = reinterpret_cast<const uint4*>(a);
+  for (u64 i = gtid(); i < nv; i += gstride()) {
+    const uint4 q = a4[i];
+    m = max(m, max(max(q.x, q.y), max(q.z, q.w)));
+  }
+  for (u64 i = 4 * nv + gtid(); i < n; i += gstride()) m = max(m, a[i]);
   for (int o = 32; o > 0; o >>= 1) m = max(m, (u32)__shfl_xor((int)m, o, 64));
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
   __syncthreads();
@@ -202,78 +210,23 @@ __global__ __launch_bounds__(256) void k_max_u32(const u32* a, u64 n, u32* out) 
     if (m) atomicMax(out, m);
   }
 }
-// Ordered stream compaction of a byte-flag array into the ascending list of
-// flagged indices, in two launches and without a look-back: each workgroup owns
-// a chunk of kSelChunk flags (kSelPer consecutive flags per thread).
-constexpr u32 kSelPer = 16, kSelChunk = 256 * kSelPer;
-__device__ __forceinline__ u32 sel_load(const u8* flags, u64 n, u64 i0, u8 (&f)[kSelPer]) {
-  u32 c = 0;
-  if (i0 + kSelPer <= n && ((reinterpret_cast<uintptr_t>(flags + i0) & 15) == 0)) {
-    const uint4 q = *reinterpret_cast<const uint4*>(flags + i0);
-    const u32 wds[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (u32 j = 0; j < kSelPer; ++j) f[j] = u8(wds[j >> 2] >> (8 * (j & 3)));
-  } else {
-#pragma unroll
-    for (u32 j = 0; j < kSelPer; ++j) f[j] = (i0 + j < n) ? flags[i0 + j] : u8(0);
-  }
-#pragma unroll
-  for (u32 j = 0; j < kSelPer; ++j) c += f[j] != 0;
-  return c;
+// counts of nonzero bytes in up to four arrays, one atomic per workgroup and
+// array; four flags per 4-byte load where the arrays are aligned
+__device__ __forceinline__ u32 nz_bytes(u32 x) {
+  return u32((x & 0xFFu) != 0) + u32((x & 0xFF00u) != 0) + u32((x & 0xFF0000u) != 0) + u32((x >> 24) != 0);
 }
-// block-wide sum of x (256 threads); every thread gets the total
-__device__ __forceinline__ u32 sel_block_sum(u32 x, u32* sm) {
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = x;
-  __syncthreads();
-  return sm[0] + sm[1] + sm[2] + sm[3];
-}
-__global__ __launch_bounds__(256) void k_sel_count(const u8* flags, u64 n, u32* bcnt) {
-  __shared__ u32 sm[4];
-  u8 f[kSelPer];
-  const u64 i0 = u64(blockIdx.x) * kSelChunk + u64(threadIdx.x) * kSelPer;
-  const u32 t = sel_block_sum(sel_load(flags, n, i0, f), sm);
-  if (threadIdx.x == 0) bcnt[blockIdx.x] = t;
-}
-__global__ __launch_bounds__(256) void k_sel_write(const u8* flags, u64 n, const u32* bcnt, u32* out,
-                                                   unsigned long long* total) {
-  __shared__ u32 wsum[4];
-  // this chunk's offset: the counts of the chunks before it
-  u64 pre = 0;
-  for (u32 b = threadIdx.x; b < blockIdx.x; b += 256) pre += bcnt[b];
-  for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o, 64);
-  __shared__ u64 psm[4];
-  if ((threadIdx.x & 63) == 0) psm[threadIdx.x >> 6] = pre;
-  __syncthreads();
-  pre = psm[0] + psm[1] + psm[2] + psm[3];
-  u8 f[kSelPer];
-  const u64 i0 = u64(blockIdx.x) * kSelChunk + u64(threadIdx.x) * kSelPer;
-  const u32 c = sel_load(flags, n, i0, f);
-  // exclusive prefix of c over the workgroup: wavefront scan, then wave sums
-  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  u32 inc = c;
-  for (int o = 1; o < 64; o <<= 1) {
-    const u32 y = __shfl_up(inc, o, 64);
-    if (lane >= u32(o)) inc += y;
-  }
-  if (lane == 63) wsum[wv] = inc;
-  __syncthreads();
-  u32 wbase = 0;
-  for (u32 i = 0; i < wv; ++i) wbase += wsum[i];
-  u64 pos = pre + wbase + inc - c;
-#pragma unroll
-  for (u32 j = 0; j < kSelPer; ++j)
-    if (f[j]) out[pos++] = u32(i0 + j);
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) *total = pos;
-}
-// counts of nonzero bytes in up to four arrays, one atomic per workgroup and array
 __global__ __launch_bounds__(256) void k_count4(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n,
                                                  unsigned long long* out) {
   __shared__ unsigned long long sm[4][4];
   const u8* arr[4] = {a0, a1, a2, a3};
+  bool vec = true;
+  for (int j = 0; j < 4; ++j) vec = vec && (reinterpret_cast<uintptr_t>(arr[j]) & 3) == 0;
+  const u64 nv = vec ? n / 4 : 0;
   unsigned long long c[4] = {0, 0, 0, 0};
-  for (u64 i = gtid(); i < n; i += gstride())
+  for (u64 i = gtid(); i < nv; i += gstride())
+    for (int j = 0; j < 4; ++j)
+      if (arr[j]) c[j] += nz_bytes(reinterpret_cast<const u32*>(arr[j])[i]);
+  for (u64 i = 4 * nv + gtid(); i < n; i += gstride())
     for (int j = 0; j < 4; ++j)
       if (arr[j]) c[j] += (arr[j][i] != 0);
   for (int j = 0; j < 4; ++j) {
@@ -1620,7 +1573,7 @@ void count4_u8(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, De
   HL_CHECK(hipMemsetAsync(cnt.p, 0, 4 * sizeof(u64), s));
   if (n) {
     HL_PROF(P_REDUCE, 4 * n, s);
-    hipLaunchKernelGGL(k_count4, blocks_for(n, 256, kReduceBlocks), 256, 0, s, a0, a1, a2, a3, n,
+    hipLaunchKernelGGL(k_count4, blocks_for(n / 4 + 1, 256, 1024), 256, 0, s, a0, a1, a2, a3, n,
                        (unsigned long long*)cnt.p);
   }
   HL_LAUNCH_CHECK();
@@ -1631,7 +1584,7 @@ void count4_u8_async(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, out, u64(0), u64(4));
   if (n) {
     HL_PROF(P_REDUCE, 4 * n, s);
-    hipLaunchKernelGGL(k_count4, blocks_for(n, 256, kReduceBlocks), 256, 0, s, a0, a1, a2, a3, n, (unsigned long long*)out);
+    hipLaunchKernelGGL(k_count4, blocks_for(n / 4 + 1, 256, 1024), 256, 0, s, a0, a1, a2, a3, n, (unsigned long long*)out);
   }
   HL_LAUNCH_CHECK();
 }
@@ -1647,7 +1600,7 @@ void fill4_u8(u8* a0, u8* a1, u8* a2, u8* a3, u8 v, u64 n, hipStream_t s) {
 }
 void max_u32(const u32* a, u64 n, u32* out, hipStream_t s) {
   HL_PROF(P_REDUCE, 4 * n, s);
-  if (n) hipLaunchKernelGGL(k_max_u32, blocks_for(n, 256, kReduceBlocks), 256, 0, s, a, n, out);
+  if (n) hipLaunchKernelGGL(k_max_u32, blocks_for(n / 4 + 1, 256, 1024), 256, 0, s, a, n, out);
   HL_LAUNCH_CHECK();
 }
 
