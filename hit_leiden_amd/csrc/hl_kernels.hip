@@ -210,6 +210,71 @@ __global__ __launch_bounds__(256) void k_max_u32(const u32* a, u64 n, u32* out) 
     if (m) atomicMax(out, m);
   }
 }
+// Ordered stream compaction of a byte-flag array into the ascending list of
+// flagged indices, in two launches and without a look-back: each workgroup owns
+// a chunk of kSelChunk flags (kSelPer consecutive flags per thread).
+constexpr u32 kSelPer = 16, kSelChunk = 256 * kSelPer;
+__device__ __forceinline__ u32 sel_load(const u8* flags, u64 n, u64 i0, u8 (&f)[kSelPer]) {
+  u32 c = 0;
+  if (i0 + kSelPer <= n && ((reinterpret_cast<uintptr_t>(flags + i0) & 15) == 0)) {
+    const uint4 q = *reinterpret_cast<const uint4*>(flags + i0);
+    const u32 wds[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (u32 j = 0; j < kSelPer; ++j) f[j] = u8(wds[j >> 2] >> (8 * (j & 3)));
+  } else {
+#pragma unroll
+    for (u32 j = 0; j < kSelPer; ++j) f[j] = (i0 + j < n) ? flags[i0 + j] : u8(0);
+  }
+#pragma unroll
+  for (u32 j = 0; j < kSelPer; ++j) c += f[j] != 0;
+  return c;
+}
+// block-wide sum of x (256 threads); every thread gets the total
+__device__ __forceinline__ u32 sel_block_sum(u32 x, u32* sm) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = x;
+  __syncthreads();
+  return sm[0] + sm[1] + sm[2] + sm[3];
+}
+__global__ __launch_bounds__(256) void k_sel_count(const u8* flags, u64 n, u32* bcnt) {
+  __shared__ u32 sm[4];
+  u8 f[kSelPer];
+  const u64 i0 = u64(blockIdx.x) * kSelChunk + u64(threadIdx.x) * kSelPer;
+  const u32 t = sel_block_sum(sel_load(flags, n, i0, f), sm);
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = t;
+}
+__global__ __launch_bounds__(256) void k_sel_write(const u8* flags, u64 n, const u32* bcnt, u32* out,
+                                                   unsigned long long* total) {
+  __shared__ u32 wsum[4];
+  // this chunk's offset: the counts of the chunks before it
+  u64 pre = 0;
+  for (u32 b = threadIdx.x; b < blockIdx.x; b += 256) pre += bcnt[b];
+  for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o, 64);
+  __shared__ u64 psm[4];
+  if ((threadIdx.x & 63) == 0) psm[threadIdx.x >> 6] = pre;
+  __syncthreads();
+  pre = psm[0] + psm[1] + psm[2] + psm[3];
+  u8 f[kSelPer];
+  const u64 i0 = u64(blockIdx.x) * kSelChunk + u64(threadIdx.x) * kSelPer;
+  const u32 c = sel_load(flags, n, i0, f);
+  // exclusive prefix of c over the workgroup: wavefront scan, then wave sums
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  u32 inc = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const u32 y = __shfl_up(inc, o, 64);
+    if (lane >= u32(o)) inc += y;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  u32 wbase = 0;
+  for (u32 i = 0; i < wv; ++i) wbase += wsum[i];
+  u64 pos = pre + wbase + inc - c;
+#pragma unroll
+  for (u32 j = 0; j < kSelPer; ++j)
+    if (f[j]) out[pos++] = u32(i0 + j);
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) *total = pos;
+}
 // counts of nonzero bytes in up to four arrays, one atomic per workgroup and
 // array; four flags per 4-byte load where the arrays are aligned
 __device__ __forceinline__ u32 nz_bytes(u32 x) {
