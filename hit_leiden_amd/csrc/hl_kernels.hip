@@ -1638,7 +1638,7 @@ void count4_u8(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, De
   HL_CHECK(hipMemsetAsync(cnt.p, 0, 4 * sizeof(u64), s));
   if (n) {
     HL_PROF(P_REDUCE, 4 * n, s);
-    hipLaunchKernelGGL(k_count4, blocks_for(n / 4 + 1, 256, 1024), 256, 0, s, a0, a1, a2, a3, n,
+    hipLaunchKernelGGL(k_count4, blocks_for(n / 4 + 1, 256, kReduceBlocks), 256, 0, s, a0, a1, a2, a3, n,
                        (unsigned long long*)cnt.p);
   }
   HL_LAUNCH_CHECK();
@@ -1649,7 +1649,7 @@ void count4_u8_async(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, out, u64(0), u64(4));
   if (n) {
     HL_PROF(P_REDUCE, 4 * n, s);
-    hipLaunchKernelGGL(k_count4, blocks_for(n / 4 + 1, 256, 1024), 256, 0, s, a0, a1, a2, a3, n, (unsigned long long*)out);
+    hipLaunchKernelGGL(k_count4, blocks_for(n / 4 + 1, 256, kReduceBlocks), 256, 0, s, a0, a1, a2, a3, n, (unsigned long long*)out);
   }
   HL_LAUNCH_CHECK();
 }
@@ -1665,7 +1665,7 @@ void fill4_u8(u8* a0, u8* a1, u8* a2, u8* a3, u8 v, u64 n, hipStream_t s) {
 }
 void max_u32(const u32* a, u64 n, u32* out, hipStream_t s) {
   HL_PROF(P_REDUCE, 4 * n, s);
-  if (n) hipLaunchKernelGGL(k_max_u32, blocks_for(n / 4 + 1, 256, 1024), 256, 0, s, a, n, out);
+  if (n) hipLaunchKernelGGL(k_max_u32, blocks_for(n / 4 + 1, 256, kReduceBlocks), 256, 0, s, a, n, out);
   HL_LAUNCH_CHECK();
 }
 

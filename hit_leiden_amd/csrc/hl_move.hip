@@ -46,6 +46,7 @@ constexpr u32 kBigDeg = 1u << 18;   // workgroup-per-vertex with an HBM table up
 constexpr u64 kMarksInBlock = 2048;  // neighbour entries marked inside the decoupling workgroup
 constexpr unsigned kMarksBlocks = 1024;
 constexpr u32 kMarkStage = 2048;
+constexpr u32 kMarkPer = 4;  // marks: row entries per thread and pass of k_marks_flat
 constexpr u64 kBigTableBytes = 3ull << 30;  // HBM hub tables of the overflow proposals
 constexpr u32 kOvfBlocks = 64;              // workgroups (and HBM tables) for the overflow list
 
@@ -1544,6 +1545,9 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
 // device atomic per flush.
 __global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u32* mark, u32 epoch, u32* next,
                                                     u32* next_cnt) {
+  // kMarkPer entries per thread and pass, their gathers issued together so
+  // that more of them are in flight
+  constexpr u32 R = kMarkPer;
   __shared__ u32 sv_[kMarkStage];
   __shared__ u32 scnt, sbase, span[2];
   const u32 c = *flat.c;
@@ -1551,12 +1555,13 @@ __global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u
   if (threadIdx.x == 0) scnt = 0;
   __syncthreads();
   const u64 total = flat.pre[c];
-  for (u64 base = u64(blockIdx.x) * blockDim.x; base < total; base += u64(gridDim.x) * blockDim.x) {
-    const u64 t = base + threadIdx.x;
+  const u64 per = u64(blockDim.x) * R;
+  for (u64 base = u64(blockIdx.x) * per; base < total; base += u64(gridDim.x) * per) {
     // the segments of the pass's first and last entries bound every thread's
     // search, so the per-thread searches take a few cached steps
     if (threadIdx.x == 0 || threadIdx.x == blockDim.x - 1) {
-      const u64 e = threadIdx.x == 0 ? base : (t < total ? t : total - 1);
+      const u64 last = base + per - 1;
+      const u64 e = threadIdx.x == 0 ? base : (last < total ? last : total - 1);
       u32 lo = 0, hi = c;
       while (hi - lo > 1) {
         const u32 mid = (lo + hi) >> 1;
@@ -1566,25 +1571,52 @@ __global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u
       span[threadIdx.x == 0 ? 0 : 1] = lo;
     }
     __syncthreads();
-    if (t < total) {
-      u32 lo = span[0], hi = span[1] + 1;
-      while (hi - lo > 1) {
-        const u32 mid = (lo + hi) >> 1;
-        if (flat.pre[mid] <= t) lo = mid;
-        else hi = mid;
+    u32 uu[R], vv[R], ct[R], fu[R], su[R], sv[R];
+    u64 ee[R];
+    u32 seg[R];
+    bool ok[R];
+#pragma unroll
+    for (u32 r = 0; r < R; ++r) {
+      const u64 t = base + u64(r) * blockDim.x + threadIdx.x;
+      ok[r] = t < total;
+      uu[r] = vv[r] = ct[r] = seg[r] = 0;
+      ee[r] = 0;
+      if (ok[r]) {
+        u32 lo = span[0], hi = span[1] + 1;
+        while (hi - lo > 1) {
+          const u32 mid = (lo + hi) >> 1;
+          if (flat.pre[mid] <= t) lo = mid;
+          else hi = mid;
+        }
+        seg[r] = lo;
+        vv[r] = flat.node[lo];
+        ct[r] = flat.to[lo];
+        ee[r] = m.g.row_off[vv[r]] + (t - flat.pre[lo]);
+        uu[r] = m.g.col[ee[r]];
       }
-      const u32 v = flat.node[lo], cto = flat.to[lo];
-      const u64 e = m.g.row_off[v] + (t - flat.pre[lo]);
-      const u32 u = m.g.col[e];
-      if (m.ht.bits && u != v) ht_move(m.ht, u, flat.from[lo], cto, m.g.w[e]);
-      if (m.f[u] != cto && claim(mark, u, epoch)) sv_[atomicAdd(&scnt, 1u)] = u;
-      if (m.s[u] == m.s[v]) {
+    }
+#pragma unroll
+    for (u32 r = 0; r < R; ++r) {
+      fu[r] = su[r] = sv[r] = 0;
+      if (ok[r]) {
+        fu[r] = m.f[uu[r]];
+        su[r] = m.s[uu[r]];
+        sv[r] = m.s[vv[r]];
+      }
+    }
+#pragma unroll
+    for (u32 r = 0; r < R; ++r) {
+      if (!ok[r]) continue;
+      const u32 u = uu[r], v = vv[r];
+      if (m.ht.bits && flat.from && u != v) ht_move(m.ht, u, flat.from[seg[r]], ct[r], m.g.w[ee[r]]);
+      if (fu[r] != ct[r] && claim(mark, u, epoch)) sv_[atomicAdd(&scnt, 1u)] = u;
+      if (su[r] == sv[r]) {
         m.K[u] = 1;
         m.K[v] = 1;
       }
     }
     __syncthreads();
-    if (scnt > kMarkStage - blockDim.x) {
+    if (scnt > kMarkStage - R * blockDim.x) {
       if (threadIdx.x == 0) sbase = atomicAdd(next_cnt, scnt);
       __syncthreads();
       for (u32 i = threadIdx.x; i < scnt; i += blockDim.x) next[sbase + i] = sv_[i];
