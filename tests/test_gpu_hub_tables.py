@@ -1,4 +1,4 @@
-"""Hub tables (RunConfig.hub_tables, on by default): rows of more than 4,096
+"""Hub tables (RunConfig.hub_tables, on by default): rows of more than 512
 entries are evaluated from per-vertex community tables that the first full
 evaluation of a level's rounds builds and the neighbours' moves keep current
 (in the one-workgroup decoupling, the grid marks and the multi-kernel path).
@@ -45,7 +45,7 @@ def test_tables_hubs(hub_deg, n):
 
 
 def test_tables_rmat():
-    """R-MAT scale 16: rows above 4,096 entries, four levels, deletions."""
+    """R-MAT scale 16: rows above 512 entries, four levels, deletions."""
     w = gen.rmat(16, seed=3)
     w.make_batches(3, 800, 100, mode=1, seed=4)
     P = 4
