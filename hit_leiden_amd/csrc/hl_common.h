@@ -59,6 +59,10 @@ struct InputError : std::runtime_error {
 // destinations, which must stay alive until then.
 void d2h_async(void* dst, const void* src, size_t bytes, hipStream_t s);
 void stream_sync(hipStream_t s);
+// Forget the values staged for stream s without delivering them, after
+// waiting (errors ignored) for their copies: a call that ended with an
+// exception leaves destinations in the frames it unwound.
+void drop_staged(hipStream_t s);
 
 // Grow-only device buffer.  reserve(n, keep) reallocates when the capacity is
 // short; keep copies the old contents (stream-ordered).

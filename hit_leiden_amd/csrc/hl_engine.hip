@@ -627,6 +627,11 @@ struct ProfBind {
   }
 };
 
+// a call that failed: its staged read-backs point into frames it unwound
+void forget_staged(hl_graph* g) {
+  if (g && g->e.s) hl::drop_staged(g->e.s);
+}
+
 template <class F>
 hl_status guard(const hl_graph* gc, F&& fn) {
   hl_graph* g = const_cast<hl_graph*>(gc);
@@ -636,15 +641,19 @@ hl_status guard(const hl_graph* gc, F&& fn) {
     if (g) g->e.err.clear();
     return HL_OK;
   } catch (const hl::InputError& x) {
+    forget_staged(g);
     if (g) g->e.err = std::string("invalid input: ") + x.what();
     return HL_INVALID_INPUT;
   } catch (const hl::HipError& x) {
+    forget_staged(g);
     if (g) g->e.err = std::string("backend: ") + x.what();
     return x.code == hipErrorOutOfMemory ? HL_OOM : HL_BACKEND;
   } catch (const std::bad_alloc&) {
+    forget_staged(g);
     if (g) g->e.err = "host allocation failed";
     return HL_OOM;
   } catch (const std::exception& x) {
+    forget_staged(g);
     if (g) g->e.err = std::string("backend: ") + x.what();
     return HL_BACKEND;
   }

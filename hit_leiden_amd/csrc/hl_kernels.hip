@@ -58,6 +58,19 @@ void stream_sync(hipStream_t s) {
   if (keep == 0) st.used = 0;
 }
 
+void drop_staged(hipStream_t s) {
+  Staging& st = staging();
+  bool any = false;
+  for (const auto& it : st.items) any = any || it.s == s;
+  if (!any) return;
+  (void)hipStreamSynchronize(s);  // no copy may land in the area once it is reused
+  size_t keep = 0;
+  for (size_t i = 0; i < st.items.size(); ++i)
+    if (st.items[i].s != s) st.items[keep++] = st.items[i];
+  st.items.resize(keep);
+  if (keep == 0) st.used = 0;
+}
+
 void d2h_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
   Staging& st = staging();
   size_t off = (st.used + 15) & ~size_t(15);
