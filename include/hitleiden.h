@@ -12,7 +12,11 @@
  * reference takes `&mut PartitionState`, hit_leiden.rs:936-941).  Bad ids or
  * non-finite weights return HL_INVALID_INPUT instead of panicking (the
  * reference panics, hit_leiden.rs:1060; `run` returns InvalidInput,
- * hit_leiden.rs:31-43).
+ * hit_leiden.rs:31-43).  Inputs are checked before anything changes, so
+ * HL_INVALID_INPUT leaves the handle as it was; after HL_BACKEND or HL_OOM
+ * from a call that updates the graph the handle's state is unspecified:
+ * destroy it, or rebuild it from a checkpoint (hl_load_graph, hl_restore_level,
+ * hl_write_partition).
  */
 #ifndef HITLEIDEN_H
 #define HITLEIDEN_H
