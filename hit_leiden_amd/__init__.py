@@ -64,7 +64,7 @@ class _Config(C.Structure):
     _fields_ = [("gamma", C.c_double), ("mode", C.c_int32), ("levels", C.c_int32), ("max_rounds", C.c_int32),
                 ("device", C.c_int32), ("small_degree", C.c_uint32), ("wave_degree", C.c_uint32),
                 ("load_pass_records", C.c_uint32), ("key_cache", C.c_uint32), ("hub_tables", C.c_uint32),
-                ("reserved", C.c_uint32 * 3)]
+                ("decide_rounds", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class BatchStats(C.Structure):
@@ -184,6 +184,7 @@ class RunConfig:
     load_pass_records: int = 0
     key_cache: int = 0  # 1: cache each row entry's neighbour community in the throughput movement
     hub_tables: bool = True  # evaluate hub rows from community tables kept current by the moves
+    decide_rounds: int = 0  # priority rounds of the multi-kernel decoupling before the in-order settle (0: 48)
 
 
 class GraphInput:
@@ -234,6 +235,7 @@ class HitLeiden:
         c.load_pass_records = int(self.config.load_pass_records)
         c.key_cache = int(self.config.key_cache)
         c.hub_tables = 0 if self.config.hub_tables else 1
+        c.decide_rounds = int(self.config.decide_rounds)
         h = C.c_void_p()
         st = lib().hl_create(C.byref(c), C.byref(h))
         if st != HL_OK:

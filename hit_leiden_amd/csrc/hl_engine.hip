@@ -701,6 +701,7 @@ hl_status hl_create(const hl_config* cfg, hl_graph** out) {
     g->e.mode = cfg->mode;
     if (cfg->small_degree) g->e.th.small_deg = cfg->small_degree;
     if (cfg->wave_degree) g->e.th.wave_deg = cfg->wave_degree;
+    if (cfg->decide_rounds) g->e.th.dec_rounds = cfg->decide_rounds;
   });
   if (st != HL_OK) {
     g_create_err = g->e.err;

@@ -129,6 +129,7 @@ struct ProposeScratch {
 struct Thresholds {
   u32 small_deg = 16;
   u32 wave_deg = 512;
+  u32 dec_rounds = 48;  // priority rounds of the multi-kernel decoupling before the in-order settle
 };
 // deterministic sweep (hit_leiden.rs:1108-1188) in one workgroup
 struct DetMoveScratch {

@@ -1780,19 +1780,40 @@ __global__ void k_decide4(u64 c, const u32* from, const u32* to, u32* state, con
   for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
   if ((threadIdx.x & 63) == 0 && local) atomicAdd(undec, local);
 }
-__global__ void k_seq_finish(u64 c, const u32* from, const u32* to, u32* state, u8* ea) {
-  if (gtid() != 0) return;
-  for (u64 p = 0; p < c; ++p) {
-    u32 f = from[p], t = to[p];
-    if (state[p] == kUndec) state[p] = ((ea[f] & 2) || (ea[t] & 1)) ? kRej : kAcc;
-    if (state[p] == kAcc) {
-      ea[f] |= 1;
-      ea[t] |= 2;
+// A chain of conflicts longer than the priority rounds allow: the undecided
+// rest is settled in priority order by one thread.  An accepted candidate
+// never has an undecided conflicting candidate before it (it would have
+// waited), so the flags of every accepted candidate, later ones included,
+// touch an undecided candidate's checks only through earlier ones, and the
+// result is the sequential greedy's (throughput.rs:204-236).  The flag arrays
+// hold kNone at every candidate's communities on entry (k_reset4); a flag is
+// any other value.  undec: 1 for each undecided candidate, for the ordered
+// select.
+__global__ void k_settle_prep(u64 c, const u32* from, const u32* to, const u32* state, u32* emit, u32* recv,
+                              u8* undec) {
+  for (u64 p = gtid(); p < c; p += gstride()) {
+    const u32 st = state[p];
+    undec[p] = st == kUndec;
+    if (st == kAcc) {
+      emit[from[p]] = 0;
+      recv[to[p]] = 0;
     }
   }
-  for (u64 p = 0; p < c; ++p) {
-    ea[from[p]] = 0;
-    ea[to[p]] = 0;
+}
+__global__ void k_settle_seq(const u32* list, const unsigned long long* cnt, const u32* from, const u32* to,
+                             u32* state, u32* emit, u32* recv) {
+  if (gtid() != 0) return;
+  const u64 n = *cnt;
+  for (u64 i = 0; i < n; ++i) {
+    const u32 p = list[i];
+    const u32 f = from[p], t = to[p];
+    if (recv[f] != kNone || emit[t] != kNone) {
+      state[p] = kRej;
+    } else {
+      state[p] = kAcc;
+      emit[f] = 0;
+      recv[t] = 0;
+    }
   }
 }
 __global__ void k_apply4(u64 c, const u32* node, const u32* from, const u32* to, const u32* state, MoveCtx m,
@@ -2132,13 +2153,22 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
         d2h_async(&und, status + 1, sizeof(u32), s);
         stream_sync(s);
         if (und == 0) break;
-        if (++iters >= 48) {
-          DevBuf<u8> ea;
-          ea.reserve(m.comm_cap, s);
-          HL_CHECK(hipMemsetAsync(ea.p, 0, m.comm_cap, s));
-          hipLaunchKernelGGL(k_seq_finish, 1, 64, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p, ea.p);
+        if (++iters >= int(th.dec_rounds)) {
+          // the undecided rest in priority order, against flags of the accepted
+          // moves in minAfrom (sources) and minAto (targets); the sort buffers
+          // are free by now: k32b holds the undecided flags, v32 their list,
+          // k64b its count
+          hipLaunchKernelGGL(k_reset4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.minUto.p,
+                             ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p);
+          u8* undec = reinterpret_cast<u8*>(ms.k32b.p);
+          hipLaunchKernelGGL(k_settle_prep, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p,
+                             ms.minAfrom.p, ms.minAto.p, undec);
           HL_LAUNCH_CHECK();
-          stream_sync(s);
+          select_flagged_async(undec, c, ms.v32.p, tmp, ms.k64b.p, s);
+          hipLaunchKernelGGL(k_settle_seq, 1, 64, 0, s, ms.v32.p,
+                             reinterpret_cast<const unsigned long long*>(ms.k64b.p), ms.from.p, ms.to.p, ms.state.p,
+                             ms.minAfrom.p, ms.minAto.p);
+          HL_LAUNCH_CHECK();
           break;
         }
       }

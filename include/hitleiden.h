@@ -81,7 +81,10 @@ typedef struct {
   uint32_t hub_tables;  /* throughput movement: rows above the wavefront bucket evaluated from
                           per-vertex community tables kept current by the moves; 0 = on, 1 = off
                           (off when key_cache is on) */
-  uint32_t reserved[3];
+  uint32_t decide_rounds; /* throughput movement, more than 4,096 candidates: priority rounds of
+                             the decoupling before the undecided rest is settled in priority
+                             order by one thread (the same result); 0 = 48 */
+  uint32_t reserved[2];
 } hl_config;
 
 #define HL_MAX_LEVELS 8

@@ -72,6 +72,24 @@ def test_config_flags_checked_before_the_device():
         assert not h.value
 
 
+def _header_struct(name):
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct \{([^}]*)\}\s*" + name + ";", text).group(1)
+    return [(t, f, int(k) if k else 1) for t, f, k in re.findall(r"(\w+)\s+(\w+)(?:\[(\w+)\])?;", body)]
+
+
+def test_config_layout_matches_header():
+    """The ctypes mirror of hl_config has the header's fields, in order, with
+    the same types, so both sides agree on every offset."""
+    import hit_leiden_amd as hl
+    ctype = {"double": ctypes.c_double, "int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32}
+    want = _header_struct("hl_config")
+    got = hl._Config._fields_
+    assert [f for _, f, _ in want] == [f for f, _ in got]
+    for (t, f, k), (_, ct) in zip(want, got):
+        assert ctypes.sizeof(ct) == k * ctypes.sizeof(ctype[t]), f
+
+
 def test_graph_input_shapes():
     import hit_leiden_amd as hl
     g = hl.GraphInput.from_edges(4, [(0, 1), (1, 2, 2.5)])
