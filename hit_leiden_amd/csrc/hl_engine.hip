@@ -676,7 +676,7 @@ hl_status hl_create(const hl_config* cfg, hl_graph** out) {
   *out = nullptr;
   if (cfg->levels < 1 || cfg->levels > HL_MAX_LEVELS || !(cfg->gamma > 0.0) || !std::isfinite(cfg->gamma) ||
       (cfg->mode != HL_MODE_DETERMINISTIC && cfg->mode != HL_MODE_THROUGHPUT) || cfg->key_cache > 1 ||
-      cfg->hub_tables > 1) {
+      cfg->hub_tables > 1 || cfg->decide_rounds > (1u << 20)) {
     g_create_err = "invalid config";
     return HL_INVALID_INPUT;
   }

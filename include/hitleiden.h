@@ -83,7 +83,7 @@ typedef struct {
                           (off when key_cache is on) */
   uint32_t decide_rounds; /* throughput movement, more than 4,096 candidates: priority rounds of
                              the decoupling before the undecided rest is settled in priority
-                             order by one thread (the same result); 0 = 48 */
+                             order by one thread (the same result); 0 = 48, at most 2^20 */
   uint32_t reserved[2];
 } hl_config;
 

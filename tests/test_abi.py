@@ -59,14 +59,15 @@ def test_invalid_config_rejected():
 
 
 def test_config_flags_checked_before_the_device():
-    """hl_config's key_cache and hub_tables take 0 or 1; anything else is an
-    invalid config (reported before any device is looked for)."""
+    """hl_config's key_cache and hub_tables take 0 or 1 and decide_rounds at
+    most 2^20; anything else is an invalid config (reported before any device
+    is looked for)."""
     import hit_leiden_amd as hl
     L = hl.lib()
-    for field in ("key_cache", "hub_tables"):
+    for field, bad in (("key_cache", 2), ("hub_tables", 2), ("decide_rounds", (1 << 20) + 1)):
         c = hl._Config()
         c.gamma, c.mode, c.levels = 1.0, 1, 2
-        setattr(c, field, 2)
+        setattr(c, field, bad)
         h = ctypes.c_void_p()
         assert L.hl_create(ctypes.byref(c), ctypes.byref(h)) == hl.HL_INVALID_INPUT
         assert not h.value
