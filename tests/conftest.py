@@ -14,11 +14,16 @@ def pytest_configure(config):
 
 
 def gpu_available():
+    """A HIP device is visible to the runtime the engine links (libamdhip64
+    from /opt/rocm), asked directly: torch bundles a runtime of its own, which
+    may not see the device once the engine's runtime holds it."""
+    import ctypes
     try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    except OSError:
         return False
+    n = ctypes.c_int(0)
+    return hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
 
 
 @pytest.fixture(scope="session")
