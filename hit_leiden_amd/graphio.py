@@ -1,8 +1,8 @@
 """Graph files and update streams for the engine.
 
 * Edge-list text: one edge per line, `src dst [weight]`, whitespace separated,
-  `#` comments (the format README.md:37-38 describes; the reference ships no
-  loader for it).
+  `#` comments (the format the reference's README.md:36-37 describes; the
+  reference ships no loader for it).
 * Binary edge list, the server's bulk format: a 32-byte header
   (magic `HLEB`, version u32 = 1, n u32, flags u32 (bit 0: weighted), m u64,
   reserved u64) followed by src u32[m], dst u32[m] and, when weighted, w f64[m],

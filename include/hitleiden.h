@@ -4,7 +4,7 @@
  * The reference crate (nameearly/hit-leiden, Rust) exposes this path as Rust
  * functions only; each entry point below names the reference interface it
  * replaces.  Plain pointers and sizes, no C++ or torch types: a Rust
- * build.rs-linked `extern "C"` block (see INTEGRATION.md) binds it as is.
+ * build.rs-linked `extern "C"` block binds it as is.
  *
  * Ownership: an hl_graph owns every device allocation of one resident graph and
  * its hierarchy.  Host arrays passed in are borrowed for the duration of the
