@@ -727,6 +727,7 @@ hl_status hl_load_graph(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src
   if (!g || (m && (!src || !dst))) return HL_INVALID_INPUT;
   return guard(g, [&] {
     Engine& e = g->e;
+    if (n == 0) throw hl::InputError("a graph needs at least one vertex");
     HL_CHECK(hipSetDevice(e.cfg.device));
     e.upload_batch(m, src, dst, w, n);
     e.reset(n, m);
@@ -740,6 +741,7 @@ hl_status hl_load(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, cons
   if (!g || (m && (!src || !dst))) return HL_INVALID_INPUT;
   return guard(g, [&] {
     Engine& e = g->e;
+    if (n == 0) throw hl::InputError("a graph needs at least one vertex");
     HL_CHECK(hipSetDevice(e.cfg.device));
     double t0 = now_s();
     if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -762,6 +764,7 @@ hl_status hl_load_graphs(hl_graph* g, uint32_t ngraphs, const uint32_t* vertex_o
     if (vertex_offsets[0] != 0) throw hl::InputError("vertex_offsets[0] must be 0");
     for (uint32_t i = 0; i < ngraphs; ++i)
       if (vertex_offsets[i + 1] < vertex_offsets[i]) throw hl::InputError("vertex_offsets must not decrease");
+    if (vertex_offsets[ngraphs] == 0) throw hl::InputError("the graphs need at least one vertex");
     HL_CHECK(hipSetDevice(e.cfg.device));
     double t0 = now_s();
     if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -917,6 +920,7 @@ hl_status hl_restore_level(hl_graph* g, int32_t level, uint32_t n, uint64_t m, c
     Engine& e = g->e;
     HL_CHECK(hipSetDevice(e.cfg.device));
     if (level < 0 || level >= e.P) throw hl::InputError("level out of range");
+    if (level == 0 && n == 0) throw hl::InputError("a graph needs at least one vertex");
     if (level == 0) {
       e.G = 0;
       e.goff.clear();

@@ -114,7 +114,9 @@ void hl_destroy(hl_graph* g);
 /* Initial load: G^0 <- edges, then Algorithm 6 with dG = all edges on the
    identity state (the reference tests' load pattern, hit_leiden.rs:3119-3126,
    with the empty level-0 CSR pre-pushed).  w == NULL means unit weights
-   (Option<f64>::unwrap_or(1.0), in_memory.rs:49).  stats may be NULL. */
+   (Option<f64>::unwrap_or(1.0), in_memory.rs:49).  stats may be NULL.
+   n >= 1: an empty vertex set is HL_INVALID_INPUT (here, in hl_load_graph,
+   hl_load_graphs and hl_restore_level). */
 hl_status hl_load(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst, const double* w,
                   hl_batch_stats* stats);
 

@@ -1,0 +1,49 @@
+"""The validation layer (hit_leiden_amd.validation) against the rules of
+src/core/validation/equivalence.rs:21-44 and invariants.rs:19-28, at their
+boundaries: equal and unequal partitions in deterministic mode, quality
+deltas on either side of 0.001 in throughput mode, ids at the node count."""
+import numpy as np
+import pytest
+
+import hit_leiden_amd as hl
+from hit_leiden_amd.validation import PartitionResult, check, validate
+
+
+def part(labels, q):
+    return PartitionResult(node_to_community=np.asarray(labels, np.int64), quality_score=q)
+
+
+def test_deterministic_identity():
+    a = part([0, 0, 1], 0.25)
+    v = validate(a, part([0, 0, 1], 0.25), hl.RunMode.DETERMINISTIC)
+    assert v.hard_invariants_passed and v.deterministic_identity_passed and v.equivalence_passed
+    assert v.quality_delta_vs_reference == 0.0
+    v = validate(a, part([0, 1, 1], 0.25), hl.RunMode.DETERMINISTIC)
+    assert v.deterministic_identity_passed is False and v.equivalence_passed is False
+    # equal quality does not make different partitions equivalent, nor does a length change
+    assert not validate(a, part([0, 0], 0.25), hl.RunMode.DETERMINISTIC).equivalence_passed
+
+
+def test_throughput_quality_bound():
+    a = part([0, 0, 1], 0.5)
+    v = validate(a, part([1, 1, 0], 0.5 - 0.00075), hl.RunMode.THROUGHPUT)
+    assert v.deterministic_identity_passed is None and v.equivalence_passed
+    assert v.quality_delta_vs_reference == pytest.approx(0.00075)
+    assert not validate(a, part([0, 0, 1], 0.5 + 0.0011), hl.RunMode.THROUGHPUT).equivalence_passed
+    # different partitions of equal quality pass in throughput mode
+    assert validate(a, part([0, 1, 2], 0.5), hl.RunMode.THROUGHPUT).equivalence_passed
+
+
+def test_self_validation_passes():
+    """tests/contract/test_run_validate.rs:20-30: a result validated against
+    itself is equivalent, in both modes."""
+    a = part([0, 1, 1, 2], 0.1)
+    for mode in hl.RunMode:
+        assert validate(a, a, mode).equivalence_passed
+
+
+def test_check_ids_below_node_count():
+    assert check(part([0, 1, 2], 0.0))
+    assert check(part([], 0.0))
+    assert not check(part([0, 3, 1], 0.0))
+    assert not check(None)
