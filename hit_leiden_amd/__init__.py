@@ -550,7 +550,10 @@ def compute_modularity(graph: GraphInput, labels, device=0) -> float:
         h.close()
 
 
+from .validation import PartitionResult, RunOutcome, ValidationOutcome, check, run, validate  # noqa: E402
+
 __all__ = ["ServeStats", "RunMode", "RunConfig", "GraphInput", "HitLeiden", "BatchStats", "compute_modularity",
+           "PartitionResult", "RunOutcome", "ValidationOutcome", "check", "run", "validate",
            "HitLeidenError", "InvalidInput", "BackendError", "AccelerationError", "OutOfMemory", "lib",
            "library_path", "MAP_F", "MAP_G", "MAP_S_CUR", "MAP_S_PRE", "SET_ACTIVE", "SET_CHANGED",
            "SET_AFFECTED", "SET_REFINED", "PART_COMMUNITY", "PART_REFINED", "PART_F0"]

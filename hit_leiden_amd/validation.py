@@ -13,6 +13,8 @@
   reported as passed, as the reference reports them.
 * ``check`` (src/core/validation/invariants.rs:19-28): a partition exists and
   every community id is below its node count.
+* ``run`` (hit_leiden::run, src/lib.rs:27-29): a static clustering on the
+  engine returned as a ``RunOutcome``.
 
 The benchmark's contingency metrics (NMI, purity, largest-community Jaccard,
 hit_leiden_incremental.rs:577-666) run on the device:
@@ -39,6 +41,30 @@ class PartitionResult:
 
 
 @dataclass
+class RunOutcome:
+    """src/core/types.rs:127-131, with the partition only."""
+    partition: Optional[PartitionResult]
+
+
+def run(graph, config=None) -> RunOutcome:
+    """hit_leiden::run (src/lib.rs:27-29) on the engine: a static clustering
+    by HitLeiden.run_static (load, then empty-delta passes to a fixpoint; not
+    the reference's sequential shuffled multilevel_leiden, hit_leiden.rs:30-118)
+    returned as a RunOutcome."""
+    from . import HitLeiden, RunConfig
+    e = HitLeiden(config or RunConfig())
+    try:
+        passes, _ = e.run_static(graph)
+        return RunOutcome(partition=partition_result(e, iteration_count=passes, run_id=""))
+    finally:
+        e.close()
+
+
+def _partition(x):
+    return x.partition if isinstance(x, RunOutcome) else x
+
+
+@dataclass
 class ValidationOutcome:
     """src/core/report.rs:28-33"""
     hard_invariants_passed: bool
@@ -57,8 +83,10 @@ def partition_result(engine, iteration_count=0, run_id="") -> PartitionResult:
                            run_id=run_id)
 
 
-def validate(reference: PartitionResult, candidate: PartitionResult, mode: RunMode) -> ValidationOutcome:
-    """equivalence.rs:21-44.  Both results must exist (the reference unwraps)."""
+def validate(reference, candidate, mode: RunMode) -> ValidationOutcome:
+    """equivalence.rs:21-44 over two RunOutcomes or PartitionResults.  Both
+    partitions must exist (the reference unwraps them)."""
+    reference, candidate = _partition(reference), _partition(candidate)
     if reference is None or candidate is None:
         raise ValueError("validate needs two partitions")
     a = np.asarray(reference.node_to_community)
@@ -70,8 +98,9 @@ def validate(reference: PartitionResult, candidate: PartitionResult, mode: RunMo
     return ValidationOutcome(True, None, delta, delta <= QUALITY_TOLERANCE)
 
 
-def check(partition: Optional[PartitionResult]) -> bool:
+def check(run_or_partition) -> bool:
     """invariants.rs:19-28: ids below the node count (an absent partition fails)."""
+    partition = _partition(run_or_partition)
     if partition is None:
         return False
     labels = np.asarray(partition.node_to_community)

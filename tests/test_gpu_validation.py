@@ -26,10 +26,10 @@ def static_result(n, edges, mode):
 def test_run_and_validate_contract():
     """RunConfig::default() is the deterministic mode (config.rs:39-50)."""
     say("validate: path of three")
-    r = static_result(3, [(0, 1), (1, 2)], DET)
-    assert check(r)
-    assert len(r.node_to_community) == 3
-    assert validate(r, r, DET).equivalence_passed
+    out = hl.run(hl.GraphInput.from_edges(3, [(0, 1), (1, 2)]), hl.RunConfig(mode=DET, levels=1))
+    assert check(out)
+    assert len(out.partition.node_to_community) == 3
+    assert validate(out, out, DET).equivalence_passed
 
 
 def test_throughput_equivalence_bounds():

@@ -42,6 +42,17 @@ def test_self_validation_passes():
         assert validate(a, a, mode).equivalence_passed
 
 
+def test_run_outcomes_accepted():
+    """validate and check take RunOutcomes as the reference's do; a missing
+    partition fails the check and cannot be validated."""
+    a = hl.RunOutcome(partition=part([0, 0, 1], 0.3))
+    assert validate(a, a, hl.RunMode.DETERMINISTIC).equivalence_passed
+    assert check(a)
+    assert not check(hl.RunOutcome(partition=None))
+    with pytest.raises(ValueError):
+        validate(a, hl.RunOutcome(partition=None), hl.RunMode.THROUGHPUT)
+
+
 def test_check_ids_below_node_count():
     assert check(part([0, 1, 2], 0.0))
     assert check(part([], 0.0))
