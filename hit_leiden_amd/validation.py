@@ -50,7 +50,9 @@ def run(graph, config=None) -> RunOutcome:
     """hit_leiden::run (src/lib.rs:27-29) on the engine: a static clustering
     by HitLeiden.run_static (load, then empty-delta passes to a fixpoint; not
     the reference's sequential shuffled multilevel_leiden, hit_leiden.rs:30-118)
-    returned as a RunOutcome."""
+    returned as a RunOutcome.  config defaults to RunConfig(), the throughput
+    mode; the reference's RunConfig::default() is deterministic
+    (config.rs:39-50), so pass RunConfig(mode=RunMode.DETERMINISTIC) for it."""
     from . import HitLeiden, RunConfig
     e = HitLeiden(config or RunConfig())
     try:
