@@ -75,20 +75,32 @@ def test_config_flags_checked_before_the_device():
 
 def _header_struct(name):
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    macros = dict(re.findall(r"#define\s+(\w+)\s+(\d+)", text))
     body = re.search(r"typedef struct \{([^}]*)\}\s*" + name + ";", text).group(1)
-    return [(t, f, int(k) if k else 1) for t, f, k in re.findall(r"(\w+)\s+(\w+)(?:\[(\w+)\])?;", body)]
+    fields = re.findall(r"([\w]+\s*\*?)\s*(\w+)(?:\[(\w+)\])?;", body)
+    return [(t.replace(" ", ""), f, int(macros.get(k, k)) if k else 1) for t, f, k in fields]
 
 
-def test_config_layout_matches_header():
-    """The ctypes mirror of hl_config has the header's fields, in order, with
-    the same types, so both sides agree on every offset."""
+@pytest.mark.parametrize("struct,mirror", [("hl_config", "_Config"), ("hl_batch_stats", "BatchStats"),
+                                           ("hl_request", "_Request"), ("hl_serve_stats", "ServeStats")])
+def test_struct_layout_matches_header(struct, mirror):
+    """Each ctypes mirror has its header struct's fields, in order, with the
+    same sizes and the same total size, so both sides agree on every offset."""
     import hit_leiden_amd as hl
-    ctype = {"double": ctypes.c_double, "int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32}
-    want = _header_struct("hl_config")
-    got = hl._Config._fields_
+    base = {"double": 8, "int32_t": 4, "uint32_t": 4, "uint64_t": 8}
+    size = {t: 8 if t.endswith("*") else base[t] for t, _, _ in _header_struct(struct)}
+    want = _header_struct(struct)
+    got = getattr(hl, mirror)._fields_
     assert [f for _, f, _ in want] == [f for f, _ in got]
     for (t, f, k), (_, ct) in zip(want, got):
-        assert ctypes.sizeof(ct) == k * ctypes.sizeof(ctype[t]), f
+        assert ctypes.sizeof(ct) == k * size[t], f
+    # C layout of the header's fields (natural alignment) against the mirror's
+    off = 0
+    for t, _, k in want:
+        a = size[t]
+        off = (off + a - 1) // a * a + k * size[t]
+    off = (off + 7) // 8 * 8
+    assert ctypes.sizeof(getattr(hl, mirror)) == off
 
 
 def test_graph_input_shapes():
