@@ -58,3 +58,31 @@ def test_check_ids_below_node_count():
     assert check(part([], 0.0))
     assert not check(part([0, 3, 1], 0.0))
     assert not check(None)
+
+
+class _Engine:
+    """Stands in for HitLeiden: a two-level hierarchy and its modularity."""
+
+    def __init__(self, rows, q):
+        self.rows, self.q, self.asked = rows, q, []
+
+    def hierarchy(self, canonical=False):
+        self.asked.append(canonical)
+        return np.asarray(self.rows, np.uint32)
+
+    def modularity(self, which):
+        assert which == hl.PART_COMMUNITY
+        return self.q
+
+
+def test_partition_result_takes_the_communities():
+    """node_to_community is the last hierarchy row (the communities pulled down
+    to the vertices), canonical, with its modularity and community count."""
+    from hit_leiden_amd.validation import partition_result
+    e = _Engine([[0, 1, 1, 2, 3], [0, 0, 0, 1, 1]], 0.42)
+    r = partition_result(e, iteration_count=3, run_id="x")
+    assert e.asked == [True]
+    assert r.node_to_community.tolist() == [0, 0, 0, 1, 1]
+    assert r.community_count == 2 and r.quality_score == 0.42 and r.iteration_count == 3
+    assert [row.tolist() for row in r.hierarchy_levels] == [[0, 1, 1, 2, 3], [0, 0, 0, 1, 1]]
+    assert check(r)
