@@ -3,14 +3,16 @@
 A step is one edge-delta batch applied by hl_apply_batch_device to the
 device-resident graph and hierarchy (apply dG, local moving, refinement,
 aggregation per level, def-update), with the batch already in HBM when the
-timed region starts.  Default workload: the planted-partition configuration
-(n = 1,000,000, 1,000 log-normal blocks, mean degree 32, mu = 0.3, weights
-U[0.5,1.5]; batches of 9,000 new-edge inserts + 1,000 deletes), 4 levels,
-throughput mode.
+timed region starts.  Default workload: R-MAT scale 24 (16,777,216 vertices,
+edge factor 16, a,b,c,d = 0.57,0.19,0.19,0.05, symmetrised, deduplicated,
+unit weights), 4 levels, throughput mode, batches of 95,000 R-MAT inserts +
+5,000 deletes (one batch per step, warmup + steps batches generated).
 
 Prints one JSON line: edges/s over the timed steps, p50/p99 batch latency,
-the roofline of the dominant kernel (HIP events on the engine stream) and the
-CPU oracle's rate on a bounded sample of the same workload on this host.
+the roofline of the dominant kernel (HIP events on the engine stream), the
+CPU oracle's rate on a bounded sample of the same workload on this host, and
+(in "extras") the batched and served forms against their plain
+one-batch-per-call paths, each measured by this script in a child process.
 """
 from __future__ import annotations
 
@@ -38,7 +40,7 @@ SLOT_KERNEL = {
     "try_lds": "k_prop2_hub<hl::(Move|Refine)PolicyT, 256u?, 2048u?>", "propose_lds": "k_prop2_lds",
     "try_hub": "k_prop2_hub<hl::(Move|Refine)PolicyT, 512u?, 4096u?>",
     "try_giant": "k_prop2_hub<hl::(Move|Refine)PolicyT, 1024u?, 8192u?>", "propose_big": "k_prop2_big",
-    "decouple": "k_dec_small", "apply_moves": "k_apply4",
+    "decouple": "k_dec_small", "apply_moves": "k_apply4", "mark_moved": "k_marks_flat",
     "row_merge": "k_row_merge", "resolve_runs": "k_resolve_runs",
 }
 
@@ -268,12 +270,57 @@ def bench_served(args):
     print(json.dumps(out), flush=True)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def batch_summary(stats):
+    """Per-batch detail of the timed steps: movement rounds per level, priority
+    rounds of the multi-kernel decoupling and the candidates its in-order
+    settle took, so that a slow batch can be attributed."""
+    return [{"seconds": round(s["seconds"], 5), "rounds": s["rounds"], "decide_rounds": s["decide_rounds"],
+             "settled": s["settled"], "evaluated": s["evaluated"], "moved": s["moved"],
+             "level_n": s["level_n"]} for s in stats]
+
+
+def run_extra(args, config, extra):
+    """The batched or served form, measured by this script in a child process
+    (a fresh device context: the large engine's memory is gone), as a summary
+    of its JSON line with the speed-up over its plain path."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--config", config, "--no-cpu-baseline", "--no-extras",
+           "--levels", str(args.levels), "--gamma", str(args.gamma)] + extra
+    log(f"[bench] extra: {' '.join(cmd[2:])}")
+    t0 = time.time()
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=sys.stderr, timeout=args.extra_timeout, text=True)
+    except subprocess.TimeoutExpired:
+        return {"error": f"timed out after {args.extra_timeout}s"}
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit {r.returncode}"}
+    j = json.loads(lines[-1])
+    keep = ("value", "unit", "ms_per_step", "p50_ms", "p99_ms", "speedup_vs_plain", "plain_edges_per_s",
+            "plain_sample_graphs", "plain", "runs", "steps", "warmup")
+    out = {k: j[k] for k in keep if k in j}
+    out["workload"] = j["config"]["workload"]
+    out["command"] = " ".join(["python", "bench.py"] + cmd[2:])
+    out["wall_seconds"] = round(time.time() - t0, 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="mid", choices=["small", "mid", "large", "batched", "served"])
+    ap.add_argument("--config", default="large", choices=["small", "mid", "large", "batched", "served"])
     ap.add_argument("--rates", default="100,500,2000", help="offered request rates of the served configuration")
     ap.add_argument("--served-seconds", type=float, default=3.0, help="arrival span per offered rate")
     ap.add_argument("--pass-records", type=int, default=1 << 20, help="record budget of a served pass")
@@ -288,6 +335,9 @@ def main():
     ap.add_argument("--gamma", type=float, default=1.0)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle batch time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched and served forms that the plain configurations report in 'extras'")
+    ap.add_argument("--extra-timeout", type=float, default=400.0, help="seconds per extra child process")
     ap.add_argument("--traffic", default=None,
                     help="JSON {slot: HBM bytes per launch} from the FETCH_SIZE / WRITE_SIZE counter runs "
                          "(default: profiles/traffic_<config>.json when present)")
@@ -365,7 +415,8 @@ def main():
         k, du, dv, da = dev[i]
         g.apply_batch_device(k, du.address, dv.address, da.address)
     free, total = hip.mem_info()
-    log(f"[bench] after warmup: device memory in use {(total - free) / 2**30:.1f} GiB")
+    mem_gib = (total - free) / 2**30
+    log(f"[bench] after warmup: device memory in use {mem_gib:.1f} GiB")
     prof_warm = g.profile_read()
     single = {k: v for k, v in prof_warm.items() if k in SLOT_KERNEL}
     dominant = max(single.items(), key=lambda kv: kv[1][1])[0] if single else None
@@ -423,7 +474,10 @@ def main():
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "path_gbs": round(bytes_alg / total / 1e9, 3), "path_hbm_frac": round(bytes_alg / total / 1e9 / HBM_PEAK_GBS, 6),
         "modularity_after_load": round(q0, 6), "modularity_after_steps": round(q1, 6), "load_seconds": round(load_s, 3),
+        "device_memory_gib": round(mem_gib, 1),
         "roofline": roof,
+        "warmup_profile_ms": {k: round(v[1], 3) for k, v in sorted(prof_warm.items(), key=lambda kv: -kv[1][1])[:10]},
+        "batches": batch_summary(stats),
     }
     if args.dump_profile:
         json.dump({"warmup": prof_warm, "timed": prof, "stats": stats}, open(args.dump_profile, "w"), indent=1)
@@ -444,10 +498,20 @@ def main():
             cb = cpu_baseline(ws, args.levels, args.gamma, args.cpu_budget, threads)
             cb["sample"] = "R-MAT scale 20 (same a,b,c, edge factor, batch shape) instead of scale 24: " + cb["sample"]
             out["cpu_baseline"] = cb
+            del ws
         else:
             out["cpu_baseline"] = cpu_baseline(w, args.levels, args.gamma, args.cpu_budget, threads)
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
     else:
         out["cpu_baseline"] = None
+
+    if not args.no_extras and args.config in ("large", "mid"):
+        # the batched and served forms against their plain one-batch-per-call GPU paths
+        out["extras"] = {
+            "batched": run_extra(args, "batched", ["--steps", "2", "--warmup", "1", "--plain-sample", "32"]),
+            "served": run_extra(args, "served", ["--rates", "2000", "--served-seconds", "2",
+                                                 "--plain-requests", "300"]),
+        }
     print(json.dumps(out), flush=True)
 
 

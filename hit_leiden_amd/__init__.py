@@ -64,7 +64,7 @@ class _Config(C.Structure):
     _fields_ = [("gamma", C.c_double), ("mode", C.c_int32), ("levels", C.c_int32), ("max_rounds", C.c_int32),
                 ("device", C.c_int32), ("small_degree", C.c_uint32), ("wave_degree", C.c_uint32),
                 ("load_pass_records", C.c_uint32), ("key_cache", C.c_uint32), ("hub_tables", C.c_uint32),
-                ("decide_rounds", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("decide_rounds", C.c_uint32), ("refine_reference", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
 
 class BatchStats(C.Structure):
@@ -73,7 +73,8 @@ class BatchStats(C.Structure):
                 ("delta_len", C.c_uint64 * MAX_LEVELS), ("active", C.c_uint64 * MAX_LEVELS),
                 ("evaluated", C.c_uint64 * MAX_LEVELS), ("moved", C.c_uint64 * MAX_LEVELS),
                 ("affected", C.c_uint64 * MAX_LEVELS), ("refined", C.c_uint64 * MAX_LEVELS),
-                ("rounds", C.c_uint64 * MAX_LEVELS), ("bytes_alg", C.c_uint64), ("seconds", C.c_double)]
+                ("rounds", C.c_uint64 * MAX_LEVELS), ("bytes_alg", C.c_uint64), ("seconds", C.c_double),
+                ("decide_rounds", C.c_uint64 * MAX_LEVELS), ("settled", C.c_uint64)]
 
     def as_dict(self):
         P = self.levels
@@ -83,6 +84,7 @@ class BatchStats(C.Structure):
             "evaluated": list(self.evaluated[:P]), "moved": list(self.moved[:P]),
             "affected": list(self.affected[:P]), "refined": list(self.refined[:P]),
             "rounds": list(self.rounds[:P]), "bytes_alg": self.bytes_alg, "seconds": self.seconds,
+            "decide_rounds": list(self.decide_rounds[:P]), "settled": self.settled,
         }
 
 
@@ -185,6 +187,10 @@ class RunConfig:
     key_cache: int = 0  # 1: cache each row entry's neighbour community in the throughput movement
     hub_tables: bool = True  # evaluate hub rows from community tables kept current by the moves
     decide_rounds: int = 0  # priority rounds of the multi-kernel decoupling before the in-order settle (0: 48)
+    # throughput mode: False keeps the hierarchy consistent (refined communities inside their
+    # communities, f^p = f^{p+1} o s^p); True applies the reference's own refinement and
+    # def_update rules (throughput.rs:278-287, hit_leiden.rs:1233-1240,1733-1793)
+    refine_reference: bool = False
 
 
 class GraphInput:
@@ -236,6 +242,7 @@ class HitLeiden:
         c.key_cache = int(self.config.key_cache)
         c.hub_tables = 0 if self.config.hub_tables else 1
         c.decide_rounds = int(self.config.decide_rounds)
+        c.refine_reference = 1 if self.config.refine_reference else 0
         h = C.c_void_p()
         st = lib().hl_create(C.byref(c), C.byref(h))
         if st != HL_OK:
@@ -275,18 +282,30 @@ class HitLeiden:
         passes (each re-evaluates every vertex, hit_leiden.rs:1054-1057) until the
         hierarchy stops changing, the community modularity gains less than tol, or
         max_passes.  Synchronous throughput rounds can swap vertices back and
-        forth, so that mode usually stops on tol.  This is not the reference's
-        multilevel_leiden (hit_leiden.rs:30-118), whose sequential shuffled order
-        it does not follow; it returns (passes, modularity of the communities)."""
+        forth: a pass that lowers the modularity is undone (the engine returns to
+        the state before it, kept as an in-memory checkpoint) and ends the run.
+        The reference's own static order, multilevel_leiden
+        (hit_leiden.rs:167-323), is `static_leiden`; this returns (passes,
+        modularity of the communities)."""
+        import io
+        from . import checkpoint
         self.load(graph)
         empty = GraphInput(graph.node_count, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
         prev = self.hierarchy()
         q = self.modularity(PART_COMMUNITY)
         passes = 0
         for passes in range(1, max_passes + 1):
+            saved = io.BytesIO()
+            checkpoint.save(self, saved)
             self.hit_leiden(empty)
             cur = self.hierarchy()
             q_new = self.modularity(PART_COMMUNITY)
+            if q_new < q:
+                saved.seek(0)
+                back = checkpoint.restore(saved, device=self.config.device)
+                self._h, back._h = back._h, self._h
+                back.close()
+                break
             if all(np.array_equal(x, y) for x, y in zip(prev, cur)) or q_new - q < tol:
                 q = q_new
                 break
@@ -339,9 +358,16 @@ class HitLeiden:
         return s
 
     def graphs_delta(self, deltas):
-        """Union-id arrays of {graph index: GraphInput batch}."""
+        """Union-id arrays of {graph index: GraphInput batch}; every id is graph-local
+        (below that graph's node_count), or InvalidInput."""
         off = self.graph_offsets.astype(np.uint64)
         items = sorted(deltas.items())
+        for i, d in items:
+            if not 0 <= i < len(off) - 1:
+                raise InvalidInput(f"graph index {i} out of range")
+            size = int(off[i + 1] - off[i])
+            if len(d) and (int(d.src.max()) >= size or int(d.dst.max()) >= size):
+                raise InvalidInput(f"a record of graph {i} names a vertex outside its {size} vertices")
         src = np.concatenate([d.src.astype(np.uint64) + off[i] for i, d in items] or [np.zeros(0)])
         dst = np.concatenate([d.dst.astype(np.uint64) + off[i] for i, d in items] or [np.zeros(0)])
         a = np.concatenate([d.weights if d.weights is not None else np.ones(len(d), np.float64)

@@ -36,6 +36,19 @@ struct Level {
   DevBuf<double> tot;
   bool tot_valid = false;
   u64 lab_bound = 0;  // every f label < lab_bound
+  // Throughput empty-community ids ebase + v.  ext bounds every f label that
+  // did not come from this level's own range [ebase, ebase + ecap) (labels
+  // pulled down from the level above, fresh ids, labels in use when the range
+  // was placed); while ext <= ebase and n <= ecap the range belongs to the
+  // level's vertices alone: v's empty community is ebase + v, and a vertex
+  // added by growth starts there.  A level that never grows and takes no
+  // labels from above keeps ebase = n, the reference's n + v
+  // (parallel_frontier.rs:214).
+  u64 ext = 0;
+  u64 ebase = 0;
+  u64 ecap = 0;
+  bool ebase_set = false;
+  u64 label_space() const { return std::max<u64>(ext, ebase_set ? ebase + ecap : 0); }
   u64 s_bound = 0;    // every s_cur / s_pre label < s_bound
   u32 deg_bound = 0;  // every row length <= deg_bound
   // several graphs per engine: graph of each vertex id (the graph count for
@@ -149,13 +162,18 @@ struct Engine {
       bs = std::max<u64>(bs, u64(std::max(h[2], h[3])) + 1);
     }
     if (p + 1 < P) bs = std::max<u64>(bs, L[p + 1]->n);
+    // throughput: within the empty range's capacity a new vertex starts in
+    // its own empty community
+    const bool own = par() && lv.ebase_set && n_new <= lv.ecap && lv.ext <= lv.ebase;
+    if (own) bf = lv.ebase + n_old;
     if (bf + (n_new - n_old) > 0xFFFFFFF0ull || bs + (n_new - n_old) > 0xFFFFFFF0ull)
       throw InputError("id space exhausted");
     reserve_level(lv, n_new);
     grow_init(n_old, n_new, u32(bf), u32(bg), u32(bs), lv.f.p, lv.g.p, lv.s_pre.p, lv.s_cur.p, lv.row_off.p,
               lv.row_len.p, lv.row_cap.p, lv.k.p, s);
     lv.n = n_new;
-    lv.lab_bound = std::max<u64>(lv.lab_bound, bf + (n_new - n_old));
+    if (!own) lv.ext = std::max<u64>(lv.ext, bf + (n_new - n_old));
+    lv.lab_bound = std::max<u64>(lv.lab_bound, std::max<u64>(lv.label_space(), bf + (n_new - n_old)));
     lv.s_bound = std::max<u64>(lv.s_bound, bs + (n_new - n_old));
     if (multi()) ensure_gid(lv, n_new);
     if (lv.sub_valid) {
@@ -191,6 +209,8 @@ struct Engine {
   }
 
   bool par() const { return mode == HL_MODE_THROUGHPUT; }
+  // throughput mode with a consistent hierarchy (hl_config.refine_reference == 0)
+  bool connected() const { return par() && cfg.refine_reference == 0; }
 
   // Release scratch sized by the largest call so far (the initial load is a
   // whole-graph batch); it grows back to batch size on demand.
@@ -290,6 +310,22 @@ struct Engine {
     return std::max<u64>(std::max<u64>(2ull * lv.n, mf), 1);
   }
 
+  // Place the level's empty-community range before its throughput rounds.  A
+  // new place is needed when labels from outside the range may lie in it
+  // (ext > ebase) or the level outgrew it; it goes above every label in use
+  // (read from the device) and above ext, with room for the level to double.
+  void set_empty_base(Level& lv) {
+    if (!lv.ebase_set || lv.ext > lv.ebase || lv.n > lv.ecap) {
+      const u64 mx = lv.n ? u64(dmax(lv.f.p, lv.n)) + 1 : 0;
+      lv.ext = std::max<u64>(lv.ext, mx);
+      lv.ecap = lv.ebase_set ? 2ull * lv.n : lv.n;
+      lv.ebase = std::max<u64>(lv.n, lv.ext);
+      lv.ebase_set = true;
+    }
+    if (lv.ebase + lv.ecap > 0xFFFFFFF0ull) throw InputError("id space exhausted");
+    lv.lab_bound = std::max<u64>(lv.lab_bound, lv.label_space());
+  }
+
   // upper bound of the row lengths (rows only shrink below it)
   u32 max_degree(Level& lv) { return lv.deg_bound; }
 
@@ -334,6 +370,7 @@ struct Engine {
       return 1;
     }
     // throughput: synchronous rounds, at most max_rounds (hit_leiden.rs:1077-1099)
+    set_empty_base(lv);
     ensure_totals(lv);
     const int max_rounds = cfg.max_rounds > 0 ? cfg.max_rounds : 20;
     MoveCtx m{g, lv.f.p, lv.s_cur.p, lv.tot.p, lv.k.p, lv.A.p, lv.B.p, lv.K.p, cfg.gamma, lv.tot.cap};
@@ -351,13 +388,14 @@ struct Engine {
       m.part = part.p;
       m.gcnt = gcnt.p;
     }
-    lv.lab_bound = std::max<u64>(lv.lab_bound, 2ull * n);  // empty-community ids n + v
+    m.ebase = u32(lv.ebase);
     MoveStats ms_st;
     const u32 maxd = max_degree(lv);
     u32 rounds = movement_throughput(m, d, k, maxd, max_rounds, mvs, th, tmp, &ms_st, s);
     if (st && p < HL_MAX_LEVELS) {
       st->rounds[p] = rounds;
       st->evaluated[p] += ms_st.evaluated;
+      st->decide_rounds[p] += ms_st.decide_rounds;
     }
     return rounds;
   }
@@ -370,7 +408,7 @@ struct Engine {
     bool identity = false;
     if (par()) ensure_sub(lv);
     const u64 sn = cc_split(g, lv.s_cur.p, lv.K.p, lv.R.p, std::max<u64>(lv.s_bound, 1), &identity, sub_of(lv), rs,
-                            tmp, s);
+                            tmp, s, connected() ? lv.f.p : nullptr);
     lv.s_bound = std::max(lv.s_bound, sn);
     if (multi())
       identity_graphs_to_R(lv.s_cur.p, lv.gid.p, n, G, part.p, gnid.p, lv.R.p, s);  // per graph
@@ -382,7 +420,7 @@ struct Engine {
                  max_degree(lv), tmp, s);
     else
       refine_parallel(g, lv.f.p, lv.s_cur.p, lv.R.p, cfg.gamma, cs, sn, rs, ps, mvs, max_degree(lv), th, tmp,
-                      lv.tot_valid ? lv.tot.p : nullptr, sub_of(lv), s);
+                      lv.tot_valid ? lv.tot.p : nullptr, sub_of(lv), s, connected());
     if (multi() && p + 1 < P) {
       // graphs of the next level's vertices: the graph of any member
       Level& up = *L[p + 1];
@@ -393,9 +431,10 @@ struct Engine {
   }
 
   void def_update_maps(bool use_g) {
-    // copies of the per-level sets (the reference mutates its bitvecs)
+    // copies of the per-level sets (the reference mutates its bitvecs); a
+    // consistent hierarchy pulls every vertex and needs none
     while (int(Bp.size()) < P) Bp.emplace_back(new DevBuf<u8>());
-    for (int p = 0; p < P; ++p) {
+    for (int p = 0; p < P && !connected(); ++p) {
       Level& lv = *L[p];
       Bp[p]->reserve(lv.n + 1, s);
       copy_u8(Bp[p]->p, use_g ? lv.R.p : lv.B.p, lv.n, s);
@@ -407,14 +446,18 @@ struct Engine {
         double* tot = nullptr;
         if (!use_g) {
           lv.lab_bound = std::max(lv.lab_bound, up.lab_bound);  // f^p takes f^{p+1} labels
+          lv.ext = std::max(lv.ext, up.lab_bound);
           if (par() && lv.tot_valid) {
             ensure_totals(lv);
             tot = lv.tot.p;
           }
         }
-        def_pull(Bp[p]->p, use_g ? lv.g.p : lv.f.p, use_g ? up.g.p : up.f.p, lv.s_cur.p, lv.n, tot, lv.k.p, s);
+        // a consistent hierarchy pulls every vertex (f^p = f^{p+1} o s^p), the
+        // reference the changed ones (hit_leiden.rs:1733-1793)
+        def_pull(connected() ? nullptr : Bp[p]->p, use_g ? lv.g.p : lv.f.p, use_g ? up.g.p : up.f.p, lv.s_cur.p,
+                 lv.n, tot, lv.k.p, s);
       }
-      if (p > 0) {
+      if (p > 0 && !connected()) {
         Level& lo = *L[p - 1];
         def_push(Bp[p]->p, Bp[p - 1]->p, lo.s_cur.p, lo.n, lv.n, s);
       }
@@ -447,7 +490,10 @@ struct Engine {
     const u64 dec = 20 * d[DS_DEC_CAND] + 28 * d[DS_DEC_ACC] + 12 * d[DS_DEC_DEG];
     const u64 multi = 20 * d[DS_MULTI_CAND], apply = 28 * d[DS_MULTI_ACC], marks = 12 * d[DS_MULTI_DEG];
     total += dec + multi + apply + marks;
-    if (st) st->bytes_alg += total;
+    if (st) {
+      st->bytes_alg += total;
+      st->settled += d[DS_SETTLED];
+    }
     if (prof.on) {
       const int slots[14] = {P_PROP_SMALL, P_TRY_HALF, P_TRY_WAVE, P_PROP_WAVE, P_TRY_LDS, P_PROP_LDS, P_TRY_HUB,
                              P_TRY_GIANT, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI, P_APPLY_MOVES, P_MARK_MOVED,
@@ -661,6 +707,21 @@ hl_status guard(const hl_graph* gc, F&& fn) {
 
 constexpr uint64_t kLoadPassRecords = 1ull << 26;  // edges per pass of hl_load_graphs
 
+// Labels a caller hands in (hl_write_partition, hl_restore_level) stay below
+// 2^31: 0xFFFFFFFF is the empty key of every hash table and the no-candidate
+// sentinel of the kernels, and a throughput level needs room above its labels
+// for the empty-community ids.
+constexpr uint32_t kMaxLabel = 0x7FFFFFFFu;
+void check_labels(const uint32_t* a, uint64_t n, const char* what) {
+  for (uint64_t i = 0; i < n; ++i)
+    if (a[i] > kMaxLabel) throw hl::InputError(std::string(what) + " labels must be below 2^31");
+}
+// throughput empty-community ids ebase + v need n < 2^31
+void check_vertex_count(const hl::Engine& e, uint64_t n) {
+  if (e.mode == HL_MODE_THROUGHPUT && n > kMaxLabel)
+    throw hl::InputError("throughput mode holds fewer than 2^31 vertices");
+}
+
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -676,7 +737,7 @@ hl_status hl_create(const hl_config* cfg, hl_graph** out) {
   *out = nullptr;
   if (cfg->levels < 1 || cfg->levels > HL_MAX_LEVELS || !(cfg->gamma > 0.0) || !std::isfinite(cfg->gamma) ||
       (cfg->mode != HL_MODE_DETERMINISTIC && cfg->mode != HL_MODE_THROUGHPUT) || cfg->key_cache > 1 ||
-      cfg->hub_tables > 1 || cfg->decide_rounds > (1u << 20)) {
+      cfg->hub_tables > 1 || cfg->decide_rounds > (1u << 20) || cfg->refine_reference > 1) {
     g_create_err = "invalid config";
     return HL_INVALID_INPUT;
   }
@@ -728,6 +789,7 @@ hl_status hl_load_graph(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src
   return guard(g, [&] {
     Engine& e = g->e;
     if (n == 0) throw hl::InputError("a graph needs at least one vertex");
+    check_vertex_count(e, n);
     HL_CHECK(hipSetDevice(e.cfg.device));
     e.upload_batch(m, src, dst, w, n);
     e.reset(n, m);
@@ -742,6 +804,7 @@ hl_status hl_load(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, cons
   return guard(g, [&] {
     Engine& e = g->e;
     if (n == 0) throw hl::InputError("a graph needs at least one vertex");
+    check_vertex_count(e, n);
     HL_CHECK(hipSetDevice(e.cfg.device));
     double t0 = now_s();
     if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -765,6 +828,7 @@ hl_status hl_load_graphs(hl_graph* g, uint32_t ngraphs, const uint32_t* vertex_o
     for (uint32_t i = 0; i < ngraphs; ++i)
       if (vertex_offsets[i + 1] < vertex_offsets[i]) throw hl::InputError("vertex_offsets must not decrease");
     if (vertex_offsets[ngraphs] == 0) throw hl::InputError("the graphs need at least one vertex");
+    check_vertex_count(e, vertex_offsets[ngraphs]);
     HL_CHECK(hipSetDevice(e.cfg.device));
     double t0 = now_s();
     if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -899,11 +963,14 @@ hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const ui
     HL_CHECK(hipSetDevice(e.cfg.device));
     hl::Level& lv = const_cast<hl::Level&>(e.level(level));
     if (n != lv.n) throw hl::InputError("length differs from the level size");
+    check_labels(in, n, "map");
     if (n) HL_CHECK(hipMemcpyAsync(e.map_of(lv, which).p, in, n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));
     if (which == HL_MAP_F) {
       // caller-written labels: rebuild the maintained totals on next use
       lv.tot_valid = false;
       lv.lab_bound = n ? std::max<uint64_t>(lv.n, uint64_t(e.dmax(lv.f.p, n)) + 1) : 0;
+      lv.ext = lv.lab_bound;
+      lv.ebase_set = false;
     } else if ((which == HL_MAP_S_CUR || which == HL_MAP_S_PRE) && n) {
       lv.s_bound = std::max<uint64_t>(lv.s_bound, uint64_t(e.dmax(e.map_of(lv, which).p, n)) + 1);
       if (which == HL_MAP_S_CUR) lv.sub_valid = false;
@@ -921,6 +988,13 @@ hl_status hl_restore_level(hl_graph* g, int32_t level, uint32_t n, uint64_t m, c
     HL_CHECK(hipSetDevice(e.cfg.device));
     if (level < 0 || level >= e.P) throw hl::InputError("level out of range");
     if (level == 0 && n == 0) throw hl::InputError("a graph needs at least one vertex");
+    check_vertex_count(e, n);
+    {
+      const uint32_t* maps[4] = {f, gmap, s_pre, s_cur};
+      const char* names[4] = {"f", "g", "s_pre", "s_cur"};
+      for (int i = 0; i < 4; ++i)
+        if (maps[i]) check_labels(maps[i], n, names[i]);
+    }
     if (level == 0) {
       e.G = 0;
       e.goff.clear();
@@ -948,6 +1022,8 @@ hl_status hl_restore_level(hl_graph* g, int32_t level, uint32_t n, uint64_t m, c
     lv.tot_valid = false;
     lv.sub_valid = false;
     lv.lab_bound = n ? std::max<uint64_t>(n, uint64_t(mx[0]) + 1) : 0;
+    lv.ext = lv.lab_bound;
+    lv.ebase_set = false;
     lv.s_bound = n ? std::max<uint64_t>(n, uint64_t(std::max(mx[2], mx[3])) + 1) : 0;
   });
 }
@@ -972,43 +1048,26 @@ hl_status hl_read_level_graph(const hl_graph* g, int32_t level, uint64_t* offset
     HL_CHECK(hipSetDevice(e.cfg.device));
     hl::Level& lv = const_cast<hl::Level&>(e.level(level));
     const uint32_t n = lv.n;
-    std::vector<uint32_t> len(n);
-    if (n) HL_CHECK(hipMemcpyAsync(len.data(), lv.row_len.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
-    hl::stream_sync(e.s);
-    uint64_t total = 0;
-    for (uint32_t v = 0; v < n; ++v) total += len[v];
+    // the rows laid out contiguously in a temporary (the live rows stay as they are)
+    hl::LevelCsrMut m = lv.mut();
+    hl::DevBuf<uint64_t> offs;
+    const uint64_t total = n ? hl::compact_plan(m, offs, e.tmp, e.s) : 0;
     *nnz = total;
     if (!offsets && !cols && !w) return;
     if (!offsets || !cols || !w) throw hl::InputError("need offsets, cols and w");
     if (cap < total) throw hl::InputError("output buffer too small");
+    offsets[n] = total;
+    if (n == 0) return;
     hl::DevBuf<uint32_t> c2;
     hl::DevBuf<double> w2;
-    c2.reserve(total + 1, e.s);
-    w2.reserve(total + 1, e.s);
-    // copy into a compact temporary without touching the live rows
-    hl::LevelCsrMut m = lv.mut();
-    hl::DevBuf<uint64_t> offs;
-    hl::DevBuf<uint64_t> lens;
-    lens.reserve(n + 1, e.s);
-    offs.reserve(n + 1, e.s);
-    std::vector<uint64_t> hl64(n);
-    for (uint32_t v = 0; v < n; ++v) hl64[v] = len[v];
-    if (n) HL_CHECK(hipMemcpyAsync(lens.p, hl64.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, e.s));
-    hl::exclusive_scan_u64(lens.p, offs.p, n, e.tmp, e.s);
-    std::vector<uint64_t> hoff(n + 1);
-    HL_CHECK(hipMemcpyAsync(hoff.data(), offs.p, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, e.s));
-    std::vector<uint64_t> roff(n);
-    if (n) HL_CHECK(hipMemcpyAsync(roff.data(), m.row_off, n * sizeof(uint64_t), hipMemcpyDeviceToHost, e.s));
-    hl::stream_sync(e.s);
-    for (uint32_t v = 0; v < n; ++v) {
-      offsets[v] = hoff[v];
-      if (len[v]) {
-        HL_CHECK(hipMemcpyAsync(cols + hoff[v], lv.col.p + roff[v], len[v] * sizeof(uint32_t),
-                                hipMemcpyDeviceToHost, e.s));
-        HL_CHECK(hipMemcpyAsync(w + hoff[v], lv.w.p + roff[v], len[v] * sizeof(double), hipMemcpyDeviceToHost, e.s));
-      }
+    c2.reserve(total + 1, e.s, false, 1.0);
+    w2.reserve(total + 1, e.s, false, 1.0);
+    hl::gather_rows(m, offs.p, c2.p, w2.p, e.s);
+    HL_CHECK(hipMemcpyAsync(offsets, offs.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, e.s));
+    if (total) {
+      HL_CHECK(hipMemcpyAsync(cols, c2.p, total * sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
+      HL_CHECK(hipMemcpyAsync(w, w2.p, total * sizeof(double), hipMemcpyDeviceToHost, e.s));
     }
-    offsets[n] = total;
     hl::stream_sync(e.s);
   });
 }
@@ -1088,7 +1147,7 @@ hl_status hl_modularity(const hl_graph* g, int32_t which, double* q) {
     else
       throw hl::InputError("unknown partition");
     if (e.multi()) throw hl::InputError("several graphs per engine: use hl_modularity_graphs");
-    *q = hl::modularity(e.L[0]->csr(), e.lab.p, e.sig, e.acc, e.mx, e.s);
+    *q = hl::modularity(e.L[0]->csr(), e.lab.p, e.L[0]->deg_bound, e.sig, e.acc, e.mx, e.s);
   });
 }
 
@@ -1276,7 +1335,7 @@ hl_status hl_modularity_labels(const hl_graph* g, const uint32_t* labels, uint64
     if (n != e.n0) throw hl::InputError("labels length differs from n");
     e.lab.reserve(n + 1, e.s);
     if (n) HL_CHECK(hipMemcpyAsync(e.lab.p, labels, n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));
-    *q = hl::modularity(e.L[0]->csr(), e.lab.p, e.sig, e.acc, e.mx, e.s);
+    *q = hl::modularity(e.L[0]->csr(), e.lab.p, e.L[0]->deg_bound, e.sig, e.acc, e.mx, e.s);
   });
 }
 

@@ -106,6 +106,8 @@ void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, co
 // [col2, w2) (row_cap = row_len) and synchronises.
 u64 compact_plan(const LevelCsrMut& g, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s);
 void compact_copy(const LevelCsrMut& g, const u64* offs, u32* col2, double* w2, hipStream_t s);
+// the rows laid out at offs (from compact_plan) into col2/w2, leaving the live rows as they are
+void gather_rows(const LevelCsrMut& g, const u64* offs, u32* col2, double* w2, hipStream_t s);
 // canonical 2m (pairwise tree); partials needs twice_m_scratch(n) doubles
 u64 twice_m_scratch(u32 n);
 void twice_m_tree(const double* k, u32 n, double* partials, double* out, hipStream_t s);
@@ -167,17 +169,21 @@ struct SubRef {
 // connected-component split of subcommunities touched by K (hit_leiden.rs:1276-1331).
 // s_bound: every label of s is below it.  Returns max(s) + 1 after the split and
 // sets *identity when s is the identity map after it (hit_leiden.rs:1333-1339).
+// f != nullptr links two vertices only through edges inside their community
+// (f(u) = f(v)); nullptr links every same-subcommunity neighbour (hit_leiden.rs:1233-1240)
 u64 cc_split(const Csr& g, u32* s, const u8* K, u8* R, u64 s_bound, bool* identity, const SubRef& sub,
-             RefineScratch& rs, Temp& tmp, hipStream_t st);
+             RefineScratch& rs, Temp& tmp, hipStream_t st, const u32* f = nullptr);
 // size[s(v)] += 1 for the vertices [n0, n1) a level grew by (their k is 0)
 void sub_add_vertices(const u32* s, u32 n0, u32 n1, u32* size, hipStream_t st);
 // from scratch: size and deg over every vertex (arrays zero-filled first, cap entries)
 void sub_recompute(const u32* s, const double* k, u32 n, u32* size, double* deg, u64 cap, hipStream_t st);
-// throughput refinement (+connected-targets rule) over R
+// throughput refinement over R; keep_targets: the connected-targets rule (a
+// singleton another singleton proposes to join stays put), otherwise every
+// snapshot proposal applies (throughput.rs:278-287)
 struct MoveScratch;
 void refine_parallel(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space, u64 sn,
                      RefineScratch& rs, ProposeScratch& ps, MoveScratch& mvs, u32 maxdeg, const Thresholds& th,
-                     Temp& tmp, const double* dpar_in, const SubRef& sub, hipStream_t st);
+                     Temp& tmp, const double* dpar_in, const SubRef& sub, hipStream_t st, bool keep_targets = true);
 // deterministic refinement (hit_leiden.rs:1359-1413)
 void refine_det(const Csr& g, const u32* f, u32* s, const u8* R, double gamma, u64 comm_space, u64 sn,
                 RefineScratch& rs, DevBuf<u32>& ks, DevBuf<u32>& vs, DevBuf<u32>& vidx, DevBuf<u8>& heads, DevBuf<u32>& starts,
@@ -199,12 +205,22 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
 
 // ---- def_update (hit_leiden.rs:1733-1793) ----
 // f^p(v) = f^{p+1}(s^p(v)) for v in B; tot (may be null) is level p's maintained totals
+// f^p(v) = f^q(s^p(v)) for v in B (every v when B is null), keeping tot current
 void def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, double* tot, const double* k,
               hipStream_t s);
 void def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np, hipStream_t s);
 
 // ---- modularity (hit_leiden.rs:121-161) ----
-double modularity(const Csr& g, const u32* lab, DevBuf<double>& sig, DevBuf<double>& acc, DevBuf<u32>& mx,
+// rows above kIntraHub entries are summed by workgroups over pieces of kIntraPiece
+// entries, from a list of at most kIntraHubList rows
+constexpr u32 kIntraHub = 4096, kIntraPiece = 16384, kIntraHubList = 65536;
+// The kernels of compute_modularity over g with labels below L, allocating
+// nothing: sig[L] per-label degree sums, acc[2] = (intra weight, sum sig^2/2m),
+// hubs[hub_cap + 1] the hub-row list and its count; Q = (acc[0] - acc[1]) / 2m.
+void modularity_launch(const Csr& g, const u32* lab, u64 L, u32 maxdeg, double* sig, double* acc, u32* hubs,
+                       u32 hub_cap, hipStream_t s);
+// maxdeg: an upper bound of the row lengths
+double modularity(const Csr& g, const u32* lab, u32 maxdeg, DevBuf<double>& sig, DevBuf<double>& acc, DevBuf<u32>& mx,
                   hipStream_t s);
 void gather_u32(u32* cur, const u32* map, u32 n, hipStream_t s);
 void canonicalize(u32* lab, u32 n, DevBuf<u32>& first, DevBuf<u8>& flag, DevBuf<u64>& pos, Temp& tmp,

@@ -654,8 +654,38 @@ __global__ void k_seed(const Rec* d, u64 k, const u32* f, const u32* s, u8* A, u
   }
 }
 
-__global__ void k_label_totals_atomic(const u32* lab, const double* k, u32 n, double* tot) {
-  for (u64 v = gtid(); v < n; v += gstride()) atomicAdd(&tot[lab[v]], k[v]);
+// tot[lab[v]] += k[v], combined per workgroup pass in an LDS hash table (one
+// device atomic per distinct label of the pass): the members of one big
+// community would otherwise serialise on one address
+constexpr u32 kTotTable = 2048, kTotPer = 4;  // kTotPer * 256 insertions never fill the table
+__global__ __launch_bounds__(256) void k_label_totals_atomic(const u32* lab, const double* k, u32 n, double* tot) {
+  __shared__ u32 tk[kTotTable];
+  __shared__ double tv[kTotTable];
+  for (u64 base = u64(blockIdx.x) * 256 * kTotPer; base < n; base += u64(gridDim.x) * 256 * kTotPer) {
+    for (u32 i = threadIdx.x; i < kTotTable; i += 256) {
+      tk[i] = kNone;
+      tv[i] = 0.0;
+    }
+    __syncthreads();
+    for (u32 r = 0; r < kTotPer; ++r) {
+      const u64 v = base + u64(r) * 256 + threadIdx.x;
+      if (v >= n) continue;
+      const double x = k[v];
+      if (x == 0.0) continue;
+      const u32 key = lab[v];
+      u32 h = hash32(key) & (kTotTable - 1);
+      while (true) {
+        const u32 old = atomicCAS(&tk[h], kNone, key);
+        if (old == kNone || old == key) break;
+        h = (h + 1) & (kTotTable - 1);
+      }
+      atomicAdd(&tv[h], x);
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < kTotTable; i += 256)
+      if (tk[i] != kNone) atomicAdd(&tot[tk[i]], tv[i]);
+    __syncthreads();
+  }
 }
 // per label segment (vertices ascending): sequential sum
 __global__ void k_segment_sums(u64 nseg, const u32* starts, u64 n, const u32* keys_s, const u32* vals_s,
@@ -958,8 +988,10 @@ __device__ __forceinline__ void uf_union(u32* parent, u32 a, u32 b) {
     b = lo;
   }
 }
-// edges (v,u) with s(u) = s(v), u > v: union; roots end as the component minimum
-__global__ __launch_bounds__(256) void k_uf_edges(Csr g, const u32* vcc, u64 m, const u32* s, u32* parent) {
+// edges (v,u) with s(u) = s(v), u > v (and f(u) = f(v) when f is given): union;
+// roots end as the component minimum
+__global__ __launch_bounds__(256) void k_uf_edges(Csr g, const u32* vcc, u64 m, const u32* s, const u32* f,
+                                                  u32* parent) {
   const u32 lane = threadIdx.x & 63;
   const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
   for (u64 q = wave; q < m; q += nwaves) {
@@ -967,9 +999,10 @@ __global__ __launch_bounds__(256) void k_uf_edges(Csr g, const u32* vcc, u64 m, 
     u64 off = g.row_off[v];
     u32 deg = g.row_len[v];
     u32 sv = s[v];
+    const u32 fv = f ? f[v] : 0;
     for (u32 j = lane; j < deg; j += 64) {
       u32 u = g.col[off + j];
-      if (u > v && s[u] == sv) uf_union(parent, v, u);
+      if (u > v && s[u] == sv && (!f || f[u] == fv)) uf_union(parent, v, u);
     }
   }
 }
@@ -1392,7 +1425,7 @@ __global__ void k_gather_rec(const u32* sel, u64 c, const Rec* in, Rec* out) {
 // ------------------------------------------------------------------ def_update / modularity
 __global__ void k_def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, double* tot, const double* k) {
   for (u64 v = gtid(); v < n; v += gstride()) {
-    if (!B[v]) continue;
+    if (B && !B[v]) continue;
     const u32 old = fp[v], nw = fq[sp[v]];
     fp[v] = nw;
     if (tot && old != nw && k[v] != 0.0) {  // keep the maintained community totals current
@@ -1410,9 +1443,15 @@ __global__ void k_def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np) 
 __global__ void k_gather_u32(u32* cur, const u32* map, u32 n) {
   for (u64 v = gtid(); v < n; v += gstride()) cur[v] = map[cur[v]];
 }
-// One lane per vertex for rows up to 64 entries; longer rows (hubs) are taken
-// by the whole wavefront, one after another, so no lane walks a hub row alone.
-__global__ void k_intra(Csr g, const u32* lab, double* acc) {
+// Intra-community weight (every edge from both ends, a self-loop's entry
+// twice; compute_modularity, hit_leiden.rs:121-161).  Each wavefront takes 64
+// consecutive vertices and walks their rows as one concatenated sequence, 64
+// consecutive entries per step (the row of entry t is found by a binary search
+// over the lanes' degree prefix), so the lanes load neighbouring addresses and
+// none idles on a short row.  Rows above kIntraHub entries are listed for
+// k_intra_hubs; when the list is full the wavefront walks them itself.
+__global__ __launch_bounds__(256) void k_intra(Csr g, const u32* lab, double* acc, u32* hubs, u32* nhubs,
+                                               u32 hub_cap, u64 hub_max) {
   double local = 0.0;
   const u32 lane = threadIdx.x & 63;
   for (u64 base = gtid() - lane; base < g.n; base += gstride()) {
@@ -1424,15 +1463,40 @@ __global__ void k_intra(Csr g, const u32* lab, double* acc) {
       deg = g.row_len[v];
       lv = lab[v];
     }
-    const bool wide = deg > 64;
-    if (!wide)
-      for (u32 j = 0; j < deg; ++j) {
-        const u32 u = g.col[off + j];
-        if (lab[u] == lv) {
-          const double w = g.w[off + j];
-          local += (u == v) ? 2.0 * w : w;
+    bool wide = deg > kIntraHub;
+    if (wide && deg <= hub_max) {  // hub_max: the rows k_intra_hubs covers in full
+      const u32 slot = atomicAdd(nhubs, 1u);
+      if (slot < hub_cap) {
+        hubs[slot] = u32(v);
+        wide = false;
+        deg = 0;
+      }
+    }
+    const u32 d0 = wide ? 0u : deg;
+    u32 x = d0;  // inclusive prefix of d0 over the lanes
+    for (u32 o = 1; o < 64; o <<= 1) {
+      const u32 y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    const u32 total = __shfl(x, 63, 64), pre = x - d0;
+    for (u32 t0 = 0; t0 < total; t0 += 64) {
+      const u32 t = t0 + lane;
+      u32 r = 0;  // the last lane whose prefix is <= t
+      for (u32 step = 32; step > 0; step >>= 1) {
+        const u32 pc = __shfl(pre, r + step, 64);
+        if (pc <= t) r += step;
+      }
+      const u64 ro = __shfl((unsigned long long)off, r, 64);
+      const u32 rp = __shfl(pre, r, 64), rl = __shfl(lv, r, 64);
+      if (t < total) {
+        const u64 e = ro + (t - rp);
+        const u32 u = g.col[e];
+        if (lab[u] == rl) {
+          const double w = g.w[e];
+          local += (u == base + r) ? 2.0 * w : w;
         }
       }
+    }
     unsigned long long m = __ballot(wide);
     while (m) {
       const int src = __ffsll((long long)m) - 1;
@@ -1446,6 +1510,29 @@ __global__ void k_intra(Csr g, const u32* lab, double* acc) {
           const double w = g.w[boff + j];
           local += (u == bv) ? 2.0 * w : w;
         }
+      }
+    }
+  }
+  double t = block_sum(local);
+  if (threadIdx.x == 0 && t != 0.0) atomicAdd(acc, t);
+}
+// the listed hub rows in pieces of kIntraPiece entries, one workgroup per piece
+__global__ __launch_bounds__(256) void k_intra_hubs(Csr g, const u32* lab, const u32* hubs, const u32* nhubs,
+                                                    u32 hub_cap, u32 pieces, double* acc) {
+  const u32 nh = min(*nhubs, hub_cap);
+  double local = 0.0;
+  for (u64 q = blockIdx.x; q < u64(nh) * pieces; q += gridDim.x) {
+    const u32 v = hubs[q / pieces];
+    const u64 b = (q % pieces) * u64(kIntraPiece);
+    const u32 deg = g.row_len[v];
+    if (b >= deg) continue;
+    const u64 end = min<u64>(deg, b + kIntraPiece), off = g.row_off[v];
+    const u32 lv = lab[v];
+    for (u64 j = b + threadIdx.x; j < end; j += 256) {
+      const u32 u = g.col[off + j];
+      if (lab[u] == lv) {
+        const double w = g.w[off + j];
+        local += (u == v) ? 2.0 * w : w;
       }
     }
   }
@@ -1887,6 +1974,11 @@ u64 compact_plan(const LevelCsrMut& g, DevBuf<u64>& off_tmp, Temp& tmp, hipStrea
   return total;
 }
 
+void gather_rows(const LevelCsrMut& g, const u64* offs, u32* col2, double* w2, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_rows, GRID(u64(g.n) * 64), g, offs, col2, w2);
+  HL_LAUNCH_CHECK();
+}
+
 void compact_copy(const LevelCsrMut& g, const u64* offs, u32* col2, double* w2, hipStream_t s) {
   hipLaunchKernelGGL(k_copy_rows, GRID(u64(g.n) * 64), g, offs, col2, w2);
   HL_LAUNCH_CHECK();
@@ -2015,7 +2107,7 @@ void sub_recompute(const u32* sm, const double* k, u32 n, u32* size, double* deg
 }
 
 u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* identity, const SubRef& sub,
-             RefineScratch& rs, Temp& tmp, hipStream_t s) {
+             RefineScratch& rs, Temp& tmp, hipStream_t s, const u32* f) {
   const u32 n = g.n;
   *identity = true;
   if (n == 0) return 1;
@@ -2056,7 +2148,7 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
   HL_PROF(P_CC_SPLIT, 16 * m, s);
   hipLaunchKernelGGL(k_uf_init, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p);
   HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_uf_edges, GRID(m * 64), g, rs.vcc.p, m, sm, rs.parent.p);
+  hipLaunchKernelGGL(k_uf_edges, GRID(m * 64), g, rs.vcc.p, m, sm, f, rs.parent.p);
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_uf_compress, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p);
   HL_LAUNCH_CHECK();
@@ -2086,7 +2178,7 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
 
 void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space, u64 sn,
                      RefineScratch& rs, ProposeScratch& ps, MoveScratch& mvs, u32 maxdeg, const Thresholds& th,
-                     Temp& tmp, const double* dpar_in, const SubRef& sub, hipStream_t s) {
+                     Temp& tmp, const double* dpar_in, const SubRef& sub, hipStream_t s, bool keep_targets) {
   const u32 n = g.n;
   if (n == 0) return;
   // subcommunity sizes and degrees (hit_leiden.rs:1341-1357): kept by the caller, or recomputed
@@ -2134,7 +2226,7 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
   // proposals against the snapshot, then the connected-targets rule
   (void)ps;
   (void)tmp;
-  refine_proposals(g, f, sm, dS, rs.blocked.p, gamma, sl.p, ns, sn, mvs, th, maxdeg, sub, s);
+  refine_proposals(g, f, sm, dS, rs.blocked.p, gamma, sl.p, ns, sn, mvs, th, maxdeg, sub, s, keep_targets);
 }
 
 void refine_det(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, u64 comm_space, u64 sn,
@@ -2269,14 +2361,36 @@ void def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np, hipStream_t s
   HL_LAUNCH_CHECK();
 }
 
-double modularity(const Csr& g, const u32* lab, DevBuf<double>& sig, DevBuf<double>& acc, DevBuf<u32>& mx,
-                  hipStream_t s) {
+void modularity_launch(const Csr& g, const u32* lab, u64 L, u32 maxdeg, double* sig, double* acc, u32* hubs,
+                       u32 hub_cap, hipStream_t s) {
+  const u32 n = g.n;
+  fill_f64(sig, 0.0, L, s);
+  fill_f64(acc, 0.0, 2, s);
+  fill_u32(hubs + hub_cap, 0, 1, s);
+  label_totals_atomic(lab, g.k, n, sig, s);
+  HL_PROF(P_MODULARITY, 0, s);
+  const u32 pieces = u32(std::max<u64>(1, (u64(maxdeg) + kIntraPiece - 1) / kIntraPiece));
+  const u64 hub_max = maxdeg > kIntraHub ? u64(pieces) * kIntraPiece : 0;
+  hipLaunchKernelGGL(k_intra, blocks_for(n, 256, 4096), 256, 0, s, g, lab, acc, hubs, hubs + hub_cap, hub_cap,
+                     hub_max);
+  HL_LAUNCH_CHECK();
+  if (hub_max) {
+    hipLaunchKernelGGL(k_intra_hubs, 1024, 256, 0, s, g, lab, hubs, hubs + hub_cap, hub_cap, pieces, acc);
+    HL_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_sq_sum, blocks_for(L, 256, 4096), 256, 0, s, sig, L, g.twice_m, acc + 1);
+  HL_LAUNCH_CHECK();
+}
+
+double modularity(const Csr& g, const u32* lab, u32 maxdeg, DevBuf<double>& sig, DevBuf<double>& acc,
+                  DevBuf<u32>& mx, hipStream_t s) {
   const u32 n = g.n;
   double t = 0.0;
   d2h_async(&t, g.twice_m, sizeof(double), s);
   stream_sync(s);
   if (t == 0.0 || n == 0) return 0.0;
-  mx.reserve(1, s);
+  const u32 hub_cap = std::min<u32>(n, kIntraHubList);
+  mx.reserve(u64(hub_cap) + 2, s);
   fill_u32(mx.p, 0, 1, s);
   max_u32(lab, n, mx.p, s);
   u32 hm = 0;
@@ -2285,14 +2399,7 @@ double modularity(const Csr& g, const u32* lab, DevBuf<double>& sig, DevBuf<doub
   const u64 L = u64(hm) + 1;
   sig.reserve(L, s);
   acc.reserve(2, s);
-  fill_f64(sig.p, 0.0, L, s);
-  fill_f64(acc.p, 0.0, 2, s);
-  label_totals_atomic(lab, g.k, n, sig.p, s);
-  HL_PROF(P_MODULARITY, 0, s);
-  hipLaunchKernelGGL(k_intra, blocks_for(n, 256, 4096), 256, 0, s, g, lab, acc.p);
-  HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sq_sum, blocks_for(L, 256, 4096), 256, 0, s, sig.p, L, g.twice_m, acc.p + 1);
-  HL_LAUNCH_CHECK();
+  modularity_launch(g, lab, L, maxdeg, sig.p, acc.p, mx.p + 1, hub_cap, s);
   double h[2];
   d2h_async(h, acc.p, 2 * sizeof(double), s);
   stream_sync(s);

@@ -84,15 +84,19 @@ struct MoveCtx {
   u32* ck = nullptr;
   u64 pool = 0;
   HubTables ht;
+  u32 ebase = 0;  // empty-community ids ebase + v (MovePolicyT)
 };
 
 // throughput movement (parallel_frontier.rs:155-236): self-loops skipped,
-// candidates in ascending id, empty community id = n + v
+// candidates in ascending id, empty community id = ebase + v.  The reference
+// uses n + v (parallel_frontier.rs:214); ebase = n there, and above n only on
+// a level whose label space also holds ids that n + v could meet (fresh ids of
+// grown vertices, labels pulled down from the level above)
 struct MovePolicyT {
   const u32* f;
   const double* tot;
   double gamma;
-  u32 n;
+  u32 ebase;
   const u32* ck = nullptr;  // neighbour-community cache (MoveCtx::ck), or none
   HubTables ht;             // hub tables (MoveCtx::ht), or none
   static constexpr bool kEmpty = true;
@@ -114,7 +118,7 @@ struct MovePolicyT {
   __device__ double cur_total(u32 c) const { return tot[c]; }
   __device__ bool ok(u32) const { return true; }
   __device__ double cand_total(u32 c) const { return tot[c]; }
-  __device__ u32 empty_id(u32 v) const { return n + v; }
+  __device__ u32 empty_id(u32 v) const { return ebase + v; }
 };
 
 // throughput refinement (try_refinement_move, throughput.rs:291-359):
@@ -163,7 +167,8 @@ enum DevStat : u32 {
   DS_TRY_REF = DS_TRY_MOVE + 3 * kBuckets,
   // hub-table evaluations: vertices, table slots scanned, live candidates
   DS_HT = DS_TRY_REF + 3 * kBuckets,
-  DS_COUNT = DS_HT + 3
+  DS_SETTLED = DS_HT + 3,  // candidates the in-order settle of the multi-kernel decoupling decided
+  DS_COUNT
 };
 
 struct MoveScratch {
@@ -195,6 +200,7 @@ struct MoveScratch {
 
 struct MoveStats {
   u64 active = 0, evaluated = 0, moved_rounds = 0;
+  u64 decide_rounds = 0;  // priority rounds of the multi-kernel decoupling
 };
 
 // Synchronous rounds (<= max_rounds) of the throughput movement at one level;
@@ -202,11 +208,11 @@ struct MoveStats {
 u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int max_rounds, MoveScratch& ms,
                         const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s);
 
-// Snapshot proposals of the singleton vertices `singles` plus the
-// connected-targets rule; updates s in place.
+// Snapshot proposals of the singleton vertices `singles`, with the
+// connected-targets rule when keep_targets; updates s in place.
 // sub (may be empty): maintained subcommunity sizes/degrees to update with the moves
 void refine_proposals(const Csr& g, const u32* f, u32* s, const double* dS, const u8* blocked, double gamma,
                       const u32* singles, u64 ns, u64 sspace, MoveScratch& ms, const Thresholds& th, u32 maxdeg,
-                      const SubRef& sub, hipStream_t st);
+                      const SubRef& sub, hipStream_t st, bool keep_targets = true);
 
 }  // namespace hl

@@ -1801,9 +1801,10 @@ __global__ void k_settle_prep(u64 c, const u32* from, const u32* to, const u32* 
   }
 }
 __global__ void k_settle_seq(const u32* list, const unsigned long long* cnt, const u32* from, const u32* to,
-                             u32* state, u32* emit, u32* recv) {
+                             u32* state, u32* emit, u32* recv, unsigned long long* dst) {
   if (gtid() != 0) return;
   const u64 n = *cnt;
+  dst[DS_SETTLED] += n;
   for (u64 i = 0; i < n; ++i) {
     const u32 p = list[i];
     const u32 f = from[p], t = to[p];
@@ -1848,7 +1849,7 @@ __global__ void k_ref_apply(CandOut cand, const u8* target, u32* s, const double
   const u32 c = *cand.cnt;
   for (u64 i = gtid(); i < c; i += gstride()) {
     const u32 v = cand.node[i], from = s[v], to = cand.to[i];
-    if (target[from]) continue;  // a target singleton keeps its place
+    if (target && target[from]) continue;  // a target singleton keeps its place
     s[v] = to;
     if (sub.size) {
       atomicSub(&sub.size[from], 1u);
@@ -2074,7 +2075,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   } else {
     m.ht = HubTables{};
   }
-  MovePolicyT pol{m.f, m.tot, m.gamma, n, m.ck, m.ht};
+  MovePolicyT pol{m.f, m.tot, m.gamma, m.ebase, m.ck, m.ht};
   u32 host_cnt = 0;
   d2h_async(&host_cnt, cnt_a, sizeof(u32), s);
   stream_sync(s);
@@ -2096,7 +2097,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
       HL_LAUNCH_CHECK();
     }
     if (m.ck) {
-      HL_PROF(P_MARK_MOVED, 0, s);
+      HL_PROF(P_CACHE_UPDATE, 0, s);
       hipLaunchKernelGGL(k_ck_flat, kMarksBlocks, 256, 0, s, m, fm);
       HL_LAUNCH_CHECK();
     }
@@ -2152,6 +2153,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
         u32 und = 0;
         d2h_async(&und, status + 1, sizeof(u32), s);
         stream_sync(s);
+        if (st) ++st->decide_rounds;
         if (und == 0) break;
         if (++iters >= int(th.dec_rounds)) {
           // the undecided rest in priority order, against flags of the accepted
@@ -2167,7 +2169,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
           select_flagged_async(undec, c, ms.v32.p, tmp, ms.k64b.p, s);
           hipLaunchKernelGGL(k_settle_seq, 1, 64, 0, s, ms.v32.p,
                              reinterpret_cast<const unsigned long long*>(ms.k64b.p), ms.from.p, ms.to.p, ms.state.p,
-                             ms.minAfrom.p, ms.minAto.p);
+                             ms.minAfrom.p, ms.minAto.p, ms.dstat.p);
           HL_LAUNCH_CHECK();
           break;
         }
@@ -2181,18 +2183,24 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
         HL_LAUNCH_CHECK();
       }
       {
-        HL_PROF(P_MARK_MOVED, 0, s);
         ms.mdeg.reserve(c, s);
         ms.mpre.reserve(u64(c) + 2, s);
-        hipLaunchKernelGGL(k_acc_degrees, fixed_grid(c), 256, 0, s, u64(c), ms.node.p, ms.state.p, m.g.row_len,
-                           ms.mdeg.p);
-        HL_LAUNCH_CHECK();
-        exclusive_scan_u64_async(ms.mdeg.p, ms.mpre.p, c, tmp, s);
-        fill_u32(ms.counters.p + 13, c, 1, s);
         const FlatMarks fmm{ms.node.p, ms.to.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 13, ms.from.p};
-        hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fmm, ms.mark.p, ep, nxt, next_cnt);
-        HL_LAUNCH_CHECK();
+        {
+          HL_PROF(P_DEC_MULTI, 0, s);
+          hipLaunchKernelGGL(k_acc_degrees, fixed_grid(c), 256, 0, s, u64(c), ms.node.p, ms.state.p, m.g.row_len,
+                             ms.mdeg.p);
+          HL_LAUNCH_CHECK();
+          exclusive_scan_u64_async(ms.mdeg.p, ms.mpre.p, c, tmp, s);
+          fill_u32(ms.counters.p + 13, c, 1, s);
+        }
+        {
+          HL_PROF(P_MARK_MOVED, 0, s);
+          hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fmm, ms.mark.p, ep, nxt, next_cnt);
+          HL_LAUNCH_CHECK();
+        }
         if (m.ck) {
+          HL_PROF(P_CACHE_UPDATE, 0, s);
           hipLaunchKernelGGL(k_ck_flat, kMarksBlocks, 256, 0, s, m, fmm);
           HL_LAUNCH_CHECK();
         }
@@ -2216,7 +2224,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
 
 void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, const u8* blocked, double gamma,
                       const u32* singles, u64 ns, u64 sspace, MoveScratch& ms, const Thresholds& th, u32 maxdeg,
-                      const SubRef& sub, hipStream_t s) {
+                      const SubRef& sub, hipStream_t s, bool keep_targets) {
   ms.ensure(g.n, 1, s);
   ms.target.reserve(sspace, s);
   if (ms.target_cap != ms.target.cap) {
@@ -2232,9 +2240,13 @@ void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, con
   propose_round(pol, g, singles, cnt, ns, maxdeg, ms, th, ms.dstat.p + DS_REF, ms.dstat.p + DS_TRY_REF, s);
   CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
   HL_PROF(P_REFINE, 0, s);
-  hipLaunchKernelGGL(k_ref_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);
-  hipLaunchKernelGGL(k_ref_apply, fixed_grid(ns), 256, 0, s, cand, ms.target.p, sm, g.k, sub);
-  hipLaunchKernelGGL(k_ref_clear_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);
+  if (keep_targets) {
+    hipLaunchKernelGGL(k_ref_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);
+    hipLaunchKernelGGL(k_ref_apply, fixed_grid(ns), 256, 0, s, cand, ms.target.p, sm, g.k, sub);
+    hipLaunchKernelGGL(k_ref_clear_targets, fixed_grid(ns), 256, 0, s, cand, ms.target.p);
+  } else {
+    hipLaunchKernelGGL(k_ref_apply, fixed_grid(ns), 256, 0, s, cand, nullptr, sm, g.k, sub);
+  }
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, nullptr, ccnt, ms.counters.p + 4);
   HL_LAUNCH_CHECK();

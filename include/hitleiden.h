@@ -84,7 +84,12 @@ typedef struct {
   uint32_t decide_rounds; /* throughput movement, more than 4,096 candidates: priority rounds of
                              the decoupling before the undecided rest is settled in priority
                              order by one thread (the same result); 0 = 48, at most 2^20 */
-  uint32_t reserved[2];
+  uint32_t refine_reference; /* throughput mode: 0 = a consistent hierarchy (a singleton another
+                                singleton proposes to join stays put, the component split links
+                                vertices only through edges inside their community, def_update
+                                pulls every vertex below the top level); 1 = the reference's
+                                rules (throughput.rs:278-287, hit_leiden.rs:1233-1240,1733-1793) */
+  uint32_t reserved[1];
 } hl_config;
 
 #define HL_MAX_LEVELS 8
@@ -103,6 +108,10 @@ typedef struct {
   uint64_t rounds[HL_MAX_LEVELS];
   uint64_t bytes_alg;                /* algorithmic HBM bytes of the batch */
   double seconds;                    /* host wall time of the call */
+  uint64_t decide_rounds[HL_MAX_LEVELS]; /* throughput: priority rounds of the grid-wide decoupling
+                                            (rounds with more than 4,096 candidates) */
+  uint64_t settled;                  /* throughput: candidates decided by the in-order settle after
+                                        decide_rounds priority rounds */
 } hl_batch_stats;
 
 typedef struct hl_graph hl_graph;
@@ -140,7 +149,9 @@ hl_status hl_level_size(const hl_graph* g, int32_t level, uint32_t* n);
 
 /* copy-out of state.*_mapping_per_level[level] (state.rs:26-31) */
 hl_status hl_read_partition(const hl_graph* g, int32_t level, int32_t which, uint32_t* out, uint64_t cap);
-/* copy-in (checkpoint restore / seeded starts); values are not validated beyond length */
+/* copy-in (checkpoint restore / seeded starts): n must equal the level size and
+   every label must be below 2^31 (0xFFFFFFFF is the kernels' empty key), else
+   HL_INVALID_INPUT with the map unchanged */
 hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const uint32_t* in, uint64_t n);
 
 /* Checkpoint restore: rebuild level `level` of a one-graph engine from its
@@ -148,7 +159,9 @@ hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const ui
    weight: the hl_read_level_graph content) and its four maps (n entries each;
    NULL leaves a map at the identity).  Restore levels 0..P-1 in order; level 0
    resets the handle.  No counterpart in the reference, which keeps state in
-   memory only (README goal 4 lists persistence). */
+   memory only (README goal 4 lists persistence).  Labels of the four maps must
+   be below 2^31, and a throughput engine holds fewer than 2^31 vertices
+   (HL_INVALID_INPUT otherwise). */
 hl_status hl_restore_level(hl_graph* g, int32_t level, uint32_t n, uint64_t m, const uint32_t* src,
                            const uint32_t* dst, const double* w, const uint32_t* f, const uint32_t* gmap,
                            const uint32_t* s_pre, const uint32_t* s_cur);

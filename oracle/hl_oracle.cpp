@@ -19,6 +19,20 @@
 //   D9 the deterministic sweep skips self-loop entries in w(v, C) as the throughput shard
 //      does (parallel_frontier.rs:164-166); the reference counts them (hit_leiden.rs:1119-1125),
 //      which only matters on levels >= 1 (level-0 graphs carry no self-loops)
+//   D10 throughput empty-community ids are ebase + v, not n + v (parallel_frontier.rs:214):
+//      ebase = n on a level that never grows and takes no labels from the level above
+//      (every level of a one-level run); elsewhere the range [ebase, ebase + ecap) is
+//      placed above every label in use whenever pulled-down labels may reach it or the
+//      level outgrows it, and a vertex added within the range starts in its own empty
+//      community ebase + v (set_empty_base, grow_level)
+//   D11 (OR_CONNECTED) the connected-component split links two vertices of a subcommunity
+//      only through edges inside their community (f(u) = f(v)), the edges the spec's
+//      subcommunity graph G_Psi keeps (hit_leiden_spec.md Alg. 2 lines 17-19, Alg. 3
+//      lines 2-4); the reference links all same-subcommunity neighbours (hit_leiden.rs:
+//      1233-1240, Psi not implemented), so a subcommunity whose vertex moved can straddle
+//      two communities
+//   D12 (OR_CONNECTED) def_update pulls every vertex of levels below the top, not only B^p /
+//      R^p (hit_leiden.rs:1733-1793): f^p = f^{p+1} o s^p holds after every batch
 #include "hl_oracle.h"
 
 #include <algorithm>
@@ -46,6 +60,12 @@ constexpr double kEpsSelf = 1e-12;      // hit_leiden.rs:1687
 constexpr double kEpsCompress = 1e-9;   // hit_leiden.rs:1711
 constexpr double kEpsTFilter = 1e-9;    // hit_leiden.rs:1547,1631
 constexpr int kMaxMoveRounds = 20;      // hit_leiden.rs:1077
+
+// test hooks: throughput moves to an empty-community id whose total was not
+// zero when the round's moves were selected (a populated community joined as
+// if it were empty), and a switch back to the reference's n + v ids
+u64 g_empty_collisions = 0;
+bool g_reference_empty_ids = false;
 
 
 struct Rec {
@@ -254,9 +274,10 @@ struct MoveResult {
 };
 
 // execute_shard's per-vertex body (parallel_frontier.rs:155-236): self-loops
-// skipped, candidates in ascending id, empty community id = n + v.
+// skipped, candidates in ascending id, empty community id = ebase + v (D10;
+// the reference's n + v is ebase = n).
 bool propose_thr(const Graph& g, u32 v, const std::vector<u32>& f, const std::vector<double>& tot, double t,
-                 double gamma, std::vector<std::pair<u32, double>>& buf, MoveCand& out) {
+                 double gamma, u64 ebase, std::vector<std::pair<u32, double>>& buf, MoveCand& out) {
   const u32 cur = f[v];
   const double k = g.k[v];
   buf.clear();
@@ -290,7 +311,7 @@ bool propose_thr(const Graph& g, u32 v, const std::vector<u32>& f, const std::ve
   double eg = empty_gain(w_cur, t, gamma, k, sc);
   if (eg > best_gain) {
     best_gain = eg;
-    best = g.n + v;
+    best = u32(ebase + v);
   }
   if (best_gain <= 0.0) return false;
   out = {v, cur, best, k, best_gain};
@@ -298,7 +319,7 @@ bool propose_thr(const Graph& g, u32 v, const std::vector<u32>& f, const std::ve
 }
 
 MoveResult inc_movement(const Graph& g, const std::vector<Rec>& dG, std::vector<u32>& f, const std::vector<u32>& s,
-                        double gamma, int mode) {
+                        double gamma, int mode, u64 ebase) {
   const u32 n = g.n;
   MoveResult r;
   std::vector<uint8_t> A(n, 0);
@@ -329,7 +350,7 @@ MoveResult inc_movement(const Graph& g, const std::vector<Rec>& dG, std::vector<
       {
         std::vector<std::pair<u32, double>> buf;
 #pragma omp for schedule(dynamic, 256)
-        for (long i = 0; i < long(active.size()); ++i) has[i] = propose_thr(g, active[i], f, tot, t, gamma, buf, props[i]);
+        for (long i = 0; i < long(active.size()); ++i) has[i] = propose_thr(g, active[i], f, tot, t, gamma, ebase, buf, props[i]);
       }
       std::vector<MoveCand> cand;
       for (size_t i = 0; i < active.size(); ++i)
@@ -342,6 +363,8 @@ MoveResult inc_movement(const Graph& g, const std::vector<Rec>& dG, std::vector<
       double total_gain = 0.0;
       for (auto& m : sel) total_gain += m.gain;
       if (total_gain <= 0.0) break;  // throughput.rs:177-180
+      for (auto& m : sel)
+        if (u64(m.to) == ebase + m.node && m.to != m.from && tot[m.to] != 0.0) ++g_empty_collisions;
       for (auto& m : sel) {
         f[m.node] = m.to;
         tot[m.from] -= m.deg;
@@ -477,7 +500,8 @@ inline double remove_gain(double w_s, double t, double gamma, double d_s, double
 }
 
 // connected-component split of the affected subcommunities (hit_leiden.rs:1276-1331)
-void cc_split(const Graph& g, std::vector<u32>& s, const std::vector<uint8_t>& K, std::vector<uint8_t>& R) {
+void cc_split(const Graph& g, std::vector<u32>& s, const std::vector<uint8_t>& K, std::vector<uint8_t>& R,
+              const std::vector<u32>* f) {
   const u32 n = g.n;
   if (n == 0) return;
   u64 smax = 0;
@@ -504,7 +528,7 @@ void cc_split(const Graph& g, std::vector<u32>& s, const std::vector<uint8_t>& K
       while (h < queue.size()) {
         u32 x = queue[h++];
         for (u32 y : g.col[x])
-          if (s[y] == u32(sc) && !visited[y]) {
+          if (s[y] == u32(sc) && !visited[y] && (!f || (*f)[y] == (*f)[x])) {  // D11
             visited[y] = 1;
             queue.push_back(y);
           }
@@ -531,7 +555,7 @@ std::vector<uint8_t> inc_refinement(const Graph& g, const std::vector<u32>& f, s
                                     const std::vector<uint8_t>& K, double gamma, int mode) {
   const u32 n = g.n;
   std::vector<uint8_t> R(n, 0);
-  cc_split(g, s, K, R);
+  cc_split(g, s, K, R, (mode & OR_CONNECTED) ? &f : nullptr);
   bool is_initial = true;
   for (u32 v = 0; v < n; ++v)
     if (s[v] != v) {
@@ -750,7 +774,7 @@ std::vector<Rec> inc_aggregation(const Graph& g, const std::vector<Rec>& dG, std
 
 // def_update (hit_leiden.rs:1733-1793), per-level sizes.
 void def_update(std::vector<std::vector<u32>*>& maps, const std::vector<const std::vector<u32>*>& sub,
-                std::vector<std::vector<uint8_t>>& B) {
+                std::vector<std::vector<uint8_t>>& B, bool all = false) {
   int P = int(maps.size());
   for (int p = P - 1; p >= 0; --p) {
     if (p < P - 1) {
@@ -758,7 +782,8 @@ void def_update(std::vector<std::vector<u32>*>& maps, const std::vector<const st
       const auto& fq = *maps[p + 1];
       const auto& sp = *sub[p];
       for (size_t v = 0; v < B[p].size(); ++v)
-        if (B[p][v]) fp[v] = fq[sp[v]];
+        if (all || B[p][v]) fp[v] = fq[sp[v]];  // D12 with all
+      if (all) continue;  // every vertex pulled: the B propagation changes nothing
     }
     if (p > 0) {
       const auto& sq = *sub[p - 1];
@@ -798,7 +823,37 @@ struct Level {
   std::vector<uint8_t> A0, B, K, R;
   std::vector<Rec> dG;
   u32 iters = 0;
+  // D10 bookkeeping: every f label < lab_bound; ext bounds the labels that did
+  // not come from the level's own empty range [ebase, ebase + ecap)
+  u64 lab_bound = 0, ext = 0, ebase = 0, ecap = 0;
+  bool ebase_set = false;
+  u64 label_space() const { return std::max<u64>(ext, ebase_set ? ebase + ecap : 0); }
 };
+
+u64 vmax_or(const std::vector<u32>& x, u64 dflt);
+
+// D10: a new place for the empty range (above every label in use and above
+// ext, with room for the level to double) when labels from outside it may lie
+// in it or the level outgrew it
+void set_empty_base(Level& lv, bool thr) {
+  const u64 n = lv.G.n;
+  if (g_reference_empty_ids) {
+    lv.ebase = n;
+    lv.ecap = n;
+    lv.ebase_set = true;
+    lv.lab_bound = std::max<u64>(lv.lab_bound, 2 * n);
+    return;
+  }
+  if (!thr) return;
+  if (!lv.ebase_set || lv.ext > lv.ebase || n > lv.ecap) {
+    const u64 mx = n ? vmax_or(lv.f, 0) + 1 : 0;
+    lv.ext = std::max<u64>(lv.ext, mx);
+    lv.ecap = lv.ebase_set ? 2 * n : n;
+    lv.ebase = std::max<u64>(n, lv.ext);
+    lv.ebase_set = true;
+  }
+  lv.lab_bound = std::max<u64>(lv.lab_bound, lv.label_space());
+}
 
 struct State {
   int P = 1;
@@ -813,17 +868,22 @@ u64 vmax_or(const std::vector<u32>& x, u64 dflt) {
   return *std::max_element(x.begin(), x.end());
 }
 
-// D6: new vertices [n_old, n_new) get fresh labels.
-void grow_level(Level& lv, u32 n_new, const Level* up) {
+// D6: new vertices [n_old, n_new) get fresh labels (D10: f labels in their own
+// empty communities while the throughput empty range has room for them).
+void grow_level(Level& lv, u32 n_new, const Level* up, bool thr) {
   u32 n_old = lv.G.n;
   if (n_new <= n_old) return;
   u64 base_f = std::max<u64>(n_old, lv.f.empty() ? 0 : vmax_or(lv.f, 0) + 1);
+  const bool own = thr && !g_reference_empty_ids && lv.ebase_set && n_new <= lv.ecap && lv.ext <= lv.ebase;
+  if (own) base_f = lv.ebase + n_old;
   u64 base_g = std::max<u64>(n_old, lv.g.empty() ? 0 : vmax_or(lv.g, 0) + 1);
   u64 ms = 0;
   bool have = !lv.s_cur.empty();
   if (have) ms = std::max(vmax_or(lv.s_cur, 0), vmax_or(lv.s_pre, 0)) + 1;
   u64 base_s = std::max<u64>(n_old, ms);
   if (up) base_s = std::max<u64>(base_s, up->G.n);
+  if (!own) lv.ext = std::max<u64>(lv.ext, base_f + (n_new - n_old));
+  lv.lab_bound = std::max<u64>(lv.lab_bound, std::max<u64>(lv.label_space(), base_f + (n_new - n_old)));
   for (u32 v = n_old; v < n_new; ++v) {
     lv.f.push_back(u32(base_f + (v - n_old)));
     lv.g.push_back(u32(base_g + (v - n_old)));
@@ -840,10 +900,11 @@ u32 hit_leiden(State& st, const std::vector<Rec>& dG0) {
   const int P = st.P;
   for (int p = 0; p < P; ++p) {
     Level& lv = st.L[p];
-    grow_level(lv, std::max(lv.G.n, cur_n), p + 1 < P ? &st.L[p + 1] : nullptr);
+    grow_level(lv, std::max(lv.G.n, cur_n), p + 1 < P ? &st.L[p + 1] : nullptr, (st.mode & OR_THR) != 0);
     apply_delta(lv.G, cur, lv.G.n);
     lv.dG = cur;
-    MoveResult mr = inc_movement(lv.G, cur, lv.f, lv.s_cur, st.gamma, st.mode);
+    set_empty_base(lv, (st.mode & OR_THR) != 0);
+    MoveResult mr = inc_movement(lv.G, cur, lv.f, lv.s_cur, st.gamma, st.mode, lv.ebase);
     lv.A0 = std::move(mr.A0);
     lv.B = std::move(mr.B);
     lv.K = std::move(mr.K);
@@ -874,15 +935,20 @@ u32 hit_leiden(State& st, const std::vector<Rec>& dG0) {
     gm[p] = &st.L[p].g;
     sm[p] = &st.L[p].s_cur;
   }
-  def_update(fm, sm, B);
-  def_update(gm, sm, R);
+  const bool all = (st.mode & OR_CONNECTED) != 0;  // D12
+  def_update(fm, sm, B, all);
+  def_update(gm, sm, R, all);
+  for (int p = P - 2; p >= 0; --p) {  // D10: f^p took f^{p+1} labels
+    st.L[p].lab_bound = std::max(st.L[p].lab_bound, st.L[p + 1].lab_bound);
+    st.L[p].ext = std::max(st.L[p].ext, st.L[p + 1].lab_bound);
+  }
   return total;
 }
 
 void reset_state(State& st, u32 n) {
   st.L.assign(st.P, Level());
   st.n0 = n;
-  grow_level(st.L[0], n, st.P > 1 ? &st.L[1] : nullptr);
+  grow_level(st.L[0], n, st.P > 1 ? &st.L[1] : nullptr, (st.mode & OR_THR) != 0);
 }
 
 bool valid_edges(u32 n, u64 m, const u32* u, const u32* v, const double* w) {
@@ -978,6 +1044,12 @@ int or_write_map(void* h, int level, int which, const uint32_t* in) {
   if (level < 0 || level >= st.P) return -1;
   auto& m = level_map(st.L[level], which);
   std::copy(in, in + m.size(), m.begin());
+  if (which == OR_MAP_F) {  // caller-written labels: a fresh empty base on the next batch
+    Level& lv = st.L[level];
+    lv.lab_bound = m.empty() ? 0 : std::max<u64>(m.size(), vmax_or(m, 0) + 1);
+    lv.ext = lv.lab_bound;
+    lv.ebase_set = false;
+  }
   return 0;
 }
 
@@ -1079,7 +1151,7 @@ int or_inc_movement_fn(uint32_t n, uint64_t m, const uint32_t* eu, const uint32_
   std::vector<u32> fv(f, f + n), sv(s, s + n);
   std::vector<Rec> d(k);
   for (u64 i = 0; i < k; ++i) d[i] = {du[i], dv[i], dw ? dw[i] : 1.0};
-  MoveResult r = inc_movement(g, d, fv, sv, gamma, mode);
+  MoveResult r = inc_movement(g, d, fv, sv, gamma, mode, g.n);
   std::copy(fv.begin(), fv.end(), f);
   if (outB) std::copy(r.B.begin(), r.B.end(), outB);
   if (outK) std::copy(r.K.begin(), r.K.end(), outK);
@@ -1174,6 +1246,18 @@ void or_canonicalize_fn(uint32_t n, uint32_t* labels) {
       labels[i] = next++;
     }
   }
+}
+
+uint64_t or_empty_collisions(int reset) {
+  const u64 c = g_empty_collisions;
+  if (reset) g_empty_collisions = 0;
+  return c;
+}
+void or_set_reference_empty_ids(int on) { g_reference_empty_ids = on != 0; }
+uint64_t or_empty_base(void* h, int level) {
+  State& st = *static_cast<State*>(h);
+  if (level < 0 || level >= st.P) return 0;
+  return st.L[level].ebase;
 }
 
 int or_should_skip_aggregation_fn(uint64_t delta_len, uint32_t n, const uint8_t* R) {

@@ -77,6 +77,12 @@ double or_modularity_fn(uint32_t n, uint64_t m, const uint32_t* eu, const uint32
 uint64_t or_select_decoupled_fn(uint64_t k, const uint32_t* node, const uint32_t* from, const uint32_t* to,
                                 const double* gain, uint32_t* out_nodes);
 void or_canonicalize_fn(uint32_t n, uint32_t* labels);
+/* test hooks: throughput moves onto an empty-community id whose total was not
+   zero (counted since the last reset), the reference's n + v ids instead of
+   the D10 base (a planted mistake for the tests), a level's empty base */
+uint64_t or_empty_collisions(int reset);
+void     or_set_reference_empty_ids(int on);
+uint64_t or_empty_base(void* h, int level);
 int  or_should_skip_aggregation_fn(uint64_t delta_len, uint32_t n, const uint8_t* R);
 
 #ifdef __cplusplus

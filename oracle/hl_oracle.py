@@ -84,6 +84,12 @@ def lib():
         L.or_select_decoupled_fn.argtypes = [C.c_uint64, _u32p, _u32p, _u32p, _f64p, _u32p]
         L.or_canonicalize_fn.argtypes = [C.c_uint32, _u32p]
         L.or_should_skip_aggregation_fn.argtypes = [C.c_uint64, C.c_uint32, _u8p]
+        L.or_empty_collisions.argtypes = [C.c_int]
+        L.or_empty_collisions.restype = C.c_uint64
+        L.or_set_reference_empty_ids.argtypes = [C.c_int]
+        L.or_set_reference_empty_ids.restype = None
+        L.or_empty_base.argtypes = [C.c_void_p, C.c_int]
+        L.or_empty_base.restype = C.c_uint64
         _lib = L
     return _lib
 

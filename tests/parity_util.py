@@ -12,18 +12,26 @@ MAP_NAMES = {hl.MAP_F: "f", hl.MAP_G: "g", hl.MAP_S_CUR: "s_cur", hl.MAP_S_PRE: 
 SET_NAMES = {hl.SET_ACTIVE: "A", hl.SET_CHANGED: "B", hl.SET_AFFECTED: "K", hl.SET_REFINED: "R"}
 
 
-def oracle_mode(mode):
-    return O.DET if mode == hl.RunMode.DETERMINISTIC else O.PAR
+def oracle_mode(mode, refine_reference=False):
+    """The oracle mode a GPU configuration is compared with: DET, PAR (the
+    parallel mode with a consistent hierarchy) or THR (the reference's own
+    throughput rules, RunConfig.refine_reference)."""
+    if mode == hl.RunMode.DETERMINISTIC:
+        return O.DET
+    return O.THR if refine_reference else O.PAR
 
 
 def pair(mode, levels, gamma=1.0, **cfg):
     g = hl.HitLeiden(hl.RunConfig(resolution=gamma, mode=mode, levels=levels, **cfg))
-    o = O.OracleState(levels=levels, gamma=gamma, mode=oracle_mode(mode))
+    o = O.OracleState(levels=levels, gamma=gamma, mode=oracle_mode(mode, cfg.get("refine_reference", False)))
     return g, o
 
 
 def assert_same_state(g, o, levels, tag="", weights_exact=True):
-    """Bit-exact comparison of every per-level map, set, CSR and delta."""
+    """Bit-exact comparison of every per-level map, set, CSR and delta (and, in
+    the parallel mode, the hierarchy invariants)."""
+    if g.config.mode == hl.RunMode.THROUGHPUT and not g.config.refine_reference:
+        assert_hierarchy_consistent(g, levels, tag)
     for p in range(levels):
         n = g.level_size(p)
         assert n == o.level_n(p), f"{tag} level {p} size {n} vs {o.level_n(p)}"
@@ -94,3 +102,33 @@ def assert_classes_connected(n, u, v, labels):
     ncomp, comp = connected_components(A, directed=False)
     nlab = len(np.unique(labels))
     assert ncomp == nlab, f"{ncomp} components for {nlab} classes"
+
+
+def hierarchy_violations(maps_f, maps_s):
+    """Per level p: (s^p-classes spanning several f^p-classes, vertices with
+    f^p(v) != f^{p+1}(s^p(v))); the second is None at the top level."""
+    out = []
+    P = len(maps_f)
+    for p in range(P):
+        f, s = np.asarray(maps_f[p], np.int64), np.asarray(maps_s[p], np.int64)
+        if len(s):
+            pairs = np.unique(np.stack([s, f], 1), axis=0)
+            straddle = len(pairs) - len(np.unique(s))
+        else:
+            straddle = 0
+        cons = None
+        if p + 1 < P:
+            fu = np.asarray(maps_f[p + 1], np.int64)
+            cons = int(np.sum(f != fu[s])) if len(s) else 0
+        out.append((straddle, cons))
+    return out
+
+
+def assert_hierarchy_consistent(g, levels, tag=""):
+    """Parallel mode: every s^p-class lies inside one f^p-class, and
+    f^p = f^{p+1} o s^p below the top level (hierarchy consistency)."""
+    fs = [g.partition(p, hl.MAP_F) for p in range(levels)]
+    ss = [g.partition(p, hl.MAP_S_CUR) for p in range(levels)]
+    for p, (straddle, cons) in enumerate(hierarchy_violations(fs, ss)):
+        assert straddle == 0, f"{tag} level {p}: {straddle} subcommunities span two communities"
+        assert not cons, f"{tag} level {p}: {cons} vertices with f^p != f^(p+1) o s^p"

@@ -441,3 +441,82 @@ def test_static_run_reaches_a_fixpoint(mode):
         for a, b in zip(h, g.hierarchy()):
             np.testing.assert_array_equal(a, b)
     g.close()
+
+
+# ---------------------------------------------------------------- the reference's throughput rules
+
+@pytest.mark.parametrize("levels", [1, 2, 4])
+def test_small_config_reference_throughput(levels):
+    """refine_reference: the throughput engine under the reference's own rules
+    (every snapshot refinement proposal applies, throughput.rs:278-287; the
+    component split over all same-subcommunity edges, hit_leiden.rs:1233-1240;
+    def_update over B^p / R^p, hit_leiden.rs:1733-1793) against oracle THR,
+    bit-exact on the small configuration."""
+    w = gen.small_config()
+    say(f"small config reference throughput levels={levels}")
+    g, o = pair(hl.RunMode.THROUGHPUT, levels, refine_reference=True)
+    g.load(hl.GraphInput(w.n, w.u, w.v))
+    o.load(w.n, w.u, w.v)
+    assert_same_state(g, o, levels, "thr load")
+    for i, (u, v, a) in enumerate(w.batches):
+        g.hit_leiden(hl.GraphInput(w.n, u, v, a))
+        o.apply(u, v, a)
+        assert_same_state(g, o, levels, f"thr batch {i}")
+        assert abs(g.modularity(hl.PART_COMMUNITY) - o.modularity(o.hierarchy()[-1])) < 1e-9
+    assert np.array_equal(g.hierarchy(), o.hierarchy())
+    g.close()
+
+
+def test_empty_ids_above_written_labels():
+    """Caller-written labels [7,7,7,8,8,8] on two triangles (n = 6): at gamma = 4
+    every vertex leaves for an empty community; the ids start above the labels
+    in use (9 + v), where the reference's n + v would put vertex 2 into the other
+    triangle's community 8."""
+    say("empty ids above written labels")
+    u = np.array([0, 1, 0, 3, 4, 3], np.uint32)
+    v = np.array([1, 2, 2, 4, 5, 5], np.uint32)
+    g = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, resolution=4.0))
+    g.load_graph(hl.GraphInput(6, u, v))
+    g.write_partition(0, hl.MAP_F, np.array([7, 7, 7, 8, 8, 8], np.uint32))
+    g.hit_leiden(hl.GraphInput(6, np.zeros(0, np.uint32), np.zeros(0, np.uint32)))
+    assert g.partition(0, hl.MAP_F).tolist() == [9, 10, 11, 12, 13, 14]
+    o = O.OracleState(levels=1, gamma=4.0, mode=O.PAR)
+    o.load_graph(6, u, v)
+    o.write_map(0, O.MAP_F, np.array([7, 7, 7, 8, 8, 8], np.uint32))
+    o.apply(np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    assert np.array_equal(g.partition(0, hl.MAP_F), o.read_map(0, O.MAP_F))
+    g.close()
+
+
+# ---------------------------------------------------------------- first-seen relabel (X)
+
+def test_device_relabel_known_answer():
+    """canonicalize_community_ids_in_place KAT (hit_leiden.rs:2448-2461):
+    [5,5,3,3,7] -> [0,0,1,1,2], through hl_read_hierarchy(canonical=1)."""
+    g = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=1))
+    g.load_graph(hl.GraphInput(5, np.zeros(0, np.uint32), np.zeros(0, np.uint32)))
+    g.write_partition(0, hl.MAP_F, np.array([5, 5, 3, 3, 7], np.uint32))
+    assert g.hierarchy(canonical=True)[0].tolist() == [0, 0, 1, 1, 2]
+    assert g.hierarchy(canonical=False)[0].tolist() == [5, 5, 3, 3, 7]
+    g.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_device_relabel_matches_oracle(mode):
+    """Every hierarchy row of the small configuration, relabelled first-seen on
+    the device, equals the oracle's canonicalize of the oracle's row."""
+    w = gen.small_config()
+    P = 4
+    g, o = pair(mode, P)
+    g.load(hl.GraphInput(w.n, w.u, w.v))
+    o.load(w.n, w.u, w.v)
+    for i, (u, v, a) in enumerate(w.batches[:2]):
+        g.hit_leiden(hl.GraphInput(w.n, u, v, a))
+        o.apply(u, v, a)
+    Hc = g.hierarchy(canonical=True)
+    Ho = o.hierarchy()
+    for p in range(P):
+        want = O.canonicalize(Ho[p])
+        assert np.array_equal(Hc[p], want), f"row {p}"
+        assert Hc[p].max() + 1 == len(np.unique(Ho[p]))
+    g.close()
