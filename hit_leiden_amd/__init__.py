@@ -139,6 +139,8 @@ def lib():
             "hl_read_hierarchy": ([vp, vp, C.c_uint64, C.c_int32], C.c_int),
             "hl_modularity": ([vp, C.c_int32, f64p], C.c_int),
             "hl_modularity_labels": ([vp, vp, C.c_uint64, f64p], C.c_int),
+            "hl_run_static": ([vp, C.c_uint32, C.c_uint64, vp, vp, vp, C.c_uint32, C.c_uint64, vp, vp, C.c_uint32,
+                               u32p, C.POINTER(C.c_int64)], C.c_int),
             "hl_last_error": ([vp], C.c_char_p),
             "hl_create_error": ([], C.c_char_p),
             "hl_profile_slots": ([], C.c_int32),
@@ -178,6 +180,7 @@ class RunConfig:
     """RunConfig (src/core/config.rs:25-51) restricted to this path."""
     resolution: float = 1.0
     mode: RunMode = RunMode.THROUGHPUT
+    max_iterations: int = 10  # config.rs:31,46: the static path's level cap (run(), multilevel_leiden)
     levels: int = 1
     max_rounds: int = 20
     device: int = 0
@@ -311,6 +314,27 @@ class HitLeiden:
                 break
             prev, q = cur, q_new
         return passes, q
+
+    def static_leiden(self, graph: GraphInput, max_levels=10, seed=0):
+        """run() / multilevel_leiden (hit_leiden.rs:30-118, 167-323) on the
+        device: movement with the shuffled FIFO queue, Leiden refinement,
+        aggregation of the input edges, up to max_levels levels, then a final
+        polish on the input graph.  The queue shuffle draws from one
+        StdRng::seed_from_u64(seed) for the whole run (the reference draws an
+        unseeded thread_rng).  The handle's loaded state is untouched.  Returns
+        (labels numbered first-seen, hierarchy rows, movement calls)."""
+        n = graph.node_count
+        rows_cap = int(max_levels) + 2
+        labels = np.zeros(max(n, 1), dtype=np.uint32)
+        hier = np.zeros(max(n, 1) * rows_cap, dtype=np.uint32)
+        rows = C.c_uint32()
+        calls = C.c_int64()
+        self._check(lib().hl_run_static(self._h, n, len(graph), _ptr(graph.src), _ptr(graph.dst),
+                                        _ptr(graph.weights), int(max_levels), int(seed), _ptr(labels), _ptr(hier),
+                                        rows_cap, C.byref(rows), C.byref(calls)))
+        r = min(rows.value, rows_cap)
+        h = hier[:r * n].reshape(r, n) if n else np.zeros((r, 0), np.uint32)
+        return labels[:n], h, int(calls.value)
 
     def load_graph(self, graph: GraphInput):
         self._check(lib().hl_load_graph(self._h, graph.node_count, len(graph), _ptr(graph.src), _ptr(graph.dst),

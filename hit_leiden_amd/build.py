@@ -51,8 +51,8 @@ def build_engine(verbose=False):
 
 def build_gen(verbose=False):
     tmp = GEN_SO + ".tmp"
-    _run_parallel([["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-shared", os.path.join(CSRC, "hl_gen.cpp"),
-                    "-o", tmp]])
+    _run_parallel([["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-shared", "-I", CSRC,
+                    os.path.join(CSRC, "hl_gen.cpp"), "-o", tmp]])
     os.replace(tmp, GEN_SO)
     if verbose:
         print("built", GEN_SO)

@@ -12,6 +12,7 @@
 #include "../../include/hitleiden.h"
 #include "hl_kernels.h"
 #include "hl_move.h"
+#include "hl_rng.h"
 
 namespace hl {
 namespace {
@@ -36,19 +37,6 @@ struct Level {
   DevBuf<double> tot;
   bool tot_valid = false;
   u64 lab_bound = 0;  // every f label < lab_bound
-  // Throughput empty-community ids ebase + v.  ext bounds every f label that
-  // did not come from this level's own range [ebase, ebase + ecap) (labels
-  // pulled down from the level above, fresh ids, labels in use when the range
-  // was placed); while ext <= ebase and n <= ecap the range belongs to the
-  // level's vertices alone: v's empty community is ebase + v, and a vertex
-  // added by growth starts there.  A level that never grows and takes no
-  // labels from above keeps ebase = n, the reference's n + v
-  // (parallel_frontier.rs:214).
-  u64 ext = 0;
-  u64 ebase = 0;
-  u64 ecap = 0;
-  bool ebase_set = false;
-  u64 label_space() const { return std::max<u64>(ext, ebase_set ? ebase + ecap : 0); }
   u64 s_bound = 0;    // every s_cur / s_pre label < s_bound
   u32 deg_bound = 0;  // every row length <= deg_bound
   // several graphs per engine: graph of each vertex id (the graph count for
@@ -96,7 +84,8 @@ struct Engine {
   RefineScratch rs;
   AggScratch ag;
   MoveScratch mvs;
-  DevBuf<double> tot, partials, sig, acc;
+  DevBuf<double> tot, partials, sig, acc, tot2;
+  DevBuf<u64> clab;  // rank compaction of labels: presence flags, then ranks
   DevBuf<u64> tot_len, cnt, offtmp, lvcnt;
   DevBuf<u32> active, cto, mx, lab, lab2, first;
   DevBuf<double> cgain;
@@ -105,6 +94,7 @@ struct Engine {
   DevBuf<u32> ks, vs, vidx, starts;
   DevBuf<u8> heads;
   std::vector<std::unique_ptr<DevBuf<u8>>> Bp, Rp;
+  StaticScratch ss;
 
   ~Engine() {
     L.clear();
@@ -162,18 +152,13 @@ struct Engine {
       bs = std::max<u64>(bs, u64(std::max(h[2], h[3])) + 1);
     }
     if (p + 1 < P) bs = std::max<u64>(bs, L[p + 1]->n);
-    // throughput: within the empty range's capacity a new vertex starts in
-    // its own empty community
-    const bool own = par() && lv.ebase_set && n_new <= lv.ecap && lv.ext <= lv.ebase;
-    if (own) bf = lv.ebase + n_old;
     if (bf + (n_new - n_old) > 0xFFFFFFF0ull || bs + (n_new - n_old) > 0xFFFFFFF0ull)
       throw InputError("id space exhausted");
     reserve_level(lv, n_new);
     grow_init(n_old, n_new, u32(bf), u32(bg), u32(bs), lv.f.p, lv.g.p, lv.s_pre.p, lv.s_cur.p, lv.row_off.p,
               lv.row_len.p, lv.row_cap.p, lv.k.p, s);
     lv.n = n_new;
-    if (!own) lv.ext = std::max<u64>(lv.ext, bf + (n_new - n_old));
-    lv.lab_bound = std::max<u64>(lv.lab_bound, std::max<u64>(lv.label_space(), bf + (n_new - n_old)));
+    lv.lab_bound = std::max<u64>(lv.lab_bound, bf + (n_new - n_old));
     lv.s_bound = std::max<u64>(lv.s_bound, bs + (n_new - n_old));
     if (multi()) ensure_gid(lv, n_new);
     if (lv.sub_valid) {
@@ -211,6 +196,7 @@ struct Engine {
   bool par() const { return mode == HL_MODE_THROUGHPUT; }
   // throughput mode with a consistent hierarchy (hl_config.refine_reference == 0)
   bool connected() const { return par() && cfg.refine_reference == 0; }
+
 
   // Release scratch sized by the largest call so far (the initial load is a
   // whole-graph batch); it grows back to batch size on demand.
@@ -310,20 +296,21 @@ struct Engine {
     return std::max<u64>(std::max<u64>(2ull * lv.n, mf), 1);
   }
 
-  // Place the level's empty-community range before its throughput rounds.  A
-  // new place is needed when labels from outside the range may lie in it
-  // (ext > ebase) or the level outgrew it; it goes above every label in use
-  // (read from the device) and above ext, with room for the level to double.
-  void set_empty_base(Level& lv) {
-    if (!lv.ebase_set || lv.ext > lv.ebase || lv.n > lv.ecap) {
-      const u64 mx = lv.n ? u64(dmax(lv.f.p, lv.n)) + 1 : 0;
-      lv.ext = std::max<u64>(lv.ext, mx);
-      lv.ecap = lv.ebase_set ? 2ull * lv.n : lv.n;
-      lv.ebase = std::max<u64>(lv.n, lv.ext);
-      lv.ebase_set = true;
-    }
-    if (lv.ebase + lv.ecap > 0xFFFFFFF0ull) throw InputError("id space exhausted");
-    lv.lab_bound = std::max<u64>(lv.lab_bound, lv.label_space());
+  // Rank-compact the level's community labels before its throughput rounds
+  // (order preserved, so no tie decision changes): afterwards every label is
+  // below the number of communities <= n, and the reference's empty ids n + v
+  // (parallel_frontier.rs:214) never name a populated community.  Without it,
+  // labels that came from elsewhere (fresh ids of grown vertices, labels
+  // pulled down from the level above, ids of earlier empty moves at a smaller
+  // n) can equal n + v.  The maintained totals move with their labels.
+  void compact_labels(Level& lv) {
+    const u32 n = lv.n;
+    if (n == 0) return;
+    const u64 L = std::max<u64>(lv.lab_bound, n);
+    if (lv.tot_valid) ensure_totals(lv);
+    rank_labels(lv.f.p, n, L, lv.tot_valid ? lv.tot.p : nullptr, lv.tot_valid ? lv.tot.cap : 0, clab, tot2, tmp, s);
+    if (lv.tot_valid) lv.tot.swap(tot2);
+    lv.lab_bound = n;
   }
 
   // upper bound of the row lengths (rows only shrink below it)
@@ -370,7 +357,8 @@ struct Engine {
       return 1;
     }
     // throughput: synchronous rounds, at most max_rounds (hit_leiden.rs:1077-1099)
-    set_empty_base(lv);
+    compact_labels(lv);
+    lv.lab_bound = std::max<u64>(lv.lab_bound, 2ull * n);  // empty-community ids n + v
     ensure_totals(lv);
     const int max_rounds = cfg.max_rounds > 0 ? cfg.max_rounds : 20;
     MoveCtx m{g, lv.f.p, lv.s_cur.p, lv.tot.p, lv.k.p, lv.A.p, lv.B.p, lv.K.p, cfg.gamma, lv.tot.cap};
@@ -388,7 +376,6 @@ struct Engine {
       m.part = part.p;
       m.gcnt = gcnt.p;
     }
-    m.ebase = u32(lv.ebase);
     MoveStats ms_st;
     const u32 maxd = max_degree(lv);
     u32 rounds = movement_throughput(m, d, k, maxd, max_rounds, mvs, th, tmp, &ms_st, s);
@@ -425,7 +412,10 @@ struct Engine {
       // graphs of the next level's vertices: the graph of any member
       Level& up = *L[p + 1];
       ensure_gid(up, sn);
-      scatter_gid(lv.R.p, lv.s_cur.p, lv.gid.p, n, up.gid.p, s);
+      // from every vertex, not only the refined ones: a vertex grown at this
+      // level keeps its fresh subcommunity id unrefined, and the next level's
+      // vertex of that id takes rows (and its graph's 2m) at once
+      scatter_gid(lv.s_cur.p, lv.gid.p, n, G, up.gid.p, s);
     }
     (void)st;
   }
@@ -446,7 +436,6 @@ struct Engine {
         double* tot = nullptr;
         if (!use_g) {
           lv.lab_bound = std::max(lv.lab_bound, up.lab_bound);  // f^p takes f^{p+1} labels
-          lv.ext = std::max(lv.ext, up.lab_bound);
           if (par() && lv.tot_valid) {
             ensure_totals(lv);
             tot = lv.tot.p;
@@ -574,6 +563,147 @@ struct Engine {
     return total_iters;
   }
 
+  // ---- the static path: run() / multilevel_leiden (hit_leiden.rs:30-118, 167-323) ----
+  // A level holding the graph of records d[0..k) on n vertices (rows as the
+  // incremental path builds them, degrees summed in row order, canonical 2m);
+  // returns 2m.
+  double static_level(Level& lv, u32 n, const Rec* d, u64 k) {
+    reserve_level(lv, std::max<u32>(n, 1));
+    grow_init(0, n, 0, 0, 0, lv.f.p, lv.g.p, lv.s_pre.p, lv.s_cur.p, lv.row_off.p, lv.row_len.p, lv.row_cap.p,
+              lv.k.p, s);
+    lv.n = n;
+    lv.pool_used = 0;
+    lv.deg_bound = 0;
+    fill_f64(lv.twice_m.p, 0.0, 1, s);
+    if (k) {
+      ApplyPlan plan;
+      apply_delta_plan(d, k, lv.mut(), as, tmp, plan, s);
+      if (plan.nseg) {
+        lv.col.reserve(plan.need + 1, s, false, 1.0);
+        lv.w.reserve(plan.need + 1, s, false, 1.0);
+        apply_delta_commit(lv.mut(), 0, as, plan, true, TotalsRef{}, s);
+        lv.pool_used = plan.need;
+        lv.deg_bound = u32(plan.max_row);
+      }
+    }
+    partials.reserve(twice_m_scratch(n), s);
+    twice_m_tree(lv.k.p, n, partials.p, lv.twice_m.p, s);
+    double t = 0.0;
+    d2h_async(&t, lv.twice_m.p, sizeof(double), s);
+    stream_sync(s);
+    return t;
+  }
+
+  // single_level_movement (:764-932) on lv from f: the active vertices of the
+  // records d[0..k) (all when k = 0) shuffled by rng, then the queue loop
+  void static_move(Level& lv, const Rec* d, u64 k, u32* f, StdRng& rng) {
+    const u32 n = lv.n;
+    if (n == 0) return;
+    static_active(d, k, f, n, lv.A.p, s);
+    ss.list.reserve(n, s);
+    const u64 c = select_flagged(lv.A.p, n, ss.list.p, tmp, ss.cnt, s);
+    std::vector<u32> q(c);
+    if (c) d2h_async(q.data(), ss.list.p, c * sizeof(u32), s);
+    ss.cdeg.reserve(n, s);
+    fill_f64(ss.cdeg.p, 0.0, n, s);
+    label_totals_atomic(f, lv.k.p, n, ss.cdeg.p, s);  // labels < n
+    stream_sync(s);
+    rng.shuffle(q.data(), c);
+    static_movement(lv.csr(), f, ss.cdeg.p, q.data(), u32(c), lv.A.p, lv.deg_bound, cfg.gamma, ss, s);
+    stream_sync(s);  // the queue's host memory is read before it goes
+  }
+
+  // refine_singleton_merge (:487-710) of comm on g0 into sub
+  void static_refine(Level& g0, double t, const u32* comm, u32* sub) {
+    const u32 n = g0.n;
+    if (n == 0) return;
+    HL_CHECK(hipMemcpyAsync(sub, comm, n * sizeof(u32), hipMemcpyDeviceToDevice, s));
+    if (t == 0.0) return;
+    fill_u8(g0.K.p, 1, n, s);
+    fill_u8(g0.R.p, 0, n, s);
+    bool ident = false;
+    const u64 sn = cc_split(g0.csr(), sub, g0.K.p, g0.R.p, n, &ident, SubRef{}, rs, tmp, s, nullptr, true);
+    rs.sizes.reserve(sn, s);
+    rs.dS.reserve(sn, s);
+    sub_recompute(sub, g0.k.p, n, rs.sizes.p, rs.dS.p, sn, s);
+    ss.dC.reserve(n, s);
+    fill_f64(ss.dC.p, 0.0, n, s);
+    label_totals_atomic(comm, g0.k.p, n, ss.dC.p, s);  // labels < n
+    static_refine_merge(g0.csr(), comm, sub, rs.sizes.p, rs.dS.p, sn, ss.dC.p, g0.R.p, g0.deg_bound, cfg.gamma, ss,
+                        tmp, s);
+  }
+
+  // first-seen relabelling of a[0..n) into out (device) and its label count
+  u32 static_first_seen(const u32* a, u32 n, u32* out) {
+    if (n == 0) return 0;
+    HL_CHECK(hipMemcpyAsync(out, a, n * sizeof(u32), hipMemcpyDeviceToDevice, s));
+    DevBuf<u8> fl;
+    DevBuf<u64> pos;
+    canonicalize(out, n, first, fl, pos, tmp, s);
+    return dmax(out, n) + 1;
+  }
+
+  // run(): the labels (first-seen) into labels, the hierarchy rows into H;
+  // returns the movement call count
+  int64_t run_static(u32 n, const Rec* ed_host, u64 m, u32 max_levels, u64 seed, u32* labels,
+                     std::vector<std::vector<u32>>& H) {
+    StdRng rng(seed);
+    Level g0, cg;
+    DevBuf<Rec> ed, crec;
+    DevBuf<u32> comm, sub, remap, cf, row;
+    ed.reserve(m + 1, s);
+    if (m) HL_CHECK(hipMemcpyAsync(ed.p, ed_host, m * sizeof(Rec), hipMemcpyHostToDevice, s));
+    const double t = static_level(g0, n, ed.p, m);
+    const u64 nn = std::max<u32>(n, 1);
+    comm.reserve(nn, s);
+    sub.reserve(nn, s);
+    remap.reserve(nn, s);
+    cf.reserve(nn, s);
+    row.reserve(nn, s);
+    iota_u32(comm.p, 0, n, s);
+    auto push_row = [&](const u32* a) {
+      std::vector<u32> h(n);
+      const u32 c = static_first_seen(a, n, row.p);
+      if (n) d2h_async(h.data(), row.p, n * sizeof(u32), s);
+      stream_sync(s);
+      H.push_back(std::move(h));
+      return c;
+    };
+    int64_t calls = 1;
+    static_move(g0, ed.p, m, comm.p, rng);
+    static_refine(g0, t, comm.p, sub.p);
+    u64 prev = push_row(comm.p);
+    for (u32 level = 1; level < max_levels; ++level) {
+      const u32 cn = static_first_seen(sub.p, n, remap.p);
+      if (cn <= 1) break;
+      crec.reserve(m + 1, s);
+      const u64 nr = static_aggregate(ed.p, m, remap.p, crec.p, ss, tmp, s);
+      if (nr == 0) break;
+      DevBuf<u32> rep;
+      rep.reserve(cn, s);
+      static_coarse_start(remap.p, n, cn, comm.p, rep.p, ss, s);
+      static_first_seen(rep.p, cn, cf.p);
+      static_level(cg, cn, crec.p, nr);
+      static_move(cg, crec.p, nr, cf.p, rng);
+      ++calls;
+      static_compose(cf.p, remap.p, n, comm.p, s);
+      const u64 now = push_row(comm.p);
+      static_refine(g0, t, comm.p, sub.p);
+      if (now >= prev) break;
+      prev = now;
+    }
+    // final polish on the input graph (:290-305)
+    static_move(g0, ed.p, m, comm.p, rng);
+    ++calls;
+    std::vector<u32> fin(n);
+    static_first_seen(comm.p, n, row.p);
+    if (n) d2h_async(fin.data(), row.p, n * sizeof(u32), s);
+    stream_sync(s);
+    if (H.empty() || H.back() != fin) H.push_back(fin);
+    std::copy(fin.begin(), fin.end(), labels);
+    return calls;
+  }
+
   void reset(u32 n, u64 m_hint) {
     L.clear();
     for (int p = 0; p < P; ++p) L.emplace_back(new Level());
@@ -679,10 +809,10 @@ void forget_staged(hl_graph* g) {
 }
 
 template <class F>
-hl_status guard(const hl_graph* gc, F&& fn) {
+hl_status guard(const hl_graph* gc, F&& fn, bool profile = true) {
   hl_graph* g = const_cast<hl_graph*>(gc);
   try {
-    ProfBind pb(g ? &g->e : nullptr);
+    ProfBind pb(g && profile ? &g->e : nullptr);
     fn();
     if (g) g->e.err.clear();
     return HL_OK;
@@ -716,7 +846,7 @@ void check_labels(const uint32_t* a, uint64_t n, const char* what) {
   for (uint64_t i = 0; i < n; ++i)
     if (a[i] > kMaxLabel) throw hl::InputError(std::string(what) + " labels must be below 2^31");
 }
-// throughput empty-community ids ebase + v need n < 2^31
+// throughput empty-community ids n + v need n < 2^31
 void check_vertex_count(const hl::Engine& e, uint64_t n) {
   if (e.mode == HL_MODE_THROUGHPUT && n > kMaxLabel)
     throw hl::InputError("throughput mode holds fewer than 2^31 vertices");
@@ -969,8 +1099,6 @@ hl_status hl_write_partition(hl_graph* g, int32_t level, int32_t which, const ui
       // caller-written labels: rebuild the maintained totals on next use
       lv.tot_valid = false;
       lv.lab_bound = n ? std::max<uint64_t>(lv.n, uint64_t(e.dmax(lv.f.p, n)) + 1) : 0;
-      lv.ext = lv.lab_bound;
-      lv.ebase_set = false;
     } else if ((which == HL_MAP_S_CUR || which == HL_MAP_S_PRE) && n) {
       lv.s_bound = std::max<uint64_t>(lv.s_bound, uint64_t(e.dmax(e.map_of(lv, which).p, n)) + 1);
       if (which == HL_MAP_S_CUR) lv.sub_valid = false;
@@ -1022,8 +1150,6 @@ hl_status hl_restore_level(hl_graph* g, int32_t level, uint32_t n, uint64_t m, c
     lv.tot_valid = false;
     lv.sub_valid = false;
     lv.lab_bound = n ? std::max<uint64_t>(n, uint64_t(mx[0]) + 1) : 0;
-    lv.ext = lv.lab_bound;
-    lv.ebase_set = false;
     lv.s_bound = n ? std::max<uint64_t>(n, uint64_t(std::max(mx[2], mx[3])) + 1) : 0;
   });
 }
@@ -1336,6 +1462,31 @@ hl_status hl_modularity_labels(const hl_graph* g, const uint32_t* labels, uint64
     e.lab.reserve(n + 1, e.s);
     if (n) HL_CHECK(hipMemcpyAsync(e.lab.p, labels, n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));
     *q = hl::modularity(e.L[0]->csr(), e.lab.p, e.L[0]->deg_bound, e.sig, e.acc, e.mx, e.s);
+  });
+}
+
+hl_status hl_run_static(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                        const double* w, uint32_t max_levels, uint64_t seed, uint32_t* labels, uint32_t* hier,
+                        uint32_t hier_rows, uint32_t* rows, int64_t* calls) {
+  if (!g || (m && (!src || !dst)) || (n && !labels) || (hier_rows && !hier)) return HL_INVALID_INPUT;
+  return guard(g, [&] {
+    Engine& e = g->e;
+    HL_CHECK(hipSetDevice(e.cfg.device));
+    if (max_levels == 0) throw hl::InputError("max_iterations must be > 0");
+    if (n > kMaxLabel) throw hl::InputError("the static path holds fewer than 2^31 vertices");
+    std::vector<hl::Rec> ed(m);
+    for (uint64_t i = 0; i < m; ++i) {
+      if (src[i] >= n || dst[i] >= n) throw hl::InputError("edge endpoint exceeds node_count");
+      const double x = w ? w[i] : 1.0;
+      if (!std::isfinite(x)) throw hl::InputError("non-finite weight");
+      ed[i] = hl::Rec{src[i], dst[i], x};
+    }
+    std::vector<std::vector<uint32_t>> H;
+    const int64_t c = e.run_static(n, ed.data(), m, max_levels, seed, labels, H);
+    const uint32_t nr = std::min<uint32_t>(uint32_t(H.size()), hier_rows);
+    for (uint32_t r = 0; r < nr; ++r) std::copy(H[r].begin(), H[r].end(), hier + uint64_t(r) * n);
+    if (rows) *rows = uint32_t(H.size());
+    if (calls) *calls = c;
   });
 }
 

@@ -1,6 +1,9 @@
 // hl_gen.cpp -- seeded synthetic graphs and edge-delta streams for the five
 // workloads (LFR-style, planted partition, R-MAT, SBM) and their batches.
-// PRNG: splitmix64-seeded xoshiro256**.  Plain C++ (host only).
+// PRNG: splitmix64-seeded xoshiro256**.  Plain C++ (host only).  Also the
+// reference loaders' seeded shuffle (hl_rng.h: StdRng + SliceRandom::shuffle).
+#include "hl_rng.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -480,5 +483,23 @@ void hg_batch(void* h, uint32_t i, uint32_t* u, uint32_t* v, double* a) {
   std::copy(es.ba.begin() + s, es.ba.begin() + e, a);
 }
 void hg_free(void* h) { delete static_cast<EdgeSet*>(h); }
+
+// StdRng::seed_from_u64(seed) + SliceRandom::shuffle of [0, n): out[i] is the
+// index of the element that lands at position i (dynamic_graph.rs:85-87)
+void hg_shuffle_order(uint64_t seed, uint64_t n, uint64_t* out) {
+  for (uint64_t i = 0; i < n; ++i) out[i] = i;
+  hl::StdRng r(seed);
+  r.shuffle(out, n);
+}
+// the first count next_u32 outputs of StdRng::seed_from_u64(seed)
+void hg_stdrng_u32(uint64_t seed, uint64_t count, uint32_t* out) {
+  hl::StdRng r(seed);
+  for (uint64_t i = 0; i < count; ++i) out[i] = r.next_u32();
+}
+// one ChaCha block (key words little-endian, 64-bit counter and stream id)
+void hg_chacha_block(const uint32_t* key, int rounds, uint64_t counter, uint64_t stream, uint32_t* out) {
+  hl::StdRng r(key, rounds, counter, stream);
+  r.block(out);
+}
 
 }  // extern "C"

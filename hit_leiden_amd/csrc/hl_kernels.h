@@ -54,6 +54,9 @@ void exclusive_scan_u64_async(const u64* in, u64* out, u64 n, Temp& tmp, hipStre
 // stable sort of (key, value) pairs by key (keys_out/vals_out separate)
 void sort_pairs_u64(const u64* kin, u64* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
                     hipStream_t s);
+// stable radix sort of (key, f64 value) pairs
+void sort_pairs_u64_f64(const u64* kin, u64* kout, const double* vin, double* vout, u64 n, int end_bit, Temp& tmp,
+                        hipStream_t s);
 void sort_pairs_u32(const u32* kin, u32* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
                     hipStream_t s);
 
@@ -171,8 +174,9 @@ struct SubRef {
 // sets *identity when s is the identity map after it (hit_leiden.rs:1333-1339).
 // f != nullptr links two vertices only through edges inside their community
 // (f(u) = f(v)); nullptr links every same-subcommunity neighbour (hit_leiden.rs:1233-1240)
+// first_largest keeps the first largest component (the static path) instead of the last
 u64 cc_split(const Csr& g, u32* s, const u8* K, u8* R, u64 s_bound, bool* identity, const SubRef& sub,
-             RefineScratch& rs, Temp& tmp, hipStream_t st, const u32* f = nullptr);
+             RefineScratch& rs, Temp& tmp, hipStream_t st, const u32* f = nullptr, bool first_largest = false);
 // size[s(v)] += 1 for the vertices [n0, n1) a level grew by (their k is 0)
 void sub_add_vertices(const u32* s, u32 n0, u32 n1, u32* size, hipStream_t st);
 // from scratch: size and deg over every vertex (arrays zero-filled first, cap entries)
@@ -205,10 +209,46 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
 
 // ---- def_update (hit_leiden.rs:1733-1793) ----
 // f^p(v) = f^{p+1}(s^p(v)) for v in B; tot (may be null) is level p's maintained totals
+// f[v] <- the rank of f[v] among the labels in use (all below L), so every
+// label ends below their count; tot (tot_cap entries, or null) moves with its
+// labels into tot2 (reserved and zeroed here), which the caller swaps in.
+// flags holds the presence flags and ranks (2 (L + 1) entries).
+void rank_labels(u32* f, u32 n, u64 L, const double* tot, u64 tot_cap, DevBuf<u64>& flags, DevBuf<double>& tot2,
+                 Temp& tmp, hipStream_t s);
 // f^p(v) = f^q(s^p(v)) for v in B (every v when B is null), keeping tot current
 void def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, double* tot, const double* k,
               hipStream_t s);
 void def_push(const u8* Bp, u8* Bq, const u32* sq, u32 nq, u64 np, hipStream_t s);
+
+// ---- the static path: run() / multilevel_leiden (hit_leiden.rs:30-118, 167-323) ----
+struct StaticScratch {
+  DevBuf<u32> queue, gk, gused, list, order, ckey, starts, first;
+  DevBuf<double> gv, cval, wc, ws, vals, vals2, cdeg, dC;
+  DevBuf<u8> inq, heads;
+  DevBuf<u64> cnt, keys, keys2;
+  DevBuf<unsigned long long> moves;
+};
+// A[v] = 1 for both ends of the records with w > 0 across communities (every
+// vertex when k = 0): single_level_movement's active set (:776-787)
+void static_active(const Rec* d, u64 k, const u32* f, u32 n, u8* A, hipStream_t s);
+// single_level_movement's queue loop (:856-928) in one workgroup from the
+// shuffled active list queue_host[0..qlen) (active: its flags); f and the
+// community degrees cdeg (indexed by label, labels < n) are updated
+void static_movement(const Csr& g, u32* f, double* cdeg, const u32* queue_host, u32 qlen, const u8* active,
+                     u32 maxdeg, double gamma, StaticScratch& ss, hipStream_t s);
+// refine_singleton_merge's merges (:617-706) over the split-off vertices R, on
+// subcommunities sub (size and dS current, sspace their id space), community
+// degrees dC indexed by f
+void static_refine_merge(const Csr& g, const u32* f, u32* sub, u32* size, double* dS, u64 sspace, const double* dC,
+                         const u8* R, u32 maxdeg, double gamma, StaticScratch& ss, Temp& tmp, hipStream_t s);
+// aggregate_graph (:751-790): records of d[0..k) summed per (min, max) of the
+// remapped ends, in input order per key; out sorted by key; returns its count
+u64 static_aggregate(const Rec* d, u64 k, const u32* remap, Rec* out, StaticScratch& ss, Temp& tmp, hipStream_t s);
+// out[c] = comm of the first (minimum) vertex v with remap[v] = c
+void static_coarse_start(const u32* remap, u32 n, u32 cn, const u32* comm, u32* out, StaticScratch& ss,
+                         hipStream_t s);
+// comm[v] = cf[remap[v]]
+void static_compose(const u32* cf, const u32* remap, u32 n, u32* comm, hipStream_t s);
 
 // ---- modularity (hit_leiden.rs:121-161) ----
 // rows above kIntraHub entries are summed by workgroups over pieces of kIntraPiece
@@ -240,8 +280,8 @@ void part_from_counts(const u32* cnt, u32 G, u8* part, hipStream_t s);
 // R = all vertices of every taking-part graph whose s is the identity (hit_leiden.rs:1333-1339, per graph)
 void identity_graphs_to_R(const u32* s, const u32* gid, u32 n, u32 G, const u8* part, u8* nid, u8* R,
                           hipStream_t st);
-// gid_q[s(v)] = gid_p[v] for v in R
-void scatter_gid(const u8* R, const u32* s, const u32* gid_p, u32 n, u32* gid_q, hipStream_t st);
+// gid_q[s(v)] = gid_p[v] for every v of a known graph (gid_p[v] < G)
+void scatter_gid(const u32* s, const u32* gid_p, u32 n, u32 G, u32* gid_q, hipStream_t st);
 // compute_modularity (hit_leiden.rs:121-161) of each graph of a level-0 CSR with gid set
 void modularity_graphs(const Csr& g, const u32* lab, u32 G, double* out, DevBuf<double>& sig, DevBuf<u32>& cg,
                        DevBuf<double>& acc, DevBuf<u32>& mx, hipStream_t s);

@@ -123,6 +123,17 @@ __device__ __forceinline__ double remove_gain(double ws, double t, double gamma,
 }
 
 __device__ __forceinline__ u32 hash32(u32 x) { return x * 2654435761u; }
+// T[key] += x in an LDS open-addressing table of cap slots (linear probing;
+// the caller keeps the number of distinct keys below cap)
+__device__ __forceinline__ void lds_add_keyed(u32* K, double* V, u32 cap, u32 key, double x) {
+  u32 h = hash32(key) & (cap - 1);
+  while (true) {
+    const u32 old = atomicCAS(&K[h], kNone, key);
+    if (old == kNone || old == key) break;
+    h = (h + 1) & (cap - 1);
+  }
+  atomicAdd(&V[h], x);
+}
 
 __device__ __forceinline__ u32 lower_bound_u32(const u32* a, u32 n, u32 x) {
   u32 lo = 0, hi = n;
@@ -1037,18 +1048,24 @@ __global__ void k_zero_best(const u32* vcc, u64 m, const u32* s, u64* best) {
 }
 // keep the LAST largest component (max_by_key, hit_leiden.rs:1317-1322):
 // discovery order is ascending minimum vertex, so the key is (size, root)
-__global__ void k_best_comp(const u32* vcc, u64 m, const u32* parent, const u32* cnt, const u32* s, u64* best) {
+// the kept component of each subcommunity: the largest, ties to the larger
+// root (the last discovered, hit_leiden.rs:1317-1322), or with first_largest
+// to the smaller root (the static path's rule, :505-518); roots are the
+// components' minimum vertices
+__global__ void k_best_comp(const u32* vcc, u64 m, const u32* parent, const u32* cnt, const u32* s, u64* best,
+                            bool first_largest) {
   for (u64 i = gtid(); i < m; i += gstride()) {
     u32 v = vcc[i];
     if (parent[v] != v) continue;
-    unsigned long long key = (u64(cnt[v]) << 32) | v;
+    unsigned long long key = (u64(cnt[v]) << 32) | (first_largest ? ~v : v);
     atomicMax((unsigned long long*)&best[s[v]], key);
   }
 }
-__global__ void k_split_roots(const u32* vcc, u64 m, const u32* parent, const u32* s, const u64* best, u8* flag) {
+__global__ void k_split_roots(const u32* vcc, u64 m, const u32* parent, const u32* s, const u64* best, u8* flag,
+                              bool first_largest) {
   for (u64 i = gtid(); i < m; i += gstride()) {
     u32 v = vcc[i];
-    flag[i] = (parent[v] == v) && (u32(best[s[v]] & 0xffffffffu) != v);
+    flag[i] = (parent[v] == v) && (u32(best[s[v]] & 0xffffffffu) != (first_largest ? ~v : v));
   }
 }
 __global__ void k_root_keys(const u32* sel, u64 c, const u32* vcc, const u32* s, u64* keys, u32* vals) {
@@ -1062,12 +1079,12 @@ __global__ void k_newids(const u32* roots_sorted, u64 c, u32 base, u32* newid) {
   for (u64 r = gtid(); r < c; r += gstride()) newid[roots_sorted[r]] = base + u32(r);
 }
 __global__ void k_split_assign(const u32* vcc, u64 m, const u32* parent, const u64* best, const u32* newid, u32* s,
-                               u8* R, const double* k, SubRef sub) {
+                               u8* R, const double* k, SubRef sub, bool first_largest) {
   for (u64 i = gtid(); i < m; i += gstride()) {
     u32 v = vcc[i];
     u32 r = parent[v];
     u32 sv = s[v];
-    if (u32(best[sv] & 0xffffffffu) != r) {
+    if (u32(best[sv] & 0xffffffffu) != (first_largest ? ~r : r)) {
       const u32 to = newid[r];
       s[v] = to;
       R[v] = 1;
@@ -1115,39 +1132,87 @@ __global__ void k_parent_idx(const u32* s, u32 n, const u8* is_cand, u32* pidx) 
 // w(S, parent(S)) over all members (hit_leiden.rs:1617-1629), self-loops twice
 // Each wavefront takes 64 vertices and walks the rows of those in candidate
 // subcommunities together (rows can be long at the upper levels).
+// w(S, C) for the candidate subcommunities S (hit_leiden.rs:1617-1629: the
+// weight from S's members to their parent community C, a self-loop twice).
+// Each wavefront walks the rows of 64 consecutive vertices as one flat
+// sequence (the row of entry t by a binary search over the lanes' prefix) and
+// adds into an LDS table per subcommunity, flushed with one device atomic per
+// distinct S and workgroup pass: the members of one big subcommunity would
+// otherwise serialise on one address.  Rows above kWsWide entries are walked
+// by the whole wavefront, one after another.
+constexpr u32 kWsTable = 2048, kWsWide = 1024;
 __global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u32* s, const u8* is_cand,
                                              const u32* pidx, double* wS) {
+  __shared__ u32 tk[kWsTable];
+  __shared__ double tv[kWsTable];
   const u32 lane = threadIdx.x & 63;
-  const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
-  for (u64 base = wave * 64; base < n; base += nwaves * 64) {
-    const u64 v0 = base + lane;
-    u32 sv = 0;
-    bool q = false;
-    if (v0 < n) {
-      sv = s[v0];
-      q = is_cand[sv];
+  for (u64 base0 = u64(blockIdx.x) * 256; base0 < n; base0 += u64(gridDim.x) * 256) {
+    for (u32 i = threadIdx.x; i < kWsTable; i += 256) {
+      tk[i] = kNone;
+      tv[i] = 0.0;
     }
-    unsigned long long mask = __ballot(q);
+    __syncthreads();
+    const u64 base = base0 + (threadIdx.x & ~63u);
+    const u64 v = base + lane;
+    u32 sv = kNone, C = 0, deg = 0;
+    u64 off = 0;
+    if (v < n) {
+      const u32 x = s[v];
+      if (is_cand[x]) {
+        sv = x;
+        C = f[pidx[x]];
+        off = g.row_off[v];
+        deg = g.row_len[v];
+      }
+    }
+    const bool wide = deg > kWsWide;
+    const u32 d0 = wide ? 0u : deg;
+    u32 x = d0;
+    for (u32 o = 1; o < 64; o <<= 1) {
+      const u32 y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    const u32 total = __shfl(x, 63, 64), pre = x - d0;
+    for (u32 t0 = 0; t0 < total; t0 += 64) {
+      const u32 t = t0 + lane;
+      u32 r = 0;
+      for (u32 step = 32; step > 0; step >>= 1) {
+        const u32 pc = __shfl(pre, r + step, 64);
+        if (pc <= t) r += step;
+      }
+      const u64 ro = __shfl((unsigned long long)off, r, 64);
+      const u32 rp = __shfl(pre, r, 64), rC = __shfl(C, r, 64), rs = __shfl(sv, r, 64);
+      if (t < total) {
+        const u64 e = ro + (t - rp);
+        const u32 u = g.col[e];
+        if (f[u] == rC) {
+          const double w = g.w[e];
+          lds_add_keyed(tk, tv, kWsTable, rs, (u == base + r) ? 2.0 * w : w);
+        }
+      }
+    }
+    unsigned long long mask = __ballot(wide);
     while (mask) {
       const u32 i = u32(__ffsll((long long)mask) - 1);
       mask &= mask - 1;
-      const u32 v = u32(base + i);
-      const u32 svv = __shfl(sv, i, 64);
-      const u32 C = f[pidx[svv]];
-      const u64 off = g.row_off[v];
-      const u32 deg = g.row_len[v];
+      const u64 bv = base + i;
+      const u64 boff = __shfl((unsigned long long)off, i, 64);
+      const u32 bdeg = __shfl(deg, i, 64), bC = __shfl(C, i, 64), bs = __shfl(sv, i, 64);
       double acc = 0.0;
-      for (u32 j = lane; j < deg; j += 64) {
-        const u32 u = g.col[off + j];
-        if (f[u] == C) {
-          const double w = g.w[off + j];
-          acc += w;
-          if (u == v) acc += w;
+      for (u32 j = lane; j < bdeg; j += 64) {
+        const u32 u = g.col[boff + j];
+        if (f[u] == bC) {
+          const double w = g.w[boff + j];
+          acc += (u == bv) ? 2.0 * w : w;
         }
       }
       for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-      if (lane == 0 && acc != 0.0) atomicAdd(&wS[svv], acc);
+      if (lane == 0 && acc != 0.0) lds_add_keyed(tk, tv, kWsTable, bs, acc);
     }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < kWsTable; i += 256)
+      if (tk[i] != kNone && tv[i] != 0.0) atomicAdd(&wS[tk[i]], tv[i]);
+    __syncthreads();
   }
 }
 __global__ void k_blocked(u64 sn, const u8* is_cand, const double* dS, const double* wS, const u32* pidx,
@@ -1642,10 +1707,14 @@ __global__ void k_fill_R_graphs(const u32* gid, u32 n, const u8* part, const u8*
     if (part[g] && !nid[g]) R[v] = 1;
   }
 }
-// the graph of each next-level vertex: the graph of a member
-__global__ void k_scatter_gid(const u8* R, const u32* s, const u32* gid_p, u32 n, u32* gid_q) {
-  for (u64 v = gtid(); v < n; v += gstride())
-    if (R[v]) gid_q[s[v]] = gid_p[v];
+// the graph of each next-level vertex: the graph of a member (every member of
+// a subcommunity lies in one graph, so concurrent writes agree); vertices no
+// member ever reached (graph G) write nothing
+__global__ void k_scatter_gid(const u32* s, const u32* gid_p, u32 n, u32 G, u32* gid_q) {
+  for (u64 v = gtid(); v < n; v += gstride()) {
+    const u32 g = gid_p[v];
+    if (g < G) gid_q[s[v]] = g;
+  }
 }
 // Per-graph modularity terms: sig[c] and the graph of community c, then the
 // intra-community weight and the sum of sig^2 per graph.  Each thread walks a
@@ -1699,6 +1768,405 @@ __global__ void k_sq_graphs(const double* sig, const u32* cg, u64 L, u64 chunk, 
     acc += x * x;
   }
   if (acc != 0.0) atomicAdd(&sq[cur], acc);
+}
+
+
+// ------------------------------------------------------------------ static path (run / multilevel_leiden)
+constexpr u32 SLCAP = 4096;  // LDS hash slots of the static sweeps
+
+// Ordered append of the flagged lanes of a 256-thread block: returns each
+// flagged thread's position among the flagged ones (thread order) and the
+// block's count in *total (all threads must call, in step).
+__device__ __forceinline__ u32 block_rank(bool want, u32* total) {
+  __shared__ u32 wc[4];
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(want);
+  if (lane == 0) wc[wv] = u32(__popcll(m));
+  __syncthreads();
+  u32 before = 0, all = 0;
+  for (u32 i = 0; i < 4; ++i) {
+    if (i < wv) before += wc[i];
+    all += wc[i];
+  }
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  *total = all;
+  const u32 r = before + u32(__popcll(m & below));
+  __syncthreads();
+  return r;
+}
+
+struct StaticMoveArgs {
+  Csr g;
+  u32* f;
+  double* cdeg;  // community degrees, indexed by label (labels < n)
+  u32* queue;    // ring of n slots; queue[0..qlen) holds the shuffled active vertices
+  u32 qlen;
+  u8* inq;
+  double gamma;
+  u32* gk;  // HBM table for rows above SLCAP / 2 entries
+  double* gv;
+  u32* gused;
+  u32 gcap;
+  unsigned long long* moves;
+};
+
+// single_level_movement's queue loop (hit_leiden.rs:856-928) in one
+// workgroup: pop the front, w(v, C) summed in row order without self-loops,
+// argmax over neighbour communities in ascending id with strict '>' (no empty
+// community), move when positive, then append the neighbours outside the new
+// community that are not queued, in row order.
+__global__ __launch_bounds__(256) void k_static_move(StaticMoveArgs a) {
+  __shared__ u32 lk[SLCAP];
+  __shared__ double lv[SLCAP];
+  __shared__ u32 lused[SLCAP];
+  __shared__ u32 cl[256];
+  __shared__ double cw[256];
+  __shared__ u64 s_head, s_tail;
+  __shared__ u32 s_nused, s_best, s_v;
+  __shared__ double s_wcur;
+  const u32 tid = threadIdx.x;
+  const Csr& g = a.g;
+  const u32 n = g.n;
+  const double tt = *g.twice_m;
+  for (u32 q = tid; q < SLCAP; q += 256) {
+    lk[q] = kNone;
+    lv[q] = 0.0;
+  }
+  if (tid == 0) {
+    s_head = 0;
+    s_tail = a.qlen;
+  }
+  __syncthreads();
+  unsigned long long moves = 0;
+  while (true) {
+    if (tid == 0) {
+      s_v = kNone;
+      if (s_head != s_tail) {
+        s_v = a.queue[s_head % n];
+        ++s_head;
+        a.inq[s_v] = 0;
+      }
+      s_nused = 0;
+    }
+    __syncthreads();
+    const u32 v = s_v;
+    if (v == kNone) break;
+    const u32 cur = a.f[v];
+    const double kv = g.k[v];
+    const u64 off = g.row_off[v];
+    const u32 deg = g.row_len[v];
+    const bool in_lds = deg <= SLCAP / 2;
+    u32* HK = in_lds ? lk : a.gk;
+    double* HV = in_lds ? lv : a.gv;
+    u32* used = in_lds ? lused : a.gused;
+    const u32 cap = in_lds ? SLCAP : a.gcap;
+    for (u32 base = 0; base < deg; base += 256) {
+      const u32 j = base + tid;
+      if (j < deg) {
+        const u32 u = g.col[off + j];
+        cl[tid] = (u == v) ? kNone : a.f[u];
+        cw[tid] = g.w[off + j];
+      }
+      __syncthreads();
+      if (tid == 0) {
+        const u32 m = min(256u, deg - base);
+        u32 nu = s_nused;
+        for (u32 q = 0; q < m; ++q) {
+          if (cl[q] == kNone) continue;
+          const u32 h = seq_insert(HK, cap, cl[q], used, nu);
+          HV[h] += cw[q];
+        }
+        s_nused = nu;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) s_wcur = table_lookup(HK, HV, cap, cur);
+    __syncthreads();
+    const double wcur = s_wcur, sc = a.cdeg[cur];
+    const u32 nused = s_nused;
+    double bg = 0.0;
+    u32 best = kNone;
+    for (u32 q = tid; q < nused; q += 256) {
+      const u32 h = used[q], key = HK[h];
+      if (key == cur) continue;
+      const double gn = move_gain(HV[h], wcur, tt, a.gamma, kv, sc, a.cdeg[key]);
+      if (gn > 0.0) better(bg, best, gn, key);
+    }
+    block_argmax(bg, best);
+    if (tid == 0) {
+      s_best = kNone;
+      if (bg > 0.0 && best != kNone) {
+        a.f[v] = best;
+        a.cdeg[cur] -= kv;
+        a.cdeg[best] += kv;
+        s_best = best;
+        ++moves;
+      }
+    }
+    for (u32 q = tid; q < nused; q += 256) {
+      const u32 h = used[q];
+      HK[h] = kNone;
+      HV[h] = 0.0;
+    }
+    __threadfence_block();
+    __syncthreads();
+    const u32 bst = s_best;
+    if (bst != kNone) {
+      for (u32 base = 0; base < deg; base += 256) {
+        const u32 j = base + tid;
+        u32 u = 0;
+        bool want = false;
+        if (j < deg) {
+          u = g.col[off + j];
+          want = u != v && a.f[u] != bst && !a.inq[u];
+        }
+        u32 cnt = 0;
+        const u32 r = block_rank(want, &cnt);
+        if (want) {
+          a.queue[(s_tail + r) % n] = u;
+          a.inq[u] = 1;
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (tid == 0) s_tail += cnt;
+        __syncthreads();
+      }
+    }
+  }
+  if (tid == 0) atomicAdd(a.moves, moves);
+}
+
+// wc[i] = weight of i's row entries inside its community (a self-loop twice),
+// added to ws[sub[i]]: w(S, C) of subcommunity_delta_q_to_empty (:729-739)
+__global__ void k_static_ws(Csr g, const u32* f, const u32* sub, double* wc, double* ws) {
+  for (u64 v = gtid(); v < g.n; v += gstride()) {
+    const u64 off = g.row_off[v];
+    const u32 deg = g.row_len[v], C = f[v];
+    double acc = 0.0;
+    for (u32 j = 0; j < deg; ++j) {
+      const u32 u = g.col[off + j];
+      if (f[u] == C) acc += (u == v) ? 2.0 * g.w[off + j] : g.w[off + j];
+    }
+    wc[v] = acc;
+    if (acc != 0.0) atomicAdd(&ws[sub[v]], acc);
+  }
+}
+
+struct StaticRefineArgs {
+  Csr g;
+  const u32* f;
+  u32* sub;
+  u32* size;
+  double* dS;   // subcommunity degrees
+  double* ws;   // w(S, C) per subcommunity
+  const double* wc;
+  const double* dC;  // community degrees
+  const u32* order;  // the split-off vertices, degree ascending then id
+  u32 nord;
+  double gamma;
+  u32* gk;
+  double* gv;
+  u32* gused;
+  u32 gcap;
+  u32* ckey;  // candidate lists longer than the LDS ones
+  double* cval;
+  u32 ccap;
+};
+
+// bitonic sort of (key, val) ascending by key over N = 2^k slots, in one block
+__device__ void block_sort_pairs(u32* K, double* V, u32 N) {
+  for (u32 kk = 2; kk <= N; kk <<= 1) {
+    for (u32 j = kk >> 1; j > 0; j >>= 1) {
+      for (u32 i = threadIdx.x; i < N; i += blockDim.x) {
+        const u32 x = i ^ j;
+        if (x > i) {
+          const bool up = (i & kk) == 0;
+          if ((K[i] > K[x]) == up) {
+            const u32 tk = K[i];
+            K[i] = K[x];
+            K[x] = tk;
+            const double tv = V[i];
+            V[i] = V[x];
+            V[x] = tv;
+          }
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+    }
+  }
+}
+
+// refine_singleton_merge's merge loop (hit_leiden.rs:617-706) in one
+// workgroup: for each split-off vertex still a singleton, the weights to the
+// subcommunities of its same-community neighbours (row order), candidates in
+// ascending id kept by the T-filter (removal gain <= 1e-12, a zero-degree S
+// kept), the argmax with the EPS rules (gain, then weight, then id), a merge
+// when the best gain exceeds 1e-12.
+constexpr u32 kStaticCand = 2048;
+__global__ __launch_bounds__(256) void k_static_refine(StaticRefineArgs a) {
+  __shared__ u32 lk[SLCAP];
+  __shared__ double lv[SLCAP];
+  __shared__ u32 lused[SLCAP];
+  __shared__ u32 cl[256];
+  __shared__ double cw[256];
+  __shared__ u32 sk[kStaticCand];
+  __shared__ double sv[kStaticCand];
+  __shared__ u32 s_nused, s_go, s_nc;
+  const u32 tid = threadIdx.x;
+  const Csr& g = a.g;
+  const double tt = *g.twice_m;
+  const double EPS = 1e-12;
+  for (u32 q = tid; q < SLCAP; q += 256) {
+    lk[q] = kNone;
+    lv[q] = 0.0;
+  }
+  __syncthreads();
+  for (u32 idx = 0; idx < a.nord; ++idx) {
+    const u32 v = a.order[idx];
+    if (tid == 0) {
+      s_go = a.size[a.sub[v]] == 1;
+      s_nused = 0;
+    }
+    __syncthreads();
+    const bool go = s_go;
+    __syncthreads();  // every thread has read s_go before thread 0 writes the next one
+    if (!go) continue;
+    const u32 cur = a.sub[v], C = a.f[v];
+    const double kv = g.k[v];
+    const u64 off = g.row_off[v];
+    const u32 deg = g.row_len[v];
+    const bool in_lds = deg <= SLCAP / 2;
+    u32* HK = in_lds ? lk : a.gk;
+    double* HV = in_lds ? lv : a.gv;
+    u32* used = in_lds ? lused : a.gused;
+    const u32 cap = in_lds ? SLCAP : a.gcap;
+    for (u32 base = 0; base < deg; base += 256) {
+      const u32 j = base + tid;
+      if (j < deg) {
+        const u32 u = g.col[off + j];
+        cl[tid] = (u == v || a.f[u] != C) ? kNone : a.sub[u];
+        cw[tid] = g.w[off + j];
+      }
+      __syncthreads();
+      if (tid == 0) {
+        const u32 m = min(256u, deg - base);
+        u32 nu = s_nused;
+        for (u32 q = 0; q < m; ++q) {
+          if (cl[q] == kNone) continue;
+          const u32 h = seq_insert(HK, cap, cl[q], used, nu);
+          HV[h] += cw[q];
+        }
+        s_nused = nu;
+      }
+      __syncthreads();
+    }
+    const u32 nused = s_nused;
+    // the candidates (their id and weight), sorted ascending
+    const bool small = nused <= kStaticCand;
+    u32* CK = small ? sk : a.ckey;
+    double* CV = small ? sv : a.cval;
+    u32 N = 1;
+    while (N < nused) N <<= 1;
+    for (u32 q = tid; q < N; q += 256) {
+      if (q < nused) {
+        const u32 h = used[q];
+        CK[q] = HK[h];
+        CV[q] = HV[h];
+      } else {
+        CK[q] = kNone;
+        CV[q] = 0.0;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (nused > 1) block_sort_pairs(CK, CV, N);
+    if (tid == 0) {
+      double wcur = 0.0;
+      for (u32 q = 0; q < nused; ++q)
+        if (CK[q] == cur) wcur = CV[q];
+      u32 best = cur;
+      double bg = 0.0, bw = -1.0;
+      const double dC = a.dC[C];
+      for (u32 q = 0; q < nused; ++q) {
+        const u32 S = CK[q];
+        if (S == cur) continue;
+        const double ds = a.dS[S];
+        const double dq = ds == 0.0 ? 0.0 : (0.0 - a.ws[S]) / tt + a.gamma * ds * (dC - ds) / (tt * tt);
+        if (!(dq <= EPS)) continue;
+        const double w = CV[q];
+        const double gain = (w - wcur) / tt + a.gamma * kv * (a.dS[cur] - kv - ds) / (tt * tt);
+        const bool bt = gain > bg + EPS ||
+                        (fabs(gain - bg) <= EPS && (w > bw + EPS || (fabs(w - bw) <= EPS && S < best)));
+        if (bt) {
+          best = S;
+          bg = gain;
+          bw = w;
+        }
+      }
+      if (bg > EPS) {
+        a.sub[v] = best;
+        a.size[cur] -= 1;
+        a.size[best] += 1;
+        a.dS[cur] -= kv;
+        a.dS[best] += kv;
+        a.ws[cur] -= a.wc[v];
+        a.ws[best] += a.wc[v];
+      }
+    }
+    for (u32 q = tid; q < nused; q += 256) {
+      const u32 h = used[q];
+      HK[h] = kNone;
+      HV[h] = 0.0;
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+// active vertices of single_level_movement (:776-787): both ends of a record
+// with a positive weight whose ends lie in different communities
+__global__ void k_static_active(const Rec* d, u64 k, const u32* f, u32 n, u8* A) {
+  for (u64 i = gtid(); i < k; i += gstride()) {
+    const Rec r = d[i];
+    if (r.w > 0.0 && r.u < n && r.v < n && f[r.u] != f[r.v]) {
+      A[r.u] = 1;
+      A[r.v] = 1;
+    }
+  }
+}
+// aggregate_graph's keys (:764-780): (min, max) of the remapped ends
+__global__ void k_static_keys(const Rec* d, u64 k, const u32* remap, u64* keys, double* vals) {
+  for (u64 i = gtid(); i < k; i += gstride()) {
+    const Rec r = d[i];
+    const u64 a = remap[r.u], b = remap[r.v];
+    keys[i] = (min(a, b) << 32) | max(a, b);
+    vals[i] = r.w;
+  }
+}
+// one record per key run of the sorted keys, its weights summed in order
+__global__ void k_static_reduce(const u64* keys, const double* vals, const u32* starts, u64 nruns, u64 k, Rec* out) {
+  for (u64 r = gtid(); r < nruns; r += gstride()) {
+    const u64 b = starts[r], e = r + 1 < nruns ? starts[r + 1] : k;
+    double w = 0.0;
+    for (u64 i = b; i < e; ++i) w += vals[i];
+    out[r] = Rec{u32(keys[b] >> 32), u32(keys[b]), w};
+  }
+}
+__global__ void k_heads_u64(const u64* keys, u64 n, u8* heads) {
+  for (u64 i = gtid(); i < n; i += gstride()) heads[i] = (i == 0 || keys[i] != keys[i - 1]);
+}
+// the first member of each coarse vertex (its minimum vertex)
+__global__ void k_first_member(const u32* remap, u32 n, u32* first) {
+  for (u64 v = gtid(); v < n; v += gstride()) atomicMin(&first[remap[v]], u32(v));
+}
+__global__ void k_rep_comm(const u32* first, u32 cn, const u32* comm, u32* out) {
+  for (u64 c = gtid(); c < cn; c += gstride()) out[c] = comm[first[c]];
+}
+__global__ void k_compose2(const u32* cf, const u32* remap, u32 n, u32* comm) {
+  for (u64 v = gtid(); v < n; v += gstride()) comm[v] = cf[remap[v]];
+}
+__global__ void k_deg_keys(const u32* list, u64 c, const double* k, u64* keys) {
+  for (u64 i = gtid(); i < c; i += gstride()) keys[i] = u64(__double_as_longlong(k[list[i]]));
 }
 
 }  // namespace
@@ -1844,6 +2312,15 @@ void sort_pairs_u64(const u64* kin, u64* kout, const u32* vin, u32* vout, u64 n,
   HL_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
   void* t = tmp.get(bytes, s);
   HL_PROF(P_SORT, 24 * n, s);
+  HL_CHECK(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
+}
+void sort_pairs_u64_f64(const u64* kin, u64* kout, const double* vin, double* vout, u64 n, int end_bit, Temp& tmp,
+                        hipStream_t s) {
+  if (n == 0) return;
+  size_t bytes = 0;
+  HL_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
+  void* t = tmp.get(bytes, s);
+  HL_PROF(P_SORT, 32 * n, s);
   HL_CHECK(rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
 }
 void sort_pairs_u32(const u32* kin, u32* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
@@ -2107,7 +2584,7 @@ void sub_recompute(const u32* sm, const double* k, u32 n, u32* size, double* deg
 }
 
 u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* identity, const SubRef& sub,
-             RefineScratch& rs, Temp& tmp, hipStream_t s, const u32* f) {
+             RefineScratch& rs, Temp& tmp, hipStream_t s, const u32* f, bool first_largest) {
   const u32 n = g.n;
   *identity = true;
   if (n == 0) return 1;
@@ -2154,9 +2631,10 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_zero_best, GRID(m), rs.vcc.p, m, sm, rs.best64.p);
   HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_best_comp, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p, sm, rs.best64.p);
+  hipLaunchKernelGGL(k_best_comp, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p, sm, rs.best64.p, first_largest);
   HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_split_roots, GRID(m), rs.vcc.p, m, rs.parent.p, sm, rs.best64.p, rs.keepflag.p);
+  hipLaunchKernelGGL(k_split_roots, GRID(m), rs.vcc.p, m, rs.parent.p, sm, rs.best64.p, rs.keepflag.p,
+                     first_largest);
   HL_LAUNCH_CHECK();
   DevBuf<u32>& sel = rs.sel;
   sel.reserve(m, s);
@@ -2169,7 +2647,8 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
     sort_pairs_u64(rs.rkeys.p, rs.rkeys_s.p, vals.p, rs.roots_s.p, c, 64, tmp, s);
     hipLaunchKernelGGL(k_newids, GRID(c), rs.roots_s.p, c, next_id, rs.newid.p);
     HL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_split_assign, GRID(m), rs.vcc.p, m, rs.parent.p, rs.best64.p, rs.newid.p, sm, R, g.k, sub);
+    hipLaunchKernelGGL(k_split_assign, GRID(m), rs.vcc.p, m, rs.parent.p, rs.best64.p, rs.newid.p, sm, R, g.k, sub,
+                       first_largest);
     HL_LAUNCH_CHECK();
     *identity = false;  // split ids exceed every vertex id of an identity map
   }
@@ -2406,6 +2885,40 @@ double modularity(const Csr& g, const u32* lab, u32 maxdeg, DevBuf<double>& sig,
   return (h[0] - h[1]) / t;
 }
 
+__global__ void k_mark_labels(const u32* f, u32 n, u64* flag) {
+  for (u64 v = gtid(); v < n; v += gstride()) flag[f[v]] = 1;
+}
+__global__ void k_move_totals(const u64* flag, const u64* rank, u64 L, const double* tot, double* tot2) {
+  for (u64 l = gtid(); l < L; l += gstride())
+    if (flag[l]) tot2[rank[l]] = tot[l];
+}
+__global__ void k_apply_ranks(u32* f, u32 n, const u64* rank) {
+  for (u64 v = gtid(); v < n; v += gstride()) f[v] = u32(rank[f[v]]);
+}
+
+void rank_labels(u32* f, u32 n, u64 L, const double* tot, u64 tot_cap, DevBuf<u64>& flags, DevBuf<double>& tot2,
+                 Temp& tmp, hipStream_t s) {
+  if (n == 0) return;
+  HL_PROF(P_RELABEL, 4ull * n + 24ull * L + 12ull * n, s);
+  flags.reserve(2 * (L + 1), s);
+  u64* flag = flags.p;
+  u64* rank = flags.p + (L + 1);
+  hipLaunchKernelGGL(k_fill_u64, GRID(L + 1), flag, u64(0), L + 1);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mark_labels, GRID(n), f, n, flag);
+  HL_LAUNCH_CHECK();
+  exclusive_scan_u64_async(flag, rank, L, tmp, s);
+  if (tot) {
+    tot2.reserve(tot_cap, s, false, 1.0);
+    hipLaunchKernelGGL(k_fill_f64, GRID(tot2.cap), tot2.p, 0.0, u64(tot2.cap));
+    HL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_move_totals, GRID(L), flag, rank, L, tot, tot2.p);
+    HL_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_apply_ranks, GRID(n), f, n, rank);
+  HL_LAUNCH_CHECK();
+}
+
 void gather_u32(u32* cur, const u32* map, u32 n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_gather_u32, GRID(n), cur, map, n);
   HL_LAUNCH_CHECK();
@@ -2461,8 +2974,8 @@ void identity_graphs_to_R(const u32* sm, const u32* gid, u32 n, u32 G, const u8*
   }
   HL_LAUNCH_CHECK();
 }
-void scatter_gid(const u8* R, const u32* sm, const u32* gid_p, u32 n, u32* gid_q, hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_scatter_gid, GRID(n), R, sm, gid_p, n, gid_q);
+void scatter_gid(const u32* sm, const u32* gid_p, u32 n, u32 G, u32* gid_q, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_scatter_gid, GRID(n), sm, gid_p, n, G, gid_q);
   HL_LAUNCH_CHECK();
 }
 void modularity_graphs(const Csr& g, const u32* lab, u32 G, double* out, DevBuf<double>& sig, DevBuf<u32>& cg,
@@ -2566,6 +3079,115 @@ void compare_partitions(const u32* a, const u32* b, u64 n, u32 la, u32 lb, doubl
   out[2] = double(p[1]) / double(n);
   const u64 uni = u64(szl[0]) + szl[1] - inter;
   out[3] = uni ? double(inter) / double(uni) : 0.0;
+}
+
+
+// ------------------------------------------------------------------ static path (host side)
+void static_active(const Rec* d, u64 k, const u32* f, u32 n, u8* A, hipStream_t s) {
+  fill_u8(A, 0, n, s);
+  if (k) hipLaunchKernelGGL(k_static_active, GRID(k), d, k, f, n, A);
+  else fill_u8(A, 1, n, s);
+  HL_LAUNCH_CHECK();
+}
+
+void static_movement(const Csr& g, u32* f, double* cdeg, const u32* queue_host, u32 qlen, const u8* active,
+                     u32 maxdeg, double gamma, StaticScratch& ss, hipStream_t s) {
+  const u32 n = g.n;
+  if (n == 0) return;
+  ss.queue.reserve(n + 1, s);
+  ss.inq.reserve(n + 1, s);
+  ss.moves.reserve(1, s);
+  copy_u8(ss.inq.p, active, n, s);
+  if (qlen) HL_CHECK(hipMemcpyAsync(ss.queue.p, queue_host, qlen * sizeof(u32), hipMemcpyHostToDevice, s));
+  u32 gcap = 1;
+  while (gcap < 2 * std::max<u32>(maxdeg, 1)) gcap <<= 1;
+  if (maxdeg > SLCAP / 2) {
+    ss.gk.reserve(gcap, s);
+    ss.gv.reserve(gcap, s);
+    ss.gused.reserve(gcap, s);
+    fill_u32(ss.gk.p, kNone, gcap, s);
+    fill_f64(ss.gv.p, 0.0, gcap, s);
+  }
+  StaticMoveArgs a{g, f, cdeg, ss.queue.p, qlen, ss.inq.p, gamma, ss.gk.p, ss.gv.p, ss.gused.p, gcap, ss.moves.p};
+  HL_PROF(P_DET_MOVE, 0, s);
+  hipLaunchKernelGGL(k_static_move, 1, 256, 0, s, a);
+  HL_LAUNCH_CHECK();
+}
+
+void static_refine_merge(const Csr& g, const u32* f, u32* sub, u32* size, double* dS, u64 sspace, const double* dC,
+                         const u8* R, u32 maxdeg, double gamma, StaticScratch& ss, Temp& tmp, hipStream_t s) {
+  const u32 n = g.n;
+  if (n == 0) return;
+  // w(S, C) per subcommunity from each vertex's weight inside its community
+  ss.wc.reserve(n, s);
+  ss.ws.reserve(sspace, s);
+  fill_f64(ss.ws.p, 0.0, sspace, s);
+  hipLaunchKernelGGL(k_static_ws, GRID(n), g, f, sub, ss.wc.p, ss.ws.p);
+  HL_LAUNCH_CHECK();
+  // the split-off vertices by degree ascending, then id (a stable sort of the id-ordered list)
+  ss.list.reserve(n, s);
+  const u64 c = select_flagged(R, n, ss.list.p, tmp, ss.cnt, s);
+  if (c == 0) return;
+  ss.keys.reserve(c, s);
+  ss.keys2.reserve(c, s);
+  ss.order.reserve(c, s);
+  hipLaunchKernelGGL(k_deg_keys, GRID(c), ss.list.p, c, g.k, ss.keys.p);
+  HL_LAUNCH_CHECK();
+  sort_pairs_u64(ss.keys.p, ss.keys2.p, ss.list.p, ss.order.p, c, 64, tmp, s);
+  u32 gcap = 1;
+  while (gcap < 2 * std::max<u32>(maxdeg, 1)) gcap <<= 1;
+  if (maxdeg > SLCAP / 2) {
+    ss.gk.reserve(gcap, s);
+    ss.gv.reserve(gcap, s);
+    ss.gused.reserve(gcap, s);
+    fill_u32(ss.gk.p, kNone, gcap, s);
+    fill_f64(ss.gv.p, 0.0, gcap, s);
+  }
+  u32 ccap = 1;
+  while (ccap < std::max<u32>(maxdeg, 1)) ccap <<= 1;
+  if (maxdeg > kStaticCand) {
+    ss.ckey.reserve(ccap, s);
+    ss.cval.reserve(ccap, s);
+  }
+  StaticRefineArgs a{g, f, sub, size, dS, ss.ws.p, ss.wc.p, dC, ss.order.p, u32(c), gamma, ss.gk.p, ss.gv.p,
+                     ss.gused.p, gcap, ss.ckey.p, ss.cval.p, ccap};
+  HL_PROF(P_DET_REFINE, 0, s);
+  hipLaunchKernelGGL(k_static_refine, 1, 256, 0, s, a);
+  HL_LAUNCH_CHECK();
+}
+
+u64 static_aggregate(const Rec* d, u64 k, const u32* remap, Rec* out, StaticScratch& ss, Temp& tmp, hipStream_t s) {
+  if (k == 0) return 0;
+  ss.keys.reserve(k, s);
+  ss.keys2.reserve(k, s);
+  ss.vals.reserve(k, s);
+  ss.vals2.reserve(k, s);
+  ss.heads.reserve(k, s);
+  ss.starts.reserve(k, s);
+  hipLaunchKernelGGL(k_static_keys, GRID(k), d, k, remap, ss.keys.p, ss.vals.p);
+  HL_LAUNCH_CHECK();
+  sort_pairs_u64_f64(ss.keys.p, ss.keys2.p, ss.vals.p, ss.vals2.p, k, 64, tmp, s);
+  hipLaunchKernelGGL(k_heads_u64, GRID(k), ss.keys2.p, k, ss.heads.p);
+  HL_LAUNCH_CHECK();
+  const u64 nruns = select_flagged(ss.heads.p, k, ss.starts.p, tmp, ss.cnt, s);
+  hipLaunchKernelGGL(k_static_reduce, GRID(nruns), ss.keys2.p, ss.vals2.p, ss.starts.p, nruns, k, out);
+  HL_LAUNCH_CHECK();
+  return nruns;
+}
+
+void static_coarse_start(const u32* remap, u32 n, u32 cn, const u32* comm, u32* out, StaticScratch& ss,
+                         hipStream_t s) {
+  ss.first.reserve(cn + 1, s);
+  fill_u32(ss.first.p, kNone, cn, s);
+  hipLaunchKernelGGL(k_first_member, GRID(n), remap, n, ss.first.p);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_rep_comm, GRID(cn), ss.first.p, cn, comm, out);
+  HL_LAUNCH_CHECK();
+}
+
+void static_compose(const u32* cf, const u32* remap, u32 n, u32* comm, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_compose2, GRID(n), cf, remap, n, comm);
+  HL_LAUNCH_CHECK();
 }
 
 }  // namespace hl

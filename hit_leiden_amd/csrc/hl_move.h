@@ -84,19 +84,16 @@ struct MoveCtx {
   u32* ck = nullptr;
   u64 pool = 0;
   HubTables ht;
-  u32 ebase = 0;  // empty-community ids ebase + v (MovePolicyT)
 };
 
 // throughput movement (parallel_frontier.rs:155-236): self-loops skipped,
-// candidates in ascending id, empty community id = ebase + v.  The reference
-// uses n + v (parallel_frontier.rs:214); ebase = n there, and above n only on
-// a level whose label space also holds ids that n + v could meet (fresh ids of
-// grown vertices, labels pulled down from the level above)
+// candidates in ascending id, empty community id = n + v (the level's labels
+// are rank-compacted below n first, so the id is always free)
 struct MovePolicyT {
   const u32* f;
   const double* tot;
   double gamma;
-  u32 ebase;
+  u32 n;
   const u32* ck = nullptr;  // neighbour-community cache (MoveCtx::ck), or none
   HubTables ht;             // hub tables (MoveCtx::ht), or none
   static constexpr bool kEmpty = true;
@@ -118,7 +115,7 @@ struct MovePolicyT {
   __device__ double cur_total(u32 c) const { return tot[c]; }
   __device__ bool ok(u32) const { return true; }
   __device__ double cand_total(u32 c) const { return tot[c]; }
-  __device__ u32 empty_id(u32 v) const { return ebase + v; }
+  __device__ u32 empty_id(u32 v) const { return n + v; }
 };
 
 // throughput refinement (try_refinement_move, throughput.rs:291-359):

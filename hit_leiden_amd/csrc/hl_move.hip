@@ -2075,7 +2075,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   } else {
     m.ht = HubTables{};
   }
-  MovePolicyT pol{m.f, m.tot, m.gamma, m.ebase, m.ck, m.ht};
+  MovePolicyT pol{m.f, m.tot, m.gamma, n, m.ck, m.ht};
   u32 host_cnt = 0;
   d2h_async(&host_cnt, cnt_a, sizeof(u32), s);
   stream_sync(s);

@@ -52,6 +52,9 @@ def glib():
         L.hg_batch_len.argtypes = [vp, C.c_uint32]
         L.hg_batch.argtypes = [vp, C.c_uint32, vp, vp, vp]
         L.hg_free.argtypes = [vp]
+        L.hg_shuffle_order.argtypes = [C.c_uint64, C.c_uint64, vp]
+        L.hg_stdrng_u32.argtypes = [C.c_uint64, C.c_uint64, vp]
+        L.hg_chacha_block.argtypes = [vp, C.c_int, C.c_uint64, C.c_uint64, vp]
         _g = L
     return _g
 
@@ -114,6 +117,29 @@ def rmat(scale, ef=16, abc=(0.57, 0.19, 0.19), seed=1, threads=0):
 
 def sbm(n, blocks, mean_deg, mu, seed=1):
     return Workload(glib().hg_sbm(n, blocks, mean_deg, mu, seed), weighted=False)
+
+
+# ---- the reference loaders' seeded shuffle (rand 0.8.5 StdRng) ----
+
+def shuffle_order(seed, n):
+    """`StdRng::seed_from_u64(seed)` + `SliceRandom::shuffle` of n elements
+    (dynamic_graph.rs:85-87): out[i] = the original index of position i."""
+    out = np.zeros(max(int(n), 1), dtype=np.uint64)
+    glib().hg_shuffle_order(int(seed), int(n), out.ctypes.data)
+    return out[:int(n)]
+
+
+def stdrng_u32(seed, count):
+    out = np.zeros(max(int(count), 1), dtype=np.uint32)
+    glib().hg_stdrng_u32(int(seed), int(count), out.ctypes.data)
+    return out[:int(count)]
+
+
+def chacha_block(key_words, rounds, counter=0, stream=0):
+    k = np.ascontiguousarray(key_words, dtype=np.uint32)
+    out = np.zeros(16, dtype=np.uint32)
+    glib().hg_chacha_block(k.ctypes.data, int(rounds), int(counter), int(stream), out.ctypes.data)
+    return out
 
 
 # ---- the named configurations ----

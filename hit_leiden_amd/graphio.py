@@ -12,8 +12,8 @@
   batches of `batch_size` edges.  The reference returns the cumulative graph
   after each round; the incremental path consumes the difference, so this
   returns the batches themselves (`cumulative()` rebuilds the states).  The
-  shuffle is numpy's PCG64 seeded with `seed`, not Rust's StdRng, so the edge
-  order differs from the reference's for the same seed.
+  shuffle is the reference's own: rand 0.8.5 `StdRng::seed_from_u64(seed)`
+  (ChaCha12) and `SliceRandom::shuffle`, restated in hl_rng.h.
 """
 from __future__ import annotations
 
@@ -106,8 +106,9 @@ def paper_split(graph: GraphInput, initial_ratio, batch_size, rounds, seed) -> I
     """Shuffle, take floor(ratio * m) edges as the initial graph, then at most
     `rounds` batches of `batch_size` (dynamic_graph.rs:78-131); unit weights as
     the reference drops them."""
+    from .gen import shuffle_order
     m = len(graph.src)
-    order = np.random.default_rng(seed).permutation(m)
+    order = shuffle_order(seed, m)
     src, dst = graph.src[order], graph.dst[order]
     ratio = min(max(float(initial_ratio), 0.0), 1.0)
     init = int(np.floor(m * ratio))

@@ -13,8 +13,9 @@
   reported as passed, as the reference reports them.
 * ``check`` (src/core/validation/invariants.rs:19-28): a partition exists and
   every community id is below its node count.
-* ``run`` (hit_leiden::run, src/lib.rs:27-29): a static clustering on the
-  engine returned as a ``RunOutcome``.
+* ``run`` (hit_leiden::run, src/lib.rs:27-29; hit_leiden.rs:30-118): the
+  reference's static multilevel_leiden on the device
+  (``HitLeiden.static_leiden``) returned as a ``RunOutcome``.
 
 The benchmark's contingency metrics (NMI, purity, largest-community Jaccard,
 hit_leiden_incremental.rs:577-666) run on the device:
@@ -46,18 +47,30 @@ class RunOutcome:
     partition: Optional[PartitionResult]
 
 
-def run(graph, config=None) -> RunOutcome:
-    """hit_leiden::run (src/lib.rs:27-29) on the engine: a static clustering
-    by HitLeiden.run_static (load, then empty-delta passes to a fixpoint; not
-    the reference's sequential shuffled multilevel_leiden, hit_leiden.rs:30-118)
-    returned as a RunOutcome.  config defaults to RunConfig(), the throughput
-    mode; the reference's RunConfig::default() is deterministic
-    (config.rs:39-50), so pass RunConfig(mode=RunMode.DETERMINISTIC) for it."""
+def run(graph, config=None, seed=0) -> RunOutcome:
+    """hit_leiden::run (src/lib.rs:27-29, hit_leiden.rs:30-118) on the device:
+    multilevel_leiden (HitLeiden.static_leiden) with max_levels =
+    config.max_iterations (config.rs:31, > 0) and the queue shuffle drawn from
+    StdRng::seed_from_u64(seed); node_to_community numbered first-seen
+    (canonicalize_community_ids_in_place), hierarchy_levels the rows,
+    iteration_count the movement calls and quality_score compute_modularity
+    of the communities (hit_leiden.rs:121-161: 0 without edge weight).  The
+    static path does not depend on the mode (single_level_movement always
+    takes the queue, :806)."""
     from . import HitLeiden, RunConfig
-    e = HitLeiden(config or RunConfig())
+    cfg = config or RunConfig()
+    e = HitLeiden(cfg)
     try:
-        passes, _ = e.run_static(graph)
-        return RunOutcome(partition=partition_result(e, iteration_count=passes, run_id=""))
+        labels, rows, calls = e.static_leiden(graph, max_levels=cfg.max_iterations, seed=seed)
+        q = 0.0
+        if graph.node_count and len(graph):
+            e.load_graph(graph)
+            q = e.modularity_labels(labels)
+        part = PartitionResult(node_to_community=labels.astype(np.int64), quality_score=float(q),
+                               hierarchy_levels=[r.astype(np.int64) for r in rows],
+                               community_count=int(len(np.unique(labels))), iteration_count=int(calls),
+                               run_id=f"run:{getattr(graph, 'dataset_id', '')}")
+        return RunOutcome(partition=part)
     finally:
         e.close()
 

@@ -190,6 +190,21 @@ hl_status hl_read_hierarchy(const hl_graph* g, uint32_t* out, uint64_t cap, int3
 hl_status hl_modularity(const hl_graph* g, int32_t which, double* q);
 hl_status hl_modularity_labels(const hl_graph* g, const uint32_t* labels, uint64_t n, double* q);
 
+/* run(graph, config) (hit_leiden.rs:30-118) with multilevel_leiden
+   (:167-323): the static multilevel Leiden of the graph (src, dst, w) on n
+   vertices (w NULL: unit weights), at the handle's gamma and device; the
+   handle's loaded state is neither read nor changed.  max_levels is
+   RunConfig.max_iterations (> 0).  The movement queue is shuffled by
+   StdRng::seed_from_u64(seed) (rand 0.8.5), one generator for the whole run,
+   where the reference draws an unseeded thread_rng (:856-860).  labels[n]:
+   the communities, numbered first-seen; hier: up to hier_rows rows of n labels
+   (hierarchy_levels); *rows: the row count (NULL allowed), which may exceed
+   hier_rows; *calls: the movement calls (iteration_count, NULL allowed). */
+hl_status hl_run_static(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                        const double* w, uint32_t max_levels, uint64_t seed, uint32_t* labels, uint32_t* hier,
+                        uint32_t hier_rows, uint32_t* rows, int64_t* calls);
+
+
 /* contingency_metrics (hit_leiden_incremental.rs:577-666, the benchmark's quality
    check) on the device for two labelings a, b of n vertices: out[0] NMI =
    MI / sqrt(H(a) H(b)), out[1] purity a->b, out[2] purity b->a, out[3] the

@@ -19,12 +19,12 @@
 //   D9 the deterministic sweep skips self-loop entries in w(v, C) as the throughput shard
 //      does (parallel_frontier.rs:164-166); the reference counts them (hit_leiden.rs:1119-1125),
 //      which only matters on levels >= 1 (level-0 graphs carry no self-loops)
-//   D10 throughput empty-community ids are ebase + v, not n + v (parallel_frontier.rs:214):
-//      ebase = n on a level that never grows and takes no labels from the level above
-//      (every level of a one-level run); elsewhere the range [ebase, ebase + ecap) is
-//      placed above every label in use whenever pulled-down labels may reach it or the
-//      level outgrows it, and a vertex added within the range starts in its own empty
-//      community ebase + v (set_empty_base, grow_level)
+//   D10 the throughput movement of a level first relabels its community labels by rank
+//      (order kept, so no tie decision changes; compact_labels): every label is then below
+//      n and the empty-community ids n + v (parallel_frontier.rs:214) are always free.  The
+//      reference's n + v can name a populated community once a level holds labels it did
+//      not make (pulled down from above, fresh ids of grown vertices, earlier empty ids at a
+//      smaller n); mode bit OR_REF_IDS skips the relabel, as the reference
 //   D11 (OR_CONNECTED) the connected-component split links two vertices of a subcommunity
 //      only through edges inside their community (f(u) = f(v)), the edges the spec's
 //      subcommunity graph G_Psi keeps (hit_leiden_spec.md Alg. 2 lines 17-19, Alg. 3
@@ -61,11 +61,10 @@ constexpr double kEpsCompress = 1e-9;   // hit_leiden.rs:1711
 constexpr double kEpsTFilter = 1e-9;    // hit_leiden.rs:1547,1631
 constexpr int kMaxMoveRounds = 20;      // hit_leiden.rs:1077
 
-// test hooks: throughput moves to an empty-community id whose total was not
+// test hook: throughput moves to an empty-community id whose total was not
 // zero when the round's moves were selected (a populated community joined as
-// if it were empty), and a switch back to the reference's n + v ids
+// if it were empty)
 u64 g_empty_collisions = 0;
-bool g_reference_empty_ids = false;
 
 
 struct Rec {
@@ -274,8 +273,8 @@ struct MoveResult {
 };
 
 // execute_shard's per-vertex body (parallel_frontier.rs:155-236): self-loops
-// skipped, candidates in ascending id, empty community id = ebase + v (D10;
-// the reference's n + v is ebase = n).
+// skipped, candidates in ascending id, empty community id = ebase + v (callers
+// pass ebase = n: the reference's n + v).
 bool propose_thr(const Graph& g, u32 v, const std::vector<u32>& f, const std::vector<double>& tot, double t,
                  double gamma, u64 ebase, std::vector<std::pair<u32, double>>& buf, MoveCand& out) {
   const u32 cur = f[v];
@@ -823,36 +822,14 @@ struct Level {
   std::vector<uint8_t> A0, B, K, R;
   std::vector<Rec> dG;
   u32 iters = 0;
-  // D10 bookkeeping: every f label < lab_bound; ext bounds the labels that did
-  // not come from the level's own empty range [ebase, ebase + ecap)
-  u64 lab_bound = 0, ext = 0, ebase = 0, ecap = 0;
-  bool ebase_set = false;
-  u64 label_space() const { return std::max<u64>(ext, ebase_set ? ebase + ecap : 0); }
 };
 
-u64 vmax_or(const std::vector<u32>& x, u64 dflt);
-
-// D10: a new place for the empty range (above every label in use and above
-// ext, with room for the level to double) when labels from outside it may lie
-// in it or the level outgrew it
-void set_empty_base(Level& lv, bool thr) {
-  const u64 n = lv.G.n;
-  if (g_reference_empty_ids) {
-    lv.ebase = n;
-    lv.ecap = n;
-    lv.ebase_set = true;
-    lv.lab_bound = std::max<u64>(lv.lab_bound, 2 * n);
-    return;
-  }
-  if (!thr) return;
-  if (!lv.ebase_set || lv.ext > lv.ebase || n > lv.ecap) {
-    const u64 mx = n ? vmax_or(lv.f, 0) + 1 : 0;
-    lv.ext = std::max<u64>(lv.ext, mx);
-    lv.ecap = lv.ebase_set ? 2 * n : n;
-    lv.ebase = std::max<u64>(n, lv.ext);
-    lv.ebase_set = true;
-  }
-  lv.lab_bound = std::max<u64>(lv.lab_bound, lv.label_space());
+// D10: rank relabel of a level's community labels (ascending label order kept)
+void compact_labels(std::vector<u32>& f) {
+  std::vector<u32> vals(f);
+  std::sort(vals.begin(), vals.end());
+  vals.erase(std::unique(vals.begin(), vals.end()), vals.end());
+  for (auto& x : f) x = u32(std::lower_bound(vals.begin(), vals.end(), x) - vals.begin());
 }
 
 struct State {
@@ -868,22 +845,17 @@ u64 vmax_or(const std::vector<u32>& x, u64 dflt) {
   return *std::max_element(x.begin(), x.end());
 }
 
-// D6: new vertices [n_old, n_new) get fresh labels (D10: f labels in their own
-// empty communities while the throughput empty range has room for them).
-void grow_level(Level& lv, u32 n_new, const Level* up, bool thr) {
+// D6: new vertices [n_old, n_new) get fresh labels.
+void grow_level(Level& lv, u32 n_new, const Level* up) {
   u32 n_old = lv.G.n;
   if (n_new <= n_old) return;
   u64 base_f = std::max<u64>(n_old, lv.f.empty() ? 0 : vmax_or(lv.f, 0) + 1);
-  const bool own = thr && !g_reference_empty_ids && lv.ebase_set && n_new <= lv.ecap && lv.ext <= lv.ebase;
-  if (own) base_f = lv.ebase + n_old;
   u64 base_g = std::max<u64>(n_old, lv.g.empty() ? 0 : vmax_or(lv.g, 0) + 1);
   u64 ms = 0;
   bool have = !lv.s_cur.empty();
   if (have) ms = std::max(vmax_or(lv.s_cur, 0), vmax_or(lv.s_pre, 0)) + 1;
   u64 base_s = std::max<u64>(n_old, ms);
   if (up) base_s = std::max<u64>(base_s, up->G.n);
-  if (!own) lv.ext = std::max<u64>(lv.ext, base_f + (n_new - n_old));
-  lv.lab_bound = std::max<u64>(lv.lab_bound, std::max<u64>(lv.label_space(), base_f + (n_new - n_old)));
   for (u32 v = n_old; v < n_new; ++v) {
     lv.f.push_back(u32(base_f + (v - n_old)));
     lv.g.push_back(u32(base_g + (v - n_old)));
@@ -900,11 +872,11 @@ u32 hit_leiden(State& st, const std::vector<Rec>& dG0) {
   const int P = st.P;
   for (int p = 0; p < P; ++p) {
     Level& lv = st.L[p];
-    grow_level(lv, std::max(lv.G.n, cur_n), p + 1 < P ? &st.L[p + 1] : nullptr, (st.mode & OR_THR) != 0);
+    grow_level(lv, std::max(lv.G.n, cur_n), p + 1 < P ? &st.L[p + 1] : nullptr);
     apply_delta(lv.G, cur, lv.G.n);
     lv.dG = cur;
-    set_empty_base(lv, (st.mode & OR_THR) != 0);
-    MoveResult mr = inc_movement(lv.G, cur, lv.f, lv.s_cur, st.gamma, st.mode, lv.ebase);
+    if ((st.mode & OR_THR) && !(st.mode & OR_REF_IDS)) compact_labels(lv.f);  // D10
+    MoveResult mr = inc_movement(lv.G, cur, lv.f, lv.s_cur, st.gamma, st.mode, lv.G.n);
     lv.A0 = std::move(mr.A0);
     lv.B = std::move(mr.B);
     lv.K = std::move(mr.K);
@@ -938,17 +910,13 @@ u32 hit_leiden(State& st, const std::vector<Rec>& dG0) {
   const bool all = (st.mode & OR_CONNECTED) != 0;  // D12
   def_update(fm, sm, B, all);
   def_update(gm, sm, R, all);
-  for (int p = P - 2; p >= 0; --p) {  // D10: f^p took f^{p+1} labels
-    st.L[p].lab_bound = std::max(st.L[p].lab_bound, st.L[p + 1].lab_bound);
-    st.L[p].ext = std::max(st.L[p].ext, st.L[p + 1].lab_bound);
-  }
   return total;
 }
 
 void reset_state(State& st, u32 n) {
   st.L.assign(st.P, Level());
   st.n0 = n;
-  grow_level(st.L[0], n, st.P > 1 ? &st.L[1] : nullptr, (st.mode & OR_THR) != 0);
+  grow_level(st.L[0], n, st.P > 1 ? &st.L[1] : nullptr);
 }
 
 bool valid_edges(u32 n, u64 m, const u32* u, const u32* v, const double* w) {
@@ -1044,12 +1012,6 @@ int or_write_map(void* h, int level, int which, const uint32_t* in) {
   if (level < 0 || level >= st.P) return -1;
   auto& m = level_map(st.L[level], which);
   std::copy(in, in + m.size(), m.begin());
-  if (which == OR_MAP_F) {  // caller-written labels: a fresh empty base on the next batch
-    Level& lv = st.L[level];
-    lv.lab_bound = m.empty() ? 0 : std::max<u64>(m.size(), vmax_or(m, 0) + 1);
-    lv.ext = lv.lab_bound;
-    lv.ebase_set = false;
-  }
   return 0;
 }
 
@@ -1248,16 +1210,371 @@ void or_canonicalize_fn(uint32_t n, uint32_t* labels) {
   }
 }
 
+// ---- rand 0.8.5 StdRng + SliceRandom::shuffle (the reference loaders'
+// seeded shuffle, dynamic_graph.rs:85-87; rand_core 0.6.4 seed_from_u64,
+// rand_chacha 0.3.1 ChaCha12, UniformInt<u32>::sample_single_inclusive) ----
+namespace {
+struct ChaChaStream {
+  uint32_t in[16];
+  uint32_t out[16];
+  int rounds;
+  int pos = 16;
+  static uint32_t rl(uint32_t x, int n) { return (x << n) | (x >> (32 - n)); }
+  void next_block() {
+    static const int idx[8][4] = {{0, 4, 8, 12}, {1, 5, 9, 13}, {2, 6, 10, 14}, {3, 7, 11, 15},
+                                  {0, 5, 10, 15}, {1, 6, 11, 12}, {2, 7, 8, 13}, {3, 4, 9, 14}};
+    uint32_t x[16];
+    for (int i = 0; i < 16; ++i) x[i] = in[i];
+    for (int r = 0; r < rounds / 2; ++r)
+      for (const auto& q : idx) {
+        uint32_t &a = x[q[0]], &b = x[q[1]], &c = x[q[2]], &d = x[q[3]];
+        a += b, d = rl(d ^ a, 16);
+        c += d, b = rl(b ^ c, 12);
+        a += b, d = rl(d ^ a, 8);
+        c += d, b = rl(b ^ c, 7);
+      }
+    for (int i = 0; i < 16; ++i) out[i] = x[i] + in[i];
+    uint64_t ctr = (uint64_t(in[13]) << 32 | in[12]) + 1;
+    in[12] = uint32_t(ctr), in[13] = uint32_t(ctr >> 32);
+    pos = 0;
+  }
+  uint32_t word() {
+    if (pos == 16) next_block();
+    return out[pos++];
+  }
+};
+ChaChaStream chacha(const uint32_t* key, int rounds, uint64_t ctr, uint64_t stream) {
+  ChaChaStream c;
+  const uint32_t sigma[4] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+  for (int i = 0; i < 4; ++i) c.in[i] = sigma[i];
+  for (int i = 0; i < 8; ++i) c.in[4 + i] = key[i];
+  c.in[12] = uint32_t(ctr), c.in[13] = uint32_t(ctr >> 32), c.in[14] = uint32_t(stream), c.in[15] = uint32_t(stream >> 32);
+  c.rounds = rounds;
+  return c;
+}
+ChaChaStream std_rng(uint64_t seed) {
+  uint32_t key[8];
+  for (auto& k : key) {  // rand_core's PCG32 seed expansion
+    seed = seed * 6364136223846793005ull + 11634580027462260723ull;
+    const uint32_t xorshifted = uint32_t(((seed >> 18) ^ seed) >> 27), rot = uint32_t(seed >> 59);
+    k = rot ? (xorshifted >> rot) | (xorshifted << (32 - rot)) : xorshifted;
+  }
+  return chacha(key, 12, 0, 0);
+}
+// gen_range(0..ubound) for u32 (rand 0.8.5): widening multiply, reject above the zone
+uint32_t gen_index(ChaChaStream& r, uint32_t ubound) {
+  const int lz = __builtin_clz(ubound);
+  const uint32_t zone = uint32_t((uint64_t(ubound) << lz) - 1);
+  for (;;) {
+    const uint64_t prod = uint64_t(r.word()) * uint64_t(ubound);
+    if (uint32_t(prod & 0xffffffffu) <= zone) return uint32_t(prod >> 32);
+  }
+}
+}  // namespace
+
+void or_stdrng_shuffle(uint64_t seed, uint64_t n, uint64_t* a) {
+  for (uint64_t i = 0; i < n; ++i) a[i] = i;
+  ChaChaStream r = std_rng(seed);
+  for (uint64_t i = n; i >= 2; --i) std::swap(a[i - 1], a[gen_index(r, uint32_t(i))]);
+}
+void or_chacha_block(const uint32_t* key, int rounds, uint64_t counter, uint64_t stream, uint32_t* out) {
+  ChaChaStream c = chacha(key, rounds, counter, stream);
+  for (int i = 0; i < 16; ++i) out[i] = c.word();
+}
+
+// ---------------------------------------------------------------------------
+// The static path: run() / multilevel_leiden (hit_leiden.rs:30-118, 167-323),
+// single_level_movement (:764-932), refine_singleton_merge (:487-710),
+// subcommunity_delta_q_to_empty (:712-749), aggregate_graph (:751-790).
+// Canonical orders as above (sorted rows, candidates in ascending id, the
+// coarse graph's edges sorted by key, the coarse initial communities numbered
+// first-seen over ascending coarse id); the unseeded thread_rng of the queue
+// shuffle (:856-860) is one StdRng::seed_from_u64(seed) for the whole run.
+// ---------------------------------------------------------------------------
+namespace {
+
+// queue-based deterministic movement (the path the reference always takes:
+// USE_PARALLEL_MULTILEVEL_MOVEMENT = false, :806)
+u32 static_movement(const Graph& g, const std::vector<Rec>& delta, std::vector<u32>& f, double gamma,
+                    ChaChaStream& rng) {
+  const u32 n = g.n;
+  std::vector<uint8_t> active(n, 0);
+  for (const Rec& e : delta)
+    if (e.w > 0.0 && e.u < n && e.v < n && f[e.u] != f[e.v]) active[e.u] = active[e.v] = 1;
+  if (delta.empty()) std::fill(active.begin(), active.end(), 1);
+  const double t = g.twice_m;
+  std::vector<double> cdeg(n, 0.0);
+  for (u32 i = 0; i < n; ++i) cdeg[f[i]] += g.k[i];
+  std::vector<u64> order;
+  for (u32 i = 0; i < n; ++i)
+    if (active[i]) order.push_back(i);
+  for (u64 i = order.size(); i >= 2; --i) std::swap(order[i - 1], order[gen_index(rng, u32(i))]);
+  std::vector<u32> queue(order.begin(), order.end());
+  std::vector<uint8_t> inq = active;
+  std::vector<std::pair<u32, double>> buf;
+  for (size_t h = 0; h < queue.size(); ++h) {
+    const u32 v = queue[h];
+    inq[v] = 0;
+    const u32 cur = f[v];
+    const double kv = g.k[v];
+    buf.clear();
+    for (size_t j = 0; j < g.col[v].size(); ++j)
+      if (g.col[v][j] != v) buf.push_back({f[g.col[v][j]], g.wt[v][j]});
+    std::stable_sort(buf.begin(), buf.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    double wcur = 0.0;
+    for (auto& e : buf)
+      if (e.first == cur) wcur += e.second;
+    const double sc = cdeg[cur];
+    u32 best = cur;
+    double bg = 0.0;
+    for (size_t i = 0; i < buf.size();) {
+      const u32 c = buf[i].first;
+      double wc = 0.0;
+      while (i < buf.size() && buf[i].first == c) wc += buf[i++].second;
+      if (c == cur) continue;
+      const double gain = move_gain(wc, wcur, t, gamma, kv, sc, cdeg[c]);
+      if (gain > bg) bg = gain, best = c;
+    }
+    if (bg <= 0.0) continue;
+    f[v] = best;
+    cdeg[cur] -= kv;
+    cdeg[best] += kv;
+    for (u32 u : g.col[v]) {
+      if (u == v || f[u] == best || inq[u]) continue;
+      queue.push_back(u);
+      inq[u] = 1;
+    }
+  }
+  return 1;
+}
+
+// Leiden refinement of the static path: subcommunities start as the
+// communities, disconnected ones split (the largest component keeps the id,
+// ties to the smaller minimum vertex), then the split-off singletons merge
+// with the T-filter and the EPS tie rules of :617-693.
+std::vector<u32> static_refine(const Graph& g, const std::vector<u32>& f, double gamma) {
+  const u32 n = g.n;
+  std::vector<u32> sub(f);
+  if (n == 0) return sub;
+  const double t = g.twice_m;
+  if (t == 0.0) return sub;
+  const double EPS = 1e-12;
+  u64 maxc = 0;
+  for (u32 x : f) maxc = std::max<u64>(maxc, x);
+  std::vector<double> dC(maxc + 1, 0.0);
+  for (u32 i = 0; i < n; ++i) dC[f[i]] += g.k[i];
+  u64 next_id = maxc + 1;
+  std::vector<uint8_t> refined(n, 0);
+  SubIndex ix = build_members(sub, n, maxc + 1);
+  std::vector<uint8_t> visited(n, 0);
+  std::vector<u32> q;
+  for (u64 sc = 0; sc <= maxc; ++sc) {
+    const u64 b = ix.off[sc], e = ix.off[sc + 1];
+    if (e - b <= 1) continue;
+    std::vector<std::vector<u32>> comps;
+    for (u64 pp = b; pp < e; ++pp) {
+      const u32 st = ix.mem[pp];
+      if (visited[st]) continue;
+      q.assign(1, st);
+      visited[st] = 1;
+      for (size_t h = 0; h < q.size(); ++h)
+        for (u32 y : g.col[q[h]])
+          if (sub[y] == u32(sc) && !visited[y]) visited[y] = 1, q.push_back(y);
+      comps.push_back(q);
+    }
+    for (u64 pp = b; pp < e; ++pp) visited[ix.mem[pp]] = 0;
+    if (comps.size() <= 1) continue;
+    // components are discovered from ascending start vertices, so a later
+    // component of equal size has a larger minimum: the first largest is kept
+    size_t keep = 0;
+    for (size_t c = 1; c < comps.size(); ++c)
+      if (comps[c].size() > comps[keep].size()) keep = c;
+    for (size_t c = 0; c < comps.size(); ++c) {
+      if (c == keep) continue;
+      const u32 nid = u32(next_id++);
+      for (u32 v : comps[c]) sub[v] = nid, refined[v] = 1;
+    }
+  }
+  std::vector<u64> size(next_id, 0);
+  std::vector<double> dS(next_id, 0.0);
+  for (u32 v = 0; v < n; ++v) size[sub[v]] += 1, dS[sub[v]] += g.k[v];
+  std::vector<u32> R;
+  for (u32 v = 0; v < n; ++v)
+    if (refined[v]) R.push_back(v);
+  std::stable_sort(R.begin(), R.end(), [&](u32 a, u32 b) { return g.k[a] < g.k[b]; });
+  std::vector<std::pair<u32, double>> buf;
+  for (u32 v : R) {
+    const u32 cur = sub[v];
+    if (size[cur] != 1) continue;
+    const u32 C = f[v];
+    const double kv = g.k[v];
+    buf.clear();
+    double wcur = 0.0;
+    for (size_t j = 0; j < g.col[v].size(); ++j) {
+      const u32 u = g.col[v][j];
+      if (u == v || f[u] != C) continue;
+      buf.push_back({sub[u], g.wt[v][j]});
+      if (sub[u] == cur) wcur += g.wt[v][j];
+    }
+    if (buf.empty()) continue;
+    std::stable_sort(buf.begin(), buf.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    u32 best = cur;
+    double bg = 0.0, bw = -1.0;
+    for (size_t i = 0; i < buf.size();) {
+      const u32 S = buf[i].first;
+      double w = 0.0;
+      while (i < buf.size() && buf[i].first == S) w += buf[i++].second;
+      if (S == cur) continue;
+      // subcommunity_delta_q_to_empty: d_S = 0 gives 0 (kept); w(S, C) sums
+      // every row entry of every member that lies in C, a self-loop twice
+      double dq = 0.0;
+      if (dS[S] != 0.0) {
+        double ws = 0.0;
+        for (u32 i2 = 0; i2 < n; ++i2) {
+          if (sub[i2] != S) continue;
+          for (size_t j = 0; j < g.col[i2].size(); ++j)
+            if (f[g.col[i2][j]] == C) ws += (g.col[i2][j] == i2) ? 2.0 * g.wt[i2][j] : g.wt[i2][j];
+        }
+        dq = (0.0 - ws) / t + gamma * dS[S] * (dC[C] - dS[S]) / (t * t);
+      }
+      if (!(dq <= EPS)) continue;
+      const double gain = (w - wcur) / t + gamma * kv * (dS[cur] - kv - dS[S]) / (t * t);
+      const bool better = gain > bg + EPS ||
+                          (std::fabs(gain - bg) <= EPS && (w > bw + EPS || (std::fabs(w - bw) <= EPS && S < best)));
+      if (better) best = S, bg = gain, bw = w;
+    }
+    if (bg > EPS) {
+      sub[v] = best;
+      size[cur] -= 1, size[best] += 1;
+      dS[cur] -= kv, dS[best] += kv;
+    }
+  }
+  return sub;
+}
+
+std::vector<u32> first_seen(const std::vector<u32>& a) {
+  std::vector<u32> out(a.size());
+  std::vector<std::pair<u32, u32>> seen;  // (label, id), sorted by label
+  u32 next = 0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    auto it = std::lower_bound(seen.begin(), seen.end(), std::make_pair(a[i], 0u),
+                               [](const auto& x, const auto& y) { return x.first < y.first; });
+    if (it != seen.end() && it->first == a[i]) {
+      out[i] = it->second;
+    } else {
+      seen.insert(it, {a[i], next});
+      out[i] = next++;
+    }
+  }
+  return out;
+}
+
+u64 count_distinct(const std::vector<u32>& a) {
+  std::vector<u32> b(a);
+  std::sort(b.begin(), b.end());
+  return u64(std::unique(b.begin(), b.end()) - b.begin());
+}
+
+}  // namespace
+
+// run(graph, config) (hit_leiden.rs:30-118): labels (first-seen contiguous),
+// hierarchy rows (first-seen per row, at most max_levels + 1 rows, *rows set),
+// returns the movement call count or -1 for bad input.
+int64_t or_static_run(uint32_t n, uint64_t m, const uint32_t* eu, const uint32_t* ev, const double* ew, double gamma,
+                      uint32_t max_levels, uint64_t seed, uint32_t* labels, uint32_t* hier, uint32_t hier_rows,
+                      uint32_t* rows) {
+  if (!valid_edges(n, m, eu, ev, ew)) return -1;
+  std::vector<Rec> edges(m);
+  for (u64 i = 0; i < m; ++i) edges[i] = {eu[i], ev[i], ew ? ew[i] : 1.0};
+  Graph g = graph_from_edges(n, m, eu, ev, ew);
+  ChaChaStream rng = std_rng(seed);
+  std::vector<u32> comm(n);
+  for (u32 i = 0; i < n; ++i) comm[i] = i;
+  int64_t iters = static_movement(g, edges, comm, gamma, rng);
+  std::vector<u32> sub = static_refine(g, comm, gamma);
+  std::vector<std::vector<u32>> H;
+  H.push_back(first_seen(comm));
+  u64 prev = count_distinct(comm);
+  for (u32 level = 1; level < max_levels; ++level) {
+    if (count_distinct(sub) <= 1) break;
+    // aggregate_graph: the input edges summed per (remapped) pair, an
+    // intra-subcommunity edge onto a self-loop, in input order
+    const std::vector<u32> remap = first_seen(sub);
+    u64 cn = 0;
+    for (u32 x : remap) cn = std::max<u64>(cn, u64(x) + 1);
+    std::vector<std::pair<u64, double>> keyed;
+    keyed.reserve(m);
+    for (const Rec& e : edges) {
+      const u64 a = remap[e.u], b = remap[e.v];
+      keyed.push_back({(std::min(a, b) << 32) | std::max(a, b), e.w});
+    }
+    std::stable_sort(keyed.begin(), keyed.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    std::vector<u32> cu, cv;
+    std::vector<double> cw;
+    for (size_t i = 0; i < keyed.size();) {
+      const u64 key = keyed[i].first;
+      double w = 0.0;
+      while (i < keyed.size() && keyed[i].first == key) w += keyed[i++].second;
+      cu.push_back(u32(key >> 32));
+      cv.push_back(u32(key & 0xffffffffu));
+      cw.push_back(w);
+    }
+    if (cn <= 1 || cu.empty()) break;
+    // each supervertex starts in its first member's community, those numbered
+    // first-seen over ascending supervertex id (build_coarse_initial_partition)
+    std::vector<u32> rep(cn, UINT32_MAX);
+    for (u32 i = 0; i < n; ++i)
+      if (rep[remap[i]] == UINT32_MAX) rep[remap[i]] = comm[i];
+    std::vector<u32> cf = first_seen(rep);
+    Graph cg = graph_from_edges(u32(cn), cu.size(), cu.data(), cv.data(), cw.data());
+    std::vector<Rec> cedges(cu.size());
+    for (size_t i = 0; i < cu.size(); ++i) cedges[i] = {cu[i], cv[i], cw[i]};
+    iters += static_movement(cg, cedges, cf, gamma, rng);
+    for (u32 i = 0; i < n; ++i) comm[i] = cf[remap[i]];
+    H.push_back(first_seen(comm));
+    const u64 now = count_distinct(comm);
+    sub = static_refine(g, comm, gamma);
+    if (now >= prev) break;
+    prev = now;
+  }
+  // final polish on the input graph (:290-305)
+  iters += static_movement(g, edges, comm, gamma, rng);
+  std::vector<u32> fin = first_seen(comm);
+  if (H.back() != fin) H.push_back(fin);
+  std::copy(fin.begin(), fin.end(), labels);
+  const u32 nr = std::min<u32>(u32(H.size()), hier_rows);
+  for (u32 r = 0; r < nr; ++r) std::copy(H[r].begin(), H[r].end(), hier + u64(r) * n);
+  *rows = u32(H.size());
+  return iters;
+}
+
+// single_level_movement on its own (a fresh StdRng(seed) for the queue shuffle)
+int or_static_movement_fn(uint32_t n, uint64_t m, const uint32_t* eu, const uint32_t* ev, const double* ew,
+                          uint32_t* f, double gamma, uint64_t seed) {
+  if (!valid_edges(n, m, eu, ev, ew)) return -1;
+  std::vector<Rec> edges(m);
+  for (u64 i = 0; i < m; ++i) edges[i] = {eu[i], ev[i], ew ? ew[i] : 1.0};
+  Graph g = graph_from_edges(n, m, eu, ev, ew);
+  std::vector<u32> fv(f, f + n);
+  ChaChaStream rng = std_rng(seed);
+  static_movement(g, edges, fv, gamma, rng);
+  std::copy(fv.begin(), fv.end(), f);
+  return 0;
+}
+// refine_singleton_merge on its own
+int or_static_refine_fn(uint32_t n, uint64_t m, const uint32_t* eu, const uint32_t* ev, const double* ew,
+                        const uint32_t* f, double gamma, uint32_t* sub) {
+  if (!valid_edges(n, m, eu, ev, ew)) return -1;
+  Graph g = graph_from_edges(n, m, eu, ev, ew);
+  std::vector<u32> s2 = static_refine(g, std::vector<u32>(f, f + n), gamma);
+  std::copy(s2.begin(), s2.end(), sub);
+  return 0;
+}
+
 uint64_t or_empty_collisions(int reset) {
   const u64 c = g_empty_collisions;
   if (reset) g_empty_collisions = 0;
   return c;
-}
-void or_set_reference_empty_ids(int on) { g_reference_empty_ids = on != 0; }
-uint64_t or_empty_base(void* h, int level) {
-  State& st = *static_cast<State*>(h);
-  if (level < 0 || level >= st.P) return 0;
-  return st.L[level].ebase;
 }
 
 int or_should_skip_aggregation_fn(uint64_t delta_len, uint32_t n, const uint8_t* R) {

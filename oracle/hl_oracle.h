@@ -28,7 +28,9 @@
 extern "C" {
 #endif
 
-enum { OR_DET = 0, OR_THR = 1, OR_CONNECTED = 2 }; /* mode bits; OR_CONNECTED only with OR_THR */
+/* mode bits: OR_CONNECTED (D8, D11, D12) and OR_REF_IDS (no D10 relabel: the
+   reference's n + v may name a populated community) only with OR_THR */
+enum { OR_DET = 0, OR_THR = 1, OR_CONNECTED = 2, OR_REF_IDS = 4 };
 enum { OR_MAP_F = 0, OR_MAP_G = 1, OR_MAP_S_CUR = 2, OR_MAP_S_PRE = 3 };
 enum { OR_SET_A = 0, OR_SET_B = 1, OR_SET_K = 2, OR_SET_R = 3 };
 
@@ -77,12 +79,22 @@ double or_modularity_fn(uint32_t n, uint64_t m, const uint32_t* eu, const uint32
 uint64_t or_select_decoupled_fn(uint64_t k, const uint32_t* node, const uint32_t* from, const uint32_t* to,
                                 const double* gain, uint32_t* out_nodes);
 void or_canonicalize_fn(uint32_t n, uint32_t* labels);
-/* test hooks: throughput moves onto an empty-community id whose total was not
-   zero (counted since the last reset), the reference's n + v ids instead of
-   the D10 base (a planted mistake for the tests), a level's empty base */
+/* test hook: throughput moves onto an empty-community id whose total was not
+   zero (counted since the last reset) */
 uint64_t or_empty_collisions(int reset);
-void     or_set_reference_empty_ids(int on);
-uint64_t or_empty_base(void* h, int level);
+/* run() / multilevel_leiden (hit_leiden.rs:30-118, 167-323) with the queue
+   shuffle seeded (StdRng::seed_from_u64(seed)): first-seen labels, up to
+   hier_rows hierarchy rows of n (first-seen per row), *rows = rows produced;
+   returns the movement call count, -1 for bad input */
+int64_t  or_static_run(uint32_t n, uint64_t m, const uint32_t* eu, const uint32_t* ev, const double* ew, double gamma,
+                       uint32_t max_levels, uint64_t seed, uint32_t* labels, uint32_t* hier, uint32_t hier_rows,
+                       uint32_t* rows);
+/* single_level_movement (hit_leiden.rs:764-932, the queue shuffled by a fresh
+   StdRng(seed)) and refine_singleton_merge (:487-710) on their own */
+int      or_static_movement_fn(uint32_t n, uint64_t m, const uint32_t* eu, const uint32_t* ev, const double* ew,
+                               uint32_t* f, double gamma, uint64_t seed);
+int      or_static_refine_fn(uint32_t n, uint64_t m, const uint32_t* eu, const uint32_t* ev, const double* ew,
+                             const uint32_t* f, double gamma, uint32_t* sub);
 int  or_should_skip_aggregation_fn(uint64_t delta_len, uint32_t n, const uint8_t* R);
 
 #ifdef __cplusplus

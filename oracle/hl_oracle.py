@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libhl_oracle.so")
 _lib = None
 
-DET, THR, CONNECTED = 0, 1, 2
+DET, THR, CONNECTED, REF_IDS = 0, 1, 2, 4
 PAR = THR | CONNECTED
 MAP_F, MAP_G, MAP_S_CUR, MAP_S_PRE = 0, 1, 2, 3
 SET_A, SET_B, SET_K, SET_R = 0, 1, 2, 3
@@ -86,10 +86,17 @@ def lib():
         L.or_should_skip_aggregation_fn.argtypes = [C.c_uint64, C.c_uint32, _u8p]
         L.or_empty_collisions.argtypes = [C.c_int]
         L.or_empty_collisions.restype = C.c_uint64
-        L.or_set_reference_empty_ids.argtypes = [C.c_int]
-        L.or_set_reference_empty_ids.restype = None
-        L.or_empty_base.argtypes = [C.c_void_p, C.c_int]
-        L.or_empty_base.restype = C.c_uint64
+        L.or_stdrng_shuffle.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p]
+        L.or_stdrng_shuffle.restype = None
+        L.or_chacha_block.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, C.c_void_p]
+        L.or_chacha_block.restype = None
+        L.or_static_run.restype = C.c_int64
+        L.or_static_run.argtypes = [C.c_uint32, C.c_uint64, _u32p, _u32p, C.c_void_p, C.c_double, C.c_uint32,
+                                    C.c_uint64, _u32p, _u32p, C.c_uint32, C.POINTER(C.c_uint32)]
+        L.or_static_movement_fn.argtypes = [C.c_uint32, C.c_uint64, _u32p, _u32p, C.c_void_p, _u32p, C.c_double,
+                                            C.c_uint64]
+        L.or_static_refine_fn.argtypes = [C.c_uint32, C.c_uint64, _u32p, _u32p, C.c_void_p, _u32p, C.c_double,
+                                          _u32p]
         _lib = L
     return _lib
 
@@ -285,6 +292,50 @@ def select_decoupled_moves(cands):
     out = np.zeros(max(k, 1), dtype=np.uint32)
     c = lib().or_select_decoupled_fn(k, node, frm, to, gain, out)
     return out[:c].tolist()
+
+
+def stdrng_shuffle(seed, n):
+    out = np.zeros(max(int(n), 1), dtype=np.uint64)
+    lib().or_stdrng_shuffle(int(seed), int(n), out.ctypes.data_as(C.c_void_p))
+    return out[:int(n)]
+
+
+def chacha_block(key_words, rounds, counter=0, stream=0):
+    k = np.ascontiguousarray(key_words, dtype=np.uint32)
+    out = np.zeros(16, dtype=np.uint32)
+    lib().or_chacha_block(k.ctypes.data_as(C.c_void_p), int(rounds), int(counter), int(stream),
+                          out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def static_run(n, u, v, w=None, gamma=1.0, max_levels=10, seed=0):
+    """run() (hit_leiden.rs:30-118): (labels, hierarchy rows, movement calls)."""
+    u = np.ascontiguousarray(u, dtype=np.uint32)
+    v = np.ascontiguousarray(v, dtype=np.uint32)
+    cap = max_levels + 2
+    labels = np.zeros(max(n, 1), dtype=np.uint32)
+    hier = np.zeros(max(n, 1) * cap, dtype=np.uint32)
+    rows = C.c_uint32(0)
+    it = lib().or_static_run(n, len(u), u, v, _wptr(w), gamma, max_levels, seed, labels, hier, cap, C.byref(rows))
+    if it < 0:
+        raise ValueError("invalid edges")
+    r = min(rows.value, cap)
+    return labels[:n], hier[:r * n].reshape(r, n) if n else np.zeros((r, 0), np.uint32), int(it)
+
+
+def static_movement(n, edges, weights, f, gamma, seed):
+    u, v, w = _edges(edges, weights)
+    fa = np.ascontiguousarray(f, dtype=np.uint32).copy()
+    lib().or_static_movement_fn(n, len(u), u, v, _wptr(w), fa, gamma, seed)
+    return fa
+
+
+def static_refine(n, edges, weights, f, gamma):
+    u, v, w = _edges(edges, weights)
+    fa = np.ascontiguousarray(f, dtype=np.uint32)
+    out = np.zeros(n, dtype=np.uint32)
+    lib().or_static_refine_fn(n, len(u), u, v, _wptr(w), fa, gamma, out)
+    return out
 
 
 def canonicalize(labels):

@@ -59,8 +59,11 @@ def make_graphs(count, seed):
     return [sbm_graph(int(n), int(rng.integers(2, 9)), seed * 1000 + i) for i, n in enumerate(sizes)]
 
 
+ALONE = dict(mode=hl.RunMode.THROUGHPUT, levels=P)
+
+
 def alone(graph, batches):
-    e = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=P))
+    e = hl.HitLeiden(hl.RunConfig(**ALONE))
     e.load(graph)
     for b in batches:
         e.hit_leiden(b)
@@ -170,7 +173,7 @@ def test_served_results_match_alone(budget, coalesce):
         assert ps == sorted(ps)  # a graph's requests complete in arrival order
         if not coalesce:
             assert len(set(ps)) == len(ps)
-        e = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=P))
+        e = hl.HitLeiden(hl.RunConfig(**ALONE))
         e.load(graph)
         for p in sorted(set(ps)):
             group = [i for _, i in mine if int(pass_of[i]) == p]

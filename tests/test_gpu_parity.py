@@ -467,19 +467,19 @@ def test_small_config_reference_throughput(levels):
     g.close()
 
 
-def test_empty_ids_above_written_labels():
+def test_empty_ids_after_written_labels():
     """Caller-written labels [7,7,7,8,8,8] on two triangles (n = 6): at gamma = 4
-    every vertex leaves for an empty community; the ids start above the labels
-    in use (9 + v), where the reference's n + v would put vertex 2 into the other
-    triangle's community 8."""
-    say("empty ids above written labels")
+    every vertex leaves for an empty community.  The labels are relabelled by
+    rank first ([0,0,0,1,1,1]), so the ids n + v = 6 + v are free, where the
+    unrelabelled 8 = n + 2 would put vertex 2 into the other triangle."""
+    say("empty ids after written labels")
     u = np.array([0, 1, 0, 3, 4, 3], np.uint32)
     v = np.array([1, 2, 2, 4, 5, 5], np.uint32)
     g = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, resolution=4.0))
     g.load_graph(hl.GraphInput(6, u, v))
     g.write_partition(0, hl.MAP_F, np.array([7, 7, 7, 8, 8, 8], np.uint32))
     g.hit_leiden(hl.GraphInput(6, np.zeros(0, np.uint32), np.zeros(0, np.uint32)))
-    assert g.partition(0, hl.MAP_F).tolist() == [9, 10, 11, 12, 13, 14]
+    assert g.partition(0, hl.MAP_F).tolist() == [6, 7, 8, 9, 10, 11]
     o = O.OracleState(levels=1, gamma=4.0, mode=O.PAR)
     o.load_graph(6, u, v)
     o.write_map(0, O.MAP_F, np.array([7, 7, 7, 8, 8, 8], np.uint32))

@@ -1,12 +1,12 @@
-"""The reference's run-level tests on the engine: a static clustering
-(HitLeiden.run_static) turned into a PartitionResult and judged with validate
-and check, as tests/contract/test_run_validate.rs:20-30,
+"""The reference's run-level tests on the device: run() (the static
+multilevel_leiden, HitLeiden.static_leiden) judged with validate and check, as
+tests/contract/test_run_validate.rs:20-30,
 tests/integration/test_throughput_equivalence.rs:20-34 and
-tests/property/test_partition_invariants.rs:20-31 do with run()."""
+tests/property/test_partition_invariants.rs:20-31 do."""
 import pytest
 
 import hit_leiden_amd as hl
-from hit_leiden_amd.validation import check, partition_result, validate
+from hit_leiden_amd.validation import check, validate
 from test_gpu_parity import say
 
 pytestmark = pytest.mark.gpu
@@ -15,12 +15,7 @@ DET, THR = hl.RunMode.DETERMINISTIC, hl.RunMode.THROUGHPUT
 
 
 def static_result(n, edges, mode):
-    e = hl.HitLeiden(hl.RunConfig(mode=mode, levels=1))
-    try:
-        passes, _ = e.run_static(hl.GraphInput.from_edges(n, edges))
-        return partition_result(e, iteration_count=passes)
-    finally:
-        e.close()
+    return hl.run(hl.GraphInput.from_edges(n, edges), hl.RunConfig(mode=mode)).partition
 
 
 def test_run_and_validate_contract():
@@ -41,7 +36,7 @@ def test_throughput_equivalence_bounds():
     assert v.equivalence_passed
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 49])
+@pytest.mark.parametrize("n", [0, 1, 2, 7, 49])
 def test_partition_len_matches_nodes(n):
     """Edgeless graphs: one community per vertex, modularity 0."""
     say(f"validate: {n} isolated vertices")
@@ -53,7 +48,7 @@ def test_partition_len_matches_nodes(n):
 
 
 def test_empty_vertex_set_rejected():
-    """The reference's property test also runs node_count 0; the engine
+    """run() takes node_count 0 (above); the incremental engine's load
     rejects an empty vertex set before any device work."""
     e = hl.HitLeiden(hl.RunConfig(mode=DET, levels=1))
     try:
