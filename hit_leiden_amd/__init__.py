@@ -139,6 +139,7 @@ def lib():
             "hl_read_hierarchy": ([vp, vp, C.c_uint64, C.c_int32], C.c_int),
             "hl_modularity": ([vp, C.c_int32, f64p], C.c_int),
             "hl_modularity_labels": ([vp, vp, C.c_uint64, f64p], C.c_int),
+            "hl_modularity_labels_noalloc": ([vp, vp, C.c_uint64, f64p], C.c_int),
             "hl_run_static": ([vp, C.c_uint32, C.c_uint64, vp, vp, vp, C.c_uint32, C.c_uint64, vp, vp, C.c_uint32,
                                u32p, C.POINTER(C.c_int64)], C.c_int),
             "hl_last_error": ([vp], C.c_char_p),

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -837,6 +838,35 @@ hl_status guard(const hl_graph* gc, F&& fn, bool profile = true) {
 
 constexpr uint64_t kLoadPassRecords = 1ull << 26;  // edges per pass of hl_load_graphs
 
+// ---- the workspace of hl_modularity_labels_noalloc ----
+// One file-scope device array per GPU, held until the process ends: the labels
+// (copied in), the per-label degree sums, the two partial sums and the hub-row
+// list of the modularity kernels.  Host code never uses the array's name as an
+// address: hl_create stores hipGetSymbolAddress's pointer per device.
+constexpr uint32_t kNoAllocVertices = 1u << 21;  // vertices of the engine's graph
+constexpr uint32_t kNoAllocLabels = 1u << 21;    // labels below this
+constexpr size_t kScrLab = 0;
+constexpr size_t kScrSig = kScrLab + size_t(kNoAllocVertices) * sizeof(uint32_t);
+constexpr size_t kScrAcc = kScrSig + size_t(kNoAllocLabels) * sizeof(double);
+constexpr size_t kScrHubs = kScrAcc + 256;
+constexpr size_t kScratchBytes = kScrHubs + (size_t(hl::kIntraHubList) + 1) * sizeof(uint32_t);
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// the current device's address of `scratch`, taken once per device
+hipError_t bind_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[dev]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch_dev[dev] = p;
+  return e;
+}
+
 // Labels a caller hands in (hl_write_partition, hl_restore_level) stay below
 // 2^31: 0xFFFFFFFF is the empty key of every hash table and the no-candidate
 // sentinel of the kernels, and a throughput level needs room above its labels
@@ -886,6 +916,7 @@ hl_status hl_create(const hl_config* cfg, hl_graph** out) {
   if (!g) return HL_OOM;
   hl_status st = guard(g, [&] {
     HL_CHECK(hipSetDevice(cfg->device));
+    HL_CHECK(bind_scratch(cfg->device));
     HL_CHECK(hipStreamCreateWithFlags(&g->e.s, hipStreamNonBlocking));
     g->e.cfg = *cfg;
     g->e.P = cfg->levels;
@@ -1488,6 +1519,45 @@ hl_status hl_run_static(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src
     if (rows) *rows = uint32_t(H.size());
     if (calls) *calls = c;
   });
+}
+
+hl_status hl_modularity_labels_noalloc(const hl_graph* g, const uint32_t* labels, uint64_t n, double* q) {
+  if (!g || !q || (n && !labels)) return HL_INVALID_INPUT;
+  return guard(
+      g,
+      [&] {
+        Engine& e = const_cast<Engine&>(g->e);
+        if (!e.loaded) throw hl::InputError("no graph loaded");
+        if (e.multi()) throw hl::InputError("several graphs per engine: use hl_modularity_graphs");
+        if (n != e.n0) throw hl::InputError("labels length differs from n");
+        if (n > kNoAllocVertices) throw hl::InputError("more than 2^21 vertices: use hl_modularity_labels");
+        uint32_t hm = 0;
+        for (uint64_t i = 0; i < n; ++i) hm = std::max(hm, labels[i]);
+        if (n && hm >= kNoAllocLabels) throw hl::InputError("labels must be below 2^21: use hl_modularity_labels");
+        HL_CHECK(hipSetDevice(e.cfg.device));
+        unsigned char* base = static_cast<unsigned char*>(g_scratch_dev[e.cfg.device]);
+        if (!base) throw hl::InputError("the device workspace is not bound");
+        uint32_t* dlab = reinterpret_cast<uint32_t*>(base + kScrLab);
+        double* sig = reinterpret_cast<double*>(base + kScrSig);
+        double* acc = reinterpret_cast<double*>(base + kScrAcc);
+        uint32_t* hubs = reinterpret_cast<uint32_t*>(base + kScrHubs);
+        const hl::Csr csr = e.L[0]->csr();
+        double t = 0.0;
+        HL_CHECK(hipMemcpyAsync(&t, csr.twice_m, sizeof(double), hipMemcpyDeviceToHost, e.s));
+        HL_CHECK(hipStreamSynchronize(e.s));
+        if (t == 0.0 || n == 0) {
+          *q = 0.0;
+          return;
+        }
+        HL_CHECK(hipMemcpyAsync(dlab, labels, n * sizeof(uint32_t), hipMemcpyHostToDevice, e.s));
+        hl::modularity_launch(csr, dlab, uint64_t(hm) + 1, e.L[0]->deg_bound, sig, acc, hubs,
+                              std::min<uint32_t>(uint32_t(n), hl::kIntraHubList), e.s);
+        double h[2] = {0.0, 0.0};
+        HL_CHECK(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, e.s));
+        HL_CHECK(hipStreamSynchronize(e.s));
+        *q = (h[0] - h[1]) / t;
+      },
+      false);
 }
 
 }  // extern "C"

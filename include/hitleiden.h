@@ -204,6 +204,16 @@ hl_status hl_run_static(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src
                         const double* w, uint32_t max_levels, uint64_t seed, uint32_t* labels, uint32_t* hier,
                         uint32_t hier_rows, uint32_t* rows, int64_t* calls);
 
+/* hl_modularity_labels without any device allocation: no counterpart in the
+   reference; for callers that cannot allocate (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  The labels, the
+   per-label sums and the partial sums live in a fixed device workspace of
+   about 24.3 MiB per GPU (a file-scope device array, bound by hl_create and
+   held until the process ends).  Limits: a one-graph engine of at most 2^21
+   vertices and labels below 2^21 (HL_INVALID_INPUT beyond them, before any
+   device work).  Every call on one GPU uses the same workspace: calls that
+   overlap on one GPU (several threads or handles) race. */
+hl_status hl_modularity_labels_noalloc(const hl_graph* g, const uint32_t* labels, uint64_t n, double* q);
 
 /* contingency_metrics (hit_leiden_incremental.rs:577-666, the benchmark's quality
    check) on the device for two labelings a, b of n vertices: out[0] NMI =
