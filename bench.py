@@ -97,6 +97,50 @@ def cpu_baseline(w, levels, gamma, budget_s, threads):
             "load_seconds": load_s, "modularity": q}
 
 
+def cpu_det_child(config, levels, gamma, budget_s, nbatches):
+    """The deterministic oracle (RunMode::Deterministic, the reference's default
+    mode, one thread) on the workload's graph and batches; prints one JSON line.
+    Runs in a child process that never touches the GPU."""
+    from oracle import hl_oracle as O
+    w, _ = make_workload(config, nbatches, 1)
+    st = O.OracleState(levels=levels, gamma=gamma, mode=O.DET, threads=1)
+    t0 = time.time()
+    st.load(w.n, w.u, w.v, w.w)
+    load_s = time.time() - t0
+    spent, edges, done = 0.0, 0, 0
+    for (u, v, a) in w.batches:
+        t = time.perf_counter()
+        st.apply(u, v, a)
+        spent += time.perf_counter() - t
+        edges += len(u)
+        done += 1
+        if spent >= budget_s:
+            break
+    print(json.dumps({"value": edges / spent if spent > 0 else 0.0, "unit": "edges/s", "cores": 1, "kind": "port",
+                      "sample": f"oracle deterministic mode ({levels} levels), one thread: load {load_s:.1f}s "
+                                f"untimed, then the first {done} batches ({edges} edges) timed",
+                      "load_seconds": round(load_s, 3)}))
+
+
+def cpu_det_variant(args, nbatches):
+    """cpu_det_child in a child process, bounded by --det-timeout."""
+    import subprocess
+    if args.config == "large":
+        return {"skipped": "the deterministic sweep is superlinear on R-MAT hubs: one thread loads an R-MAT "
+                           "scale-14 sample (16,384 vertices) in 31 s and a scale-18 one in more than 10 minutes"}
+    cmd = [sys.executable, os.path.abspath(__file__), "--det-child", args.config, "--levels", str(args.levels),
+           "--gamma", str(args.gamma), "--cpu-budget", str(args.det_budget), "--steps", str(nbatches),
+           "--warmup", "0"]
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=sys.stderr, timeout=args.det_timeout, text=True)
+    except subprocess.TimeoutExpired:
+        return {"skipped": f"the deterministic oracle did not finish within {args.det_timeout:.0f} s"}
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit {r.returncode}"}
+    return json.loads(lines[-1])
+
+
 def bench_batched(args):
     """The batched form: many independent graphs in one engine, one pass per set
     of batches, against the plain path (one engine per graph, one call per batch)
@@ -335,6 +379,10 @@ def main():
     ap.add_argument("--gamma", type=float, default=1.0)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle batch time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--det-budget", type=float, default=10.0, help="seconds of batch time of the deterministic "
+                    "one-thread CPU variant")
+    ap.add_argument("--det-timeout", type=float, default=180.0, help="wall-time bound of that variant")
+    ap.add_argument("--det-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched and served forms that the plain configurations report in 'extras'")
     ap.add_argument("--extra-timeout", type=float, default=400.0, help="seconds per extra child process")
@@ -354,6 +402,9 @@ def main():
     ap.add_argument("--command", default="", help="command line recorded in --rocprof-summary")
     args = ap.parse_args()
 
+    if args.det_child:
+        cpu_det_child(args.det_child, args.levels, args.gamma, args.cpu_budget, args.steps)
+        return
     if args.rocprof_summary:
         from hit_leiden_amd import profiling
         print(profiling.summary_text(args.rocprof_summary, args.command, pattern=args.kernel), end="")
@@ -502,6 +553,7 @@ def main():
         else:
             out["cpu_baseline"] = cpu_baseline(w, args.levels, args.gamma, args.cpu_budget, threads)
         out["cpu_baseline"]["cpu_model"] = cpu_model()
+        out["cpu_baseline"]["variants"] = {"deterministic_1_thread": cpu_det_variant(args, nb)}
     else:
         out["cpu_baseline"] = None
 

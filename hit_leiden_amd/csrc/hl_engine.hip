@@ -396,7 +396,7 @@ struct Engine {
     bool identity = false;
     if (par()) ensure_sub(lv);
     const u64 sn = cc_split(g, lv.s_cur.p, lv.K.p, lv.R.p, std::max<u64>(lv.s_bound, 1), &identity, sub_of(lv), rs,
-                            tmp, s, connected() ? lv.f.p : nullptr);
+                            tmp, s, connected() ? lv.f.p : nullptr, false, max_degree(lv));
     lv.s_bound = std::max(lv.s_bound, sn);
     if (multi())
       identity_graphs_to_R(lv.s_cur.p, lv.gid.p, n, G, part.p, gnid.p, lv.R.p, s);  // per graph
@@ -623,7 +623,8 @@ struct Engine {
     fill_u8(g0.K.p, 1, n, s);
     fill_u8(g0.R.p, 0, n, s);
     bool ident = false;
-    const u64 sn = cc_split(g0.csr(), sub, g0.K.p, g0.R.p, n, &ident, SubRef{}, rs, tmp, s, nullptr, true);
+    const u64 sn =
+        cc_split(g0.csr(), sub, g0.K.p, g0.R.p, n, &ident, SubRef{}, rs, tmp, s, nullptr, true, g0.deg_bound);
     rs.sizes.reserve(sn, s);
     rs.dS.reserve(sn, s);
     sub_recompute(sub, g0.k.p, n, rs.sizes.p, rs.dS.p, sn, s);

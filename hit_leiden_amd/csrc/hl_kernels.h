@@ -62,7 +62,7 @@ void sort_pairs_u32(const u32* kin, u32* kout, const u32* vin, u32* vout, u64 n,
 
 // ---- graph maintenance ----
 struct ApplyScratch {
-  DevBuf<u64> keys, keys_s, ukey, ukey_s, newcap, newoff, chunks, chunkbase, cnt;
+  DevBuf<u64> keys, keys_s, ukey, ukey_s, newcap, newoff, chunks, chunkbase, cnt, ucode, uscan;
   DevBuf<u32> idx, idx_s, runs, uidx, perm, segs, seg_row, seg_len, new_len, ins_pre, del_pre, seg_ins, seg_del;
   DevBuf<u8> heads, uop;
   DevBuf<double> uw;
@@ -159,6 +159,7 @@ struct RefineScratch {
   DevBuf<u32> mhead, mnext;
   DevBuf<u64> kkeys, kkeys_s;
   DevBuf<u32> rsorted;
+  DevBuf<u32> hubs;  // k_ws: hub rows listed for the piece kernel, then their count
   DevBuf<u32> sel, sl, vals;
 };
 // Subcommunity sizes and degrees a caller keeps current across batches
@@ -174,9 +175,11 @@ struct SubRef {
 // sets *identity when s is the identity map after it (hit_leiden.rs:1333-1339).
 // f != nullptr links two vertices only through edges inside their community
 // (f(u) = f(v)); nullptr links every same-subcommunity neighbour (hit_leiden.rs:1233-1240)
-// first_largest keeps the first largest component (the static path) instead of the last
+// first_largest keeps the first largest component (the static path) instead of the last;
+// maxdeg bounds the row lengths (rows above 4,096 entries are walked in pieces over the grid)
 u64 cc_split(const Csr& g, u32* s, const u8* K, u8* R, u64 s_bound, bool* identity, const SubRef& sub,
-             RefineScratch& rs, Temp& tmp, hipStream_t st, const u32* f = nullptr, bool first_largest = false);
+             RefineScratch& rs, Temp& tmp, hipStream_t st, const u32* f = nullptr, bool first_largest = false,
+             u32 maxdeg = 0);
 // size[s(v)] += 1 for the vertices [n0, n1) a level grew by (their k is 0)
 void sub_add_vertices(const u32* s, u32 n0, u32 n1, u32* size, hipStream_t st);
 // from scratch: size and deg over every vertex (arrays zero-filled first, cap entries)

@@ -412,23 +412,39 @@ __global__ void k_resolve_runs(const unsigned long long* nruns_dev, const u32* r
   }
 }
 
-__global__ void k_row_plan(u64 nseg, const u32* segs, u64 nupd, const u64* ukey_s, const u32* perm, const u8* uop,
+// Per sorted update: (is insert) << 32 | (is delete), for one scan that gives
+// every update its inserts and deletes before it (a row's updates are
+// contiguous, so the row's share is a difference of two scan entries).
+__global__ void k_upd_code(u64 nupd, const u32* perm, const u8* uop, u64* code) {
+  for (u64 t = gtid(); t < nupd; t += gstride()) {
+    const u8 op = uop[perm[t]];
+    code[t] = (u64(op == OP_INS) << 32) | u64(op == OP_DEL);
+  }
+}
+__global__ void k_upd_pre(u64 nupd, u64 nseg, const u32* segs, const u64* scan, u32* ins_pre, u32* del_pre) {
+  for (u64 t = gtid(); t < nupd; t += gstride()) {
+    u64 lo = 0, hi = nseg;  // the last segment starting at or before t
+    while (hi - lo > 1) {
+      const u64 mid = (lo + hi) >> 1;
+      if (segs[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    const u64 d = scan[t] - scan[segs[lo]];
+    ins_pre[t] = u32(d >> 32);
+    del_pre[t] = u32(d);
+  }
+}
+__global__ void k_row_plan(u64 nseg, const u32* segs, u64 nupd, const u64* ukey_s, const u64* scan,
                            const u32* row_len, u32* seg_row, u32* seg_len, u32* new_len, u32* seg_ins, u32* seg_del,
-                           u32* ins_pre, u32* del_pre, u64* newcap, u64* chunks, u64* old_sum) {
+                           u64* newcap, u64* chunks, u64* old_sum) {
   // old_sum[0] += old row lengths, old_sum[1] = max new length: per-thread
   // partials, one atomic each per workgroup
   unsigned long long osum = 0, omax = 0;
   for (u64 j = gtid(); j < nseg; j += gstride()) {
     u64 s = segs[j], e = (j + 1 < nseg) ? segs[j + 1] : nupd;
     u32 r = u32(ukey_s[s] >> 32);
-    u32 ins = 0, del = 0;
-    for (u64 t = s; t < e; ++t) {
-      u8 op = uop[perm[t]];
-      ins_pre[t] = ins;
-      del_pre[t] = del;
-      ins += (op == OP_INS);
-      del += (op == OP_DEL);
-    }
+    const u64 d = scan[e] - scan[s];
+    const u32 ins = u32(d >> 32), del = u32(d);
     u32 old = row_len[r];
     u32 nl = old + ins - del;
     seg_row[j] = r;
@@ -1001,18 +1017,56 @@ __device__ __forceinline__ void uf_union(u32* parent, u32 a, u32 b) {
 }
 // edges (v,u) with s(u) = s(v), u > v (and f(u) = f(v) when f is given): union;
 // roots end as the component minimum
+// A row above kIntraHub entries goes to the hub list (for a kernel that walks
+// it in pieces over the grid) instead of the one wavefront that reached it;
+// true when listed.  Every lane of the wavefront calls it for the same row.
+__device__ __forceinline__ bool wave_list_hub(u32 v, u32 deg, u32* hubs, u32* nhubs, u32 hub_cap) {
+  if (!hubs || deg <= kIntraHub) return false;
+  u32 slot = 0;
+  if ((threadIdx.x & 63) == 0) slot = atomicAdd(nhubs, 1u);
+  slot = __shfl(slot, 0, 64);
+  if (slot >= hub_cap) return false;
+  if ((threadIdx.x & 63) == 0) hubs[slot] = v;
+  return true;
+}
+// piece q of the listed hubs: its row and entry range; false when empty
+__device__ __forceinline__ bool hub_piece(const Csr& g, const u32* hubs, u64 q, u32 pieces, u32& v, u64& b,
+                                          u64& e) {
+  v = hubs[q / pieces];
+  b = (q % pieces) * u64(kIntraPiece);
+  const u32 deg = g.row_len[v];
+  if (b >= deg) return false;
+  e = min<u64>(deg, b + kIntraPiece);
+  return true;
+}
 __global__ __launch_bounds__(256) void k_uf_edges(Csr g, const u32* vcc, u64 m, const u32* s, const u32* f,
-                                                  u32* parent) {
+                                                  u32* parent, u32* hubs, u32* nhubs, u32 hub_cap) {
   const u32 lane = threadIdx.x & 63;
   const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
   for (u64 q = wave; q < m; q += nwaves) {
     u32 v = vcc[q];
     u64 off = g.row_off[v];
     u32 deg = g.row_len[v];
+    if (wave_list_hub(v, deg, hubs, nhubs, hub_cap)) continue;
     u32 sv = s[v];
     const u32 fv = f ? f[v] : 0;
     for (u32 j = lane; j < deg; j += 64) {
       u32 u = g.col[off + j];
+      if (u > v && s[u] == sv && (!f || f[u] == fv)) uf_union(parent, v, u);
+    }
+  }
+}
+__global__ __launch_bounds__(256) void k_uf_edges_hubs(Csr g, const u32* s, const u32* f, u32* parent,
+                                                       const u32* hubs, const u32* nhubs, u32 hub_cap, u32 pieces) {
+  const u32 nh = min(*nhubs, hub_cap);
+  for (u64 q = blockIdx.x; q < u64(nh) * pieces; q += gridDim.x) {
+    u32 v;
+    u64 b, e;
+    if (!hub_piece(g, hubs, q, pieces, v, b, e)) continue;
+    const u64 off = g.row_off[v];
+    const u32 sv = s[v], fv = f ? f[v] : 0;
+    for (u64 j = b + threadIdx.x; j < e; j += 256) {
+      const u32 u = g.col[off + j];
       if (u > v && s[u] == sv && (!f || f[u] == fv)) uf_union(parent, v, u);
     }
   }
@@ -1111,16 +1165,33 @@ __global__ void k_single(const u8* R, u32 n, const u32* s, const u32* sizes, u8*
 }
 // candidate subcommunities of singleton R vertices (hit_leiden.rs:1569-1586)
 __global__ __launch_bounds__(256) void k_cand_mark(Csr g, const u32* sl, u64 ns, const u32* f, const u32* s,
-                                                    u8* is_cand) {
+                                                    u8* is_cand, u32* hubs, u32* nhubs, u32 hub_cap) {
   const u32 lane = threadIdx.x & 63;
   const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
   for (u64 q = wave; q < ns; q += nwaves) {
     u32 v = sl[q];
     u64 off = g.row_off[v];
     u32 deg = g.row_len[v];
+    if (wave_list_hub(v, deg, hubs, nhubs, hub_cap)) continue;
     u32 fv = f[v], sv = s[v];
     for (u32 j = lane; j < deg; j += 64) {
       u32 u = g.col[off + j];
+      if (f[u] == fv && s[u] != sv) is_cand[s[u]] = 1;
+    }
+  }
+}
+__global__ __launch_bounds__(256) void k_cand_mark_hubs(Csr g, const u32* f, const u32* s, u8* is_cand,
+                                                        const u32* hubs, const u32* nhubs, u32 hub_cap,
+                                                        u32 pieces) {
+  const u32 nh = min(*nhubs, hub_cap);
+  for (u64 q = blockIdx.x; q < u64(nh) * pieces; q += gridDim.x) {
+    u32 v;
+    u64 b, e;
+    if (!hub_piece(g, hubs, q, pieces, v, b, e)) continue;
+    const u64 off = g.row_off[v];
+    const u32 fv = f[v], sv = s[v];
+    for (u64 j = b + threadIdx.x; j < e; j += 256) {
+      const u32 u = g.col[off + j];
       if (f[u] == fv && s[u] != sv) is_cand[s[u]] = 1;
     }
   }
@@ -1129,28 +1200,37 @@ __global__ void k_parent_idx(const u32* s, u32 n, const u8* is_cand, u32* pidx) 
   for (u64 v = gtid(); v < n; v += gstride())
     if (is_cand[s[v]]) atomicMin(&pidx[s[v]], u32(v));
 }
-// w(S, parent(S)) over all members (hit_leiden.rs:1617-1629), self-loops twice
-// Each wavefront takes 64 vertices and walks the rows of those in candidate
-// subcommunities together (rows can be long at the upper levels).
 // w(S, C) for the candidate subcommunities S (hit_leiden.rs:1617-1629: the
 // weight from S's members to their parent community C, a self-loop twice).
-// Each wavefront walks the rows of 64 consecutive vertices as one flat
-// sequence (the row of entry t by a binary search over the lanes' prefix) and
-// adds into an LDS table per subcommunity, flushed with one device atomic per
-// distinct S and workgroup pass: the members of one big subcommunity would
-// otherwise serialise on one address.  Rows above kWsWide entries are walked
-// by the whole wavefront, one after another.
+// Each wavefront takes 64 consecutive vertices.  Their rows (up to kWsWide
+// entries) are walked as one flat sequence, 64 entries per step (the row of
+// entry t by a binary search over the lanes' degree prefix); a segmented scan
+// over the lanes sums each row's part of the step, and the last lane of the
+// segment adds it to the row's total in LDS (one writer per row and step, so
+// no atomics).  Longer rows are summed by the whole wavefront, one after
+// another.  The 64 vertex totals are then combined per subcommunity in the
+// wavefront (one reduction per distinct S among them) before one keyed add per
+// S into the workgroup's LDS table, flushed with one device atomic per S and
+// pass: the members of a big subcommunity would otherwise serialise on one
+// address, in LDS as in HBM.  Rows above kIntraHub entries are listed for
+// k_ws_hubs (pieces over the grid: one wavefront would walk a hub of 10^6
+// entries for tens of ms while the rest of the grid idles); when the list is
+// full the wavefront walks them itself.
 constexpr u32 kWsTable = 2048, kWsWide = 1024;
 __global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u32* s, const u8* is_cand,
-                                             const u32* pidx, double* wS) {
+                                             const u32* pidx, double* wS, u32* hubs, u32* nhubs, u32 hub_cap,
+                                             u64 hub_max) {
   __shared__ u32 tk[kWsTable];
   __shared__ double tv[kWsTable];
+  __shared__ double rowacc[256];
   const u32 lane = threadIdx.x & 63;
+  double* racc = rowacc + (threadIdx.x & ~63u);
   for (u64 base0 = u64(blockIdx.x) * 256; base0 < n; base0 += u64(gridDim.x) * 256) {
     for (u32 i = threadIdx.x; i < kWsTable; i += 256) {
       tk[i] = kNone;
       tv[i] = 0.0;
     }
+    rowacc[threadIdx.x] = 0.0;
     __syncthreads();
     const u64 base = base0 + (threadIdx.x & ~63u);
     const u64 v = base + lane;
@@ -1163,6 +1243,13 @@ __global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u3
         C = f[pidx[x]];
         off = g.row_off[v];
         deg = g.row_len[v];
+      }
+    }
+    if (deg > kIntraHub && deg <= hub_max) {
+      const u32 slot = atomicAdd(nhubs, 1u);
+      if (slot < hub_cap) {
+        hubs[slot] = u32(v);
+        deg = 0;  // k_ws_hubs adds this row
       }
     }
     const bool wide = deg > kWsWide;
@@ -1181,15 +1268,26 @@ __global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u3
         if (pc <= t) r += step;
       }
       const u64 ro = __shfl((unsigned long long)off, r, 64);
-      const u32 rp = __shfl(pre, r, 64), rC = __shfl(C, r, 64), rs = __shfl(sv, r, 64);
+      const u32 rp = __shfl(pre, r, 64), rC = __shfl(C, r, 64);
+      double val = 0.0;
       if (t < total) {
         const u64 e = ro + (t - rp);
         const u32 u = g.col[e];
         if (f[u] == rC) {
           const double w = g.w[e];
-          lds_add_keyed(tk, tv, kWsTable, rs, (u == base + r) ? 2.0 * w : w);
+          val = (u == base + r) ? 2.0 * w : w;
         }
       }
+      // inclusive segmented sum over lanes of the same row (rows are
+      // contiguous in lane order)
+      for (u32 o = 1; o < 64; o <<= 1) {
+        const double y = __shfl_up(val, o, 64);
+        const u32 ry = __shfl_up(r, o, 64);
+        if (lane >= o && ry == r) val += y;
+      }
+      const u32 rn = __shfl_down(r, 1, 64);
+      const bool last = t < total && (lane == 63 || t + 1 >= total || rn != r);
+      if (last) racc[r] += val;
     }
     unsigned long long mask = __ballot(wide);
     while (mask) {
@@ -1197,7 +1295,7 @@ __global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u3
       mask &= mask - 1;
       const u64 bv = base + i;
       const u64 boff = __shfl((unsigned long long)off, i, 64);
-      const u32 bdeg = __shfl(deg, i, 64), bC = __shfl(C, i, 64), bs = __shfl(sv, i, 64);
+      const u32 bdeg = __shfl(deg, i, 64), bC = __shfl(C, i, 64);
       double acc = 0.0;
       for (u32 j = lane; j < bdeg; j += 64) {
         const u32 u = g.col[boff + j];
@@ -1207,12 +1305,52 @@ __global__ __launch_bounds__(256) void k_ws(Csr g, u32 n, const u32* f, const u3
         }
       }
       for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-      if (lane == 0 && acc != 0.0) lds_add_keyed(tk, tv, kWsTable, bs, acc);
+      if (lane == 0) racc[i] = acc;
+    }
+    __threadfence_block();
+    // the wavefront's vertex totals, combined per subcommunity
+    const double mine = racc[lane];
+    bool pending = sv != kNone && mine != 0.0;
+    unsigned long long pm = __ballot(pending);
+    while (pm) {
+      const u32 lead = u32(__ffsll((long long)pm) - 1);
+      const u32 key = __shfl(sv, lead, 64);
+      const bool member = pending && sv == key;
+      double y = member ? mine : 0.0;
+      for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
+      if (lane == lead) lds_add_keyed(tk, tv, kWsTable, key, y);
+      pending = pending && !member;
+      pm = __ballot(pending);
     }
     __syncthreads();
     for (u32 i = threadIdx.x; i < kWsTable; i += 256)
       if (tk[i] != kNone && tv[i] != 0.0) atomicAdd(&wS[tk[i]], tv[i]);
     __syncthreads();
+  }
+}
+// the listed hub rows of k_ws in pieces of kIntraPiece entries, one
+// workgroup per piece and one device atomic per piece
+__global__ __launch_bounds__(256) void k_ws_hubs(Csr g, const u32* f, const u32* s, const u32* pidx,
+                                                 const u32* hubs, const u32* nhubs, u32 hub_cap, u32 pieces,
+                                                 double* wS) {
+  const u32 nh = min(*nhubs, hub_cap);
+  for (u64 q = blockIdx.x; q < u64(nh) * pieces; q += gridDim.x) {
+    const u32 v = hubs[q / pieces];
+    const u64 b = (q % pieces) * u64(kIntraPiece);
+    const u32 deg = g.row_len[v];
+    if (b >= deg) continue;
+    const u64 end = min<u64>(deg, b + kIntraPiece), off = g.row_off[v];
+    const u32 S = s[v], C = f[pidx[S]];
+    double local = 0.0;
+    for (u64 j = b + threadIdx.x; j < end; j += 256) {
+      const u32 u = g.col[off + j];
+      if (f[u] == C) {
+        const double w = g.w[off + j];
+        local += (u == v) ? 2.0 * w : w;
+      }
+    }
+    const double t = block_sum(local);
+    if (threadIdx.x == 0 && t != 0.0) atomicAdd(&wS[S], t);
   }
 }
 __global__ void k_blocked(u64 sn, const u8* is_cand, const double* dS, const double* wS, const u32* pidx,
@@ -2403,9 +2541,18 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   a.chunks.reserve(ns, s);
   a.chunkbase.reserve(ns + 1, s);
   HL_CHECK(hipMemsetAsync(a.cnt.p, 0, 2 * sizeof(u64), s));
-  hipLaunchKernelGGL(k_row_plan, GRID(ns), ns, a.segs.p, u64(nupd), a.ukey_s.p, a.perm.p, a.uop.p, g.row_len,
-                     a.seg_row.p, a.seg_len.p, a.new_len.p, a.seg_ins.p, a.seg_del.p, a.ins_pre.p, a.del_pre.p,
-                     a.newcap.p, a.chunks.p, a.cnt.p);
+  // inserts and deletes before each update, and per row, from one scan (a
+  // coarse hub row can take millions of updates: no thread walks a row)
+  a.ucode.reserve(nupd, s);
+  a.uscan.reserve(u64(nupd) + 1, s);
+  hipLaunchKernelGGL(k_upd_code, GRID(nupd), u64(nupd), a.perm.p, a.uop.p, a.ucode.p);
+  HL_LAUNCH_CHECK();
+  exclusive_scan_u64_async(a.ucode.p, a.uscan.p, nupd, tmp, s);
+  hipLaunchKernelGGL(k_upd_pre, GRID(nupd), u64(nupd), ns, a.segs.p, a.uscan.p, a.ins_pre.p, a.del_pre.p);
+  HL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_row_plan, GRID(ns), ns, a.segs.p, u64(nupd), a.ukey_s.p, a.uscan.p, g.row_len,
+                     a.seg_row.p, a.seg_len.p, a.new_len.p, a.seg_ins.p, a.seg_del.p, a.newcap.p, a.chunks.p,
+                     a.cnt.p);
   HL_LAUNCH_CHECK();
   exclusive_scan_u64_async(a.newcap.p, a.newoff.p, ns, tmp, s);
   exclusive_scan_u64_async(a.chunks.p, a.chunkbase.p, ns, tmp, s);
@@ -2584,7 +2731,7 @@ void sub_recompute(const u32* sm, const double* k, u32 n, u32* size, double* deg
 }
 
 u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* identity, const SubRef& sub,
-             RefineScratch& rs, Temp& tmp, hipStream_t s, const u32* f, bool first_largest) {
+             RefineScratch& rs, Temp& tmp, hipStream_t s, const u32* f, bool first_largest, u32 maxdeg) {
   const u32 n = g.n;
   *identity = true;
   if (n == 0) return 1;
@@ -2625,8 +2772,21 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
   HL_PROF(P_CC_SPLIT, 16 * m, s);
   hipLaunchKernelGGL(k_uf_init, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p);
   HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_uf_edges, GRID(m * 64), g, rs.vcc.p, m, sm, f, rs.parent.p);
-  HL_LAUNCH_CHECK();
+  {
+    const u32 hub_cap = u32(std::min<u64>(m, kIntraHubList));
+    const u32 pieces = u32(std::max<u64>(1, (u64(maxdeg) + kIntraPiece - 1) / kIntraPiece));
+    const bool hub = maxdeg > kIntraHub;
+    rs.hubs.reserve(u64(hub_cap) + 1, s);
+    fill_u32(rs.hubs.p + hub_cap, 0, 1, s);
+    hipLaunchKernelGGL(k_uf_edges, GRID(m * 64), g, rs.vcc.p, m, sm, f, rs.parent.p, hub ? rs.hubs.p : nullptr,
+                       rs.hubs.p + hub_cap, hub_cap);
+    HL_LAUNCH_CHECK();
+    if (hub) {
+      hipLaunchKernelGGL(k_uf_edges_hubs, 2048, 256, 0, s, g, sm, f, rs.parent.p, rs.hubs.p, rs.hubs.p + hub_cap,
+                         hub_cap, pieces);
+      HL_LAUNCH_CHECK();
+    }
+  }
   hipLaunchKernelGGL(k_uf_compress, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p);
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_zero_best, GRID(m), rs.vcc.p, m, sm, rs.best64.p);
@@ -2684,14 +2844,40 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
   rs.wS.reserve(sn, s);
   fill_u8(rs.is_cand.p, 0, sn, s);
   HL_PROF(P_REFINE, 16ull * n, s);
-  hipLaunchKernelGGL(k_cand_mark, GRID(ns * 64), g, sl.p, ns, f, sm, rs.is_cand.p);
-  HL_LAUNCH_CHECK();
+  {
+    const u32 hub_cap = u32(std::min<u64>(ns, kIntraHubList));
+    const u32 pieces = u32(std::max<u64>(1, (u64(maxdeg) + kIntraPiece - 1) / kIntraPiece));
+    const bool hub = maxdeg > kIntraHub;
+    rs.hubs.reserve(u64(hub_cap) + 1, s);
+    fill_u32(rs.hubs.p + hub_cap, 0, 1, s);
+    hipLaunchKernelGGL(k_cand_mark, GRID(ns * 64), g, sl.p, ns, f, sm, rs.is_cand.p, hub ? rs.hubs.p : nullptr,
+                       rs.hubs.p + hub_cap, hub_cap);
+    HL_LAUNCH_CHECK();
+    if (hub) {
+      hipLaunchKernelGGL(k_cand_mark_hubs, 2048, 256, 0, s, g, f, sm, rs.is_cand.p, rs.hubs.p, rs.hubs.p + hub_cap,
+                         hub_cap, pieces);
+      HL_LAUNCH_CHECK();
+    }
+  }
   fill_u32(rs.parentIdx.p, kNone, sn, s);
   hipLaunchKernelGGL(k_parent_idx, GRID(n), sm, n, rs.is_cand.p, rs.parentIdx.p);
   HL_LAUNCH_CHECK();
   fill_f64(rs.wS.p, 0.0, sn, s);
-  hipLaunchKernelGGL(k_ws, blocks_for(n, 256, 4096), 256, 0, s, g, n, f, sm, rs.is_cand.p, rs.parentIdx.p, rs.wS.p);
-  HL_LAUNCH_CHECK();
+  {
+    const u32 hub_cap = std::min<u32>(n, kIntraHubList);
+    const u32 pieces = u32(std::max<u64>(1, (u64(maxdeg) + kIntraPiece - 1) / kIntraPiece));
+    const u64 hub_max = maxdeg > kIntraHub ? u64(pieces) * kIntraPiece : 0;
+    rs.hubs.reserve(u64(hub_cap) + 1, s);
+    fill_u32(rs.hubs.p + hub_cap, 0, 1, s);
+    hipLaunchKernelGGL(k_ws, blocks_for(n, 256, 4096), 256, 0, s, g, n, f, sm, rs.is_cand.p, rs.parentIdx.p,
+                       rs.wS.p, rs.hubs.p, rs.hubs.p + hub_cap, hub_cap, hub_max);
+    HL_LAUNCH_CHECK();
+    if (hub_max) {
+      hipLaunchKernelGGL(k_ws_hubs, 2048, 256, 0, s, g, f, sm, rs.parentIdx.p, rs.hubs.p, rs.hubs.p + hub_cap,
+                         hub_cap, pieces, rs.wS.p);
+      HL_LAUNCH_CHECK();
+    }
+  }
   const double* dpar = dpar_in;  // the maintained community totals, when the caller has them
   if (!dpar) {
     rs.dpar.reserve(comm_space, s);

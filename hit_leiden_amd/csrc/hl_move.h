@@ -84,6 +84,9 @@ struct MoveCtx {
   u32* ck = nullptr;
   u64 pool = 0;
   HubTables ht;
+  // fs[v] = (s[v] << 32) | f[v], built when the rounds of a level start and
+  // kept current by the moves: the neighbour marks read both in one gather
+  u64* fs = nullptr;
 };
 
 // throughput movement (parallel_frontier.rs:155-236): self-loops skipped,
@@ -188,11 +191,13 @@ struct MoveScratch {
   DevBuf<HtSlot> ht_slot;
   DevBuf<u64> ht_loc;
   DevBuf<unsigned long long> ht_used;
+  DevBuf<u64> fs;  // (s << 32) | f of the level in movement
   size_t target_cap = 0;
-  u32 epoch = 0;
+  u32 mark_n = 0;  // vertices the mark bitmap covers
   void ensure(u64 n, u64 comm_cap, hipStream_t s);
   void trim();  // release the candidate-sized buffers of the multi-kernel path
-  u32 next_epoch(hipStream_t s);
+  // clear the list-deduplication bitmap `mark` for a new round (a fill kernel)
+  void fresh_marks(hipStream_t s);
 };
 
 struct MoveStats {

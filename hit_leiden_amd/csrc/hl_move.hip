@@ -10,7 +10,7 @@
 //                  accepted moves, mark next-active neighbours and K
 //                  (throughput.rs:182-199) -- no host round trip inside a round
 // Large candidate sets (the initial pass) take the multi-kernel path.
-// Lists are deduplicated with an epoch-stamped mark array instead of O(n) flags.
+// Lists are deduplicated with a bitmap of claimed vertices, cleared per round.
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "hl_move.h"
@@ -152,11 +152,23 @@ __device__ __forceinline__ u32 lds_insert(u32* K, u32 cap, u32 key) {
     h = (h + 1) & (cap - 1);
   }
 }
-// Add v to a list once per epoch (a plain read first spares the atomic for
-// vertices already claimed, which hubs make common).
-__device__ __forceinline__ bool claim(u32* mark, u32 v, u32 epoch) {
-  if (mark[v] == epoch) return false;
-  return atomicExch(&mark[v], epoch) != epoch;
+// Add v to a list once per round: one bit per vertex in a bitmap cleared when
+// the round starts (n / 8 bytes, which stay in L2, where a per-vertex stamp
+// costs a line of HBM traffic per random test).  A plain read first spares the
+// atomic for vertices already claimed, which hubs make common.
+__device__ __forceinline__ bool claim(u32* bits, u32 v) {
+  u32* w = bits + (v >> 5);
+  const u32 b = 1u << (v & 31);
+  if (*w & b) return false;
+  return (atomicOr(w, b) & b) == 0;
+}
+__device__ __forceinline__ u64 ld_agent64(const u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// the f half of fs[v]
+__device__ __forceinline__ u32* fs_f(u64* fs, u32 v) { return reinterpret_cast<u32*>(fs + v); }
+__global__ void k_pack_fs(const u32* f, const u32* s, u32 n, u64* fs) {
+  for (u64 v = gtid(); v < n; v += gstride()) fs[v] = (u64(s[v]) << 32) | f[v];
 }
 // Keep the neighbour-community cache current after v moved to community to:
 // the entry for v in neighbour u's row (rows sorted by neighbour id) gets to.
@@ -241,7 +253,7 @@ __device__ __forceinline__ double tab_lookup(const u32* K, const double* V, u32 
 }
 
 // ------------------------------------------------------------------ seeding
-__global__ void k_seed2(const Rec* d, u64 k, const u32* f, const u32* s, u8* A, u8* K, u32* mark, u32 epoch,
+__global__ void k_seed2(const Rec* d, u64 k, const u32* f, const u32* s, u8* A, u8* K, u32* mark,
                         u32* list, u32* cnt) {
   for (u64 base = u64(blockIdx.x) * blockDim.x; base < k; base += gstride()) {  // uniform per workgroup
     const u64 i = base + threadIdx.x;
@@ -252,8 +264,8 @@ __global__ void k_seed2(const Rec* d, u64 k, const u32* f, const u32* s, u8* A, 
       const bool same_f = f[r.u] == f[r.v];
       const bool act = (r.w > 0.0 && !same_f) || (r.w < 0.0 && same_f);
       const bool kk = s[r.u] == s[r.v];
-      cu = act && claim(mark, r.u, epoch);
-      cv = act && r.v != r.u && claim(mark, r.v, epoch);
+      cu = act && claim(mark, r.u);
+      cv = act && r.v != r.u && claim(mark, r.v);
       if (act) {
         A[r.u] = 1;
         A[r.v] = 1;
@@ -271,7 +283,7 @@ __global__ void k_seed2(const Rec* d, u64 k, const u32* f, const u32* s, u8* A, 
 }
 // vertices of taking-part graphs without records at this level
 __global__ void k_activate_graphs(const u32* gid, u32 n, const u8* part, const u32* gcnt, u8* A, u32* mark,
-                                  u32 epoch, u32* list, u32* cnt) {
+                                  u32* list, u32* cnt) {
   for (u64 base = u64(blockIdx.x) * blockDim.x; base < n; base += gstride()) {
     const u64 v = base + threadIdx.x;
     bool want = false;
@@ -279,7 +291,7 @@ __global__ void k_activate_graphs(const u32* gid, u32 n, const u8* part, const u
       const u32 g = gid[v];
       if (part[g] && gcnt[g] == 0) {
         A[v] = 1;
-        want = claim(mark, u32(v), epoch);
+        want = claim(mark, u32(v));
       }
     }
     const u32 p = block_append(want, cnt);
@@ -1194,7 +1206,7 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
 __device__ __forceinline__ bool pri_less(u64 ka, u32 na, u64 kb, u32 nb) { return ka < kb || (ka == kb && na < nb); }
 
 __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32* minUto, u32* minUfrom,
-                                                     u32* minAto, u32* minAfrom, u32* mark, u32 epoch, u32* next,
+                                                     u32* minAto, u32* minAfrom, u32* mark, u32* next,
                                                      u32* next_cnt, u32* status, unsigned long long* dst,
                                                      u32* cur_cnt, u32* bucket_counts, FlatMarks flat) {
   __shared__ u64 skey[kDecSmall];
@@ -1261,6 +1273,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
       if (acc) {
         const double kv = m.k[node];
         st_agent(&m.f[node], to);
+        if (m.fs) st_agent(fs_f(m.fs, node), to);
         atomicAdd(&m.tot[from], -kv);
         atomicAdd(&m.tot[to], kv);
         m.B[node] = 1;
@@ -1446,6 +1459,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
       u32 v = snode[p];
       double kv = m.k[v];
       st_agent(&m.f[v], sto[p]);
+      if (m.fs) st_agent(fs_f(m.fs, v), sto[p]);
       atomicAdd(&m.tot[sfrom[p]], -kv);
       atomicAdd(&m.tot[sto[p]], kv);
       m.B[v] = 1;
@@ -1500,8 +1514,17 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
         const u64 e = m.g.row_off[v] + (t - pre[lo]);
         u = m.g.col[e];
         if (m.ht.bits && u != v) ht_move(m.ht, u, sfrom[lo], cto, m.g.w[e]);
-        if (ld_agent(&m.f[u]) != cto) want = claim(mark, u, epoch);
-        if (m.s[u] == m.s[v]) {
+        u32 fu, su;
+        if (m.fs) {
+          const u64 x = ld_agent64(m.fs + u);
+          fu = u32(x);
+          su = u32(x >> 32);
+        } else {
+          fu = ld_agent(&m.f[u]);
+          su = m.s[u];
+        }
+        if (fu != cto) want = claim(mark, u);
+        if (su == m.s[v]) {
           m.K[u] = 1;
           m.K[v] = 1;
         }
@@ -1543,7 +1566,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
 // with flat.pre the exclusive prefix of their list lengths (flat.pre[c] =
 // total); next-active vertices are staged per workgroup and appended with one
 // device atomic per flush.
-__global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u32* mark, u32 epoch, u32* next,
+__global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u32* mark, u32* next,
                                                     u32* next_cnt) {
   // kMarkPer entries per thread and pass, their gathers issued together so
   // that more of them are in flight
@@ -1599,8 +1622,14 @@ __global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u
     for (u32 r = 0; r < R; ++r) {
       fu[r] = su[r] = sv[r] = 0;
       if (ok[r]) {
-        fu[r] = m.f[uu[r]];
-        su[r] = m.s[uu[r]];
+        if (m.fs) {
+          const u64 x = m.fs[uu[r]];
+          fu[r] = u32(x);
+          su[r] = u32(x >> 32);
+        } else {
+          fu[r] = m.f[uu[r]];
+          su[r] = m.s[uu[r]];
+        }
         sv[r] = m.s[vv[r]];
       }
     }
@@ -1609,7 +1638,7 @@ __global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u
       if (!ok[r]) continue;
       const u32 u = uu[r], v = vv[r];
       if (m.ht.bits && flat.from && u != v) ht_move(m.ht, u, flat.from[seg[r]], ct[r], m.g.w[ee[r]]);
-      if (fu[r] != ct[r] && claim(mark, u, epoch)) sv_[atomicAdd(&scnt, 1u)] = u;
+      if (fu[r] != ct[r] && claim(mark, u)) sv_[atomicAdd(&scnt, 1u)] = u;
       if (su[r] == sv[r]) {
         m.K[u] = 1;
         m.K[v] = 1;
@@ -1829,6 +1858,7 @@ __global__ void k_apply4(u64 c, const u32* node, const u32* from, const u32* to,
       nd += m.g.row_len[v];
       double kv = m.k[v];
       m.f[v] = to[p];
+      if (m.fs) *fs_f(m.fs, v) = to[p];
       atomicAdd(&m.tot[from[p]], -kv);
       atomicAdd(&m.tot[to[p]], kv);
       m.B[v] = 1;
@@ -1880,12 +1910,8 @@ void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
     dstat.reserve(DS_COUNT, s);
     HL_CHECK(hipMemsetAsync(dstat.p, 0, DS_COUNT * sizeof(unsigned long long), s));
   }
-  if (mark.cap < n + 1) {
-    mark.reserve(n + 1, s, true);
-    // new entries must not equal a live epoch: zero the whole array, restart epochs
-    HL_CHECK(hipMemsetAsync(mark.p, 0, mark.cap * sizeof(u32), s));
-    epoch = 0;
-  }
+  mark.reserve(n / 32 + 2, s);
+  mark_n = u32(n);
   list_a.reserve(n + 1, s);
   list_b.reserve(n + 1, s);
   blists.reserve((kHtList + 1) * (n + 1), s);
@@ -1914,13 +1940,7 @@ void MoveScratch::trim() {
   k64b.release();
 }
 
-u32 MoveScratch::next_epoch(hipStream_t s) {
-  if (++epoch == 0xFFFFFFF0u) {
-    HL_CHECK(hipMemsetAsync(mark.p, 0, mark.cap * sizeof(u32), s));
-    epoch = 1;
-  }
-  return epoch;
-}
+void MoveScratch::fresh_marks(hipStream_t s) { fill_u32(mark.p, 0, u64(mark_n) / 32 + 1, s); }
 
 template <class P>
 static void propose_round(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u64 list_cap, u32 maxdeg,
@@ -2026,6 +2046,10 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   MoveCtx m = m_in;
   const u32 n = m.g.n;
   ms.ensure(n, m.comm_cap, s);
+  ms.fs.reserve(u64(n) + 1, s);
+  hipLaunchKernelGGL(k_pack_fs, fixed_grid(n, 256, 4096), 256, 0, s, m.f, m.s, n, ms.fs.p);
+  HL_LAUNCH_CHECK();
+  m.fs = ms.fs.p;
   if (m.pool) {
     ms.ck.reserve(m.pool, s);
     m.ck = ms.ck.p;
@@ -2049,15 +2073,15 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     hipLaunchKernelGGL(k_iota_list, fixed_grid(n, 256, 4096), 256, 0, s, cur, n, cnt_a);
     HL_LAUNCH_CHECK();
   } else {
-    const u32 ep = ms.next_epoch(s);
+    ms.fresh_marks(s);
     if (k) {
       HL_PROF(P_SEED, 24 * k, s);
-      hipLaunchKernelGGL(k_seed2, fixed_grid(k), 256, 0, s, d, k, m.f, m.s, m.A, m.K, ms.mark.p, ep, cur, cnt_a);
+      hipLaunchKernelGGL(k_seed2, fixed_grid(k), 256, 0, s, d, k, m.f, m.s, m.A, m.K, ms.mark.p, cur, cnt_a);
       HL_LAUNCH_CHECK();
     }
     if (m.gid) {
       hipLaunchKernelGGL(k_activate_graphs, fixed_grid(n, 256, 4096), 256, 0, s, m.gid, n, m.part, m.gcnt, m.A,
-                         ms.mark.p, ep, cur, cnt_a);
+                         ms.mark.p, cur, cnt_a);
       HL_LAUNCH_CHECK();
     }
   }
@@ -2087,13 +2111,13 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     u32* next_cnt = lcnt[ci ^ 1];
     propose_round(pol, m.g, cur, cur_cnt, host_cnt, maxdeg, ms, th, ms.dstat.p + DS_MOVE,
                   ms.dstat.p + DS_TRY_MOVE, s);
-    const u32 ep = ms.next_epoch(s);
+    ms.fresh_marks(s);
     CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
     const FlatMarks fm{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 12, ms.mfrom.p};
     {
       HL_PROF(P_DECOUPLE, 0, s);
       hipLaunchKernelGGL(k_dec_small, 1, 1024, 0, s, m, cand, ms.minUto.p, ms.minUfrom.p, ms.minAto.p,
-                         ms.minAfrom.p, ms.mark.p, ep, nxt, next_cnt, status, ms.dstat.p, cur_cnt, bcounts, fm);
+                         ms.minAfrom.p, ms.mark.p, nxt, next_cnt, status, ms.dstat.p, cur_cnt, bcounts, fm);
       HL_LAUNCH_CHECK();
     }
     if (m.ck) {
@@ -2106,7 +2130,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
       // grid (the kernel returns at once when the workgroup marked them all or
       // the candidates took the multi-kernel path: the count it reads is 0)
       HL_PROF(P_MARK_MOVED, 0, s);
-      hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fm, ms.mark.p, ep, nxt, next_cnt);
+      hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fm, ms.mark.p, nxt, next_cnt);
       HL_LAUNCH_CHECK();
     }
     u32 h[16];
@@ -2196,7 +2220,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
         }
         {
           HL_PROF(P_MARK_MOVED, 0, s);
-          hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fmm, ms.mark.p, ep, nxt, next_cnt);
+          hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fmm, ms.mark.p, nxt, next_cnt);
           HL_LAUNCH_CHECK();
         }
         if (m.ck) {
