@@ -101,9 +101,11 @@ class _Request(C.Structure):
 
 
 class ServeStats(C.Structure):
-    """hl_serve_stats: passes, records, largest pass (requests), wall and busy seconds."""
+    """hl_serve_stats: passes, records, largest pass (requests), wall and busy seconds,
+    the modelled device footprint of the largest pass and the passes free memory cut short."""
     _fields_ = [("passes", C.c_uint64), ("records", C.c_uint64), ("max_pass_requests", C.c_uint64),
-                ("seconds", C.c_double), ("busy_seconds", C.c_double)]
+                ("seconds", C.c_double), ("busy_seconds", C.c_double), ("max_pass_bytes", C.c_uint64),
+                ("footprint_cut_passes", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -148,7 +150,7 @@ def lib():
             "hl_profile_enable": ([vp, C.c_int32], C.c_int),
             "hl_profile_read": ([vp, C.c_int32, C.c_char_p, C.c_uint64, u64p, f64p, u64p], C.c_int),
             "hl_load_graphs": ([vp, C.c_uint32, vp, C.c_uint64, vp, vp, vp, C.POINTER(BatchStats)], C.c_int),
-            "hl_serve": ([vp, C.POINTER(_Request), C.c_uint64, C.c_uint64, C.c_int32, vp, vp, _EMIT, vp,
+            "hl_serve": ([vp, C.POINTER(_Request), C.c_uint64, C.c_uint64, C.c_int32, vp, vp, vp, _EMIT, vp,
                           C.POINTER(ServeStats)], C.c_int),
             "hl_graph_count": ([vp, u32p], C.c_int),
             "hl_compare_partitions": ([vp, vp, vp, C.c_uint64, C.c_uint32, C.c_uint32, vp], C.c_int),
@@ -411,7 +413,10 @@ class HitLeiden:
     def serve(self, requests, max_pass_records=1 << 20, emit=None, coalesce=False):
         """Open-loop serving of [(graph, GraphInput batch with graph-local ids, arrival seconds)]
         (hl_serve).  emit(request, labels) receives each request's community labels
-        right after its pass.  Returns (done seconds, pass index, ServeStats)."""
+        as soon as they are final: at the round of its pass where its graph's
+        frontier at the top level empties, or at the pass's end.  Returns (done
+        seconds, pass index, ServeStats); the seconds at which each request's
+        communities were final are left in self.last_emitted."""
         keep, reqs = [], (_Request * max(len(requests), 1))()
         for i, (gi, b, t) in enumerate(requests):
             src = np.ascontiguousarray(b.src, dtype=np.uint32)
@@ -422,6 +427,7 @@ class HitLeiden:
                                None if a is None else a.ctypes.data, float(t))
         done = np.zeros(max(len(requests), 1), dtype=np.float64)
         pass_of = np.zeros(max(len(requests), 1), dtype=np.uint64)
+        emitted = np.zeros(max(len(requests), 1), dtype=np.float64)
         st = ServeStats()
         cb = _EMIT(0)
         if emit is not None:
@@ -429,7 +435,8 @@ class HitLeiden:
                 emit(int(r), np.ctypeslib.as_array(labels, shape=(n,)).copy() if n else np.zeros(0, np.uint32))
             cb = _EMIT(_cb)
         self._check(lib().hl_serve(self._h, reqs, len(requests), int(max_pass_records), int(bool(coalesce)),
-                                   _ptr(done), _ptr(pass_of), cb, None, C.byref(st)))
+                                   _ptr(done), _ptr(pass_of), _ptr(emitted), cb, None, C.byref(st)))
+        self.last_emitted = emitted[:len(requests)]
         return done[:len(requests)], pass_of[:len(requests)], st
 
     def compare_partitions(self, a, b, a_largest=None, b_largest=None):

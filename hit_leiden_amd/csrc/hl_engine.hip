@@ -96,6 +96,8 @@ struct Engine {
   DevBuf<u8> heads;
   std::vector<std::unique_ptr<DevBuf<u8>>> Bp, Rp;
   StaticScratch ss;
+  // told the top level's frontier after every round (the served form's early emission)
+  const RoundHook* top_hook = nullptr;
 
   ~Engine() {
     L.clear();
@@ -379,7 +381,8 @@ struct Engine {
     }
     MoveStats ms_st;
     const u32 maxd = max_degree(lv);
-    u32 rounds = movement_throughput(m, d, k, maxd, max_rounds, mvs, th, tmp, &ms_st, s);
+    u32 rounds = movement_throughput(m, d, k, maxd, max_rounds, mvs, th, tmp, &ms_st, s,
+                                     p == P - 1 ? top_hook : nullptr);
     if (st && p < HL_MAX_LEVELS) {
       st->rounds[p] = rounds;
       st->evaluated[p] += ms_st.evaluated;
@@ -770,6 +773,14 @@ struct Engine {
       case HL_SET_REFINED: return lv.R;
       default: throw InputError("unknown set");
     }
+  }
+
+  // the communities of level-0 vertices [lo, lo + cnt) into out[0..cnt)
+  void compose_range(u32 lo, u32 cnt, u32* out) {
+    if (cnt == 0) return;
+    iota_u32(out, lo, cnt, s);
+    for (int p = 0; p < P - 1; ++p) gather_u32(out, L[p]->s_cur.p, cnt, s);
+    gather_u32(out, L[P - 1]->f.p, cnt, s);
   }
 
   // community labels f^{P-1} o s^{P-2} o .. o s^0 (or rows of the hierarchy) into lab
@@ -1323,8 +1334,84 @@ hl_status hl_modularity_graphs(const hl_graph* g, double* q, uint64_t cap) {
   });
 }
 
+namespace {
+
+// Device bytes a served pass holds per record, the model of its admission by
+// HBM footprint: the batch and its next-level copy (2 x 16), the level's delta
+// copy (16 per level), the row-update plan (canonical keys and their sort,
+// about 29 per record, and up to two updates per record with their sort,
+// prefixes and segment plan, about 230) and the aggregation's records of the
+// next level (2 x 16 per emitted record, bounded here by the record count),
+// rounded up.
+constexpr uint64_t kServeRecordBytes = 384;
+constexpr uint64_t kServeRecordBytesPerLevel = 16;
+constexpr uint64_t kServeHeadroom = 1ull << 30;  // device bytes a pass leaves free
+
+// The graphs of one served pass and their requests.  A graph's communities are
+// final once its frontier at the top level is empty (the top level's f and
+// the s maps below it change no more in the pass), so they are handed out at
+// that round, before the pass ends.
+struct ServePass {
+  Engine* e = nullptr;
+  const hl_request* reqs = nullptr;
+  hl_emit_fn emit = nullptr;
+  void* ctx = nullptr;
+  double* emitted = nullptr;
+  double t0 = 0.0;
+  std::vector<uint32_t> graphs;            // graphs of the pass
+  std::vector<std::vector<uint64_t>> of;   // their requests, in arrival order
+  std::vector<char> out;                   // handed out
+  uint32_t left = 0;
+  std::vector<uint32_t> lab, counts;
+  hl::DevBuf<uint32_t> dl, dcnt;
+
+  // t: when the graph's communities were final (seconds from the start)
+  void hand_out(size_t i, double t) {
+    Engine& en = *e;
+    const uint32_t gr = graphs[i], lo = en.goff[gr], ng = en.goff[gr + 1] - lo;
+    if (emit && ng) {
+      dl.reserve(ng, en.s);
+      en.compose_range(lo, ng, dl.p);
+      lab.resize(ng);
+      HL_CHECK(hipMemcpyAsync(lab.data(), dl.p, ng * sizeof(uint32_t), hipMemcpyDeviceToHost, en.s));
+      hl::stream_sync(en.s);
+    }
+    for (uint64_t r : of[i]) {
+      if (emitted) emitted[r] = t;
+      if (emit) emit(ctx, r, lab.data(), ng);
+    }
+    out[i] = 1;
+    --left;
+  }
+  void rest(double t) {
+    for (size_t i = 0; i < graphs.size(); ++i)
+      if (!out[i]) hand_out(i, t);
+  }
+  static void round(void* c, const uint32_t* list, uint32_t count) {
+    ServePass& sp = *static_cast<ServePass*>(c);
+    if (sp.left == 0) return;
+    const double t = now_s() - sp.t0;
+    if (count == 0) {
+      sp.rest(t);
+      return;
+    }
+    Engine& en = *sp.e;
+    sp.dcnt.reserve(uint64_t(en.G) + 1, en.s);
+    hl::list_graph_counts(list, count, en.L[en.P - 1]->gid.p, en.G, sp.dcnt.p, en.s);
+    sp.counts.resize(uint64_t(en.G) + 1);
+    HL_CHECK(hipMemcpyAsync(sp.counts.data(), sp.dcnt.p, (uint64_t(en.G) + 1) * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, en.s));
+    hl::stream_sync(en.s);
+    for (size_t i = 0; i < sp.graphs.size(); ++i)
+      if (!sp.out[i] && sp.counts[sp.graphs[i]] == 0) sp.hand_out(i, t);
+  }
+};
+
+}  // namespace
+
 hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t max_pass_records, int32_t coalesce,
-                   double* done, uint64_t* pass_of, hl_emit_fn emit, void* ctx, hl_serve_stats* stats) {
+                   double* done, uint64_t* pass_of, double* emitted, hl_emit_fn emit, void* ctx,
+                   hl_serve_stats* stats) {
   if (!g || (nreq && (!reqs || !done))) return HL_INVALID_INPUT;
   return guard(g, [&] {
     Engine& e = g->e;
@@ -1349,10 +1436,19 @@ hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t 
     std::vector<uint32_t> active;                     // graphs with waiting requests
     std::vector<char> is_active(e.G, 0);
     std::vector<hl::Rec> h;
-    std::vector<uint32_t> lab;
     hl_serve_stats st{};
     uint64_t next = 0, completed = 0, pass = 0;
+    const uint64_t per_record = kServeRecordBytes + kServeRecordBytesPerLevel * uint64_t(e.P);
     const double t0 = now_s();
+    ServePass sp;
+    sp.e = &e;
+    sp.reqs = reqs;
+    sp.emit = emit;
+    sp.ctx = ctx;
+    sp.emitted = emitted;
+    sp.t0 = t0;
+    const bool early = (emit || emitted) && e.mode == HL_MODE_THROUGHPUT;
+    hl::RoundHook hook{&ServePass::round, &sp};
     while (completed < nreq) {
       double now = now_s() - t0;
       while (next < nreq && reqs[order[next]].arrival <= now) {
@@ -1369,21 +1465,35 @@ hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t 
         if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
         continue;
       }
-      // admit the oldest waiting request of each graph, oldest first, within the record budget
+      // admission by HBM footprint: the records whose pass fits the device's
+      // free memory (less a headroom), within the record budget when one is set
+      size_t free_b = 0, total_b = 0;
+      HL_CHECK(hipMemGetInfo(&free_b, &total_b));
+      const uint64_t fit = free_b > kServeHeadroom ? (uint64_t(free_b) - kServeHeadroom) / per_record : 0;
+      const uint64_t cap = std::min<uint64_t>(max_pass_records, fit);
+      // admit the oldest waiting request of each graph, oldest first, within the cap
       std::vector<std::pair<double, uint32_t>> heads;
       for (uint32_t gr : active) heads.push_back({reqs[waiting[gr][head[gr]]].arrival, gr});
       std::sort(heads.begin(), heads.end());
       std::vector<uint64_t> admitted;  // grouped by graph, each graph's requests in arrival order
+      sp.graphs.clear();
+      sp.of.clear();
       uint64_t recs = 0;
-      bool full = false;
+      bool full = false, by_bytes = false;
       for (const auto& hg : heads) {
         const uint32_t gr = hg.second;
         for (size_t j = head[gr]; j < waiting[gr].size() && !full; ++j) {
           const uint64_t r = waiting[gr][j];
-          if (!admitted.empty() && (max_pass_records == 0 || recs + reqs[r].k > max_pass_records)) {
+          if (!admitted.empty() && (max_pass_records == 0 || recs + reqs[r].k > cap)) {
             full = true;
+            by_bytes = max_pass_records != 0 && recs + reqs[r].k > fit;
             break;
           }
+          if (sp.graphs.empty() || sp.graphs.back() != gr) {
+            sp.graphs.push_back(gr);
+            sp.of.emplace_back();
+          }
+          sp.of.back().push_back(r);
           admitted.push_back(r);
           recs += reqs[r].k;
           if (!coalesce) break;  // one request of this graph per pass
@@ -1397,28 +1507,26 @@ hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t 
         const uint32_t base = e.goff[q.graph];
         for (uint64_t i = 0; i < q.k; ++i) h[o++] = hl::Rec{base + q.src[i], base + q.dst[i], q.alpha ? q.alpha[i] : 1.0};
       }
+      sp.out.assign(sp.graphs.size(), 0);
+      sp.left = uint32_t(sp.graphs.size());
       const double tp = now_s();
       e.dcur.reserve(recs + 1, e.s);
       if (recs) HL_CHECK(hipMemcpyAsync(e.dcur.p, h.data(), recs * sizeof(hl::Rec), hipMemcpyHostToDevice, e.s));
-      e.run_batch(recs, nullptr);
+      e.top_hook = early ? &hook : nullptr;
+      try {
+        e.run_batch(recs, nullptr);
+      } catch (...) {
+        e.top_hook = nullptr;
+        throw;
+      }
+      e.top_hook = nullptr;
       const double tdone = now_s() - t0;
       st.busy_seconds += now_s() - tp;
-      if (emit) {
-        e.lab.reserve(e.n0 + 1, e.s);
-        e.compose(e.P - 1, true, e.lab.p);
-      }
+      if (emit || emitted) sp.rest(tdone);  // graphs whose top frontier never emptied: at the pass end
       for (uint64_t r : admitted) {
         const uint32_t gr = reqs[r].graph;
         done[r] = tdone;
         if (pass_of) pass_of[r] = pass;
-        if (emit) {
-          const uint32_t lo = e.goff[gr], ng = e.goff[gr + 1] - lo;
-          lab.resize(std::max<uint32_t>(ng, 1));
-          if (ng)
-            HL_CHECK(hipMemcpyAsync(lab.data(), e.lab.p + lo, ng * sizeof(uint32_t), hipMemcpyDeviceToHost, e.s));
-          hl::stream_sync(e.s);
-          emit(ctx, r, lab.data(), ng);
-        }
         if (++head[gr] == waiting[gr].size()) {
           waiting[gr].clear();
           head[gr] = 0;
@@ -1430,6 +1538,8 @@ hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t 
       completed += admitted.size();
       st.records += recs;
       st.max_pass_requests = std::max<uint64_t>(st.max_pass_requests, admitted.size());
+      st.max_pass_bytes = std::max<uint64_t>(st.max_pass_bytes, recs * per_record);
+      if (by_bytes) ++st.footprint_cut_passes;
       ++pass;
     }
     st.passes = pass;

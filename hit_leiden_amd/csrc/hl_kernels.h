@@ -279,6 +279,8 @@ void compare_partitions(const u32* a, const u32* b, u64 n, u32 la, u32 lb, doubl
 void gid_from_offsets(const u32* offs, u32 G, u32 n, u32* gid, hipStream_t s);
 // cnt[g] = records of graph g (cnt has G + 1 entries)
 void graph_counts(const Rec* d, u64 k, const u32* gid, u32 G, u32* cnt, hipStream_t s);
+// cnt[g] = vertices of list[0..c) in graph g (cnt has G + 1 entries)
+void list_graph_counts(const u32* list, u64 c, const u32* gid, u32 G, u32* cnt, hipStream_t s);
 void part_from_counts(const u32* cnt, u32 G, u8* part, hipStream_t s);
 // R = all vertices of every taking-part graph whose s is the identity (hit_leiden.rs:1333-1339, per graph)
 void identity_graphs_to_R(const u32* s, const u32* gid, u32 n, u32 G, const u8* part, u8* nid, u8* R,

@@ -1626,14 +1626,42 @@ __global__ void k_gather_rec(const u32* sel, u64 c, const Rec* in, Rec* out) {
 }
 
 // ------------------------------------------------------------------ def_update / modularity
+// tot[key] += x for the lanes that want it: the lanes of the first few
+// distinct keys of the wavefront are summed first (one atomic per key: a pull
+// moves whole communities, and atomics on one address serialise), the rest
+// add their own
+__device__ __forceinline__ void wave_add_keyed(double* tot, u32 key, double x, bool want) {
+  const u32 lane = threadIdx.x & 63;
+  unsigned long long act = __ballot(want);
+  for (int it = 0; it < 4 && act; ++it) {
+    const int leader = __ffsll((long long)act) - 1;
+    const u32 kk = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(want && key == kk) & act;
+    double y = ((same >> lane) & 1ull) ? x : 0.0;
+    for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
+    if (lane == u32(leader)) atomicAdd(&tot[kk], y);
+    act &= ~same;
+  }
+  if ((act >> lane) & 1ull) atomicAdd(&tot[key], x);
+}
 __global__ void k_def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, double* tot, const double* k) {
-  for (u64 v = gtid(); v < n; v += gstride()) {
-    if (B && !B[v]) continue;
-    const u32 old = fp[v], nw = fq[sp[v]];
-    fp[v] = nw;
-    if (tot && old != nw && k[v] != 0.0) {  // keep the maintained community totals current
-      atomicAdd(&tot[old], -k[v]);
-      atomicAdd(&tot[nw], k[v]);
+  for (u64 base = gtid() - (threadIdx.x & 63); base < n; base += gstride()) {
+    const u64 v = base + (threadIdx.x & 63);
+    u32 old = 0, nw = 0;
+    double kv = 0.0;
+    bool ch = false;
+    if (v < n && (!B || B[v])) {
+      old = fp[v];
+      nw = fq[sp[v]];
+      fp[v] = nw;
+      if (tot && old != nw) {
+        kv = k[v];
+        ch = kv != 0.0;
+      }
+    }
+    if (tot) {  // keep the maintained community totals current
+      wave_add_keyed(tot, old, -kv, ch);
+      wave_add_keyed(tot, nw, kv, ch);
     }
   }
 }
@@ -1827,6 +1855,14 @@ __global__ void k_graph_counts(const Rec* d, u64 k, const u32* gid, u32* cnt) {
   for (u64 base = gtid() - (threadIdx.x & 63); base < k; base += gstride()) {
     const u64 i = base + (threadIdx.x & 63);
     wave_count_keyed(cnt, i < k ? gid[d[i].u] : 0u, i < k);
+  }
+}
+// vertices of a list per graph (the lists hold a graph's vertices mostly
+// together: one atomic per graph and wavefront)
+__global__ void k_list_graph_counts(const u32* list, u64 c, const u32* gid, u32* cnt) {
+  for (u64 base = gtid() - (threadIdx.x & 63); base < c; base += gstride()) {
+    const u64 i = base + (threadIdx.x & 63);
+    wave_count_keyed(cnt, i < c ? gid[list[i]] : 0u, i < c);
   }
 }
 __global__ void k_part_from_counts(const u32* cnt, u32 G, u8* part) {
@@ -3145,6 +3181,11 @@ void gid_from_offsets(const u32* offs, u32 G, u32 n, u32* gid, hipStream_t s) {
 void graph_counts(const Rec* d, u64 k, const u32* gid, u32 G, u32* cnt, hipStream_t s) {
   fill_u32(cnt, 0, u64(G) + 1, s);
   if (k) hipLaunchKernelGGL(k_graph_counts, GRID(k), d, k, gid, cnt);
+  HL_LAUNCH_CHECK();
+}
+void list_graph_counts(const u32* list, u64 c, const u32* gid, u32 G, u32* cnt, hipStream_t s) {
+  fill_u32(cnt, 0, u64(G) + 1, s);
+  if (c) hipLaunchKernelGGL(k_list_graph_counts, GRID(c), list, c, gid, cnt);
   HL_LAUNCH_CHECK();
 }
 void part_from_counts(const u32* cnt, u32 G, u8* part, hipStream_t s) {

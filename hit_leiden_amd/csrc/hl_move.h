@@ -205,10 +205,19 @@ struct MoveStats {
   u64 decide_rounds = 0;  // priority rounds of the multi-kernel decoupling
 };
 
+// Told the vertices active next after the seeding and after every round (a
+// device list and its count; count 0 once the frontier is empty).  The rounds
+// wait for it, so it may read the level's maps.
+struct RoundHook {
+  void (*fn)(void* ctx, const u32* list, u32 count) = nullptr;
+  void* ctx = nullptr;
+};
+
 // Synchronous rounds (<= max_rounds) of the throughput movement at one level;
 // seeds A/K from the delta (hit_leiden.rs:1037-1057).  Returns the round count.
 u32 movement_throughput(const MoveCtx& m, const Rec* d, u64 k, u32 maxdeg, int max_rounds, MoveScratch& ms,
-                        const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s);
+                        const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s,
+                        const RoundHook* hook = nullptr);
 
 // Snapshot proposals of the singleton vertices `singles`, with the
 // connected-targets rule when keep_targets; updates s in place.

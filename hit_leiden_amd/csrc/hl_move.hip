@@ -2042,7 +2042,7 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
 }
 
 u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, int max_rounds, MoveScratch& ms,
-                        const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s) {
+                        const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s, const RoundHook* hook) {
   MoveCtx m = m_in;
   const u32 n = m.g.n;
   ms.ensure(n, m.comm_cap, s);
@@ -2104,6 +2104,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   d2h_async(&host_cnt, cnt_a, sizeof(u32), s);
   stream_sync(s);
   if (st) st->active = host_cnt;
+  if (hook && hook->fn) hook->fn(hook->ctx, cur, host_cnt);
   u32 rounds = 0;
   while (host_cnt > 0 && int(rounds) < max_rounds) {
     if (st) st->evaluated += host_cnt;
@@ -2238,6 +2239,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     }
     ++rounds;
     if (st) st->moved_rounds += accepted;
+    if (hook && hook->fn) hook->fn(hook->ctx, nxt, accepted ? next_host : 0u);
     if (accepted == 0) break;  // throughput.rs:177-180 (every accepted gain is > 0)
     std::swap(cur, nxt);
     ci ^= 1;

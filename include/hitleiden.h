@@ -246,13 +246,16 @@ hl_status hl_modularity_graphs(const hl_graph* g, double* q, uint64_t cap);
    a several-graph engine: request r becomes visible at reqs[r].arrival seconds
    after the call starts; each pass admits, in arrival order, the oldest
    waiting request of every graph (requests of one graph stay in order, one per
-   pass) while the pass holds at most max_pass_records delta records (the
-   device footprint of a pass grows with them; 0 admits one request per pass,
-   the plain one-batch-per-call path), applies them together in one
-   hit_leiden pass and, after it, reports each request's completion time
-   (done[r], seconds from the start) and pass index, and hands the graph's
-   community labels (graph-local ids, the top hierarchy row) to emit when emit
-   is not NULL.  With coalesce != 0 a pass takes every waiting request of a
+   pass) while the pass holds at most max_pass_records delta records (0 admits
+   one request per pass, the plain one-batch-per-call path) and its modelled
+   device footprint (384 + 16 P bytes per record) fits the device's free
+   memory less 1 GiB, applies them together in one hit_leiden pass and reports
+   each request's completion time (done[r], seconds from the start) and pass
+   index.  A graph's communities (graph-local ids, the top hierarchy row) are
+   final once its frontier at the top level is empty, and are handed out at
+   that round of the pass (emitted[r], seconds from the start, <= done[r]):
+   to emit when emit is not NULL.  Graphs whose top frontier never empties are
+   handed out when the pass ends.  With coalesce != 0 a pass takes every waiting request of a
    graph (in arrival order, within the budget) and applies their records as
    one batch of that graph; their results are the graph after that batch.
    Each graph's result is the one it gets alone: the graphs of one pass do not
@@ -272,12 +275,15 @@ typedef struct {
   uint64_t max_pass_requests;
   double seconds;       /* wall time from the start to the last completion */
   double busy_seconds;  /* time spent inside passes */
+  uint64_t max_pass_bytes;        /* modelled device footprint of the largest pass */
+  uint64_t footprint_cut_passes;  /* passes the free device memory cut short */
 } hl_serve_stats;
 
 typedef void (*hl_emit_fn)(void* ctx, uint64_t request, const uint32_t* labels, uint32_t n);
 
 hl_status hl_serve(hl_graph* g, const hl_request* reqs, uint64_t nreq, uint64_t max_pass_records, int32_t coalesce,
-                   double* done, uint64_t* pass_of, hl_emit_fn emit, void* ctx, hl_serve_stats* stats);
+                   double* done, uint64_t* pass_of, double* emitted, hl_emit_fn emit, void* ctx,
+                   hl_serve_stats* stats);
 
 /* Per-kernel device timing for roofline accounting.  No counterpart in the
    reference (it profiles out of process with perf/samply, profiling.rs:74-152).

@@ -123,6 +123,30 @@ def test_graphs_match_the_oracle():
     u.close()
 
 
+def test_served_emission_before_the_pass_ends():
+    """One pass over graphs whose top frontiers empty at different rounds: a
+    graph with a one-edge request is handed out at the round its frontier
+    empties, while a graph with a large request keeps the pass going; the
+    early labels equal the final ones."""
+    graphs = make_graphs(8, 41)
+    u = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=P))
+    u.load_graphs(graphs)
+    big = max(range(len(graphs)), key=lambda i: graphs[i].node_count)
+    reqs = [(big, insert_batch(graphs[big], 800, 5, dels=40), 0.0)]
+    reqs += [(g, insert_batch(graphs[g], 1, 9 + g), 0.0) for g in range(len(graphs)) if g != big]
+    got = {}
+    done, pass_of, st = u.serve(reqs, max_pass_records=1 << 20, coalesce=True,
+                                emit=lambda r, lab: got.__setitem__(r, lab))
+    em = u.last_emitted
+    say(f"one pass: {st.passes}; emitted/done ms: " +
+        ", ".join(f"{1e3 * a:.2f}/{1e3 * b:.2f}" for a, b in zip(em, done)))
+    assert st.passes == 1 and np.all(em <= done)
+    assert np.sum(em < done) >= 1
+    for i, (g, _, _) in enumerate(reqs):
+        np.testing.assert_array_equal(_first_seen(got[i]), u.graph_hierarchy(g)[-1], err_msg=f"graph {g}")
+    u.close()
+
+
 def test_graph_set_errors():
     graphs = make_graphs(3, 23)
     u = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=2))
@@ -161,8 +185,13 @@ def test_served_results_match_alone(budget, coalesce):
     got = {}
     done, pass_of, st = u.serve(reqs, max_pass_records=budget, coalesce=coalesce,
                                 emit=lambda r, lab: got.__setitem__(r, lab))
-    say(f"budget={budget} coalesce={coalesce} passes={st.passes} max_pass_requests={st.max_pass_requests}")
+    em = u.last_emitted
+    say(f"budget={budget} coalesce={coalesce} passes={st.passes} max_pass_requests={st.max_pass_requests} "
+        f"handed out before the pass end: {int(np.sum(em < done))} of {len(reqs)}")
     assert len(got) == len(reqs) and np.all(done > 0)
+    # the labels are handed out when they are final, never after the pass ends
+    assert np.all(em > 0) and np.all(em <= done)
+    assert st.max_pass_bytes > 0 and st.footprint_cut_passes == 0
     if budget == 0:
         assert st.passes == len(reqs)
     else:
