@@ -154,6 +154,18 @@ def bench_batched(args):
     sum_n, sum_m = sum(w.n for w in ws), sum(w.m for w in ws)
     log(f"[bench] batched: {len(ws)} graphs, {sum_n} vertices, {sum_m} edges, gen {time.time() - t0:.1f}s")
     cfg = hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=args.levels, resolution=args.gamma)
+    shape = {}
+    if args.autotune:
+        # launch shapes picked on a sample of the graphs (their first step's deltas), then used for all
+        from hit_leiden_amd import autotune
+        t0 = time.time()
+        pick = np.random.default_rng(5).choice(len(ws), size=min(args.tune_sample, len(ws)), replace=False)
+        tuned = autotune.tune_graphs([graphs[i] for i in pick],
+                                     {j: hl.GraphInput(ws[i].n, *ws[i].batches[0]) for j, i in enumerate(pick)},
+                                     levels=args.levels, gamma=args.gamma, log=log)
+        shape = tuned._asdict()
+        cfg = tuned.config(mode=hl.RunMode.THROUGHPUT, levels=args.levels, resolution=args.gamma)
+        log(f"[bench] autotune picked {shape} on {len(pick)} graphs in {time.time() - t0:.1f}s")
     u = hl.HitLeiden(cfg)
     t0 = time.time()
     u.load_graphs(graphs)
@@ -216,7 +228,8 @@ def bench_batched(args):
                                "mu=0.25, unit weights; per graph and step one batch of k new edges, k log-uniform "
                                "in [16, 2000]; a step is one pass over every graph's batch",
                    "graphs": len(ws), "vertices": sum_n, "edges": sum_m, "records_per_step": edges // args.steps,
-                   "levels": args.levels, "mode": "throughput", "gamma": args.gamma},
+                   "levels": args.levels, "mode": "throughput", "gamma": args.gamma,
+                   "launch_shape": shape or "defaults"},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "plain_edges_per_s": round(plain, 1), "plain_sample_graphs": len(sample),
         "plain_p50_ms_per_graph": round(float(np.percentile(per_graph, 50)) * 1e3, 4),
@@ -358,6 +371,8 @@ def run_extra(args, config, extra):
             "plain_sample_graphs", "plain", "runs", "steps", "warmup")
     out = {k: j[k] for k in keep if k in j}
     out["workload"] = j["config"]["workload"]
+    if "launch_shape" in j["config"]:
+        out["launch_shape"] = j["config"]["launch_shape"]
     out["command"] = " ".join(["python", "bench.py"] + cmd[2:])
     out["wall_seconds"] = round(time.time() - t0, 1)
     return out
@@ -396,6 +411,7 @@ def main():
     ap.add_argument("--dump-profile", default=None)
     ap.add_argument("--key-cache", action="store_true",
                     help="cache each row entry's neighbour community for the throughput movement")
+    ap.add_argument("--tune-sample", type=int, default=128, help="graphs the batched form's autotuning times")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the degree-bucket thresholds on the warmup batches first")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -442,15 +458,16 @@ def main():
         raise SystemExit("not enough batches generated")
 
     mode = hl.RunMode.THROUGHPUT if args.mode == "throughput" else hl.RunMode.DETERMINISTIC
-    small_deg = wave_deg = 0
+    shape = {}
     if args.autotune and mode == hl.RunMode.THROUGHPUT:
         from hit_leiden_amd import autotune
         t0 = time.time()
-        small_deg, wave_deg = autotune.tune(hl.GraphInput(w.n, w.u, w.v, w.w), w.batches[:args.warmup],
-                                            levels=args.levels, gamma=args.gamma, log=log)
-        log(f"[bench] autotune picked small_degree={small_deg} wave_degree={wave_deg} in {time.time() - t0:.1f}s")
-    g = hl.HitLeiden(hl.RunConfig(resolution=args.gamma, mode=mode, levels=args.levels, small_degree=small_deg,
-                                  wave_degree=wave_deg, key_cache=int(args.key_cache)))
+        tuned = autotune.tune(hl.GraphInput(w.n, w.u, w.v, w.w), w.batches[:args.warmup], levels=args.levels,
+                              gamma=args.gamma, log=log)
+        shape = tuned._asdict()
+        log(f"[bench] autotune picked {shape} in {time.time() - t0:.1f}s")
+    g = hl.HitLeiden(hl.RunConfig(resolution=args.gamma, mode=mode, levels=args.levels, key_cache=int(args.key_cache),
+                                  **shape))
     t0 = time.time()
     ls = g.load(hl.GraphInput(w.n, w.u, w.v, w.w))
     load_s = time.time() - t0
@@ -524,8 +541,9 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(float(lat_ms.mean()), 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": dict(desc, levels=args.levels, mode=args.mode, gamma=args.gamma,
-                       small_degree=small_deg or 16, wave_degree=wave_deg or 512,
-                       key_cache=int(args.key_cache)),
+                       small_degree=shape.get("small_degree", 16), wave_degree=shape.get("wave_degree", 512),
+                       marks_block=shape.get("marks_block", 256), marks_grid=shape.get("marks_grid", 1024),
+                       autotuned=bool(shape), key_cache=int(args.key_cache)),
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "path_gbs": round(bytes_alg / total / 1e9, 3), "path_hbm_frac": round(bytes_alg / total / 1e9 / HBM_PEAK_GBS, 6),
         "modularity_after_load": round(q0, 6), "modularity_after_steps": round(q1, 6), "load_seconds": round(load_s, 3),
@@ -564,7 +582,8 @@ def main():
     if not args.no_extras and args.config in ("large", "mid"):
         # the batched and served forms against their plain one-batch-per-call GPU paths
         out["extras"] = {
-            "batched": run_extra(args, "batched", ["--steps", "2", "--warmup", "1", "--plain-sample", "32"]),
+            "batched": run_extra(args, "batched", ["--steps", "2", "--warmup", "1", "--plain-sample", "32",
+                                                   "--autotune", "--tune-sample", "96"]),
             "served": run_extra(args, "served", ["--rates", "2000", "--served-seconds", "2",
                                                  "--plain-requests", "300"]),
         }

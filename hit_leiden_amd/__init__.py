@@ -64,7 +64,8 @@ class _Config(C.Structure):
     _fields_ = [("gamma", C.c_double), ("mode", C.c_int32), ("levels", C.c_int32), ("max_rounds", C.c_int32),
                 ("device", C.c_int32), ("small_degree", C.c_uint32), ("wave_degree", C.c_uint32),
                 ("load_pass_records", C.c_uint32), ("key_cache", C.c_uint32), ("hub_tables", C.c_uint32),
-                ("decide_rounds", C.c_uint32), ("refine_reference", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("decide_rounds", C.c_uint32), ("refine_reference", C.c_uint32), ("marks_block", C.c_uint32),
+                ("marks_grid", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
 
 class BatchStats(C.Structure):
@@ -197,6 +198,8 @@ class RunConfig:
     # communities, f^p = f^{p+1} o s^p); True applies the reference's own refinement and
     # def_update rules (throughput.rs:278-287, hit_leiden.rs:1233-1240,1733-1793)
     refine_reference: bool = False
+    marks_block: int = 0  # workgroup size of the grid-wide neighbour marks (0: 256; 128, 256, 512)
+    marks_grid: int = 0  # their workgroups (0: 1,024)
 
 
 class GraphInput:
@@ -249,6 +252,8 @@ class HitLeiden:
         c.hub_tables = 0 if self.config.hub_tables else 1
         c.decide_rounds = int(self.config.decide_rounds)
         c.refine_reference = 1 if self.config.refine_reference else 0
+        c.marks_block = int(self.config.marks_block)
+        c.marks_grid = int(self.config.marks_grid)
         h = C.c_void_p()
         st = lib().hl_create(C.byref(c), C.byref(h))
         if st != HL_OK:

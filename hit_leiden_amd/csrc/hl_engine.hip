@@ -98,6 +98,13 @@ struct Engine {
   StaticScratch ss;
   // told the top level's frontier after every round (the served form's early emission)
   const RoundHook* top_hook = nullptr;
+  // the refined maps below the top level await their pull
+  bool g_stale = false;
+  void materialize_g() {
+    if (!g_stale) return;
+    g_stale = false;
+    def_update_maps(true);
+  }
 
   ~Engine() {
     L.clear();
@@ -440,7 +447,12 @@ struct Engine {
         double* tot = nullptr;
         if (!use_g) {
           lv.lab_bound = std::max(lv.lab_bound, up.lab_bound);  // f^p takes f^{p+1} labels
-          if (par() && lv.tot_valid) {
+          if (connected()) {
+            // the full pull renames nearly every label (level p's ranks back to
+            // the labels of the level above): the totals are rebuilt in one pass
+            // when the next rounds start, not kept by two atomics per vertex
+            lv.tot_valid = false;
+          } else if (par() && lv.tot_valid) {
             ensure_totals(lv);
             tot = lv.tot.p;
           }
@@ -542,7 +554,13 @@ struct Engine {
       }
     }
     def_update_maps(false);
-    def_update_maps(true);
+    // a consistent hierarchy's refined maps are g^p = g^{p+1} o s^p at every
+    // level below the top, which no step of a pass reads: they are pulled when
+    // something reads them (materialize_g) instead of after every pass
+    if (connected())
+      g_stale = true;
+    else
+      def_update_maps(true);
     unsigned long long dnow[DS_COUNT];
     const bool have_dstat = mvs.dstat.p != nullptr;
     if (have_dstat)
@@ -711,6 +729,7 @@ struct Engine {
 
   void reset(u32 n, u64 m_hint) {
     L.clear();
+    g_stale = false;
     for (int p = 0; p < P; ++p) L.emplace_back(new Level());
     n0 = n;
     Level& l0 = *L[0];
@@ -757,6 +776,7 @@ struct Engine {
     return *L[p];
   }
   DevBuf<u32>& map_of(Level& lv, int which) {
+    if (which == HL_MAP_G) materialize_g();
     switch (which) {
       case HL_MAP_F: return lv.f;
       case HL_MAP_G: return lv.g;
@@ -909,7 +929,9 @@ hl_status hl_create(const hl_config* cfg, hl_graph** out) {
   *out = nullptr;
   if (cfg->levels < 1 || cfg->levels > HL_MAX_LEVELS || !(cfg->gamma > 0.0) || !std::isfinite(cfg->gamma) ||
       (cfg->mode != HL_MODE_DETERMINISTIC && cfg->mode != HL_MODE_THROUGHPUT) || cfg->key_cache > 1 ||
-      cfg->hub_tables > 1 || cfg->decide_rounds > (1u << 20) || cfg->refine_reference > 1) {
+      cfg->hub_tables > 1 || cfg->decide_rounds > (1u << 20) || cfg->refine_reference > 1 ||
+      (cfg->marks_block != 0 && cfg->marks_block != 128 && cfg->marks_block != 256 && cfg->marks_block != 512) ||
+      cfg->marks_grid > 65536) {
     g_create_err = "invalid config";
     return HL_INVALID_INPUT;
   }
@@ -936,6 +958,8 @@ hl_status hl_create(const hl_config* cfg, hl_graph** out) {
     if (cfg->small_degree) g->e.th.small_deg = cfg->small_degree;
     if (cfg->wave_degree) g->e.th.wave_deg = cfg->wave_degree;
     if (cfg->decide_rounds) g->e.th.dec_rounds = cfg->decide_rounds;
+    if (cfg->marks_block) g->e.th.marks_block = cfg->marks_block;
+    if (cfg->marks_grid) g->e.th.marks_grid = cfg->marks_grid;
   });
   if (st != HL_OK) {
     g_create_err = g->e.err;
@@ -1309,8 +1333,10 @@ hl_status hl_modularity(const hl_graph* g, int32_t which, double* q) {
     e.lab.reserve(e.n0 + 1, e.s);
     if (which == HL_PART_COMMUNITY)
       e.compose(e.P - 1, true, e.lab.p);
-    else if (which == HL_PART_REFINED)
+    else if (which == HL_PART_REFINED) {
+      e.materialize_g();
       HL_CHECK(hipMemcpyAsync(e.lab.p, e.L[0]->g.p, e.n0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, e.s));
+    }
     else if (which == HL_PART_F0)
       HL_CHECK(hipMemcpyAsync(e.lab.p, e.L[0]->f.p, e.n0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, e.s));
     else

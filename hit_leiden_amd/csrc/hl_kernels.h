@@ -68,6 +68,8 @@ struct ApplyScratch {
   DevBuf<double> uw;
   DevBuf<u32> ucount;
   DevBuf<u64> c1;
+  DevBuf<u32> dhubs;  // rows whose degree is summed in pieces, then their count
+  DevBuf<double> dacc;
 };
 
 struct LevelCsrMut {
@@ -135,6 +137,8 @@ struct Thresholds {
   u32 small_deg = 16;
   u32 wave_deg = 512;
   u32 dec_rounds = 48;  // priority rounds of the multi-kernel decoupling before the in-order settle
+  u32 marks_block = 256;  // workgroup size of the grid-wide neighbour marks (128, 256 or 512)
+  u32 marks_grid = 1024;  // their workgroups
 };
 // deterministic sweep (hit_leiden.rs:1108-1188) in one workgroup
 struct DetMoveScratch {
@@ -215,7 +219,7 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
 // f[v] <- the rank of f[v] among the labels in use (all below L), so every
 // label ends below their count; tot (tot_cap entries, or null) moves with its
 // labels into tot2 (reserved and zeroed here), which the caller swaps in.
-// flags holds the presence flags and ranks (2 (L + 1) entries).
+// flags holds the presence bitmap, its word popcounts and their prefix.
 void rank_labels(u32* f, u32 n, u64 L, const double* tot, u64 tot_cap, DevBuf<u64>& flags, DevBuf<double>& tot2,
                  Temp& tmp, hipStream_t s);
 // f^p(v) = f^q(s^p(v)) for v in B (every v when B is null), keeping tot current

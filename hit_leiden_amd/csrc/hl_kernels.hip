@@ -582,15 +582,37 @@ __global__ void k_row_degree(u64 nseg, const u32* seg_row, LevelCsrMut g, Totals
   }
 }
 
+// the new degree of row r (acc), and the totals' share of its change
+__device__ __forceinline__ void set_degree(const LevelCsrMut& g, const TotalsRef& tr, u32 r, double acc) {
+  const double old = g.k[r];
+  g.k[r] = acc;
+  if (acc != old) {
+    if (tr.tot) atomicAdd(&tr.tot[tr.f[r]], acc - old);
+    if (tr.dS) atomicAdd(&tr.dS[tr.s[r]], acc - old);
+    if (tr.tm) atomicAdd(&tr.tm[tr.gid[r]], acc - old);
+  }
+}
 // Throughput mode: one wavefront per updated row, a fixed-shape tree sum
-// (the deterministic mode keeps the sequential row fold above).
-__global__ __launch_bounds__(256) void k_row_degree_wave(u64 nseg, const u32* seg_row, LevelCsrMut g, TotalsRef tr) {
+// (the deterministic mode keeps the sequential row fold above).  Rows above
+// kIntraHub entries go to the hub list when there is one: their sums come from
+// pieces over the grid (k_row_degree_hubs), not from the one wavefront.
+__global__ __launch_bounds__(256) void k_row_degree_wave(u64 nseg, const u32* seg_row, LevelCsrMut g, TotalsRef tr,
+                                                         u32* hubs, u32* nhubs, u32 hub_cap) {
   const u32 lane = threadIdx.x & 63;
   const u64 wave = gtid() >> 6, nwaves = gstride() >> 6;
   for (u64 j = wave; j < nseg; j += nwaves) {
     const u32 r = seg_row[j];
     const u64 off = g.row_off[r];
     const u32 len = g.row_len[r];
+    if (hubs && len > kIntraHub) {
+      u32 slot = 0;
+      if (lane == 0) slot = atomicAdd(nhubs, 1u);
+      slot = __shfl(slot, 0, 64);
+      if (slot < hub_cap) {
+        if (lane == 0) hubs[slot] = r;
+        continue;
+      }
+    }
     double acc = 0.0;
     for (u32 i = lane; i < len; i += 64) {
       const double w = g.w[off + i];
@@ -598,16 +620,34 @@ __global__ __launch_bounds__(256) void k_row_degree_wave(u64 nseg, const u32* se
       if (g.col[off + i] == r) acc += w;
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (lane == 0) {
-      const double old = g.k[r];
-      g.k[r] = acc;
-      if (acc != old) {
-        if (tr.tot) atomicAdd(&tr.tot[tr.f[r]], acc - old);
-        if (tr.dS) atomicAdd(&tr.dS[tr.s[r]], acc - old);
-        if (tr.tm) atomicAdd(&tr.tm[tr.gid[r]], acc - old);
-      }
-    }
+    if (lane == 0) set_degree(g, tr, r, acc);
   }
+}
+// the listed rows in pieces of kIntraPiece entries, one workgroup per piece:
+// each piece's sum into the row's accumulator
+__global__ __launch_bounds__(256) void k_row_degree_hubs(LevelCsrMut g, const u32* hubs, const u32* nhubs,
+                                                         u32 hub_cap, u32 pieces, double* acc) {
+  const u32 nh = min(*nhubs, hub_cap);
+  for (u64 q = blockIdx.x; q < u64(nh) * pieces; q += gridDim.x) {
+    const u32 r = hubs[q / pieces];
+    const u64 b = (q % pieces) * u64(kIntraPiece);
+    const u32 len = g.row_len[r];
+    if (b >= len) continue;
+    const u64 e = min<u64>(len, b + kIntraPiece), off = g.row_off[r];
+    double local = 0.0;
+    for (u64 i = b + threadIdx.x; i < e; i += 256) {
+      const double w = g.w[off + i];
+      local += w;
+      if (g.col[off + i] == r) local += w;
+    }
+    const double t = block_sum(local);
+    if (threadIdx.x == 0) atomicAdd(&acc[q / pieces], t);
+  }
+}
+__global__ void k_row_degree_fin(LevelCsrMut g, TotalsRef tr, const u32* hubs, const u32* nhubs, u32 hub_cap,
+                                 const double* acc) {
+  const u32 nh = min(*nhubs, hub_cap);
+  for (u64 i = gtid(); i < nh; i += gstride()) set_degree(g, tr, hubs[i], acc[i]);
 }
 
 __global__ void k_copy_rows(LevelCsrMut g, const u64* newoff, u32* col2, double* w2) {
@@ -2617,11 +2657,29 @@ void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, co
   hipLaunchKernelGGL(k_row_commit, GRID(ns), ns, a.seg_row.p, a.new_len.p, a.newoff.p, pool_base, g);
   HL_LAUNCH_CHECK();
   HL_PROF(P_ROW_DEGREE, 12 * plan.need + 12 * ns, s);
-  if (det)
+  if (det) {
     hipLaunchKernelGGL(k_row_degree, GRID(ns), ns, a.seg_row.p, g, tr);
-  else
-    hipLaunchKernelGGL(k_row_degree_wave, blocks_for(ns * 64, 256, 4096), 256, 0, s, ns, a.seg_row.p, g, tr);
-  HL_LAUNCH_CHECK();
+    HL_LAUNCH_CHECK();
+  } else if (plan.max_row > kIntraHub) {
+    const u32 hub_cap = u32(std::min<u64>(ns, kIntraHubList));
+    const u32 pieces = u32((plan.max_row + kIntraPiece - 1) / kIntraPiece);
+    a.dhubs.reserve(u64(hub_cap) + 1, s);
+    a.dacc.reserve(hub_cap, s);
+    fill_u32(a.dhubs.p + hub_cap, 0, 1, s);
+    fill_f64(a.dacc.p, 0.0, hub_cap, s);
+    hipLaunchKernelGGL(k_row_degree_wave, blocks_for(ns * 64, 256, 4096), 256, 0, s, ns, a.seg_row.p, g, tr,
+                       a.dhubs.p, a.dhubs.p + hub_cap, hub_cap);
+    HL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_row_degree_hubs, 2048, 256, 0, s, g, a.dhubs.p, a.dhubs.p + hub_cap, hub_cap, pieces,
+                       a.dacc.p);
+    HL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_row_degree_fin, GRID(hub_cap), g, tr, a.dhubs.p, a.dhubs.p + hub_cap, hub_cap, a.dacc.p);
+    HL_LAUNCH_CHECK();
+  } else {
+    hipLaunchKernelGGL(k_row_degree_wave, blocks_for(ns * 64, 256, 4096), 256, 0, s, ns, a.seg_row.p, g, tr,
+                       nullptr, nullptr, 0u);
+    HL_LAUNCH_CHECK();
+  }
 }
 
 u64 compact_plan(const LevelCsrMut& g, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s) {
@@ -3107,37 +3165,75 @@ double modularity(const Csr& g, const u32* lab, u32 maxdeg, DevBuf<double>& sig,
   return (h[0] - h[1]) / t;
 }
 
-__global__ void k_mark_labels(const u32* f, u32 n, u64* flag) {
+// Presence of the labels in use: a byte per label set by plain stores (no
+// atomics: every member of a community writes the same 1), then packed into
+// a bitmap, 32 labels a word, with the words' popcounts.  The rank of label l
+// is the popcount of the words before it (an exclusive scan over L / 32
+// words) plus the bits below it in its word.
+__global__ void k_mark_label_bytes(const u32* f, u32 n, u8* flag) {
   for (u64 v = gtid(); v < n; v += gstride()) flag[f[v]] = 1;
 }
-__global__ void k_move_totals(const u64* flag, const u64* rank, u64 L, const double* tot, double* tot2) {
-  for (u64 l = gtid(); l < L; l += gstride())
-    if (flag[l]) tot2[rank[l]] = tot[l];
+__global__ void k_pack_label_bits(const u8* flag, u64 W, u32* bits, u64* pc) {
+  for (u64 i = gtid(); i < W; i += gstride()) {
+    const uint4* q = reinterpret_cast<const uint4*>(flag + 32 * i);
+    const uint4 a = q[0], b = q[1];
+    const u32 wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    u32 m = 0;
+#pragma unroll
+    for (u32 j = 0; j < 32; ++j) m |= u32((wd[j >> 2] >> (8 * (j & 3))) & 1u) << j;
+    bits[i] = m;
+    pc[i] = u64(__popc(m));
+  }
 }
-__global__ void k_apply_ranks(u32* f, u32 n, const u64* rank) {
-  for (u64 v = gtid(); v < n; v += gstride()) f[v] = u32(rank[f[v]]);
+__device__ __forceinline__ u32 label_rank(const u32* bits, const u64* pre, u32 l) {
+  return u32(pre[l >> 5]) + u32(__popc(bits[l >> 5] & ((1u << (l & 31)) - 1u)));
+}
+// the totals of the labels in use to their ranks (one thread per bitmap word)
+__global__ void k_move_totals(const u32* bits, const u64* pre, u64 W, const double* tot, u64 tot_cap,
+                              double* tot2) {
+  for (u64 i = gtid(); i < W; i += gstride()) {
+    u32 w = bits[i];
+    u64 r = pre[i];
+    while (w) {
+      const u32 b = u32(__ffs(w) - 1);
+      w &= w - 1;
+      const u64 l = i * 32 + b;
+      tot2[r++] = l < tot_cap ? tot[l] : 0.0;
+    }
+  }
+}
+__global__ void k_apply_ranks(u32* f, u32 n, const u32* bits, const u64* pre) {
+  for (u64 v = gtid(); v < n; v += gstride()) f[v] = label_rank(bits, pre, f[v]);
 }
 
 void rank_labels(u32* f, u32 n, u64 L, const double* tot, u64 tot_cap, DevBuf<u64>& flags, DevBuf<double>& tot2,
                  Temp& tmp, hipStream_t s) {
   if (n == 0) return;
-  HL_PROF(P_RELABEL, 4ull * n + 24ull * L + 12ull * n, s);
-  flags.reserve(2 * (L + 1), s);
-  u64* flag = flags.p;
-  u64* rank = flags.p + (L + 1);
-  hipLaunchKernelGGL(k_fill_u64, GRID(L + 1), flag, u64(0), L + 1);
+  const u64 W = L / 32 + 1;
+  HL_PROF(P_RELABEL, 4ull * n + 20ull * W + 16ull * n, s);
+  // flags: a byte per label (32 W bytes, as 4 W u64), the bitmap (W words, as
+  // W / 2 + 1 u64), the word popcounts (W) and their prefix (W + 1)
+  const u64 fw = 4 * W, bw = W / 2 + 1;
+  flags.reserve(fw + bw + 2 * W + 1, s);
+  u8* flag = reinterpret_cast<u8*>(flags.p);
+  u32* bits = reinterpret_cast<u32*>(flags.p + fw);
+  u64* pc = flags.p + fw + bw;
+  u64* pre = pc + W;
+  hipLaunchKernelGGL(k_fill_u64, GRID(fw), flags.p, u64(0), fw);
   HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_mark_labels, GRID(n), f, n, flag);
+  hipLaunchKernelGGL(k_mark_label_bytes, GRID(n), f, n, flag);
   HL_LAUNCH_CHECK();
-  exclusive_scan_u64_async(flag, rank, L, tmp, s);
+  hipLaunchKernelGGL(k_pack_label_bits, GRID(W), flag, W, bits, pc);
+  HL_LAUNCH_CHECK();
+  exclusive_scan_u64_async(pc, pre, W, tmp, s);
   if (tot) {
     tot2.reserve(tot_cap, s, false, 1.0);
     hipLaunchKernelGGL(k_fill_f64, GRID(tot2.cap), tot2.p, 0.0, u64(tot2.cap));
     HL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_move_totals, GRID(L), flag, rank, L, tot, tot2.p);
+    hipLaunchKernelGGL(k_move_totals, GRID(W), bits, pre, W, tot, tot_cap, tot2.p);
     HL_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_apply_ranks, GRID(n), f, n, rank);
+  hipLaunchKernelGGL(k_apply_ranks, GRID(n), f, n, bits, pre);
   HL_LAUNCH_CHECK();
 }
 

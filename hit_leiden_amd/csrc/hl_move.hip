@@ -45,7 +45,6 @@ constexpr u32 kLdsCap = 8192;       // its table slots (big_cap(kLdsDeg))
 constexpr u32 kBigDeg = 1u << 18;   // workgroup-per-vertex with an HBM table up to this degree
 constexpr u64 kMarksInBlock = 2048;  // neighbour entries marked inside the decoupling workgroup
 constexpr unsigned kMarksBlocks = 1024;
-constexpr u32 kMarkStage = 2048;
 constexpr u32 kMarkPer = 4;  // marks: row entries per thread and pass of k_marks_flat
 constexpr u64 kBigTableBytes = 3ull << 30;  // HBM hub tables of the overflow proposals
 constexpr u32 kOvfBlocks = 64;              // workgroups (and HBM tables) for the overflow list
@@ -1566,12 +1565,14 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
 // with flat.pre the exclusive prefix of their list lengths (flat.pre[c] =
 // total); next-active vertices are staged per workgroup and appended with one
 // device atomic per flush.
-__global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u32* mark, u32* next,
-                                                    u32* next_cnt) {
+template <u32 BS>
+__global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u32* mark, u32* next,
+                                                   u32* next_cnt) {
   // kMarkPer entries per thread and pass, their gathers issued together so
-  // that more of them are in flight
+  // that more of them are in flight; the stage holds two passes of marks
   constexpr u32 R = kMarkPer;
-  __shared__ u32 sv_[kMarkStage];
+  constexpr u32 kStageN = 2 * R * BS;
+  __shared__ u32 sv_[kStageN];
   __shared__ u32 scnt, sbase, span[2];
   const u32 c = *flat.c;
   if (c == 0) return;
@@ -1645,7 +1646,7 @@ __global__ __launch_bounds__(256) void k_marks_flat(MoveCtx m, FlatMarks flat, u
       }
     }
     __syncthreads();
-    if (scnt > kMarkStage - R * blockDim.x) {
+    if (scnt > kStageN - R * BS) {
       if (threadIdx.x == 0) sbase = atomicAdd(next_cnt, scnt);
       __syncthreads();
       for (u32 i = threadIdx.x; i < scnt; i += blockDim.x) next[sbase + i] = sv_[i];
@@ -2041,6 +2042,19 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
   }
 }
 
+// the grid-wide neighbour marks at the workgroup shape the thresholds name
+static void launch_marks(const Thresholds& th, const MoveCtx& m, const FlatMarks& fm, u32* mark, u32* nxt,
+                         u32* next_cnt, hipStream_t s) {
+  const unsigned grid = th.marks_grid ? th.marks_grid : kMarksBlocks;
+  if (th.marks_block == 128)
+    hipLaunchKernelGGL(k_marks_flat<128>, grid, 128, 0, s, m, fm, mark, nxt, next_cnt);
+  else if (th.marks_block == 512)
+    hipLaunchKernelGGL(k_marks_flat<512>, grid, 512, 0, s, m, fm, mark, nxt, next_cnt);
+  else
+    hipLaunchKernelGGL(k_marks_flat<256>, grid, 256, 0, s, m, fm, mark, nxt, next_cnt);
+  HL_LAUNCH_CHECK();
+}
+
 u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, int max_rounds, MoveScratch& ms,
                         const Thresholds& th, Temp& tmp, MoveStats* st, hipStream_t s, const RoundHook* hook) {
   MoveCtx m = m_in;
@@ -2131,7 +2145,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
       // grid (the kernel returns at once when the workgroup marked them all or
       // the candidates took the multi-kernel path: the count it reads is 0)
       HL_PROF(P_MARK_MOVED, 0, s);
-      hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fm, ms.mark.p, nxt, next_cnt);
+      launch_marks(th, m, fm, ms.mark.p, nxt, next_cnt, s);
       HL_LAUNCH_CHECK();
     }
     u32 h[16];
@@ -2221,7 +2235,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
         }
         {
           HL_PROF(P_MARK_MOVED, 0, s);
-          hipLaunchKernelGGL(k_marks_flat, kMarksBlocks, 256, 0, s, m, fmm, ms.mark.p, nxt, next_cnt);
+          launch_marks(th, m, fmm, ms.mark.p, nxt, next_cnt, s);
           HL_LAUNCH_CHECK();
         }
         if (m.ck) {

@@ -89,6 +89,9 @@ typedef struct {
                                 vertices only through edges inside their community, def_update
                                 pulls every vertex below the top level); 1 = the reference's
                                 rules (throughput.rs:278-287, hit_leiden.rs:1233-1240,1733-1793) */
+  uint32_t marks_block;  /* throughput movement: workgroup size of the grid-wide neighbour marks
+                            (rows of the moved vertices), 0 = 256, else 128, 256 or 512 */
+  uint32_t marks_grid;   /* their workgroups, 0 = 1,024, at most 65,536 */
   uint32_t reserved[1];
 } hl_config;
 

@@ -383,6 +383,33 @@ def test_autotune_picks_a_candidate():
     batches = [random_batch(n, u, v, None, 200, 6 + i) for i in range(2)]
     best = autotune.tune(hl.GraphInput(n, u, v), batches, levels=2)
     assert best[0] in autotune.SMALL and best[1] in autotune.WAVE
+    assert best.marks_block in autotune.MARKS_BLOCK and best.marks_grid in autotune.MARKS_GRID
+    graphs = [hl.GraphInput(300 + 50 * i, *random_graph(300 + 50 * i, 1200, 30 + i)[:2]) for i in range(6)]
+    deltas = {i: hl.GraphInput(g.node_count, *random_batch(g.node_count, g.src, g.dst, None, 40, 50 + i))
+              for i, g in enumerate(graphs)}
+    tb = autotune.tune_graphs(graphs, deltas, levels=2)
+    assert tb.small_degree in autotune.SMALL and tb.marks_block in autotune.MARKS_BLOCK
+
+
+@pytest.mark.parametrize("block,grid", [(128, 512), (512, 2048), (256, 1)])
+def test_marks_shape_leaves_results_unchanged(block, grid):
+    """The grid-wide marks at another workgroup shape give the same state bit
+    for bit (hubs make the marks take the grid path)."""
+    from test_gpu_hub_tables import with_hubs
+    n = 30000
+    u, v = with_hubs(n, 5000, 3, 5)
+    out = []
+    THR = hl.RunMode.THROUGHPUT
+    for cfg in (hl.RunConfig(mode=THR, levels=2),
+                hl.RunConfig(mode=THR, levels=2, marks_block=block, marks_grid=grid)):
+        e = hl.HitLeiden(cfg)
+        e.load(hl.GraphInput(n, u, v))
+        for step in range(2):
+            e.hit_leiden(hl.GraphInput(n, *random_batch(n, u, v, None, 600, 80 + step)))
+        out.append([e.partition(p, w) for p in range(2) for w in (hl.MAP_F, hl.MAP_S_CUR)])
+        e.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
 
 
 # ---------------------------------------------------------------- quality harness
