@@ -344,10 +344,12 @@ def cpu_model():
 def batch_summary(stats):
     """Per-batch detail of the timed steps: movement rounds per level, priority
     rounds of the multi-kernel decoupling and the candidates its in-order
-    settle took, so that a slow batch can be attributed."""
+    settle took, and the device allocations the batch made (count, GiB, host
+    seconds), so that a slow batch can be attributed."""
     return [{"seconds": round(s["seconds"], 5), "rounds": s["rounds"], "decide_rounds": s["decide_rounds"],
              "settled": s["settled"], "evaluated": s["evaluated"], "moved": s["moved"],
-             "level_n": s["level_n"]} for s in stats]
+             "level_n": s["level_n"], "allocs": s["alloc_count"], "alloc_gib": round(s["alloc_bytes"] / 2**30, 3),
+             "alloc_seconds": round(s["alloc_seconds"], 4)} for s in stats]
 
 
 def run_extra(args, config, extra):

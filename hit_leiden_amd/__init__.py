@@ -75,7 +75,8 @@ class BatchStats(C.Structure):
                 ("evaluated", C.c_uint64 * MAX_LEVELS), ("moved", C.c_uint64 * MAX_LEVELS),
                 ("affected", C.c_uint64 * MAX_LEVELS), ("refined", C.c_uint64 * MAX_LEVELS),
                 ("rounds", C.c_uint64 * MAX_LEVELS), ("bytes_alg", C.c_uint64), ("seconds", C.c_double),
-                ("decide_rounds", C.c_uint64 * MAX_LEVELS), ("settled", C.c_uint64)]
+                ("decide_rounds", C.c_uint64 * MAX_LEVELS), ("settled", C.c_uint64),
+                ("alloc_count", C.c_uint64), ("alloc_bytes", C.c_uint64), ("alloc_seconds", C.c_double)]
 
     def as_dict(self):
         P = self.levels
@@ -86,6 +87,7 @@ class BatchStats(C.Structure):
             "affected": list(self.affected[:P]), "refined": list(self.refined[:P]),
             "rounds": list(self.rounds[:P]), "bytes_alg": self.bytes_alg, "seconds": self.seconds,
             "decide_rounds": list(self.decide_rounds[:P]), "settled": self.settled,
+            "alloc_count": self.alloc_count, "alloc_bytes": self.alloc_bytes, "alloc_seconds": self.alloc_seconds,
         }
 
 

@@ -64,6 +64,21 @@ void stream_sync(hipStream_t s);
 // exception leaves destinations in the frames it unwound.
 void drop_staged(hipStream_t s);
 
+// Device allocations made by the calling thread (the engine call in progress):
+// how many, their bytes, and the host seconds spent in hipMalloc / hipFree and
+// the synchronisation a free needs.  An engine call resets it when it starts
+// and reports it in its batch counters.
+struct AllocStats {
+  u64 count = 0;
+  u64 bytes = 0;
+  double seconds = 0.0;
+};
+extern thread_local AllocStats g_alloc;
+double host_seconds();
+// with HL_ALLOC_TRACE=<MiB> in the environment, every allocation of at least
+// that many MiB is printed to stderr with its call site (library offset)
+void alloc_trace(size_t bytes, double seconds, const void* site);
+
 // Grow-only device buffer.  reserve(n, keep) reallocates when the capacity is
 // short; keep copies the old contents (stream-ordered).
 template <class T>
@@ -79,8 +94,9 @@ struct DevBuf {
     p = nullptr;
     cap = 0;
   }
-  void reserve(size_t n, hipStream_t s, bool keep = false, double growth = 1.5) {
+  __attribute__((noinline)) void reserve(size_t n, hipStream_t s, bool keep = false, double growth = 1.5) {
     if (n <= cap) return;
+    const double t0 = host_seconds();
     size_t nc = size_t(double(n) * growth) + 64;
     T* np = nullptr;
     HL_CHECK(hipMalloc(&np, nc * sizeof(T)));
@@ -91,6 +107,11 @@ struct DevBuf {
     }
     p = np;
     cap = nc;
+    const double dt = host_seconds() - t0;
+    g_alloc.count += 1;
+    g_alloc.bytes += nc * sizeof(T);
+    g_alloc.seconds += dt;
+    alloc_trace(nc * sizeof(T), dt, __builtin_return_address(0));
   }
   T* get() const { return p; }
   void swap(DevBuf& o) {

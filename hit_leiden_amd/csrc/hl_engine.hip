@@ -253,14 +253,13 @@ struct Engine {
     apply_delta_plan(d, k, g, as, tmp, plan, s);
     if (plan.nseg) {
       if (lv.pool_used + plan.need > lv.col.cap) {
-        // compact live rows into the spare pool (kept across batches, so no
-        // allocation once it has grown) with room for this batch and headroom:
-        // a quarter of the live entries, or three more batches that rewrite as
-        // many rows as this one, within twice the live entries (merged rows
-        // move to the pool's end, and hub-heavy batches rewrite a large share
-        // of the entries)
-        const u64 live = compact_plan(g, offtmp, tmp, s);
-        const u64 room = std::max(live / 4, std::min(3 * plan.need, 2 * live));
+        // compact the rows (with their capacities) into the spare pool (kept
+        // across batches, so no allocation once it has grown) with room for
+        // this batch and headroom: an eighth of the live capacity, or four more
+        // batches that move as many rows as this one, within the live capacity
+        // (rows that fit their capacity are updated in place and take none)
+        const u64 live = compact_plan(g, offtmp, tmp, s, true);
+        const u64 room = std::max(live / 8, std::min(4 * plan.need, live));
         const u64 want = std::max<u64>(live + plan.need + room, 1024);
         if (lv.col_spare.cap < live + plan.need + room / 2) {
           lv.col_spare.release();
@@ -297,7 +296,7 @@ struct Engine {
       twice_m_tree(lv.k.p, lv.n, partials.p, lv.twice_m.p, s);
     }
     if (st && p < HL_MAX_LEVELS)
-      st->bytes_alg += 16 * k + 24 * plan.old_entries + 12 * plan.need + 20 * plan.nseg;
+      st->bytes_alg += 16 * k + 24 * plan.old_entries + 12 * plan.new_entries + 20 * plan.nseg;
   }
 
   u64 comm_space(Level& lv) {
@@ -846,6 +845,7 @@ hl_status guard(const hl_graph* gc, F&& fn, bool profile = true) {
   hl_graph* g = const_cast<hl_graph*>(gc);
   try {
     ProfBind pb(g && profile ? &g->e : nullptr);
+    hl::g_alloc = hl::AllocStats{};
     fn();
     if (g) g->e.err.clear();
     return HL_OK;
@@ -914,8 +914,15 @@ void check_vertex_count(const hl::Engine& e, uint64_t n) {
     throw hl::InputError("throughput mode holds fewer than 2^31 vertices");
 }
 
-double now_s() {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+double now_s() { return hl::host_seconds(); }
+
+// wall time and the call's device allocations into the batch counters
+void finish_stats(hl_batch_stats* st, double t0) {
+  if (!st) return;
+  st->seconds = now_s() - t0;
+  st->alloc_count = hl::g_alloc.count;
+  st->alloc_bytes = hl::g_alloc.bytes;
+  st->alloc_seconds = hl::g_alloc.seconds;
 }
 }  // namespace
 
@@ -1011,7 +1018,7 @@ hl_status hl_load(hl_graph* g, uint32_t n, uint64_t m, const uint32_t* src, cons
     e.reset(n, m);
     e.run_batch(m, stats);
     e.trim();
-    if (stats) stats->seconds = now_s() - t0;
+    finish_stats(stats, t0);
   });
 }
 
@@ -1066,7 +1073,7 @@ hl_status hl_load_graphs(hl_graph* g, uint32_t ngraphs, const uint32_t* vertex_o
       g0 = g1;
     } while (g0 < ngraphs);
     e.trim();
-    if (stats) stats->seconds = now_s() - t0;
+    finish_stats(stats, t0);
   });
 }
 
@@ -1099,7 +1106,7 @@ hl_status hl_apply_batch(hl_graph* g, uint64_t k, const uint32_t* src, const uin
     if (stats) std::memset(stats, 0, sizeof(*stats));
     e.upload_batch(k, src, dst, alpha, e.n0);
     e.run_batch(k, stats);
-    if (stats) stats->seconds = now_s() - t0;
+    finish_stats(stats, t0);
   });
 }
 
@@ -1124,7 +1131,7 @@ hl_status hl_apply_batch_device(hl_graph* g, uint64_t k, const uint32_t* d_src, 
     hl::stream_sync(e.s);
     if (bad) throw hl::InputError("endpoint id out of range, non-finite weight or a record joining two graphs");
     e.run_batch(k, stats);
-    if (stats) stats->seconds = now_s() - t0;
+    finish_stats(stats, t0);
   });
 }
 

@@ -63,6 +63,11 @@ void sort_pairs_u32(const u32* kin, u32* kout, const u32* vin, u32* vout, u64 n,
 // ---- graph maintenance ----
 struct ApplyScratch {
   DevBuf<u64> keys, keys_s, ukey, ukey_s, newcap, newoff, chunks, chunkbase, cnt, ucode, uscan;
+  // rows updated in place: their old entries staged here first (offsets by a
+  // scan of the staged lengths)
+  DevBuf<u64> stlen, stoff;
+  DevBuf<u32> st_col;
+  DevBuf<double> st_w;
   DevBuf<u32> idx, idx_s, runs, uidx, perm, segs, seg_row, seg_len, new_len, ins_pre, del_pre, seg_ins, seg_del;
   DevBuf<u8> heads, uop;
   DevBuf<double> uw;
@@ -100,16 +105,20 @@ struct TotalsRef {
 // affected rows and writes into *need the pool entries the relocation needs;
 // the engine must call apply_delta_commit after making room.
 struct ApplyPlan {
-  u64 nruns = 0, nupd = 0, nseg = 0, need = 0, total_chunks = 0, old_entries = 0, max_row = 0;
+  // need: pool entries for the rows that outgrow their capacity (moved, with
+  // slack); staged: old entries of the rows updated in place
+  u64 nruns = 0, nupd = 0, nseg = 0, need = 0, total_chunks = 0, old_entries = 0, max_row = 0, staged = 0;
+  u64 new_entries = 0;  // entries of the updated rows afterwards
 };
 void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a, Temp& tmp, ApplyPlan& plan,
                       hipStream_t s);
 void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, const ApplyPlan& plan, bool det,
                         const TotalsRef& tr, hipStream_t s);
-// Pool compaction in two steps: compact_plan returns the live entries and
-// leaves the new row offsets in off_tmp; compact_copy moves every row into
-// [col2, w2) (row_cap = row_len) and synchronises.
-u64 compact_plan(const LevelCsrMut& g, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s);
+// Pool compaction in two steps: compact_plan returns the pool entries the rows
+// take (their capacities when caps is set, else their lengths) and leaves the
+// new row offsets in off_tmp; compact_copy moves every row into [col2, w2)
+// (capacities kept) and synchronises.
+u64 compact_plan(const LevelCsrMut& g, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s, bool caps = false);
 void compact_copy(const LevelCsrMut& g, const u64* offs, u32* col2, double* w2, hipStream_t s);
 // the rows laid out at offs (from compact_plan) into col2/w2, leaving the live rows as they are
 void gather_rows(const LevelCsrMut& g, const u64* offs, u32* col2, double* w2, hipStream_t s);

@@ -13,8 +13,12 @@
 #include <rocprim/device/device_select.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "hl_kernels.h"
@@ -22,6 +26,23 @@
 
 namespace hl {
 thread_local Profiler* g_prof = nullptr;
+thread_local AllocStats g_alloc;
+
+double host_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void alloc_trace(size_t bytes, double seconds, const void* site) {
+  static const long min_mib = [] {
+    const char* e = std::getenv("HL_ALLOC_TRACE");
+    return e ? std::atol(e) : -1L;
+  }();
+  if (min_mib < 0 || bytes < (size_t(min_mib) << 20)) return;
+  Dl_info di{};
+  const uintptr_t base = dladdr(site, &di) ? uintptr_t(di.dli_fbase) : 0;
+  std::fprintf(stderr, "[hl alloc] %.1f MiB in %.3f ms at +0x%lx\n", double(bytes) / (1 << 20), seconds * 1e3,
+               (unsigned long)(uintptr_t(site) - base));
+}
 
 namespace {
 struct Staging {
@@ -434,12 +455,20 @@ __global__ void k_upd_pre(u64 nupd, u64 nseg, const u32* segs, const u64* scan, 
     del_pre[t] = u32(d);
   }
 }
+// Pool room of a row that outgrows its capacity: an eighth more than its new
+// length, and at least two entries, so that the next batches' inserts into it
+// are merged in place.
+__device__ __forceinline__ u32 row_slack(u32 nl) { return nl ? nl / 8 + 2 : 0; }
+
+// Per updated row: its counts, whether its new length fits its capacity (then
+// it is rewritten in place from a staged copy of its old entries: stlen = old
+// length, newcap = 0) or it moves to new pool space of newcap entries.
 __global__ void k_row_plan(u64 nseg, const u32* segs, u64 nupd, const u64* ukey_s, const u64* scan,
-                           const u32* row_len, u32* seg_row, u32* seg_len, u32* new_len, u32* seg_ins, u32* seg_del,
-                           u64* newcap, u64* chunks, u64* old_sum) {
-  // old_sum[0] += old row lengths, old_sum[1] = max new length: per-thread
-  // partials, one atomic each per workgroup
-  unsigned long long osum = 0, omax = 0;
+                           const u32* row_len, const u32* row_cap, u32* seg_row, u32* seg_len, u32* new_len,
+                           u32* seg_ins, u32* seg_del, u64* newcap, u64* stlen, u64* chunks, u64* old_sum) {
+  // old_sum[0] += old row lengths, old_sum[1] = max new length, old_sum[2] +=
+  // new row lengths: per-thread partials, one atomic each per workgroup
+  unsigned long long osum = 0, omax = 0, nsum = 0;
   for (u64 j = gtid(); j < nseg; j += gstride()) {
     u64 s = segs[j], e = (j + 1 < nseg) ? segs[j + 1] : nupd;
     u32 r = u32(ukey_s[s] >> 32);
@@ -447,50 +476,48 @@ __global__ void k_row_plan(u64 nseg, const u32* segs, u64 nupd, const u64* ukey_
     const u32 ins = u32(d >> 32), del = u32(d);
     u32 old = row_len[r];
     u32 nl = old + ins - del;
+    const bool fits = nl <= row_cap[r];
     seg_row[j] = r;
     seg_len[j] = u32(e - s);
     new_len[j] = nl;
     seg_ins[j] = ins;
     seg_del[j] = del;
-    newcap[j] = nl;
+    newcap[j] = fits ? 0 : u64(nl) + row_slack(nl);
+    stlen[j] = fits ? old : 0;
     u64 items = u64(old) + (e - s);
     chunks[j] = (items + MERGE_CHUNK - 1) / MERGE_CHUNK;
     osum += old;
+    nsum += nl;
     omax = omax > nl ? omax : nl;
   }
-  __shared__ unsigned long long ps[4], pm[4];
+  __shared__ unsigned long long ps[4], pm[4], pn[4];
   for (int o = 32; o > 0; o >>= 1) {
     osum += __shfl_xor(osum, o, 64);
+    nsum += __shfl_xor(nsum, o, 64);
     const unsigned long long y = __shfl_xor(omax, o, 64);
     omax = omax > y ? omax : y;
   }
   if ((threadIdx.x & 63) == 0) {
     ps[threadIdx.x >> 6] = osum;
     pm[threadIdx.x >> 6] = omax;
+    pn[threadIdx.x >> 6] = nsum;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t = 0, m = 0;
+    unsigned long long t = 0, m = 0, q = 0;
     for (u32 w = 0; w < blockDim.x / 64; ++w) {
       t += ps[w];
+      q += pn[w];
       m = m > pm[w] ? m : pm[w];
     }
     if (t) atomicAdd((unsigned long long*)old_sum, t);
     if (m) atomicMax((unsigned long long*)(old_sum + 1), m);
+    if (q) atomicAdd((unsigned long long*)(old_sum + 2), q);
   }
 }
 
-// Merge an old sorted row with its sorted updates into the new location.
-// Existing entry t goes to t - #deleted-before + #inserted-before; an insert
-// goes to lower_bound(old row) - #deleted-before + #inserted-before.
-__global__ __launch_bounds__(256) void k_row_merge(u64 nseg, const u64* chunkbase, const u32* segs,
-                                                    const u32* seg_row, const u32* seg_len, const u32* seg_ins,
-                                                    const u32* seg_del, const u64* ukey_s, const u32* perm,
-                                                    const u8* uop, const double* uw, const u32* ins_pre,
-                                                    const u32* del_pre, const u64* newoff, u64 pool_base,
-                                                    LevelCsrMut g) {
-  u64 b = blockIdx.x;
-  // segment j with chunkbase[j] <= b < chunkbase[j+1]
+// segment j of merge block b: chunkbase[j] <= b < chunkbase[j+1]
+__device__ __forceinline__ u64 chunk_segment(u64 nseg, const u64* chunkbase, u64 b) {
   u64 lo = 0, hi = nseg;
   while (hi - lo > 1) {
     u64 mid = (lo + hi) >> 1;
@@ -499,19 +526,55 @@ __global__ __launch_bounds__(256) void k_row_merge(u64 nseg, const u64* chunkbas
     else
       hi = mid;
   }
-  const u64 j = lo;
+  return lo;
+}
+
+// The old entries of the rows updated in place, copied aside (the same
+// block-to-chunk mapping as the merge), so that the merge never reads what it
+// overwrites.
+__global__ __launch_bounds__(256) void k_row_stage(u64 nseg, const u64* chunkbase, const u32* seg_row, const u64* stlen,
+                                                    const u64* stoff, LevelCsrMut g, u32* st_col, double* st_w) {
+  const u64 b = blockIdx.x;
+  const u64 j = chunk_segment(nseg, chunkbase, b);
+  const u64 olen = stlen[j];
+  if (olen == 0) return;
   const u64 c = b - chunkbase[j];
+  const u64 oo = g.row_off[seg_row[j]], so = stoff[j];
+  const u64 end = umin64(olen, (c + 1) * MERGE_CHUNK);
+  for (u64 t = c * MERGE_CHUNK + threadIdx.x; t < end; t += blockDim.x) {
+    st_col[so + t] = g.col[oo + t];
+    st_w[so + t] = g.w[oo + t];
+  }
+}
+
+// Merge an old sorted row with its sorted updates into its place: its own
+// position when it fits its capacity (the old entries read from their staged
+// copy), else its new pool space.  Existing entry t goes to t - #deleted-before
+// + #inserted-before; an insert goes to lower_bound(old row) - #deleted-before
+// + #inserted-before.
+__global__ __launch_bounds__(256) void k_row_merge(u64 nseg, const u64* chunkbase, const u32* segs,
+                                                    const u32* seg_row, const u32* seg_len, const u32* seg_ins,
+                                                    const u32* seg_del, const u64* ukey_s, const u32* perm,
+                                                    const u8* uop, const double* uw, const u32* ins_pre,
+                                                    const u32* del_pre, const u64* newoff, const u64* newcap,
+                                                    const u64* stoff, const u32* st_col, const double* st_w,
+                                                    u64 pool_base, LevelCsrMut g) {
+  const u64 j = chunk_segment(nseg, chunkbase, blockIdx.x);
+  const u64 c = blockIdx.x - chunkbase[j];
   const u32 r = seg_row[j];
   const u64 oo = g.row_off[r];
   const u32 olen = g.row_len[r];
   const u64 s = segs[j];
   const u32 ul = seg_len[j];
-  const u64 dst = pool_base + newoff[j];
+  const bool inplace = newcap[j] == 0;
+  const u32* ocol = inplace ? st_col + stoff[j] : g.col + oo;
+  const double* ow = inplace ? st_w + stoff[j] : g.w + oo;
+  const u64 dst = inplace ? oo : pool_base + newoff[j];
   const u64 items = u64(olen) + ul;
   const u64 end = umin64(items, (c + 1) * MERGE_CHUNK);
   for (u64 t = c * MERGE_CHUNK + threadIdx.x; t < end; t += blockDim.x) {
     if (t < olen) {
-      u32 col = g.col[oo + t];
+      u32 col = ocol[t];
       // first update with col >= this col
       u32 lo2 = 0, hi2 = ul;
       while (lo2 < hi2) {
@@ -525,7 +588,7 @@ __global__ __launch_bounds__(256) void k_row_merge(u64 nseg, const u64* chunkbas
       u32 dp = q < ul ? del_pre[s + q] : seg_del[j];
       u32 ip = q < ul ? ins_pre[s + q] : seg_ins[j];
       bool match = q < ul && u32(ukey_s[s + q] & 0xffffffffu) == col;
-      double w = g.w[oo + t];
+      double w = ow[t];
       if (match) {
         u32 pi = perm[s + q];
         u8 op = uop[pi];
@@ -540,7 +603,7 @@ __global__ __launch_bounds__(256) void k_row_merge(u64 nseg, const u64* chunkbas
       u32 pi = perm[s + q];
       if (uop[pi] != OP_INS) continue;
       u32 col = u32(ukey_s[s + q] & 0xffffffffu);
-      u32 lb = lower_bound_u32(g.col + oo, olen, col);
+      u32 lb = lower_bound_u32(ocol, olen, col);
       u64 pos = u64(lb) - del_pre[s + q] + ins_pre[s + q];
       g.col[dst + pos] = col;
       g.w[dst + pos] = uw[pi];
@@ -548,13 +611,15 @@ __global__ __launch_bounds__(256) void k_row_merge(u64 nseg, const u64* chunkbas
   }
 }
 
-__global__ void k_row_commit(u64 nseg, const u32* seg_row, const u32* new_len, const u64* newoff, u64 pool_base,
-                             LevelCsrMut g) {
+__global__ void k_row_commit(u64 nseg, const u32* seg_row, const u32* new_len, const u64* newoff, const u64* newcap,
+                             u64 pool_base, LevelCsrMut g) {
   for (u64 j = gtid(); j < nseg; j += gstride()) {
     u32 r = seg_row[j];
-    g.row_off[r] = pool_base + newoff[j];
     g.row_len[r] = new_len[j];
-    g.row_cap[r] = new_len[j];
+    if (newcap[j]) {
+      g.row_off[r] = pool_base + newoff[j];
+      g.row_cap[r] = u32(newcap[j]);
+    }
   }
 }
 
@@ -663,10 +728,7 @@ __global__ void k_copy_rows(LevelCsrMut g, const u64* newoff, u32* col2, double*
   }
 }
 __global__ void k_set_offsets(LevelCsrMut g, const u64* newoff) {
-  for (u64 r = gtid(); r < g.n; r += gstride()) {
-    g.row_off[r] = newoff[r];
-    g.row_cap[r] = g.row_len[r];
-  }
+  for (u64 r = gtid(); r < g.n; r += gstride()) g.row_off[r] = newoff[r];
 }
 __global__ void k_len_u64(const u32* len, u32 n, u64* out) {
   for (u64 i = gtid(); i < n; i += gstride()) out[i] = len[i];
@@ -2614,9 +2676,11 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   a.del_pre.reserve(nupd, s);
   a.newcap.reserve(ns, s);
   a.newoff.reserve(ns + 1, s);
+  a.stlen.reserve(ns, s);
+  a.stoff.reserve(ns + 1, s);
   a.chunks.reserve(ns, s);
   a.chunkbase.reserve(ns + 1, s);
-  HL_CHECK(hipMemsetAsync(a.cnt.p, 0, 2 * sizeof(u64), s));
+  HL_CHECK(hipMemsetAsync(a.cnt.p, 0, 3 * sizeof(u64), s));
   // inserts and deletes before each update, and per row, from one scan (a
   // coarse hub row can take millions of updates: no thread walks a row)
   a.ucode.reserve(nupd, s);
@@ -2626,37 +2690,49 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   exclusive_scan_u64_async(a.ucode.p, a.uscan.p, nupd, tmp, s);
   hipLaunchKernelGGL(k_upd_pre, GRID(nupd), u64(nupd), ns, a.segs.p, a.uscan.p, a.ins_pre.p, a.del_pre.p);
   HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_row_plan, GRID(ns), ns, a.segs.p, u64(nupd), a.ukey_s.p, a.uscan.p, g.row_len,
-                     a.seg_row.p, a.seg_len.p, a.new_len.p, a.seg_ins.p, a.seg_del.p, a.newcap.p, a.chunks.p,
-                     a.cnt.p);
+  hipLaunchKernelGGL(k_row_plan, GRID(ns), ns, a.segs.p, u64(nupd), a.ukey_s.p, a.uscan.p, g.row_len, g.row_cap,
+                     a.seg_row.p, a.seg_len.p, a.new_len.p, a.seg_ins.p, a.seg_del.p, a.newcap.p, a.stlen.p,
+                     a.chunks.p, a.cnt.p);
   HL_LAUNCH_CHECK();
   exclusive_scan_u64_async(a.newcap.p, a.newoff.p, ns, tmp, s);
+  exclusive_scan_u64_async(a.stlen.p, a.stoff.p, ns, tmp, s);
   exclusive_scan_u64_async(a.chunks.p, a.chunkbase.p, ns, tmp, s);
-  u64 h[4] = {0, 0, 0, 0};
-  d2h_async(h, a.cnt.p, 2 * sizeof(u64), s);
-  d2h_async(h + 2, a.newoff.p + ns, sizeof(u64), s);
-  d2h_async(h + 3, a.chunkbase.p + ns, sizeof(u64), s);
+  u64 h[6] = {0, 0, 0, 0, 0, 0};
+  d2h_async(h, a.cnt.p, 3 * sizeof(u64), s);
+  d2h_async(h + 3, a.newoff.p + ns, sizeof(u64), s);
+  d2h_async(h + 4, a.chunkbase.p + ns, sizeof(u64), s);
+  d2h_async(h + 5, a.stoff.p + ns, sizeof(u64), s);
   stream_sync(s);
   plan.old_entries = h[0];
   plan.max_row = h[1];
-  plan.need = h[2];
-  plan.total_chunks = h[3];
+  plan.new_entries = h[2];
+  plan.need = h[3];
+  plan.total_chunks = h[4];
+  plan.staged = h[5];
 }
 
 void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, const ApplyPlan& plan, bool det,
                         const TotalsRef& tr, hipStream_t s) {
   if (plan.nseg == 0) return;
   const u64 ns = plan.nseg;
+  if (plan.staged) {
+    a.st_col.reserve(plan.staged, s, false, 1.25);
+    a.st_w.reserve(plan.staged, s, false, 1.25);
+    hipLaunchKernelGGL(k_row_stage, dim3(unsigned(plan.total_chunks)), dim3(256), 0, s, ns, a.chunkbase.p,
+                       a.seg_row.p, a.stlen.p, a.stoff.p, g, a.st_col.p, a.st_w.p);
+    HL_LAUNCH_CHECK();
+  }
   {
-  HL_PROF(P_ROW_MERGE, 12 * (plan.old_entries + plan.need) + 21 * plan.nupd, s);
+  HL_PROF(P_ROW_MERGE, 12 * (plan.old_entries + plan.new_entries) + 21 * plan.nupd, s);
   hipLaunchKernelGGL(k_row_merge, dim3(unsigned(plan.total_chunks)), dim3(256), 0, s, ns, a.chunkbase.p, a.segs.p,
                      a.seg_row.p, a.seg_len.p, a.seg_ins.p, a.seg_del.p, a.ukey_s.p, a.perm.p, a.uop.p, a.uw.p,
-                     a.ins_pre.p, a.del_pre.p, a.newoff.p, pool_base, g);
+                     a.ins_pre.p, a.del_pre.p, a.newoff.p, a.newcap.p, a.stoff.p, a.st_col.p, a.st_w.p, pool_base,
+                     g);
   HL_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_row_commit, GRID(ns), ns, a.seg_row.p, a.new_len.p, a.newoff.p, pool_base, g);
+  hipLaunchKernelGGL(k_row_commit, GRID(ns), ns, a.seg_row.p, a.new_len.p, a.newoff.p, a.newcap.p, pool_base, g);
   HL_LAUNCH_CHECK();
-  HL_PROF(P_ROW_DEGREE, 12 * plan.need + 12 * ns, s);
+  HL_PROF(P_ROW_DEGREE, 12 * plan.new_entries + 12 * ns, s);
   if (det) {
     hipLaunchKernelGGL(k_row_degree, GRID(ns), ns, a.seg_row.p, g, tr);
     HL_LAUNCH_CHECK();
@@ -2682,11 +2758,11 @@ void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, co
   }
 }
 
-u64 compact_plan(const LevelCsrMut& g, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s) {
+u64 compact_plan(const LevelCsrMut& g, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s, bool caps) {
   off_tmp.reserve(u64(g.n) + 1, s);
   DevBuf<u64> lens;
   lens.reserve(g.n + 1, s);
-  hipLaunchKernelGGL(k_len_u64, GRID(g.n), g.row_len, g.n, lens.p);
+  hipLaunchKernelGGL(k_len_u64, GRID(g.n), caps ? g.row_cap : g.row_len, g.n, lens.p);
   HL_LAUNCH_CHECK();
   const u64 total = exclusive_scan_u64(lens.p, off_tmp.p, g.n, tmp, s);  // synchronises
   return total;
@@ -3227,7 +3303,7 @@ void rank_labels(u32* f, u32 n, u64 L, const double* tot, u64 tot_cap, DevBuf<u6
   HL_LAUNCH_CHECK();
   exclusive_scan_u64_async(pc, pre, W, tmp, s);
   if (tot) {
-    tot2.reserve(tot_cap, s, false, 1.0);
+    tot2.reserve(tot_cap, s, false, 1.25);  // it rotates through the levels: room for all of them
     hipLaunchKernelGGL(k_fill_f64, GRID(tot2.cap), tot2.p, 0.0, u64(tot2.cap));
     HL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_move_totals, GRID(W), bits, pre, W, tot, tot_cap, tot2.p);

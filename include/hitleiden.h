@@ -115,6 +115,9 @@ typedef struct {
                                             (rounds with more than 4,096 candidates) */
   uint64_t settled;                  /* throughput: candidates decided by the in-order settle after
                                         decide_rounds priority rounds */
+  uint64_t alloc_count;              /* device allocations the call made (grow-only buffers) */
+  uint64_t alloc_bytes;              /* their bytes */
+  double alloc_seconds;              /* host wall time in those allocations and the frees they replace */
 } hl_batch_stats;
 
 typedef struct hl_graph hl_graph;
