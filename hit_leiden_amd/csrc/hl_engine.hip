@@ -317,7 +317,9 @@ struct Engine {
     if (n == 0) return;
     const u64 L = std::max<u64>(lv.lab_bound, n);
     if (lv.tot_valid) ensure_totals(lv);
-    rank_labels(lv.f.p, n, L, lv.tot_valid ? lv.tot.p : nullptr, lv.tot_valid ? lv.tot.cap : 0, clab, tot2, tmp, s);
+    // the totals' need once the rounds take the empty-community ids n + v
+    rank_labels(lv.f.p, n, L, lv.tot_valid ? lv.tot.p : nullptr, lv.tot_valid ? lv.tot.cap : 0, 2ull * n + 1, clab,
+                tot2, tmp, s);
     if (lv.tot_valid) lv.tot.swap(tot2);
     lv.lab_bound = n;
   }

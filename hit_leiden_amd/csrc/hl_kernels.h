@@ -227,10 +227,11 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
 // f^p(v) = f^{p+1}(s^p(v)) for v in B; tot (may be null) is level p's maintained totals
 // f[v] <- the rank of f[v] among the labels in use (all below L), so every
 // label ends below their count; tot (tot_cap entries, or null) moves with its
-// labels into tot2 (reserved and zeroed here), which the caller swaps in.
-// flags holds the presence bitmap, its word popcounts and their prefix.
-void rank_labels(u32* f, u32 n, u64 L, const double* tot, u64 tot_cap, DevBuf<u64>& flags, DevBuf<double>& tot2,
-                 Temp& tmp, hipStream_t s);
+// labels into tot2 (reserved for at least tot_need entries and zeroed here),
+// which the caller swaps in.  flags holds the presence bitmap, its word
+// popcounts and their prefix.
+void rank_labels(u32* f, u32 n, u64 L, const double* tot, u64 tot_cap, u64 tot_need, DevBuf<u64>& flags,
+                 DevBuf<double>& tot2, Temp& tmp, hipStream_t s);
 // f^p(v) = f^q(s^p(v)) for v in B (every v when B is null), keeping tot current
 void def_pull(const u8* B, u32* fp, const u32* fq, const u32* sp, u32 n, double* tot, const double* k,
               hipStream_t s);

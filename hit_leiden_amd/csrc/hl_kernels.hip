@@ -3282,8 +3282,8 @@ __global__ void k_apply_ranks(u32* f, u32 n, const u32* bits, const u64* pre) {
   for (u64 v = gtid(); v < n; v += gstride()) f[v] = label_rank(bits, pre, f[v]);
 }
 
-void rank_labels(u32* f, u32 n, u64 L, const double* tot, u64 tot_cap, DevBuf<u64>& flags, DevBuf<double>& tot2,
-                 Temp& tmp, hipStream_t s) {
+void rank_labels(u32* f, u32 n, u64 L, const double* tot, u64 tot_cap, u64 tot_need, DevBuf<u64>& flags,
+                 DevBuf<double>& tot2, Temp& tmp, hipStream_t s) {
   if (n == 0) return;
   const u64 W = L / 32 + 1;
   HL_PROF(P_RELABEL, 4ull * n + 20ull * W + 16ull * n, s);
@@ -3303,7 +3303,9 @@ void rank_labels(u32* f, u32 n, u64 L, const double* tot, u64 tot_cap, DevBuf<u6
   HL_LAUNCH_CHECK();
   exclusive_scan_u64_async(pc, pre, W, tmp, s);
   if (tot) {
-    tot2.reserve(tot_cap, s, false, 1.25);  // it rotates through the levels: room for all of them
+    // the buffer rotates through the levels: a quarter more than this one needs
+    // (relative to the need, so that the rotation does not compound it)
+    tot2.reserve(std::max<u64>(tot_need, n), s, false, 1.25);
     hipLaunchKernelGGL(k_fill_f64, GRID(tot2.cap), tot2.p, 0.0, u64(tot2.cap));
     HL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_move_totals, GRID(W), bits, pre, W, tot, tot_cap, tot2.p);

@@ -23,6 +23,10 @@ struct FlatMarks {
   u32* c;    // device count for k_marks_flat (0: nothing to do)
   u32* ck_c = nullptr;  // device count for the cache updates of k_ck_flat
   u32* from = nullptr;  // source community of each move (hub-table updates)
+  // 1 where the moved vertex already moved in an earlier round of this level:
+  // its affected (K) marks were set then (they depend on s only, which the
+  // rounds do not change), so only its next-active marks are left to do
+  u8* kd = nullptr;
 };
 
 // Hub tables: for a vertex of degree above the LDS workgroup bucket, the
@@ -186,6 +190,7 @@ struct MoveScratch {
   DevBuf<u8> target;
   DevBuf<u32> ck;  // neighbour-community cache of the level in movement
   DevBuf<u32> mfrom;  // source communities of the moves marked over the grid
+  DevBuf<u8> mkd, kd;  // repeat-move flags of those moves (FlatMarks::kd), both paths
   // hub tables (HubTables)
   DevBuf<u32> ht_bits;
   DevBuf<HtSlot> ht_slot;

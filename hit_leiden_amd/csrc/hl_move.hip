@@ -155,6 +155,7 @@ __device__ __forceinline__ u32 lds_insert(u32* K, u32 cap, u32 key) {
 // the round starts (n / 8 bytes, which stay in L2, where a per-vertex stamp
 // costs a line of HBM traffic per random test).  A plain read first spares the
 // atomic for vertices already claimed, which hubs make common.
+__device__ __forceinline__ bool marked(const u32* bits, u32 v) { return (bits[v >> 5] >> (v & 31)) & 1u; }
 __device__ __forceinline__ bool claim(u32* bits, u32 v) {
   u32* w = bits + (v >> 5);
   const u32 b = 1u << (v & 31);
@@ -1214,6 +1215,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
   __shared__ u32 sfrom[kDecSmall];
   __shared__ u32 sto[kDecSmall];
   __shared__ u8 sst[kDecSmall];
+  __shared__ u8 skd[kDecSmall];  // the accepted move's vertex moved in an earlier round of this level
   __shared__ u32 mAto[kDecHash], mAfrom[kDecHash];
   __shared__ u16 fslot[kDecLds], tslot[kDecLds];
   __shared__ u32 und, nacc;
@@ -1268,6 +1270,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
         sto[lane] = to;
         sfrom[lane] = from;
         sst[lane] = acc ? kAcc : kRej;
+        skd[lane] = acc ? m.B[node] : u8(0);  // moved in an earlier round of this level
       }
       if (acc) {
         const double kv = m.k[node];
@@ -1461,6 +1464,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
       if (m.fs) st_agent(fs_f(m.fs, v), sto[p]);
       atomicAdd(&m.tot[sfrom[p]], -kv);
       atomicAdd(&m.tot[sto[p]], kv);
+      skd[p] = m.B[v];  // one move per vertex and round: no race on B[v]
       m.B[v] = 1;
     }
     if (la) atomicAdd(&nacc, la);
@@ -1513,19 +1517,24 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
         const u64 e = m.g.row_off[v] + (t - pre[lo]);
         u = m.g.col[e];
         if (m.ht.bits && u != v) ht_move(m.ht, u, sfrom[lo], cto, m.g.w[e]);
-        u32 fu, su;
-        if (m.fs) {
-          const u64 x = ld_agent64(m.fs + u);
-          fu = u32(x);
-          su = u32(x >> 32);
+        if (skd[lo]) {
+          // a repeat move: K marks done; a neighbour already claimed needs no gather
+          if (!marked(mark, u) && ld_agent(m.fs ? fs_f(m.fs, u) : &m.f[u]) != cto) want = claim(mark, u);
         } else {
-          fu = ld_agent(&m.f[u]);
-          su = m.s[u];
-        }
-        if (fu != cto) want = claim(mark, u);
-        if (su == m.s[v]) {
-          m.K[u] = 1;
-          m.K[v] = 1;
+          u32 fu, su;
+          if (m.fs) {
+            const u64 x = ld_agent64(m.fs + u);
+            fu = u32(x);
+            su = u32(x >> 32);
+          } else {
+            fu = ld_agent(&m.f[u]);
+            su = m.s[u];
+          }
+          if (fu != cto) want = claim(mark, u);
+          if (su == m.s[v]) {
+            m.K[u] = 1;
+            m.K[v] = 1;
+          }
         }
       }
       const u32 q = wave_append(want, next_cnt);
@@ -1540,6 +1549,7 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
       flat.to[p] = sto[p];
       flat.pre[p] = pre[p];
       if (flat.from) flat.from[p] = sfrom[p];
+      if (flat.kd) flat.kd[p] = sst[p] == kAcc ? skd[p] : u8(0);
     }
     if (tid == 0) {
       flat.pre[c] = total;
@@ -1598,13 +1608,14 @@ __global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u3
     u32 uu[R], vv[R], ct[R], fu[R], su[R], sv[R];
     u64 ee[R];
     u32 seg[R];
-    bool ok[R];
+    bool ok[R], rep[R], need[R];
 #pragma unroll
     for (u32 r = 0; r < R; ++r) {
       const u64 t = base + u64(r) * blockDim.x + threadIdx.x;
       ok[r] = t < total;
       uu[r] = vv[r] = ct[r] = seg[r] = 0;
       ee[r] = 0;
+      rep[r] = false;
       if (ok[r]) {
         u32 lo = span[0], hi = span[1] + 1;
         while (hi - lo > 1) {
@@ -1615,14 +1626,19 @@ __global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u3
         seg[r] = lo;
         vv[r] = flat.node[lo];
         ct[r] = flat.to[lo];
+        rep[r] = flat.kd && flat.kd[lo];
         ee[r] = m.g.row_off[vv[r]] + (t - flat.pre[lo]);
         uu[r] = m.g.col[ee[r]];
       }
     }
+    // a repeat move's K marks are done, and its next-active mark of a
+    // neighbour that is already claimed needs no gather
+#pragma unroll
+    for (u32 r = 0; r < R; ++r) need[r] = ok[r] && !(rep[r] && marked(mark, uu[r]));
 #pragma unroll
     for (u32 r = 0; r < R; ++r) {
       fu[r] = su[r] = sv[r] = 0;
-      if (ok[r]) {
+      if (need[r]) {
         if (m.fs) {
           const u64 x = m.fs[uu[r]];
           fu[r] = u32(x);
@@ -1631,7 +1647,7 @@ __global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u3
           fu[r] = m.f[uu[r]];
           su[r] = m.s[uu[r]];
         }
-        sv[r] = m.s[vv[r]];
+        if (!rep[r]) sv[r] = m.s[vv[r]];
       }
     }
 #pragma unroll
@@ -1639,8 +1655,8 @@ __global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u3
       if (!ok[r]) continue;
       const u32 u = uu[r], v = vv[r];
       if (m.ht.bits && flat.from && u != v) ht_move(m.ht, u, flat.from[seg[r]], ct[r], m.g.w[ee[r]]);
-      if (fu[r] != ct[r] && claim(mark, u)) sv_[atomicAdd(&scnt, 1u)] = u;
-      if (su[r] == sv[r]) {
+      if (need[r] && fu[r] != ct[r] && claim(mark, u)) sv_[atomicAdd(&scnt, 1u)] = u;
+      if (need[r] && !rep[r] && su[r] == sv[r]) {
         m.K[u] = 1;
         m.K[v] = 1;
       }
@@ -1848,11 +1864,12 @@ __global__ void k_settle_seq(const u32* list, const unsigned long long* cnt, con
   }
 }
 __global__ void k_apply4(u64 c, const u32* node, const u32* from, const u32* to, const u32* state, MoveCtx m,
-                         u32* acc_list, u32* acc_cnt, unsigned long long* dst) {
+                         u32* acc_list, u32* acc_cnt, unsigned long long* dst, u8* kd) {
   unsigned long long na = 0, nd = 0;
   for (u64 base = u64(blockIdx.x) * blockDim.x; base < c; base += gstride()) {
     u64 p = base + threadIdx.x;
     bool acc = p < c && state[p] == kAcc;
+    if (p < c) kd[p] = acc ? m.B[node[p]] : u8(0);  // moved in an earlier round of this level
     if (acc) {
       u32 v = node[p];
       ++na;
@@ -1924,6 +1941,7 @@ void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
   mto.reserve(kDecSmall + 1, s);
   mfrom.reserve(kDecSmall + 1, s);
   mpre.reserve(kDecSmall + 2, s);
+  mkd.reserve(kDecSmall + 1, s);
   if (minUto.cap < comm_cap) {
     minUto.reserve(comm_cap, s);
     minUfrom.reserve(comm_cap, s);
@@ -2128,7 +2146,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
                   ms.dstat.p + DS_TRY_MOVE, s);
     ms.fresh_marks(s);
     CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
-    const FlatMarks fm{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 12, ms.mfrom.p};
+    const FlatMarks fm{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 12, ms.mfrom.p, ms.mkd.p};
     {
       HL_PROF(P_DECOUPLE, 0, s);
       hipLaunchKernelGGL(k_dec_small, 1, 1024, 0, s, m, cand, ms.minUto.p, ms.minUfrom.p, ms.minAto.p,
@@ -2168,6 +2186,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
       ms.to.reserve(c, s);
       ms.state.reserve(c, s);
       ms.acc.reserve(c, s);
+      ms.kd.reserve(c, s);
       hipLaunchKernelGGL(k_cand_node_keys, fixed_grid(c), 256, 0, s, cand, c, ms.k32.p, ms.v32.p);
       HL_LAUNCH_CHECK();
       sort_pairs_u32(ms.k32.p, ms.k32b.p, ms.v32.p, ms.o1.p, c, 32, tmp, s);
@@ -2218,13 +2237,14 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
       {
         HL_PROF(P_APPLY_MOVES, 0, s);
         hipLaunchKernelGGL(k_apply4, fixed_grid(c), 256, 0, s, u64(c), ms.node.p, ms.from.p, ms.to.p, ms.state.p, m,
-                           ms.acc.p, acc_cnt, ms.dstat.p);
+                           ms.acc.p, acc_cnt, ms.dstat.p, ms.kd.p);
         HL_LAUNCH_CHECK();
       }
       {
         ms.mdeg.reserve(c, s);
         ms.mpre.reserve(u64(c) + 2, s);
-        const FlatMarks fmm{ms.node.p, ms.to.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 13, ms.from.p};
+        const FlatMarks fmm{ms.node.p, ms.to.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 13, ms.from.p,
+                            ms.kd.p};
         {
           HL_PROF(P_DEC_MULTI, 0, s);
           hipLaunchKernelGGL(k_acc_degrees, fixed_grid(c), 256, 0, s, u64(c), ms.node.p, ms.state.p, m.g.row_len,
