@@ -349,7 +349,7 @@ def batch_summary(stats):
     return [{"seconds": round(s["seconds"], 5), "rounds": s["rounds"], "decide_rounds": s["decide_rounds"],
              "settled": s["settled"], "evaluated": s["evaluated"], "moved": s["moved"],
              "level_n": s["level_n"], "allocs": s["alloc_count"], "alloc_gib": round(s["alloc_bytes"] / 2**30, 3),
-             "alloc_seconds": round(s["alloc_seconds"], 4)} for s in stats]
+             "alloc_seconds": round(s["alloc_seconds"], 4), "host_syncs": s["host_syncs"]} for s in stats]
 
 
 def run_extra(args, config, extra):

@@ -76,7 +76,8 @@ class BatchStats(C.Structure):
                 ("affected", C.c_uint64 * MAX_LEVELS), ("refined", C.c_uint64 * MAX_LEVELS),
                 ("rounds", C.c_uint64 * MAX_LEVELS), ("bytes_alg", C.c_uint64), ("seconds", C.c_double),
                 ("decide_rounds", C.c_uint64 * MAX_LEVELS), ("settled", C.c_uint64),
-                ("alloc_count", C.c_uint64), ("alloc_bytes", C.c_uint64), ("alloc_seconds", C.c_double)]
+                ("alloc_count", C.c_uint64), ("alloc_bytes", C.c_uint64), ("alloc_seconds", C.c_double),
+                ("host_syncs", C.c_uint64)]
 
     def as_dict(self):
         P = self.levels
@@ -88,6 +89,7 @@ class BatchStats(C.Structure):
             "rounds": list(self.rounds[:P]), "bytes_alg": self.bytes_alg, "seconds": self.seconds,
             "decide_rounds": list(self.decide_rounds[:P]), "settled": self.settled,
             "alloc_count": self.alloc_count, "alloc_bytes": self.alloc_bytes, "alloc_seconds": self.alloc_seconds,
+            "host_syncs": self.host_syncs,
         }
 
 

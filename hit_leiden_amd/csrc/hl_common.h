@@ -72,6 +72,7 @@ struct AllocStats {
   u64 count = 0;
   u64 bytes = 0;
   double seconds = 0.0;
+  u64 syncs = 0;  // host waits on the engine stream (stream_sync)
 };
 extern thread_local AllocStats g_alloc;
 double host_seconds();

@@ -925,6 +925,7 @@ void finish_stats(hl_batch_stats* st, double t0) {
   st->alloc_count = hl::g_alloc.count;
   st->alloc_bytes = hl::g_alloc.bytes;
   st->alloc_seconds = hl::g_alloc.seconds;
+  st->host_syncs = hl::g_alloc.syncs;
 }
 }  // namespace
 

@@ -64,8 +64,17 @@ Staging& staging() {
 }
 }  // namespace
 
-void stream_sync(hipStream_t s) {
+__attribute__((noinline)) void stream_sync(hipStream_t s) {
   HL_CHECK(hipStreamSynchronize(s));
+  g_alloc.syncs += 1;
+  static const bool trace = std::getenv("HL_SYNC_TRACE") != nullptr;
+  if (trace) {
+    // with HL_SYNC_TRACE set, every wait is printed with its call site (library offset)
+    Dl_info di{};
+    const void* site = __builtin_return_address(0);
+    const uintptr_t base = dladdr(site, &di) ? uintptr_t(di.dli_fbase) : 0;
+    std::fprintf(stderr, "[hl sync] +0x%lx\n", (unsigned long)(uintptr_t(site) - base));
+  }
   Staging& st = staging();
   size_t keep = 0;
   for (size_t i = 0; i < st.items.size(); ++i) {

@@ -118,6 +118,7 @@ typedef struct {
   uint64_t alloc_count;              /* device allocations the call made (grow-only buffers) */
   uint64_t alloc_bytes;              /* their bytes */
   double alloc_seconds;              /* host wall time in those allocations and the frees they replace */
+  uint64_t host_syncs;               /* times the host waited for the device during the call */
 } hl_batch_stats;
 
 typedef struct hl_graph hl_graph;
