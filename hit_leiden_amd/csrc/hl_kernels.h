@@ -18,6 +18,12 @@ struct Csr {
 };
 // 2m of the graph vertex v belongs to
 __device__ __forceinline__ double tm_of(const Csr& g, u32 v) { return g.gid ? g.twice_m[g.gid[v]] : g.twice_m[0]; }
+// Row entries (col, w) that a kernel reads once, streamed (non-temporal) so
+// that they do not evict the label arrays the same kernel gathers at random.
+template <class T>
+__device__ __forceinline__ T ld_row(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
 
 // scratch allocator used by the launchers for library temporaries (rocprim)
 struct Temp {

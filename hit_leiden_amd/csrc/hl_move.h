@@ -107,13 +107,16 @@ struct MovePolicyT {
   static constexpr bool kTables = true;
   __device__ bool table_ok(u32 v) const { return ht.bits && ht.valid(v); }
   // row entry e of vertex v (neighbour col[e])
+  // Stream: the row is read by a whole group of lanes at once (non-temporal
+  // loads); a lane walking its own short row reads it cached instead
+  template <bool Stream = true>
   __device__ bool accept_e(u32 v, const u32* col, u64 e, u32& key, int& mult) const {
     mult = 1;
     if (ck) {
       key = ck[e];
       return key != kNone;
     }
-    const u32 u = col[e];
+    const u32 u = Stream ? ld_row(col + e) : col[e];
     if (u == v) return false;
     key = f[u];
     return true;
@@ -137,8 +140,11 @@ struct RefinePolicyT {
   static constexpr bool kEmpty = false;
   static constexpr bool kTables = false;
   __device__ bool table_ok(u32) const { return false; }
+  // Stream: the row is read by a whole group of lanes at once (non-temporal
+  // loads); a lane walking its own short row reads it cached instead
+  template <bool Stream = true>
   __device__ bool accept_e(u32 v, const u32* col, u64 e, u32& key, int& mult) const {
-    const u32 u = col[e];
+    const u32 u = Stream ? ld_row(col + e) : col[e];
     if (f[u] != f[v]) return false;
     key = s[u];
     mult = (u == v) ? 2 : 1;

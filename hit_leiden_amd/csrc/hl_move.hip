@@ -433,7 +433,7 @@ __global__ __launch_bounds__(256) void k_prop2_first(P pol, Csr g, const u32* li
         const double w = g.w[off + j];
         u32 key;
         int mult;
-        if (!pol.accept_e(v, g.col, off + j, key, mult)) continue;
+        if (!pol.template accept_e<false>(v, g.col, off + j, key, mult)) continue;
         bool found = false;
 #pragma unroll
         for (u32 q = 0; q < SMALL_MAX2; ++q)
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(256) void k_prop2_wave(P pol, Csr g, const u32* lis
         for (int r = 0; r < 4; ++r) {
           const u32 j = j0 + r * G;
           ok[r] = j < deg;
-          ww[r] = ok[r] ? g.w[off + j] : 0.0;
+          ww[r] = ok[r] ? ld_row(g.w + off + j) : 0.0;
           ok[r] = ok[r] && pol.accept_e(v, g.col, off + j, kk[r], mm[r]);
         }
 #pragma unroll
@@ -644,7 +644,7 @@ __global__ __launch_bounds__(1024) void k_prop2_lds(P pol, Csr g, const u32* lis
       u32 key;
       int mult;
       if (!pol.accept_e(v, g.col, off + j, key, mult)) continue;
-      const double w = g.w[off + j];
+      const double w = ld_row(g.w + off + j);
       const u32 h = tab_insert(K, cap, key);
       atomicAdd(&V[h], w);
       if (mult == 2) atomicAdd(&V[h], w);
@@ -794,7 +794,7 @@ __global__ __launch_bounds__(T) void k_prop2_hub(P pol, Csr g, const u32* list, 
       for (int r = 0; r < 4; ++r) {
         const u32 j = j0 + r * T;
         ok[r] = j < deg;
-        ww[r] = ok[r] ? g.w[off + j] : 0.0;
+        ww[r] = ok[r] ? ld_row(g.w + off + j) : 0.0;
         ok[r] = ok[r] && pol.accept_e(v, g.col, off + j, kk[r], mm[r]);
       }
       bool spilled = false;
@@ -1039,7 +1039,7 @@ __global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list
         for (int r = 0; r < 4; ++r) {
           const u32 j = j0 + r * G;
           ok[r] = j < deg;
-          ww[r] = ok[r] ? g.w[off + j] : 0.0;
+          ww[r] = ok[r] ? ld_row(g.w + off + j) : 0.0;
           ok[r] = ok[r] && pol.accept_e(v, g.col, off + j, kk[r], mm[r]);
         }
 #pragma unroll
@@ -1139,7 +1139,7 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
       for (int r = 0; r < 4; ++r) {
         const u32 j = j0 + r * blockDim.x;
         ok[r] = j < deg;
-        ww[r] = ok[r] ? g.w[off + j] : 0.0;
+        ww[r] = ok[r] ? ld_row(g.w + off + j) : 0.0;
         ok[r] = ok[r] && pol.accept_e(v, g.col, off + j, kk[r], mm[r]);
       }
 #pragma unroll
@@ -1515,8 +1515,8 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
         }
         const u32 v = snode[lo], cto = sto[lo];
         const u64 e = m.g.row_off[v] + (t - pre[lo]);
-        u = m.g.col[e];
-        if (m.ht.bits && u != v) ht_move(m.ht, u, sfrom[lo], cto, m.g.w[e]);
+        u = ld_row(m.g.col + e);
+        if (m.ht.bits && u != v) ht_move(m.ht, u, sfrom[lo], cto, ld_row(m.g.w + e));
         if (skd[lo]) {
           // a repeat move: K marks done; a neighbour already claimed needs no gather
           if (!marked(mark, u) && ld_agent(m.fs ? fs_f(m.fs, u) : &m.f[u]) != cto) want = claim(mark, u);
@@ -1628,7 +1628,7 @@ __global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u3
         ct[r] = flat.to[lo];
         rep[r] = flat.kd && flat.kd[lo];
         ee[r] = m.g.row_off[vv[r]] + (t - flat.pre[lo]);
-        uu[r] = m.g.col[ee[r]];
+        uu[r] = ld_row(m.g.col + ee[r]);
       }
     }
     // a repeat move's K marks are done, and its next-active mark of a
@@ -1654,7 +1654,7 @@ __global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u3
     for (u32 r = 0; r < R; ++r) {
       if (!ok[r]) continue;
       const u32 u = uu[r], v = vv[r];
-      if (m.ht.bits && flat.from && u != v) ht_move(m.ht, u, flat.from[seg[r]], ct[r], m.g.w[ee[r]]);
+      if (m.ht.bits && flat.from && u != v) ht_move(m.ht, u, flat.from[seg[r]], ct[r], ld_row(m.g.w + ee[r]));
       if (need[r] && fu[r] != ct[r] && claim(mark, u)) sv_[atomicAdd(&scnt, 1u)] = u;
       if (need[r] && !rep[r] && su[r] == sv[r]) {
         m.K[u] = 1;
