@@ -197,6 +197,7 @@ struct MoveScratch {
   DevBuf<u32> ck;  // neighbour-community cache of the level in movement
   DevBuf<u32> mfrom;  // source communities of the moves marked over the grid
   DevBuf<u8> mkd, kd;  // repeat-move flags of those moves (FlatMarks::kd), both paths
+  DevBuf<u32> und;     // undecided counts of a chunk of decoupling priority rounds
   // hub tables (HubTables)
   DevBuf<u32> ht_bits;
   DevBuf<HtSlot> ht_slot;

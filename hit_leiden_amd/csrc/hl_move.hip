@@ -1747,7 +1747,12 @@ __global__ void k_cand_arrange(const u32* order2, const u32* order1, u32 c, Cand
     state[p] = kUndec;
   }
 }
-__global__ void k_reset4(u64 c, const u32* from, const u32* to, u32* a, u32* b, u32* x, u32* y) {
+// The priority-round kernels take the previous round's undecided count
+// (prev, or null for the first round of a chunk) and do nothing once it is
+// zero, so a chunk of rounds is enqueued with one read-back at its end.
+__global__ void k_reset4(u64 c, const u32* from, const u32* to, u32* a, u32* b, u32* x, u32* y,
+                         const u32* prev) {
+  if (prev && *prev == 0) return;
   for (u64 p = gtid(); p < c; p += gstride()) {
     u32 f = from[p], t = to[p];
     a[f] = kNone; a[t] = kNone;
@@ -1779,8 +1784,9 @@ __device__ __forceinline__ void lds_min_insert(u32* K, u32* M, u32 key, u32 p) {
   }
 }
 __global__ __launch_bounds__(256) void k_mark4(u64 c, const u32* from, const u32* to, const u32* state, u32* minUto,
-                                               u32* minUfrom, u32* minAto, u32* minAfrom) {
+                                               u32* minUfrom, u32* minAto, u32* minAfrom, const u32* prev) {
   __shared__ u32 K[4 * kMarkSlots], M[4 * kMarkSlots];
+  if (prev && *prev == 0) return;
   for (u64 base = u64(blockIdx.x) * kMarkChunk; base < c; base += u64(gridDim.x) * kMarkChunk) {
     for (u32 q = threadIdx.x; q < 4 * kMarkSlots; q += blockDim.x) {
       K[q] = kNone;
@@ -1811,7 +1817,8 @@ __global__ __launch_bounds__(256) void k_mark4(u64 c, const u32* from, const u32
   }
 }
 __global__ void k_decide4(u64 c, const u32* from, const u32* to, u32* state, const u32* minUto, const u32* minUfrom,
-                          const u32* minAto, const u32* minAfrom, u32* undec) {
+                          const u32* minAto, const u32* minAfrom, u32* undec, const u32* prev) {
+  if (prev && *prev == 0) return;
   u32 local = 0;
   for (u64 p = gtid(); p < c; p += gstride()) {
     if (state[p] != kUndec) continue;
@@ -2197,29 +2204,46 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
       hipLaunchKernelGGL(k_cand_arrange, fixed_grid(c), 256, 0, s, ms.o2.p, ms.o1.p, c, cand, m.f, ms.node.p,
                          ms.from.p, ms.to.p, ms.state.p, ms.dstat.p);
       HL_LAUNCH_CHECK();
+      // priority rounds in chunks of up to kDecChunk, each chunk read back once
+      // (a round after one that decided everything does nothing)
+      constexpr int kDecChunk = 4;
+      ms.und.reserve(kDecChunk, s);
       int iters = 0;
       while (true) {
-        HL_PROF(P_DEC_MULTI, 0, s);
-        hipLaunchKernelGGL(k_reset4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.minUto.p,
-                           ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p);
-        hipLaunchKernelGGL(k_mark4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p, ms.minUto.p,
-                           ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p);
-        HL_CHECK(hipMemsetAsync(status + 1, 0, sizeof(u32), s));
-        hipLaunchKernelGGL(k_decide4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p,
-                           ms.minUto.p, ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p, status + 1);
-        HL_LAUNCH_CHECK();
-        u32 und = 0;
-        d2h_async(&und, status + 1, sizeof(u32), s);
+        const int chunk = std::min<int>(kDecChunk, int(th.dec_rounds) - iters);
+        u32 und_h[kDecChunk] = {};
+        {
+          HL_PROF(P_DEC_MULTI, 0, s);
+          fill_u32(ms.und.p, 0, u64(chunk), s);
+          for (int r = 0; r < chunk; ++r) {
+            const u32* prev = r ? ms.und.p + r - 1 : nullptr;
+            hipLaunchKernelGGL(k_reset4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.minUto.p,
+                               ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p, prev);
+            hipLaunchKernelGGL(k_mark4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p,
+                               ms.minUto.p, ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p, prev);
+            hipLaunchKernelGGL(k_decide4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p,
+                               ms.minUto.p, ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p, ms.und.p + r, prev);
+            HL_LAUNCH_CHECK();
+          }
+          d2h_async(und_h, ms.und.p, size_t(chunk) * sizeof(u32), s);
+        }
         stream_sync(s);
-        if (st) ++st->decide_rounds;
-        if (und == 0) break;
-        if (++iters >= int(th.dec_rounds)) {
+        int ran = chunk;  // the rounds that decided something: up to the first with none undecided
+        for (int r = 0; r < chunk; ++r)
+          if (und_h[r] == 0) {
+            ran = r + 1;
+            break;
+          }
+        if (st) st->decide_rounds += u32(ran);
+        iters += ran;
+        if (und_h[ran - 1] == 0) break;
+        if (iters >= int(th.dec_rounds)) {
           // the undecided rest in priority order, against flags of the accepted
           // moves in minAfrom (sources) and minAto (targets); the sort buffers
           // are free by now: k32b holds the undecided flags, v32 their list,
           // k64b its count
           hipLaunchKernelGGL(k_reset4, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.minUto.p,
-                             ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p);
+                             ms.minUfrom.p, ms.minAto.p, ms.minAfrom.p, static_cast<const u32*>(nullptr));
           u8* undec = reinterpret_cast<u8*>(ms.k32b.p);
           hipLaunchKernelGGL(k_settle_prep, fixed_grid(c), 256, 0, s, u64(c), ms.from.p, ms.to.p, ms.state.p,
                              ms.minAfrom.p, ms.minAto.p, undec);
