@@ -2522,15 +2522,79 @@ void max_u32(const u32* a, u64 n, u32* out, hipStream_t s) {
   HL_LAUNCH_CHECK();
 }
 
+// ---- one-launch forms of select and scan for small inputs: each launch costs
+// a few microseconds whatever its size, and the library forms take two or three
+// launches; batches of a few thousand records run dozens of these.
+
+// ordered select of up to kSelSmall flags by one workgroup of 1,024 threads
+constexpr u32 kSelSmall = 1024 * kSelPer;
+__global__ __launch_bounds__(1024) void k_sel_small(const u8* flags, u64 n, u32* out, unsigned long long* total) {
+  __shared__ u32 wsum[16];
+  u8 f[kSelPer];
+  const u64 i0 = u64(threadIdx.x) * kSelPer;
+  const u32 c = sel_load(flags, n, i0, f);
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  u32 inc = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const u32 y = __shfl_up(inc, o, 64);
+    if (lane >= u32(o)) inc += y;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  u32 wbase = 0;
+  for (u32 i = 0; i < wv; ++i) wbase += wsum[i];
+  u64 pos = wbase + inc - c;
+#pragma unroll
+  for (u32 j = 0; j < kSelPer; ++j)
+    if (f[j]) out[pos++] = u32(i0 + j);
+  if (threadIdx.x == 1023) *total = pos;
+}
+
+// exclusive scan of up to kScanSmall u64 by one workgroup, out[n] = the total
+constexpr u32 kScanPer = 16, kScanSmall = 1024 * kScanPer;
+__global__ __launch_bounds__(1024) void k_scan_small(const u64* in, u64* out, u64 n) {
+  __shared__ u64 wsum[16];
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const u64 i0 = u64(threadIdx.x) * kScanPer;
+  u64 v[kScanPer], sum = 0;
+#pragma unroll
+  for (u32 j = 0; j < kScanPer; ++j) {
+    v[j] = i0 + j < n ? in[i0 + j] : 0;
+    sum += v[j];
+  }
+  u64 inc = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const u32 lo = __shfl_up(u32(inc), o, 64), hi = __shfl_up(u32(inc >> 32), o, 64);
+    if (lane >= u32(o)) inc += (u64(hi) << 32) | lo;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  u64 base = 0;
+  for (u32 i = 0; i < wv; ++i) base += wsum[i];
+  u64 run = base + inc - sum;
+#pragma unroll
+  for (u32 j = 0; j < kScanPer; ++j)
+    if (i0 + j < n) {
+      out[i0 + j] = run;
+      run += v[j];
+    }
+  if (threadIdx.x == 1023) out[n] = base + inc;
+}
+
 void select_flagged_async(const u8* flags, u64 n, u32* out, Temp& tmp, u64* cnt_dev, hipStream_t s) {
   if (n == 0) {
     hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, cnt_dev, u64(0), u64(1));
     HL_LAUNCH_CHECK();
     return;
   }
+  HL_PROF(P_SELECT, n, s);
+  if (n <= kSelSmall) {
+    hipLaunchKernelGGL(k_sel_small, 1, 1024, 0, s, flags, n, out, reinterpret_cast<unsigned long long*>(cnt_dev));
+    HL_LAUNCH_CHECK();
+    return;
+  }
   const u64 nb = (n + kSelChunk - 1) / kSelChunk;
   u32* bcnt = static_cast<u32*>(tmp.get(nb * sizeof(u32), s));
-  HL_PROF(P_SELECT, n, s);
   hipLaunchKernelGGL(k_sel_count, dim3(unsigned(nb)), dim3(256), 0, s, flags, n, bcnt);
   hipLaunchKernelGGL(k_sel_write, dim3(unsigned(nb)), dim3(256), 0, s, flags, n, bcnt, out,
                      reinterpret_cast<unsigned long long*>(cnt_dev));
@@ -2540,15 +2604,7 @@ void select_flagged_async(const u8* flags, u64 n, u32* out, Temp& tmp, u64* cnt_
 u64 select_flagged(const u8* flags, u64 n, u32* out, Temp& tmp, DevBuf<u64>& cnt, hipStream_t s) {
   if (n == 0) return 0;
   cnt.reserve(1, s);
-  const u64 nb = (n + kSelChunk - 1) / kSelChunk;
-  u32* bcnt = static_cast<u32*>(tmp.get(nb * sizeof(u32), s));
-  {
-    HL_PROF(P_SELECT, n, s);
-    hipLaunchKernelGGL(k_sel_count, dim3(unsigned(nb)), dim3(256), 0, s, flags, n, bcnt);
-    hipLaunchKernelGGL(k_sel_write, dim3(unsigned(nb)), dim3(256), 0, s, flags, n, bcnt, out,
-                       reinterpret_cast<unsigned long long*>(cnt.p));
-    HL_LAUNCH_CHECK();
-  }
+  select_flagged_async(flags, n, out, tmp, cnt.p, s);
   u64 c = 0;
   d2h_async(&c, cnt.p, sizeof(u64), s);
   stream_sync(s);
@@ -2561,10 +2617,15 @@ void exclusive_scan_u64_async(const u64* in, u64* out, u64 n, Temp& tmp, hipStre
     HL_LAUNCH_CHECK();
     return;
   }
+  HL_PROF(P_SCAN, 16 * n, s);
+  if (n <= kScanSmall) {
+    hipLaunchKernelGGL(k_scan_small, 1, 1024, 0, s, in, out, n);
+    HL_LAUNCH_CHECK();
+    return;
+  }
   size_t bytes = 0;
   HL_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, u64(0), n, rocprim::plus<u64>(), s));
   void* t = tmp.get(bytes, s);
-  HL_PROF(P_SCAN, 16 * n, s);
   HL_CHECK(rocprim::exclusive_scan(t, bytes, in, out, u64(0), n, rocprim::plus<u64>(), s));
   hipLaunchKernelGGL(k_scan_tail, 1, 1, 0, s, in, out, n);
   HL_LAUNCH_CHECK();
@@ -2577,13 +2638,7 @@ u64 exclusive_scan_u64(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s)
     stream_sync(s);
     return 0;
   }
-  size_t bytes = 0;
-  HL_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, u64(0), n, rocprim::plus<u64>(), s));
-  void* t = tmp.get(bytes, s);
-  HL_PROF(P_SCAN, 16 * n, s);
-  HL_CHECK(rocprim::exclusive_scan(t, bytes, in, out, u64(0), n, rocprim::plus<u64>(), s));
-  hipLaunchKernelGGL(k_scan_tail, 1, 1, 0, s, in, out, n);
-  HL_LAUNCH_CHECK();
+  exclusive_scan_u64_async(in, out, n, tmp, s);
   u64 total = 0;
   d2h_async(&total, out + n, sizeof(u64), s);
   stream_sync(s);

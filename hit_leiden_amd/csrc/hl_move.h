@@ -206,6 +206,14 @@ struct MoveScratch {
   DevBuf<u64> fs;  // (s << 32) | f of the level in movement
   size_t target_cap = 0;
   u32 mark_n = 0;  // vertices the mark bitmap covers
+  // pinned copies of two rounds' counters and the events that mark them
+  // written: the next round is enqueued before the host waits for this one's
+  u32* snap = nullptr;
+  hipEvent_t snap_ev[2] = {nullptr, nullptr};
+  MoveScratch() = default;
+  MoveScratch(const MoveScratch&) = delete;
+  MoveScratch& operator=(const MoveScratch&) = delete;
+  ~MoveScratch();
   void ensure(u64 n, u64 comm_cap, hipStream_t s);
   void trim();  // release the candidate-sized buffers of the multi-kernel path
   // clear the list-deduplication bitmap `mark` for a new round (a fill kernel)

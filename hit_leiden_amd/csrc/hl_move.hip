@@ -13,6 +13,8 @@
 // Lists are deduplicated with a bitmap of claimed vertices, cleared per round.
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include <cstring>
+
 #include "hl_move.h"
 
 namespace hl {
@@ -1208,7 +1210,8 @@ __device__ __forceinline__ bool pri_less(u64 ka, u32 na, u64 kb, u32 nb) { retur
 __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32* minUto, u32* minUfrom,
                                                      u32* minAto, u32* minAfrom, u32* mark, u32* next,
                                                      u32* next_cnt, u32* status, unsigned long long* dst,
-                                                     u32* cur_cnt, u32* bucket_counts, FlatMarks flat) {
+                                                     u32* cur_cnt, u32* bucket_counts, FlatMarks flat,
+                                                     u32* round_cand) {
   __shared__ u64 skey[kDecSmall];
   __shared__ u32 snode[kDecSmall];
   __shared__ u32 sidx[kDecSmall];
@@ -1222,11 +1225,16 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
   __shared__ u64 wsum[16], mtotal;
   const u32 c = *cand.cnt;
   const u32 tid = threadIdx.x;
+  if (tid == 0) *round_cand = c;  // read by the host with the round's other counters
   if (c > kDecSmall) {
     if (tid == 0) {
-      status[0] = 1;  // multi-kernel path
+      status[0] = 1;  // multi-kernel path: it keeps the candidates and their count
       *flat.c = 0;
       if (m.ck) *flat.ck_c = 0;
+      // the round's list and bucket counts are spent: a round enqueued after
+      // this one finds them empty
+      *cur_cnt = 0;
+      clear_bucket_counts(bucket_counts);
     }
     return;
   }
@@ -1949,12 +1957,22 @@ void MoveScratch::ensure(u64 n, u64 comm_cap, hipStream_t s) {
   mfrom.reserve(kDecSmall + 1, s);
   mpre.reserve(kDecSmall + 2, s);
   mkd.reserve(kDecSmall + 1, s);
+  if (!snap) {
+    HL_CHECK(hipHostMalloc((void**)&snap, 2 * 16 * sizeof(u32), hipHostMallocDefault));
+    for (hipEvent_t& e : snap_ev) HL_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   if (minUto.cap < comm_cap) {
     minUto.reserve(comm_cap, s);
     minUfrom.reserve(comm_cap, s);
     minAto.reserve(comm_cap, s);
     minAfrom.reserve(comm_cap, s);
   }
+}
+
+MoveScratch::~MoveScratch() {
+  if (snap) (void)hipHostFree(snap);
+  for (hipEvent_t e : snap_ev)
+    if (e) (void)hipEventDestroy(e);
 }
 
 void MoveScratch::trim() {
@@ -1971,7 +1989,11 @@ void MoveScratch::fresh_marks(hipStream_t s) { fill_u32(mark.p, 0, u64(mark_n) /
 template <class P>
 static void propose_round(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u64 list_cap, u32 maxdeg,
                           MoveScratch& ms, const Thresholds& th, unsigned long long* st, unsigned long long* tst,
-                          hipStream_t s) {
+                          hipStream_t s, u64 grid_work = 0) {
+  // list_cap: at least the list's count, the stride of the bucket lists;
+  // grid_work: the count the grids are sized for (every kernel strides over
+  // the count it reads on the device, so a short estimate only costs time)
+  if (grid_work == 0) grid_work = list_cap;
   // counters [4..4 + kBuckets) bucket counts, [kOvfSlot..kOvfSlot + kOvfLists)
   // overflow counts; all zero on entry.  st: the proposal block of the device
   // work counters; tst: the first-try block (3 counters per bucket)
@@ -1983,14 +2005,14 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
   unsigned long long* cst = st + 2 * kBuckets;
   {
     HL_PROF(P_PROP_SMALL, 0, s);
-    hipLaunchKernelGGL(k_prop2_first<P>, fixed_grid(list_cap), 256, 0, s, pol, g, list, cnt_dev, small, half, wave,
+    hipLaunchKernelGGL(k_prop2_first<P>, fixed_grid(grid_work), 256, 0, s, pol, g, list, cnt_dev, small, half, wave,
                        ms.blists.p, list_cap, counts, out, st);
     HL_LAUNCH_CHECK();
   }
   if constexpr (P::kTables) {
     if (pol.ht.bits) {
       HL_PROF(P_PROP_TABLE, 0, s);
-      hipLaunchKernelGGL(k_prop2_ht<P>, dim3(fixed_grid(list_cap, 4, 2048)), dim3(256), 0, s, pol, g,
+      hipLaunchKernelGGL(k_prop2_ht<P>, dim3(fixed_grid(grid_work, 4, 2048)), dim3(256), 0, s, pol, g,
                          ms.blists.p + kHtList * list_cap, counts + kHtList, out, ms.dstat.p + DS_HT);
       HL_LAUNCH_CHECK();
     }
@@ -2004,26 +2026,26 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
   if (maxdeg > small) {
     {
       HL_PROF(P_TRY_HALF, 0, s);
-      hipLaunchKernelGGL((k_prop2_tab<P, 32, 64>), dim3(fixed_grid(list_cap, 8, kTabBlocks)), dim3(256), 0, s, pol,
+      hipLaunchKernelGGL((k_prop2_tab<P, 32, 64>), dim3(fixed_grid(grid_work, 8, kTabBlocks)), dim3(256), 0, s, pol,
                          g, ms.blists.p + list_cap, counts + 1, ovf[0], ovf_cnt + 0, out, cst + 1, tst + 3 * 1);
       HL_LAUNCH_CHECK();
     }
     {
       HL_PROF(P_PROP_WAVE, 0, s);
-      hipLaunchKernelGGL((k_prop2_wave<P, 32, 2 * kHalfDeg>), dim3(fixed_grid(list_cap, 8, kHalfBlocks)), dim3(256),
+      hipLaunchKernelGGL((k_prop2_wave<P, 32, 2 * kHalfDeg>), dim3(fixed_grid(grid_work, 8, kHalfBlocks)), dim3(256),
                          0, s, pol, g, ovf[0], ovf_cnt + 0, out, cst + 1);
       HL_LAUNCH_CHECK();
     }
     if (maxdeg > half) {
       {
         HL_PROF(P_TRY_WAVE, 0, s);
-        hipLaunchKernelGGL((k_prop2_tab<P, 64, 128>), dim3(fixed_grid(list_cap, 4, kTabBlocks)), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_prop2_tab<P, 64, 128>), dim3(fixed_grid(grid_work, 4, kTabBlocks)), dim3(256), 0, s,
                            pol, g, ms.blists.p + 2 * list_cap, counts + 2, ovf[1], ovf_cnt + 1, out, cst + 2,
                            tst + 3 * 2);
         HL_LAUNCH_CHECK();
       }
       HL_PROF(P_PROP_WAVE, 0, s);
-      hipLaunchKernelGGL((k_prop2_wave<P, 64, WAVE_HW2>), dim3(fixed_grid(list_cap, 4, kWaveBlocks)), dim3(256), 0, s,
+      hipLaunchKernelGGL((k_prop2_wave<P, 64, WAVE_HW2>), dim3(fixed_grid(grid_work, 4, kWaveBlocks)), dim3(256), 0, s,
                          pol, g, ovf[1], ovf_cnt + 1, out, cst + 2);
       HL_LAUNCH_CHECK();
     }
@@ -2036,7 +2058,7 @@ static void propose_round(const P& pol, const Csr& g, const u32* list, const u32
       HL_LAUNCH_CHECK();
     }
     HL_PROF(P_PROP_LDS, 0, s);
-    hipLaunchKernelGGL(k_prop2_lds<P>, dim3(fixed_grid(list_cap, 1, 256)), dim3(1024), 0, s, pol, g, ovf[2],
+    hipLaunchKernelGGL(k_prop2_lds<P>, dim3(fixed_grid(grid_work, 1, 256)), dim3(1024), 0, s, pol, g, ovf[2],
                        ovf_cnt + 2, out, cst + 3);
     HL_LAUNCH_CHECK();
   }
@@ -2145,19 +2167,25 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   if (st) st->active = host_cnt;
   if (hook && hook->fn) hook->fn(hook->ctx, cur, host_cnt);
   u32 rounds = 0;
-  while (host_cnt > 0 && int(rounds) < max_rounds) {
-    if (st) st->evaluated += host_cnt;
-    u32* cur_cnt = lcnt[ci];
-    u32* next_cnt = lcnt[ci ^ 1];
-    propose_round(pol, m.g, cur, cur_cnt, host_cnt, maxdeg, ms, th, ms.dstat.p + DS_MOVE,
-                  ms.dstat.p + DS_TRY_MOVE, s);
+  // One round: proposals over `lst` (count lcnt[cidx], read on the device),
+  // the one-workgroup decoupling with its marks into `out` (count
+  // lcnt[cidx ^ 1]), then the grid-wide marks of long neighbour lists.  The
+  // bucket lists have stride n + 1, so a round can be enqueued before its
+  // count is known; grid_work sizes its grids.
+  const u64 list_cap = u64(n) + 1;
+  u32* ccand = ms.counters.p + 3;  // candidates of the round (k_dec_small)
+  auto enqueue_round = [&](const u32* lst, u32* out, int cidx, u64 grid_work) {
+    u32* c_cnt = lcnt[cidx];
+    u32* n_cnt = lcnt[cidx ^ 1];
+    propose_round(pol, m.g, lst, c_cnt, list_cap, maxdeg, ms, th, ms.dstat.p + DS_MOVE, ms.dstat.p + DS_TRY_MOVE, s,
+                  grid_work);
     ms.fresh_marks(s);
     CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
     const FlatMarks fm{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 12, ms.mfrom.p, ms.mkd.p};
     {
       HL_PROF(P_DECOUPLE, 0, s);
       hipLaunchKernelGGL(k_dec_small, 1, 1024, 0, s, m, cand, ms.minUto.p, ms.minUfrom.p, ms.minAto.p,
-                         ms.minAfrom.p, ms.mark.p, nxt, next_cnt, status, ms.dstat.p, cur_cnt, bcounts, fm);
+                         ms.minAfrom.p, ms.mark.p, out, n_cnt, status, ms.dstat.p, c_cnt, bcounts, fm, ccand);
       HL_LAUNCH_CHECK();
     }
     if (m.ck) {
@@ -2170,15 +2198,42 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
       // grid (the kernel returns at once when the workgroup marked them all or
       // the candidates took the multi-kernel path: the count it reads is 0)
       HL_PROF(P_MARK_MOVED, 0, s);
-      launch_marks(th, m, fm, ms.mark.p, nxt, next_cnt, s);
+      launch_marks(th, m, fm, ms.mark.p, out, n_cnt, s);
       HL_LAUNCH_CHECK();
     }
+  };
+  // The next round is enqueued before the host waits for this one's counters
+  // when the last round's candidates fitted the one-workgroup decoupling with
+  // room to spare (no served-form hook reads each round).  Such a round reads
+  // this round's output count on the device: after the last round, or after
+  // one that takes the multi-kernel path (whose k_dec_small leaves the list
+  // and bucket counts at zero), it finds an empty list and does nothing, and
+  // then runs again for real.
+  const bool may_queue = !(hook && hook->fn);
+  bool queued = false;
+  u32 last_cand = kNone;
+  int slot = 0;
+  while (host_cnt > 0 && int(rounds) < max_rounds) {
+    if (st) st->evaluated += host_cnt;
+    u32* cur_cnt = lcnt[ci];
+    u32* next_cnt = lcnt[ci ^ 1];
+    CandOut cand{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
+    if (!queued) enqueue_round(cur, nxt, ci, host_cnt);
+    u32* snap = ms.snap + 16 * slot;
+    HL_CHECK(hipMemcpyAsync(snap, ms.counters.p, 16 * sizeof(u32), hipMemcpyDeviceToHost, s));
+    HL_CHECK(hipEventRecord(ms.snap_ev[slot], s));
+    queued = may_queue && int(rounds) + 1 < max_rounds && last_cand <= kDecSmall / 2;
+    if (queued) enqueue_round(nxt, cur, ci ^ 1, host_cnt);
+    HL_CHECK(hipEventSynchronize(ms.snap_ev[slot]));
+    g_alloc.syncs += 1;
     u32 h[16];
-    d2h_async(h, ms.counters.p, 16 * sizeof(u32), s);
-    stream_sync(s);
+    std::memcpy(h, snap, sizeof(h));
+    slot ^= 1;
     u32 accepted = h[15];
     u32 next_host = h[2 * (ci ^ 1)];
+    last_cand = h[3];
     if (h[14] == 1) {
+      queued = false;  // a round enqueued behind this one did nothing: it runs again
       // many candidates: sort by node, then stably by gain; priority rounds over the grid
       const u32 c = h[1];
       ms.k32.reserve(c, s);
