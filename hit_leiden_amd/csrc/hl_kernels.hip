@@ -1786,7 +1786,9 @@ __global__ void k_gather_u32(u32* cur, const u32* map, u32 n) {
   for (u64 v = gtid(); v < n; v += gstride()) cur[v] = map[cur[v]];
 }
 // Intra-community weight (every edge from both ends, a self-loop's entry
-// twice; compute_modularity, hit_leiden.rs:121-161).  Each wavefront takes 64
+// twice; compute_modularity, hit_leiden.rs:121-161).  Every edge is stored in
+// both of its rows, so 2w over the entries with u >= v (a self-loop is the one
+// entry u = v) is the same sum with half the label gathers.  Each wavefront takes 64
 // consecutive vertices and walks their rows as one concatenated sequence, 64
 // consecutive entries per step (the row of entry t is found by a binary search
 // over the lanes' degree prefix), so the lanes load neighbouring addresses and
@@ -1833,10 +1835,7 @@ __global__ __launch_bounds__(256) void k_intra(Csr g, const u32* lab, double* ac
       if (t < total) {
         const u64 e = ro + (t - rp);
         const u32 u = g.col[e];
-        if (lab[u] == rl) {
-          const double w = g.w[e];
-          local += (u == base + r) ? 2.0 * w : w;
-        }
+        if (u >= base + r && lab[u] == rl) local += 2.0 * g.w[e];
       }
     }
     unsigned long long m = __ballot(wide);
@@ -1848,10 +1847,7 @@ __global__ __launch_bounds__(256) void k_intra(Csr g, const u32* lab, double* ac
       const u32 bdeg = __shfl(deg, src, 64), blv = __shfl(lv, src, 64);
       for (u32 j = lane; j < bdeg; j += 64) {
         const u32 u = g.col[boff + j];
-        if (lab[u] == blv) {
-          const double w = g.w[boff + j];
-          local += (u == bv) ? 2.0 * w : w;
-        }
+        if (u >= bv && lab[u] == blv) local += 2.0 * g.w[boff + j];
       }
     }
   }
@@ -1872,10 +1868,7 @@ __global__ __launch_bounds__(256) void k_intra_hubs(Csr g, const u32* lab, const
     const u32 lv = lab[v];
     for (u64 j = b + threadIdx.x; j < end; j += 256) {
       const u32 u = g.col[off + j];
-      if (lab[u] == lv) {
-        const double w = g.w[off + j];
-        local += (u == v) ? 2.0 * w : w;
-      }
+      if (u >= v && lab[u] == lv) local += 2.0 * g.w[off + j];
     }
   }
   double t = block_sum(local);
@@ -2028,10 +2021,7 @@ __global__ void k_intra_graphs(Csr g, const u32* lab, u64 chunk, double* intra) 
     const u32 deg = g.row_len[v], lv = lab[v];
     for (u32 j = 0; j < deg; ++j) {
       const u32 u = g.col[off + j];
-      if (lab[u] == lv) {
-        const double w = g.w[off + j];
-        acc += (u == v) ? 2.0 * w : w;
-      }
+      if (u >= v && lab[u] == lv) acc += 2.0 * g.w[off + j];
     }
   }
   if (acc != 0.0) atomicAdd(&intra[cur], acc);
