@@ -418,6 +418,8 @@ def main():
                     help="pick the degree-bucket thresholds on the warmup batches first")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run device-to-device copies of this many bytes (counter calibration)")
+    ap.add_argument("--gather-probe", action="store_true",
+                    help="only run the random-gather ceiling probe (hit_leiden_amd/hl_gather_probe)")
     ap.add_argument("--rocprof-summary", default=None, metavar="DIR",
                     help="only print a text summary of a rocprofv3 csv output directory")
     ap.add_argument("--kernel", default=None, help="kernel regex for --rocprof-summary")
@@ -431,6 +433,12 @@ def main():
         from hit_leiden_amd import profiling
         print(profiling.summary_text(args.rocprof_summary, args.command, pattern=args.kernel), end="")
         return
+    if args.gather_probe:
+        import subprocess
+        exe = os.path.join(ROOT, "hit_leiden_amd", "hl_gather_probe")
+        if not os.path.exists(exe):
+            raise SystemExit("hl_gather_probe is not built (python -m hit_leiden_amd.build)")
+        sys.exit(subprocess.run([exe]).returncode)
     if args.copy_probe:
         from hit_leiden_amd import profiling
         profiling.copy_probe(args.copy_probe)

@@ -58,9 +58,22 @@ def build_gen(verbose=False):
         print("built", GEN_SO)
 
 
+PROBE_EXE = os.path.join(HERE, "hl_gather_probe")
+
+
+def build_probe(verbose=False):
+    """The random-gather ceiling probe (bench.py --gather-probe)."""
+    tmp = PROBE_EXE + ".tmp"
+    _run_parallel([[HIPCC, *HIP_FLAGS, os.path.join(CSRC, "hl_gather_probe.hip"), "-o", tmp]])
+    os.replace(tmp, PROBE_EXE)
+    if verbose:
+        print("built", PROBE_EXE)
+
+
 def build_all(verbose=False):
     build_gen(verbose)
     build_engine(verbose)
+    build_probe(verbose)
 
 
 if __name__ == "__main__":
