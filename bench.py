@@ -97,13 +97,32 @@ def cpu_baseline(w, levels, gamma, budget_s, threads):
             "load_seconds": load_s, "modularity": q}
 
 
-def cpu_det_child(config, levels, gamma, budget_s, nbatches):
-    """The deterministic oracle (RunMode::Deterministic, the reference's default
-    mode, one thread) on the workload's graph and batches; prints one JSON line.
-    Runs in a child process that never touches the GPU."""
+CPU_VARIANTS = {
+    # name: (oracle mode, threads (0: the box's), description)
+    "deterministic_1_thread": ("DET", 1, "oracle deterministic mode (RunMode::Deterministic), one thread"),
+    "throughput_as_written": ("PAR|AS_WRITTEN", 0, "oracle throughput mode with the reference's own "
+                              "recomputations (CSR rebuilt through a hash map per batch and level, O(n) scans "
+                              "per refined vertex and candidate subcommunity)"),
+    "deterministic_as_written_1_thread": ("DET|AS_WRITTEN", 1, "oracle deterministic mode with the reference's "
+                                          "own recomputations, one thread"),
+}
+
+
+def cpu_variant_child(config, variant, levels, gamma, budget_s, nbatches, threads):
+    """One CPU_VARIANTS entry on the workload's graph and batches; prints one
+    JSON line.  Runs in a child process that never touches the GPU."""
     from oracle import hl_oracle as O
-    w, _ = make_workload(config, nbatches, 1)
-    st = O.OracleState(levels=levels, gamma=gamma, mode=O.DET, threads=1)
+    mode_s, t, desc = CPU_VARIANTS[variant]
+    mode = 0
+    for part in mode_s.split("|"):
+        mode |= getattr(O, part)
+    threads = t or threads
+    if config == "large":
+        from hit_leiden_amd import gen
+        w = gen.large_config(scale=20, batches=nbatches)
+    else:
+        w, _ = make_workload(config, nbatches, 1)
+    st = O.OracleState(levels=levels, gamma=gamma, mode=mode, threads=threads)
     t0 = time.time()
     st.load(w.n, w.u, w.v, w.w)
     load_s = time.time() - t0
@@ -114,30 +133,37 @@ def cpu_det_child(config, levels, gamma, budget_s, nbatches):
         spent += time.perf_counter() - t
         edges += len(u)
         done += 1
+        print(json.dumps({"value": edges / spent if spent > 0 else 0.0, "unit": "edges/s", "cores": threads,
+                          "kind": "port", "sample": f"{desc} ({levels} levels)"
+                          + (" on an R-MAT scale-20 sample" if config == "large" else "")
+                          + f": load {load_s:.1f}s untimed, then the first {done} batches ({edges} edges) timed",
+                          "load_seconds": round(load_s, 3), "batches": done, "seconds_per_batch":
+                          round(spent / done, 4)}), flush=True)
         if spent >= budget_s:
             break
-    print(json.dumps({"value": edges / spent if spent > 0 else 0.0, "unit": "edges/s", "cores": 1, "kind": "port",
-                      "sample": f"oracle deterministic mode ({levels} levels), one thread: load {load_s:.1f}s "
-                                f"untimed, then the first {done} batches ({edges} edges) timed",
-                      "load_seconds": round(load_s, 3)}))
 
 
-def cpu_det_variant(args, nbatches):
-    """cpu_det_child in a child process, bounded by --det-timeout."""
+def cpu_variant(args, nbatches, variant, threads):
+    """cpu_variant_child in a child process, bounded by --det-timeout; the last
+    line it printed (the batches done by then) when the bound cuts it short."""
     import subprocess
-    if args.config == "large":
+    if args.config == "large" and variant.startswith("deterministic"):
         return {"skipped": "the deterministic sweep is superlinear on R-MAT hubs: one thread loads an R-MAT "
                            "scale-14 sample (16,384 vertices) in 31 s and a scale-18 one in more than 10 minutes"}
-    cmd = [sys.executable, os.path.abspath(__file__), "--det-child", args.config, "--levels", str(args.levels),
-           "--gamma", str(args.gamma), "--cpu-budget", str(args.det_budget), "--steps", str(nbatches),
-           "--warmup", "0"]
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", args.config, "--cpu-variant", variant,
+           "--levels", str(args.levels), "--gamma", str(args.gamma), "--cpu-budget", str(args.det_budget),
+           "--steps", str(nbatches), "--warmup", "0", "--cpu-threads", str(threads)]
+    out = ""
     try:
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=sys.stderr, timeout=args.det_timeout, text=True)
-    except subprocess.TimeoutExpired:
-        return {"skipped": f"the deterministic oracle did not finish within {args.det_timeout:.0f} s"}
-    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
-    if r.returncode != 0 or not lines:
-        return {"error": f"exit {r.returncode}"}
+        out = r.stdout
+    except subprocess.TimeoutExpired as e:
+        out = e.stdout.decode() if isinstance(e.stdout, bytes) else (e.stdout or "")
+        if not any(ln.startswith("{") for ln in out.splitlines()):
+            return {"skipped": f"the load and first batch did not finish within {args.det_timeout:.0f} s"}
+    lines = [ln for ln in out.splitlines() if ln.startswith("{")]
+    if not lines:
+        return {"error": "no result"}
     return json.loads(lines[-1])
 
 
@@ -400,10 +426,12 @@ def main():
     ap.add_argument("--gamma", type=float, default=1.0)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle batch time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--det-budget", type=float, default=10.0, help="seconds of batch time of the deterministic "
-                    "one-thread CPU variant")
-    ap.add_argument("--det-timeout", type=float, default=180.0, help="wall-time bound of that variant")
-    ap.add_argument("--det-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--det-budget", type=float, default=10.0, help="seconds of batch time of each CPU variant "
+                    "(deterministic one-thread, as-written)")
+    ap.add_argument("--det-timeout", type=float, default=60.0, help="wall-time bound of each CPU variant")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-variant", default="deterministic_1_thread", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-threads", type=int, default=1, help=argparse.SUPPRESS)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched and served forms that the plain configurations report in 'extras'")
     ap.add_argument("--extra-timeout", type=float, default=400.0, help="seconds per extra child process")
@@ -426,8 +454,9 @@ def main():
     ap.add_argument("--command", default="", help="command line recorded in --rocprof-summary")
     args = ap.parse_args()
 
-    if args.det_child:
-        cpu_det_child(args.det_child, args.levels, args.gamma, args.cpu_budget, args.steps)
+    if args.cpu_child:
+        cpu_variant_child(args.cpu_child, args.cpu_variant, args.levels, args.gamma, args.cpu_budget, args.steps,
+                          args.cpu_threads)
         return
     if args.rocprof_summary:
         from hit_leiden_amd import profiling
@@ -585,7 +614,7 @@ def main():
         else:
             out["cpu_baseline"] = cpu_baseline(w, args.levels, args.gamma, args.cpu_budget, threads)
         out["cpu_baseline"]["cpu_model"] = cpu_model()
-        out["cpu_baseline"]["variants"] = {"deterministic_1_thread": cpu_det_variant(args, nb)}
+        out["cpu_baseline"]["variants"] = {v: cpu_variant(args, nb, v, threads) for v in CPU_VARIANTS}
     else:
         out["cpu_baseline"] = None
 

@@ -41,6 +41,7 @@
 #include <cstring>
 #include <numeric>
 #include <queue>
+#include <unordered_map>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -181,6 +182,72 @@ void apply_delta(Graph& g, const std::vector<Rec>& d, u32 n_new) {
   std::sort(touched.begin(), touched.end());
   touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
   finish_degrees(g, touched);
+}
+
+// apply_delta as written (in_memory.rs:103-131): every edge of the graph into
+// a hash map keyed by (min, max) (collect_canonical_edges), the delta added
+// per key in delta order (apply_delta_edges), the graph rebuilt from the keys
+// above 1e-12 (InMemoryGraph::from) and every degree recomputed.  The map adds
+// each key's weights in the same order as apply_delta, and the rows are
+// sorted again, so the result is apply_delta's; only the cost is the
+// reference's, O(n + m) per batch and level (the CPU baseline's variant (i)).
+void apply_delta_rebuild(Graph& g, const std::vector<Rec>& d, u32 n_new) {
+  grow_graph(g, std::max(g.n, n_new));
+  std::unordered_map<u64, double> e;
+  e.reserve(size_t(g.twice_m > 0 ? g.n : 0) + d.size());
+  for (u32 a = 0; a < g.n; ++a)
+    for (size_t j = 0; j < g.col[a].size(); ++j) {
+      const u32 b = g.col[a][j];
+      if (a <= b) e[(u64(a) << 32) | b] = g.wt[a][j];
+    }
+  for (const Rec& r : d) {
+    const u32 a = std::min(r.u, r.v), b = std::max(r.u, r.v);
+    e[(u64(a) << 32) | b] += r.w;  // a new key starts from 0.0
+  }
+  for (u32 v = 0; v < g.n; ++v) {
+    g.col[v].clear();
+    g.wt[v].clear();
+  }
+  for (const auto& kv : e) {
+    if (!(kv.second > kEpsApply)) continue;
+    const u32 a = u32(kv.first >> 32), b = u32(kv.first & 0xffffffffu);
+    g.col[a].push_back(b);
+    g.wt[a].push_back(kv.second);
+    if (a != b) {
+      g.col[b].push_back(a);
+      g.wt[b].push_back(kv.second);
+    }
+  }
+  std::vector<std::pair<u32, double>> row;
+  for (u32 v = 0; v < g.n; ++v) {
+    row.resize(g.col[v].size());
+    for (size_t j = 0; j < row.size(); ++j) row[j] = {g.col[v][j], g.wt[v][j]};
+    std::sort(row.begin(), row.end(), [](const std::pair<u32, double>& x, const std::pair<u32, double>& y) {
+      return x.first < y.first;
+    });
+    for (size_t j = 0; j < row.size(); ++j) {
+      g.col[v][j] = row[j].first;
+      g.wt[v][j] = row[j].second;
+    }
+    g.k[v] = row_degree(g, v);
+  }
+  g.twice_m = tree_sum(g.k, g.n);
+}
+
+// The reference's O(n) scans (the CPU baseline's variant (i)): the members of
+// subcommunity sc in ascending order, and the degree total of label c summed
+// in vertex order -- the values the member index and the totals hold.
+std::vector<u32> members_scan(const std::vector<u32>& s, u32 n, u32 sc) {
+  std::vector<u32> m;
+  for (u32 i = 0; i < n; ++i)
+    if (s[i] == sc) m.push_back(i);
+  return m;
+}
+double total_scan(const Graph& g, const std::vector<u32>& lab, u32 c) {
+  double acc = 0.0;
+  for (u32 i = 0; i < g.n; ++i)
+    if (lab[i] == c) acc += g.k[i];
+  return acc;
 }
 
 Graph graph_from_edges(u32 n, u64 m, const u32* eu, const u32* ev, const double* ew) {
@@ -598,6 +665,10 @@ std::vector<uint8_t> inc_refinement(const Graph& g, const std::vector<u32>& f, s
       u32 parent = b < e ? f[ix.mem[b]] : 0;
       double d_par = parent < dparent.size() ? dparent[parent] : 0.0;
       std::vector<u32> mem(ix.mem.begin() + b, ix.mem.begin() + e);
+      if (mode & OR_AS_WRITTEN) {  // hit_leiden.rs:1604-1629: the parent and the members by O(n) scans
+        mem = members_scan(s, n, sc);
+        parent = mem.empty() ? 0 : f[mem[0]];
+      }
       double w_s = weight_to_comm(g, mem, f, parent);
       if (remove_gain(w_s, t, gamma, d_s, d_par) > kEpsTFilter) blocked[sc] = 1;
     }
@@ -680,7 +751,8 @@ std::vector<uint8_t> inc_refinement(const Graph& g, const std::vector<u32>& f, s
       if (e.first == sc) w_cur += e.second;
     u32 best = sc;
     double best_gain = 0.0;
-    const double d_par = C < dparent.size() ? dparent[C] : 0.0;
+    // hit_leiden.rs:1454-1458 sums the parent's degrees by an O(n) scan
+    const double d_par = (mode & OR_AS_WRITTEN) ? total_scan(g, f, C) : (C < dparent.size() ? dparent[C] : 0.0);
     size_t p = 0;
     while (p < buf.size()) {
       u32 cs = buf[p].first;
@@ -689,7 +761,8 @@ std::vector<uint8_t> inc_refinement(const Graph& g, const std::vector<u32>& f, s
       if (cs == sc) continue;
       double d_s = dS[cs];
       if (d_s > 0.0) {  // should_skip_candidate_t_filter
-        double w_s = weight_to_comm(g, members[cs], f, C);
+        double w_s = (mode & OR_AS_WRITTEN) ? weight_to_comm(g, members_scan(s, n, cs), f, C)  // :1535-1544
+                                            : weight_to_comm(g, members[cs], f, C);
         if (remove_gain(w_s, t, gamma, d_s, d_par) > kEpsTFilter) continue;
       }
       double gain = move_gain(acc, w_cur, t, gamma, k, dS[sc], dS[cs]);
@@ -873,7 +946,10 @@ u32 hit_leiden(State& st, const std::vector<Rec>& dG0) {
   for (int p = 0; p < P; ++p) {
     Level& lv = st.L[p];
     grow_level(lv, std::max(lv.G.n, cur_n), p + 1 < P ? &st.L[p + 1] : nullptr);
-    apply_delta(lv.G, cur, lv.G.n);
+    if (st.mode & OR_AS_WRITTEN)
+      apply_delta_rebuild(lv.G, cur, lv.G.n);
+    else
+      apply_delta(lv.G, cur, lv.G.n);
     lv.dG = cur;
     if ((st.mode & OR_THR) && !(st.mode & OR_REF_IDS)) compact_labels(lv.f);  // D10
     MoveResult mr = inc_movement(lv.G, cur, lv.f, lv.s_cur, st.gamma, st.mode, lv.G.n);

@@ -29,8 +29,12 @@ extern "C" {
 #endif
 
 /* mode bits: OR_CONNECTED (D8, D11, D12) and OR_REF_IDS (no D10 relabel: the
-   reference's n + v may name a populated community) only with OR_THR */
-enum { OR_DET = 0, OR_THR = 1, OR_CONNECTED = 2, OR_REF_IDS = 4 };
+   reference's n + v may name a populated community) only with OR_THR;
+   OR_AS_WRITTEN (any mode): the reference's full recomputations -- the CSR
+   rebuilt through a hash map per batch and level, O(n) scans per refined
+   vertex and per candidate subcommunity -- with the same results (the CPU
+   baseline's as-written variant) */
+enum { OR_DET = 0, OR_THR = 1, OR_CONNECTED = 2, OR_REF_IDS = 4, OR_AS_WRITTEN = 8 };
 enum { OR_MAP_F = 0, OR_MAP_G = 1, OR_MAP_S_CUR = 2, OR_MAP_S_PRE = 3 };
 enum { OR_SET_A = 0, OR_SET_B = 1, OR_SET_K = 2, OR_SET_R = 3 };
 

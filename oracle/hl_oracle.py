@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libhl_oracle.so")
 _lib = None
 
-DET, THR, CONNECTED, REF_IDS = 0, 1, 2, 4
+DET, THR, CONNECTED, REF_IDS, AS_WRITTEN = 0, 1, 2, 4, 8
 PAR = THR | CONNECTED
 MAP_F, MAP_G, MAP_S_CUR, MAP_S_PRE = 0, 1, 2, 3
 SET_A, SET_B, SET_K, SET_R = 0, 1, 2, 3
