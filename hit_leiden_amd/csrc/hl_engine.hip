@@ -501,10 +501,15 @@ struct Engine {
       st->settled += d[DS_SETTLED];
     }
     if (prof.on) {
+      // k_prop2_first also sorts every listed vertex above the lane bucket into
+      // its bucket's list: 8 B read (list entry, row length) + 4 B written per
+      // such vertex, movement and refinement, hub-table vertices included
+      u64 listed = d[DS_HT];
+      for (u32 b = 1; b < kBuckets; ++b) listed += d[DS_MOVE + b] + d[DS_REF + b];
       const int slots[14] = {P_PROP_SMALL, P_TRY_HALF, P_TRY_WAVE, P_PROP_WAVE, P_TRY_LDS, P_PROP_LDS, P_TRY_HUB,
                              P_TRY_GIANT, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI, P_APPLY_MOVES, P_MARK_MOVED,
                              P_PROP_TABLE};
-      const u64 by[14] = {prop[0], tried[1], tried[2], prop[1] + prop[2] - tried[1] - tried[2], tried[3],
+      const u64 by[14] = {prop[0] + 12 * listed, tried[1], tried[2], prop[1] + prop[2] - tried[1] - tried[2], tried[3],
                           prop[3] - tried[3], tried[4], tried[5], prop[4] + prop[5] - tried[4] - tried[5],
                           dec, multi, apply, marks, htab};
       for (int i = 0; i < 14; ++i)
