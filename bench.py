@@ -621,7 +621,7 @@ def main():
     if not args.no_extras and args.config in ("large", "mid"):
         # the batched and served forms against their plain one-batch-per-call GPU paths
         out["extras"] = {
-            "batched": run_extra(args, "batched", ["--steps", "2", "--warmup", "1", "--plain-sample", "32",
+            "batched": run_extra(args, "batched", ["--steps", "4", "--warmup", "1", "--plain-sample", "32",
                                                    "--autotune", "--tune-sample", "96"]),
             "served": run_extra(args, "served", ["--rates", "2000", "--served-seconds", "2",
                                                  "--plain-requests", "300"]),

@@ -71,6 +71,11 @@ def _time(config, graph, batches, repeat):
         e.close()
 
 
+# a candidate replaces the incumbent only when it is this much faster: timings of
+# near-equal shapes differ by run-to-run noise, not by their shape
+MARGIN = 0.03
+
+
 def _search(time_of, base, log):
     """The two stages over time_of(Tuned) -> seconds, starting from base."""
     results = {}
@@ -82,8 +87,15 @@ def _search(time_of, base, log):
                 log(f"[autotune] {t._asdict()}: {results[t] * 1e3:.2f} ms")
         return results[t]
 
-    best = min((base._replace(small_degree=sd, wave_degree=wd) for sd in SMALL for wd in WAVE), key=timed)
-    best = min((best._replace(marks_block=mb, marks_grid=mg) for mb in MARKS_BLOCK for mg in MARKS_GRID), key=timed)
+    def pick(incumbent, candidates):
+        best = incumbent
+        for t in candidates:
+            if timed(t) < timed(best) * (1.0 - MARGIN):
+                best = t
+        return best
+
+    best = pick(base, [base._replace(small_degree=sd, wave_degree=wd) for sd in SMALL for wd in WAVE])
+    best = pick(best, [best._replace(marks_block=mb, marks_grid=mg) for mb in MARKS_BLOCK for mg in MARKS_GRID])
     return best
 
 
@@ -120,15 +132,18 @@ def tune_graphs(graphs, deltas, levels=4, gamma=1.0, cache=None, log=None) -> Tu
     sig = signature(sum(g.node_count for g in graphs[:1]), graphs[0].src, graphs[0].dst) if graphs else "empty"
     key = f"graphs{len(graphs)}_{sum(len(d) for d in deltas.values())}_{sig}"
 
-    def time_of(t):
-        e = HitLeiden(t.config(mode=RunMode.THROUGHPUT, levels=levels, resolution=gamma))
-        try:
-            e.load_graphs(graphs)
-            hip.synchronize()
-            t0 = time.perf_counter()
-            e.apply_graphs(deltas)
-            return time.perf_counter() - t0
-        finally:
-            e.close()
+    def time_of(t, repeat=2):
+        best = float("inf")
+        for _ in range(repeat):  # fresh engines: the pass starts from the same state each time
+            e = HitLeiden(t.config(mode=RunMode.THROUGHPUT, levels=levels, resolution=gamma))
+            try:
+                e.load_graphs(graphs)
+                hip.synchronize()
+                t0 = time.perf_counter()
+                e.apply_graphs(deltas)
+                best = min(best, time.perf_counter() - t0)
+            finally:
+                e.close()
+        return best
 
     return _cached(cache, key, lambda: _search(time_of, Tuned(16, 512, 256, 1024), log))

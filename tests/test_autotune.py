@@ -39,3 +39,17 @@ def test_cache_round_trip(tmp_path):
     assert json.load(open(path))["k"] == list(t1)
     cfg = t1.config(levels=3)
     assert (cfg.small_degree, cfg.wave_degree, cfg.marks_block, cfg.marks_grid, cfg.levels) == (8, 256, 512, 2048, 3)
+
+
+def test_noise_sized_differences_keep_the_built_in_shape():
+    """Candidates within the margin of the incumbent (run-to-run noise) do not
+    replace it; one clearly faster does."""
+    base = autotune.Tuned(16, 512, 256, 1024)
+
+    def flat(t):
+        return 1.0 - 0.01 * (t != base)  # every other shape 1 % faster
+    assert autotune._search(flat, base, None) == base
+
+    def one_better(t):
+        return 0.9 if t == base._replace(marks_grid=2048) else 1.0
+    assert autotune._search(one_better, base, None) == base._replace(marks_grid=2048)
