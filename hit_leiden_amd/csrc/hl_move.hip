@@ -976,17 +976,31 @@ __device__ void ht_move(const HubTables& ht, u32 u, u32 from, u32 to, double w) 
 // 129-512), and small tables give more resident wavefronts and less to clear
 // and scan.  A vertex whose keys do not fit is appended to the overflow list
 // for the degree-sized tables instead.
-template <class P, u32 G, u32 CAP>
-__global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* ovf,
-                                                    u32* ovf_cnt, CandOut out, unsigned long long* cst,
-                                                    unsigned long long* tst) {
-  constexpr u32 NG = 256 / G;
-  __shared__ u32 hk[NG * CAP];
-  __shared__ double hv[NG * CAP];
-  __shared__ u32 gfull[NG];
-  __shared__ u32 stn[kStage], stt[kStage];
-  __shared__ double stg[kStage];
-  __shared__ u32 scnt, sbase;
+template <u32 G, u32 CAP, u32 BS>
+struct TabShared {
+  u32 hk[BS / G * CAP];
+  double hv[BS / G * CAP];
+  u32 gfull[BS / G];
+  u32 stn[kStage], stt[kStage];
+  double stg[kStage];
+  u32 scnt, sbase;
+};
+// G lanes per vertex in workgroups of BS threads; tst2 (or null) takes the
+// evaluated vertices and row entries as well
+template <class P, u32 G, u32 CAP, u32 BS>
+__device__ __forceinline__ void tab_body(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u32* ovf,
+                                         u32* ovf_cnt, const CandOut& out, unsigned long long* cst,
+                                         unsigned long long* tst, TabShared<G, CAP, BS>& sh,
+                                         unsigned long long* tst2 = nullptr) {
+  constexpr u32 NG = BS / G;
+  u32* hk = sh.hk;
+  double* hv = sh.hv;
+  u32* gfull = sh.gfull;
+  u32* stn = sh.stn;
+  u32* stt = sh.stt;
+  double* stg = sh.stg;
+  u32& scnt = sh.scnt;
+  u32& sbase = sh.sbase;
   if (threadIdx.x == 0) scnt = 0;
   const u32 cnt = *cnt_dev;
   const u32 lane = threadIdx.x % G, grp = threadIdx.x / G;
@@ -1093,7 +1107,19 @@ __global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list
   block_add_u64(cst, ncand);
   const u32 idx[3] = {0, 1, 2};
   unsigned long long vals[3] = {nv, ne, ncand};
+  if (tst2) {
+    const u32 idx2[2] = {0, kBuckets};
+    unsigned long long vals2[2] = {nv, ne};
+    block_add_multi(tst2, idx2, vals2);
+  }
   block_add_multi(tst, idx, vals);
+}
+template <class P, u32 G, u32 CAP>
+__global__ __launch_bounds__(256) void k_prop2_tab(P pol, Csr g, const u32* list, const u32* cnt_dev, u32* ovf,
+                                                    u32* ovf_cnt, CandOut out, unsigned long long* cst,
+                                                    unsigned long long* tst) {
+  __shared__ TabShared<G, CAP, 256> sh;
+  tab_body<P, G, CAP, 256>(pol, g, list, cnt_dev, ovf, ovf_cnt, out, cst, tst, sh);
 }
 
 // Hubs whose keys overflowed the LDS table: one workgroup per vertex; each
@@ -1207,22 +1233,41 @@ __global__ __launch_bounds__(1024) void k_prop2_big(P pol, Csr g, const u32* lis
 // (gain desc, node asc) as a sortable pair: key = ~bits(gain) (gains are > 0)
 __device__ __forceinline__ bool pri_less(u64 ka, u32 na, u64 kb, u32 nb) { return ka < kb || (ka == kb && na < nb); }
 
-__global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32* minUto, u32* minUfrom,
-                                                     u32* minAto, u32* minAfrom, u32* mark, u32* next,
-                                                     u32* next_cnt, u32* status, unsigned long long* dst,
-                                                     u32* cur_cnt, u32* bucket_counts, FlatMarks flat,
-                                                     u32* round_cand) {
-  __shared__ u64 skey[kDecSmall];
-  __shared__ u32 snode[kDecSmall];
-  __shared__ u32 sidx[kDecSmall];
-  __shared__ u32 sfrom[kDecSmall];
-  __shared__ u32 sto[kDecSmall];
-  __shared__ u8 sst[kDecSmall];
-  __shared__ u8 skd[kDecSmall];  // the accepted move's vertex moved in an earlier round of this level
-  __shared__ u32 mAto[kDecHash], mAfrom[kDecHash];
-  __shared__ u16 fslot[kDecLds], tslot[kDecLds];
-  __shared__ u32 und, nacc;
-  __shared__ u64 wsum[16], mtotal;
+// the decoupling workgroup's LDS (k_dec_small, and the persistent rounds)
+struct DecShared {
+  u64 skey[kDecSmall];
+  u32 snode[kDecSmall];
+  u32 sidx[kDecSmall];
+  u32 sfrom[kDecSmall];
+  u32 sto[kDecSmall];
+  u8 sst[kDecSmall];
+  u8 skd[kDecSmall];  // the accepted move's vertex moved in an earlier round of this level
+  u32 mAto[kDecHash], mAfrom[kDecHash];
+  u16 fslot[kDecLds], tslot[kDecLds];
+  u32 und, nacc;
+  u64 wsum[16], mtotal;
+};
+// one 1,024-thread workgroup
+__device__ __forceinline__ void dec_small_body(const MoveCtx& m, const CandOut& cand, u32* minUto, u32* minUfrom,
+                                               u32* minAto, u32* minAfrom, u32* mark, u32* next, u32* next_cnt,
+                                               u32* status, unsigned long long* dst, u32* cur_cnt,
+                                               u32* bucket_counts, const FlatMarks& flat, u32* round_cand,
+                                               DecShared& sh) {
+  u64* skey = sh.skey;
+  u32* snode = sh.snode;
+  u32* sidx = sh.sidx;
+  u32* sfrom = sh.sfrom;
+  u32* sto = sh.sto;
+  u8* sst = sh.sst;
+  u8* skd = sh.skd;
+  u32* mAto = sh.mAto;
+  u32* mAfrom = sh.mAfrom;
+  u16* fslot = sh.fslot;
+  u16* tslot = sh.tslot;
+  u32& und = sh.und;
+  u32& nacc = sh.nacc;
+  u64* wsum = sh.wsum;
+  u64& mtotal = sh.mtotal;
   const u32 c = *cand.cnt;
   const u32 tid = threadIdx.x;
   if (tid == 0) *round_cand = c;  // read by the host with the round's other counters
@@ -1579,19 +1624,36 @@ __global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32
   }
 }
 
+__global__ __launch_bounds__(1024) void k_dec_small(MoveCtx m, CandOut cand, u32* minUto, u32* minUfrom,
+                                                     u32* minAto, u32* minAfrom, u32* mark, u32* next,
+                                                     u32* next_cnt, u32* status, unsigned long long* dst,
+                                                     u32* cur_cnt, u32* bucket_counts, FlatMarks flat,
+                                                     u32* round_cand) {
+  __shared__ DecShared sh;
+  dec_small_body(m, cand, minUto, minUfrom, minAto, minAfrom, mark, next, next_cnt, status, dst, cur_cnt,
+                 bucket_counts, flat, round_cand, sh);
+}
+
 // Marks of long neighbour lists over the whole grid: positions p < *flat.c
 // with flat.pre the exclusive prefix of their list lengths (flat.pre[c] =
 // total); next-active vertices are staged per workgroup and appended with one
 // device atomic per flush.
 template <u32 BS>
-__global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u32* mark, u32* next,
-                                                   u32* next_cnt) {
+struct MarksShared {
+  u32 sv_[2 * kMarkPer * BS];  // two passes of next-active marks
+  u32 scnt, sbase, span[2];
+};
+template <u32 BS>
+__device__ __forceinline__ void marks_flat_body(const MoveCtx& m, const FlatMarks& flat, u32* mark, u32* next,
+                                                u32* next_cnt, MarksShared<BS>& sh) {
   // kMarkPer entries per thread and pass, their gathers issued together so
   // that more of them are in flight; the stage holds two passes of marks
   constexpr u32 R = kMarkPer;
   constexpr u32 kStageN = 2 * R * BS;
-  __shared__ u32 sv_[kStageN];
-  __shared__ u32 scnt, sbase, span[2];
+  u32* sv_ = sh.sv_;
+  u32& scnt = sh.scnt;
+  u32& sbase = sh.sbase;
+  u32* span = sh.span;
   const u32 c = *flat.c;
   if (c == 0) return;
   if (threadIdx.x == 0) scnt = 0;
@@ -1685,6 +1747,12 @@ __global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u3
     __syncthreads();
     for (u32 i = threadIdx.x; i < scnt; i += blockDim.x) next[sbase + i] = sv_[i];
   }
+}
+template <u32 BS>
+__global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u32* mark, u32* next,
+                                                   u32* next_cnt) {
+  __shared__ MarksShared<BS> sh;
+  marks_flat_body<BS>(m, flat, mark, next, next_cnt, sh);
 }
 // The moves' cache updates over the grid: entry t of the moved vertices'
 // concatenated rows (one thread each) updates v's entry in neighbour u's row.
