@@ -65,7 +65,7 @@ class _Config(C.Structure):
                 ("device", C.c_int32), ("small_degree", C.c_uint32), ("wave_degree", C.c_uint32),
                 ("load_pass_records", C.c_uint32), ("key_cache", C.c_uint32), ("hub_tables", C.c_uint32),
                 ("decide_rounds", C.c_uint32), ("refine_reference", C.c_uint32), ("marks_block", C.c_uint32),
-                ("marks_grid", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("marks_grid", C.c_uint32), ("persist_rounds", C.c_uint32)]
 
 
 class BatchStats(C.Structure):
@@ -77,7 +77,7 @@ class BatchStats(C.Structure):
                 ("rounds", C.c_uint64 * MAX_LEVELS), ("bytes_alg", C.c_uint64), ("seconds", C.c_double),
                 ("decide_rounds", C.c_uint64 * MAX_LEVELS), ("settled", C.c_uint64),
                 ("alloc_count", C.c_uint64), ("alloc_bytes", C.c_uint64), ("alloc_seconds", C.c_double),
-                ("host_syncs", C.c_uint64)]
+                ("host_syncs", C.c_uint64), ("persist_rounds", C.c_uint64)]
 
     def as_dict(self):
         P = self.levels
@@ -89,7 +89,7 @@ class BatchStats(C.Structure):
             "rounds": list(self.rounds[:P]), "bytes_alg": self.bytes_alg, "seconds": self.seconds,
             "decide_rounds": list(self.decide_rounds[:P]), "settled": self.settled,
             "alloc_count": self.alloc_count, "alloc_bytes": self.alloc_bytes, "alloc_seconds": self.alloc_seconds,
-            "host_syncs": self.host_syncs,
+            "host_syncs": self.host_syncs, "persist_rounds": self.persist_rounds,
         }
 
 
@@ -204,6 +204,8 @@ class RunConfig:
     refine_reference: bool = False
     marks_block: int = 0  # workgroup size of the grid-wide neighbour marks (0: 256; 128, 256, 512)
     marks_grid: int = 0  # their workgroups (0: 1,024)
+    # a level's throughput rounds in one cooperative kernel while they fit it (same results)
+    persist_rounds: int = True  # True / False, or the kernel's workgroups (2..1024)
 
 
 class GraphInput:
@@ -258,6 +260,8 @@ class HitLeiden:
         c.refine_reference = 1 if self.config.refine_reference else 0
         c.marks_block = int(self.config.marks_block)
         c.marks_grid = int(self.config.marks_grid)
+        pr = self.config.persist_rounds
+        c.persist_rounds = (0 if pr else 1) if isinstance(pr, bool) else int(pr)
         h = C.c_void_p()
         st = lib().hl_create(C.byref(c), C.byref(h))
         if st != HL_OK:

@@ -395,6 +395,7 @@ struct Engine {
       st->rounds[p] = rounds;
       st->evaluated[p] += ms_st.evaluated;
       st->decide_rounds[p] += ms_st.decide_rounds;
+      st->persist_rounds += ms_st.persist_rounds;
     }
     return rounds;
   }
@@ -946,7 +947,7 @@ hl_status hl_create(const hl_config* cfg, hl_graph** out) {
       (cfg->mode != HL_MODE_DETERMINISTIC && cfg->mode != HL_MODE_THROUGHPUT) || cfg->key_cache > 1 ||
       cfg->hub_tables > 1 || cfg->decide_rounds > (1u << 20) || cfg->refine_reference > 1 ||
       (cfg->marks_block != 0 && cfg->marks_block != 128 && cfg->marks_block != 256 && cfg->marks_block != 512) ||
-      cfg->marks_grid > 65536) {
+      cfg->marks_grid > 65536 || cfg->persist_rounds > 1024) {
     g_create_err = "invalid config";
     return HL_INVALID_INPUT;
   }
@@ -975,6 +976,8 @@ hl_status hl_create(const hl_config* cfg, hl_graph** out) {
     if (cfg->decide_rounds) g->e.th.dec_rounds = cfg->decide_rounds;
     if (cfg->marks_block) g->e.th.marks_block = cfg->marks_block;
     if (cfg->marks_grid) g->e.th.marks_grid = cfg->marks_grid;
+    g->e.th.persist = cfg->persist_rounds != 1;
+    if (cfg->persist_rounds > 1) g->e.th.persist_blocks = cfg->persist_rounds;
   });
   if (st != HL_OK) {
     g_create_err = g->e.err;

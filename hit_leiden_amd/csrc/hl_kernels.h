@@ -154,6 +154,8 @@ struct Thresholds {
   u32 dec_rounds = 48;  // priority rounds of the multi-kernel decoupling before the in-order settle
   u32 marks_block = 256;  // workgroup size of the grid-wide neighbour marks (128, 256 or 512)
   u32 marks_grid = 1024;  // their workgroups
+  bool persist = true;    // a level's throughput rounds in one cooperative kernel while they fit it
+  u32 persist_blocks = 32;  // its workgroups (a grid barrier costs about 0.13 us per workgroup)
 };
 // deterministic sweep (hit_leiden.rs:1108-1188) in one workgroup
 struct DetMoveScratch {

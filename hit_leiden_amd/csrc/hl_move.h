@@ -198,6 +198,7 @@ struct MoveScratch {
   DevBuf<u32> mfrom;  // source communities of the moves marked over the grid
   DevBuf<u8> mkd, kd;  // repeat-move flags of those moves (FlatMarks::kd), both paths
   DevBuf<u32> und;     // undecided counts of a chunk of decoupling priority rounds
+  DevBuf<unsigned long long> pout;  // what the persistent rounds report
   // hub tables (HubTables)
   DevBuf<u32> ht_bits;
   DevBuf<HtSlot> ht_slot;
@@ -223,6 +224,7 @@ struct MoveScratch {
 struct MoveStats {
   u64 active = 0, evaluated = 0, moved_rounds = 0;
   u64 decide_rounds = 0;  // priority rounds of the multi-kernel decoupling
+  u64 persist_rounds = 0;  // rounds run inside the cooperative kernel
 };
 
 // Told the vertices active next after the seeding and after every round (a

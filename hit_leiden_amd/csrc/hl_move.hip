@@ -11,6 +11,7 @@
 //                  (throughput.rs:182-199) -- no host round trip inside a round
 // Large candidate sets (the initial pass) take the multi-kernel path.
 // Lists are deduplicated with a bitmap of claimed vertices, cleared per round.
+#include <hip/hip_cooperative_groups.h>
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <cstring>
@@ -1754,6 +1755,113 @@ __global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u3
   __shared__ MarksShared<BS> sh;
   marks_flat_body<BS>(m, flat, mark, next, next_cnt, sh);
 }
+// ------------------------------------------------------------------ persistent rounds
+// A level's throughput rounds in one cooperative kernel of 1,024-thread
+// workgroups (at most one per CU; a grid barrier costs about 0.13 us per
+// workgroup, so the grid is small and the lists it takes are short): per round the proposals (a wavefront per listed vertex,
+// its neighbour communities in 512 LDS slots), a grid barrier, the decoupling
+// with its apply and in-block marks by workgroup 0 (dec_small_body), a grid
+// barrier, the grid-wide marks of long neighbour lists, a grid barrier.  The
+// rounds are the per-round launches' rounds: the same bodies, the same
+// counters, with no host round trip between them.  A round whose keys do not
+// fit the slots or whose candidates exceed the decoupling workgroup stops the
+// kernel before anything of it is applied, and the host runs it and the rest
+// with the per-round launches, as does a list longer than kPersistActive.
+constexpr u32 kPersistBS = 1024;
+constexpr u32 kPersistG = 64, kPersistCap = 512;
+constexpr u32 kPersistDeg = 256;       // rows of at most this many entries: tables of kPersistCap slots
+constexpr u32 kPersistActive = 16384;  // longer lists take the per-round launches (wider grids)
+struct PersistArgs {
+  u32* list[2];  // the alternating frontier lists and their counts
+  u32* lcnt[2];
+  int ci0;     // the list of the first round
+  u32 rounds;  // at most this many rounds
+  u64 mark_words;
+  u32* mark;
+  CandOut cand;
+  u32 *minUto, *minUfrom, *minAto, *minAfrom;
+  u32* status;
+  u32* bcounts;
+  u32* ccand;
+  u32* ovf;
+  u32* ovf_cnt;
+  FlatMarks fm;
+  unsigned long long* dstat;
+  // [0] rounds completed, [1] 1 when the rounds from [0] on are left to the
+  // per-round launches,
+  // [2] vertices evaluated and [3] moves accepted in the completed rounds
+  unsigned long long* out;
+};
+union PersistShared {
+  DecShared dec;
+  TabShared<kPersistG, kPersistCap, kPersistBS> tab;
+  MarksShared<kPersistBS> marks;
+};
+__global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolicyT pol, PersistArgs a) {
+  __shared__ PersistShared sh;
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  unsigned long long* cst = a.dstat + DS_MOVE + 2 * kBuckets + 1;  // counted as the half-wavefront bucket
+  unsigned long long* tst = a.dstat + DS_TRY_MOVE + 3;
+  unsigned long long* tst2 = a.dstat + DS_MOVE + 1;
+  for (u32 r = 0; r < a.rounds; ++r) {
+    const int ci = (a.ci0 + int(r)) & 1;
+    const u32* cur = a.list[ci];
+    u32* nxt = a.list[ci ^ 1];
+    u32* cur_cnt = a.lcnt[ci];
+    u32* next_cnt = a.lcnt[ci ^ 1];
+    // the claim bitmap of this round's marks, and the proposals
+    for (u64 w = gtid(); w < a.mark_words; w += gstride()) a.mark[w] = 0;
+    tab_body<MovePolicyT, kPersistG, kPersistCap, kPersistBS>(pol, m.g, cur, cur_cnt, a.ovf, a.ovf_cnt, a.cand, cst,
+                                                              tst, sh.tab, tst2);
+    grid.sync();
+    // every workgroup reads the same counts here (workgroup 0 clears the
+    // candidate count only on the way on), so all stop or go on together
+    if (*a.ovf_cnt != 0 || *a.cand.cnt > kDecSmall) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.out[0] = r;
+        a.out[1] = 1;
+      }
+      return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.out[2] += *cur_cnt;  // read before the decoupling clears it
+    if (blockIdx.x == 0)
+      dec_small_body(m, a.cand, a.minUto, a.minUfrom, a.minAto, a.minAfrom, a.mark, nxt, next_cnt, a.status, a.dstat,
+                     cur_cnt, a.bcounts, a.fm, a.ccand, sh.dec);
+    grid.sync();
+    marks_flat_body<kPersistBS>(m, a.fm, a.mark, nxt, next_cnt, sh.marks);
+    grid.sync();
+    const u32 acc = a.status[1], nx = *next_cnt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.out[3] += acc;
+      a.out[0] = r + 1;
+      if (acc != 0 && nx > kPersistActive) a.out[1] = 1;
+    }
+    // throughput.rs:177-180, nothing left to evaluate, or a list for the
+    // per-round launches' wider grids
+    if (acc == 0 || nx == 0 || nx > kPersistActive) return;
+  }
+}
+
+// workgroups of the persistent rounds on the current device (0: not available)
+static u32 persist_grid() {
+  static u32 grid[64];
+  static bool known[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (!known[dev]) {
+    known[dev] = true;
+    int coop = 0, cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess && coop &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_move_rounds),
+                                                     int(kPersistBS), 0) == hipSuccess &&
+        per > 0 && cus > 0)
+      grid[dev] = u32(per) * u32(cus);
+    (void)hipGetLastError();
+  }
+  return grid[dev];
+}
+
 // The moves' cache updates over the grid: entry t of the moved vertices'
 // concatenated rows (one thread each) updates v's entry in neighbour u's row.
 __global__ __launch_bounds__(256) void k_ck_flat(MoveCtx m, FlatMarks flat) {
@@ -2281,7 +2389,81 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   bool queued = false;
   u32 last_cand = kNone;
   int slot = 0;
+  // Rounds in one cooperative kernel when they fit it: no hub tables or
+  // neighbour cache at this level, rows of at most kPersistDeg entries, no
+  // served-form hook reading every round, a list of at most kPersistActive
+  // vertices.  The kernel runs from round `rounds` on until the level ends or
+  // a round does not fit it; a round it left runs with the launches, and the
+  // kernel is tried again once a later round's list fits.  While it may be
+  // used, no round is enqueued ahead (the kernel starts from a known count).
+  const u32 pgrid = th.persist ? std::min(persist_grid(), th.persist_blocks) : 0;
+  const bool may_persist = pgrid && !m.ck && !m.ht.bits && !(hook && hook->fn) && maxdeg <= kPersistDeg;
+  u32 left_round = kNone;  // the round the kernel last left to the launches
+  auto run_persist = [&]() {
+    ms.hovf.reserve(u64(n) + 1, s);
+    ms.pout.reserve(4, s);
+    fill_u32(reinterpret_cast<u32*>(ms.pout.p), 0, 8, s);
+    PersistArgs a{};
+    a.list[0] = ms.list_a.p;
+    a.list[1] = ms.list_b.p;
+    a.lcnt[0] = lcnt[0];
+    a.lcnt[1] = lcnt[1];
+    a.ci0 = ci;
+    a.rounds = u32(max_rounds) - rounds;
+    a.mark_words = u64(n) / 32 + 1;
+    a.mark = ms.mark.p;
+    a.cand = CandOut{ms.cnode.p, ms.cto.p, ms.cgain.p, ccnt};
+    a.minUto = ms.minUto.p;
+    a.minUfrom = ms.minUfrom.p;
+    a.minAto = ms.minAto.p;
+    a.minAfrom = ms.minAfrom.p;
+    a.status = status;
+    a.bcounts = bcounts;
+    a.ccand = ccand;
+    a.ovf = ms.hovf.p;
+    a.ovf_cnt = ms.counters.p + kOvfSlot;  // zero: every round leaves it so
+    a.fm = FlatMarks{ms.mnode.p, ms.mto.p, ms.mpre.p, ms.counters.p + 13, ms.counters.p + 12, ms.mfrom.p, ms.mkd.p};
+    a.dstat = ms.dstat.p;
+    a.out = ms.pout.p;
+    void* args[] = {&m, &pol, &a};
+    {
+      HL_PROF(P_DECOUPLE, 0, s);
+      HL_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_move_rounds), dim3(pgrid),
+                                          dim3(kPersistBS), args, 0, s));
+    }
+    unsigned long long o[4] = {0, 0, 0, 0};
+    u32 cnts[3] = {0, 0, 0};  // counters [0] and [2]: the two lists' counts
+    d2h_async(o, ms.pout.p, sizeof(o), s);
+    d2h_async(cnts, lcnt[0], sizeof(cnts), s);
+    stream_sync(s);
+    const u32 done = u32(o[0]);
+    rounds += done;
+    if (st) {
+      st->evaluated += o[2];
+      st->moved_rounds += o[3];
+      st->persist_rounds += done;
+    }
+    if (done & 1u) {
+      std::swap(cur, nxt);
+      ci ^= 1;
+    }
+    if (o[1]) {
+      // the rounds from here on run with the launches; a round the kernel
+      // left (its list and count untouched) runs again, without the
+      // candidates and spills of the kernel's attempt
+      host_cnt = cnts[2 * ci];
+      fill_u32(ccnt, 0, 1, s);
+      fill_u32(ms.counters.p + kOvfSlot, 0, kOvfLists, s);
+      left_round = rounds;
+    } else {
+      host_cnt = 0;  // the last round accepted nothing, left nothing to evaluate, or was the last allowed
+    }
+  };
   while (host_cnt > 0 && int(rounds) < max_rounds) {
+    if (may_persist && !queued && host_cnt <= kPersistActive && rounds != left_round) {
+      run_persist();
+      continue;
+    }
     if (st) st->evaluated += host_cnt;
     u32* cur_cnt = lcnt[ci];
     u32* next_cnt = lcnt[ci ^ 1];
@@ -2290,7 +2472,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     u32* snap = ms.snap + 16 * slot;
     HL_CHECK(hipMemcpyAsync(snap, ms.counters.p, 16 * sizeof(u32), hipMemcpyDeviceToHost, s));
     HL_CHECK(hipEventRecord(ms.snap_ev[slot], s));
-    queued = may_queue && int(rounds) + 1 < max_rounds && last_cand <= kDecSmall / 2;
+    queued = may_queue && !may_persist && int(rounds) + 1 < max_rounds && last_cand <= kDecSmall / 2;
     if (queued) enqueue_round(nxt, cur, ci ^ 1, host_cnt);
     HL_CHECK(hipEventSynchronize(ms.snap_ev[slot]));
     g_alloc.syncs += 1;

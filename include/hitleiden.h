@@ -92,7 +92,12 @@ typedef struct {
   uint32_t marks_block;  /* throughput movement: workgroup size of the grid-wide neighbour marks
                             (rows of the moved vertices), 0 = 256, else 128, 256 or 512 */
   uint32_t marks_grid;   /* their workgroups, 0 = 1,024, at most 65,536 */
-  uint32_t reserved[1];
+  uint32_t persist_rounds; /* throughput movement: a level's rounds in one cooperative kernel (grid barriers
+                              between proposals, decoupling and marks, no host round trip) while they fit
+                              it -- rows of at most 256 entries, no hub tables, at most 4,096 candidates and
+                              16,384 listed vertices a round -- else the per-round launches; 0 = on with 32
+                              workgroups, 1 = off, 2..1024 = on with that many workgroups (at most one per
+                              CU).  The results are the same either way */
 } hl_config;
 
 #define HL_MAX_LEVELS 8
@@ -119,6 +124,8 @@ typedef struct {
   uint64_t alloc_bytes;              /* their bytes */
   double alloc_seconds;              /* host wall time in those allocations and the frees they replace */
   uint64_t host_syncs;               /* times the host waited for the device during the call */
+  uint64_t persist_rounds;           /* throughput: movement rounds run inside the cooperative kernel
+                                        (hl_config.persist_rounds), summed over levels */
 } hl_batch_stats;
 
 typedef struct hl_graph hl_graph;
