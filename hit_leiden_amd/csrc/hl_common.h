@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -58,6 +59,16 @@ struct InputError : std::runtime_error {
 // the stream and then delivers that stream's staged values to their
 // destinations, which must stay alive until then.
 void d2h_async(void* dst, const void* src, size_t bytes, hipStream_t s);
+// Several small readbacks at once: one kernel writes them all into the pinned
+// staging area (one launch in place of one copy each).  Each item is whole
+// 4-byte words, 4-byte aligned, at most kD2HGroupBytes; at most kD2HGroupMax items.
+struct D2H {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+constexpr size_t kD2HGroupMax = 8, kD2HGroupBytes = 1024;
+void d2h_group(std::initializer_list<D2H> items, hipStream_t s);
 void stream_sync(hipStream_t s);
 // Forget the values staged for stream s without delivering them, after
 // waiting (errors ignored) for their copies: a call that ended with an

@@ -569,12 +569,16 @@ struct Engine {
     else
       def_update_maps(true);
     unsigned long long dnow[DS_COUNT];
+    static_assert(sizeof(dnow) <= hl::kD2HGroupBytes, "work counters exceed one readback group item");
     const bool have_dstat = mvs.dstat.p != nullptr;
-    if (have_dstat)
-      hl::d2h_async(dnow, mvs.dstat.p, sizeof(dnow), s);
     u64 c4[4 * HL_MAX_LEVELS];
     const int np = std::min(P, HL_MAX_LEVELS);
-    if (st) hl::d2h_async(c4, lvcnt.p, 4 * np * sizeof(u64), s);
+    if (have_dstat && st)
+      hl::d2h_group({{dnow, mvs.dstat.p, sizeof(dnow)}, {c4, lvcnt.p, 4 * np * sizeof(u64)}}, s);
+    else if (have_dstat)
+      hl::d2h_async(dnow, mvs.dstat.p, sizeof(dnow), s);
+    else if (st)
+      hl::d2h_async(c4, lvcnt.p, 4 * np * sizeof(u64), s);
     hl::stream_sync(s);
     if (have_dstat) account_work(dnow, st);
     if (st)

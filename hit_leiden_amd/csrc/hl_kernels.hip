@@ -47,6 +47,7 @@ void alloc_trace(size_t bytes, double seconds, const void* site) {
 namespace {
 struct Staging {
   u8* buf = nullptr;
+  u8* dbuf = nullptr;  // the area's device address
   size_t cap = 0, used = 0;
   struct Item {
     void* dst;
@@ -101,7 +102,8 @@ void drop_staged(hipStream_t s) {
   if (keep == 0) st.used = 0;
 }
 
-void d2h_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
+// room for `bytes` in the pinned staging area (stream s about to write it)
+static size_t staging_reserve(size_t bytes) {
   Staging& st = staging();
   size_t off = (st.used + 15) & ~size_t(15);
   if (off + bytes > st.cap) {
@@ -115,15 +117,65 @@ void d2h_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
     off = 0;
     if (bytes > st.cap) {
       if (st.buf) HL_CHECK(hipHostFree(st.buf));
-      st.buf = nullptr;
+      st.buf = st.dbuf = nullptr;
       const size_t cap = std::max<size_t>(bytes, 1 << 16);
       HL_CHECK(hipHostMalloc((void**)&st.buf, cap, hipHostMallocDefault));
+      HL_CHECK(hipHostGetDevicePointer((void**)&st.dbuf, st.buf, 0));
       st.cap = cap;
     }
   }
+  return off;
+}
+
+void d2h_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes && bytes % 4 == 0 && bytes <= kD2HGroupBytes && reinterpret_cast<uintptr_t>(src) % 4 == 0) {
+    // a small readback: the gather kernel, which costs less than a runtime copy
+    d2h_group({{dst, src, bytes}}, s);
+    return;
+  }
+  Staging& st = staging();
+  const size_t off = staging_reserve(bytes);
   HL_CHECK(hipMemcpyAsync(st.buf + off, src, bytes, hipMemcpyDeviceToHost, s));
   st.items.push_back({dst, off, bytes, s});
   st.used = off + bytes;
+}
+
+// the items of a d2h_group, written into the pinned staging area by one kernel
+struct D2HGather {
+  const u32* src[kD2HGroupMax];
+  u32 words[kD2HGroupMax];
+  u32 off[kD2HGroupMax];  // in words from the area's start
+  u32 n;
+};
+__global__ void k_d2h_gather(D2HGather g, u32* area) {
+  for (u32 i = 0; i < g.n; ++i)
+    for (u32 w = threadIdx.x; w < g.words[i]; w += blockDim.x) area[g.off[i] + w] = g.src[i][w];
+}
+
+void d2h_group(std::initializer_list<D2H> items, hipStream_t s) {
+  if (items.size() == 0) return;
+  if (items.size() > kD2HGroupMax) throw std::logic_error("d2h_group: more than kD2HGroupMax items");
+  Staging& st = staging();
+  size_t total = 0;
+  for (const D2H& it : items) {
+    if (it.bytes % 4 || it.bytes > kD2HGroupBytes || reinterpret_cast<uintptr_t>(it.src) % 4)
+      throw std::logic_error("d2h_group: items must be 4-byte words, at most kD2HGroupBytes each");
+    total += it.bytes;
+  }
+  const size_t off = staging_reserve(total);
+  D2HGather g{};
+  size_t at = off;
+  for (const D2H& it : items) {
+    g.src[g.n] = static_cast<const u32*>(it.src);
+    g.words[g.n] = u32(it.bytes / 4);
+    g.off[g.n] = u32(at / 4);
+    ++g.n;
+    st.items.push_back({it.dst, at, it.bytes, s});
+    at += it.bytes;
+  }
+  hipLaunchKernelGGL(k_d2h_gather, 1, 64, 0, s, g, reinterpret_cast<u32*>(st.dbuf));
+  HL_LAUNCH_CHECK();
+  st.used = off + total;
 }
 namespace {
 
@@ -2707,8 +2759,7 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   }
   u32 nupd = 0;
   u64 nruns = 0;
-  d2h_async(&nupd, a.ucount.p, sizeof(u32), s);
-  d2h_async(&nruns, c1.p, sizeof(u64), s);
+  d2h_group({{&nupd, a.ucount.p, sizeof(u32)}, {&nruns, c1.p, sizeof(u64)}}, s);
   stream_sync(s);
   plan.nruns = nruns;
   plan.nupd = nupd;
@@ -2752,10 +2803,11 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   exclusive_scan_u64_async(a.stlen.p, a.stoff.p, ns, tmp, s);
   exclusive_scan_u64_async(a.chunks.p, a.chunkbase.p, ns, tmp, s);
   u64 h[6] = {0, 0, 0, 0, 0, 0};
-  d2h_async(h, a.cnt.p, 3 * sizeof(u64), s);
-  d2h_async(h + 3, a.newoff.p + ns, sizeof(u64), s);
-  d2h_async(h + 4, a.chunkbase.p + ns, sizeof(u64), s);
-  d2h_async(h + 5, a.stoff.p + ns, sizeof(u64), s);
+  d2h_group({{h, a.cnt.p, 3 * sizeof(u64)},
+             {h + 3, a.newoff.p + ns, sizeof(u64)},
+             {h + 4, a.chunkbase.p + ns, sizeof(u64)},
+             {h + 5, a.stoff.p + ns, sizeof(u64)}},
+            s);
   stream_sync(s);
   plan.old_entries = h[0];
   plan.max_row = h[1];
@@ -2977,8 +3029,7 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
   select_flagged_async(rs.incc.p, n, rs.vcc.p, tmp, rs.cnt.p, s);
   u32 hs[3] = {0, 0, 0};
   u64 m = 0;
-  d2h_async(hs, rs.status.p, 3 * sizeof(u32), s);
-  d2h_async(&m, rs.cnt.p, sizeof(u64), s);
+  d2h_group({{hs, rs.status.p, 3 * sizeof(u32)}, {&m, rs.cnt.p, sizeof(u64)}}, s);
   stream_sync(s);
   *identity = hs[1] == 0;
   const u32 next_id = hs[2] + 1;

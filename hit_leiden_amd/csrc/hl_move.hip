@@ -986,13 +986,19 @@ struct TabShared {
   double stg[kStage];
   u32 scnt, sbase;
 };
+// a thread's share of the work counters of tab_body calls, added to the
+// device counters once (the persistent rounds add them when they end)
+struct TabTally {
+  unsigned long long nv = 0, ne = 0, ncand = 0;
+};
 // G lanes per vertex in workgroups of BS threads; tst2 (or null) takes the
-// evaluated vertices and row entries as well
+// evaluated vertices and row entries as well; with tally the counters are
+// added there instead
 template <class P, u32 G, u32 CAP, u32 BS>
 __device__ __forceinline__ void tab_body(const P& pol, const Csr& g, const u32* list, const u32* cnt_dev, u32* ovf,
                                          u32* ovf_cnt, const CandOut& out, unsigned long long* cst,
                                          unsigned long long* tst, TabShared<G, CAP, BS>& sh,
-                                         unsigned long long* tst2 = nullptr) {
+                                         unsigned long long* tst2 = nullptr, TabTally* tally = nullptr) {
   constexpr u32 NG = BS / G;
   u32* hk = sh.hk;
   double* hv = sh.hv;
@@ -1105,6 +1111,12 @@ __device__ __forceinline__ void tab_body(const P& pol, const Csr& g, const u32* 
     if (scnt > kStage - NG) flush_stage(stn, stt, stg, scnt, sbase, out);
   }
   flush_stage(stn, stt, stg, scnt, sbase, out);
+  if (tally) {
+    tally->nv += nv;
+    tally->ne += ne;
+    tally->ncand += ncand;
+    return;
+  }
   block_add_u64(cst, ncand);
   const u32 idx[3] = {0, 1, 2};
   unsigned long long vals[3] = {nv, ne, ncand};
@@ -1758,15 +1770,17 @@ __global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u3
 // ------------------------------------------------------------------ persistent rounds
 // A level's throughput rounds in one cooperative kernel of 1,024-thread
 // workgroups (at most one per CU; a grid barrier costs about 0.13 us per
-// workgroup, so the grid is small and the lists it takes are short): per round the proposals (a wavefront per listed vertex,
-// its neighbour communities in 512 LDS slots), a grid barrier, the decoupling
-// with its apply and in-block marks by workgroup 0 (dec_small_body), a grid
-// barrier, the grid-wide marks of long neighbour lists, a grid barrier.  The
-// rounds are the per-round launches' rounds: the same bodies, the same
-// counters, with no host round trip between them.  A round whose keys do not
-// fit the slots or whose candidates exceed the decoupling workgroup stops the
-// kernel before anything of it is applied, and the host runs it and the rest
-// with the per-round launches, as does a list longer than kPersistActive.
+// workgroup, so the grid is small and the lists it takes are short).  Per
+// round: the proposals (a wavefront per listed vertex, its neighbour
+// communities in up to 512 LDS slots), a grid barrier, the decoupling with its
+// apply and in-block marks by workgroup 0 (dec_small_body), a grid barrier,
+// and, when the decoupling left long neighbour lists, the grid-wide marks and
+// a grid barrier.  The rounds are the per-round launches' rounds: the same
+// bodies, the same counters, with no host round trip between them.  A round
+// whose keys do not fit the slots or whose candidates exceed the decoupling
+// workgroup stops the kernel before anything of it is applied, and the host
+// runs it and the rest with the per-round launches, as it does the rounds
+// after one that lists more than kPersistActive vertices.
 constexpr u32 kPersistBS = 1024;
 constexpr u32 kPersistG = 64, kPersistCap = 512;
 constexpr u32 kPersistDeg = 256;       // rows of at most this many entries: tables of kPersistCap slots
@@ -1788,8 +1802,8 @@ struct PersistArgs {
   FlatMarks fm;
   unsigned long long* dstat;
   // [0] rounds completed, [1] 1 when the rounds from [0] on are left to the
-  // per-round launches,
-  // [2] vertices evaluated and [3] moves accepted in the completed rounds
+  // per-round launches, [2] vertices evaluated and [3] moves accepted in the
+  // completed rounds
   unsigned long long* out;
 };
 union PersistShared {
@@ -1803,6 +1817,7 @@ __global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolic
   unsigned long long* cst = a.dstat + DS_MOVE + 2 * kBuckets + 1;  // counted as the half-wavefront bucket
   unsigned long long* tst = a.dstat + DS_TRY_MOVE + 3;
   unsigned long long* tst2 = a.dstat + DS_MOVE + 1;
+  TabTally tally;
   for (u32 r = 0; r < a.rounds; ++r) {
     const int ci = (a.ci0 + int(r)) & 1;
     const u32* cur = a.list[ci];
@@ -1812,7 +1827,7 @@ __global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolic
     // the claim bitmap of this round's marks, and the proposals
     for (u64 w = gtid(); w < a.mark_words; w += gstride()) a.mark[w] = 0;
     tab_body<MovePolicyT, kPersistG, kPersistCap, kPersistBS>(pol, m.g, cur, cur_cnt, a.ovf, a.ovf_cnt, a.cand, cst,
-                                                              tst, sh.tab, tst2);
+                                                              tst, sh.tab, tst2, &tally);
     grid.sync();
     // every workgroup reads the same counts here (workgroup 0 clears the
     // candidate count only on the way on), so all stop or go on together
@@ -1821,15 +1836,19 @@ __global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolic
         a.out[0] = r;
         a.out[1] = 1;
       }
-      return;
+      break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) a.out[2] += *cur_cnt;  // read before the decoupling clears it
     if (blockIdx.x == 0)
       dec_small_body(m, a.cand, a.minUto, a.minUfrom, a.minAto, a.minAfrom, a.mark, nxt, next_cnt, a.status, a.dstat,
                      cur_cnt, a.bcounts, a.fm, a.ccand, sh.dec);
     grid.sync();
-    marks_flat_body<kPersistBS>(m, a.fm, a.mark, nxt, next_cnt, sh.marks);
-    grid.sync();
+    // the grid-wide marks only when the decoupling workgroup left rows to them
+    // (the count it set stays until the next round's decoupling)
+    if (*a.fm.c != 0) {
+      marks_flat_body<kPersistBS>(m, a.fm, a.mark, nxt, next_cnt, sh.marks);
+      grid.sync();
+    }
     const u32 acc = a.status[1], nx = *next_cnt;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       a.out[3] += acc;
@@ -1838,7 +1857,16 @@ __global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolic
     }
     // throughput.rs:177-180, nothing left to evaluate, or a list for the
     // per-round launches' wider grids
-    if (acc == 0 || nx == 0 || nx > kPersistActive) return;
+    if (acc == 0 || nx == 0 || nx > kPersistActive) break;
+  }
+  block_add_u64(cst, tally.ncand);
+  {
+    const u32 idx[3] = {0, 1, 2};
+    unsigned long long vals[3] = {tally.nv, tally.ne, tally.ncand};
+    const u32 idx2[2] = {0, kBuckets};
+    unsigned long long vals2[2] = {tally.nv, tally.ne};
+    block_add_multi(tst2, idx2, vals2);
+    block_add_multi(tst, idx, vals);
   }
 }
 
@@ -2433,8 +2461,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     }
     unsigned long long o[4] = {0, 0, 0, 0};
     u32 cnts[3] = {0, 0, 0};  // counters [0] and [2]: the two lists' counts
-    d2h_async(o, ms.pout.p, sizeof(o), s);
-    d2h_async(cnts, lcnt[0], sizeof(cnts), s);
+    d2h_group({{o, ms.pout.p, sizeof(o)}, {cnts, lcnt[0], sizeof(cnts)}}, s);
     stream_sync(s);
     const u32 done = u32(o[0]);
     rounds += done;
@@ -2593,8 +2620,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
           HL_LAUNCH_CHECK();
         }
       }
-      d2h_async(&accepted, acc_cnt, sizeof(u32), s);
-      d2h_async(&next_host, next_cnt, sizeof(u32), s);
+      d2h_group({{&accepted, acc_cnt, sizeof(u32)}, {&next_host, next_cnt, sizeof(u32)}}, s);
       hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, cur_cnt, ccnt, bcounts);
       HL_LAUNCH_CHECK();
       fill_u32(status, 0, 2, s);
