@@ -41,7 +41,7 @@ SLOT_KERNEL = {
     "try_hub": "k_prop2_hub<hl::(Move|Refine)PolicyT, 512u?, 4096u?>",
     "try_giant": "k_prop2_hub<hl::(Move|Refine)PolicyT, 1024u?, 8192u?>", "propose_big": "k_prop2_big",
     "decouple": "k_dec_small", "apply_moves": "k_apply4", "mark_moved": "k_marks_flat",
-    "row_merge": "k_row_merge", "resolve_runs": "k_resolve_runs",
+    "row_merge": "k_row_merge", "resolve_runs": "k_resolve_runs", "move_rounds": "k_move_rounds",
 }
 
 
@@ -81,6 +81,7 @@ def cpu_baseline(w, levels, gamma, budget_s, threads):
     t0 = time.time()
     st.load(w.n, w.u, w.v, w.w)
     load_s = time.time() - t0
+    log(f"[bench] cpu baseline: load {load_s:.1f}s")
     spent, edges, done = 0.0, 0, 0
     for (u, v, a) in w.batches:
         t = time.perf_counter()
@@ -90,6 +91,7 @@ def cpu_baseline(w, levels, gamma, budget_s, threads):
         done += 1
         if spent >= budget_s:
             break
+    log(f"[bench] cpu baseline: {done} batches in {spent:.1f}s")
     q = st.modularity(st.hierarchy()[-1])
     return {"value": edges / spent if spent > 0 else 0.0, "unit": "edges/s", "cores": threads, "kind": "port",
             "sample": f"oracle throughput mode ({levels} levels) on the same graph: load {load_s:.1f}s untimed, "
@@ -126,6 +128,7 @@ def cpu_variant_child(config, variant, levels, gamma, budget_s, nbatches, thread
     t0 = time.time()
     st.load(w.n, w.u, w.v, w.w)
     load_s = time.time() - t0
+    log(f"[bench] cpu variant {variant}: load {load_s:.1f}s")
     spent, edges, done = 0.0, 0, 0
     for (u, v, a) in w.batches:
         t = time.perf_counter()
@@ -154,6 +157,7 @@ def cpu_variant(args, nbatches, variant, threads):
            "--levels", str(args.levels), "--gamma", str(args.gamma), "--cpu-budget", str(args.det_budget),
            "--steps", str(nbatches), "--warmup", "0", "--cpu-threads", str(threads)]
     out = ""
+    log(f"[bench] cpu variant {variant}: at most {args.det_timeout:.0f}s")
     try:
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=sys.stderr, timeout=args.det_timeout, text=True)
         out = r.stdout

@@ -142,7 +142,7 @@ enum ProfSlot {
   P_MARK_MOVED, P_LABEL_TOTALS, P_SELECT, P_SORT, P_SCAN, P_CC_SPLIT, P_REFINE, P_AGG_EMIT, P_COMPRESS,
   P_DEF_UPDATE, P_FILL, P_DET_MOVE, P_DET_REFINE, P_TWICE_M, P_SEED, P_MODULARITY, P_REDUCE, P_DEC_MULTI,
   P_PROP_LDS, P_TRY_HALF, P_TRY_WAVE, P_TRY_LDS, P_TRY_HUB, P_TRY_GIANT, P_PROP_TABLE, P_CACHE_UPDATE, P_RELABEL,
-  P_COUNT
+  P_MOVE_ROUNDS, P_COUNT
 };
 inline const char* prof_name(int s) {
   static const char* names[P_COUNT] = {
@@ -150,7 +150,7 @@ inline const char* prof_name(int s) {
       "apply_moves", "mark_moved", "label_totals", "select", "sort", "scan", "cc_split", "refine", "agg_emit",
       "compress", "def_update", "fill", "det_move", "det_refine", "twice_m", "seed", "modularity", "reduce",
       "decouple_multi", "propose_lds", "try_half", "try_wave", "try_lds", "try_hub", "try_giant", "propose_table",
-      "cache_update", "relabel"};
+      "cache_update", "relabel", "move_rounds"};
   return (s >= 0 && s < P_COUNT) ? names[s] : "?";
 }
 

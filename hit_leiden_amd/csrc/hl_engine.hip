@@ -497,6 +497,10 @@ struct Engine {
     const u64 dec = 20 * d[DS_DEC_CAND] + 28 * d[DS_DEC_ACC] + 12 * d[DS_DEC_DEG];
     const u64 multi = 20 * d[DS_MULTI_CAND], apply = 28 * d[DS_MULTI_ACC], marks = 12 * d[DS_MULTI_DEG];
     total += dec + multi + apply + marks;
+    // the persistent rounds: their proposals (as a bucket's), decoupling and marks
+    const u64 pr = 20 * d[DS_PR_PROP] + 16 * d[DS_PR_PROP + 1] + 8 * d[DS_PR_PROP + 2] + 20 * d[DS_PR_DEC] +
+                   28 * d[DS_PR_DEC + 1] + 12 * d[DS_PR_DEC + 2] + 12 * d[DS_PR_DEC + 5];
+    total += pr;
     if (st) {
       st->bytes_alg += total;
       st->settled += d[DS_SETTLED];
@@ -507,13 +511,13 @@ struct Engine {
       // such vertex, movement and refinement, hub-table vertices included
       u64 listed = d[DS_HT];
       for (u32 b = 1; b < kBuckets; ++b) listed += d[DS_MOVE + b] + d[DS_REF + b];
-      const int slots[14] = {P_PROP_SMALL, P_TRY_HALF, P_TRY_WAVE, P_PROP_WAVE, P_TRY_LDS, P_PROP_LDS, P_TRY_HUB,
+      const int slots[15] = {P_PROP_SMALL, P_TRY_HALF, P_TRY_WAVE, P_PROP_WAVE, P_TRY_LDS, P_PROP_LDS, P_TRY_HUB,
                              P_TRY_GIANT, P_PROP_BIG, P_DECOUPLE, P_DEC_MULTI, P_APPLY_MOVES, P_MARK_MOVED,
-                             P_PROP_TABLE};
-      const u64 by[14] = {prop[0] + 12 * listed, tried[1], tried[2], prop[1] + prop[2] - tried[1] - tried[2], tried[3],
+                             P_PROP_TABLE, P_MOVE_ROUNDS};
+      const u64 by[15] = {prop[0] + 12 * listed, tried[1], tried[2], prop[1] + prop[2] - tried[1] - tried[2], tried[3],
                           prop[3] - tried[3], tried[4], tried[5], prop[4] + prop[5] - tried[4] - tried[5],
-                          dec, multi, apply, marks, htab};
-      for (int i = 0; i < 14; ++i)
+                          dec, multi, apply, marks, htab, pr};
+      for (int i = 0; i < 15; ++i)
         if (prof.records(slots[i])) prof.bytes[slots[i]] += by[i];
     }
   }

@@ -178,7 +178,12 @@ enum DevStat : u32 {
   // hub-table evaluations: vertices, table slots scanned, live candidates
   DS_HT = DS_TRY_REF + 3 * kBuckets,
   DS_SETTLED = DS_HT + 3,  // candidates the in-order settle of the multi-kernel decoupling decided
-  DS_COUNT
+  // the persistent rounds (k_move_rounds): vertices, row entries and distinct
+  // candidate communities of their proposals, then their decoupling's
+  // counters in the order of DS_DEC_CAND .. DS_MULTI_DEG
+  DS_PR_PROP,
+  DS_PR_DEC = DS_PR_PROP + 3,
+  DS_COUNT = DS_PR_DEC + 6
 };
 
 struct MoveScratch {

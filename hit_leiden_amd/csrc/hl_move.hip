@@ -1814,9 +1814,8 @@ union PersistShared {
 __global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolicyT pol, PersistArgs a) {
   __shared__ PersistShared sh;
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-  unsigned long long* cst = a.dstat + DS_MOVE + 2 * kBuckets + 1;  // counted as the half-wavefront bucket
-  unsigned long long* tst = a.dstat + DS_TRY_MOVE + 3;
-  unsigned long long* tst2 = a.dstat + DS_MOVE + 1;
+  unsigned long long* pst = a.dstat + DS_PR_PROP;
+  unsigned long long* dst = a.dstat + (DS_PR_DEC - DS_DEC_CAND);  // dst[DS_DEC_CAND] is dstat[DS_PR_DEC]
   TabTally tally;
   for (u32 r = 0; r < a.rounds; ++r) {
     const int ci = (a.ci0 + int(r)) & 1;
@@ -1826,8 +1825,8 @@ __global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolic
     u32* next_cnt = a.lcnt[ci ^ 1];
     // the claim bitmap of this round's marks, and the proposals
     for (u64 w = gtid(); w < a.mark_words; w += gstride()) a.mark[w] = 0;
-    tab_body<MovePolicyT, kPersistG, kPersistCap, kPersistBS>(pol, m.g, cur, cur_cnt, a.ovf, a.ovf_cnt, a.cand, cst,
-                                                              tst, sh.tab, tst2, &tally);
+    tab_body<MovePolicyT, kPersistG, kPersistCap, kPersistBS>(pol, m.g, cur, cur_cnt, a.ovf, a.ovf_cnt, a.cand, pst,
+                                                              pst, sh.tab, nullptr, &tally);
     grid.sync();
     // every workgroup reads the same counts here (workgroup 0 clears the
     // candidate count only on the way on), so all stop or go on together
@@ -1840,7 +1839,7 @@ __global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolic
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) a.out[2] += *cur_cnt;  // read before the decoupling clears it
     if (blockIdx.x == 0)
-      dec_small_body(m, a.cand, a.minUto, a.minUfrom, a.minAto, a.minAfrom, a.mark, nxt, next_cnt, a.status, a.dstat,
+      dec_small_body(m, a.cand, a.minUto, a.minUfrom, a.minAto, a.minAfrom, a.mark, nxt, next_cnt, a.status, dst,
                      cur_cnt, a.bcounts, a.fm, a.ccand, sh.dec);
     grid.sync();
     // the grid-wide marks only when the decoupling workgroup left rows to them
@@ -1859,21 +1858,17 @@ __global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolic
     // per-round launches' wider grids
     if (acc == 0 || nx == 0 || nx > kPersistActive) break;
   }
-  block_add_u64(cst, tally.ncand);
-  {
-    const u32 idx[3] = {0, 1, 2};
-    unsigned long long vals[3] = {tally.nv, tally.ne, tally.ncand};
-    const u32 idx2[2] = {0, kBuckets};
-    unsigned long long vals2[2] = {tally.nv, tally.ne};
-    block_add_multi(tst2, idx2, vals2);
-    block_add_multi(tst, idx, vals);
-  }
+  const u32 idx[3] = {0, 1, 2};
+  unsigned long long vals[3] = {tally.nv, tally.ne, tally.ncand};
+  block_add_multi(pst, idx, vals);
 }
 
-// workgroups of the persistent rounds on the current device (0: not available)
+// workgroups of the persistent rounds per device (0: not available)
+static u32 g_persist_grid[64];
+static bool g_persist_known[64];
 static u32 persist_grid() {
-  static u32 grid[64];
-  static bool known[64];
+  u32* grid = g_persist_grid;
+  bool* known = g_persist_known;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
   if (!known[dev]) {
@@ -1888,6 +1883,16 @@ static u32 persist_grid() {
     (void)hipGetLastError();
   }
   return grid[dev];
+}
+// A cooperative launch the runtime refused: the device's rounds take the
+// per-round launches from then on (the same results), and stderr says why, once.
+static void persist_refused(hipError_t e, hipError_t before) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+  if (g_persist_grid[dev])
+    std::fprintf(stderr, "[hl] device %d refused the cooperative rounds kernel (%s; earlier error state: %s); "
+                 "the per-round launches run from here on\n", dev, hipGetErrorString(e), hipGetErrorString(before));
+  g_persist_grid[dev] = 0;
 }
 
 // The moves' cache updates over the grid: entry t of the moved vertices'
@@ -2427,7 +2432,9 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   const u32 pgrid = th.persist ? std::min(persist_grid(), th.persist_blocks) : 0;
   const bool may_persist = pgrid && !m.ck && !m.ht.bits && !(hook && hook->fn) && maxdeg <= kPersistDeg;
   u32 left_round = kNone;  // the round the kernel last left to the launches
-  auto run_persist = [&]() {
+  bool refused = false;    // the runtime refused the cooperative launch
+  // false when the kernel could not be launched (nothing was enqueued)
+  auto run_persist = [&]() -> bool {
     ms.hovf.reserve(u64(n) + 1, s);
     ms.pout.reserve(4, s);
     fill_u32(reinterpret_cast<u32*>(ms.pout.p), 0, 8, s);
@@ -2455,9 +2462,16 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     a.out = ms.pout.p;
     void* args[] = {&m, &pol, &a};
     {
-      HL_PROF(P_DECOUPLE, 0, s);
-      HL_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_move_rounds), dim3(pgrid),
-                                          dim3(kPersistBS), args, 0, s));
+      HL_PROF(P_MOVE_ROUNDS, 0, s);
+      const hipError_t before = hipGetLastError();
+      const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_move_rounds), dim3(pgrid),
+                                                      dim3(kPersistBS), args, 0, s);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        persist_refused(e, before);
+        refused = true;
+        return false;
+      }
     }
     unsigned long long o[4] = {0, 0, 0, 0};
     u32 cnts[3] = {0, 0, 0};  // counters [0] and [2]: the two lists' counts
@@ -2485,12 +2499,11 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     } else {
       host_cnt = 0;  // the last round accepted nothing, left nothing to evaluate, or was the last allowed
     }
+    return true;
   };
   while (host_cnt > 0 && int(rounds) < max_rounds) {
-    if (may_persist && !queued && host_cnt <= kPersistActive && rounds != left_round) {
-      run_persist();
+    if (may_persist && !refused && !queued && host_cnt <= kPersistActive && rounds != left_round && run_persist())
       continue;
-    }
     if (st) st->evaluated += host_cnt;
     u32* cur_cnt = lcnt[ci];
     u32* next_cnt = lcnt[ci ^ 1];

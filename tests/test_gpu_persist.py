@@ -4,8 +4,10 @@ give the states of the per-round launches, bit-exact with unit weights, and
 the same counters (rounds, evaluations, moves).  Covered: rounds that stay in
 the kernel to convergence, a first round with more candidates than the
 decoupling workgroup takes (the kernel leaves it and the rest to the per-round
-launches), rows longer than the kernel takes (the level skips it), and round
-caps that end the kernel early."""
+launches), rows longer than the kernel takes (the level skips it), round
+caps that end the kernel early, grids of 2 to 64 workgroups, and a second
+process, to which the runtime refuses cooperative launches while this one
+holds them (its rounds take the per-round launches)."""
 import numpy as np
 import pytest
 
@@ -163,3 +165,47 @@ def test_round_caps(max_rounds):
     u, v = random_edges(n, 10000, rng)
     say(f"persist: max_rounds={max_rounds}")
     run_both(n, u, v, insert_batches(n, 3, 150, rng), 3, max_rounds=max_rounds)
+
+
+_CHILD = r"""
+import json, sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import hit_leiden_amd as hl
+from hit_leiden_amd import gen
+w = gen.small_config()
+e = hl.HitLeiden(hl.RunConfig(mode=hl.RunMode.THROUGHPUT, levels=4))
+e.load(hl.GraphInput(w.n, w.u, w.v))
+ran = 0
+for (u, v, a) in w.batches[:2]:
+    e.hit_leiden(hl.GraphInput(w.n, u, v, a))
+    ran += e.last_stats.persist_rounds
+print(json.dumps({"ran": int(ran), "f": [e.partition(p, hl.MAP_F).tolist() for p in range(4)]}))
+"""
+
+
+def test_second_process_takes_the_launches():
+    """While this process holds the device's cooperative launches, the runtime
+    refuses them to a second process: its rounds take the per-round launches
+    and give the same hierarchy."""
+    import json
+    import os
+    import subprocess
+    import sys
+    w = gen.small_config()
+    e = engine(4, True)
+    e.load(hl.GraphInput(w.n, w.u, w.v))
+    ran = 0
+    for (u, v, a) in w.batches[:2]:
+        e.hit_leiden(hl.GraphInput(w.n, u, v, a))
+        ran += e.last_stats.persist_rounds
+    assert ran > 0
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    say("persist: a second process")
+    r = subprocess.run([sys.executable, "-c", _CHILD, root], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    say(f"persist: the second process ran {out['ran']} rounds in the kernel")
+    for p in range(4):
+        assert np.array_equal(np.array(out["f"][p], np.uint32), e.partition(p, hl.MAP_F)), f"level {p}"
+    e.close()
