@@ -384,7 +384,7 @@ def batch_summary(stats):
 
 def run_extra(args, config, extra):
     """The batched or served form, measured by this script in a child process
-    (a fresh device context: the large engine's memory is gone), as a summary
+    (a fresh device context, run before this process uses the device), as a summary
     of its JSON line with the speed-up over its plain path."""
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--config", config, "--no-cpu-baseline", "--no-extras",
@@ -489,6 +489,19 @@ def main():
     if args.config == "served":
         bench_served(args)
         return
+
+    extras = None
+    if not args.no_extras and args.config in ("large", "mid"):
+        # the batched and served forms against their plain one-batch-per-call
+        # GPU paths, each in a child process, run before this process uses the
+        # device: the runtime grants cooperative launches (the persistent
+        # movement rounds) to one process at a time
+        extras = {
+            "batched": run_extra(args, "batched", ["--steps", "4", "--warmup", "1", "--plain-sample", "32",
+                                                   "--autotune", "--tune-sample", "96"]),
+            "served": run_extra(args, "served", ["--rates", "2000", "--served-seconds", "2",
+                                                 "--plain-requests", "300"]),
+        }
 
     import hit_leiden_amd as hl
     from hit_leiden_amd import hip
@@ -622,14 +635,8 @@ def main():
     else:
         out["cpu_baseline"] = None
 
-    if not args.no_extras and args.config in ("large", "mid"):
-        # the batched and served forms against their plain one-batch-per-call GPU paths
-        out["extras"] = {
-            "batched": run_extra(args, "batched", ["--steps", "4", "--warmup", "1", "--plain-sample", "32",
-                                                   "--autotune", "--tune-sample", "96"]),
-            "served": run_extra(args, "served", ["--rates", "2000", "--served-seconds", "2",
-                                                 "--plain-requests", "300"]),
-        }
+    if extras is not None:
+        out["extras"] = extras
     print(json.dumps(out), flush=True)
 
 
