@@ -2687,9 +2687,114 @@ u64 exclusive_scan_u64(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s)
   return total;
 }
 
+// ---- one-launch stable sort of up to kSortSmall (key, value) pairs by one
+// workgroup of 1,024 threads (the library sort takes three to five launches):
+// each thread sorts 4 pairs in registers (odd-even transposition, swapping only
+// on a strictly greater key), then log2(N / 4) merge stages through LDS, each
+// thread placing 4 outputs from a merge-path search that takes ties from the
+// left run, so equal keys keep their input order, as the radix sort does.
+constexpr u32 kSortPer = 4, kSortSmall = 1024 * kSortPer;
+template <class K, class V>
+__global__ __launch_bounds__(1024) void k_sort_small(const K* kin, K* kout, const V* vin, V* vout, u32 n, u32 N) {
+  // N: a power of two, 4 <= N <= kSortSmall, N >= n; entries n .. N-1 are
+  // padding with the largest key, which stays behind every real entry
+  __shared__ K ks[2][kSortSmall];
+  __shared__ V vs[2][kSortSmall];
+  const u32 i0 = threadIdx.x * kSortPer;
+  const bool act = i0 < N;
+  if (act) {
+    K k[kSortPer];
+    V v[kSortPer];
+#pragma unroll
+    for (u32 r = 0; r < kSortPer; ++r) {
+      const u32 i = i0 + r;
+      k[r] = i < n ? kin[i] : ~K(0);
+      v[r] = i < n ? vin[i] : V();
+    }
+#pragma unroll
+    for (u32 round = 0; round < kSortPer; ++round) {
+#pragma unroll
+      for (u32 a = round & 1; a + 1 < kSortPer; a += 2)
+        if (k[a + 1] < k[a]) {
+          const K tk = k[a];
+          k[a] = k[a + 1];
+          k[a + 1] = tk;
+          const V tv = v[a];
+          v[a] = v[a + 1];
+          v[a + 1] = tv;
+        }
+    }
+#pragma unroll
+    for (u32 r = 0; r < kSortPer; ++r) {
+      ks[0][i0 + r] = k[r];
+      vs[0][i0 + r] = v[r];
+    }
+  }
+  __syncthreads();
+  u32 cur = 0;
+  for (u32 w = kSortPer; w < N; w <<= 1) {
+    if (act) {
+      // runs A = [start, start + w) and B = [start + w, start + 2w); this
+      // thread writes merged positions d .. d + 3 of the pair
+      const u32 start = i0 & ~(2 * w - 1), d = i0 - start;
+      const K* A = ks[cur] + start;
+      const K* B = A + w;
+      u32 lo = d > w ? d - w : 0, hi = d < w ? d : w;
+      while (lo < hi) {
+        const u32 mid = (lo + hi) >> 1;
+        if (!(B[d - 1 - mid] < A[mid]))
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      u32 i = lo, j = d - lo;
+#pragma unroll
+      for (u32 r = 0; r < kSortPer; ++r) {
+        const bool take_a = i < w && (j >= w || !(B[j] < A[i]));
+        const u32 from = take_a ? start + i : start + w + j;
+        ks[cur ^ 1][i0 + r] = ks[cur][from];
+        vs[cur ^ 1][i0 + r] = vs[cur][from];
+        if (take_a)
+          ++i;
+        else
+          ++j;
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (act) {
+#pragma unroll
+    for (u32 r = 0; r < kSortPer; ++r) {
+      const u32 i = i0 + r;
+      if (i < n) {
+        kout[i] = ks[cur][i];
+        vout[i] = vs[cur][i];
+      }
+    }
+  }
+}
+// the one-launch sort applies to full-width keys and n <= kSortSmall
+template <class K>
+static bool sort_small_ok(u64 n, int end_bit) {
+  return n <= kSortSmall && end_bit == int(8 * sizeof(K));
+}
+template <class K, class V>
+static void sort_small(const K* kin, K* kout, const V* vin, V* vout, u64 n, hipStream_t s) {
+  u32 N = kSortPer;
+  while (N < n) N <<= 1;
+  hipLaunchKernelGGL((k_sort_small<K, V>), 1, 1024, 0, s, kin, kout, vin, vout, u32(n), N);
+  HL_LAUNCH_CHECK();
+}
+
 void sort_pairs_u64(const u64* kin, u64* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
                     hipStream_t s) {
   if (n == 0) return;
+  if (sort_small_ok<u64>(n, end_bit)) {
+    HL_PROF(P_SORT, 24 * n, s);
+    sort_small(kin, kout, vin, vout, n, s);
+    return;
+  }
   size_t bytes = 0;
   HL_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
   void* t = tmp.get(bytes, s);
@@ -2699,6 +2804,11 @@ void sort_pairs_u64(const u64* kin, u64* kout, const u32* vin, u32* vout, u64 n,
 void sort_pairs_u64_f64(const u64* kin, u64* kout, const double* vin, double* vout, u64 n, int end_bit, Temp& tmp,
                         hipStream_t s) {
   if (n == 0) return;
+  if (sort_small_ok<u64>(n, end_bit)) {
+    HL_PROF(P_SORT, 32 * n, s);
+    sort_small(kin, kout, vin, vout, n, s);
+    return;
+  }
   size_t bytes = 0;
   HL_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
   void* t = tmp.get(bytes, s);
@@ -2708,6 +2818,11 @@ void sort_pairs_u64_f64(const u64* kin, u64* kout, const double* vin, double* vo
 void sort_pairs_u32(const u32* kin, u32* kout, const u32* vin, u32* vout, u64 n, int end_bit, Temp& tmp,
                     hipStream_t s) {
   if (n == 0) return;
+  if (sort_small_ok<u32>(n, end_bit)) {
+    HL_PROF(P_SORT, 16 * n, s);
+    sort_small(kin, kout, vin, vout, n, s);
+    return;
+  }
   size_t bytes = 0;
   HL_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, unsigned(end_bit), s));
   void* t = tmp.get(bytes, s);
