@@ -1803,7 +1803,7 @@ struct PersistArgs {
   unsigned long long* dstat;
   // [0] rounds completed, [1] 1 when the rounds from [0] on are left to the
   // per-round launches, [2] vertices evaluated and [3] moves accepted in the
-  // completed rounds
+  // completed rounds, [4] the first round's list count
   unsigned long long* out;
 };
 union PersistShared {
@@ -1816,6 +1816,17 @@ __global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolic
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   unsigned long long* pst = a.dstat + DS_PR_PROP;
   unsigned long long* dst = a.dstat + (DS_PR_DEC - DS_DEC_CAND);  // dst[DS_DEC_CAND] is dstat[DS_PR_DEC]
+  {
+    // the first list, read by every workgroup before any barrier (nothing
+    // changes it before the first one): an empty one ends the level, a long
+    // one goes to the per-round launches
+    const u32 c0 = *a.lcnt[a.ci0 & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.out[4] = c0;
+      if (c0 > kPersistActive) a.out[1] = 1;
+    }
+    if (c0 == 0 || c0 > kPersistActive) return;
+  }
   TabTally tally;
   for (u32 r = 0; r < a.rounds; ++r) {
     const int ci = (a.ci0 + int(r)) & 1;
@@ -2371,10 +2382,6 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   }
   MovePolicyT pol{m.f, m.tot, m.gamma, n, m.ck, m.ht};
   u32 host_cnt = 0;
-  d2h_async(&host_cnt, cnt_a, sizeof(u32), s);
-  stream_sync(s);
-  if (st) st->active = host_cnt;
-  if (hook && hook->fn) hook->fn(hook->ctx, cur, host_cnt);
   u32 rounds = 0;
   // One round: proposals over `lst` (count lcnt[cidx], read on the device),
   // the one-workgroup decoupling with its marks into `out` (count
@@ -2433,11 +2440,12 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   const bool may_persist = pgrid && !m.ck && !m.ht.bits && !(hook && hook->fn) && maxdeg <= kPersistDeg;
   u32 left_round = kNone;  // the round the kernel last left to the launches
   bool refused = false;    // the runtime refused the cooperative launch
+  u32 first_cnt = 0;       // the list count the kernel's first round read
   // false when the kernel could not be launched (nothing was enqueued)
   auto run_persist = [&]() -> bool {
     ms.hovf.reserve(u64(n) + 1, s);
-    ms.pout.reserve(4, s);
-    fill_u32(reinterpret_cast<u32*>(ms.pout.p), 0, 8, s);
+    ms.pout.reserve(8, s);
+    fill_u32(reinterpret_cast<u32*>(ms.pout.p), 0, 16, s);
     PersistArgs a{};
     a.list[0] = ms.list_a.p;
     a.list[1] = ms.list_b.p;
@@ -2473,11 +2481,12 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
         return false;
       }
     }
-    unsigned long long o[4] = {0, 0, 0, 0};
+    unsigned long long o[5] = {0, 0, 0, 0, 0};
     u32 cnts[3] = {0, 0, 0};  // counters [0] and [2]: the two lists' counts
     d2h_group({{o, ms.pout.p, sizeof(o)}, {cnts, lcnt[0], sizeof(cnts)}}, s);
     stream_sync(s);
     const u32 done = u32(o[0]);
+    first_cnt = u32(o[4]);
     rounds += done;
     if (st) {
       st->evaluated += o[2];
@@ -2501,6 +2510,24 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     }
     return true;
   };
+  // The level's first list count: read by the kernel itself when it may take
+  // the first rounds and the list is known to fit it (the seeding lists the
+  // records' endpoints: at most 2k vertices, or all n when k = 0), so the host
+  // does not wait before them; else read back.
+  bool counted = false;
+  if (may_persist && !m.gid && (k == 0 ? u64(n) : 2 * k) <= kPersistActive) {
+    host_cnt = 1;  // not known here: the kernel reads it
+    if (run_persist()) {
+      counted = true;
+      if (st) st->active = first_cnt;
+    }
+  }
+  if (!counted) {
+    d2h_async(&host_cnt, cnt_a, sizeof(u32), s);
+    stream_sync(s);
+    if (st) st->active = host_cnt;
+    if (hook && hook->fn) hook->fn(hook->ctx, cur, host_cnt);
+  }
   while (host_cnt > 0 && int(rounds) < max_rounds) {
     if (may_persist && !refused && !queued && host_cnt <= kPersistActive && rounds != left_round && run_persist())
       continue;
