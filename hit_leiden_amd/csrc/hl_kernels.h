@@ -69,6 +69,7 @@ void sort_pairs_u32(const u32* kin, u32* kout, const u32* vin, u32* vout, u64 n,
 // ---- graph maintenance ----
 struct ApplyScratch {
   DevBuf<u64> keys, keys_s, ukey, ukey_s, newcap, newoff, chunks, chunkbase, cnt, ucode, uscan;
+  DevBuf<u32> chunk_seg;  // merge block -> its segment
   // rows updated in place: their old entries staged here first (offsets by a
   // scan of the staged lengths)
   DevBuf<u64> stlen, stoff;

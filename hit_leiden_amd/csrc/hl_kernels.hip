@@ -577,26 +577,24 @@ __global__ void k_row_plan(u64 nseg, const u32* segs, u64 nupd, const u64* ukey_
   }
 }
 
-// segment j of merge block b: chunkbase[j] <= b < chunkbase[j+1]
-__device__ __forceinline__ u64 chunk_segment(u64 nseg, const u64* chunkbase, u64 b) {
-  u64 lo = 0, hi = nseg;
-  while (hi - lo > 1) {
-    u64 mid = (lo + hi) >> 1;
-    if (chunkbase[mid] <= b)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
+// chunk_seg[b] = the segment j of merge block b (chunkbase[j] <= b <
+// chunkbase[j+1]), one thread per segment: the stage and merge kernels then
+// find their segment with one load instead of a binary search over the
+// chunk bases (a chain of dependent loads in every block).  When every row
+// is one chunk (the common case) there is no table: block b is segment b.
+__global__ void k_chunk_seg(u64 nseg, const u64* chunkbase, u32* chunk_seg) {
+  for (u64 j = gtid(); j < nseg; j += gstride())
+    for (u64 b = chunkbase[j]; b < chunkbase[j + 1]; ++b) chunk_seg[b] = u32(j);
 }
 
 // The old entries of the rows updated in place, copied aside (the same
 // block-to-chunk mapping as the merge), so that the merge never reads what it
 // overwrites.
-__global__ __launch_bounds__(256) void k_row_stage(u64 nseg, const u64* chunkbase, const u32* seg_row, const u64* stlen,
-                                                    const u64* stoff, LevelCsrMut g, u32* st_col, double* st_w) {
+__global__ __launch_bounds__(256) void k_row_stage(const u32* chunk_seg, const u64* chunkbase, const u32* seg_row,
+                                                    const u64* stlen, const u64* stoff, LevelCsrMut g, u32* st_col,
+                                                    double* st_w) {
   const u64 b = blockIdx.x;
-  const u64 j = chunk_segment(nseg, chunkbase, b);
+  const u64 j = chunk_seg ? chunk_seg[b] : b;
   const u64 olen = stlen[j];
   if (olen == 0) return;
   const u64 c = b - chunkbase[j];
@@ -613,14 +611,14 @@ __global__ __launch_bounds__(256) void k_row_stage(u64 nseg, const u64* chunkbas
 // copy), else its new pool space.  Existing entry t goes to t - #deleted-before
 // + #inserted-before; an insert goes to lower_bound(old row) - #deleted-before
 // + #inserted-before.
-__global__ __launch_bounds__(256) void k_row_merge(u64 nseg, const u64* chunkbase, const u32* segs,
+__global__ __launch_bounds__(256) void k_row_merge(const u32* chunk_seg, const u64* chunkbase, const u32* segs,
                                                     const u32* seg_row, const u32* seg_len, const u32* seg_ins,
                                                     const u32* seg_del, const u64* ukey_s, const u32* perm,
                                                     const u8* uop, const double* uw, const u32* ins_pre,
                                                     const u32* del_pre, const u64* newoff, const u64* newcap,
                                                     const u64* stoff, const u32* st_col, const double* st_w,
                                                     u64 pool_base, LevelCsrMut g) {
-  const u64 j = chunk_segment(nseg, chunkbase, blockIdx.x);
+  const u64 j = chunk_seg ? chunk_seg[blockIdx.x] : blockIdx.x;
   const u64 c = blockIdx.x - chunkbase[j];
   const u32 r = seg_row[j];
   const u64 oo = g.row_off[r];
@@ -2936,16 +2934,24 @@ void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, co
                         const TotalsRef& tr, hipStream_t s) {
   if (plan.nseg == 0) return;
   const u64 ns = plan.nseg;
+  const u32* chunk_seg = nullptr;  // block b is segment b when every row is one chunk
+  if (plan.total_chunks != ns) {
+    a.chunk_seg.reserve(plan.total_chunks, s);
+    hipLaunchKernelGGL(k_chunk_seg, GRID(ns), ns, a.chunkbase.p, a.chunk_seg.p);
+    HL_LAUNCH_CHECK();
+    chunk_seg = a.chunk_seg.p;
+  }
   if (plan.staged) {
     a.st_col.reserve(plan.staged, s, false, 1.25);
     a.st_w.reserve(plan.staged, s, false, 1.25);
-    hipLaunchKernelGGL(k_row_stage, dim3(unsigned(plan.total_chunks)), dim3(256), 0, s, ns, a.chunkbase.p,
-                       a.seg_row.p, a.stlen.p, a.stoff.p, g, a.st_col.p, a.st_w.p);
+    hipLaunchKernelGGL(k_row_stage, dim3(unsigned(plan.total_chunks)), dim3(256), 0, s, chunk_seg,
+                       a.chunkbase.p, a.seg_row.p, a.stlen.p, a.stoff.p, g, a.st_col.p, a.st_w.p);
     HL_LAUNCH_CHECK();
   }
   {
   HL_PROF(P_ROW_MERGE, 12 * (plan.old_entries + plan.new_entries) + 21 * plan.nupd, s);
-  hipLaunchKernelGGL(k_row_merge, dim3(unsigned(plan.total_chunks)), dim3(256), 0, s, ns, a.chunkbase.p, a.segs.p,
+  hipLaunchKernelGGL(k_row_merge, dim3(unsigned(plan.total_chunks)), dim3(256), 0, s, chunk_seg, a.chunkbase.p,
+                     a.segs.p,
                      a.seg_row.p, a.seg_len.p, a.seg_ins.p, a.seg_del.p, a.ukey_s.p, a.perm.p, a.uop.p, a.uw.p,
                      a.ins_pre.p, a.del_pre.p, a.newoff.p, a.newcap.p, a.stoff.p, a.st_col.p, a.st_w.p, pool_base,
                      g);
