@@ -115,8 +115,8 @@ struct Engine {
   void dmax4(const u32* const a[4], u64 n, u32 out[4]) {
     mx.reserve(4, s);
     fill_u32(mx.p, 0, 4, s);
-    for (int i = 0; i < 4; ++i)
-      if (a[i]) max_u32(a[i], n, mx.p + i, s);
+    u32* const outs[4] = {mx.p, mx.p + 1, mx.p + 2, mx.p + 3};
+    max_u32_multi(a, n, outs, s);
     hl::d2h_async(out, mx.p, 4 * sizeof(u32), s);
     hl::stream_sync(s);
   }

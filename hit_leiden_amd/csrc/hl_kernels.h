@@ -49,6 +49,8 @@ void count4_u8_async(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64
 void fill4_u8(u8* a0, u8* a1, u8* a2, u8* a3, u8 v, u64 n, hipStream_t s);
 // *out = max(*out, max(a[0..n)))  (out must be initialised)
 void max_u32(const u32* a, u64 n, u32* out, hipStream_t s);
+// out[i] = max(out[i], max of a[i]) for the non-null a[i], one launch
+void max_u32_multi(const u32* const a[4], u64 n, u32* const out[4], hipStream_t s);
 // indices i with flags[i] != 0, ascending; returns the count (synchronises)
 u64 select_flagged(const u8* flags, u64 n, u32* out, Temp& tmp, DevBuf<u64>& cnt, hipStream_t s);
 // the same without the read-back: the count lands in *cnt_dev

@@ -316,6 +316,40 @@ __global__ __launch_bounds__(256) void k_max_u32(const u32* a, u64 n, u32* out) 
     if (m) atomicMax(out, m);
   }
 }
+// the maxima of up to four arrays of one length in one launch: out[i] takes
+// the maximum of a[i] (null arrays are skipped; two may share an output)
+struct MaxArrays {
+  const u32* a[4];
+  u32* out[4];
+};
+__global__ __launch_bounds__(256) void k_max_multi(MaxArrays g, u64 n) {
+  __shared__ u32 sm[4][4];
+  u32 m[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const u32* a = g.a[k];
+    if (!a) continue;
+    const bool vec = (reinterpret_cast<uintptr_t>(a) & 15) == 0;
+    const u64 nv = vec ? n / 4 : 0;
+    const uint4* a4 = reinterpret_cast<const uint4*>(a);
+    for (u64 i = gtid(); i < nv; i += gstride()) {
+      const uint4 q = a4[i];
+      m[k] = max(m[k], max(max(q.x, q.y), max(q.z, q.w)));
+    }
+    for (u64 i = 4 * nv + gtid(); i < n; i += gstride()) m[k] = max(m[k], a[i]);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    for (int o = 32; o > 0; o >>= 1) m[k] = max(m[k], (u32)__shfl_xor((int)m[k], o, 64));
+    if ((threadIdx.x & 63) == 0) sm[k][threadIdx.x >> 6] = m[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = int(threadIdx.x);
+    const u32 x = max(max(sm[k][0], sm[k][1]), max(sm[k][2], sm[k][3]));
+    if (g.a[k] && x) atomicMax(g.out[k], x);
+  }
+}
 // Ordered stream compaction of a byte-flag array into the ascending list of
 // flagged indices, in two launches and without a look-back: each workgroup owns
 // a chunk of kSelChunk flags (kSelPer consecutive flags per thread).
@@ -2561,6 +2595,18 @@ void max_u32(const u32* a, u64 n, u32* out, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_max_u32, blocks_for(n / 4 + 1, 256, kReduceBlocks), 256, 0, s, a, n, out);
   HL_LAUNCH_CHECK();
 }
+void max_u32_multi(const u32* const a[4], u64 n, u32* const out[4], hipStream_t s) {
+  MaxArrays g{};
+  u32 arrays = 0;
+  for (int k = 0; k < 4; ++k) {
+    g.a[k] = a[k];
+    g.out[k] = out[k];
+    arrays += a[k] != nullptr;
+  }
+  HL_PROF(P_REDUCE, 4 * n * arrays, s);
+  if (n && arrays) hipLaunchKernelGGL(k_max_multi, blocks_for(n / 4 + 1, 256, kReduceBlocks), 256, 0, s, g, n);
+  HL_LAUNCH_CHECK();
+}
 
 // ---- one-launch forms of select and scan for small inputs: each launch costs
 // a few microseconds whatever its size, and the library forms take two or three
@@ -3350,8 +3396,11 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
   DevBuf<u32>& mx = as.mx;
   mx.reserve(1, s);
   fill_u32(mx.p, 0, 1, s);
-  max_u32(s_cur, n, mx.p, s);
-  max_u32(s_pre, n, mx.p, s);
+  {
+    const u32* const arr[4] = {s_cur, s_pre, nullptr, nullptr};
+    u32* const outs[4] = {mx.p, mx.p, nullptr, nullptr};
+    max_u32_multi(arr, n, outs, s);
+  }
   u32 hm = 0;
   d2h_async(&hm, mx.p, sizeof(u32), s);
   const u64 nr = n ? select_flagged(R, n, as.rlist.p, tmp, c1, s) : 0;
@@ -3649,8 +3698,11 @@ void compare_partitions(const u32* a, const u32* b, u64 n, u32 la, u32 lb, doubl
   DevBuf<u32> mx;
   mx.reserve(2, s);
   fill_u32(mx.p, 0, 2, s);
-  max_u32(a, n, mx.p, s);
-  max_u32(b, n, mx.p + 1, s);
+  {
+    const u32* const arr[4] = {a, b, nullptr, nullptr};
+    u32* const outs[4] = {mx.p, mx.p + 1, nullptr, nullptr};
+    max_u32_multi(arr, n, outs, s);
+  }
   u32 hm[2];
   d2h_async(hm, mx.p, sizeof(hm), s);
   stream_sync(s);
