@@ -70,6 +70,14 @@ struct D2H {
 constexpr size_t kD2HGroupMax = 8, kD2HGroupBytes = 1024;
 void d2h_group(std::initializer_list<D2H> items, hipStream_t s);
 void stream_sync(hipStream_t s);
+// Waits until the values read back on stream s (d2h_async, d2h_group) have
+// arrived and delivers them, like stream_sync, but not for work enqueued on s
+// after them: the gather kernel stores a token after its values, and the host
+// polls the token instead of waiting for the stream (about 5 us less per wait).
+// For waits whose only purpose is the values read back; a read-back of a
+// runtime copy, or a token that does not come within a second (a failed
+// stream), takes stream_sync.
+void stream_wait_reads(hipStream_t s);
 // Forget the values staged for stream s without delivering them, after
 // waiting (errors ignored) for their copies: a call that ended with an
 // exception leaves destinations in the frames it unwound.
@@ -181,10 +189,12 @@ struct Profiler {
   void reset() {
     for (int i = 0; i < P_COUNT; ++i) ms[i] = 0.0, launches[i] = 0, bytes[i] = 0;
   }
-  // call after the stream has been synchronised
+  // waits for the recorded events (a wait for read-back values alone leaves
+  // the stream's last kernels running)
   void flush() {
     for (auto& p : pending) {
       float t = 0.f;
+      HL_CHECK(hipEventSynchronize(p.b));
       HL_CHECK(hipEventElapsedTime(&t, p.a, p.b));
       ms[p.slot] += t;
       launches[p.slot] += 1;

@@ -118,7 +118,7 @@ struct Engine {
     u32* const outs[4] = {mx.p, mx.p + 1, mx.p + 2, mx.p + 3};
     max_u32_multi(a, n, outs, s);
     hl::d2h_async(out, mx.p, 4 * sizeof(u32), s);
-    hl::stream_sync(s);
+    hl::stream_wait_reads(s);
   }
 
   u32 dmax(const u32* a, u64 n) {
@@ -127,7 +127,7 @@ struct Engine {
     max_u32(a, n, mx.p, s);
     u32 h = 0;
     hl::d2h_async(&h, mx.p, sizeof(u32), s);
-    hl::stream_sync(s);
+    hl::stream_wait_reads(s);
     return h;
   }
 
@@ -361,7 +361,7 @@ struct Engine {
       if (st && p < HL_MAX_LEVELS) {
         u64 pops = 0;
         hl::d2h_async(&pops, dms.pops.p, sizeof(u64), s);
-        hl::stream_sync(s);
+        hl::stream_wait_reads(s);
         st->evaluated[p] += pops;
         st->rounds[p] = 1;
       }
@@ -841,7 +841,7 @@ struct ProfBind {
   explicit ProfBind(hl::Engine* e) { hl::g_prof = e ? &e->prof : nullptr; }
   ~ProfBind() {
     if (hl::g_prof && hl::g_prof->on && !hl::g_prof->pending.empty()) {
-      // stream work of this call has completed when the call returns normally
+      // flush waits for the events the call recorded
       try {
         hl::g_prof->flush();
       } catch (...) {

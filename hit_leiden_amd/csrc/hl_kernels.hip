@@ -53,7 +53,10 @@ struct Staging {
     void* dst;
     size_t off, bytes;
     hipStream_t s;
+    u64 seq;         // its gather kernel's call number (0: a runtime copy)
+    size_t tok_off;  // where that kernel stores its token, the low 32 bits of seq
   };
+  u64 next_seq = 0;
   std::vector<Item> items;
   ~Staging() {
     if (buf) (void)hipHostFree(buf);
@@ -116,10 +119,14 @@ static size_t staging_reserve(size_t bytes) {
     }
     off = 0;
     if (bytes > st.cap) {
+      // a gather kernel whose values were taken by their token may not have
+      // ended yet: no kernel may run while the area goes
+      if (st.buf) HL_CHECK(hipDeviceSynchronize());
       if (st.buf) HL_CHECK(hipHostFree(st.buf));
       st.buf = st.dbuf = nullptr;
       const size_t cap = std::max<size_t>(bytes, 1 << 16);
-      HL_CHECK(hipHostMalloc((void**)&st.buf, cap, hipHostMallocDefault));
+      // the host polls tokens the gather kernel stores here: coherent memory
+      HL_CHECK(hipHostMalloc((void**)&st.buf, cap, hipHostMallocCoherent));
       HL_CHECK(hipHostGetDevicePointer((void**)&st.dbuf, st.buf, 0));
       st.cap = cap;
     }
@@ -136,7 +143,7 @@ void d2h_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
   Staging& st = staging();
   const size_t off = staging_reserve(bytes);
   HL_CHECK(hipMemcpyAsync(st.buf + off, src, bytes, hipMemcpyDeviceToHost, s));
-  st.items.push_back({dst, off, bytes, s});
+  st.items.push_back({dst, off, bytes, s, 0, 0});
   st.used = off + bytes;
 }
 
@@ -147,9 +154,12 @@ struct D2HGather {
   u32 off[kD2HGroupMax];  // in words from the area's start
   u32 n;
 };
-__global__ void k_d2h_gather(D2HGather g, u32* area) {
+// one wavefront: its stores of the values, a system-scope fence, then the token
+__global__ __launch_bounds__(64) void k_d2h_gather(D2HGather g, u32* area, u32 tok_word, u32 tok) {
   for (u32 i = 0; i < g.n; ++i)
     for (u32 w = threadIdx.x; w < g.words[i]; w += blockDim.x) area[g.off[i] + w] = g.src[i][w];
+  __threadfence_system();
+  if (threadIdx.x == 0) __hip_atomic_store(area + tok_word, tok, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 void d2h_group(std::initializer_list<D2H> items, hipStream_t s) {
@@ -162,7 +172,14 @@ void d2h_group(std::initializer_list<D2H> items, hipStream_t s) {
       throw std::logic_error("d2h_group: items must be 4-byte words, at most kD2HGroupBytes each");
     total += it.bytes;
   }
-  const size_t off = staging_reserve(total);
+  const size_t off = staging_reserve(total + 4);  // the values, then the token
+  const size_t tok_off = off + total;
+  do ++st.next_seq;
+  while (u32(st.next_seq) == 0);  // a token is never 0, the value the slot is cleared to
+  const u64 seq = st.next_seq;
+  const u32 tok = u32(seq);
+  // a slot reused from an earlier call holds an older token (or data): clear it
+  __atomic_store_n(reinterpret_cast<u32*>(st.buf + tok_off), 0u, __ATOMIC_RELEASE);
   D2HGather g{};
   size_t at = off;
   for (const D2H& it : items) {
@@ -170,12 +187,61 @@ void d2h_group(std::initializer_list<D2H> items, hipStream_t s) {
     g.words[g.n] = u32(it.bytes / 4);
     g.off[g.n] = u32(at / 4);
     ++g.n;
-    st.items.push_back({it.dst, at, it.bytes, s});
+    st.items.push_back({it.dst, at, it.bytes, s, seq, tok_off});
     at += it.bytes;
   }
-  hipLaunchKernelGGL(k_d2h_gather, 1, 64, 0, s, g, reinterpret_cast<u32*>(st.dbuf));
+  hipLaunchKernelGGL(k_d2h_gather, 1, 64, 0, s, g, reinterpret_cast<u32*>(st.dbuf), u32(tok_off / 4), tok);
   HL_LAUNCH_CHECK();
-  st.used = off + total;
+  st.used = off + total + 4;
+}
+
+void stream_wait_reads(hipStream_t s) {
+  Staging& st = staging();
+  // the stream's latest gather (stream order makes its token the last of the
+  // stream's values to land); any runtime copy among them: wait for the stream
+  u64 seq = 0;
+  size_t tok_off = 0;
+  bool copies = false, any = false;
+  for (const auto& it : st.items) {
+    if (it.s != s) continue;
+    any = true;
+    if (it.seq == 0)
+      copies = true;
+    else if (it.seq > seq) {
+      seq = it.seq;
+      tok_off = it.tok_off;
+    }
+  }
+  const u32 tok = u32(seq);
+  if (!any || copies) {
+    stream_sync(s);
+    return;
+  }
+  const volatile u32* slot = reinterpret_cast<const volatile u32*>(st.buf + tok_off);
+  bool arrived = false;
+  const double t0 = host_seconds();
+  for (u64 spin = 0;; ++spin) {
+    if (__atomic_load_n(const_cast<const u32*>(slot), __ATOMIC_ACQUIRE) == tok) {
+      arrived = true;
+      break;
+    }
+    if ((spin & 1023) == 1023 && host_seconds() - t0 > 1.0) break;
+  }
+  if (!arrived) {
+    stream_sync(s);  // reports the stream's error, or waits for a slow one
+    return;
+  }
+  g_alloc.syncs += 1;
+  size_t keep = 0;
+  for (size_t i = 0; i < st.items.size(); ++i) {
+    const auto& it = st.items[i];
+    if (it.s == s)
+      std::memcpy(it.dst, st.buf + it.off, it.bytes);
+    else
+      st.items[keep++] = it;
+  }
+  st.items.resize(keep);
+  if (keep == 0) st.used = 0;
 }
 namespace {
 
@@ -2570,7 +2636,7 @@ void count4_u8(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, De
   }
   HL_LAUNCH_CHECK();
   d2h_async(out, cnt.p, 4 * sizeof(u64), s);
-  stream_sync(s);
+  stream_wait_reads(s);
 }
 void count4_u8_async(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, u64* out, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, out, u64(0), u64(4));
@@ -2693,7 +2759,7 @@ u64 select_flagged(const u8* flags, u64 n, u32* out, Temp& tmp, DevBuf<u64>& cnt
   select_flagged_async(flags, n, out, tmp, cnt.p, s);
   u64 c = 0;
   d2h_async(&c, cnt.p, sizeof(u64), s);
-  stream_sync(s);
+  stream_wait_reads(s);
   return c;
 }
 
@@ -2727,7 +2793,7 @@ u64 exclusive_scan_u64(const u64* in, u64* out, u64 n, Temp& tmp, hipStream_t s)
   exclusive_scan_u64_async(in, out, n, tmp, s);
   u64 total = 0;
   d2h_async(&total, out + n, sizeof(u64), s);
-  stream_sync(s);
+  stream_wait_reads(s);
   return total;
 }
 
@@ -2919,7 +2985,7 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   u32 nupd = 0;
   u64 nruns = 0;
   d2h_group({{&nupd, a.ucount.p, sizeof(u32)}, {&nruns, c1.p, sizeof(u64)}}, s);
-  stream_sync(s);
+  stream_wait_reads(s);
   plan.nruns = nruns;
   plan.nupd = nupd;
   if (g_prof && g_prof->records(P_RESOLVE)) g_prof->bytes[P_RESOLVE] += 16 * k + 32 * nruns;
@@ -2967,7 +3033,7 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
              {h + 4, a.chunkbase.p + ns, sizeof(u64)},
              {h + 5, a.stoff.p + ns, sizeof(u64)}},
             s);
-  stream_sync(s);
+  stream_wait_reads(s);
   plan.old_entries = h[0];
   plan.max_row = h[1];
   plan.new_entries = h[2];
@@ -3156,7 +3222,7 @@ int det_move(const Csr& g, u32* f, const u32* sm, double* tot, u64 tot_cap, u64*
   HL_LAUNCH_CHECK();
   u32 st = 0;
   d2h_async(&st, ds.status.p, sizeof(u32), s);
-  stream_sync(s);
+  stream_wait_reads(s);
   return int(st);
 }
 
@@ -3197,7 +3263,7 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
   u32 hs[3] = {0, 0, 0};
   u64 m = 0;
   d2h_group({{hs, rs.status.p, 3 * sizeof(u32)}, {&m, rs.cnt.p, sizeof(u64)}}, s);
-  stream_sync(s);
+  stream_wait_reads(s);
   *identity = hs[1] == 0;
   const u32 next_id = hs[2] + 1;
   if (!hs[0]) return next_id;
@@ -3358,7 +3424,7 @@ void refine_det(const Csr& g, const u32* f, u32* sm, const u8* R, double gamma, 
     // starts/ks/vs now hold the (s, v) segments
     u64 nseg = 0;
     d2h_async(&nseg, cnt.p, sizeof(u64), s);
-    stream_sync(s);
+    stream_wait_reads(s);
     fill_u32(rs.mhead.p, kNone, sn, s);
     hipLaunchKernelGGL(k_member_lists, GRID(nseg), nseg, starts.p, u64(n), ks.p, vs.p, rs.mhead.p, rs.mnext.p);
     HL_LAUNCH_CHECK();
@@ -3404,7 +3470,7 @@ u64 aggregate(const Csr& g, const Rec* dG, u64 k, u32* s_pre, const u32* s_cur, 
   u32 hm = 0;
   d2h_async(&hm, mx.p, sizeof(u32), s);
   const u64 nr = n ? select_flagged(R, n, as.rlist.p, tmp, c1, s) : 0;
-  if (!n) stream_sync(s);
+  if (!n) stream_wait_reads(s);
   *next_n = hm + 1;
   as.cnt64.reserve(nr + 1, s);
   as.off64.reserve(nr + 1, s);
@@ -3497,7 +3563,7 @@ double modularity(const Csr& g, const u32* lab, u32 maxdeg, DevBuf<double>& sig,
   const u32 n = g.n;
   double t = 0.0;
   d2h_async(&t, g.twice_m, sizeof(double), s);
-  stream_sync(s);
+  stream_wait_reads(s);
   if (t == 0.0 || n == 0) return 0.0;
   const u32 hub_cap = std::min<u32>(n, kIntraHubList);
   mx.reserve(u64(hub_cap) + 2, s);
@@ -3505,14 +3571,14 @@ double modularity(const Csr& g, const u32* lab, u32 maxdeg, DevBuf<double>& sig,
   max_u32(lab, n, mx.p, s);
   u32 hm = 0;
   d2h_async(&hm, mx.p, sizeof(u32), s);
-  stream_sync(s);
+  stream_wait_reads(s);
   const u64 L = u64(hm) + 1;
   sig.reserve(L, s);
   acc.reserve(2, s);
   modularity_launch(g, lab, L, maxdeg, sig.p, acc.p, mx.p + 1, hub_cap, s);
   double h[2];
   d2h_async(h, acc.p, 2 * sizeof(double), s);
-  stream_sync(s);
+  stream_wait_reads(s);
   return (h[0] - h[1]) / t;
 }
 
@@ -3604,7 +3670,7 @@ void canonicalize(u32* lab, u32 n, DevBuf<u32>& first, DevBuf<u8>& flag, DevBuf<
   max_u32(lab, n, mx.p, s);
   u32 hm = 0;
   d2h_async(&hm, mx.p, sizeof(u32), s);
-  stream_sync(s);
+  stream_wait_reads(s);
   const u64 L = u64(hm) + 1;
   first.reserve(L, s);
   fill_u32(first.p, kNone, L, s);
@@ -3663,7 +3729,7 @@ void modularity_graphs(const Csr& g, const u32* lab, u32 G, double* out, DevBuf<
   if (n) max_u32(lab, n, mx.p, s);
   u32 hm = 0;
   d2h_async(&hm, mx.p, sizeof(u32), s);
-  stream_sync(s);
+  stream_wait_reads(s);
   const u64 L = u64(hm) + 1;
   sig.reserve(L, s);
   cg.reserve(L, s);
@@ -3705,7 +3771,7 @@ void compare_partitions(const u32* a, const u32* b, u64 n, u32 la, u32 lb, doubl
   }
   u32 hm[2];
   d2h_async(hm, mx.p, sizeof(hm), s);
-  stream_sync(s);
+  stream_wait_reads(s);
   const u64 LA = u64(hm[0]) + 1, LB = u64(hm[1]) + 1;
   DevBuf<u32> sa, sb, besta, bestb, idx, idx_s, starts, misc;
   DevBuf<u64> keys, keys_s, cnt;

@@ -2484,7 +2484,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
     unsigned long long o[5] = {0, 0, 0, 0, 0};
     u32 cnts[3] = {0, 0, 0};  // counters [0] and [2]: the two lists' counts
     d2h_group({{o, ms.pout.p, sizeof(o)}, {cnts, lcnt[0], sizeof(cnts)}}, s);
-    stream_sync(s);
+    stream_wait_reads(s);
     const u32 done = u32(o[0]);
     first_cnt = u32(o[4]);
     rounds += done;
@@ -2524,7 +2524,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   }
   if (!counted) {
     d2h_async(&host_cnt, cnt_a, sizeof(u32), s);
-    stream_sync(s);
+    stream_wait_reads(s);
     if (st) st->active = host_cnt;
     if (hook && hook->fn) hook->fn(hook->ctx, cur, host_cnt);
   }
@@ -2599,7 +2599,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
           }
           d2h_async(und_h, ms.und.p, size_t(chunk) * sizeof(u32), s);
         }
-        stream_sync(s);
+        stream_wait_reads(s);
         int ran = chunk;  // the rounds that decided something: up to the first with none undecided
         for (int r = 0; r < chunk; ++r)
           if (und_h[r] == 0) {
