@@ -262,6 +262,7 @@ struct Engine {
         const u64 room = std::max(live / 8, std::min(4 * plan.need, live));
         const u64 want = std::max<u64>(live + plan.need + room, 1024);
         if (lv.col_spare.cap < live + plan.need + room / 2) {
+          stream_sync(s);  // the spare pool may be the source of the last compaction
           lv.col_spare.release();
           lv.w_spare.release();
           lv.col_spare.reserve(want, s, false, 1.0);
@@ -1147,7 +1148,7 @@ hl_status hl_apply_batch_device(hl_graph* g, uint64_t k, const uint32_t* d_src, 
     HL_LAUNCH_CHECK();
     uint32_t bad = 0;
     hl::d2h_async(&bad, e.mx.p, sizeof(uint32_t), e.s);
-    hl::stream_sync(e.s);
+    hl::stream_wait_reads(e.s);
     if (bad) throw hl::InputError("endpoint id out of range, non-finite weight or a record joining two graphs");
     e.run_batch(k, stats);
     finish_stats(stats, t0);

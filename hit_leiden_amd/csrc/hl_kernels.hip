@@ -3117,7 +3117,9 @@ void compact_copy(const LevelCsrMut& g, const u64* offs, u32* col2, double* w2, 
   HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_set_offsets, GRID(g.n), g, offs);
   HL_LAUNCH_CHECK();
-  stream_sync(s);
+  // no wait: the pools swap on the host, and every later use of either one
+  // (the merges into the new pool, the next compaction into the old one, or
+  // its release, which synchronises) comes after these kernels on the stream
 }
 
 u64 twice_m_scratch(u32 n) {
