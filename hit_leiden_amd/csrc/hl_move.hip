@@ -1822,8 +1822,12 @@ __global__ __launch_bounds__(kPersistBS) void k_move_rounds(MoveCtx m, MovePolic
     // one goes to the per-round launches
     const u32 c0 = *a.lcnt[a.ci0 & 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+      // out[] is this thread's alone: it starts from zero here
+      a.out[0] = 0;
+      a.out[1] = c0 > kPersistActive ? 1 : 0;
+      a.out[2] = 0;
+      a.out[3] = 0;
       a.out[4] = c0;
-      if (c0 > kPersistActive) a.out[1] = 1;
     }
     if (c0 == 0 || c0 > kPersistActive) return;
   }
@@ -2444,8 +2448,7 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   // false when the kernel could not be launched (nothing was enqueued)
   auto run_persist = [&]() -> bool {
     ms.hovf.reserve(u64(n) + 1, s);
-    ms.pout.reserve(8, s);
-    fill_u32(reinterpret_cast<u32*>(ms.pout.p), 0, 16, s);
+    ms.pout.reserve(8, s);  // the kernel clears it
     PersistArgs a{};
     a.list[0] = ms.list_a.p;
     a.list[1] = ms.list_b.p;
