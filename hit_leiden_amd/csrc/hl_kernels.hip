@@ -523,7 +523,9 @@ __global__ void k_scan_tail(const u64* in, u64* out, u64 n) {
 }
 
 // ------------------------------------------------------------------ apply delta
-__global__ void k_canon_keys(const Rec* d, u64 k, u64* keys, u32* idx) {
+// also clears the resolve kernel's update count (one launch less)
+__global__ void k_canon_keys(const Rec* d, u64 k, u64* keys, u32* idx, u32* ucount) {
+  if (gtid() == 0) *ucount = 0;
   for (u64 i = gtid(); i < k; i += gstride()) {
     Rec r = d[i];
     u32 a = min(r.u, r.v), b = max(r.u, r.v);
@@ -597,7 +599,9 @@ __global__ void k_resolve_runs(const unsigned long long* nruns_dev, const u32* r
 // Per sorted update: (is insert) << 32 | (is delete), for one scan that gives
 // every update its inserts and deletes before it (a row's updates are
 // contiguous, so the row's share is a difference of two scan entries).
-__global__ void k_upd_code(u64 nupd, const u32* perm, const u8* uop, u64* code) {
+// also clears the row plan's three counters (one launch less)
+__global__ void k_upd_code(u64 nupd, const u32* perm, const u8* uop, u64* code, u64* plan_cnt) {
+  if (gtid() < 3) plan_cnt[gtid()] = 0;
   for (u64 t = gtid(); t < nupd; t += gstride()) {
     const u8 op = uop[perm[t]];
     code[t] = (u64(op == OP_INS) << 32) | u64(op == OP_DEL);
@@ -2957,7 +2961,8 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   a.heads.reserve(k, s);
   a.runs.reserve(k, s);
   a.cnt.reserve(4, s);
-  hipLaunchKernelGGL(k_canon_keys, GRID(k), d, k, a.keys.p, a.idx.p);
+  a.ucount.reserve(1, s);
+  hipLaunchKernelGGL(k_canon_keys, GRID(k), d, k, a.keys.p, a.idx.p, a.ucount.p);
   HL_LAUNCH_CHECK();
   sort_pairs_u64(a.keys.p, a.keys_s.p, a.idx.p, a.idx_s.p, k, 64, tmp, s);
   hipLaunchKernelGGL(k_heads_u64, GRID(k), a.keys_s.p, k, 0, a.heads.p);
@@ -2974,8 +2979,6 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   a.perm.reserve(maxupd, s);
   a.uop.reserve(maxupd, s);
   a.uw.reserve(maxupd, s);
-  a.ucount.reserve(1, s);
-  fill_u32(a.ucount.p, 0, 1, s);
   {
   HL_PROF(P_RESOLVE, 0, s);
   hipLaunchKernelGGL(k_resolve_runs, GRID(k), reinterpret_cast<const unsigned long long*>(c1.p), a.runs.p, k,
@@ -3010,12 +3013,11 @@ void apply_delta_plan(const Rec* d, u64 k, const LevelCsrMut& g, ApplyScratch& a
   a.stoff.reserve(ns + 1, s);
   a.chunks.reserve(ns, s);
   a.chunkbase.reserve(ns + 1, s);
-  HL_CHECK(hipMemsetAsync(a.cnt.p, 0, 3 * sizeof(u64), s));
   // inserts and deletes before each update, and per row, from one scan (a
   // coarse hub row can take millions of updates: no thread walks a row)
   a.ucode.reserve(nupd, s);
   a.uscan.reserve(u64(nupd) + 1, s);
-  hipLaunchKernelGGL(k_upd_code, GRID(nupd), u64(nupd), a.perm.p, a.uop.p, a.ucode.p);
+  hipLaunchKernelGGL(k_upd_code, GRID(nupd), u64(nupd), a.perm.p, a.uop.p, a.ucode.p, a.cnt.p);
   HL_LAUNCH_CHECK();
   exclusive_scan_u64_async(a.ucode.p, a.uscan.p, nupd, tmp, s);
   hipLaunchKernelGGL(k_upd_pre, GRID(nupd), u64(nupd), ns, a.segs.p, a.uscan.p, a.ins_pre.p, a.del_pre.p);
