@@ -1253,7 +1253,9 @@ __global__ void k_aff_mark(const u8* K, const u32* s, u32 n, u8* aff, u32* statu
 __global__ void k_incc(const u32* s, u32 n, const u8* aff, u8* incc) {
   for (u64 v = gtid(); v < n; v += gstride()) incc[v] = aff[s[v]];
 }
-__global__ void k_uf_init(const u32* vcc, u64 m, u32* parent, u32* cnt) {
+// also clears the hub-row count of the edge pass that follows
+__global__ void k_uf_init(const u32* vcc, u64 m, u32* parent, u32* cnt, u32* hub_cnt) {
+  if (gtid() == 0) *hub_cnt = 0;
   for (u64 i = gtid(); i < m; i += gstride()) {
     u32 v = vcc[i];
     parent[v] = v;
@@ -1425,6 +1427,21 @@ __global__ void k_sizes(const u32* s, u32 n, u32* sizes) {
   for (u64 v = gtid(); v < n; v += gstride()) atomicAdd(&sizes[s[v]], 1u);
 }
 // v in R and alone in its subcommunity
+// the refinement's per-subcommunity arrays cleared in one launch: candidate
+// flags, parent members (none), w(S, C), and the candidate marks' hub-row count
+__global__ void k_refine_clear(u8* is_cand, u32* pidx, double* wS, u64 sn, u32* hub_cnt) {
+  if (gtid() == 0) *hub_cnt = 0;
+  for (u64 i = gtid(); i < sn; i += gstride()) {
+    is_cand[i] = 0;
+    pidx[i] = kNone;
+    wS[i] = 0.0;
+  }
+}
+// the label array of cc_split cleared, and its three status words
+__global__ void k_cc_clear(u8* aff, u64 n, u32* status) {
+  if (gtid() < 3) status[gtid()] = 0;
+  for (u64 i = gtid(); i < n; i += gstride()) aff[i] = 0;
+}
 __global__ void k_single(const u8* R, u32 n, const u32* s, const u32* sizes, u8* single) {
   for (u64 v = gtid(); v < n; v += gstride()) single[v] = R[v] && sizes[s[v]] == 1;
 }
@@ -1461,7 +1478,9 @@ __global__ __launch_bounds__(256) void k_cand_mark_hubs(Csr g, const u32* f, con
     }
   }
 }
-__global__ void k_parent_idx(const u32* s, u32 n, const u8* is_cand, u32* pidx) {
+// also clears the hub-row count of k_ws (the candidate marks' hub rows are done)
+__global__ void k_parent_idx(const u32* s, u32 n, const u8* is_cand, u32* pidx, u32* hub_cnt) {
+  if (gtid() == 0) *hub_cnt = 0;
   for (u64 v = gtid(); v < n; v += gstride())
     if (is_cand[s[v]]) atomicMin(&pidx[s[v]], u32(v));
 }
@@ -3251,8 +3270,9 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
   rs.aff.reserve(s_bound, s);
   rs.status.reserve(4, s);
   rs.smax.reserve(1, s);
-  fill_u8(rs.aff.p, 0, s_bound, s);
-  fill_u32(rs.status.p, 0, 3, s);  // [0] any affected, [1] not identity, [2] max label
+  // status: [0] any affected, [1] not identity, [2] max label
+  hipLaunchKernelGGL(k_cc_clear, GRID(s_bound), rs.aff.p, s_bound, rs.status.p);
+  HL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_aff_mark, GRID(n), K, sm, n, rs.aff.p, rs.status.p);
   HL_LAUNCH_CHECK();
   max_u32(sm, n, rs.status.p + 2, s);
@@ -3282,14 +3302,13 @@ u64 cc_split(const Csr& g, u32* sm, const u8* K, u8* R, u64 s_bound, bool* ident
   rs.rkeys_s.reserve(m, s);
   rs.newid.reserve(n, s);
   HL_PROF(P_CC_SPLIT, 16 * m, s);
-  hipLaunchKernelGGL(k_uf_init, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p);
-  HL_LAUNCH_CHECK();
   {
     const u32 hub_cap = u32(std::min<u64>(m, kIntraHubList));
     const u32 pieces = u32(std::max<u64>(1, (u64(maxdeg) + kIntraPiece - 1) / kIntraPiece));
     const bool hub = maxdeg > kIntraHub;
     rs.hubs.reserve(u64(hub_cap) + 1, s);
-    fill_u32(rs.hubs.p + hub_cap, 0, 1, s);
+    hipLaunchKernelGGL(k_uf_init, GRID(m), rs.vcc.p, m, rs.parent.p, rs.cnt32.p, rs.hubs.p + hub_cap);
+    HL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_uf_edges, GRID(m * 64), g, rs.vcc.p, m, sm, f, rs.parent.p, hub ? rs.hubs.p : nullptr,
                        rs.hubs.p + hub_cap, hub_cap);
     HL_LAUNCH_CHECK();
@@ -3354,14 +3373,18 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
   rs.blocked.reserve(sn, s);
   rs.parentIdx.reserve(sn, s);
   rs.wS.reserve(sn, s);
-  fill_u8(rs.is_cand.p, 0, sn, s);
+  // hub-row lists of the candidate marks (hub_cap1 entries, then its count)
+  // and of k_ws (hub_cap2), in one buffer reserved for both
+  const u32 hub_cap1 = u32(std::min<u64>(ns, kIntraHubList));
+  const u32 hub_cap2 = std::min<u32>(n, kIntraHubList);
+  rs.hubs.reserve(u64(std::max(hub_cap1, hub_cap2)) + 1, s);
+  hipLaunchKernelGGL(k_refine_clear, GRID(sn), rs.is_cand.p, rs.parentIdx.p, rs.wS.p, sn, rs.hubs.p + hub_cap1);
+  HL_LAUNCH_CHECK();
   HL_PROF(P_REFINE, 16ull * n, s);
   {
-    const u32 hub_cap = u32(std::min<u64>(ns, kIntraHubList));
+    const u32 hub_cap = hub_cap1;
     const u32 pieces = u32(std::max<u64>(1, (u64(maxdeg) + kIntraPiece - 1) / kIntraPiece));
     const bool hub = maxdeg > kIntraHub;
-    rs.hubs.reserve(u64(hub_cap) + 1, s);
-    fill_u32(rs.hubs.p + hub_cap, 0, 1, s);
     hipLaunchKernelGGL(k_cand_mark, GRID(ns * 64), g, sl.p, ns, f, sm, rs.is_cand.p, hub ? rs.hubs.p : nullptr,
                        rs.hubs.p + hub_cap, hub_cap);
     HL_LAUNCH_CHECK();
@@ -3371,16 +3394,12 @@ void refine_parallel(const Csr& g, const u32* f, u32* sm, const u8* R, double ga
       HL_LAUNCH_CHECK();
     }
   }
-  fill_u32(rs.parentIdx.p, kNone, sn, s);
-  hipLaunchKernelGGL(k_parent_idx, GRID(n), sm, n, rs.is_cand.p, rs.parentIdx.p);
+  hipLaunchKernelGGL(k_parent_idx, GRID(n), sm, n, rs.is_cand.p, rs.parentIdx.p, rs.hubs.p + hub_cap2);
   HL_LAUNCH_CHECK();
-  fill_f64(rs.wS.p, 0.0, sn, s);
   {
-    const u32 hub_cap = std::min<u32>(n, kIntraHubList);
+    const u32 hub_cap = hub_cap2;
     const u32 pieces = u32(std::max<u64>(1, (u64(maxdeg) + kIntraPiece - 1) / kIntraPiece));
     const u64 hub_max = maxdeg > kIntraHub ? u64(pieces) * kIntraPiece : 0;
-    rs.hubs.reserve(u64(hub_cap) + 1, s);
-    fill_u32(rs.hubs.p + hub_cap, 0, 1, s);
     hipLaunchKernelGGL(k_ws, blocks_for(n, 256, 4096), 256, 0, s, g, n, f, sm, rs.is_cand.p, rs.parentIdx.p,
                        rs.wS.p, rs.hubs.p, rs.hubs.p + hub_cap, hub_cap, hub_max);
     HL_LAUNCH_CHECK();
