@@ -304,6 +304,11 @@ __global__ void k_activate_graphs(const u32* gid, u32 n, const u8* part, const u
 __global__ void k_fill_u64_one(unsigned long long* p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
+// the movement's 32 counters and its claim bitmap, cleared
+__global__ void k_move_clear(u32* counters, u32* mark, u64 words) {
+  if (gtid() < 32) counters[gtid()] = 0;
+  for (u64 w = gtid(); w < words; w += gstride()) mark[w] = 0;
+}
 __global__ void k_iota_list(u32* list, u32 n, u32* cnt) {
   for (u64 i = gtid(); i < n; i += gstride()) list[i] = u32(i);
   if (gtid() == 0) *cnt = n;
@@ -2351,14 +2356,17 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
   u32* status = ms.counters.p + 14;
   int ci = 0;
   u32* cnt_a = lcnt[0];
-  HL_CHECK(hipMemsetAsync(ms.counters.p, 0, 32 * sizeof(u32), s));
   u32* cur = ms.list_a.p;
   u32* nxt = ms.list_b.p;
   if (k == 0 && !m.gid) {
+    fill_u32(ms.counters.p, 0, 32, s);
     hipLaunchKernelGGL(k_iota_list, fixed_grid(n, 256, 4096), 256, 0, s, cur, n, cnt_a);
     HL_LAUNCH_CHECK();
   } else {
-    ms.fresh_marks(s);
+    // the counters and the claim bitmap cleared in one launch
+    const u64 words = u64(ms.mark_n) / 32 + 1;
+    hipLaunchKernelGGL(k_move_clear, fixed_grid(words, 256, 4096), 256, 0, s, ms.counters.p, ms.mark.p, words);
+    HL_LAUNCH_CHECK();
     if (k) {
       HL_PROF(P_SEED, 24 * k, s);
       hipLaunchKernelGGL(k_seed2, fixed_grid(k), 256, 0, s, d, k, m.f, m.s, m.A, m.K, ms.mark.p, cur, cnt_a);
