@@ -526,6 +526,8 @@ struct Engine {
   // Algorithm 6 (hit_leiden.rs:936-1018) for a batch already in dcur[0..k)
   u64 run_batch(u64 k, hl_batch_stats* st) {
     lvcnt.reserve(4 * HL_MAX_LEVELS, s);
+    // every level's four frontier counts, cleared once for the batch
+    if (st) fill_u32(reinterpret_cast<u32*>(lvcnt.p), 0, 8 * HL_MAX_LEVELS, s);
     u64 cur_len = k;
     u32 cur_n = n0;
     u64 total_iters = 0;
@@ -543,7 +545,7 @@ struct Engine {
       refinement(lv, st, p);
       if (st && p < HL_MAX_LEVELS) {
         // frontier sizes, read back with the batch's last synchronisation
-        count4_u8_async(lv.A.p, lv.B.p, lv.K.p, lv.R.p, lv.n, lvcnt.p + 4 * p, s);
+        count4_u8_async(lv.A.p, lv.B.p, lv.K.p, lv.R.p, lv.n, lvcnt.p + 4 * p, s, true);
         st->level_n[p] = lv.n;
         st->delta_len[p] = cur_len;
         st->bytes_alg += 24 * cur_len;

@@ -45,7 +45,8 @@ u64 count_u8(const u8* a, u64 n, DevBuf<u64>& cnt, hipStream_t s);
 void count4_u8(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, DevBuf<u64>& cnt, u64 out[4],
                hipStream_t s);
 // the same into out[0..4) on the device, without synchronising
-void count4_u8_async(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, u64* out, hipStream_t s);
+void count4_u8_async(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, u64* out, hipStream_t s,
+                     bool cleared = false);
 void fill4_u8(u8* a0, u8* a1, u8* a2, u8* a3, u8 v, u64 n, hipStream_t s);
 // *out = max(*out, max(a[0..n)))  (out must be initialised)
 void max_u32(const u32* a, u64 n, u32* out, hipStream_t s);
@@ -126,7 +127,7 @@ void apply_delta_commit(const LevelCsrMut& g, u64 pool_base, ApplyScratch& a, co
 // Pool compaction in two steps: compact_plan returns the pool entries the rows
 // take (their capacities when caps is set, else their lengths) and leaves the
 // new row offsets in off_tmp; compact_copy moves every row into [col2, w2)
-// (capacities kept) and synchronises.
+// (capacities kept) without waiting for it.
 u64 compact_plan(const LevelCsrMut& g, DevBuf<u64>& off_tmp, Temp& tmp, hipStream_t s, bool caps = false);
 void compact_copy(const LevelCsrMut& g, const u64* offs, u32* col2, double* w2, hipStream_t s);
 // the rows laid out at offs (from compact_plan) into col2/w2, leaving the live rows as they are

@@ -2661,8 +2661,9 @@ void count4_u8(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, De
   d2h_async(out, cnt.p, 4 * sizeof(u64), s);
   stream_wait_reads(s);
 }
-void count4_u8_async(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, u64* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, out, u64(0), u64(4));
+void count4_u8_async(const u8* a0, const u8* a1, const u8* a2, const u8* a3, u64 n, u64* out, hipStream_t s,
+                     bool cleared) {
+  if (!cleared) hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, out, u64(0), u64(4));
   if (n) {
     HL_PROF(P_REDUCE, 4 * n, s);
     hipLaunchKernelGGL(k_count4, blocks_for(n / 4 + 1, 256, kReduceBlocks), 256, 0, s, a0, a1, a2, a3, n, (unsigned long long*)out);
