@@ -1954,11 +1954,16 @@ __global__ void k_acc_degrees(u64 c, const u32* node, const u32* state, const u3
 }
 
 // zero the per-round counters the kernels above leave set
-__global__ void k_round_reset(u32* cur_cnt, u32* ccnt, u32* bucket_counts) {
+// the round's counts back to zero (null pointers skipped); also *set = val and
+// zero2[0..1] = 0 where given
+__global__ void k_round_reset(u32* cur_cnt, u32* ccnt, u32* bucket_counts, u32* set = nullptr, u32 val = 0,
+                              u32* zero2 = nullptr) {
   if (threadIdx.x == 0) {
     if (cur_cnt) *cur_cnt = 0;
     if (ccnt) *ccnt = 0;
     clear_bucket_counts(bucket_counts);
+    if (set) *set = val;
+    if (zero2) zero2[0] = zero2[1] = 0;
   }
 }
 
@@ -2672,9 +2677,8 @@ u32 movement_throughput(const MoveCtx& m_in, const Rec* d, u64 k, u32 maxdeg, in
         }
       }
       d2h_group({{&accepted, acc_cnt, sizeof(u32)}, {&next_host, next_cnt, sizeof(u32)}}, s);
-      hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, cur_cnt, ccnt, bcounts);
+      hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, cur_cnt, ccnt, bcounts, static_cast<u32*>(nullptr), 0u, status);
       HL_LAUNCH_CHECK();
-      fill_u32(status, 0, 2, s);
       stream_sync(s);
     }
     ++rounds;
@@ -2699,8 +2703,8 @@ void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, con
   }
   u32* cnt = ms.counters.p + 3;
   u32* ccnt = ms.counters.p + 1;
-  fill_u32(cnt, u32(ns), 1, s);
-  hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, nullptr, ccnt, ms.counters.p + 4);
+  hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, static_cast<u32*>(nullptr), ccnt, ms.counters.p + 4, cnt, u32(ns),
+                     static_cast<u32*>(nullptr));
   HL_LAUNCH_CHECK();
   RefinePolicyT pol{f, sm, dS, blocked, gamma};
   propose_round(pol, g, singles, cnt, ns, maxdeg, ms, th, ms.dstat.p + DS_REF, ms.dstat.p + DS_TRY_REF, s);
@@ -2714,7 +2718,8 @@ void refine_proposals(const Csr& g, const u32* f, u32* sm, const double* dS, con
     hipLaunchKernelGGL(k_ref_apply, fixed_grid(ns), 256, 0, s, cand, nullptr, sm, g.k, sub);
   }
   HL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, nullptr, ccnt, ms.counters.p + 4);
+  hipLaunchKernelGGL(k_round_reset, 1, 64, 0, s, static_cast<u32*>(nullptr), ccnt, ms.counters.p + 4,
+                     static_cast<u32*>(nullptr), 0u, static_cast<u32*>(nullptr));
   HL_LAUNCH_CHECK();
 }
 
