@@ -1780,12 +1780,13 @@ __global__ __launch_bounds__(BS) void k_marks_flat(MoveCtx m, FlatMarks flat, u3
 // communities in up to 512 LDS slots), a grid barrier, the decoupling with its
 // apply and in-block marks by workgroup 0 (dec_small_body), a grid barrier,
 // and, when the decoupling left long neighbour lists, the grid-wide marks and
-// a grid barrier.  The rounds are the per-round launches' rounds: the same
-// bodies, the same counters, with no host round trip between them.  A round
-// whose keys do not fit the slots or whose candidates exceed the decoupling
-// workgroup stops the kernel before anything of it is applied, and the host
-// runs it and the rest with the per-round launches, as it does the rounds
-// after one that lists more than kPersistActive vertices.
+// a grid barrier.  The rounds are the per-round launches' rounds (the same
+// device bodies, with work counters of their own) without a host round trip
+// between them.  A round whose keys do not fit the slots or whose candidates
+// exceed the decoupling workgroup stops the kernel before anything of it is
+// applied; the host runs that round with the per-round launches and may start
+// the kernel again for a later round, as it does after a round that lists more
+// than kPersistActive vertices.
 constexpr u32 kPersistBS = 1024;
 constexpr u32 kPersistG = 64, kPersistCap = 512;
 constexpr u32 kPersistDeg = 256;       // rows of at most this many entries: tables of kPersistCap slots
